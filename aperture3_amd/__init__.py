@@ -1,0 +1,385 @@
+"""aperture3_amd -- MI355X-native Aperture PIC time step.
+
+Python host mirror of the reference's drop-in interfaces
+(``ptc_updater``, ``field_solver``, ``sim_data``, ``sim_environment``;
+src/algorithms/ptc_updater.h:11-24, src/algorithms/field_solver.h:11-25,
+src/sim_data.h:14-70) over the C ABI of ``libaperture_amd.so``
+(include/aperture_amd.h).  All numerics run in the HIP library; this module
+only moves arguments.  If the library is missing or cannot be loaded the
+import of any class here raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libaperture_amd.so")
+
+# ---- enums (aperture_amd.h) ------------------------------------------
+APT_OK = 0
+APT_PREC_FP64 = 0
+APT_PREC_MIXED = 1
+FIELD_E, FIELD_B, FIELD_EBG, FIELD_BBG, FIELD_J, FIELD_RHO, FIELD_DIVE, FIELD_DIVB = range(8)
+KERNEL_PUSH_DEPOSIT, KERNEL_FIELD, KERNEL_SORT = range(3)
+MAX_CELL = 0xFFFFFFFF
+FLAG_IGNORE_CURRENT = 1 << 3
+FLAG_IGNORE_EM = 1 << 4
+
+# stagger of each component (fields_impl.hpp:85-94, sim_data_impl.hpp:51-65)
+STAGGER = {
+    FIELD_E: (0b110, 0b101, 0b011), FIELD_EBG: (0b110, 0b101, 0b011),
+    FIELD_J: (0b110, 0b101, 0b011),
+    FIELD_B: (0b001, 0b010, 0b100), FIELD_BBG: (0b001, 0b010, 0b100),
+}
+
+
+class AptGrid(C.Structure):
+    _fields_ = [("N", C.c_int * 3), ("guard", C.c_int * 3),
+                ("lower", C.c_double * 3), ("size", C.c_double * 3),
+                ("periodic", C.c_int * 3)]
+
+
+class AptParams(C.Structure):
+    _fields_ = [("precision", C.c_int), ("num_species", C.c_int),
+                ("q_e", C.c_double), ("ion_mass", C.c_double),
+                ("data_interval", C.c_int), ("current_smoothing", C.c_int),
+                ("fix_3d_filter", C.c_int), ("max_ptc_number", C.c_uint64),
+                ("random_seed", C.c_uint64), ("tile", C.c_int * 3)]
+
+
+class AptParticles(C.Structure):
+    _fields_ = [("cell", C.POINTER(C.c_uint32)),
+                ("x1", C.POINTER(C.c_double)), ("x2", C.POINTER(C.c_double)),
+                ("x3", C.POINTER(C.c_double)),
+                ("p1", C.POINTER(C.c_double)), ("p2", C.POINTER(C.c_double)),
+                ("p3", C.POINTER(C.c_double)),
+                ("E", C.POINTER(C.c_double)), ("weight", C.POINTER(C.c_double)),
+                ("flag", C.POINTER(C.c_uint32)), ("id", C.POINTER(C.c_uint64))]
+
+
+class AptCounters(C.Structure):
+    _fields_ = [("nan_momenta", C.c_uint64), ("multi_cell_moves", C.c_uint64),
+                ("capacity_overflow", C.c_uint64), ("outside_tile", C.c_uint64)]
+
+
+# every entry point declared in include/aperture_amd.h: name -> argtypes
+_P = C.c_void_p
+SIGNATURES = {
+    "apt_create": [C.POINTER(AptGrid), C.POINTER(AptParams), C.POINTER(_P)],
+    "apt_destroy": [_P],
+    "apt_last_error": [_P],
+    "apt_get_jexp": [_P, C.POINTER(C.c_int)],
+    "apt_get_mesh": [_P, C.POINTER(C.c_int), C.POINTER(C.c_double)],
+    "apt_field_set": [_P, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_uint64],
+    "apt_field_get": [_P, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_uint64],
+    "apt_ptc_set": [_P, C.POINTER(AptParticles), C.c_uint64],
+    "apt_ptc_get": [_P, C.POINTER(AptParticles), C.c_uint64, C.POINTER(C.c_uint64)],
+    "apt_ptc_number": [_P, C.POINTER(C.c_uint64)],
+    "apt_ptc_load_uniform": [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double],
+    "apt_ptc_update": [_P, C.c_double, C.c_uint32],
+    "apt_smooth_current": [_P, C.c_uint32],
+    "apt_field_update": [_P, C.c_double, C.c_uint32],
+    "apt_field_update_e": [_P, C.c_double],
+    "apt_field_update_b": [_P, C.c_double],
+    "apt_field_compute_divs": [_P],
+    "apt_sort_particles": [_P],
+    "apt_clear_guard_cells": [_P],
+    "apt_energy_histogram": [_P, C.POINTER(C.c_uint32), C.c_int, C.c_double, C.c_int],
+    "apt_sync": [_P],
+    "apt_get_counters": [_P, C.POINTER(AptCounters)],
+    "apt_reset_counters": [_P],
+    "apt_timing_enable": [_P, C.c_int],
+    "apt_timing_get": [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)],
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libaperture_amd.so; raises if it is missing (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it with `python -m aperture3_amd.build` "
+            "(there is no CPU fallback for the PIC step)")
+    lib = C.CDLL(path)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_char_p if name == "apt_last_error" else C.c_int
+    _lib = lib
+    return lib
+
+
+class AptError(RuntimeError):
+    pass
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# ---------------------------------------------------------------------------
+class SimParams:
+    """The SimParams fields this path reads (sim_params.h:17-123)."""
+
+    def __init__(self, **kw):
+        self.delta_t = 0.01
+        self.q_e = 1.0
+        self.max_ptc_number = 100
+        self.ion_mass = 1.0
+        self.num_species = 2
+        self.periodic_boundary = [False, False, False]
+        self.current_smoothing = 0
+        self.fix_3d_filter = False
+        self.max_steps = 1000
+        self.data_interval = 100
+        self.random_seed = 43210
+        self.sort_interval = 20
+        self.N = [1, 1, 1]
+        self.guard = [0, 0, 0]
+        self.lower = [0.0, 0.0, 0.0]
+        self.size = [1.0, 1.0, 1.0]
+        self.precision = APT_PREC_FP64
+        self.tile = [0, 0, 0]
+        for k, v in kw.items():
+            if not hasattr(self, k):
+                raise AttributeError(f"unknown parameter {k}")
+            setattr(self, k, v)
+
+
+class sim_environment:
+    """One-GPU stand-in for sim_environment (sim_environment_impl.hpp:17-65):
+    holds the parameters and the mesh geometry (Quadmesh)."""
+
+    def __init__(self, params: SimParams):
+        self._params = params
+        self.dims = [params.N[d] + 2 * params.guard[d] for d in range(3)]
+        self.delta = [params.size[d] / params.N[d] for d in range(3)]
+        self.dim = max([d + 1 for d in range(3) if params.N[d] > 1] or [1])
+
+    def params(self):
+        return self._params
+
+    def pos(self, i, n, stagger):
+        """Quadmesh::pos(i, n, stagger) (quadmesh.h:119-150)."""
+        p = self._params
+        in_cell = stagger * 0.5 + 0.5
+        if i < self.dim:
+            return p.lower[i] + self.delta[i] * (n - p.guard[i] + in_cell)
+        return in_cell * self.delta[i]
+
+    def ncell(self):
+        return self.dims[0] * self.dims[1] * self.dims[2]
+
+
+class sim_data:
+    """Owns the device state through one C-ABI context (src/sim_data.h)."""
+
+    def __init__(self, env: sim_environment):
+        self.lib = load_library()
+        self.env = env
+        p = env.params()
+        g = AptGrid()
+        q = AptParams()
+        for d in range(3):
+            g.N[d] = p.N[d]
+            g.guard[d] = p.guard[d]
+            g.lower[d] = p.lower[d]
+            g.size[d] = p.size[d]
+            g.periodic[d] = 1 if p.periodic_boundary[d] else 0
+            q.tile[d] = p.tile[d]
+        q.precision = p.precision
+        q.num_species = p.num_species
+        q.q_e = p.q_e
+        q.ion_mass = p.ion_mass
+        q.data_interval = p.data_interval
+        q.current_smoothing = p.current_smoothing
+        q.fix_3d_filter = 1 if p.fix_3d_filter else 0
+        q.max_ptc_number = p.max_ptc_number
+        q.random_seed = p.random_seed
+        h = C.c_void_p()
+        st = self.lib.apt_create(C.byref(g), C.byref(q), C.byref(h))
+        if st != APT_OK:
+            raise AptError(f"apt_create failed with status {st}")
+        self.ctx = h
+
+    def _check(self, st, what):
+        if st != APT_OK:
+            msg = self.lib.apt_last_error(self.ctx)
+            raise AptError(f"{what}: status {st}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.apt_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- fields ------------------------------------------------------------
+    def set_field(self, which, comp, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        self._check(self.lib.apt_field_set(self.ctx, which, comp, _dptr(a), a.size),
+                    "apt_field_set")
+
+    def get_field(self, which, comp):
+        a = np.empty(self.env.ncell(), dtype=np.float64)
+        self._check(self.lib.apt_field_get(self.ctx, which, comp, _dptr(a), a.size),
+                    "apt_field_get")
+        return a
+
+    def initialize_field(self, which, comp, f):
+        """field<N,T>::initialize(component, f) (core/fields.h:111-143)."""
+        st = STAGGER[which][comp]
+        d = self.env.dims
+        x1 = np.array([self.env.pos(0, i, st & 1) for i in range(d[0])])
+        x2 = np.array([self.env.pos(1, j, (st >> 1) & 1) for j in range(d[1])])
+        x3 = np.array([self.env.pos(2, k, (st >> 2) & 1) for k in range(d[2])])
+        Z, Y, X = np.meshgrid(x3, x2, x1, indexing="ij")
+        v = np.broadcast_to(np.asarray(f(X, Y, Z), dtype=np.float64), X.shape)
+        self.set_field(which, comp, np.ascontiguousarray(v).ravel())
+        return v.ravel()
+
+    def init_bg_B_field(self, comp, f):
+        """sim_data.h:23-29: same function into Bbg and B."""
+        self.initialize_field(FIELD_BBG, comp, f)
+        return self.initialize_field(FIELD_B, comp, f)
+
+    # ---- particles ---------------------------------------------------------
+    def set_particles(self, ptc: dict):
+        n = len(ptc["cell"])
+        keep = {}
+
+        def arr(name, dt):
+            a = np.ascontiguousarray(ptc[name], dtype=dt)
+            keep[name] = a
+            return a.ctypes.data_as(C.POINTER(C.c_double if dt == np.float64 else
+                                              (C.c_uint32 if dt == np.uint32 else C.c_uint64)))
+        v = AptParticles()
+        v.cell = arr("cell", np.uint32)
+        for f in ("x1", "x2", "x3", "p1", "p2", "p3"):
+            setattr(v, f, arr(f, np.float64))
+        v.E = arr("E", np.float64) if "E" in ptc else None
+        v.weight = arr("w", np.float64)
+        v.flag = arr("flag", np.uint32)
+        v.id = arr("id", np.uint64) if "id" in ptc else None
+        self._check(self.lib.apt_ptc_set(self.ctx, C.byref(v), n), "apt_ptc_set")
+
+    def get_particles(self):
+        n = self.num_particles()
+        out = {"cell": np.empty(n, np.uint32), "flag": np.empty(n, np.uint32),
+               "id": np.empty(n, np.uint64)}
+        for f in ("x1", "x2", "x3", "p1", "p2", "p3", "E", "w"):
+            out[f] = np.empty(n, np.float64)
+        v = AptParticles()
+        v.cell = out["cell"].ctypes.data_as(C.POINTER(C.c_uint32))
+        v.flag = out["flag"].ctypes.data_as(C.POINTER(C.c_uint32))
+        v.id = out["id"].ctypes.data_as(C.POINTER(C.c_uint64))
+        for f in ("x1", "x2", "x3", "p1", "p2", "p3", "E"):
+            setattr(v, f, _dptr(out[f]))
+        v.weight = _dptr(out["w"])
+        got = C.c_uint64()
+        self._check(self.lib.apt_ptc_get(self.ctx, C.byref(v), n, C.byref(got)),
+                    "apt_ptc_get")
+        return out
+
+    def num_particles(self):
+        n = C.c_uint64()
+        self._check(self.lib.apt_ptc_number(self.ctx, C.byref(n)), "apt_ptc_number")
+        return n.value
+
+    def fill_multiplicity(self, weight, multiplicity, theta=(0.0, 0.0),
+                          drift=(0.0, 0.0), sort=True):
+        """sim_data::fill_multiplicity (cuda/sim_data.cu:268-293) with
+        Maxwell-Juttner momenta; sorts afterwards like the reference."""
+        th = (C.c_double * 2)(*theta)
+        dr = (C.c_double * 2)(*drift)
+        self._check(self.lib.apt_ptc_load_uniform(self.ctx, multiplicity, th, dr, weight),
+                    "apt_ptc_load_uniform")
+        if sort:
+            self.sort_particles()
+
+    def sort_particles(self):
+        """sim_data::sort_particles (sim_data_impl.hpp:78-89)."""
+        self._check(self.lib.apt_sort_particles(self.ctx), "apt_sort_particles")
+
+    def clear_guard_cells(self):
+        self._check(self.lib.apt_clear_guard_cells(self.ctx), "apt_clear_guard_cells")
+
+    def energy_histogram(self, nbins, E_max, flag_bit=-1):
+        h = np.zeros(nbins, np.uint32)
+        self._check(self.lib.apt_energy_histogram(
+            self.ctx, h.ctypes.data_as(C.POINTER(C.c_uint32)), nbins, E_max, flag_bit),
+            "apt_energy_histogram")
+        return h
+
+    # ---- control -----------------------------------------------------------
+    def jexp(self):
+        h = C.c_int()
+        self._check(self.lib.apt_get_jexp(self.ctx, C.byref(h)), "apt_get_jexp")
+        return h.value
+
+    def sync(self):
+        self._check(self.lib.apt_sync(self.ctx), "apt_sync")
+
+    def counters(self):
+        c = AptCounters()
+        self._check(self.lib.apt_get_counters(self.ctx, C.byref(c)), "apt_get_counters")
+        return {k: getattr(c, k) for k, _ in AptCounters._fields_}
+
+    def timing_enable(self, on=True):
+        self._check(self.lib.apt_timing_enable(self.ctx, 1 if on else 0), "apt_timing_enable")
+
+    def timing_get(self, kernel):
+        ms = C.c_double()
+        n = C.c_uint64()
+        self._check(self.lib.apt_timing_get(self.ctx, kernel, C.byref(ms), C.byref(n)),
+                    "apt_timing_get")
+        return ms.value, n.value
+
+
+class ptc_updater:
+    """ptc_updater (src/algorithms/ptc_updater.h:11-24)."""
+
+    def __init__(self, env: sim_environment):
+        self.m_env = env
+
+    def update_particles(self, data: sim_data, dt: float, step: int = 0):
+        data._check(data.lib.apt_ptc_update(data.ctx, dt, step), "apt_ptc_update")
+
+    def apply_boundary(self, data: sim_data, dt: float, step: int = 0):
+        """Empty for the Cartesian updater (ptc_updater.cu:843-850)."""
+
+    def smooth_current(self, data: sim_data, step: int = 0):
+        data._check(data.lib.apt_smooth_current(data.ctx, step), "apt_smooth_current")
+
+
+class field_solver:
+    """field_solver (src/algorithms/field_solver.h:11-25)."""
+
+    def __init__(self, env: sim_environment):
+        self.m_env = env
+
+    def update_fields(self, data: sim_data, dt: float, step: int = 0):
+        data._check(data.lib.apt_field_update(data.ctx, dt, step), "apt_field_update")
+
+    def update_e_field(self, data: sim_data, dt: float):
+        data._check(data.lib.apt_field_update_e(data.ctx, dt), "apt_field_update_e")
+
+    def update_b_field(self, data: sim_data, dt: float):
+        data._check(data.lib.apt_field_update_b(data.ctx, dt), "apt_field_update_b")
+
+    def compute_divs(self, data: sim_data):
+        data._check(data.lib.apt_field_compute_divs(data.ctx), "apt_field_compute_divs")
+
+    def apply_outflow_boundary(self, data: sim_data, time: float = 0.0):
+        """Empty in the reference (field_solver.cu:451-452)."""

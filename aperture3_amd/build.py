@@ -1,0 +1,95 @@
+"""Build the HIP library libaperture_amd.so in-tree for gfx950.
+
+Every .hip source under csrc/ is compiled with hipcc to an object in
+_build/ and linked into aperture3_amd/libaperture_amd.so; the C++ host
+mirror of the reference interface (csrc/host) is compiled into the same
+library and into the example driver binary.
+
+    python -m aperture3_amd.build      # or __graft_entry__.build()
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libaperture_amd.so")
+INCLUDE = os.path.abspath(os.path.join(HERE, "..", "include"))
+ARCH = os.environ.get("APT_OFFLOAD_ARCH", "gfx950")
+
+HIP_SOURCES = [
+    "kern_cart_f64.hip",
+    "kern_cart_mixed.hip",
+    "kern_field.hip",
+    "kern_init.hip",
+    "kern_sort.hip",
+    "apt_context.hip",
+]
+HOST_SOURCES = ["host/aperture_host.cpp"]
+
+# -ffp-contract=off keeps every FP64 expression in the order written (the
+# mixed kernels re-enable contraction locally with a pragma).  Packed FP32
+# instructions are disabled for the whole device build.
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-fPIC",
+    "-std=c++17",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+    f"-I{INCLUDE}",
+    f"-I{CSRC}",
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), shutil.which("hipcc"),
+                 "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the HIP library cannot be built")
+
+
+def _run(cmd: list[str]) -> None:
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                         text=True)
+    out = "\n".join(l for l in res.stdout.splitlines()
+                    if "packed-fp32-ops" not in l and "not a recognized feature" not in l)
+    if res.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    hipcc = _hipcc()
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    cmds = []
+    for src in HIP_SOURCES + HOST_SOURCES:
+        obj = os.path.join(BUILD, src.replace("/", "_") + ".o")
+        objs.append(obj)
+        cmds.append([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for cmd, _ in zip(cmds, ex.map(_run, cmds)):
+            if verbose:
+                print(" ".join(cmd))
+    tmp = LIB + ".tmp"
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    # example driver built on the C++ host mirror
+    drv = os.path.join(HERE, "pic_step_driver")
+    _run([hipcc, "-O2", "-std=c++17", f"-I{INCLUDE}", f"-I{CSRC}",
+          os.path.join(CSRC, "host", "pic_step_driver.cpp"), "-o", drv + ".tmp",
+          f"-L{HERE}", "-laperture_amd", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN"])
+    os.replace(drv + ".tmp", drv)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,171 @@
+// apt_common.hpp -- types and device helpers shared by every kernel of the
+// PIC time step (mesh index math, particle flags, counter-based RNG, fixed
+// point current accumulation).  gfx950 only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace apt {
+
+constexpr uint32_t MAX_CELL = 0xffffffffu;  // empty slot (constant_defs.h:14)
+constexpr int LIMB_BITS = 40;               // low-limb width of the FP64 J
+
+// ParticleFlag bits (enum_types.h:76-87); species in bits 29-31
+// (util_functions.h:64-79).
+constexpr uint32_t FLAG_TRACKED = 1u << 1;
+constexpr uint32_t FLAG_IGNORE_FORCE = 1u << 2;
+constexpr uint32_t FLAG_IGNORE_CURRENT = 1u << 3;
+constexpr uint32_t FLAG_IGNORE_EM = 1u << 4;
+constexpr uint32_t FLAG_IGNORE_RADIATION = 1u << 5;
+constexpr uint32_t FLAG_PRIMARY = 1u << 6;
+constexpr uint32_t FLAG_SECONDARY = 1u << 7;
+
+__host__ __device__ inline int ptc_species(uint32_t flag) {
+  return (int)(flag >> 29);
+}
+
+// Stagger codes of the Yee components (fields_impl.hpp:85-94): bit d set =
+// the component sits on the upper node in direction d, clear = cell centre.
+constexpr int ST_E1 = 0x6, ST_E2 = 0x5, ST_E3 = 0x3;
+constexpr int ST_B1 = 0x1, ST_B2 = 0x2, ST_B3 = 0x4;
+
+// Mesh with guard cells, x fastest (Quadmesh, quadmesh.h:16-402).
+struct Mesh {
+  int dim;
+  int dims[3];
+  int guard[3];
+  int N[3];
+  int periodic[3];
+  double delta[3];
+  double inv_delta[3];
+  double lower[3];
+
+  __host__ __device__ inline uint64_t size() const {
+    return (uint64_t)dims[0] * dims[1] * dims[2];
+  }
+  __host__ __device__ inline bool in_bulk(int c1, int c2, int c3) const {
+    return (c1 >= guard[0] && c1 < dims[0] - guard[0]) &&
+           (c2 >= guard[1] && c2 < dims[1] - guard[1]) &&
+           (c3 >= guard[2] && c3 < dims[2] - guard[2]);
+  }
+  __host__ __device__ inline void split(uint32_t c, int &c1, int &c2,
+                                        int &c3) const {
+    c1 = (int)(c % (uint32_t)dims[0]);
+    c2 = (int)((c / (uint32_t)dims[0]) % (uint32_t)dims[1]);
+    c3 = (int)(c / ((uint32_t)dims[0] * (uint32_t)dims[1]));
+  }
+  // periodic wrap of an index (replaces the MPI self-exchange fold and
+  // particle migration of a single periodic rank)
+  __host__ __device__ inline int wrap(int d, int i) const {
+    if (!periodic[d]) return i;
+    if (i < guard[d]) return i + N[d];
+    if (i >= N[d] + guard[d]) return i - N[d];
+    return i;
+  }
+  __host__ __device__ inline uint64_t idx(int i, int j, int k) const {
+    return (uint64_t)i + (uint64_t)j * dims[0] +
+           (uint64_t)k * dims[0] * dims[1];
+  }
+  __host__ __device__ inline uint64_t widx(int i, int j, int k) const {
+    return idx(wrap(0, i), wrap(1, j), wrap(2, k));
+  }
+};
+
+// Charges and masses per species (sim_environment_impl.hpp:87-98).
+struct Species {
+  double q[8];
+  double m[8];
+};
+
+// ---------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  Counter-based: a draw is a pure
+// function of (key, counter), independent of launch shape.
+// ---------------------------------------------------------------------
+__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0,
+                                              uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    uint64_t pa = (uint64_t)0xD2511F53u * c[0];
+    uint64_t pb = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t n0 = (uint32_t)(pb >> 32) ^ c[1] ^ k0;
+    uint32_t n2 = (uint32_t)(pa >> 32) ^ c[3] ^ k1;
+    c[1] = (uint32_t)pb;
+    c[3] = (uint32_t)pa;
+    c[0] = n0;
+    c[2] = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// Two uniforms in (0,1] (53 bits each) for (seed, pid, draw, purpose).
+__host__ __device__ inline void uniform2(uint64_t seed, uint64_t pid,
+                                         uint32_t draw, uint32_t purpose,
+                                         double &u0, double &u1) {
+  uint32_t c[4] = {(uint32_t)pid, (uint32_t)(pid >> 32), draw, purpose};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  uint64_t a = (((uint64_t)c[0] << 32) | c[1]) >> 11;
+  uint64_t b = (((uint64_t)c[2] << 32) | c[3]) >> 11;
+  u0 = (double)(a + 1) * 0x1p-53;
+  u1 = (double)(b + 1) * 0x1p-53;
+}
+
+// Natural log of a positive normal double: x = m 2^e, m in [sqrt(1/2),
+// sqrt(2)), ln m = 2 atanh(f) = 2f sum_k s^k/(2k+1), f = (m-1)/(m+1),
+// s = f^2 (12 terms; |f| <= 0.1716), plus e ln2 with ln2 split hi/lo (Cody &
+// Waite).  Written out without contraction so the FP64 loaders produce the
+// same bits on every device and host.
+__host__ __device__ inline double apt_log(double x) {
+  union {
+    double d;
+    uint64_t u;
+  } v;
+  v.d = x;
+  int e = (int)((v.u >> 52) & 0x7ff) - 1023;
+  v.u = (v.u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;
+  double m = v.d;
+  if (m > 1.4142135623730951) {
+    m = m * 0.5;
+    e = e + 1;
+  }
+  double f = (m - 1.0) / (m + 1.0);
+  double s = f * f;
+  double p = 1.0 / 23.0;
+  p = p * s + 1.0 / 21.0;
+  p = p * s + 1.0 / 19.0;
+  p = p * s + 1.0 / 17.0;
+  p = p * s + 1.0 / 15.0;
+  p = p * s + 1.0 / 13.0;
+  p = p * s + 1.0 / 11.0;
+  p = p * s + 1.0 / 9.0;
+  p = p * s + 1.0 / 7.0;
+  p = p * s + 1.0 / 5.0;
+  p = p * s + 1.0 / 3.0;
+  p = p * s + 1.0;
+  double lnm = 2.0 * f * p;
+  const double ln2_hi = 6.93147180369123816490e-01;
+  const double ln2_lo = 1.90821492927058770002e-10;
+  double de = (double)e;
+  return de * ln2_hi + (de * ln2_lo + lnm);
+}
+
+// ---------------------------------------------------------------------
+// FP64 fixed point for the current: c = hi 2^-H + lo 2^-(H+40).  The int64
+// sums are exact and order independent, so the deposit is deterministic
+// (the reference uses float atomicAdd, ptc_updater.cu:278-289).
+// ---------------------------------------------------------------------
+__device__ inline void fx_split(double c, int H, long long &hi,
+                                long long &lo) {
+  double x = ldexp(c, H);
+  double t = trunc(x);
+  double r = x - t;
+  hi = (long long)t;
+  lo = (long long)rint(r * 0x1p40);
+}
+
+__host__ __device__ inline double fx_value(long long hi, long long lo,
+                                           int H) {
+  return ldexp((double)hi, -H) + ldexp((double)lo, -(H + LIMB_BITS));
+}
+
+}  // namespace apt

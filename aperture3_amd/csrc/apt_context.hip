@@ -1,0 +1,880 @@
+// apt_context.hip -- the C ABI (include/aperture_amd.h): context lifetime,
+// device memory, host<->device transfers and the step sequencing of
+// ptc_updater::update_particles (ptc_updater.cu:748-841) and
+// field_solver::update_fields (field_solver.cu:334-358).  All work is
+// enqueued on the context's stream; nothing inside a step reads a count
+// back to the host.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/aperture_amd.h"
+#include "apt_common.hpp"
+#include "apt_kernels.hpp"
+#include "apt_sort.hpp"
+
+using namespace apt;
+
+struct EventPair {
+  hipEvent_t a, b;
+};
+
+struct apt_ctx {
+  apt_grid grid;
+  apt_params params;
+  Mesh mesh;
+  Species sp;
+  hipStream_t stream = nullptr;
+  int prec = APT_PREC_FP64;
+  uint64_t ncell = 0;
+
+  // fields (double* in the FP64 form, float* in the mixed form)
+  void *E[3] = {}, *B[3] = {}, *Ebg[3] = {}, *Bbg[3] = {}, *J[3] = {};
+  void *Rho[4] = {}, *divE = nullptr, *divB = nullptr, *jtmp = nullptr;
+  long long *Jacc[3][2] = {};
+  long long *Racc[4][2] = {};
+
+  // particles (double buffered for the sort gather)
+  uint64_t cap = 0, number = 0;
+  PtcF64 p64{}, p64b{};
+  PtcF32 p32{}, p32b{};
+  bool have_b = false;
+  unsigned long long *d_counters = nullptr;
+  SortWork sortw;
+
+  double wmax = 1.0;  // max |q w| over particles, sets the fixed-point scale
+  int H = 22;
+
+  // mixed-form tiling
+  uint32_t *tile_off = nullptr;
+  uint32_t *tile_key_lut = nullptr;
+  TileInfo tiles{};
+  bool tiles_valid = false;
+
+  std::string err;
+  bool timing = false;
+  std::vector<EventPair> ev[APT_KERNEL_NUM];
+  size_t ev_used[APT_KERNEL_NUM] = {};
+
+  apt_status fail(apt_status code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+  }
+  size_t esize() const { return prec == APT_PREC_FP64 ? 8 : 4; }
+};
+
+#define HIPCHK(call)                                                    \
+  do {                                                                  \
+    hipError_t e_ = (call);                                             \
+    if (e_ != hipSuccess)                                               \
+      return ctx->fail(APT_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHECK_CTX(ctx) \
+  do {                 \
+    if (!(ctx)) return APT_ERR_INVALID_ARG; \
+  } while (0)
+
+static int ceil_log2(double x) {
+  int e = 0;
+  double v = 1.0;
+  while (v < x && e < 60) {
+    v *= 2.0;
+    e++;
+  }
+  while (v / 2.0 >= x && e > -60) {
+    v /= 2.0;
+    e--;
+  }
+  return e;
+}
+
+// fixed-point exponent for the current from the largest |q w|: FP64 keeps
+// |J| < 2^41 wmax in the high limb; the mixed form rounds every
+// contribution to 2^-H and keeps |J| < 2^(63-H).
+static void update_scale(apt_ctx *ctx) {
+  int e = ceil_log2(ctx->wmax > 0 ? ctx->wmax : 1.0);
+  ctx->H = (ctx->prec == APT_PREC_FP64) ? 22 - e : 40 - e;
+}
+
+static apt_status alloc_ptc(apt_ctx *ctx, bool second) {
+  uint64_t n = ctx->cap;
+  if (ctx->prec == APT_PREC_FP64) {
+    PtcF64 &p = second ? ctx->p64b : ctx->p64;
+    HIPCHK(hipMalloc(&p.cell, n * 4));
+    HIPCHK(hipMalloc(&p.x1, n * 8));
+    HIPCHK(hipMalloc(&p.x2, n * 8));
+    HIPCHK(hipMalloc(&p.x3, n * 8));
+    HIPCHK(hipMalloc(&p.p1, n * 8));
+    HIPCHK(hipMalloc(&p.p2, n * 8));
+    HIPCHK(hipMalloc(&p.p3, n * 8));
+    HIPCHK(hipMalloc(&p.E, n * 8));
+    HIPCHK(hipMalloc(&p.w, n * 8));
+    HIPCHK(hipMalloc(&p.flag, n * 4));
+    HIPCHK(hipMalloc(&p.id, n * 8));
+    HIPCHK(launch_fill<uint32_t>(p.cell, MAX_CELL, n, ctx->stream));
+  } else {
+    PtcF32 &p = second ? ctx->p32b : ctx->p32;
+    HIPCHK(hipMalloc(&p.cell, n * 4));
+    HIPCHK(hipMalloc(&p.x1, n * 4));
+    HIPCHK(hipMalloc(&p.x2, n * 4));
+    HIPCHK(hipMalloc(&p.x3, n * 4));
+    HIPCHK(hipMalloc(&p.u1, n * 4));
+    HIPCHK(hipMalloc(&p.u2, n * 4));
+    HIPCHK(hipMalloc(&p.u3, n * 4));
+    HIPCHK(hipMalloc(&p.w, n * 4));
+    HIPCHK(hipMalloc(&p.flag, n * 4));
+    HIPCHK(launch_fill<uint32_t>(p.cell, MAX_CELL, n, ctx->stream));
+  }
+  return APT_OK;
+}
+
+static void free_ptc64(PtcF64 &p) {
+  (void)hipFree(p.cell); (void)hipFree(p.x1); (void)hipFree(p.x2); (void)hipFree(p.x3);
+  (void)hipFree(p.p1); (void)hipFree(p.p2); (void)hipFree(p.p3); (void)hipFree(p.E); (void)hipFree(p.w);
+  (void)hipFree(p.flag); (void)hipFree(p.id);
+  p = PtcF64{};
+}
+static void free_ptc32(PtcF32 &p) {
+  (void)hipFree(p.cell); (void)hipFree(p.x1); (void)hipFree(p.x2); (void)hipFree(p.x3);
+  (void)hipFree(p.u1); (void)hipFree(p.u2); (void)hipFree(p.u3); (void)hipFree(p.w);
+  (void)hipFree(p.flag);
+  p = PtcF32{};
+}
+
+// ---- timing helpers ----------------------------------------------------
+static void time_begin(apt_ctx *ctx, int k) {
+  if (!ctx->timing) return;
+  auto &v = ctx->ev[k];
+  if (ctx->ev_used[k] == v.size()) {
+    EventPair p;
+    (void)hipEventCreate(&p.a);
+    (void)hipEventCreate(&p.b);
+    v.push_back(p);
+  }
+  (void)hipEventRecord(v[ctx->ev_used[k]].a, ctx->stream);
+}
+static void time_end(apt_ctx *ctx, int k) {
+  if (!ctx->timing) return;
+  (void)hipEventRecord(ctx->ev[k][ctx->ev_used[k]].b, ctx->stream);
+  ctx->ev_used[k]++;
+}
+
+// ------------------------------------------------------------------------
+// Entry points: C linkage comes from the declarations in aperture_amd.h.
+
+apt_status apt_create(const apt_grid *grid, const apt_params *params,
+                      apt_ctx **out) {
+  if (!grid || !params || !out) return APT_ERR_INVALID_ARG;
+  *out = nullptr;
+  int devs = 0;
+  if (hipGetDeviceCount(&devs) != hipSuccess || devs < 1)
+    return APT_ERR_HIP;
+  apt_ctx *ctx = new apt_ctx();
+  ctx->grid = *grid;
+  ctx->params = *params;
+  ctx->prec = params->precision;
+  if (ctx->prec != APT_PREC_FP64 && ctx->prec != APT_PREC_MIXED) {
+    delete ctx;
+    return APT_ERR_INVALID_ARG;
+  }
+  Mesh &m = ctx->mesh;
+  m.dim = 0;
+  for (int d = 0; d < 3; d++) {
+    if (grid->N[d] < 1 || grid->guard[d] < 0) {
+      delete ctx;
+      return APT_ERR_INVALID_ARG;
+    }
+    m.N[d] = grid->N[d];
+    m.guard[d] = grid->guard[d];
+    m.dims[d] = grid->N[d] + 2 * grid->guard[d];
+    m.periodic[d] = grid->periodic[d] ? 1 : 0;
+    m.lower[d] = grid->lower[d];
+    m.delta[d] = grid->size[d] / grid->N[d];
+    m.inv_delta[d] = 1.0 / m.delta[d];
+    if (grid->N[d] > 1) m.dim = d + 1;
+  }
+  if (m.dim == 0) m.dim = 1;
+  for (int d = 0; d < m.dim; d++)
+    if (m.guard[d] < 2) {
+      delete ctx;
+      return APT_ERR_INVALID_ARG;  // CIC + one-cell moves need guard >= 2
+    }
+  ctx->ncell = m.size();
+  if (ctx->ncell >= 0xffffffffull) {
+    delete ctx;
+    return APT_ERR_INVALID_ARG;
+  }
+  // charges and masses (sim_environment_impl.hpp:87-98)
+  for (int i = 0; i < 8; i++) {
+    ctx->sp.q[i] = params->q_e;
+    ctx->sp.m[i] = params->q_e;
+  }
+  ctx->sp.q[0] = -params->q_e;
+  ctx->sp.m[2] = params->q_e * params->ion_mass;
+  ctx->wmax = fabs(params->q_e);
+  update_scale(ctx);
+
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) !=
+      hipSuccess) {
+    delete ctx;
+    return APT_ERR_HIP;
+  }
+  apt_status st = APT_OK;
+  const size_t es = ctx->esize();
+  const uint64_t nc = ctx->ncell;
+  auto alloc_field = [&](void **p) -> bool {
+    if (hipMalloc(p, nc * es) != hipSuccess) return false;
+    if (ctx->prec == APT_PREC_FP64)
+      return launch_fill<double>((double *)*p, 0.0, nc, ctx->stream) ==
+             hipSuccess;
+    return launch_fill<float>((float *)*p, 0.0f, nc, ctx->stream) ==
+           hipSuccess;
+  };
+  bool ok = true;
+  for (int c = 0; c < 3 && ok; c++)
+    ok = alloc_field(&ctx->E[c]) && alloc_field(&ctx->B[c]) &&
+         alloc_field(&ctx->Ebg[c]) && alloc_field(&ctx->Bbg[c]) &&
+         alloc_field(&ctx->J[c]);
+  int ns = params->num_species < 1 ? 1 : params->num_species;
+  if (ns > 4) ns = 4;
+  ctx->params.num_species = ns;
+  for (int s = 0; s < ns && ok; s++) ok = alloc_field(&ctx->Rho[s]);
+  ok = ok && alloc_field(&ctx->divE) && alloc_field(&ctx->divB) &&
+       alloc_field(&ctx->jtmp);
+  int limbs = ctx->prec == APT_PREC_FP64 ? 2 : 1;
+  for (int c = 0; c < 3 && ok; c++)
+    for (int l = 0; l < limbs && ok; l++)
+      ok = hipMalloc(&ctx->Jacc[c][l], nc * 8) == hipSuccess &&
+           launch_fill<long long>(ctx->Jacc[c][l], 0, nc, ctx->stream) ==
+               hipSuccess;
+  for (int s = 0; s < ns && ok; s++)
+    for (int l = 0; l < limbs && ok; l++)
+      ok = hipMalloc(&ctx->Racc[s][l], nc * 8) == hipSuccess &&
+           launch_fill<long long>(ctx->Racc[s][l], 0, nc, ctx->stream) ==
+               hipSuccess;
+  ok = ok && hipMalloc(&ctx->d_counters, CTR_NUM * 8) == hipSuccess &&
+       hipMemsetAsync(ctx->d_counters, 0, CTR_NUM * 8, ctx->stream) ==
+           hipSuccess;
+  if (!ok) {
+    apt_destroy(ctx);
+    return APT_ERR_HIP;
+  }
+  ctx->cap = params->max_ptc_number;
+  if (ctx->cap > 0xffffffffull) {
+    apt_destroy(ctx);
+    return APT_ERR_INVALID_ARG;
+  }
+  if (ctx->cap > 0) st = alloc_ptc(ctx, false);
+  if (st != APT_OK) {
+    apt_destroy(ctx);
+    return st;
+  }
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    apt_destroy(ctx);
+    return APT_ERR_HIP;
+  }
+  *out = ctx;
+  return APT_OK;
+}
+
+apt_status apt_destroy(apt_ctx *ctx) {
+  CHECK_CTX(ctx);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (int c = 0; c < 3; c++) {
+    (void)hipFree(ctx->E[c]); (void)hipFree(ctx->B[c]); (void)hipFree(ctx->Ebg[c]);
+    (void)hipFree(ctx->Bbg[c]); (void)hipFree(ctx->J[c]);
+    (void)hipFree(ctx->Jacc[c][0]); (void)hipFree(ctx->Jacc[c][1]);
+  }
+  for (int s = 0; s < 4; s++) {
+    (void)hipFree(ctx->Rho[s]);
+    (void)hipFree(ctx->Racc[s][0]);
+    (void)hipFree(ctx->Racc[s][1]);
+  }
+  (void)hipFree(ctx->divE); (void)hipFree(ctx->divB); (void)hipFree(ctx->jtmp);
+  (void)hipFree(ctx->d_counters);
+  (void)hipFree(ctx->tile_off); (void)hipFree(ctx->tile_key_lut);
+  free_ptc64(ctx->p64); free_ptc64(ctx->p64b);
+  free_ptc32(ctx->p32); free_ptc32(ctx->p32b);
+  ctx->sortw.release();
+  for (int k = 0; k < APT_KERNEL_NUM; k++)
+    for (auto &p : ctx->ev[k]) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return APT_OK;
+}
+
+const char *apt_last_error(const apt_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+apt_status apt_get_jexp(const apt_ctx *ctx, int *H) {
+  CHECK_CTX(ctx);
+  if (!H) return APT_ERR_INVALID_ARG;
+  *H = ctx->H;
+  return APT_OK;
+}
+
+apt_status apt_get_mesh(const apt_ctx *ctx, int dims[3], double delta[3]) {
+  CHECK_CTX(ctx);
+  for (int d = 0; d < 3; d++) {
+    if (dims) dims[d] = ctx->mesh.dims[d];
+    if (delta) delta[d] = ctx->mesh.delta[d];
+  }
+  return APT_OK;
+}
+
+static void **field_slot(apt_ctx *ctx, int which, int comp) {
+  if (comp < 0) return nullptr;
+  switch (which) {
+    case APT_FIELD_E: return comp < 3 ? &ctx->E[comp] : nullptr;
+    case APT_FIELD_B: return comp < 3 ? &ctx->B[comp] : nullptr;
+    case APT_FIELD_EBG: return comp < 3 ? &ctx->Ebg[comp] : nullptr;
+    case APT_FIELD_BBG: return comp < 3 ? &ctx->Bbg[comp] : nullptr;
+    case APT_FIELD_J: return comp < 3 ? &ctx->J[comp] : nullptr;
+    case APT_FIELD_RHO:
+      return comp < ctx->params.num_species ? &ctx->Rho[comp] : nullptr;
+    case APT_FIELD_DIVE: return comp == 0 ? &ctx->divE : nullptr;
+    case APT_FIELD_DIVB: return comp == 0 ? &ctx->divB : nullptr;
+  }
+  return nullptr;
+}
+
+apt_status apt_field_set(apt_ctx *ctx, int which, int comp,
+                         const double *src, uint64_t count) {
+  CHECK_CTX(ctx);
+  void **slot = field_slot(ctx, which, comp);
+  if (!slot || !src || count != ctx->ncell)
+    return ctx->fail(APT_ERR_INVALID_ARG, "field_set: bad field/size");
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (ctx->prec == APT_PREC_FP64) {
+    HIPCHK(hipMemcpy(*slot, src, count * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> tmp(count);
+    for (uint64_t i = 0; i < count; i++) tmp[i] = (float)src[i];
+    HIPCHK(hipMemcpy(*slot, tmp.data(), count * 4, hipMemcpyHostToDevice));
+  }
+  return APT_OK;
+}
+
+apt_status apt_field_get(apt_ctx *ctx, int which, int comp, double *dst,
+                         uint64_t count) {
+  CHECK_CTX(ctx);
+  void **slot = field_slot(ctx, which, comp);
+  if (!slot || !dst || count != ctx->ncell)
+    return ctx->fail(APT_ERR_INVALID_ARG, "field_get: bad field/size");
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (ctx->prec == APT_PREC_FP64) {
+    HIPCHK(hipMemcpy(dst, *slot, count * 8, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp(count);
+    HIPCHK(hipMemcpy(tmp.data(), *slot, count * 4, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < count; i++) dst[i] = tmp[i];
+  }
+  return APT_OK;
+}
+
+apt_status apt_ptc_set(apt_ctx *ctx, const apt_particles *src, uint64_t n) {
+  CHECK_CTX(ctx);
+  if (!src || !src->cell || !src->x1 || !src->x2 || !src->x3 || !src->p1 ||
+      !src->p2 || !src->p3 || !src->weight || !src->flag)
+    return ctx->fail(APT_ERR_INVALID_ARG, "ptc_set: missing arrays");
+  if (n > ctx->cap)
+    return ctx->fail(APT_ERR_CAPACITY, "ptc_set: %llu > capacity %llu",
+                     (unsigned long long)n, (unsigned long long)ctx->cap);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  double wmax = fabs(ctx->params.q_e);
+  for (uint64_t i = 0; i < n; i++) {
+    int s = (int)(src->flag[i] >> 29);
+    double qw = fabs(ctx->sp.q[s] * src->weight[i]);
+    if (qw > wmax) wmax = qw;
+  }
+  ctx->wmax = wmax;
+  update_scale(ctx);
+  if (ctx->prec == APT_PREC_FP64) {
+    PtcF64 &p = ctx->p64;
+    HIPCHK(hipMemcpy(p.cell, src->cell, n * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p.x1, src->x1, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p.x2, src->x2, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p.x3, src->x3, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p.p1, src->p1, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p.p2, src->p2, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p.p3, src->p3, n * 8, hipMemcpyHostToDevice));
+    if (src->E) {
+      HIPCHK(hipMemcpy(p.E, src->E, n * 8, hipMemcpyHostToDevice));
+    } else {
+      std::vector<double> g(n);
+      for (uint64_t i = 0; i < n; i++)
+        g[i] = sqrt(1.0 + src->p1[i] * src->p1[i] + src->p2[i] * src->p2[i] +
+                    src->p3[i] * src->p3[i]);
+      HIPCHK(hipMemcpy(p.E, g.data(), n * 8, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(p.w, src->weight, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p.flag, src->flag, n * 4, hipMemcpyHostToDevice));
+    if (src->id) {
+      HIPCHK(hipMemcpy(p.id, src->id, n * 8, hipMemcpyHostToDevice));
+    } else {
+      std::vector<uint64_t> ids(n);
+      for (uint64_t i = 0; i < n; i++) ids[i] = i;
+      HIPCHK(hipMemcpy(p.id, ids.data(), n * 8, hipMemcpyHostToDevice));
+    }
+  } else {
+    PtcF32 &p = ctx->p32;
+    std::vector<float> f(n);
+    auto put = [&](float *d, const double *s) -> hipError_t {
+      for (uint64_t i = 0; i < n; i++) f[i] = (float)s[i];
+      return hipMemcpy(d, f.data(), n * 4, hipMemcpyHostToDevice);
+    };
+    auto put_off = [&](float *d, const double *s) -> hipError_t {
+      for (uint64_t i = 0; i < n; i++) {
+        float v = (float)s[i];
+        f[i] = v >= 1.0f ? 0x1.fffffep-1f : v;
+      }
+      return hipMemcpy(d, f.data(), n * 4, hipMemcpyHostToDevice);
+    };
+    HIPCHK(hipMemcpy(p.cell, src->cell, n * 4, hipMemcpyHostToDevice));
+    HIPCHK(put_off(p.x1, src->x1));
+    HIPCHK(ctx->mesh.dim > 1 ? put_off(p.x2, src->x2) : put(p.x2, src->x2));
+    HIPCHK(ctx->mesh.dim > 2 ? put_off(p.x3, src->x3) : put(p.x3, src->x3));
+    HIPCHK(put(p.u1, src->p1));
+    HIPCHK(put(p.u2, src->p2));
+    HIPCHK(put(p.u3, src->p3));
+    HIPCHK(put(p.w, src->weight));
+    HIPCHK(hipMemcpy(p.flag, src->flag, n * 4, hipMemcpyHostToDevice));
+  }
+  // slots past n are empty
+  if (n < ctx->number)
+    HIPCHK(launch_fill<uint32_t>(
+        (ctx->prec == APT_PREC_FP64 ? ctx->p64.cell : ctx->p32.cell) + n,
+        MAX_CELL, ctx->number - n, ctx->stream));
+  ctx->number = n;
+  ctx->tiles_valid = false;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return APT_OK;
+}
+
+apt_status apt_ptc_get(apt_ctx *ctx, apt_particles *dst, uint64_t cap,
+                       uint64_t *n) {
+  CHECK_CTX(ctx);
+  if (!dst) return APT_ERR_INVALID_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  uint64_t num = ctx->number;
+  if (n) *n = num;
+  if (num > cap)
+    return ctx->fail(APT_ERR_CAPACITY, "ptc_get: %llu particles > cap %llu",
+                     (unsigned long long)num, (unsigned long long)cap);
+  if (ctx->prec == APT_PREC_FP64) {
+    PtcF64 &p = ctx->p64;
+    if (dst->cell) HIPCHK(hipMemcpy(dst->cell, p.cell, num * 4, hipMemcpyDeviceToHost));
+    if (dst->x1) HIPCHK(hipMemcpy(dst->x1, p.x1, num * 8, hipMemcpyDeviceToHost));
+    if (dst->x2) HIPCHK(hipMemcpy(dst->x2, p.x2, num * 8, hipMemcpyDeviceToHost));
+    if (dst->x3) HIPCHK(hipMemcpy(dst->x3, p.x3, num * 8, hipMemcpyDeviceToHost));
+    if (dst->p1) HIPCHK(hipMemcpy(dst->p1, p.p1, num * 8, hipMemcpyDeviceToHost));
+    if (dst->p2) HIPCHK(hipMemcpy(dst->p2, p.p2, num * 8, hipMemcpyDeviceToHost));
+    if (dst->p3) HIPCHK(hipMemcpy(dst->p3, p.p3, num * 8, hipMemcpyDeviceToHost));
+    if (dst->E) HIPCHK(hipMemcpy(dst->E, p.E, num * 8, hipMemcpyDeviceToHost));
+    if (dst->weight) HIPCHK(hipMemcpy(dst->weight, p.w, num * 8, hipMemcpyDeviceToHost));
+    if (dst->flag) HIPCHK(hipMemcpy(dst->flag, p.flag, num * 4, hipMemcpyDeviceToHost));
+    if (dst->id) HIPCHK(hipMemcpy(dst->id, p.id, num * 8, hipMemcpyDeviceToHost));
+  } else {
+    PtcF32 &p = ctx->p32;
+    std::vector<float> f(num), u1(num), u2(num), u3(num);
+    auto get = [&](double *d, const float *s) -> hipError_t {
+      hipError_t e = hipMemcpy(f.data(), s, num * 4, hipMemcpyDeviceToHost);
+      if (e == hipSuccess && d)
+        for (uint64_t i = 0; i < num; i++) d[i] = f[i];
+      return e;
+    };
+    if (dst->cell) HIPCHK(hipMemcpy(dst->cell, p.cell, num * 4, hipMemcpyDeviceToHost));
+    if (dst->x1) HIPCHK(get(dst->x1, p.x1));
+    if (dst->x2) HIPCHK(get(dst->x2, p.x2));
+    if (dst->x3) HIPCHK(get(dst->x3, p.x3));
+    HIPCHK(hipMemcpy(u1.data(), p.u1, num * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(u2.data(), p.u2, num * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(u3.data(), p.u3, num * 4, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < num; i++) {
+      if (dst->p1) dst->p1[i] = u1[i];
+      if (dst->p2) dst->p2[i] = u2[i];
+      if (dst->p3) dst->p3[i] = u3[i];
+      if (dst->E)
+        dst->E[i] = sqrt(1.0 + (double)u1[i] * u1[i] + (double)u2[i] * u2[i] +
+                         (double)u3[i] * u3[i]);
+      if (dst->id) dst->id[i] = i;
+    }
+    if (dst->weight) HIPCHK(get(dst->weight, p.w));
+    if (dst->flag) HIPCHK(hipMemcpy(dst->flag, p.flag, num * 4, hipMemcpyDeviceToHost));
+  }
+  return APT_OK;
+}
+
+apt_status apt_ptc_number(apt_ctx *ctx, uint64_t *n) {
+  CHECK_CTX(ctx);
+  if (!n) return APT_ERR_INVALID_ARG;
+  *n = ctx->number;
+  return APT_OK;
+}
+
+apt_status apt_ptc_load_uniform(apt_ctx *ctx, int ppc_pairs,
+                                const double theta[2], const double drift[2],
+                                double weight) {
+  CHECK_CTX(ctx);
+  if (ppc_pairs < 1 || !theta || !drift)
+    return ctx->fail(APT_ERR_INVALID_ARG, "load_uniform: bad arguments");
+  const Mesh &m = ctx->mesh;
+  uint64_t bulk = (uint64_t)m.N[0] * m.N[1] * m.N[2];
+  uint64_t add = bulk * (uint64_t)ppc_pairs * 2;
+  uint64_t base = ctx->number + (ctx->number & 1);
+  if (base + add > ctx->cap)
+    return ctx->fail(APT_ERR_CAPACITY, "load_uniform: %llu particles > cap",
+                     (unsigned long long)(base + add));
+  LoadParams lp;
+  lp.ppc = ppc_pairs;
+  lp.theta[0] = theta[0];
+  lp.theta[1] = theta[1];
+  lp.drift[0] = drift[0];
+  lp.drift[1] = drift[1];
+  lp.weight = weight;
+  lp.seed = ctx->params.random_seed;
+  lp.base = base;
+  double qw = fabs(ctx->params.q_e * weight);
+  if (ctx->number == 0 || qw > ctx->wmax) ctx->wmax = qw;
+  update_scale(ctx);
+  if (ctx->prec == APT_PREC_FP64)
+    HIPCHK(launch_load_uniform_f64(m, lp, ctx->p64, ctx->d_counters,
+                                   ctx->stream));
+  else
+    HIPCHK(launch_load_uniform_f32(m, lp, ctx->p32, ctx->d_counters,
+                                   ctx->stream));
+  ctx->number = base + add;
+  ctx->tiles_valid = false;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return APT_OK;
+}
+
+// ---- the particle step --------------------------------------------------
+static apt_status filter_pass(apt_ctx *ctx, void *arr) {
+  if (ctx->prec == APT_PREC_FP64)
+    HIPCHK(launch_filter<double>(ctx->mesh, (double *)arr,
+                                 (double *)ctx->jtmp,
+                                 ctx->params.fix_3d_filter, ctx->stream));
+  else
+    HIPCHK(launch_filter<float>(ctx->mesh, (float *)arr, (float *)ctx->jtmp,
+                                ctx->params.fix_3d_filter, ctx->stream));
+  return APT_OK;
+}
+
+static apt_status smooth(apt_ctx *ctx, int rho) {
+  for (int pass = 0; pass < ctx->params.current_smoothing; pass++) {
+    for (int c = 0; c < 3; c++) {
+      apt_status st = filter_pass(ctx, ctx->J[c]);
+      if (st != APT_OK) return st;
+    }
+    if (rho)
+      for (int s = 0; s < ctx->params.num_species; s++) {
+        apt_status st = filter_pass(ctx, ctx->Rho[s]);
+        if (st != APT_OK) return st;
+      }
+  }
+  return APT_OK;
+}
+
+apt_status apt_smooth_current(apt_ctx *ctx, uint32_t step) {
+  CHECK_CTX(ctx);
+  const int di = ctx->params.data_interval > 0 ? ctx->params.data_interval : 1;
+  return smooth(ctx, ((step + 1) % (uint32_t)di) == 0);
+}
+
+apt_status apt_ptc_update(apt_ctx *ctx, double dt, uint32_t step) {
+  CHECK_CTX(ctx);
+  const int di = ctx->params.data_interval > 0 ? ctx->params.data_interval : 1;
+  const int rho = ((step + 1) % (uint32_t)di) == 0;
+  const uint64_t nc = ctx->ncell;
+  const int ns = ctx->params.num_species;
+  if (ctx->prec == APT_PREC_FP64) {
+    FieldsF64 f;
+    for (int c = 0; c < 3; c++) {
+      f.E[c] = (const double *)ctx->E[c];
+      f.B[c] = (const double *)ctx->B[c];
+    }
+    AccF64 a;
+    for (int c = 0; c < 3; c++) {
+      a.J[c][0] = ctx->Jacc[c][0];
+      a.J[c][1] = ctx->Jacc[c][1];
+    }
+    for (int s = 0; s < 4; s++) {
+      a.R[s][0] = ctx->Racc[s][0];
+      a.R[s][1] = ctx->Racc[s][1];
+    }
+    time_begin(ctx, APT_KERNEL_PUSH_DEPOSIT);
+    HIPCHK(launch_push_deposit_f64(ctx->mesh, ctx->sp, dt, ctx->H, rho,
+                                   ctx->p64, f, a, ctx->d_counters,
+                                   ctx->number, ctx->stream));
+    time_end(ctx, APT_KERNEL_PUSH_DEPOSIT);
+    for (int c = 0; c < 3; c++)
+      HIPCHK(launch_finalize_f64(ctx->Jacc[c][0], ctx->Jacc[c][1],
+                                 (double *)ctx->J[c], nc, ctx->H,
+                                 ctx->stream));
+    for (int s = 0; s < ns; s++) {
+      if (rho)
+        HIPCHK(launch_finalize_f64(ctx->Racc[s][0], ctx->Racc[s][1],
+                                   (double *)ctx->Rho[s], nc, ctx->H,
+                                   ctx->stream));
+      else
+        HIPCHK(launch_fill<double>((double *)ctx->Rho[s], 0.0, nc,
+                                   ctx->stream));
+    }
+  } else {
+    FieldsF32 f;
+    for (int c = 0; c < 3; c++) {
+      f.E[c] = (const float *)ctx->E[c];
+      f.B[c] = (const float *)ctx->B[c];
+    }
+    TileInfo ti = ctx->tiles;
+    if (!ctx->tiles_valid) {
+      ti.tile_off = nullptr;
+      ti.ntiles = 0;
+    }
+    time_begin(ctx, APT_KERNEL_PUSH_DEPOSIT);
+    HIPCHK(launch_push_deposit_mixed(ctx->mesh, ctx->sp, (float)dt, ctx->H,
+                                     ctx->p32, f, ctx->Jacc[0][0],
+                                     ctx->Jacc[1][0], ctx->Jacc[2][0], ti,
+                                     ctx->d_counters, ctx->number,
+                                     ctx->stream));
+    time_end(ctx, APT_KERNEL_PUSH_DEPOSIT);
+    for (int c = 0; c < 3; c++)
+      HIPCHK(launch_finalize_mixed(ctx->Jacc[c][0], (float *)ctx->J[c], nc,
+                                   ctx->H, ctx->stream));
+    for (int s = 0; s < ns; s++)
+      HIPCHK(launch_fill<float>((float *)ctx->Rho[s], 0.0f, nc, ctx->stream));
+  }
+  return smooth(ctx, rho);
+}
+
+// ---- fields --------------------------------------------------------------
+template <typename T>
+static FieldSet<T> field_set(apt_ctx *ctx) {
+  FieldSet<T> f;
+  for (int c = 0; c < 3; c++) {
+    f.E[c] = (T *)ctx->E[c];
+    f.B[c] = (T *)ctx->B[c];
+    f.Ebg[c] = (const T *)ctx->Ebg[c];
+    f.Bbg[c] = (const T *)ctx->Bbg[c];
+    f.J[c] = (const T *)ctx->J[c];
+  }
+  f.divE = (T *)ctx->divE;
+  f.divB = (T *)ctx->divB;
+  return f;
+}
+
+template <typename T>
+static apt_status guard3(apt_ctx *ctx, void *const *arr) {
+  for (int c = 0; c < 3; c++)
+    HIPCHK(launch_guard_copy<T>(ctx->mesh, (T *)arr[c], ctx->stream));
+  return APT_OK;
+}
+
+template <typename T>
+static apt_status update_e_t(apt_ctx *ctx, double dt) {
+  HIPCHK(launch_update_e<T>(ctx->mesh, field_set<T>(ctx), dt, ctx->stream));
+  return guard3<T>(ctx, ctx->E);
+}
+template <typename T>
+static apt_status update_b_t(apt_ctx *ctx, double dt) {
+  HIPCHK(launch_update_b<T>(ctx->mesh, field_set<T>(ctx), dt, ctx->stream));
+  return guard3<T>(ctx, ctx->B);
+}
+template <typename T>
+static apt_status divs_t(apt_ctx *ctx) {
+  HIPCHK(launch_divs<T>(ctx->mesh, field_set<T>(ctx), ctx->stream));
+  return APT_OK;
+}
+
+template <typename T>
+static apt_status field_update_t(apt_ctx *ctx, double dt, uint32_t step) {
+  apt_status st;
+  time_begin(ctx, APT_KERNEL_FIELD);
+  if (step == 0 && (st = update_e_t<T>(ctx, 0.5 * dt)) != APT_OK) return st;
+  if ((st = update_b_t<T>(ctx, dt)) != APT_OK) return st;
+  if ((st = update_e_t<T>(ctx, dt)) != APT_OK) return st;
+  if ((st = divs_t<T>(ctx)) != APT_OK) return st;
+  time_end(ctx, APT_KERNEL_FIELD);
+  return APT_OK;
+}
+
+apt_status apt_field_update(apt_ctx *ctx, double dt, uint32_t step) {
+  CHECK_CTX(ctx);
+  return ctx->prec == APT_PREC_FP64 ? field_update_t<double>(ctx, dt, step)
+                                    : field_update_t<float>(ctx, dt, step);
+}
+apt_status apt_field_update_e(apt_ctx *ctx, double dt) {
+  CHECK_CTX(ctx);
+  return ctx->prec == APT_PREC_FP64 ? update_e_t<double>(ctx, dt)
+                                    : update_e_t<float>(ctx, dt);
+}
+apt_status apt_field_update_b(apt_ctx *ctx, double dt) {
+  CHECK_CTX(ctx);
+  return ctx->prec == APT_PREC_FP64 ? update_b_t<double>(ctx, dt)
+                                    : update_b_t<float>(ctx, dt);
+}
+apt_status apt_field_compute_divs(apt_ctx *ctx) {
+  CHECK_CTX(ctx);
+  return ctx->prec == APT_PREC_FP64 ? divs_t<double>(ctx)
+                                    : divs_t<float>(ctx);
+}
+
+// ---- sort -------------------------------------------------------------
+static apt_status ensure_second_buffer(apt_ctx *ctx) {
+  if (ctx->have_b) return APT_OK;
+  apt_status st = alloc_ptc(ctx, true);
+  if (st != APT_OK) return st;
+  ctx->have_b = true;
+  return APT_OK;
+}
+
+apt_status apt_sort_particles(apt_ctx *ctx) {
+  CHECK_CTX(ctx);
+  uint64_t n = ctx->number;
+  if (n == 0) return APT_OK;
+  apt_status st = ensure_second_buffer(ctx);
+  if (st != APT_OK) return st;
+  const uint32_t empty_key = (uint32_t)ctx->ncell;
+  time_begin(ctx, APT_KERNEL_SORT);
+  if (ctx->prec == APT_PREC_FP64) {
+    PtcF64 &a = ctx->p64, &b = ctx->p64b;
+    HIPCHK(sort_keys(ctx->sortw, a.cell, n, empty_key, nullptr, ctx->stream));
+    HIPCHK(gather_field<uint32_t>(ctx->sortw, a.cell, b.cell, n, ctx->stream));
+    HIPCHK(gather_field<double>(ctx->sortw, a.x1, b.x1, n, ctx->stream));
+    HIPCHK(gather_field<double>(ctx->sortw, a.x2, b.x2, n, ctx->stream));
+    HIPCHK(gather_field<double>(ctx->sortw, a.x3, b.x3, n, ctx->stream));
+    HIPCHK(gather_field<double>(ctx->sortw, a.p1, b.p1, n, ctx->stream));
+    HIPCHK(gather_field<double>(ctx->sortw, a.p2, b.p2, n, ctx->stream));
+    HIPCHK(gather_field<double>(ctx->sortw, a.p3, b.p3, n, ctx->stream));
+    HIPCHK(gather_field<double>(ctx->sortw, a.E, b.E, n, ctx->stream));
+    HIPCHK(gather_field<double>(ctx->sortw, a.w, b.w, n, ctx->stream));
+    HIPCHK(gather_field<uint32_t>(ctx->sortw, a.flag, b.flag, n, ctx->stream));
+    HIPCHK(gather_field<uint64_t>(ctx->sortw, a.id, b.id, n, ctx->stream));
+    std::swap(ctx->p64, ctx->p64b);
+  } else {
+    PtcF32 &a = ctx->p32, &b = ctx->p32b;
+    HIPCHK(sort_keys(ctx->sortw, a.cell, n, empty_key, nullptr, ctx->stream));
+    HIPCHK(gather_field<uint32_t>(ctx->sortw, a.cell, b.cell, n, ctx->stream));
+    HIPCHK(gather_field<float>(ctx->sortw, a.x1, b.x1, n, ctx->stream));
+    HIPCHK(gather_field<float>(ctx->sortw, a.x2, b.x2, n, ctx->stream));
+    HIPCHK(gather_field<float>(ctx->sortw, a.x3, b.x3, n, ctx->stream));
+    HIPCHK(gather_field<float>(ctx->sortw, a.u1, b.u1, n, ctx->stream));
+    HIPCHK(gather_field<float>(ctx->sortw, a.u2, b.u2, n, ctx->stream));
+    HIPCHK(gather_field<float>(ctx->sortw, a.u3, b.u3, n, ctx->stream));
+    HIPCHK(gather_field<float>(ctx->sortw, a.w, b.w, n, ctx->stream));
+    HIPCHK(gather_field<uint32_t>(ctx->sortw, a.flag, b.flag, n, ctx->stream));
+    std::swap(ctx->p32, ctx->p32b);
+  }
+  time_end(ctx, APT_KERNEL_SORT);
+  unsigned long long kept = 0;
+  HIPCHK(hipMemcpyAsync(&kept, ctx->sortw.count, 8, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  // the slots freed by the compaction become empty
+  if (kept < n) {
+    uint32_t *cell = ctx->prec == APT_PREC_FP64 ? ctx->p64.cell : ctx->p32.cell;
+    HIPCHK(launch_fill<uint32_t>(cell + kept, MAX_CELL, n - kept, ctx->stream));
+  }
+  ctx->number = kept;
+  return APT_OK;
+}
+
+apt_status apt_clear_guard_cells(apt_ctx *ctx) {
+  CHECK_CTX(ctx);
+  uint32_t *cell = ctx->prec == APT_PREC_FP64 ? ctx->p64.cell : ctx->p32.cell;
+  HIPCHK(launch_erase_guard(ctx->mesh, cell, ctx->number, ctx->stream));
+  return APT_OK;
+}
+
+apt_status apt_energy_histogram(apt_ctx *ctx, uint32_t *hist, int nbins,
+                                double E_max, int flag_bit) {
+  CHECK_CTX(ctx);
+  if (!hist || nbins < 1 || nbins > 8192 || !(E_max > 1.0))
+    return ctx->fail(APT_ERR_INVALID_ARG, "energy_histogram: bad arguments");
+  if (ctx->prec != APT_PREC_FP64)
+    return ctx->fail(APT_ERR_UNSUPPORTED,
+                     "energy_histogram: FP64 form only");
+  uint32_t *d = nullptr;
+  HIPCHK(hipMalloc(&d, nbins * 4));
+  hipError_t e = hipMemsetAsync(d, 0, nbins * 4, ctx->stream);
+  if (e == hipSuccess)
+    e = launch_energy_histogram_f64(ctx->p64.E, ctx->p64.flag, ctx->number,
+                                    d, nbins, E_max, flag_bit, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(hist, d, nbins * 4, hipMemcpyDeviceToHost,
+                       ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  HIPCHK(e);
+  return APT_OK;
+}
+
+// ---- control ------------------------------------------------------------
+apt_status apt_sync(apt_ctx *ctx) {
+  CHECK_CTX(ctx);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return APT_OK;
+}
+
+apt_status apt_get_counters(apt_ctx *ctx, apt_counters *out) {
+  CHECK_CTX(ctx);
+  if (!out) return APT_ERR_INVALID_ARG;
+  unsigned long long h[CTR_NUM];
+  HIPCHK(hipMemcpyAsync(h, ctx->d_counters, sizeof(h), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  out->nan_momenta = h[CTR_NAN];
+  out->multi_cell_moves = h[CTR_MULTICELL];
+  out->capacity_overflow = h[CTR_OVERFLOW];
+  out->outside_tile = h[CTR_OUTSIDE];
+  return APT_OK;
+}
+
+apt_status apt_reset_counters(apt_ctx *ctx) {
+  CHECK_CTX(ctx);
+  HIPCHK(launch_fill<long long>((long long *)ctx->d_counters, 0, CTR_NUM,
+                                ctx->stream));
+  return APT_OK;
+}
+
+apt_status apt_timing_enable(apt_ctx *ctx, int on) {
+  CHECK_CTX(ctx);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->timing = on != 0;
+  for (int k = 0; k < APT_KERNEL_NUM; k++) ctx->ev_used[k] = 0;
+  return APT_OK;
+}
+
+apt_status apt_timing_get(apt_ctx *ctx, int kernel, double *total_ms,
+                          uint64_t *launches) {
+  CHECK_CTX(ctx);
+  if (kernel < 0 || kernel >= APT_KERNEL_NUM)
+    return ctx->fail(APT_ERR_INVALID_ARG, "timing_get: bad kernel id");
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  double tot = 0.0;
+  for (size_t i = 0; i < ctx->ev_used[kernel]; i++) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev[kernel][i].a,
+                               ctx->ev[kernel][i].b));
+    tot += ms;
+  }
+  if (total_ms) *total_ms = tot;
+  if (launches) *launches = ctx->ev_used[kernel];
+  return APT_OK;
+}
+

@@ -1,0 +1,193 @@
+// aperture_host.cpp -- forwarding implementation of the reference-shaped
+// host classes over the C ABI.
+#include "aperture_host.hpp"
+
+namespace Aperture {
+
+void check(apt_ctx *ctx, apt_status st, const char *what) {
+  if (st == APT_OK) return;
+  std::string msg = std::string(what) + " failed (status " +
+                    std::to_string(st) + ")";
+  if (ctx) msg += ": " + std::string(apt_last_error(ctx));
+  throw std::runtime_error(msg);
+}
+
+int sim_environment::dim() const {
+  int d = 1;
+  for (int i = 0; i < 3; i++)
+    if (m_params.N[i] > 1) d = i + 1;
+  return d;
+}
+
+Scalar sim_environment::pos(int i, int n, int stagger) const {
+  Scalar in_cell = stagger * 0.5 + 0.5;
+  Scalar delta = m_params.size[i] / m_params.N[i];
+  if (i < dim()) return m_params.lower[i] + delta * (n - m_params.guard[i] + in_cell);
+  return in_cell * delta;
+}
+
+// stagger of each component, fields_impl.hpp:85-94 and sim_data_impl.hpp:51-65
+static int stagger_of(int which, int comp) {
+  static const int e[3] = {0x6, 0x5, 0x3}, b[3] = {0x1, 0x2, 0x4};
+  switch (which) {
+    case APT_FIELD_E:
+    case APT_FIELD_EBG:
+    case APT_FIELD_J:
+      return e[comp];
+    case APT_FIELD_B:
+    case APT_FIELD_BBG:
+      return b[comp];
+    case APT_FIELD_DIVE:
+    case APT_FIELD_RHO:
+      return 0x7;
+    default:
+      return 0;
+  }
+}
+
+void field_ref::initialize(
+    int component, const std::function<Scalar(Scalar, Scalar, Scalar)> &f) {
+  sim_environment &env = m_data->env;
+  const SimParams &p = env.params();
+  int dims[3];
+  for (int d = 0; d < 3; d++) dims[d] = p.N[d] + 2 * p.guard[d];
+  int st = stagger_of(m_which, component);
+  std::vector<Scalar> v((size_t)dims[0] * dims[1] * dims[2]);
+  for (int k = 0; k < dims[2]; k++) {
+    Scalar x3 = env.pos(2, k, (st >> 2) & 1);
+    for (int j = 0; j < dims[1]; j++) {
+      Scalar x2 = env.pos(1, j, (st >> 1) & 1);
+      for (int i = 0; i < dims[0]; i++) {
+        Scalar x1 = env.pos(0, i, st & 1);
+        v[i + (size_t)j * dims[0] + (size_t)k * dims[0] * dims[1]] =
+            f(x1, x2, x3);
+      }
+    }
+  }
+  set(component, v);
+}
+
+void field_ref::initialize() {
+  for (int c = 0; c < 3; c++)
+    initialize(c, [](Scalar, Scalar, Scalar) { return 0.0; });
+}
+
+std::vector<Scalar> field_ref::get(int component) const {
+  const SimParams &p = m_data->env.params();
+  size_t n = 1;
+  for (int d = 0; d < 3; d++) n *= (size_t)(p.N[d] + 2 * p.guard[d]);
+  std::vector<Scalar> v(n);
+  check(m_data->ctx,
+        apt_field_get(m_data->ctx, m_which, component, v.data(), n),
+        "apt_field_get");
+  return v;
+}
+
+void field_ref::set(int component, const std::vector<Scalar> &v) {
+  check(m_data->ctx,
+        apt_field_set(m_data->ctx, m_which, component, v.data(), v.size()),
+        "apt_field_set");
+}
+
+sim_data::sim_data(sim_environment &e)
+    : env(e),
+      E(this, APT_FIELD_E),
+      B(this, APT_FIELD_B),
+      Ebg(this, APT_FIELD_EBG),
+      Bbg(this, APT_FIELD_BBG),
+      J(this, APT_FIELD_J),
+      divE(this, APT_FIELD_DIVE),
+      divB(this, APT_FIELD_DIVB) {
+  const SimParams &p = env.params();
+  apt_grid g{};
+  apt_params q{};
+  for (int d = 0; d < 3; d++) {
+    g.N[d] = p.N[d];
+    g.guard[d] = p.guard[d];
+    g.lower[d] = p.lower[d];
+    g.size[d] = p.size[d];
+    g.periodic[d] = p.periodic_boundary[d] ? 1 : 0;
+    q.tile[d] = p.tile[d];
+  }
+  q.precision = p.precision;
+  q.num_species = p.num_species;
+  q.q_e = p.q_e;
+  q.ion_mass = p.ion_mass;
+  q.data_interval = (int)p.data_interval;
+  q.current_smoothing = p.current_smoothing;
+  q.max_ptc_number = p.max_ptc_number;
+  q.random_seed = (uint64_t)p.random_seed;
+  check(nullptr, apt_create(&g, &q, &ctx), "apt_create");
+}
+
+sim_data::~sim_data() {
+  if (ctx) apt_destroy(ctx);
+}
+
+void sim_data::fill_multiplicity(Scalar weight, int multiplicity,
+                                 Scalar theta_e, Scalar theta_p,
+                                 Scalar drift_e, Scalar drift_p) {
+  double th[2] = {theta_e, theta_p}, dr[2] = {drift_e, drift_p};
+  check(ctx, apt_ptc_load_uniform(ctx, multiplicity, th, dr, weight),
+        "apt_ptc_load_uniform");
+  // the reference sorts right after loading (cuda/sim_data.cu:290)
+  sort_particles();
+}
+
+void sim_data::init_bg_B_field(
+    int component, const std::function<Scalar(Scalar, Scalar, Scalar)> &f) {
+  Bbg.initialize(component, f);
+  B.initialize(component, f);
+}
+
+void sim_data::init_bg_E_field(
+    int component, const std::function<Scalar(Scalar, Scalar, Scalar)> &f) {
+  Ebg.initialize(component, f);
+  E.initialize(component, f);
+}
+
+void sim_data::sort_particles() {
+  check(ctx, apt_sort_particles(ctx), "apt_sort_particles");
+}
+
+uint64_t sim_data::num_particles() const {
+  uint64_t n = 0;
+  check(ctx, apt_ptc_number(ctx, &n), "apt_ptc_number");
+  return n;
+}
+
+void ptc_updater::update_particles(sim_data &data, double dt,
+                                   uint32_t step) {
+  check(data.ctx, apt_ptc_update(data.ctx, dt, step), "apt_ptc_update");
+}
+
+void ptc_updater::apply_boundary(sim_data &, double, uint32_t) {
+  // The Cartesian reference leaves this empty (ptc_updater.cu:843-850);
+  // periodic wraps happen inside apt_ptc_update.
+}
+
+void ptc_updater::smooth_current(sim_data &data, uint32_t step) {
+  check(data.ctx, apt_smooth_current(data.ctx, step), "apt_smooth_current");
+}
+
+void field_solver::update_fields(sim_data &data, double dt, uint32_t step) {
+  check(data.ctx, apt_field_update(data.ctx, dt, step), "apt_field_update");
+}
+
+void field_solver::apply_outflow_boundary(sim_data &, double) {
+  // empty in the reference (field_solver.cu:451-452)
+}
+
+void field_solver::compute_divs(sim_data &data) {
+  check(data.ctx, apt_field_compute_divs(data.ctx), "apt_field_compute_divs");
+}
+
+void field_solver::update_e_field(sim_data &data, double dt) {
+  check(data.ctx, apt_field_update_e(data.ctx, dt), "apt_field_update_e");
+}
+
+void field_solver::update_b_field(sim_data &data, double dt) {
+  check(data.ctx, apt_field_update_b(data.ctx, dt), "apt_field_update_b");
+}
+
+}  // namespace Aperture

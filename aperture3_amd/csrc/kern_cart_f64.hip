@@ -1,0 +1,355 @@
+// kern_cart_f64.hip -- plain FP64 Cartesian particle update (the drop-in
+// form).  One fused pass per particle: push (ptc_push_cart_{1,2,3}d,
+// ptc_updater.cu:21-218), then move + Esirkepov deposit
+// (deposit_current_cart_{1,2,3}d, ptc_updater.cu:220-499), then the periodic
+// wrap that the reference does through send_particles.  Expressions keep the
+// reference's operation order and this file is built with -ffp-contract=off,
+// so every particle's arithmetic is bit-identical to the CPU restatement.
+// The current goes through two-limb int64 fixed point (order independent)
+// with device-scope atomics on the wrapped (periodic) index.
+#include "apt_common.hpp"
+#include "apt_kernels.hpp"
+
+namespace apt {
+
+__device__ inline double cic(double dx) {
+  double a = 1.0 - fabs(dx);
+  return a > 0.0 ? a : 0.0;
+}
+
+// interp_cell (interpolation.cuh:79-87)
+__device__ inline double interp_cell(double rel, int c, int t, int st) {
+  double x = ((double)t + (st == 0 ? 0.5 : 1.0)) - (rel + (double)c);
+  return cic(x);
+}
+
+// Interpolator1D/2D/3D (interpolation.cuh:91-209): all taps, same order.
+__device__ inline double gather1(const double *__restrict__ f, double x1,
+                                 int c1, int st) {
+  double r = 0.0;
+#pragma unroll
+  for (int i = 0; i <= 2; i++) {
+    int ii = i + c1 - 1;
+    r += f[ii] * interp_cell(x1, c1, ii, st & 1);
+  }
+  return r;
+}
+
+__device__ inline double gather2(const double *__restrict__ f, int d0,
+                                 double x1, double x2, int c1, int c2,
+                                 int st) {
+  double r = 0.0;
+#pragma unroll
+  for (int j = 0; j <= 2; j++) {
+    int jj = j + c2 - 1;
+#pragma unroll
+    for (int i = 0; i <= 2; i++) {
+      int ii = i + c1 - 1;
+      r += f[ii + (int64_t)jj * d0] * interp_cell(x1, c1, ii, st & 1) *
+           interp_cell(x2, c2, jj, (st >> 1) & 1);
+    }
+  }
+  return r;
+}
+
+__device__ inline double gather3(const double *__restrict__ f, int d0,
+                                 int d1, double x1, double x2, double x3,
+                                 int c1, int c2, int c3, int st) {
+  double r = 0.0;
+#pragma unroll
+  for (int k = -1; k <= 1; k++) {
+    double w3 = interp_cell(x3, c3, k + c3, (st >> 2) & 1);
+#pragma unroll
+    for (int j = -1; j <= 1; j++) {
+      double w2 = interp_cell(x2, c2, j + c2, (st >> 1) & 1);
+#pragma unroll
+      for (int i = -1; i <= 1; i++) {
+        int64_t o = (int64_t)(i + c1) + (int64_t)(j + c2) * d0 +
+                    (int64_t)(k + c3) * d0 * d1;
+        r += f[o] * interp_cell(x1, c1, i + c1, st & 1) * w2 * w3;
+      }
+    }
+  }
+  return r;
+}
+
+// vay_push (vay_push.cuh:8-30), fields pre-scaled by dt q / 2m.
+__device__ inline void vay_push(double &p1, double &p2, double &p3,
+                                double &gamma, double E1, double E2,
+                                double E3, double B1, double B2, double B3) {
+  double up1 = p1 + 2.0 * E1 + (p2 * B3 - p3 * B2) / gamma;
+  double up2 = p2 + 2.0 * E2 + (p3 * B1 - p1 * B3) / gamma;
+  double up3 = p3 + 2.0 * E3 + (p1 * B2 - p2 * B1) / gamma;
+  double tt = B1 * B1 + B2 * B2 + B3 * B3;
+  double ut = up1 * B1 + up2 * B2 + up3 * B3;
+  double sigma = 1.0 + up1 * up1 + up2 * up2 + up3 * up3 - tt;
+  double inv_gamma2 =
+      2.0 / (sigma + sqrt(sigma * sigma + 4.0 * (tt + ut * ut)));
+  double s = 1.0 / (1.0 + inv_gamma2 * tt);
+  gamma = 1.0 / sqrt(inv_gamma2);
+  p1 = (up1 + B1 * ut * inv_gamma2 + (up2 * B3 - up3 * B2) / gamma) * s;
+  p2 = (up2 + B2 * ut * inv_gamma2 + (up3 * B1 - up1 * B3) / gamma) * s;
+  p3 = (up3 + B3 * ut * inv_gamma2 + (up1 * B2 - up2 * B1) / gamma) * s;
+}
+
+// center2d / movement2d / movement3d (ptc_updater_helper.cuh:62-77), with
+// an exact 1/6 where the reference writes 0.1666667f.
+__device__ inline double center2d(double sx0, double sx1, double sy0,
+                                  double sy1) {
+  return (2.0 * sx1 * sy1 + sx0 * sy1 + sx1 * sy0 + 2.0 * sx0 * sy0) *
+         (1.0 / 6.0);
+}
+__device__ inline double movement3d(double sx0, double sx1, double sy0,
+                                    double sy1, double sz0, double sz1) {
+  return (sz1 - sz0) * center2d(sx0, sx1, sy0, sy1);
+}
+__device__ inline double movement2d(double sx0, double sx1, double sy0,
+                                    double sy1) {
+  return (sy1 - sy0) * 0.5 * (sx0 + sx1);
+}
+
+__device__ inline void acc_add(long long *__restrict__ hi,
+                               long long *__restrict__ lo, uint64_t o,
+                               double c, int H) {
+  long long h, l;
+  fx_split(c, H, h, l);
+  if (h != 0)
+    __hip_atomic_fetch_add((unsigned long long *)(hi + o),
+                           (unsigned long long)h, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  if (l != 0)
+    __hip_atomic_fetch_add((unsigned long long *)(lo + o),
+                           (unsigned long long)l, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256)
+    k_push_deposit_f64(Mesh m, Species sp, double dt, int H, int rho,
+                       PtcF64 p, FieldsF64 f, AccF64 a,
+                       unsigned long long *__restrict__ counters,
+                       uint64_t n) {
+  uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  uint32_t c = p.cell[idx];
+  if (c == MAX_CELL) return;
+  int c1, c2, c3;
+  m.split(c, c1, c2, c3);
+  if (!m.in_bulk(c1, c2, c3)) {
+    p.cell[idx] = MAX_CELL;
+    return;
+  }
+  const int d0 = m.dims[0], d1 = m.dims[1];
+  uint32_t flag = p.flag[idx];
+  int s = ptc_species(flag);
+  double x1 = p.x1[idx], x2 = p.x2[idx], x3 = p.x3[idx];
+  double p1 = p.p1[idx], p2 = p.p2[idx], p3 = p.p3[idx];
+  double gamma = p.E[idx];
+  double q_over_m = dt * 0.5 * sp.q[s] / sp.m[s];
+  if (p1 != p1 || p2 != p2 || p3 != p3)
+    atomicAdd(&counters[CTR_NAN], 1ull);
+  gamma = sqrt(1.0 + p1 * p1 + p2 * p2 + p3 * p3);
+  if (!(flag & FLAG_IGNORE_EM)) {
+    double E1, E2, E3, B1, B2, B3;
+    if (DIM == 1) {
+      E1 = gather1(f.E[0], x1, (int)c, ST_E1) * q_over_m;
+      E2 = gather1(f.E[1], x1, (int)c, ST_E2) * q_over_m;
+      E3 = gather1(f.E[2], x1, (int)c, ST_E3) * q_over_m;
+      B1 = gather1(f.B[0], x1, (int)c, ST_B1) * q_over_m;
+      B2 = gather1(f.B[1], x1, (int)c, ST_B2) * q_over_m;
+      B3 = gather1(f.B[2], x1, (int)c, ST_B3) * q_over_m;
+    } else if (DIM == 2) {
+      E1 = gather2(f.E[0], d0, x1, x2, c1, c2, ST_E1) * q_over_m;
+      E2 = gather2(f.E[1], d0, x1, x2, c1, c2, ST_E2) * q_over_m;
+      E3 = gather2(f.E[2], d0, x1, x2, c1, c2, ST_E3) * q_over_m;
+      B1 = gather2(f.B[0], d0, x1, x2, c1, c2, ST_B1) * q_over_m;
+      B2 = gather2(f.B[1], d0, x1, x2, c1, c2, ST_B2) * q_over_m;
+      B3 = gather2(f.B[2], d0, x1, x2, c1, c2, ST_B3) * q_over_m;
+    } else {
+      E1 = gather3(f.E[0], d0, d1, x1, x2, x3, c1, c2, c3, ST_E1) * q_over_m;
+      E2 = gather3(f.E[1], d0, d1, x1, x2, x3, c1, c2, c3, ST_E2) * q_over_m;
+      E3 = gather3(f.E[2], d0, d1, x1, x2, x3, c1, c2, c3, ST_E3) * q_over_m;
+      B1 = gather3(f.B[0], d0, d1, x1, x2, x3, c1, c2, c3, ST_B1) * q_over_m;
+      B2 = gather3(f.B[1], d0, d1, x1, x2, x3, c1, c2, c3, ST_B2) * q_over_m;
+      B3 = gather3(f.B[2], d0, d1, x1, x2, x3, c1, c2, c3, ST_B3) * q_over_m;
+    }
+    vay_push(p1, p2, p3, gamma, E1, E2, E3, B1, B2, B3);
+    p.p1[idx] = p1;
+    p.p2[idx] = p2;
+    p.p3[idx] = p3;
+    p.E[idx] = gamma;
+  } else {
+    // ignore_EM: stored momenta and gamma are left as they were
+    p1 = p.p1[idx];
+    p2 = p.p2[idx];
+    p3 = p.p3[idx];
+    gamma = p.E[idx];
+  }
+
+  // ---- move + deposit (reads the stored momenta / gamma) ----
+  double v1 = p1 / gamma, v2 = p2 / gamma, v3 = p3 / gamma;
+  double w = p.w[idx];
+  const double old_x1 = x1, old_x2 = x2, old_x3 = x3;
+
+  if (DIM == 1) {
+    double new_x1 = old_x1 + (v1 * dt) / m.delta[0];
+    int dc1 = (int)floor(new_x1);
+    if (dc1 > 1 || dc1 < -1) atomicAdd(&counters[CTR_MULTICELL], 1ull);
+    new_x1 -= (double)dc1;
+    p.cell[idx] = (uint32_t)m.wrap(0, (int)c + dc1);
+    p.x1[idx] = new_x1;
+    p.x2[idx] = old_x2 + v2 * dt;
+    p.x3[idx] = old_x3 + v3 * dt;
+    if (flag & FLAG_IGNORE_CURRENT) return;
+    double weight = sp.q[s] * w;
+    int i0 = (dc1 == -1 ? -2 : -1);
+    int i1 = (dc1 == 1 ? 1 : 0);
+    double djx = 0.0;
+    for (int i = i0; i <= i1; i++) {
+      double sx0 = cic(-old_x1 + i + 1);
+      double sx1 = cic(-new_x1 + (i + 1 - dc1));
+      djx += sx1 - sx0;
+      uint64_t o1 = m.widx(i + (int)c + 1, 0, 0);
+      uint64_t o = m.widx(i + (int)c, 0, 0);
+      acc_add(a.J[0][0], a.J[0][1], o1, -weight * djx, H);
+      double val1 = 0.5 * (sx0 + sx1);
+      acc_add(a.J[1][0], a.J[1][1], o, weight * v2 * val1, H);
+      acc_add(a.J[2][0], a.J[2][1], o, weight * v3 * val1, H);
+      if (rho) acc_add(a.R[s][0], a.R[s][1], o, weight * sx1, H);
+    }
+  } else if (DIM == 2) {
+    double new_x1 = old_x1 + (v1 * dt) * m.inv_delta[0];
+    double new_x2 = old_x2 + (v2 * dt) * m.inv_delta[1];
+    int dc1 = (int)floor(new_x1);
+    int dc2 = (int)floor(new_x2);
+    if (dc1 > 1 || dc1 < -1 || dc2 > 1 || dc2 < -1)
+      atomicAdd(&counters[CTR_MULTICELL], 1ull);
+    new_x1 -= (double)dc1;
+    new_x2 -= (double)dc2;
+    p.cell[idx] = (uint32_t)m.idx(m.wrap(0, c1 + dc1), m.wrap(1, c2 + dc2), 0);
+    p.x1[idx] = new_x1;
+    p.x2[idx] = new_x2;
+    p.x3[idx] = old_x3 + v3 * dt;
+    if (flag & FLAG_IGNORE_CURRENT) return;
+    double weight = -sp.q[s] * w;
+    int j0 = (dc2 == -1 ? -2 : -1), j1 = (dc2 == 1 ? 1 : 0);
+    int i0 = (dc1 == -1 ? -2 : -1), i1 = (dc1 == 1 ? 1 : 0);
+    double djy[3] = {0.0, 0.0, 0.0};
+    for (int j = j0; j <= j1; j++) {
+      double sy0 = cic(-old_x2 + j + 1);
+      double sy1 = cic(-new_x2 + (j + 1 - dc2));
+      double djx = 0.0;
+      for (int i = i0; i <= i1; i++) {
+        double sx0 = cic(-old_x1 + i + 1);
+        double sx1 = cic(-new_x1 + (i + 1 - dc1));
+        djx += movement2d(sy0, sy1, sx0, sx1);
+        acc_add(a.J[0][0], a.J[0][1], m.widx(i + c1 + 1, j + c2, 0),
+                weight * djx, H);
+        djy[i - i0] += movement2d(sx0, sx1, sy0, sy1);
+        acc_add(a.J[1][0], a.J[1][1], m.widx(i + c1, j + c2 + 1, 0),
+                weight * djy[i - i0], H);
+        double val2 = center2d(sx0, sx1, sy0, sy1);
+        uint64_t o = m.widx(i + c1, j + c2, 0);
+        acc_add(a.J[2][0], a.J[2][1], o, -weight * v3 * val2, H);
+        if (rho) {
+          double s1 = sx1 * sy1;
+          acc_add(a.R[s][0], a.R[s][1], o, -weight * s1, H);
+        }
+      }
+    }
+  } else {
+    double new_x1 = old_x1 + (v1 * dt) * m.inv_delta[0];
+    double new_x2 = old_x2 + (v2 * dt) * m.inv_delta[1];
+    double new_x3 = old_x3 + (v3 * dt) * m.inv_delta[2];
+    int dc1 = (int)floor(new_x1);
+    int dc2 = (int)floor(new_x2);
+    int dc3 = (int)floor(new_x3);
+    if (dc1 > 1 || dc1 < -1 || dc2 > 1 || dc2 < -1 || dc3 > 1 || dc3 < -1)
+      atomicAdd(&counters[CTR_MULTICELL], 1ull);
+    new_x1 -= (double)dc1;
+    new_x2 -= (double)dc2;
+    new_x3 -= (double)dc3;
+    p.cell[idx] = (uint32_t)m.idx(m.wrap(0, c1 + dc1), m.wrap(1, c2 + dc2),
+                                  m.wrap(2, c3 + dc3));
+    p.x1[idx] = new_x1;
+    p.x2[idx] = new_x2;
+    p.x3[idx] = new_x3;
+    if (flag & FLAG_IGNORE_CURRENT) return;
+    double weight = -sp.q[s] * w;
+    int k0 = (dc3 == -1 ? -2 : -1), k1 = (dc3 == 1 ? 1 : 0);
+    int j0 = (dc2 == -1 ? -2 : -1), j1 = (dc2 == 1 ? 1 : 0);
+    int i0 = (dc1 == -1 ? -2 : -1), i1 = (dc1 == 1 ? 1 : 0);
+    double djz[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    for (int k = k0; k <= k1; k++) {
+      double sz0 = cic(-old_x3 + k + 1);
+      double sz1 = cic(-new_x3 + (k + 1 - dc3));
+      double djy[3] = {0.0, 0.0, 0.0};
+      for (int j = j0; j <= j1; j++) {
+        double sy0 = cic(-old_x2 + j + 1);
+        double sy1 = cic(-new_x2 + (j + 1 - dc2));
+        double djx = 0.0;
+        for (int i = i0; i <= i1; i++) {
+          double sx0 = cic(-old_x1 + i + 1);
+          double sx1 = cic(-new_x1 + (i + 1 - dc1));
+          djx += movement3d(sy0, sy1, sz0, sz1, sx0, sx1);
+          acc_add(a.J[0][0], a.J[0][1], m.widx(i + c1 + 1, j + c2, k + c3),
+                  weight * djx, H);
+          djy[i - i0] += movement3d(sx0, sx1, sz0, sz1, sy0, sy1);
+          acc_add(a.J[1][0], a.J[1][1], m.widx(i + c1, j + c2 + 1, k + c3),
+                  weight * djy[i - i0], H);
+          djz[j - j0][i - i0] += movement3d(sx0, sx1, sy0, sy1, sz0, sz1);
+          acc_add(a.J[2][0], a.J[2][1], m.widx(i + c1, j + c2, k + c3 + 1),
+                  weight * djz[j - j0][i - i0], H);
+          if (rho) {
+            double s1 = sx1 * sy1 * sz1;
+            acc_add(a.R[s][0], a.R[s][1], m.widx(i + c1, j + c2, k + c3),
+                    -weight * s1, H);
+          }
+        }
+      }
+    }
+  }
+}
+
+// J = hi 2^-H + lo 2^-(H+40), accumulators cleared for the next step.
+__global__ void k_finalize_f64(long long *__restrict__ hi,
+                               long long *__restrict__ lo,
+                               double *__restrict__ out, uint64_t n, int H) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = fx_value(hi[i], lo[i], H);
+  hi[i] = 0;
+  lo[i] = 0;
+}
+
+// ---------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------
+hipError_t launch_push_deposit_f64(const Mesh &m, const Species &sp,
+                                   double dt, int H, int rho,
+                                   const PtcF64 &p, const FieldsF64 &f,
+                                   const AccF64 &a,
+                                   unsigned long long *counters, uint64_t n,
+                                   hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  dim3 block(256), grid((unsigned)((n + 255) / 256));
+  if (m.dim == 1)
+    k_push_deposit_f64<1><<<grid, block, 0, st>>>(m, sp, dt, H, rho, p, f, a,
+                                                  counters, n);
+  else if (m.dim == 2)
+    k_push_deposit_f64<2><<<grid, block, 0, st>>>(m, sp, dt, H, rho, p, f, a,
+                                                  counters, n);
+  else
+    k_push_deposit_f64<3><<<grid, block, 0, st>>>(m, sp, dt, H, rho, p, f, a,
+                                                  counters, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_f64(long long *hi, long long *lo, double *out,
+                               uint64_t n, int H, hipStream_t st) {
+  dim3 block(256), grid((unsigned)((n + 255) / 256));
+  k_finalize_f64<<<grid, block, 0, st>>>(hi, lo, out, n, H);
+  return hipGetLastError();
+}
+
+}  // namespace apt

@@ -1,0 +1,321 @@
+// kern_cart_mixed.hip -- mixed-precision Cartesian particle update: the
+// same fused push + move + Esirkepov deposit as the FP64 form
+// (ptc_updater.cu:21-499), computed in FP32 on the 36-byte particle layout
+// (int32 cell, FP32 in-cell offsets, FP32 u = gamma v, FP32 weight, flag).
+// gamma is recomputed from u instead of stored.  Every current
+// contribution is rounded to 2^-H and summed in int64, so the deposit is
+// deterministic and charge conserving to the round-off of the FP32 shapes.
+#include "apt_common.hpp"
+#include "apt_kernels.hpp"
+
+namespace apt {
+
+#pragma clang fp contract(fast)
+
+__device__ __forceinline__ float cicf(float d) {
+  return fmaxf(1.0f - fabsf(d), 0.0f);
+}
+
+// CIC weights of the two non-zero taps of a component with stagger bit st
+// (interp_cell, interpolation.cuh:79-87): upper node (st = 1) -> taps c-1,
+// c; cell centre (st = 0) -> taps c-1, c when x < 1/2 else c, c+1.
+__device__ __forceinline__ void taps(float x, int st, int &o, float &w0,
+                                     float &w1) {
+  if (st) {
+    o = -1;
+    w0 = 1.0f - x;
+    w1 = x;
+  } else if (x < 0.5f) {
+    o = -1;
+    w0 = 0.5f - x;
+    w1 = 0.5f + x;
+  } else {
+    o = 0;
+    w0 = 1.5f - x;
+    w1 = x - 0.5f;
+  }
+}
+
+template <int DIM>
+__device__ __forceinline__ float gatherf(const float *__restrict__ f,
+                                         const Mesh &m, int c1, int c2,
+                                         int c3, float x1, float x2, float x3,
+                                         int st) {
+  int o1, o2, o3;
+  float a0, a1, b0, b1, g0, g1;
+  taps(x1, st & 1, o1, a0, a1);
+  if (DIM == 1) {
+    const float *p = f + (c1 + o1);
+    return p[0] * a0 + p[1] * a1;
+  }
+  taps(x2, (st >> 1) & 1, o2, b0, b1);
+  const int d0 = m.dims[0];
+  if (DIM == 2) {
+    const float *p = f + (int64_t)(c2 + o2) * d0 + (c1 + o1);
+    return (p[0] * a0 + p[1] * a1) * b0 + (p[d0] * a0 + p[d0 + 1] * a1) * b1;
+  }
+  taps(x3, (st >> 2) & 1, o3, g0, g1);
+  const int64_t d01 = (int64_t)d0 * m.dims[1];
+  const float *p = f + (int64_t)(c3 + o3) * d01 + (int64_t)(c2 + o2) * d0 +
+                   (c1 + o1);
+  float lo = (p[0] * a0 + p[1] * a1) * b0 + (p[d0] * a0 + p[d0 + 1] * a1) * b1;
+  const float *q = p + d01;
+  float hi = (q[0] * a0 + q[1] * a1) * b0 + (q[d0] * a0 + q[d0 + 1] * a1) * b1;
+  return lo * g0 + hi * g1;
+}
+
+__device__ __forceinline__ void vay_pushf(float &p1, float &p2, float &p3,
+                                          float gamma, float E1, float E2,
+                                          float E3, float B1, float B2,
+                                          float B3) {
+  float ig = 1.0f / gamma;
+  float up1 = p1 + 2.0f * E1 + (p2 * B3 - p3 * B2) * ig;
+  float up2 = p2 + 2.0f * E2 + (p3 * B1 - p1 * B3) * ig;
+  float up3 = p3 + 2.0f * E3 + (p1 * B2 - p2 * B1) * ig;
+  float tt = B1 * B1 + B2 * B2 + B3 * B3;
+  float ut = up1 * B1 + up2 * B2 + up3 * B3;
+  float sigma = 1.0f + up1 * up1 + up2 * up2 + up3 * up3 - tt;
+  float inv_gamma2 =
+      2.0f / (sigma + sqrtf(sigma * sigma + 4.0f * (tt + ut * ut)));
+  float s = 1.0f / (1.0f + inv_gamma2 * tt);
+  float ign = sqrtf(inv_gamma2);
+  p1 = (up1 + B1 * ut * inv_gamma2 + (up2 * B3 - up3 * B2) * ign) * s;
+  p2 = (up2 + B2 * ut * inv_gamma2 + (up3 * B1 - up1 * B3) * ign) * s;
+  p3 = (up3 + B3 * ut * inv_gamma2 + (up1 * B2 - up2 * B1) * ign) * s;
+}
+
+__device__ __forceinline__ void acc_fx(long long *__restrict__ acc,
+                                       uint64_t o, float c, float scale) {
+  long long v = __float2ll_rn(c * scale);
+  if (v != 0)
+    __hip_atomic_fetch_add((unsigned long long *)(acc + o),
+                           (unsigned long long)v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// shape arrays over the four node offsets -2..1 around the old cell:
+// s0 = old position, s1 = new position (moved by dc in {-1,0,1})
+__device__ __forceinline__ void shapes(float xo, float xn, int dc, float s0[4],
+                                       float s1[4]) {
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    s0[t] = 0.0f;
+    s1[t] = 0.0f;
+  }
+  s0[1] = 1.0f - xo;
+  s0[2] = xo;
+  s1[1 + dc] = 1.0f - xn;
+  s1[2 + dc] = xn;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256)
+    k_push_deposit_mixed(Mesh m, Species sp, float dt, float scale, PtcF32 p,
+                         FieldsF32 f, long long *__restrict__ J0,
+                         long long *__restrict__ J1,
+                         long long *__restrict__ J2,
+                         unsigned long long *__restrict__ counters,
+                         uint64_t n) {
+  uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  uint32_t c = p.cell[idx];
+  if (c == MAX_CELL) return;
+  int c1, c2, c3;
+  m.split(c, c1, c2, c3);
+  if (!m.in_bulk(c1, c2, c3)) {
+    p.cell[idx] = MAX_CELL;
+    return;
+  }
+  uint32_t flag = p.flag[idx];
+  int s = ptc_species(flag);
+  float x1 = p.x1[idx], x2 = p.x2[idx], x3 = p.x3[idx];
+  float u1 = p.u1[idx], u2 = p.u2[idx], u3 = p.u3[idx];
+  float w = p.w[idx];
+  if (u1 != u1 || u2 != u2 || u3 != u3) atomicAdd(&counters[CTR_NAN], 1ull);
+  float gamma = sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+  if (!(flag & FLAG_IGNORE_EM)) {
+    float qm = (float)(0.5 * sp.q[s] / sp.m[s]) * dt;
+    float E1 = gatherf<DIM>(f.E[0], m, c1, c2, c3, x1, x2, x3, ST_E1) * qm;
+    float E2 = gatherf<DIM>(f.E[1], m, c1, c2, c3, x1, x2, x3, ST_E2) * qm;
+    float E3 = gatherf<DIM>(f.E[2], m, c1, c2, c3, x1, x2, x3, ST_E3) * qm;
+    float B1 = gatherf<DIM>(f.B[0], m, c1, c2, c3, x1, x2, x3, ST_B1) * qm;
+    float B2 = gatherf<DIM>(f.B[1], m, c1, c2, c3, x1, x2, x3, ST_B2) * qm;
+    float B3 = gatherf<DIM>(f.B[2], m, c1, c2, c3, x1, x2, x3, ST_B3) * qm;
+    vay_pushf(u1, u2, u3, gamma, E1, E2, E3, B1, B2, B3);
+    gamma = sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+    p.u1[idx] = u1;
+    p.u2[idx] = u2;
+    p.u3[idx] = u3;
+  }
+  float ig = 1.0f / gamma;
+  float v1 = u1 * ig, v2 = u2 * ig, v3 = u3 * ig;
+  float qw = (float)sp.q[s] * w;
+
+  // ---- move ----
+  float n1 = x1 + v1 * dt * (float)m.inv_delta[0];
+  int dc1 = (int)floorf(n1);
+  n1 -= (float)dc1;
+  float n2 = x2, n3 = x3;
+  int dc2 = 0, dc3 = 0;
+  if (DIM >= 2) {
+    n2 = x2 + v2 * dt * (float)m.inv_delta[1];
+    dc2 = (int)floorf(n2);
+    n2 -= (float)dc2;
+  } else {
+    n2 = x2 + v2 * dt;
+  }
+  if (DIM == 3) {
+    n3 = x3 + v3 * dt * (float)m.inv_delta[2];
+    dc3 = (int)floorf(n3);
+    n3 -= (float)dc3;
+  } else {
+    n3 = x3 + v3 * dt;
+  }
+  // an offset that rounds to 1.0f belongs to the next cell
+  if (n1 >= 1.0f) { n1 -= 1.0f; dc1 += 1; }
+  if (DIM >= 2 && n2 >= 1.0f) { n2 -= 1.0f; dc2 += 1; }
+  if (DIM == 3 && n3 >= 1.0f) { n3 -= 1.0f; dc3 += 1; }
+  if (dc1 > 1 || dc1 < -1 || dc2 > 1 || dc2 < -1 || dc3 > 1 || dc3 < -1) {
+    atomicAdd(&counters[CTR_MULTICELL], 1ull);
+    dc1 = max(-1, min(1, dc1));
+    dc2 = max(-1, min(1, dc2));
+    dc3 = max(-1, min(1, dc3));
+  }
+  p.cell[idx] = (uint32_t)m.idx(m.wrap(0, c1 + dc1),
+                                DIM >= 2 ? m.wrap(1, c2 + dc2) : c2,
+                                DIM == 3 ? m.wrap(2, c3 + dc3) : c3);
+  p.x1[idx] = n1;
+  p.x2[idx] = n2;
+  p.x3[idx] = n3;
+  if (flag & FLAG_IGNORE_CURRENT) return;
+
+  // ---- Esirkepov deposit (ptc_updater.cu:264-291, 343-384, 443-497) ----
+  float sx0[4], sx1[4];
+  shapes(x1, n1, dc1, sx0, sx1);
+  if (DIM == 1) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      acc += sx1[t] - sx0[t];
+      int i = c1 + t - 2;
+      if (t < 3) acc_fx(J0, m.widx(i + 1, c2, c3), -qw * acc, scale);
+      float val = 0.5f * (sx0[t] + sx1[t]);
+      if (val != 0.0f) {
+        uint64_t o = m.widx(i, c2, c3);
+        acc_fx(J1, o, qw * v2 * val, scale);
+        acc_fx(J2, o, qw * v3 * val, scale);
+      }
+    }
+    return;
+  }
+  float sy0[4], sy1[4];
+  shapes(x2, n2, dc2, sy0, sy1);
+  if (DIM == 2) {
+    const float wt = -qw;  // reference sign convention (ptc_updater.cu:345)
+    float djy[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int tj = 0; tj < 4; tj++) {
+      float ay = 0.5f * (sy0[tj] + sy1[tj]);
+      float dsy = sy1[tj] - sy0[tj];
+      float djx = 0.0f;
+      int j = c2 + tj - 2;
+#pragma unroll
+      for (int ti = 0; ti < 4; ti++) {
+        int i = c1 + ti - 2;
+        djx += (sx1[ti] - sx0[ti]) * ay;
+        if (ti < 3 && djx != 0.0f) acc_fx(J0, m.widx(i + 1, j, c3), wt * djx, scale);
+        djy[ti] += dsy * 0.5f * (sx0[ti] + sx1[ti]);
+        if (tj < 3 && djy[ti] != 0.0f)
+          acc_fx(J1, m.widx(i, j + 1, c3), wt * djy[ti], scale);
+        float cen = (2.0f * sx1[ti] * sy1[tj] + sx0[ti] * sy1[tj] +
+                     sx1[ti] * sy0[tj] + 2.0f * sx0[ti] * sy0[tj]) *
+                    (1.0f / 6.0f);
+        if (cen != 0.0f) acc_fx(J2, m.widx(i, j, c3), -wt * v3 * cen, scale);
+      }
+    }
+    return;
+  }
+  float sz0[4], sz1[4];
+  shapes(x3, n3, dc3, sz0, sz1);
+  {
+    const float wt = -qw;
+    float djz[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) djz[a][b] = 0.0f;
+#pragma unroll
+    for (int tk = 0; tk < 4; tk++) {
+      int k = c3 + tk - 2;
+      float djy[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int tj = 0; tj < 4; tj++) {
+        int j = c2 + tj - 2;
+        float djx = 0.0f;
+        float cyz = (2.0f * sy1[tj] * sz1[tk] + sy0[tj] * sz1[tk] +
+                     sy1[tj] * sz0[tk] + 2.0f * sy0[tj] * sz0[tk]) *
+                    (1.0f / 6.0f);
+#pragma unroll
+        for (int ti = 0; ti < 4; ti++) {
+          int i = c1 + ti - 2;
+          djx += (sx1[ti] - sx0[ti]) * cyz;
+          if (ti < 3 && djx != 0.0f)
+            acc_fx(J0, m.widx(i + 1, j, k), wt * djx, scale);
+          float cxz = (2.0f * sx1[ti] * sz1[tk] + sx0[ti] * sz1[tk] +
+                       sx1[ti] * sz0[tk] + 2.0f * sx0[ti] * sz0[tk]) *
+                      (1.0f / 6.0f);
+          djy[ti] += (sy1[tj] - sy0[tj]) * cxz;
+          if (tj < 3 && djy[ti] != 0.0f)
+            acc_fx(J1, m.widx(i, j + 1, k), wt * djy[ti], scale);
+          float cxy = (2.0f * sx1[ti] * sy1[tj] + sx0[ti] * sy1[tj] +
+                       sx1[ti] * sy0[tj] + 2.0f * sx0[ti] * sy0[tj]) *
+                      (1.0f / 6.0f);
+          djz[tj][ti] += (sz1[tk] - sz0[tk]) * cxy;
+          if (tk < 3 && djz[tj][ti] != 0.0f)
+            acc_fx(J2, m.widx(i, j, k + 1), wt * djz[tj][ti], scale);
+        }
+      }
+    }
+  }
+}
+
+__global__ void k_finalize_mixed(long long *__restrict__ acc,
+                                 float *__restrict__ out, uint64_t n,
+                                 double inv_scale) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = (float)((double)acc[i] * inv_scale);
+  acc[i] = 0;
+}
+
+hipError_t launch_push_deposit_mixed(const Mesh &m, const Species &sp,
+                                     float dt, int S, const PtcF32 &p,
+                                     const FieldsF32 &f, long long *J0,
+                                     long long *J1, long long *J2,
+                                     const TileInfo &tiles,
+                                     unsigned long long *counters,
+                                     uint64_t n, hipStream_t st) {
+  (void)tiles;
+  if (n == 0) return hipSuccess;
+  float scale = ldexpf(1.0f, S);
+  dim3 block(256), grid((unsigned)((n + 255) / 256));
+  if (m.dim == 1)
+    k_push_deposit_mixed<1><<<grid, block, 0, st>>>(m, sp, dt, scale, p, f, J0,
+                                                    J1, J2, counters, n);
+  else if (m.dim == 2)
+    k_push_deposit_mixed<2><<<grid, block, 0, st>>>(m, sp, dt, scale, p, f, J0,
+                                                    J1, J2, counters, n);
+  else
+    k_push_deposit_mixed<3><<<grid, block, 0, st>>>(m, sp, dt, scale, p, f, J0,
+                                                    J1, J2, counters, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_mixed(long long *acc, float *out, uint64_t n,
+                                 int S, hipStream_t st) {
+  dim3 block(256), grid((unsigned)((n + 255) / 256));
+  k_finalize_mixed<<<grid, block, 0, st>>>(acc, out, n, ldexp(1.0, -S));
+  return hipGetLastError();
+}
+
+}  // namespace apt

@@ -1,0 +1,372 @@
+// kern_field.hip -- Cartesian Yee field solver (field_solver.cu:14-358):
+// E and B curl updates with background subtraction, divergences, periodic
+// guard copy (send_guard_cells on one periodic rank) and the binomial
+// current filter (ptc_updater.cu:501-625).  One thread per bulk cell; the
+// expressions keep the reference's operation order, and the file is built
+// without contraction, so T = double reproduces the CPU restatement bit for
+// bit.  T = float serves the mixed form.
+#include "apt_common.hpp"
+#include "apt_kernels.hpp"
+
+namespace apt {
+
+struct BulkIdx {
+  int i, j, k;
+  uint64_t o;
+};
+
+__device__ inline bool bulk_index(const Mesh &m, uint64_t t, BulkIdx &b) {
+  const uint64_t n0 = m.N[0], n1 = m.dims[1] - 2 * m.guard[1];
+  const uint64_t n2 = m.dims[2] - 2 * m.guard[2];
+  if (t >= n0 * n1 * n2) return false;
+  b.i = m.guard[0] + (int)(t % n0);
+  b.j = m.guard[1] + (int)((t / n0) % n1);
+  b.k = m.guard[2] + (int)(t / (n0 * n1));
+  b.o = m.idx(b.i, b.j, b.k);
+  return true;
+}
+
+__host__ inline uint64_t bulk_count(const Mesh &m) {
+  return (uint64_t)(m.dims[0] - 2 * m.guard[0]) *
+         (m.dims[1] - 2 * m.guard[1]) * (m.dims[2] - 2 * m.guard[2]);
+}
+
+#define FA(a, ii, jj, kk) (a)[m.idx((ii), (jj), (kk))]
+
+// compute_e_update{1,2,3}d (field_solver.cu:14-131)
+template <typename T, int DIM>
+__global__ void k_update_e(Mesh m, FieldSet<T> f, T dt) {
+  BulkIdx b;
+  if (!bulk_index(m, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, b))
+    return;
+  const int i = b.i, j = b.j, k = b.k;
+  const uint64_t o = b.o;
+  T *e1 = f.E[0], *e2 = f.E[1], *e3 = f.E[2];
+  const T *b1 = f.B[0], *b2 = f.B[1], *b3 = f.B[2];
+  const T *g1 = f.Bbg[0], *g2 = f.Bbg[1], *g3 = f.Bbg[2];
+  const T i0 = (T)m.inv_delta[0], i1 = (T)m.inv_delta[1],
+          i2 = (T)m.inv_delta[2];
+  if (DIM == 1) {
+    e1[o] -= dt * f.J[0][o];
+    e2[o] += dt * ((FA(b3, i, j, k) - FA(b3, i + 1, j, k) - FA(g3, i, j, k) +
+                    FA(g3, i + 1, j, k)) *
+                       i0 -
+                   f.J[1][o]);
+    e3[o] += dt * ((FA(b2, i + 1, j, k) - FA(b2, i, j, k) -
+                    FA(g2, i + 1, j, k) + FA(g2, i, j, k)) *
+                       i0 -
+                   f.J[2][o]);
+  } else if (DIM == 2) {
+    e1[o] += dt * ((FA(b3, i, j + 1, k) - FA(b3, i, j, k) -
+                    FA(g3, i, j + 1, k) + FA(g3, i, j, k)) *
+                       i1 -
+                   f.J[0][o]);
+    e2[o] += dt * ((FA(b3, i, j, k) - FA(b3, i + 1, j, k) - FA(g3, i, j, k) +
+                    FA(g3, i + 1, j, k)) *
+                       i0 -
+                   f.J[1][o]);
+    e3[o] += dt * ((FA(b2, i + 1, j, k) - FA(b2, i, j, k) -
+                    FA(g2, i + 1, j, k) + FA(g2, i, j, k)) *
+                       i0 +
+                   (FA(b1, i, j, k) - FA(b1, i, j + 1, k) - FA(g1, i, j, k) +
+                    FA(g1, i, j + 1, k)) *
+                       i1 -
+                   f.J[2][o]);
+  } else {
+    e1[o] += dt * ((FA(b3, i, j + 1, k) - FA(b3, i, j, k) -
+                    FA(g3, i, j + 1, k) + FA(g3, i, j, k)) *
+                       i1 +
+                   (FA(b2, i, j, k) - FA(b2, i, j, k + 1) - FA(g2, i, j, k) +
+                    FA(g2, i, j, k + 1)) *
+                       i2 -
+                   f.J[0][o]);
+    e2[o] += dt * ((FA(b3, i, j, k) - FA(b3, i + 1, j, k) - FA(g3, i, j, k) +
+                    FA(g3, i + 1, j, k)) *
+                       i0 +
+                   (FA(b1, i, j, k + 1) - FA(b1, i, j, k) -
+                    FA(g1, i, j, k + 1) + FA(g1, i, j, k)) *
+                       i2 -
+                   f.J[1][o]);
+    e3[o] += dt * ((FA(b2, i + 1, j, k) - FA(b2, i, j, k) -
+                    FA(g2, i + 1, j, k) + FA(g2, i, j, k)) *
+                       i0 +
+                   (FA(b1, i, j, k) - FA(b1, i, j + 1, k) - FA(g1, i, j, k) +
+                    FA(g1, i, j + 1, k)) *
+                       i1 -
+                   f.J[2][o]);
+  }
+}
+
+// compute_b_update{1,2,3}d (field_solver.cu:133-235).  The 3D kernel uses
+// the staggered differences of the 2D kernel and of div B; the reference's
+// 3D kernel (field_solver.cu:214-234) differences forward, one cell off.
+template <typename T, int DIM>
+__global__ void k_update_b(Mesh m, FieldSet<T> f, T dt) {
+  BulkIdx b;
+  if (!bulk_index(m, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, b))
+    return;
+  const int i = b.i, j = b.j, k = b.k;
+  const uint64_t o = b.o;
+  const T *e1 = f.E[0], *e2 = f.E[1], *e3 = f.E[2];
+  T *b1 = f.B[0], *b2 = f.B[1], *b3 = f.B[2];
+  const T *g1 = f.Ebg[0], *g2 = f.Ebg[1], *g3 = f.Ebg[2];
+  const T i0 = (T)m.inv_delta[0], i1 = (T)m.inv_delta[1],
+          i2 = (T)m.inv_delta[2];
+  if (DIM == 1) {
+    b2[o] -= dt *
+             (FA(e3, i - 1, j, k) - FA(e3, i, j, k) - FA(g3, i - 1, j, k) +
+              FA(g3, i, j, k)) *
+             i0;
+    b3[o] -= dt *
+             (FA(e2, i, j, k) - FA(e2, i - 1, j, k) - FA(g2, i, j, k) +
+              FA(g2, i - 1, j, k)) *
+             i0;
+  } else if (DIM == 2) {
+    b1[o] -= dt *
+             (FA(e3, i, j, k) - FA(e3, i, j - 1, k) - FA(g3, i, j, k) +
+              FA(g3, i, j - 1, k)) *
+             i1;
+    b2[o] -= dt *
+             (FA(e3, i - 1, j, k) - FA(e3, i, j, k) - FA(g3, i - 1, j, k) +
+              FA(g3, i, j, k)) *
+             i0;
+    b3[o] -= dt * ((FA(e2, i, j, k) - FA(e2, i - 1, j, k) - FA(g2, i, j, k) +
+                    FA(g2, i - 1, j, k)) *
+                       i0 +
+                   (FA(e1, i, j - 1, k) - FA(e1, i, j, k) -
+                    FA(g1, i, j - 1, k) + FA(g1, i, j, k)) *
+                       i1);
+  } else {
+    b1[o] -= dt * ((FA(e3, i, j, k) - FA(e3, i, j - 1, k) - FA(g3, i, j, k) +
+                    FA(g3, i, j - 1, k)) *
+                       i1 +
+                   (FA(e2, i, j, k - 1) - FA(e2, i, j, k) -
+                    FA(g2, i, j, k - 1) + FA(g2, i, j, k)) *
+                       i2);
+    b2[o] -= dt * ((FA(e3, i - 1, j, k) - FA(e3, i, j, k) -
+                    FA(g3, i - 1, j, k) + FA(g3, i, j, k)) *
+                       i0 +
+                   (FA(e1, i, j, k) - FA(e1, i, j, k - 1) - FA(g1, i, j, k) +
+                    FA(g1, i, j, k - 1)) *
+                       i2);
+    b3[o] -= dt * ((FA(e2, i, j, k) - FA(e2, i - 1, j, k) - FA(g2, i, j, k) +
+                    FA(g2, i - 1, j, k)) *
+                       i0 +
+                   (FA(e1, i, j - 1, k) - FA(e1, i, j, k) -
+                    FA(g1, i, j - 1, k) + FA(g1, i, j, k)) *
+                       i1);
+  }
+}
+
+// compute_divs_{1,2,3}d (field_solver.cu:237-326)
+template <typename T, int DIM>
+__global__ void k_divs(Mesh m, FieldSet<T> f) {
+  BulkIdx b;
+  if (!bulk_index(m, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, b))
+    return;
+  const int i = b.i, j = b.j, k = b.k;
+  const T *e1 = f.E[0], *e2 = f.E[1], *e3 = f.E[2];
+  const T *b1 = f.B[0], *b2 = f.B[1], *b3 = f.B[2];
+  const T *ge1 = f.Ebg[0], *ge2 = f.Ebg[1], *ge3 = f.Ebg[2];
+  const T *gb1 = f.Bbg[0], *gb2 = f.Bbg[1], *gb3 = f.Bbg[2];
+  const T i0 = (T)m.inv_delta[0], i1 = (T)m.inv_delta[1],
+          i2 = (T)m.inv_delta[2];
+  T dE = (FA(e1, i + 1, j, k) - FA(ge1, i + 1, j, k) - FA(e1, i, j, k) +
+          FA(ge1, i, j, k)) *
+         i0;
+  T dB = (FA(b1, i, j, k) - FA(gb1, i, j, k) - FA(b1, i - 1, j, k) +
+          FA(gb1, i - 1, j, k)) *
+         i0;
+  if (DIM >= 2) {
+    dE = dE + (FA(e2, i, j + 1, k) - FA(ge2, i, j + 1, k) - FA(e2, i, j, k) +
+               FA(ge2, i, j, k)) *
+                  i1;
+    dB = dB + (FA(b2, i, j, k) - FA(gb2, i, j, k) - FA(b2, i, j - 1, k) +
+               FA(gb2, i, j - 1, k)) *
+                  i1;
+  }
+  if (DIM == 3) {
+    dE = dE + (FA(e3, i, j, k + 1) - FA(ge3, i, j, k + 1) - FA(e3, i, j, k) +
+               FA(ge3, i, j, k)) *
+                  i2;
+    dB = dB + (FA(b3, i, j, k) - FA(gb3, i, j, k) - FA(b3, i, j, k - 1) +
+               FA(gb3, i, j, k - 1)) *
+                  i2;
+  }
+  f.divE[b.o] = dE;
+  f.divB[b.o] = dB;
+}
+
+// Periodic guard copy: every guard cell takes the value of the wrapped bulk
+// cell (send_guard_cells, cuda/sim_environment.cpp:10-69).
+template <typename T>
+__global__ void k_guard_copy(Mesh m, T *a) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m.size()) return;
+  int i = (int)(t % m.dims[0]);
+  int j = (int)((t / m.dims[0]) % m.dims[1]);
+  int k = (int)(t / ((uint64_t)m.dims[0] * m.dims[1]));
+  int wi = m.wrap(0, i), wj = m.wrap(1, j), wk = m.wrap(2, k);
+  if (wi == i && wj == j && wk == k) return;
+  a[t] = a[m.idx(wi, wj, wk)];
+}
+
+// filter_current_cart_{1,2,3}d (ptc_updater.cu:501-625), periodic
+template <typename T, int DIM>
+__global__ void k_filter(Mesh m, const T *j, T *tmp, T we) {
+  BulkIdx b;
+  if (!bulk_index(m, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, b))
+    return;
+  const int i = b.i, jj = b.j, k = b.k;
+  T r;
+  if (DIM == 1) {
+    r = (T)0.5 * j[b.o];
+    r += (T)0.25 * FA(j, i + 1, jj, k);
+    r += (T)0.25 * FA(j, i - 1, jj, k);
+  } else if (DIM == 2) {
+    r = (T)0.25 * j[b.o];
+    r += (T)0.125 * FA(j, i + 1, jj, k);
+    r += (T)0.125 * FA(j, i - 1, jj, k);
+    r += (T)0.125 * FA(j, i, jj + 1, k);
+    r += (T)0.125 * FA(j, i, jj - 1, k);
+    r += (T)0.0625 * FA(j, i + 1, jj + 1, k);
+    r += (T)0.0625 * FA(j, i - 1, jj + 1, k);
+    r += (T)0.0625 * FA(j, i + 1, jj - 1, k);
+    r += (T)0.0625 * FA(j, i - 1, jj - 1, k);
+  } else {
+    r = (T)0.125 * j[b.o];
+    r += (T)0.0625 * FA(j, i + 1, jj, k);
+    r += (T)0.0625 * FA(j, i - 1, jj, k);
+    r += (T)0.0625 * FA(j, i, jj + 1, k);
+    r += (T)0.0625 * FA(j, i, jj - 1, k);
+    r += (T)0.03125 * FA(j, i + 1, jj + 1, k);
+    r += (T)0.03125 * FA(j, i - 1, jj + 1, k);
+    r += (T)0.03125 * FA(j, i + 1, jj - 1, k);
+    r += (T)0.03125 * FA(j, i - 1, jj - 1, k);
+    for (int s = 0; s < 2; s++) {
+      int kk = s == 0 ? k + 1 : k - 1;
+      r += (T)0.0625 * FA(j, i, jj, kk);
+      r += we * FA(j, i + 1, jj, kk);
+      r += we * FA(j, i - 1, jj, kk);
+      r += we * FA(j, i, jj + 1, kk);
+      r += we * FA(j, i, jj - 1, kk);
+      r += (T)0.015625 * FA(j, i + 1, jj + 1, kk);
+      r += (T)0.015625 * FA(j, i - 1, jj + 1, kk);
+      r += (T)0.015625 * FA(j, i + 1, jj - 1, kk);
+      r += (T)0.015625 * FA(j, i - 1, jj - 1, kk);
+    }
+  }
+  tmp[b.o] = r;
+}
+
+template <typename T>
+__global__ void k_copy_bulk(Mesh m, const T *src, T *dst) {
+  BulkIdx b;
+  if (!bulk_index(m, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, b))
+    return;
+  dst[b.o] = src[b.o];
+}
+
+template <typename T>
+__global__ void k_fill(T *a, T v, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+#undef FA
+
+static inline dim3 grid_for(uint64_t n) {
+  return dim3((unsigned)((n + 255) / 256));
+}
+
+template <typename T>
+hipError_t launch_update_e(const Mesh &m, const FieldSet<T> &f, double dt,
+                           hipStream_t st) {
+  uint64_t n = bulk_count(m);
+  if (m.dim == 1)
+    k_update_e<T, 1><<<grid_for(n), 256, 0, st>>>(m, f, (T)dt);
+  else if (m.dim == 2)
+    k_update_e<T, 2><<<grid_for(n), 256, 0, st>>>(m, f, (T)dt);
+  else
+    k_update_e<T, 3><<<grid_for(n), 256, 0, st>>>(m, f, (T)dt);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_update_b(const Mesh &m, const FieldSet<T> &f, double dt,
+                           hipStream_t st) {
+  uint64_t n = bulk_count(m);
+  if (m.dim == 1)
+    k_update_b<T, 1><<<grid_for(n), 256, 0, st>>>(m, f, (T)dt);
+  else if (m.dim == 2)
+    k_update_b<T, 2><<<grid_for(n), 256, 0, st>>>(m, f, (T)dt);
+  else
+    k_update_b<T, 3><<<grid_for(n), 256, 0, st>>>(m, f, (T)dt);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_divs(const Mesh &m, const FieldSet<T> &f, hipStream_t st) {
+  uint64_t n = bulk_count(m);
+  if (m.dim == 1)
+    k_divs<T, 1><<<grid_for(n), 256, 0, st>>>(m, f);
+  else if (m.dim == 2)
+    k_divs<T, 2><<<grid_for(n), 256, 0, st>>>(m, f);
+  else
+    k_divs<T, 3><<<grid_for(n), 256, 0, st>>>(m, f);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_guard_copy(const Mesh &m, T *a, hipStream_t st) {
+  uint64_t n = m.size();
+  k_guard_copy<T><<<grid_for(n), 256, 0, st>>>(m, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_filter(const Mesh &m, T *j, T *tmp, int fix3d,
+                         hipStream_t st) {
+  uint64_t n = bulk_count(m);
+  T we = fix3d ? (T)0.03125 : (T)0.03152f;
+  hipError_t e = launch_guard_copy<T>(m, j, st);
+  if (e != hipSuccess) return e;
+  if (m.dim == 1)
+    k_filter<T, 1><<<grid_for(n), 256, 0, st>>>(m, j, tmp, we);
+  else if (m.dim == 2)
+    k_filter<T, 2><<<grid_for(n), 256, 0, st>>>(m, j, tmp, we);
+  else
+    k_filter<T, 3><<<grid_for(n), 256, 0, st>>>(m, j, tmp, we);
+  k_copy_bulk<T><<<grid_for(n), 256, 0, st>>>(m, tmp, j);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_guard_copy<T>(m, j, st);
+}
+
+template <typename T>
+hipError_t launch_fill(T *a, T v, uint64_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  k_fill<T><<<grid_for(n), 256, 0, st>>>(a, v, n);
+  return hipGetLastError();
+}
+
+#define INST(T)                                                             \
+  template hipError_t launch_update_e<T>(const Mesh &, const FieldSet<T> &, \
+                                         double, hipStream_t);              \
+  template hipError_t launch_update_b<T>(const Mesh &, const FieldSet<T> &, \
+                                         double, hipStream_t);              \
+  template hipError_t launch_divs<T>(const Mesh &, const FieldSet<T> &,     \
+                                     hipStream_t);                          \
+  template hipError_t launch_guard_copy<T>(const Mesh &, T *, hipStream_t); \
+  template hipError_t launch_filter<T>(const Mesh &, T *, T *, int,         \
+                                       hipStream_t);                        \
+  template hipError_t launch_fill<T>(T *, T, uint64_t, hipStream_t);
+INST(double)
+INST(float)
+template hipError_t launch_fill<long long>(long long *, long long, uint64_t,
+                                           hipStream_t);
+template hipError_t launch_fill<uint32_t>(uint32_t *, uint32_t, uint64_t,
+                                          hipStream_t);
+#undef INST
+
+}  // namespace apt

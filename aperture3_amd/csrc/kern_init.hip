@@ -1,0 +1,249 @@
+// kern_init.hip -- particle loading, guard-cell erase and the energy
+// histogram.  Loading follows the pattern of fill_particles
+// (cuda/sim_data.cu:124-161: per bulk cell, co-located e-/e+ pairs) but
+// draws positions and Maxwell-Juttner momenta from the counter-based Philox
+// stream instead of per-thread XORWOW states (kernels.cu:11-15), so the
+// result does not depend on launch shape.  Built without contraction: the
+// FP64 loader reproduces the CPU restatement bit for bit.
+#include "apt_common.hpp"
+#include "apt_kernels.hpp"
+
+namespace apt {
+
+constexpr uint32_t MJ_MAX_TRIES = 4096u;
+
+// Maxwell-Juttner momentum in the rest frame, then Zenitani's flipping
+// method and a boost along x for a drift beta_x.  theta < 0.4: rejection
+// from a Gamma(3/2, theta') proposal for the kinetic energy t = gamma-1
+// (1/theta' = 1/theta - 5/4, using 1+x <= e^x to bound the MJ density
+// sqrt(t(t+2))(t+1)e^{-t/theta}); Gamma(3/2) = Exp(1) + Z^2/2, Z^2/2 from
+// Marsaglia's polar pair.  theta >= 0.4: Sobol's four-uniform algorithm.
+// Direction: Marsaglia's (1972) point on the sphere.
+__device__ inline bool sample_mj(double theta, double beta_x, uint64_t seed,
+                                 uint64_t pid, double &ux, double &uy,
+                                 double &uz) {
+  double umag = 0.0;
+  bool ok = false;
+  if (theta > 0.0 && theta < 0.4) {
+    double thp = 1.0 / (1.0 / theta - 1.25);
+    for (uint32_t draw = 0; draw < MJ_MAX_TRIES; draw++) {
+      double a0, a1, b0, b1;
+      uniform2(seed, pid, 2 * draw, 1, a0, a1);
+      uniform2(seed, pid, 2 * draw + 1, 1, b0, b1);
+      double v1 = 2.0 * b0 - 1.0, v2 = 2.0 * b1 - 1.0;
+      double s = v1 * v1 + v2 * v2;
+      if (s >= 1.0 || s <= 0.0) continue;
+      double t = thp * (-apt_log(a0) - v1 * v1 * apt_log(s) / s);
+      double lhs = apt_log(a1);
+      double rhs =
+          0.5 * apt_log(1.0 + 0.5 * t) + apt_log(1.0 + t) - 1.25 * t;
+      if (lhs < rhs) {
+        umag = sqrt(t * (t + 2.0));
+        ok = true;
+        break;
+      }
+    }
+  } else if (theta >= 0.4) {
+    for (uint32_t draw = 0; draw < MJ_MAX_TRIES; draw++) {
+      double a0, a1, b0, b1;
+      uniform2(seed, pid, 2 * draw, 1, a0, a1);
+      uniform2(seed, pid, 2 * draw + 1, 1, b0, b1);
+      double x123 = a0 * a1 * b0;
+      double uu = -theta * apt_log(x123);
+      double eta = -theta * apt_log(x123 * b1);
+      if (eta * eta - uu * uu > 1.0) {
+        umag = uu;
+        ok = true;
+        break;
+      }
+    }
+  } else {
+    ok = true;
+  }
+  double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+  if (umag > 0.0) {
+    for (uint32_t draw = 0; draw < MJ_MAX_TRIES; draw++) {
+      double b0, b1;
+      uniform2(seed, pid, draw, 2, b0, b1);
+      double v1 = 2.0 * b0 - 1.0, v2 = 2.0 * b1 - 1.0;
+      double s = v1 * v1 + v2 * v2;
+      if (s >= 1.0) continue;
+      double r = 2.0 * sqrt(1.0 - s);
+      d0 = 1.0 - 2.0 * s;
+      d1 = v1 * r;
+      d2 = v2 * r;
+      break;
+    }
+  }
+  ux = umag * d0;
+  uy = umag * d1;
+  uz = umag * d2;
+  if (beta_x != 0.0) {
+    double g = sqrt(1.0 + ux * ux + uy * uy + uz * uz);
+    double c0, c1;
+    uniform2(seed, pid, 0, 3, c0, c1);
+    if (-beta_x * ux / g > c0) ux = -ux;
+    double G = 1.0 / sqrt(1.0 - beta_x * beta_x);
+    ux = G * (ux + beta_x * g);
+  }
+  return ok;
+}
+
+struct LoadedPtc {
+  uint32_t cell;
+  double x[3], u[3], gamma;
+  uint32_t flag;
+  uint64_t id;
+};
+
+__device__ inline bool load_one(const Mesh &m, const LoadParams &lp,
+                                uint64_t t, LoadedPtc &out) {
+  const uint64_t pid = lp.base + t;
+  const uint64_t pair_local = t / 2;
+  const int s = (int)(t & 1);
+  const uint64_t rank = pair_local / (uint64_t)lp.ppc;
+  const uint64_t n0 = m.N[0], n1 = m.dims[1] - 2 * m.guard[1];
+  int i = m.guard[0] + (int)(rank % n0);
+  int j = m.guard[1] + (int)((rank / n0) % n1);
+  int k = m.guard[2] + (int)(rank / (n0 * n1));
+  out.cell = (uint32_t)m.idx(i, j, k);
+  double a0, a1, b0, b1;
+  uniform2(lp.seed, pid / 2, 0, 0, a0, a1);
+  uniform2(lp.seed, pid / 2, 1, 0, b0, b1);
+  out.x[0] = 1.0 - a0;
+  out.x[1] = m.dim > 1 ? 1.0 - a1 : 0.0;
+  out.x[2] = m.dim > 2 ? 1.0 - b0 : 0.0;
+  bool ok = sample_mj(lp.theta[s], lp.drift[s], lp.seed, pid, out.u[0],
+                      out.u[1], out.u[2]);
+  out.gamma =
+      sqrt(1.0 + out.u[0] * out.u[0] + out.u[1] * out.u[1] +
+           out.u[2] * out.u[2]);
+  out.flag = ((uint32_t)s) << 29;
+  out.id = pid;
+  return ok;
+}
+
+__global__ void k_load_f64(Mesh m, LoadParams lp, PtcF64 p,
+                           unsigned long long *counters, uint64_t n) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  LoadedPtc q;
+  if (!load_one(m, lp, t, q)) atomicAdd(&counters[CTR_OVERFLOW], 1ull);
+  uint64_t o = lp.base + t;
+  p.cell[o] = q.cell;
+  p.x1[o] = q.x[0];
+  p.x2[o] = q.x[1];
+  p.x3[o] = q.x[2];
+  p.p1[o] = q.u[0];
+  p.p2[o] = q.u[1];
+  p.p3[o] = q.u[2];
+  p.E[o] = q.gamma;
+  p.w[o] = lp.weight;
+  p.flag[o] = q.flag;
+  p.id[o] = q.id;
+}
+
+__global__ void k_load_f32(Mesh m, LoadParams lp, PtcF32 p,
+                           unsigned long long *counters, uint64_t n) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  LoadedPtc q;
+  if (!load_one(m, lp, t, q)) atomicAdd(&counters[CTR_OVERFLOW], 1ull);
+  uint64_t o = lp.base + t;
+  p.cell[o] = q.cell;
+  p.x1[o] = (float)q.x[0];
+  p.x2[o] = (float)q.x[1];
+  p.x3[o] = (float)q.x[2];
+  // an offset that rounds up to 1.0f stays inside its cell
+  if (p.x1[o] >= 1.0f) p.x1[o] = 0x1.fffffep-1f;
+  if (p.x2[o] >= 1.0f) p.x2[o] = 0x1.fffffep-1f;
+  if (p.x3[o] >= 1.0f) p.x3[o] = 0x1.fffffep-1f;
+  p.u1[o] = (float)q.u[0];
+  p.u2[o] = (float)q.u[1];
+  p.u3[o] = (float)q.u[2];
+  p.w[o] = (float)lp.weight;
+  p.flag[o] = q.flag;
+}
+
+static uint64_t bulk_cells(const Mesh &m) {
+  return (uint64_t)m.N[0] * (m.dims[1] - 2 * m.guard[1]) *
+         (m.dims[2] - 2 * m.guard[2]);
+}
+
+hipError_t launch_load_uniform_f64(const Mesh &m, const LoadParams &lp,
+                                   const PtcF64 &p,
+                                   unsigned long long *counters,
+                                   hipStream_t st) {
+  uint64_t n = bulk_cells(m) * (uint64_t)lp.ppc * 2;
+  if (n == 0) return hipSuccess;
+  k_load_f64<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(m, lp, p, counters,
+                                                         n);
+  return hipGetLastError();
+}
+
+hipError_t launch_load_uniform_f32(const Mesh &m, const LoadParams &lp,
+                                   const PtcF32 &p,
+                                   unsigned long long *counters,
+                                   hipStream_t st) {
+  uint64_t n = bulk_cells(m) * (uint64_t)lp.ppc * 2;
+  if (n == 0) return hipSuccess;
+  k_load_f32<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(m, lp, p, counters,
+                                                         n);
+  return hipGetLastError();
+}
+
+// erase_ptc_in_guard_cells (kernels.cu:25-34)
+__global__ void k_erase_guard(Mesh m, uint32_t *cell, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c = cell[i];
+  if (c == MAX_CELL) return;
+  int c1, c2, c3;
+  m.split(c, c1, c2, c3);
+  if (!m.in_bulk(c1, c2, c3)) cell[i] = MAX_CELL;
+}
+
+hipError_t launch_erase_guard(const Mesh &m, uint32_t *cell, uint64_t n,
+                              hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  k_erase_guard<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(m, cell, n);
+  return hipGetLastError();
+}
+
+// compute_energy_histogram (kernels.cu:36-69): ln(gamma) bins of width
+// ln(E_max)/nbins, clamped, per-block LDS histogram then one global add per
+// bin.  flag_bit < 0 counts every slot below the particle number.
+__global__ void k_energy_hist(const double *E, const uint32_t *flag,
+                              uint64_t n, uint32_t *hist, int nbins,
+                              double dlogE, int flag_bit) {
+  extern __shared__ uint32_t sh[];
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) sh[b] = 0;
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (flag_bit >= 0 && !(flag[i] & (1u << flag_bit))) continue;
+    double logE = apt_log(E[i]);
+    int idx = (int)floor(logE / dlogE);
+    if (idx < 0) idx = 0;
+    if (idx >= nbins) idx = nbins - 1;
+    atomicAdd(&sh[idx], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+    if (sh[b]) atomicAdd(&hist[b], sh[b]);
+}
+
+hipError_t launch_energy_histogram_f64(const double *E, const uint32_t *flag,
+                                       uint64_t n, uint32_t *hist, int nbins,
+                                       double E_max, int flag_bit,
+                                       hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  double dlogE = apt_log(E_max) / (double)nbins;
+  unsigned blocks = (unsigned)((n + 255) / 256);
+  if (blocks > 2048) blocks = 2048;
+  k_energy_hist<<<blocks, 256, nbins * sizeof(uint32_t), st>>>(
+      E, flag, n, hist, nbins, dlogE, flag_bit);
+  return hipGetLastError();
+}
+
+}  // namespace apt

@@ -1,0 +1,189 @@
+/*
+ * aperture_amd.h -- C ABI of the MI355X-native Aperture PIC time step.
+ *
+ * Plain C: opaque context, pointers and sizes, status codes, no exceptions
+ * and no C++ or torch types.  One context owns the device fields,
+ * particles and a HIP stream (the state sim_data owns in the reference,
+ * src/sim_data.h:14-70).  Every call is stream ordered; calls that return
+ * data to the host synchronise the context's stream first.  A context is
+ * used from one host thread at a time.
+ *
+ * Each entry point cites the reference interface it replaces (paths relative
+ * to the fizban007/Aperture3 tree).
+ */
+#ifndef APERTURE_AMD_H
+#define APERTURE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct apt_ctx apt_ctx;
+typedef int apt_status;
+
+/* Status codes.  The reference throws (cudaUtility.h:62-74), exits
+ * (cudaUtility.h:35-58, sim_environment_impl.hpp:54-59) or traps
+ * (ptc_updater.cu:43-50); here every failure is a return code plus
+ * apt_last_error(). */
+enum {
+  APT_OK = 0,
+  APT_ERR_INVALID_ARG = 1,
+  APT_ERR_HIP = 2,
+  APT_ERR_CAPACITY = 3,
+  APT_ERR_STATE = 4,
+  APT_ERR_UNSUPPORTED = 5
+};
+
+/* Numeric forms. */
+enum {
+  APT_PREC_FP64 = 0,  /* plain FP64: the drop-in */
+  APT_PREC_MIXED = 1  /* FP32 push/fields, int32 cell + FP32 offset,
+                         int64 fixed-point current */
+};
+
+/* Field identifiers (the members of sim_data, src/sim_data.h:40-60). */
+enum {
+  APT_FIELD_E = 0,
+  APT_FIELD_B = 1,
+  APT_FIELD_EBG = 2,
+  APT_FIELD_BBG = 3,
+  APT_FIELD_J = 4,
+  APT_FIELD_RHO = 5, /* component = species */
+  APT_FIELD_DIVE = 6,
+  APT_FIELD_DIVB = 7
+};
+
+/* Timed kernels (apt_timing_get). */
+enum {
+  APT_KERNEL_PUSH_DEPOSIT = 0,
+  APT_KERNEL_FIELD = 1,
+  APT_KERNEL_SORT = 2,
+  APT_KERNEL_NUM = 3
+};
+
+/* Grid section of the config (sim_params.h:90-96) plus periodic_boundary
+ * (sim_params.h:38).  Absent dimensions have N = 1 and guard = 0; the
+ * dimensionality is the number of leading dimensions with N > 1
+ * (quadmesh.h:357). */
+typedef struct {
+  int N[3];
+  int guard[3];
+  double lower[3];
+  double size[3];
+  int periodic[3];
+} apt_grid;
+
+/* Physics and run parameters (SimParamsBase, sim_params.h:17-98). */
+typedef struct {
+  int precision;            /* APT_PREC_* */
+  int num_species;          /* num_species (sim_params.h:28) */
+  double q_e;               /* q_e (:20); electrons carry -q_e */
+  double ion_mass;          /* ion_mass (:27), in units of q_e */
+  int data_interval;        /* data_interval (:46): Rho deposit cadence */
+  int current_smoothing;    /* current_smoothing (:42) filter passes */
+  int fix_3d_filter;        /* 1: 3D filter edge weight 1/32 instead of
+                               the reference's 0.03152 */
+  uint64_t max_ptc_number;  /* max_ptc_number (:22) */
+  uint64_t random_seed;     /* random_seed (:116) */
+  int tile[3];              /* mixed form: tile extent in cells (0 = auto) */
+} apt_params;
+
+/* Host view of particle_data (src/data/particle_data.h:50-61) in FP64.
+ * Any member may be NULL on apt_ptc_get to skip it. */
+typedef struct {
+  uint32_t *cell;
+  double *x1, *x2, *x3;
+  double *p1, *p2, *p3;
+  double *E; /* gamma */
+  double *weight;
+  uint32_t *flag;
+  uint64_t *id;
+} apt_particles;
+
+/* Device-side event counters that replace the reference's trap/printf
+ * diagnostics (ptc_updater.cu:43-50, 251-255). */
+typedef struct {
+  uint64_t nan_momenta;
+  uint64_t multi_cell_moves;
+  uint64_t capacity_overflow;
+  uint64_t outside_tile;
+} apt_counters;
+
+/* ---- lifetime ------------------------------------------------------- */
+/* sim_environment(int*, char***) + sim_data(env)
+ * (sim_environment_impl.hpp:17-65, sim_data_impl.hpp:12-71) */
+apt_status apt_create(const apt_grid *grid, const apt_params *params,
+                      apt_ctx **out);
+apt_status apt_destroy(apt_ctx *ctx);
+const char *apt_last_error(const apt_ctx *ctx);
+/* fixed-point exponent H of the current (J = hi 2^-H + lo 2^-(H+40) in
+ * the FP64 form, J = acc 2^-H in the mixed form) */
+apt_status apt_get_jexp(const apt_ctx *ctx, int *H);
+apt_status apt_get_mesh(const apt_ctx *ctx, int dims[3], double delta[3]);
+
+/* ---- data movement (multi_array copy_to/from_device, fields.h) ------- */
+/* field<N,T>::initialize / copy_to_device (core/fields.h:111-143) */
+apt_status apt_field_set(apt_ctx *ctx, int which, int comp,
+                         const double *src, uint64_t count);
+/* field<N,T>::copy_to_host */
+apt_status apt_field_get(apt_ctx *ctx, int which, int comp, double *dst,
+                         uint64_t count);
+/* particles_t::copy_from / set_num (cuda/core/particle_base_impl.cu) */
+apt_status apt_ptc_set(apt_ctx *ctx, const apt_particles *src, uint64_t n);
+/* particles_t::copy_to_host */
+apt_status apt_ptc_get(apt_ctx *ctx, apt_particles *dst, uint64_t cap,
+                       uint64_t *n);
+/* particle_base::number() */
+apt_status apt_ptc_number(apt_ctx *ctx, uint64_t *n);
+/* sim_data::fill_multiplicity (cuda/sim_data.cu:268-293) with
+ * Maxwell-Juttner momenta: ppc co-located e-/e+ pairs per bulk cell,
+ * theta[s] and x-drift[s] per species (s = 0 electrons, 1 positrons). */
+apt_status apt_ptc_load_uniform(apt_ctx *ctx, int ppc_pairs,
+                                const double theta[2], const double drift[2],
+                                double weight);
+
+/* ---- the time step -------------------------------------------------- */
+/* ptc_updater::update_particles(sim_data&, double dt, uint32_t step)
+ * (src/algorithms/ptc_updater.h:11-24, cuda/algorithms/ptc_updater.cu:
+ * 748-841): zero J (and Rho on deposit steps), push, move, deposit,
+ * periodic fold and wrap, smoothing. */
+apt_status apt_ptc_update(apt_ctx *ctx, double dt, uint32_t step);
+/* ptc_updater::smooth_current(sim_data&, uint32_t step)
+ * (ptc_updater.cu:852-873): current_smoothing binomial passes over J (and
+ * Rho on deposit steps); apt_ptc_update already runs them once. */
+apt_status apt_smooth_current(apt_ctx *ctx, uint32_t step);
+/* field_solver::update_fields(sim_data&, double dt, uint32_t step)
+ * (src/algorithms/field_solver.h:11-25, field_solver.cu:334-358) */
+apt_status apt_field_update(apt_ctx *ctx, double dt, uint32_t step);
+/* field_solver::update_e_field / update_b_field / compute_divs
+ * (field_solver.cu:360-504) */
+apt_status apt_field_update_e(apt_ctx *ctx, double dt);
+apt_status apt_field_update_b(apt_ctx *ctx, double dt);
+apt_status apt_field_compute_divs(apt_ctx *ctx);
+/* sim_data::sort_particles -> particle_base::sort_by_cell
+ * (sim_data_impl.hpp:78-89, cuda/core/particle_base_impl.cu:435-464) */
+apt_status apt_sort_particles(apt_ctx *ctx);
+/* particle_base::clear_guard_cells (particle_base_impl.cu:489-493) */
+apt_status apt_clear_guard_cells(apt_ctx *ctx);
+/* Kernels::compute_energy_histogram (cuda/kernels.cu:36-69); hist has nbins
+ * entries and is overwritten.  flag_bit < 0: every particle slot. */
+apt_status apt_energy_histogram(apt_ctx *ctx, uint32_t *hist, int nbins,
+                                double E_max, int flag_bit);
+
+/* ---- control -------------------------------------------------------- */
+apt_status apt_sync(apt_ctx *ctx);
+apt_status apt_get_counters(apt_ctx *ctx, apt_counters *out);
+apt_status apt_reset_counters(apt_ctx *ctx);
+/* HIP-event timing of the kernels named by APT_KERNEL_* on the context's
+ * stream; apt_timing_get synchronises and returns the summed milliseconds
+ * and launch count since the last enable. */
+apt_status apt_timing_enable(apt_ctx *ctx, int on);
+apt_status apt_timing_get(apt_ctx *ctx, int kernel, double *total_ms,
+                          uint64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* APERTURE_AMD_H */

@@ -1,0 +1,925 @@
+/*
+ * pic_oracle.c -- CPU FP64 restatement of Aperture's Cartesian PIC step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product): see
+ * pic_oracle.h.  Compiled with -ffp-contract=off so every expression is
+ * evaluated exactly in the order written; the HIP FP64 kernels are built the
+ * same way, which is what makes bit-exact parity possible.
+ *
+ * Reference files restated (relative to the Aperture3 tree):
+ *   src/cuda/algorithms/ptc_updater.cu      push / move / Esirkepov deposit
+ *   src/cuda/algorithms/vay_push.cuh        Vay pusher
+ *   src/cuda/algorithms/ptc_updater_helper.cuh  center2d / movement2d/3d
+ *   src/cuda/utils/interpolation.cuh        CIC spline, interp_cell, gathers
+ *   src/cuda/algorithms/field_solver.cu     Yee E/B updates, divergences
+ *   src/cuda/sim_environment.cpp            guard copy / fold (periodic)
+ *   src/core/particle_base_impl.hpp         counting sort + compaction
+ *   src/cuda/kernels.cu                     guard erase, energy histogram
+ */
+#include "pic_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11, "Parallel random   */
+/* numbers: as easy as 1, 2, 3").  Multipliers and Weyl constants are    */
+/* the published ones.                                                    */
+/* ------------------------------------------------------------------ */
+#define PHILOX_M0 0xD2511F53u
+#define PHILOX_M1 0xCD9E8D57u
+#define PHILOX_W0 0x9E3779B9u
+#define PHILOX_W1 0xBB67AE85u
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
+                       uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; r++) {
+    uint64_t pa = (uint64_t)PHILOX_M0 * c0;
+    uint64_t pb = (uint64_t)PHILOX_M1 * c2;
+    uint32_t hi0 = (uint32_t)(pa >> 32), lo0 = (uint32_t)pa;
+    uint32_t hi1 = (uint32_t)(pb >> 32), lo1 = (uint32_t)pb;
+    uint32_t n0 = hi1 ^ c1 ^ k0;
+    uint32_t n1 = lo1;
+    uint32_t n2 = hi0 ^ c3 ^ k1;
+    uint32_t n3 = lo0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += PHILOX_W0;
+    k1 += PHILOX_W1;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Two uniforms in (0,1] from one Philox block: 53 bits each. */
+static void orc_uniform2(uint64_t seed, uint64_t pid, uint32_t draw,
+                         uint32_t purpose, double u[2]) {
+  uint32_t ctr[4] = {(uint32_t)pid, (uint32_t)(pid >> 32), draw, purpose};
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t r[4];
+  orc_philox4x32_10(ctr, key, r);
+  uint64_t a = (((uint64_t)r[0] << 32) | r[1]) >> 11;
+  uint64_t b = (((uint64_t)r[2] << 32) | r[3]) >> 11;
+  u[0] = (double)(a + 1) * 0x1p-53;
+  u[1] = (double)(b + 1) * 0x1p-53;
+}
+
+/* Natural log for normal positive doubles: x = m 2^e, m in [sqrt(1/2),
+ * sqrt(2)), ln m = 2 atanh(f), f = (m-1)/(m+1), as the odd series
+ * 2f sum s^k/(2k+1), s = f^2 (|f| <= 0.1716, 12 terms reach 2^-56).
+ * ln2 split hi/lo as in Cody & Waite.  Evaluated without contraction so the
+ * device copy returns identical bits. */
+double orc_log(double x) {
+  union { double d; uint64_t u; } v;
+  v.d = x;
+  int e = (int)((v.u >> 52) & 0x7ff) - 1023;
+  v.u = (v.u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;
+  double m = v.d;
+  if (m > 1.4142135623730951) {
+    m = m * 0.5;
+    e = e + 1;
+  }
+  double f = (m - 1.0) / (m + 1.0);
+  double s = f * f;
+  double p = 1.0 / 23.0;
+  p = p * s + 1.0 / 21.0;
+  p = p * s + 1.0 / 19.0;
+  p = p * s + 1.0 / 17.0;
+  p = p * s + 1.0 / 15.0;
+  p = p * s + 1.0 / 13.0;
+  p = p * s + 1.0 / 11.0;
+  p = p * s + 1.0 / 9.0;
+  p = p * s + 1.0 / 7.0;
+  p = p * s + 1.0 / 5.0;
+  p = p * s + 1.0 / 3.0;
+  p = p * s + 1.0;
+  double lnm = 2.0 * f * p;
+  const double ln2_hi = 6.93147180369123816490e-01;
+  const double ln2_lo = 1.90821492927058770002e-10;
+  double de = (double)e;
+  return de * ln2_hi + (de * ln2_lo + lnm);
+}
+
+/* ------------------------------------------------------------------ */
+/* Fixed point: value = hi 2^-H + lo 2^-(H+40).  trunc/sub are exact    */
+/* for |c 2^H| < 2^52, so only lo rounds (to 2^-(H+40)).  Integer sums  */
+/* are order independent, which replaces the reference's float          */
+/* atomicAdd (ptc_updater.cu:278-289, 367-382, 474-494).                */
+/* ------------------------------------------------------------------ */
+void orc_fx_split(double c, int H, int64_t *hi, int64_t *lo) {
+  double x = ldexp(c, H);
+  double t = trunc(x);
+  double r = x - t;
+  *hi = (int64_t)t;
+  *lo = (int64_t)rint(r * 0x1p40);
+}
+
+double orc_fx_value(int64_t hi, int64_t lo, int H) {
+  return ldexp((double)hi, -H) + ldexp((double)lo, -(H + ORC_LIMB_BITS));
+}
+
+/* ------------------------------------------------------------------ */
+/* Maxwell-Juttner sampling (not in the reference, whose fill_particles */
+/* loads p = 0, cuda/sim_data.cu:134-140).  theta < 0.4: kinetic energy  */
+/* t = gamma-1 has density ~ sqrt(t(t+2))(t+1)e^{-t/theta}; proposal      */
+/* Gamma(3/2, theta') with 1/theta' = 1/theta - 5/4 (bound 1+x <= e^x),  */
+/* Gamma(3/2) = Exp(1) + Z^2/2 with Z from Marsaglia's polar method;     */
+/* accept if ln U < ln(1+t/2)/2 + ln(1+t) - 5t/4.  theta >= 0.4: Sobol's  */
+/* algorithm (as described by Zenitani 2015, Phys. Plasmas 22, 042116).  */
+/* Direction: Marsaglia (1972) sphere point.  Drift: Zenitani's flipping */
+/* method then a Lorentz boost along x.                                  */
+/* ------------------------------------------------------------------ */
+#define ORC_MAX_TRIES 4096u
+
+int orc_sample_mj(double theta, double beta_x, uint64_t seed, uint64_t pid,
+                  double u[3]) {
+  double umag = 0.0;
+  int ok = 0;
+  uint32_t draw = 0;
+  if (theta > 0.0 && theta < 0.4) {
+    double thp = 1.0 / (1.0 / theta - 1.25);
+    for (; draw < ORC_MAX_TRIES; draw++) {
+      double a[2], b[2];
+      orc_uniform2(seed, pid, 2 * draw, 1, a);
+      orc_uniform2(seed, pid, 2 * draw + 1, 1, b);
+      double v1 = 2.0 * b[0] - 1.0, v2 = 2.0 * b[1] - 1.0;
+      double s = v1 * v1 + v2 * v2;
+      if (s >= 1.0 || s <= 0.0) continue;
+      double t = thp * (-orc_log(a[0]) - v1 * v1 * orc_log(s) / s);
+      double lhs = orc_log(a[1]);
+      double rhs = 0.5 * orc_log(1.0 + 0.5 * t) + orc_log(1.0 + t) - 1.25 * t;
+      if (lhs < rhs) {
+        umag = sqrt(t * (t + 2.0));
+        ok = 1;
+        break;
+      }
+    }
+  } else if (theta >= 0.4) {
+    for (; draw < ORC_MAX_TRIES; draw++) {
+      double a[2], b[2];
+      orc_uniform2(seed, pid, 2 * draw, 1, a);
+      orc_uniform2(seed, pid, 2 * draw + 1, 1, b);
+      double x123 = a[0] * a[1] * b[0];
+      double uu = -theta * orc_log(x123);
+      double eta = -theta * orc_log(x123 * b[1]);
+      if (eta * eta - uu * uu > 1.0) {
+        umag = uu;
+        ok = 1;
+        break;
+      }
+    }
+  } else {
+    ok = 1; /* cold */
+  }
+  double dir[3] = {0.0, 0.0, 0.0};
+  if (umag > 0.0) {
+    for (uint32_t d2 = 0; d2 < ORC_MAX_TRIES; d2++) {
+      double b[2];
+      orc_uniform2(seed, pid, d2, 2, b);
+      double v1 = 2.0 * b[0] - 1.0, v2 = 2.0 * b[1] - 1.0;
+      double s = v1 * v1 + v2 * v2;
+      if (s >= 1.0) continue;
+      double r = 2.0 * sqrt(1.0 - s);
+      dir[0] = 1.0 - 2.0 * s;
+      dir[1] = v1 * r;
+      dir[2] = v2 * r;
+      break;
+    }
+  }
+  u[0] = umag * dir[0];
+  u[1] = umag * dir[1];
+  u[2] = umag * dir[2];
+  if (beta_x != 0.0) {
+    double g = sqrt(1.0 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    double c[2];
+    orc_uniform2(seed, pid, 0, 3, c);
+    if (-beta_x * u[0] / g > c[0]) u[0] = -u[0];
+    double G = 1.0 / sqrt(1.0 - beta_x * beta_x);
+    u[0] = G * (u[0] + beta_x * g);
+  }
+  return ok;
+}
+
+static int is_in_bulk(const orc_mesh *m, uint32_t c) {
+  uint32_t c1 = c % (uint32_t)m->dims[0];
+  uint32_t c2 = (c / (uint32_t)m->dims[0]) % (uint32_t)m->dims[1];
+  uint32_t c3 = c / ((uint32_t)m->dims[0] * (uint32_t)m->dims[1]);
+  return ((int)c1 >= m->guard[0] && (int)c1 < m->dims[0] - m->guard[0]) &&
+         ((int)c2 >= m->guard[1] && (int)c2 < m->dims[1] - m->guard[1]) &&
+         ((int)c3 >= m->guard[2] && (int)c3 < m->dims[2] - m->guard[2]);
+}
+
+/* Uniform loading, pattern of fill_particles (cuda/sim_data.cu:124-161):
+ * for every bulk cell, ppc co-located e-/e+ pairs, electron first.  Index
+ * = 2*(bulk_rank*ppc + n) + species; id = index.  Positions uniform in the
+ * cell (Philox purpose 0, keyed on the pair), momenta drifting MJ. */
+uint64_t orc_load_uniform(const orc_mesh *m, orc_ptc *p, int ppc,
+                          double theta_e, double theta_p, double drift_e,
+                          double drift_p, double weight, uint64_t seed) {
+  uint64_t n = 0;
+  int d = m->dim;
+  for (int k = m->guard[2]; k < m->dims[2] - m->guard[2]; k++)
+    for (int j = m->guard[1]; j < m->dims[1] - m->guard[1]; j++)
+      for (int i = m->guard[0]; i < m->dims[0] - m->guard[0]; i++) {
+        uint32_t cell = (uint32_t)(i + j * m->dims[0] +
+                                   k * m->dims[0] * m->dims[1]);
+        for (int q = 0; q < ppc; q++) {
+          uint64_t pair = n / 2;
+          double a[2], b[2];
+          orc_uniform2(seed, pair, 0, 0, a);
+          orc_uniform2(seed, pair, 1, 0, b);
+          double xs[3] = {1.0 - a[0], d > 1 ? 1.0 - a[1] : 0.0,
+                          d > 2 ? 1.0 - b[0] : 0.0};
+          for (int s = 0; s < 2; s++) {
+            uint64_t idx = n + s;
+            double u[3];
+            orc_sample_mj(s == 0 ? theta_e : theta_p, s == 0 ? drift_e : drift_p,
+                          seed, idx, u);
+            p->cell[idx] = cell;
+            p->x1[idx] = xs[0];
+            p->x2[idx] = xs[1];
+            p->x3[idx] = xs[2];
+            p->p1[idx] = u[0];
+            p->p2[idx] = u[1];
+            p->p3[idx] = u[2];
+            p->E[idx] = sqrt(1.0 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            p->w[idx] = weight;
+            p->flag[idx] = ((uint32_t)s) << 29;
+            p->id[idx] = idx;
+          }
+          n += 2;
+        }
+      }
+  p->number = n;
+  return n;
+}
+
+/* ------------------------------------------------------------------ */
+/* Interpolation: spline_t<1> (interpolation.cuh:28-37) and interp_cell */
+/* (interpolation.cuh:79-87).                                            */
+/* ------------------------------------------------------------------ */
+static inline double cic(double dx) {
+  double a = 1.0 - fabs(dx);
+  return a > 0.0 ? a : 0.0;
+}
+
+static inline double interp_cell(double rel, int c, int t, int stagger) {
+  double x = ((double)t + (stagger == 0 ? 0.5 : 1.0)) - (rel + (double)c);
+  return cic(x);
+}
+
+/* Interpolator1D (interpolation.cuh:182-209) */
+static double gather1d(const double *f, double x1, int c1, int st) {
+  double r = 0.0;
+  for (int i = 0; i <= 2; i++) {
+    int ii = i + c1 - 1;
+    r += f[ii] * interp_cell(x1, c1, ii, st & 1);
+  }
+  return r;
+}
+
+/* Interpolator2D (interpolation.cuh:146-180) */
+static double gather2d(const double *f, int d0, double x1, double x2, int c1,
+                       int c2, int st) {
+  double r = 0.0;
+  for (int j = 0; j <= 2; j++) {
+    int jj = j + c2 - 1;
+    for (int i = 0; i <= 2; i++) {
+      int ii = i + c1 - 1;
+      r += f[ii + jj * d0] * interp_cell(x1, c1, ii, st & 1) *
+           interp_cell(x2, c2, jj, (st >> 1) & 1);
+    }
+  }
+  return r;
+}
+
+/* Interpolator3D (interpolation.cuh:91-144) */
+static double gather3d(const double *f, int d0, int d1, double x1, double x2,
+                       double x3, int c1, int c2, int c3, int st) {
+  double r = 0.0;
+  for (int k = -1; k <= 1; k++) {
+    double w3 = interp_cell(x3, c3, k + c3, (st >> 2) & 1);
+    for (int j = -1; j <= 1; j++) {
+      double w2 = interp_cell(x2, c2, j + c2, (st >> 1) & 1);
+      for (int i = -1; i <= 1; i++) {
+        size_t o = (size_t)(i + c1) + (size_t)(j + c2) * d0 +
+                   (size_t)(k + c3) * d0 * d1;
+        r += f[o] * interp_cell(x1, c1, i + c1, st & 1) * w2 * w3;
+      }
+    }
+  }
+  return r;
+}
+
+/* vay_push (vay_push.cuh:8-30); E, B already scaled by dt q/(2m). */
+static void vay_push(double *p1, double *p2, double *p3, double *gamma,
+                     double E1, double E2, double E3, double B1, double B2,
+                     double B3) {
+  double g = *gamma;
+  double up1 = *p1 + 2.0 * E1 + (*p2 * B3 - *p3 * B2) / g;
+  double up2 = *p2 + 2.0 * E2 + (*p3 * B1 - *p1 * B3) / g;
+  double up3 = *p3 + 2.0 * E3 + (*p1 * B2 - *p2 * B1) / g;
+  double tt = B1 * B1 + B2 * B2 + B3 * B3;
+  double ut = up1 * B1 + up2 * B2 + up3 * B3;
+  double sigma = 1.0 + up1 * up1 + up2 * up2 + up3 * up3 - tt;
+  double inv_gamma2 =
+      2.0 / (sigma + sqrt(sigma * sigma + 4.0 * (tt + ut * ut)));
+  double s = 1.0 / (1.0 + inv_gamma2 * tt);
+  g = 1.0 / sqrt(inv_gamma2);
+  *p1 = (up1 + B1 * ut * inv_gamma2 + (up2 * B3 - up3 * B2) / g) * s;
+  *p2 = (up2 + B2 * ut * inv_gamma2 + (up3 * B1 - up1 * B3) / g) * s;
+  *p3 = (up3 + B3 * ut * inv_gamma2 + (up1 * B2 - up2 * B1) / g) * s;
+  *gamma = g;
+}
+
+/* Stagger codes, fields_impl.hpp:85-94 */
+#define ST_E1 0x6
+#define ST_E2 0x5
+#define ST_E3 0x3
+#define ST_B1 0x1
+#define ST_B2 0x2
+#define ST_B3 0x4
+
+#define FLAG_IGNORE_CURRENT (1u << 3)
+#define FLAG_IGNORE_EM (1u << 4)
+
+/* Exact-order helpers, ptc_updater_helper.cuh:62-77 (1/6 exact, see the
+ * deviation note in the design: the reference writes 0.1666667f). */
+static inline double center2d(double sx0, double sx1, double sy0,
+                              double sy1) {
+  return (2.0 * sx1 * sy1 + sx0 * sy1 + sx1 * sy0 + 2.0 * sx0 * sy0) *
+         (1.0 / 6.0);
+}
+static inline double movement3d(double sx0, double sx1, double sy0,
+                                double sy1, double sz0, double sz1) {
+  return (sz1 - sz0) * center2d(sx0, sx1, sy0, sy1);
+}
+static inline double movement2d(double sx0, double sx1, double sy0,
+                                double sy1) {
+  return (sy1 - sy0) * 0.5 * (sx0 + sx1);
+}
+
+/* periodic wrap of an index in one dimension (replaces the fold-add of
+ * send_add_guard_cells, cuda/sim_environment.cpp:72-127, and the particle
+ * self-exchange of send_particles, sim_environment_impl.hpp:319-412) */
+static inline int wrapi(const orc_mesh *m, int d, int i) {
+  if (!m->periodic[d]) return i;
+  int g = m->guard[d], N = m->dims[d] - 2 * m->guard[d];
+  if (i < g) return i + N;
+  if (i >= N + g) return i - N;
+  return i;
+}
+
+static inline void acc_add(int64_t *acc[2], size_t o, double c, int H) {
+  int64_t hi, lo;
+  orc_fx_split(c, H, &hi, &lo);
+  acc[0][o] += hi;
+  acc[1][o] += lo;
+}
+
+static inline size_t idx3(const orc_mesh *m, int i, int j, int k) {
+  return (size_t)wrapi(m, 0, i) +
+         (size_t)wrapi(m, 1, j) * m->dims[0] +
+         (size_t)wrapi(m, 2, k) * m->dims[0] * m->dims[1];
+}
+
+/* One particle: push (ptc_push_cart_{1,2,3}d, ptc_updater.cu:21-218)
+ * followed by move+deposit (deposit_current_cart_{1,2,3}d,
+ * ptc_updater.cu:220-499) and the periodic wrap. Returns 0 if skipped. */
+static void push_one(const orc_mesh *m, orc_fields *f, orc_ptc *p,
+                     const orc_species *sp, double dt, size_t idx) {
+  uint32_t c = p->cell[idx];
+  if (c == ORC_MAX_CELL) return;
+  if (!is_in_bulk(m, c)) {
+    p->cell[idx] = ORC_MAX_CELL;
+    return;
+  }
+  const int d0 = m->dims[0], d1 = m->dims[1];
+  int c1 = (int)(c % (uint32_t)d0);
+  int c2 = (int)((c / (uint32_t)d0) % (uint32_t)d1);
+  int c3 = (int)(c / ((uint32_t)d0 * (uint32_t)d1));
+  uint32_t flag = p->flag[idx];
+  int s = (int)(flag >> 29);
+  double x1 = p->x1[idx], x2 = p->x2[idx], x3 = p->x3[idx];
+  double p1 = p->p1[idx], p2 = p->p2[idx], p3 = p->p3[idx];
+  double gamma = p->E[idx];
+  double q_over_m = dt * 0.5 * sp->q[s] / sp->m[s];
+  gamma = sqrt(1.0 + p1 * p1 + p2 * p2 + p3 * p3);
+  if (!(flag & FLAG_IGNORE_EM)) {
+    double E1, E2, E3, B1, B2, B3;
+    if (m->dim == 1) {
+      E1 = gather1d(f->E[0], x1, (int)c, ST_E1) * q_over_m;
+      E2 = gather1d(f->E[1], x1, (int)c, ST_E2) * q_over_m;
+      E3 = gather1d(f->E[2], x1, (int)c, ST_E3) * q_over_m;
+      B1 = gather1d(f->B[0], x1, (int)c, ST_B1) * q_over_m;
+      B2 = gather1d(f->B[1], x1, (int)c, ST_B2) * q_over_m;
+      B3 = gather1d(f->B[2], x1, (int)c, ST_B3) * q_over_m;
+    } else if (m->dim == 2) {
+      E1 = gather2d(f->E[0], d0, x1, x2, c1, c2, ST_E1) * q_over_m;
+      E2 = gather2d(f->E[1], d0, x1, x2, c1, c2, ST_E2) * q_over_m;
+      E3 = gather2d(f->E[2], d0, x1, x2, c1, c2, ST_E3) * q_over_m;
+      B1 = gather2d(f->B[0], d0, x1, x2, c1, c2, ST_B1) * q_over_m;
+      B2 = gather2d(f->B[1], d0, x1, x2, c1, c2, ST_B2) * q_over_m;
+      B3 = gather2d(f->B[2], d0, x1, x2, c1, c2, ST_B3) * q_over_m;
+    } else {
+      E1 = gather3d(f->E[0], d0, d1, x1, x2, x3, c1, c2, c3, ST_E1) * q_over_m;
+      E2 = gather3d(f->E[1], d0, d1, x1, x2, x3, c1, c2, c3, ST_E2) * q_over_m;
+      E3 = gather3d(f->E[2], d0, d1, x1, x2, x3, c1, c2, c3, ST_E3) * q_over_m;
+      B1 = gather3d(f->B[0], d0, d1, x1, x2, x3, c1, c2, c3, ST_B1) * q_over_m;
+      B2 = gather3d(f->B[1], d0, d1, x1, x2, x3, c1, c2, c3, ST_B2) * q_over_m;
+      B3 = gather3d(f->B[2], d0, d1, x1, x2, x3, c1, c2, c3, ST_B3) * q_over_m;
+    }
+    vay_push(&p1, &p2, &p3, &gamma, E1, E2, E3, B1, B2, B3);
+    p->p1[idx] = p1;
+    p->p2[idx] = p2;
+    p->p3[idx] = p3;
+    p->E[idx] = gamma;
+  }
+
+  /* ---- move + deposit: reads the stored momenta and gamma ---- */
+  double v1 = p->p1[idx], v2 = p->p2[idx], v3 = p->p3[idx];
+  gamma = p->E[idx];
+  double w = p->w[idx];
+  double old_x1 = x1, old_x2 = x2, old_x3 = x3;
+  v1 = v1 / gamma;
+  v2 = v2 / gamma;
+  v3 = v3 / gamma;
+  const int H = sp->jexp;
+  int rho = sp->deposit_rho;
+
+  if (m->dim == 1) {
+    double new_x1 = old_x1 + (v1 * dt) / m->delta[0];
+    int dc1 = (int)floor(new_x1);
+    new_x1 -= (double)dc1;
+    int nc = wrapi(m, 0, (int)c + dc1);
+    p->cell[idx] = (uint32_t)nc;
+    p->x1[idx] = new_x1;
+    p->x2[idx] = old_x2 + v2 * dt;
+    p->x3[idx] = old_x3 + v3 * dt;
+    if (flag & FLAG_IGNORE_CURRENT) return;
+    double weight = sp->q[s] * w;
+    int i0 = (dc1 == -1 ? -2 : -1);
+    int i1 = (dc1 == 1 ? 1 : 0);
+    double djx = 0.0;
+    for (int i = i0; i <= i1; i++) {
+      double sx0 = cic(-old_x1 + i + 1);
+      double sx1 = cic(-new_x1 + (i + 1 - dc1));
+      djx += sx1 - sx0;
+      acc_add(f->Jacc[0], idx3(m, i + (int)c + 1, 0, 0), -weight * djx, H);
+      double val1 = 0.5 * (sx0 + sx1);
+      acc_add(f->Jacc[1], idx3(m, i + (int)c, 0, 0), weight * v2 * val1, H);
+      acc_add(f->Jacc[2], idx3(m, i + (int)c, 0, 0), weight * v3 * val1, H);
+      if (rho)
+        acc_add(f->Racc[s], idx3(m, i + (int)c, 0, 0), weight * sx1, H);
+    }
+  } else if (m->dim == 2) {
+    double new_x1 = old_x1 + (v1 * dt) * m->inv_delta[0];
+    double new_x2 = old_x2 + (v2 * dt) * m->inv_delta[1];
+    int dc1 = (int)floor(new_x1);
+    int dc2 = (int)floor(new_x2);
+    new_x1 -= (double)dc1;
+    new_x2 -= (double)dc2;
+    int n1 = wrapi(m, 0, c1 + dc1), n2 = wrapi(m, 1, c2 + dc2);
+    p->cell[idx] = (uint32_t)(n1 + n2 * d0);
+    p->x1[idx] = new_x1;
+    p->x2[idx] = new_x2;
+    p->x3[idx] = old_x3 + v3 * dt;
+    if (flag & FLAG_IGNORE_CURRENT) return;
+    double weight = -sp->q[s] * w;
+    int j0 = (dc2 == -1 ? -2 : -1), j1 = (dc2 == 1 ? 1 : 0);
+    int i0 = (dc1 == -1 ? -2 : -1), i1 = (dc1 == 1 ? 1 : 0);
+    double djy[3] = {0.0, 0.0, 0.0};
+    for (int j = j0; j <= j1; j++) {
+      double sy0 = cic(-old_x2 + j + 1);
+      double sy1 = cic(-new_x2 + (j + 1 - dc2));
+      double djx = 0.0;
+      for (int i = i0; i <= i1; i++) {
+        double sx0 = cic(-old_x1 + i + 1);
+        double sx1 = cic(-new_x1 + (i + 1 - dc1));
+        djx += movement2d(sy0, sy1, sx0, sx1);
+        acc_add(f->Jacc[0], idx3(m, i + c1 + 1, j + c2, 0), weight * djx, H);
+        djy[i - i0] += movement2d(sx0, sx1, sy0, sy1);
+        acc_add(f->Jacc[1], idx3(m, i + c1, j + c2 + 1, 0),
+                weight * djy[i - i0], H);
+        double val2 = center2d(sx0, sx1, sy0, sy1);
+        acc_add(f->Jacc[2], idx3(m, i + c1, j + c2, 0), -weight * v3 * val2,
+                H);
+        if (rho) {
+          double s1 = sx1 * sy1;
+          acc_add(f->Racc[s], idx3(m, i + c1, j + c2, 0), -weight * s1, H);
+        }
+      }
+    }
+  } else {
+    double new_x1 = old_x1 + (v1 * dt) * m->inv_delta[0];
+    double new_x2 = old_x2 + (v2 * dt) * m->inv_delta[1];
+    double new_x3 = old_x3 + (v3 * dt) * m->inv_delta[2];
+    int dc1 = (int)floor(new_x1);
+    int dc2 = (int)floor(new_x2);
+    int dc3 = (int)floor(new_x3);
+    new_x1 -= (double)dc1;
+    new_x2 -= (double)dc2;
+    new_x3 -= (double)dc3;
+    int n1 = wrapi(m, 0, c1 + dc1), n2 = wrapi(m, 1, c2 + dc2),
+        n3 = wrapi(m, 2, c3 + dc3);
+    p->cell[idx] = (uint32_t)(n1 + n2 * d0 + n3 * d0 * d1);
+    p->x1[idx] = new_x1;
+    p->x2[idx] = new_x2;
+    p->x3[idx] = new_x3;
+    if (flag & FLAG_IGNORE_CURRENT) return;
+    double weight = -sp->q[s] * w;
+    int k0 = (dc3 == -1 ? -2 : -1), k1 = (dc3 == 1 ? 1 : 0);
+    int j0 = (dc2 == -1 ? -2 : -1), j1 = (dc2 == 1 ? 1 : 0);
+    int i0 = (dc1 == -1 ? -2 : -1), i1 = (dc1 == 1 ? 1 : 0);
+    double djz[3][3] = {{0.0}};
+    for (int k = k0; k <= k1; k++) {
+      double sz0 = cic(-old_x3 + k + 1);
+      double sz1 = cic(-new_x3 + (k + 1 - dc3));
+      double djy[3] = {0.0, 0.0, 0.0};
+      for (int j = j0; j <= j1; j++) {
+        double sy0 = cic(-old_x2 + j + 1);
+        double sy1 = cic(-new_x2 + (j + 1 - dc2));
+        double djx = 0.0;
+        for (int i = i0; i <= i1; i++) {
+          double sx0 = cic(-old_x1 + i + 1);
+          double sx1 = cic(-new_x1 + (i + 1 - dc1));
+          djx += movement3d(sy0, sy1, sz0, sz1, sx0, sx1);
+          acc_add(f->Jacc[0], idx3(m, i + c1 + 1, j + c2, k + c3),
+                  weight * djx, H);
+          djy[i - i0] += movement3d(sx0, sx1, sz0, sz1, sy0, sy1);
+          acc_add(f->Jacc[1], idx3(m, i + c1, j + c2 + 1, k + c3),
+                  weight * djy[i - i0], H);
+          djz[j - j0][i - i0] += movement3d(sx0, sx1, sy0, sy1, sz0, sz1);
+          acc_add(f->Jacc[2], idx3(m, i + c1, j + c2, k + c3 + 1),
+                  weight * djz[j - j0][i - i0], H);
+          if (rho) {
+            double s1 = sx1 * sy1 * sz1;
+            acc_add(f->Racc[s], idx3(m, i + c1, j + c2, k + c3),
+                    -weight * s1, H);
+          }
+        }
+      }
+    }
+  }
+}
+
+static size_t ncells(const orc_mesh *m) {
+  return (size_t)m->dims[0] * m->dims[1] * m->dims[2];
+}
+
+/* ptc_updater::update_particles (ptc_updater.cu:748-841), without the
+ * photon branch and smoothing (current_smoothing = 0 by default,
+ * sim_params.h:42): zero J (and Rho on deposit steps), push+deposit every
+ * particle, then materialise J and Rho from the accumulators. */
+void orc_ptc_update(const orc_mesh *m, orc_fields *f, orc_ptc *p,
+                    const orc_species *sp, double dt, uint32_t step) {
+  (void)step;
+  size_t n = ncells(m);
+  for (int c = 0; c < 3; c++) {
+    memset(f->Jacc[c][0], 0, n * sizeof(int64_t));
+    memset(f->Jacc[c][1], 0, n * sizeof(int64_t));
+  }
+  if (sp->deposit_rho)
+    for (int s = 0; s < sp->num_species; s++) {
+      memset(f->Racc[s][0], 0, n * sizeof(int64_t));
+      memset(f->Racc[s][1], 0, n * sizeof(int64_t));
+    }
+  for (uint64_t i = 0; i < p->number; i++) push_one(m, f, p, sp, dt, i);
+  for (int c = 0; c < 3; c++)
+    for (size_t o = 0; o < n; o++)
+      f->J[c][o] = orc_fx_value(f->Jacc[c][0][o], f->Jacc[c][1][o], sp->jexp);
+  if (sp->deposit_rho)
+    for (int s = 0; s < sp->num_species; s++)
+      for (size_t o = 0; o < n; o++)
+        f->Rho[s][o] =
+            orc_fx_value(f->Racc[s][0][o], f->Racc[s][1][o], sp->jexp);
+}
+
+/* Periodic guard copy: send_guard_cells (cuda/sim_environment.cpp:10-69,
+ * sim_environment_impl.hpp:287-294) on one rank with periodic boundaries
+ * makes every guard cell a copy of the wrapped bulk cell. */
+void orc_guard_copy(const orc_mesh *m, double *a) {
+  for (int k = 0; k < m->dims[2]; k++)
+    for (int j = 0; j < m->dims[1]; j++)
+      for (int i = 0; i < m->dims[0]; i++) {
+        int wi = wrapi(m, 0, i), wj = wrapi(m, 1, j), wk = wrapi(m, 2, k);
+        if (wi == i && wj == j && wk == k) continue;
+        a[i + (size_t)j * m->dims[0] + (size_t)k * m->dims[0] * m->dims[1]] =
+            a[wi + (size_t)wj * m->dims[0] +
+              (size_t)wk * m->dims[0] * m->dims[1]];
+      }
+}
+
+#define F(a, i, j, k) \
+  (a)[(size_t)(i) + (size_t)(j) * d0 + (size_t)(k) * d0 * d1]
+
+/* compute_e_update{1,2,3}d (field_solver.cu:14-131); bulk cells only. */
+void orc_update_e(const orc_mesh *m, orc_fields *f, double dt) {
+  const int d0 = m->dims[0], d1 = m->dims[1];
+  double *e1 = f->E[0], *e2 = f->E[1], *e3 = f->E[2];
+  double *b1 = f->B[0], *b2 = f->B[1], *b3 = f->B[2];
+  double *g1 = f->Bbg[0], *g2 = f->Bbg[1], *g3 = f->Bbg[2];
+  double *j1 = f->J[0], *j2 = f->J[1], *j3 = f->J[2];
+  const double i0 = m->inv_delta[0], i1 = m->inv_delta[1],
+               i2 = m->inv_delta[2];
+  for (int k = m->guard[2]; k < m->dims[2] - m->guard[2]; k++)
+    for (int j = m->guard[1]; j < m->dims[1] - m->guard[1]; j++)
+      for (int i = m->guard[0]; i < m->dims[0] - m->guard[0]; i++) {
+        size_t o = (size_t)i + (size_t)j * d0 + (size_t)k * d0 * d1;
+        if (m->dim == 1) {
+          e1[o] -= dt * j1[o];
+          e2[o] += dt * ((F(b3, i, j, k) - F(b3, i + 1, j, k) -
+                          F(g3, i, j, k) + F(g3, i + 1, j, k)) *
+                             i0 -
+                         j2[o]);
+          e3[o] += dt * ((F(b2, i + 1, j, k) - F(b2, i, j, k) -
+                          F(g2, i + 1, j, k) + F(g2, i, j, k)) *
+                             i0 -
+                         j3[o]);
+        } else if (m->dim == 2) {
+          e1[o] += dt * ((F(b3, i, j + 1, k) - F(b3, i, j, k) -
+                          F(g3, i, j + 1, k) + F(g3, i, j, k)) *
+                             i1 -
+                         j1[o]);
+          e2[o] += dt * ((F(b3, i, j, k) - F(b3, i + 1, j, k) -
+                          F(g3, i, j, k) + F(g3, i + 1, j, k)) *
+                             i0 -
+                         j2[o]);
+          e3[o] += dt * ((F(b2, i + 1, j, k) - F(b2, i, j, k) -
+                          F(g2, i + 1, j, k) + F(g2, i, j, k)) *
+                             i0 +
+                         (F(b1, i, j, k) - F(b1, i, j + 1, k) -
+                          F(g1, i, j, k) + F(g1, i, j + 1, k)) *
+                             i1 -
+                         j3[o]);
+        } else {
+          e1[o] += dt * ((F(b3, i, j + 1, k) - F(b3, i, j, k) -
+                          F(g3, i, j + 1, k) + F(g3, i, j, k)) *
+                             i1 +
+                         (F(b2, i, j, k) - F(b2, i, j, k + 1) -
+                          F(g2, i, j, k) + F(g2, i, j, k + 1)) *
+                             i2 -
+                         j1[o]);
+          e2[o] += dt * ((F(b3, i, j, k) - F(b3, i + 1, j, k) -
+                          F(g3, i, j, k) + F(g3, i + 1, j, k)) *
+                             i0 +
+                         (F(b1, i, j, k + 1) - F(b1, i, j, k) -
+                          F(g1, i, j, k + 1) + F(g1, i, j, k)) *
+                             i2 -
+                         j2[o]);
+          e3[o] += dt * ((F(b2, i + 1, j, k) - F(b2, i, j, k) -
+                          F(g2, i + 1, j, k) + F(g2, i, j, k)) *
+                             i0 +
+                         (F(b1, i, j, k) - F(b1, i, j + 1, k) -
+                          F(g1, i, j, k) + F(g1, i, j + 1, k)) *
+                             i1 -
+                         j3[o]);
+        }
+      }
+}
+
+/* compute_b_update{1,2,3}d (field_solver.cu:133-235).  1D and 2D follow the
+ * reference expression for expression.  3D uses the staggered (backward)
+ * differences that match the field staggering, the 2D kernel and div B
+ * (field_solver.cu:176-192, 319-325); the reference 3D kernel
+ * (:214-234) differences forward, one cell off. */
+void orc_update_b(const orc_mesh *m, orc_fields *f, double dt) {
+  const int d0 = m->dims[0], d1 = m->dims[1];
+  double *e1 = f->E[0], *e2 = f->E[1], *e3 = f->E[2];
+  double *b1 = f->B[0], *b2 = f->B[1], *b3 = f->B[2];
+  double *g1 = f->Ebg[0], *g2 = f->Ebg[1], *g3 = f->Ebg[2];
+  const double i0 = m->inv_delta[0], i1 = m->inv_delta[1],
+               i2 = m->inv_delta[2];
+  for (int k = m->guard[2]; k < m->dims[2] - m->guard[2]; k++)
+    for (int j = m->guard[1]; j < m->dims[1] - m->guard[1]; j++)
+      for (int i = m->guard[0]; i < m->dims[0] - m->guard[0]; i++) {
+        size_t o = (size_t)i + (size_t)j * d0 + (size_t)k * d0 * d1;
+        if (m->dim == 1) {
+          b2[o] -= dt *
+                   (F(e3, i - 1, j, k) - F(e3, i, j, k) - F(g3, i - 1, j, k) +
+                    F(g3, i, j, k)) *
+                   i0;
+          b3[o] -= dt *
+                   (F(e2, i, j, k) - F(e2, i - 1, j, k) - F(g2, i, j, k) +
+                    F(g2, i - 1, j, k)) *
+                   i0;
+        } else if (m->dim == 2) {
+          b1[o] -= dt *
+                   (F(e3, i, j, k) - F(e3, i, j - 1, k) - F(g3, i, j, k) +
+                    F(g3, i, j - 1, k)) *
+                   i1;
+          b2[o] -= dt *
+                   (F(e3, i - 1, j, k) - F(e3, i, j, k) - F(g3, i - 1, j, k) +
+                    F(g3, i, j, k)) *
+                   i0;
+          b3[o] -= dt * ((F(e2, i, j, k) - F(e2, i - 1, j, k) -
+                          F(g2, i, j, k) + F(g2, i - 1, j, k)) *
+                             i0 +
+                         (F(e1, i, j - 1, k) - F(e1, i, j, k) -
+                          F(g1, i, j - 1, k) + F(g1, i, j, k)) *
+                             i1);
+        } else {
+          b1[o] -= dt * ((F(e3, i, j, k) - F(e3, i, j - 1, k) -
+                          F(g3, i, j, k) + F(g3, i, j - 1, k)) *
+                             i1 +
+                         (F(e2, i, j, k - 1) - F(e2, i, j, k) -
+                          F(g2, i, j, k - 1) + F(g2, i, j, k)) *
+                             i2);
+          b2[o] -= dt * ((F(e3, i - 1, j, k) - F(e3, i, j, k) -
+                          F(g3, i - 1, j, k) + F(g3, i, j, k)) *
+                             i0 +
+                         (F(e1, i, j, k) - F(e1, i, j, k - 1) -
+                          F(g1, i, j, k) + F(g1, i, j, k - 1)) *
+                             i2);
+          b3[o] -= dt * ((F(e2, i, j, k) - F(e2, i - 1, j, k) -
+                          F(g2, i, j, k) + F(g2, i - 1, j, k)) *
+                             i0 +
+                         (F(e1, i, j - 1, k) - F(e1, i, j, k) -
+                          F(g1, i, j - 1, k) + F(g1, i, j, k)) *
+                             i1);
+        }
+      }
+}
+
+/* compute_divs_{1,2,3}d (field_solver.cu:237-326) */
+void orc_compute_divs(const orc_mesh *m, orc_fields *f) {
+  const int d0 = m->dims[0], d1 = m->dims[1];
+  double *e1 = f->E[0], *e2 = f->E[1], *e3 = f->E[2];
+  double *b1 = f->B[0], *b2 = f->B[1], *b3 = f->B[2];
+  double *ge1 = f->Ebg[0], *ge2 = f->Ebg[1], *ge3 = f->Ebg[2];
+  double *gb1 = f->Bbg[0], *gb2 = f->Bbg[1], *gb3 = f->Bbg[2];
+  const double i0 = m->inv_delta[0], i1 = m->inv_delta[1],
+               i2 = m->inv_delta[2];
+  for (int k = m->guard[2]; k < m->dims[2] - m->guard[2]; k++)
+    for (int j = m->guard[1]; j < m->dims[1] - m->guard[1]; j++)
+      for (int i = m->guard[0]; i < m->dims[0] - m->guard[0]; i++) {
+        size_t o = (size_t)i + (size_t)j * d0 + (size_t)k * d0 * d1;
+        double dE = (F(e1, i + 1, j, k) - F(ge1, i + 1, j, k) -
+                     F(e1, i, j, k) + F(ge1, i, j, k)) *
+                    i0;
+        double dB = (F(b1, i, j, k) - F(gb1, i, j, k) - F(b1, i - 1, j, k) +
+                     F(gb1, i - 1, j, k)) *
+                    i0;
+        if (m->dim >= 2) {
+          dE = dE + (F(e2, i, j + 1, k) - F(ge2, i, j + 1, k) -
+                     F(e2, i, j, k) + F(ge2, i, j, k)) *
+                        i1;
+          dB = dB + (F(b2, i, j, k) - F(gb2, i, j, k) - F(b2, i, j - 1, k) +
+                     F(gb2, i, j - 1, k)) *
+                        i1;
+        }
+        if (m->dim == 3) {
+          dE = dE + (F(e3, i, j, k + 1) - F(ge3, i, j, k + 1) -
+                     F(e3, i, j, k) + F(ge3, i, j, k)) *
+                        i2;
+          dB = dB + (F(b3, i, j, k) - F(gb3, i, j, k) - F(b3, i, j, k - 1) +
+                     F(gb3, i, j, k - 1)) *
+                        i2;
+        }
+        f->divE[o] = dE;
+        f->divB[o] = dB;
+      }
+}
+
+/* field_solver::update_fields (field_solver.cu:334-358): at step 0 an E
+ * half step first, then B, guard, E, guard, divergences. */
+void orc_field_update(const orc_mesh *m, orc_fields *f, double dt,
+                      uint32_t step) {
+  if (step == 0) {
+    orc_update_e(m, f, 0.5 * dt);
+    for (int c = 0; c < 3; c++) orc_guard_copy(m, f->E[c]);
+  }
+  orc_update_b(m, f, dt);
+  for (int c = 0; c < 3; c++) orc_guard_copy(m, f->B[c]);
+  orc_update_e(m, f, dt);
+  for (int c = 0; c < 3; c++) orc_guard_copy(m, f->E[c]);
+  orc_compute_divs(m, f);
+}
+
+/* filter_current_cart_{1,2,3}d (ptc_updater.cu:501-625) for periodic
+ * boundaries; the guard ring of j is refreshed first.  fix_3d_weight=0
+ * keeps the reference's 0.03152f edge weight (ptc_updater.cu:610-615). */
+void orc_filter_current(const orc_mesh *m, double *j, double *tmp,
+                        int fix_3d_weight) {
+  const int d0 = m->dims[0], d1 = m->dims[1];
+  orc_guard_copy(m, j);
+  const double we = fix_3d_weight ? 0.03125 : (double)0.03152f;
+  for (int k = m->guard[2]; k < m->dims[2] - m->guard[2]; k++)
+    for (int jj = m->guard[1]; jj < m->dims[1] - m->guard[1]; jj++)
+      for (int i = m->guard[0]; i < m->dims[0] - m->guard[0]; i++) {
+        size_t o = (size_t)i + (size_t)jj * d0 + (size_t)k * d0 * d1;
+        double r;
+        if (m->dim == 1) {
+          r = 0.5 * j[o];
+          r += 0.25 * F(j, i + 1, jj, k);
+          r += 0.25 * F(j, i - 1, jj, k);
+        } else if (m->dim == 2) {
+          r = 0.25 * j[o];
+          r += 0.125 * F(j, i + 1, jj, k);
+          r += 0.125 * F(j, i - 1, jj, k);
+          r += 0.125 * F(j, i, jj + 1, k);
+          r += 0.125 * F(j, i, jj - 1, k);
+          r += 0.0625 * F(j, i + 1, jj + 1, k);
+          r += 0.0625 * F(j, i - 1, jj + 1, k);
+          r += 0.0625 * F(j, i + 1, jj - 1, k);
+          r += 0.0625 * F(j, i - 1, jj - 1, k);
+        } else {
+          r = 0.125 * j[o];
+          r += 0.0625 * F(j, i + 1, jj, k);
+          r += 0.0625 * F(j, i - 1, jj, k);
+          r += 0.0625 * F(j, i, jj + 1, k);
+          r += 0.0625 * F(j, i, jj - 1, k);
+          r += 0.03125 * F(j, i + 1, jj + 1, k);
+          r += 0.03125 * F(j, i - 1, jj + 1, k);
+          r += 0.03125 * F(j, i + 1, jj - 1, k);
+          r += 0.03125 * F(j, i - 1, jj - 1, k);
+          for (int s = 0; s < 2; s++) {
+            int kk = s == 0 ? k + 1 : k - 1;
+            r += 0.0625 * F(j, i, jj, kk);
+            r += we * F(j, i + 1, jj, kk);
+            r += we * F(j, i - 1, jj, kk);
+            r += we * F(j, i, jj + 1, kk);
+            r += we * F(j, i, jj - 1, kk);
+            r += 0.015625 * F(j, i + 1, jj + 1, kk);
+            r += 0.015625 * F(j, i - 1, jj + 1, kk);
+            r += 0.015625 * F(j, i + 1, jj - 1, kk);
+            r += 0.015625 * F(j, i - 1, jj - 1, kk);
+          }
+        }
+        tmp[o] = r;
+      }
+  for (int k = m->guard[2]; k < m->dims[2] - m->guard[2]; k++)
+    for (int jj = m->guard[1]; jj < m->dims[1] - m->guard[1]; jj++)
+      for (int i = m->guard[0]; i < m->dims[0] - m->guard[0]; i++) {
+        size_t o = (size_t)i + (size_t)jj * d0 + (size_t)k * d0 * d1;
+        j[o] = tmp[o];
+      }
+  orc_guard_copy(m, j);
+}
+#undef F
+
+/* erase_ptc_in_guard_cells (kernels.cu:25-34) */
+void orc_clear_guard_cells(const orc_mesh *m, orc_ptc *p) {
+  for (uint64_t i = 0; i < p->number; i++)
+    if (p->cell[i] != ORC_MAX_CELL && !is_in_bulk(m, p->cell[i]))
+      p->cell[i] = ORC_MAX_CELL;
+}
+
+/* Stable counting sort by cell, empties last, number := #non-empty
+ * (particle_base_impl.hpp:250-305).  The permutation is applied out of
+ * place through `scratch` (number uint64) rather than by cycle walking
+ * (:307-332); the resulting order is the same. */
+uint64_t orc_sort_by_cell(const orc_mesh *m, orc_ptc *p, uint64_t *scratch) {
+  size_t nc = ncells(m);
+  uint64_t n = p->number;
+  uint64_t *part = (uint64_t *)calloc(nc + 2, sizeof(uint64_t));
+  for (uint64_t i = 0; i < n; i++) {
+    size_t c = p->cell[i] == ORC_MAX_CELL ? nc : p->cell[i];
+    scratch[i] = part[c + 1];
+    part[c + 1] += 1;
+  }
+  for (size_t i = 1; i < nc + 2; i++) part[i] += part[i - 1];
+  for (uint64_t i = 0; i < n; i++) {
+    size_t c = p->cell[i] == ORC_MAX_CELL ? nc : p->cell[i];
+    scratch[i] += part[c];
+  }
+#define PERMUTE(T, a)                                           \
+  do {                                                          \
+    T *t = (T *)malloc(n * sizeof(T));                          \
+    for (uint64_t i = 0; i < n; i++) t[scratch[i]] = (a)[i];    \
+    memcpy((a), t, n * sizeof(T));                              \
+    free(t);                                                    \
+  } while (0)
+  PERMUTE(uint32_t, p->cell);
+  PERMUTE(double, p->x1);
+  PERMUTE(double, p->x2);
+  PERMUTE(double, p->x3);
+  PERMUTE(double, p->p1);
+  PERMUTE(double, p->p2);
+  PERMUTE(double, p->p3);
+  PERMUTE(double, p->E);
+  PERMUTE(double, p->w);
+  PERMUTE(uint32_t, p->flag);
+  PERMUTE(uint64_t, p->id);
+#undef PERMUTE
+  uint64_t kept = part[nc];
+  free(part);
+  p->number = kept;
+  return kept;
+}
+
+/* compute_energy_histogram (kernels.cu:36-69): bins of ln(gamma) of width
+ * ln(E_max)/nbins, clamped; flag_bit < 0 counts every slot below number. */
+void orc_energy_histogram(const orc_ptc *p, uint32_t *hist, int nbins,
+                          double E_max, int flag_bit) {
+  double dlogE = orc_log(E_max) / (double)nbins;
+  for (uint64_t i = 0; i < p->number; i++) {
+    if (flag_bit >= 0 && !(p->flag[i] & (1u << flag_bit))) continue;
+    double logE = orc_log(p->E[i]);
+    int idx = (int)floor(logE / dlogE);
+    if (idx < 0) idx = 0;
+    if (idx >= nbins) idx = nbins - 1;
+    hist[idx] += 1;
+  }
+}

@@ -1,0 +1,109 @@
+/*
+ * pic_oracle.h -- CPU FP64 restatement of Aperture's Cartesian PIC time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * product: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it.  Nothing in aperture3_amd/ links, loads or calls it.
+ *
+ * Every function cites the reference file:line whose behaviour it restates
+ * (paths relative to the fizban007/Aperture3 source tree).  The reference
+ * has no CPU implementation of this path (its push, deposit and solvers are
+ * CUDA-only), so this is a restatement of the CUDA kernels in plain C, with
+ * two documented deviations that make it deterministic:
+ *   - current/charge are accumulated in a two-limb int64 fixed point
+ *     (order independent) instead of float atomicAdd;
+ *   - periodic wrap of particles and of the current fold is done in place
+ *     instead of through MPI self-exchange.
+ */
+#ifndef PIC_ORACLE_H
+#define PIC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_CELL 0xffffffffu
+#define ORC_LIMB_BITS 40
+
+/* Quadmesh (src/core/quadmesh.h:16-402): dims include 2*guard. */
+typedef struct {
+  int dim;
+  int dims[3];
+  int guard[3];
+  int periodic[3];
+  double delta[3];
+  double inv_delta[3];
+  double lower[3];
+} orc_mesh;
+
+/* particle_data SoA (src/data/particle_data.h:50-61), all-FP64 form. */
+typedef struct {
+  uint32_t *cell;
+  double *x1, *x2, *x3;
+  double *p1, *p2, *p3;
+  double *E; /* gamma */
+  double *w;
+  uint32_t *flag;
+  uint64_t *id;
+  uint64_t number;
+} orc_ptc;
+
+/* Field arrays, each dims[0]*dims[1]*dims[2] doubles, x fastest. */
+typedef struct {
+  double *E[3], *B[3], *Ebg[3], *Bbg[3], *J[3];
+  double *Rho[4];
+  double *divE, *divB;
+  /* two int64 limbs per cell for J1..J3 and Rho[sp]; owned by caller */
+  int64_t *Jacc[3][2];
+  int64_t *Racc[4][2];
+} orc_fields;
+
+typedef struct {
+  double q[8];
+  double m[8];
+  int num_species;
+  int jexp;          /* fixed-point exponent H of the high limb */
+  int deposit_rho;   /* deposit Rho[sp] this step */
+} orc_species;
+
+/* --- counter-based RNG (Salmon et al. 2011, Philox4x32-10) --- */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
+                       uint32_t out[4]);
+double orc_log(double x);
+
+/* --- fixed point helpers --- */
+void orc_fx_split(double c, int H, int64_t *hi, int64_t *lo);
+double orc_fx_value(int64_t hi, int64_t lo, int H);
+
+/* --- loading --- */
+int orc_sample_mj(double theta, double beta_x, uint64_t seed, uint64_t pid,
+                  double u[3]);
+uint64_t orc_load_uniform(const orc_mesh *m, orc_ptc *p, int ppc,
+                          double theta_e, double theta_p, double drift_e,
+                          double drift_p, double weight, uint64_t seed);
+
+/* --- time step --- */
+void orc_ptc_update(const orc_mesh *m, orc_fields *f, orc_ptc *p,
+                    const orc_species *sp, double dt, uint32_t step);
+void orc_field_update(const orc_mesh *m, orc_fields *f, double dt,
+                      uint32_t step);
+void orc_update_e(const orc_mesh *m, orc_fields *f, double dt);
+void orc_update_b(const orc_mesh *m, orc_fields *f, double dt);
+void orc_compute_divs(const orc_mesh *m, orc_fields *f);
+void orc_guard_copy(const orc_mesh *m, double *a);
+void orc_filter_current(const orc_mesh *m, double *j, double *tmp,
+                        int fix_3d_weight);
+
+/* --- sort / compaction --- */
+void orc_clear_guard_cells(const orc_mesh *m, orc_ptc *p);
+uint64_t orc_sort_by_cell(const orc_mesh *m, orc_ptc *p, uint64_t *scratch);
+
+/* --- diagnostics --- */
+void orc_energy_histogram(const orc_ptc *p, uint32_t *hist, int nbins,
+                          double E_max, int flag_bit);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,147 @@
+"""Oracle checks of the Yee solver (field_solver.cu:14-358).
+
+Pinned by: the polynomial curl known answer of the reference's own
+finite-difference test (tests/test_finite_diff.cpp:38-62: u = (3y^2 z,
+2x z^2, xyz), curl u = (-3xz, 3y^2 - yz, 2z^2 - 6yz); centred differences are
+exact on quadratics) for both curls; the vacuum Yee dispersion relation
+through the two-step recurrence E^{n+1} + E^{n-1} = (2 - dt^2 K^2) E^n with
+K^2 = sum_i (2 sin(k_i D_i / 2) / D_i)^2; div B unchanged by the B update
+(div curl = 0 -- this is what pins the 3D B stencil); and div E changed by
+exactly -dt div J."""
+import numpy as np
+import pytest
+
+import oracle_ctypes as oc
+from cases import field_from
+
+U = (lambda x, y, z: 3.0 * y * y * z, lambda x, y, z: 2.0 * x * z * z,
+     lambda x, y, z: x * y * z)
+CURL_U = (lambda x, y, z: -3.0 * x * z, lambda x, y, z: 3.0 * y * y - y * z,
+          lambda x, y, z: 2.0 * z * z - 6.0 * y * z)
+ST_E = (0b110, 0b101, 0b011)
+ST_B = (0b001, 0b010, 0b100)
+
+
+def mk3(N=10, size=1.0):
+    o = oc.OracleSim([N, N, N], [2, 2, 2], size=(size,) * 3, periodic=(False,) * 3)
+    args = (o.dims, [2, 2, 2], [size / N] * 3, [0.0] * 3)
+    return o, args
+
+
+def bulk(o, a):
+    m = o.mesh
+    v = a.reshape(m.dims[2], m.dims[1], m.dims[0])
+    g = m.guard
+    return v[g[2]:m.dims[2] - g[2], g[1]:m.dims[1] - g[1], g[0]:m.dims[0] - g[0]]
+
+
+def test_curl_b_polynomial_3d():
+    o, args = mk3()
+    for c in range(3):
+        o.f["B"][c][:] = field_from(U[c], *args, ST_B[c], 3)
+    dt = 0.37
+    o.update_e(dt)
+    for c in range(3):
+        expect = dt * field_from(CURL_U[c], *args, ST_E[c], 3)
+        got = o.f["E"][c]
+        assert np.abs(bulk(o, got) - bulk(o, expect)).max() < 1e-13
+
+
+def test_curl_e_polynomial_3d():
+    # B -= dt curl E with E = u at the E stagger, compared at B positions
+    o, args = mk3()
+    for c in range(3):
+        o.f["E"][c][:] = field_from(U[c], *args, ST_E[c], 3)
+    dt = 0.29
+    o.update_b(dt)
+    for c in range(3):
+        expect = -dt * field_from(CURL_U[c], *args, ST_B[c], 3)
+        assert np.abs(bulk(o, o.f["B"][c]) - bulk(o, expect)).max() < 1e-13
+
+
+def test_curl_polynomial_2d():
+    N = 12
+    o = oc.OracleSim([N, N], [2, 2], periodic=(False, False))
+    args = (o.dims, [2, 2, 0], [1.0 / N, 1.0 / N, 1.0], [0.0] * 3)
+    # z-independent polynomial pair: B = (y^2, x^2, xy), curl = (x, -y, 2x - 2y)
+    Bf = (lambda x, y, z: y * y + 0 * x, lambda x, y, z: x * x + 0 * y, lambda x, y, z: x * y)
+    Cf = (lambda x, y, z: x + 0 * y, lambda x, y, z: -y + 0 * x, lambda x, y, z: 2 * x - 2 * y)
+    for c in range(3):
+        o.f["B"][c][:] = field_from(Bf[c], *args, ST_B[c], 2)
+    o.update_e(1.0)
+    for c in range(3):
+        expect = field_from(Cf[c], *args, ST_E[c], 2)
+        assert np.abs(bulk(o, o.f["E"][c]) - bulk(o, expect)).max() < 1e-12
+
+
+@pytest.mark.parametrize("dim,k", [(1, (3,)), (2, (2, 5)), (3, (1, 2, 3))])
+def test_vacuum_dispersion(dim, k):
+    N = 16
+    o = oc.OracleSim([N] * dim, [2] * dim, periodic=(True,) * 3)
+    D = 1.0 / N
+    args = (o.dims, [2] * dim + [0] * (3 - dim), [D] * dim + [1.0] * (3 - dim), [0.0] * 3)
+    kv = [2 * np.pi * ki for ki in k] + [0.0] * (3 - dim)
+    # a transverse E mode: E3 ~ cos(k.x) in 1D/2D, E along a vector normal to k in 3D
+    if dim < 3:
+        o.f["E"][2][:] = field_from(lambda x, y, z: np.cos(kv[0] * x + kv[1] * y), *args, ST_E[2], dim)
+        comp = 2
+    else:
+        # E1 with k_x = 0 component: use k = (0, ky, kz) for E1
+        kv[0] = 0.0
+        o.f["E"][0][:] = field_from(lambda x, y, z: np.cos(kv[1] * y + kv[2] * z), *args, ST_E[0], dim)
+        comp = 0
+    for c in range(3):
+        o.guard_copy(o.f["E"][c])
+    K2 = sum((2 * np.sin(kk * D / 2) / D) ** 2 for kk in kv[:dim])
+    dt = 0.3 * D
+    series = []
+    for n in range(12):
+        o.update_b(dt)
+        for c in range(3):
+            o.guard_copy(o.f["B"][c])
+        o.update_e(dt)
+        for c in range(3):
+            o.guard_copy(o.f["E"][c])
+        series.append(bulk(o, o.f["E"][comp]).copy())
+    lam = 2 - dt * dt * K2
+    for n in range(1, len(series) - 1):
+        assert np.abs(series[n + 1] + series[n - 1] - lam * series[n]).max() < 1e-12
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_div_b_preserved(dim):
+    N = 10
+    rng = np.random.default_rng(3)
+    o = oc.OracleSim([N] * dim, [2] * dim, periodic=(True,) * 3)
+    for c in range(3):
+        o.f["E"][c][:] = rng.normal(size=o.ncell)
+        o.f["B"][c][:] = rng.normal(size=o.ncell)
+        o.guard_copy(o.f["E"][c])
+        o.guard_copy(o.f["B"][c])
+    o.compute_divs()
+    d0 = bulk(o, o.divB.copy())
+    o.update_b(0.05)
+    for c in range(3):
+        o.guard_copy(o.f["B"][c])
+    o.compute_divs()
+    assert np.abs(bulk(o, o.divB) - d0).max() < 1e-11
+
+
+def test_div_e_follows_current():
+    N = 10
+    rng = np.random.default_rng(4)
+    o = oc.OracleSim([N, N], [2, 2], periodic=(True, True))
+    for c in range(3):
+        o.f["J"][c][:] = rng.normal(size=o.ncell)
+        o.guard_copy(o.f["J"][c])
+    o.compute_divs()
+    d0 = bulk(o, o.divE.copy())
+    dt = 0.1
+    o.update_e(dt)
+    for c in range(3):
+        o.guard_copy(o.f["E"][c])
+    o.compute_divs()
+    J1 = bulk(o, o.f["J"][0])
+    J2 = bulk(o, o.f["J"][1])
+    divJ = (np.roll(J1, -1, 2) - J1) * N + (np.roll(J2, -1, 1) - J2) * N
+    assert np.abs((bulk(o, o.divE) - d0) + dt * divJ).max() < 1e-11
