@@ -646,18 +646,25 @@ apt_status apt_ptc_update(apt_ctx *ctx, double dt, uint32_t step) {
       ti.tile_off = nullptr;
       ti.ntiles = 0;
     }
+    AccMixed a;
+    for (int c = 0; c < 3; c++) a.J[c] = ctx->Jacc[c][0];
+    for (int s = 0; s < 4; s++) a.R[s] = ctx->Racc[s][0];
+    a.rho = rho;
     time_begin(ctx, APT_KERNEL_PUSH_DEPOSIT);
     HIPCHK(launch_push_deposit_mixed(ctx->mesh, ctx->sp, (float)dt, ctx->H,
-                                     ctx->p32, f, ctx->Jacc[0][0],
-                                     ctx->Jacc[1][0], ctx->Jacc[2][0], ti,
-                                     ctx->d_counters, ctx->number,
-                                     ctx->stream));
+                                     ctx->p32, f, a, ti, ctx->d_counters,
+                                     ctx->number, ctx->stream));
     time_end(ctx, APT_KERNEL_PUSH_DEPOSIT);
     for (int c = 0; c < 3; c++)
       HIPCHK(launch_finalize_mixed(ctx->Jacc[c][0], (float *)ctx->J[c], nc,
                                    ctx->H, ctx->stream));
-    for (int s = 0; s < ns; s++)
-      HIPCHK(launch_fill<float>((float *)ctx->Rho[s], 0.0f, nc, ctx->stream));
+    for (int s = 0; s < ns; s++) {
+      if (rho)
+        HIPCHK(launch_finalize_mixed(ctx->Racc[s][0], (float *)ctx->Rho[s], nc,
+                                     ctx->H, ctx->stream));
+      else
+        HIPCHK(launch_fill<float>((float *)ctx->Rho[s], 0.0f, nc, ctx->stream));
+    }
   }
   return smooth(ctx, rho);
 }

@@ -69,10 +69,16 @@ struct TileInfo {
   uint32_t ntiles;
 };
 
+// int64 accumulators of the mixed form: J[c] always, R[s] when rho != 0
+struct AccMixed {
+  long long *J[3];
+  long long *R[4];
+  int rho;
+};
+
 hipError_t launch_push_deposit_mixed(const Mesh &m, const Species &sp,
                                      float dt, int S, const PtcF32 &p,
-                                     const FieldsF32 &f, long long *J0,
-                                     long long *J1, long long *J2,
+                                     const FieldsF32 &f, const AccMixed &a,
                                      const TileInfo &tiles,
                                      unsigned long long *counters,
                                      uint64_t n, hipStream_t st);

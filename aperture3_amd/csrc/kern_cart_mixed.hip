@@ -111,11 +111,12 @@ __device__ __forceinline__ void shapes(float xo, float xn, int dc, float s0[4],
 template <int DIM>
 __global__ void __launch_bounds__(256)
     k_push_deposit_mixed(Mesh m, Species sp, float dt, float scale, PtcF32 p,
-                         FieldsF32 f, long long *__restrict__ J0,
-                         long long *__restrict__ J1,
-                         long long *__restrict__ J2,
+                         FieldsF32 f, AccMixed a,
                          unsigned long long *__restrict__ counters,
                          uint64_t n) {
+  long long *__restrict__ J0 = a.J[0];
+  long long *__restrict__ J1 = a.J[1];
+  long long *__restrict__ J2 = a.J[2];
   uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n) return;
   uint32_t c = p.cell[idx];
@@ -204,6 +205,7 @@ __global__ void __launch_bounds__(256)
         uint64_t o = m.widx(i, c2, c3);
         acc_fx(J1, o, qw * v2 * val, scale);
         acc_fx(J2, o, qw * v3 * val, scale);
+        if (a.rho && sx1[t] != 0.0f) acc_fx(a.R[s], o, qw * sx1[t], scale);
       }
     }
     return;
@@ -231,6 +233,8 @@ __global__ void __launch_bounds__(256)
                      sx1[ti] * sy0[tj] + 2.0f * sx0[ti] * sy0[tj]) *
                     (1.0f / 6.0f);
         if (cen != 0.0f) acc_fx(J2, m.widx(i, j, c3), -wt * v3 * cen, scale);
+        if (a.rho && sx1[ti] * sy1[tj] != 0.0f)
+          acc_fx(a.R[s], m.widx(i, j, c3), qw * (sx1[ti] * sy1[tj]), scale);
       }
     }
     return;
@@ -273,6 +277,8 @@ __global__ void __launch_bounds__(256)
           djz[tj][ti] += (sz1[tk] - sz0[tk]) * cxy;
           if (tk < 3 && djz[tj][ti] != 0.0f)
             acc_fx(J2, m.widx(i, j, k + 1), wt * djz[tj][ti], scale);
+          float s3 = sx1[ti] * sy1[tj] * sz1[tk];
+          if (a.rho && s3 != 0.0f) acc_fx(a.R[s], m.widx(i, j, k), qw * s3, scale);
         }
       }
     }
@@ -290,8 +296,7 @@ __global__ void k_finalize_mixed(long long *__restrict__ acc,
 
 hipError_t launch_push_deposit_mixed(const Mesh &m, const Species &sp,
                                      float dt, int S, const PtcF32 &p,
-                                     const FieldsF32 &f, long long *J0,
-                                     long long *J1, long long *J2,
+                                     const FieldsF32 &f, const AccMixed &a,
                                      const TileInfo &tiles,
                                      unsigned long long *counters,
                                      uint64_t n, hipStream_t st) {
@@ -300,14 +305,14 @@ hipError_t launch_push_deposit_mixed(const Mesh &m, const Species &sp,
   float scale = ldexpf(1.0f, S);
   dim3 block(256), grid((unsigned)((n + 255) / 256));
   if (m.dim == 1)
-    k_push_deposit_mixed<1><<<grid, block, 0, st>>>(m, sp, dt, scale, p, f, J0,
-                                                    J1, J2, counters, n);
+    k_push_deposit_mixed<1><<<grid, block, 0, st>>>(m, sp, dt, scale, p, f, a,
+                                                    counters, n);
   else if (m.dim == 2)
-    k_push_deposit_mixed<2><<<grid, block, 0, st>>>(m, sp, dt, scale, p, f, J0,
-                                                    J1, J2, counters, n);
+    k_push_deposit_mixed<2><<<grid, block, 0, st>>>(m, sp, dt, scale, p, f, a,
+                                                    counters, n);
   else
-    k_push_deposit_mixed<3><<<grid, block, 0, st>>>(m, sp, dt, scale, p, f, J0,
-                                                    J1, J2, counters, n);
+    k_push_deposit_mixed<3><<<grid, block, 0, st>>>(m, sp, dt, scale, p, f, a,
+                                                    counters, n);
   return hipGetLastError();
 }
 

@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the PIC time step on one MI355X.
+
+Workload (the headline configuration): 2D Cartesian periodic box of
+2048 x 2048 cells, electrons and positrons at 32 particles per cell each
+(268,435,456 particles), drifting Maxwell-Juttner momenta theta = 0.05 with
+counter-streaming drift +0.3c / -0.3c along x, uniform B_z = 1, E = 0, Yee
+solver, particles re-sorted by cell every 20 steps; the mixed form (FP32
+push and fields, int32 cell + FP32 offset, int64 fixed-point current).
+
+A step is one full PIC step: push + move + deposit of every particle, the
+current finalisation, the Yee E/B update with divergences, and the sort on
+its cadence.  Inputs are resident in HBM before timing starts (synthetic,
+loaded on the GPU from the Philox stream).
+
+Prints one JSON line with pushes/s (value), the plain FP64 form's pushes/s
+on the same configuration, field cell-updates/s, the roofline of the
+dominant kernel (push+deposit, HBM bound) and the CPU baseline (the FP64
+oracle, one core, on a bounded sample of the same workload).
+
+    python bench.py [--gpus 1] [--steps 40] [--warmup 5]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_CELLS = 2048
+PPC = 32                # per species
+THETA = 0.05
+DRIFT = 0.3
+SORT_INTERVAL = 20
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+BYTES_PER_PUSH = {"mixed": 64, "fp64": 132}   # algorithmic particle stream
+BYTES_PER_CELL = {"mixed": 60, "fp64": 120}   # Yee: read E,B,J; write E,B
+
+
+def make_sim(apt, precision, n_cells=N_CELLS, ppc=PPC):
+    p = apt.SimParams(N=[n_cells, n_cells, 1], guard=[2, 2, 0],
+                      size=[1.0, 1.0, 1.0], periodic_boundary=[True, True, True],
+                      q_e=1.0, max_ptc_number=n_cells * n_cells * ppc * 2 + 1024,
+                      precision=precision, data_interval=10 ** 9,
+                      sort_interval=SORT_INTERVAL, random_seed=43210)
+    env = apt.sim_environment(p)
+    data = apt.sim_data(env)
+    data.init_bg_B_field(2, lambda x, y, z: 1.0 + 0.0 * x)
+    data.fill_multiplicity(1.0, ppc, (THETA, THETA), (DRIFT, -DRIFT))
+    return env, data, p
+
+
+def run_steps(apt, env, data, dt, first, count):
+    pu, fs = apt.ptc_updater(env), apt.field_solver(env)
+    for step in range(first, first + count):
+        pu.update_particles(data, dt, step)
+        fs.update_fields(data, dt, step)
+        if step % SORT_INTERVAL == 0 and step != 0:
+            data.sort_particles()
+
+
+def measure(apt, precision, steps, warmup):
+    env, data, p = make_sim(apt, precision)
+    n = data.num_particles()
+    dt = 0.5 / N_CELLS
+    run_steps(apt, env, data, dt, 0, warmup)
+    data.sync()
+    data.timing_enable(True)
+    t0 = time.perf_counter()
+    run_steps(apt, env, data, dt, warmup, steps)
+    data.sync()
+    t1 = time.perf_counter()
+    push_ms, push_n = data.timing_get(apt.KERNEL_PUSH_DEPOSIT)
+    field_ms, field_n = data.timing_get(apt.KERNEL_FIELD)
+    sort_ms, sort_n = data.timing_get(apt.KERNEL_SORT)
+    counters = data.counters()
+    data.timing_enable(False)
+    data.close()
+    el = t1 - t0
+    return {
+        "particles": n, "elapsed_s": el, "steps": steps,
+        "pushes_per_s": n * steps / el,
+        "ms_per_step": 1e3 * el / steps,
+        "push_kernel_ms": push_ms / max(push_n, 1),
+        "field_ms": field_ms / max(field_n, 1),
+        "sort_ms": sort_ms / max(sort_n, 1) if sort_n else None,
+        "sorts": sort_n,
+        "counters": counters,
+    }
+
+
+def cpu_baseline():
+    """The FP64 oracle (a C port of the reference CUDA path, one core) on a
+    bounded sample of the same workload: 512 x 512 cells at 32 ppc per
+    species (16.8 M particles), 2 steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ctypes as oc
+    n = 512
+    o = oc.OracleSim([n, n], [2, 2], periodic=(True, True, True), q_e=1.0,
+                     capacity=n * n * PPC * 2, data_interval=10 ** 9)
+    o.f["B"][2][:] = 1.0
+    o.f["Bbg"][2][:] = 1.0
+    o.load_uniform(PPC, (THETA, THETA), (DRIFT, -DRIFT), 1.0, 43210)
+    dt = 0.5 / n
+    steps = 2
+    t0 = time.perf_counter()
+    for s in range(steps):
+        o.ptc_update(dt, s)
+        o.field_update(dt, s)
+    el = time.perf_counter() - t0
+    return {"value": o.number * steps / el, "unit": "particle pushes/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{n}x{n} cells, {PPC} ppc per species ({o.number} particles), "
+                      f"{steps} full FP64 steps (push+deposit+Yee), single thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--fp64-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the PIC step is "
+                          "not sharded (see the sharding design); use --gpus 1"}))
+        return 2
+    import aperture3_amd as apt
+    apt.load_library()
+
+    mixed = measure(apt, apt.APT_PREC_MIXED, args.steps, args.warmup)
+    fp64 = None if args.no_fp64 else measure(apt, apt.APT_PREC_FP64, args.fp64_steps, 2)
+
+    n = mixed["particles"]
+    avg_s = mixed["push_kernel_ms"] * 1e-3
+    achieved = n * BYTES_PER_PUSH["mixed"] / avg_s / 1e9
+    cells = N_CELLS * N_CELLS
+    line = {
+        "metric": "particle pushes/s (push+deposit), HBM roofline frac; mixed vs FP64 on 1 MI355X",
+        "value": mixed["pushes_per_s"],
+        "unit": "particle pushes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (Philox-seeded Maxwell-Juttner loading on the GPU)",
+        "config": {
+            "workload": "2D Cartesian periodic box 2048x2048, e-/e+ at 32 ppc each "
+                        "(268,435,456 particles), drifting MJ theta=0.05 +/-0.3c in x, "
+                        "B_z=1, E=0, Yee FDTD, sort every 20 steps; mixed form "
+                        "(FP32 push/fields, int32 cell + FP32 offset, int64 fixed-point J)",
+            "particles": n,
+            "cells": cells,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "push+move+deposit (mixed)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_push": BYTES_PER_PUSH["mixed"],
+            "avg_launch_ms": mixed["push_kernel_ms"],
+        },
+        "field_cell_updates_per_s": cells / (mixed["field_ms"] * 1e-3),
+        "sort_ms": mixed["sort_ms"],
+        "mixed": mixed,
+    }
+    if fp64:
+        line["fp64_pushes_per_s"] = fp64["pushes_per_s"]
+        line["mixed_over_fp64"] = mixed["pushes_per_s"] / fp64["pushes_per_s"]
+        line["fp64"] = fp64
+        line["fp64_roofline_frac"] = (fp64["particles"] * BYTES_PER_PUSH["fp64"]
+                                      / (fp64["push_kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,254 @@
+"""GPU parity of the Cartesian time step against the CPU oracle, through the
+C ABI (HIP library, no fallback).
+
+Bar: the FP64 form is bit-exact (cells, positions, momenta, gamma, E, B, J,
+Rho, div E, sort order); the mixed form is within the stated FP32
+tolerances of the FP64 oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import aperture3_amd as apt
+import oracle_ctypes as oc
+from cases import Case
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "1d": dict(N=(256,), ppc=10, Bbg=(1.0, 0.0, 0.0)),
+    "2d": dict(N=(16, 16), ppc=4, Bbg=(0.0, 0.0, 1.0)),
+    "3d": dict(N=(6, 6, 6), ppc=2, Bbg=(0.0, 0.0, 1.0)),
+}
+PTC_KEYS = ("cell", "x1", "x2", "x3", "p1", "p2", "p3", "E", "w", "flag", "id")
+
+
+def assert_same_particles(pg, po, keys=PTC_KEYS):
+    for k in keys:
+        assert np.array_equal(pg[k], po[k]), k
+
+
+def run_both(c, o, env, d, steps, sort_every=0):
+    pu, fs = apt.ptc_updater(env), apt.field_solver(env)
+    for step in range(steps):
+        o.ptc_update(c.dt, step)
+        pu.update_particles(d, c.dt, step)
+        o.field_update(c.dt, step)
+        fs.update_fields(d, c.dt, step)
+        if sort_every and step % sort_every == 0 and step != 0:
+            o.sort()
+            d.sort_particles()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_loader_bit_exact(name):
+    c = Case(theta=(0.1, 0.1), drift=(0.2, -0.2), **CASES[name])
+    o = c.oracle()
+    env, d = c.gpu(precision=apt.APT_PREC_FP64)
+    assert d.num_particles() == o.number
+    assert_same_particles(d.get_particles(), o.particles())
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp64_step_bit_exact(name):
+    c = Case(theta=(0.1, 0.1), drift=(0.2, -0.2), data_interval=4, **CASES[name])
+    o = c.oracle()
+    env, d = c.gpu(precision=apt.APT_PREC_FP64, load=False)
+    d.set_particles(o.particles())
+    assert d.jexp() == o.sp.jexp
+    run_both(c, o, env, d, 12, sort_every=5)
+    assert d.num_particles() == o.number
+    assert_same_particles(d.get_particles(), o.particles())
+    for comp in range(3):
+        for which, key in ((apt.FIELD_E, "E"), (apt.FIELD_B, "B"), (apt.FIELD_J, "J")):
+            assert np.array_equal(d.get_field(which, comp), o.f[key][comp]), (key, comp)
+    # step 11 is a Rho deposit step ((11 + 1) % 4 == 0)
+    for s in range(2):
+        r = d.get_field(apt.FIELD_RHO, s)
+        assert np.abs(r).max() > 0
+        assert np.array_equal(r, o.rho[s])
+    assert np.array_equal(d.get_field(apt.FIELD_DIVE, 0), o.divE)
+    assert np.array_equal(d.get_field(apt.FIELD_DIVB, 0), o.divB)
+    assert d.counters() == {"nan_momenta": 0, "multi_cell_moves": 0,
+                            "capacity_overflow": 0, "outside_tile": 0}
+
+
+def test_c1_full_size_bit_exact():
+    """The 1D gap case at full size (4096 cells, 100 ppc each species,
+    819,200 particles, dt = dx/2): GPU loader and 60 steps bit-exact."""
+    c = Case((4096,), 100, theta=(0.1, 0.1), Bbg=(1.0, 0.0, 0.0))
+    assert c.nptc == 819200
+    o = c.oracle()
+    env, d = c.gpu(precision=apt.APT_PREC_FP64)
+    assert_same_particles(d.get_particles(), o.particles())
+    run_both(c, o, env, d, 60, sort_every=20)
+    assert_same_particles(d.get_particles(), o.particles(), ("cell", "x1", "p1", "p2", "p3", "E", "id"))
+    for comp in range(3):
+        assert np.array_equal(d.get_field(apt.FIELD_E, comp), o.f["E"][comp])
+        assert np.array_equal(d.get_field(apt.FIELD_B, comp), o.f["B"][comp])
+
+
+def test_c1_1000_steps_bit_exact_and_gauss():
+    """The 1D gap case for its full 1000 steps at 4 pairs per cell: bit-exact
+    to the end, and the GPU's Gauss residual div E - (dt/dx) rho drifts only
+    by round-off after the step-0 half step."""
+    c = Case((4096,), 4, theta=(0.1, 0.1), Bbg=(1.0, 0.0, 0.0), data_interval=50)
+    o = c.oracle()
+    env, d = c.gpu(precision=apt.APT_PREC_FP64)
+    kappa = c.dt / c.delta[0]
+    pu, fs = apt.ptc_updater(env), apt.field_solver(env)
+    res = []
+    for step in range(1000):
+        o.ptc_update(c.dt, step)
+        pu.update_particles(d, c.dt, step)
+        o.field_update(c.dt, step)
+        fs.update_fields(d, c.dt, step)
+        if step % 20 == 0 and step != 0:
+            o.sort()
+            d.sort_particles()
+        if (step + 1) % 50 == 0:
+            g = c.guard[0]
+            rho = d.get_field(apt.FIELD_RHO, 0) + d.get_field(apt.FIELD_RHO, 1)
+            res.append((d.get_field(apt.FIELD_DIVE, 0) - kappa * rho)[g:-g])
+    assert_same_particles(d.get_particles(), o.particles(), ("cell", "x1", "p1", "p2", "p3", "E", "id"))
+    for comp in range(3):
+        assert np.array_equal(d.get_field(apt.FIELD_E, comp), o.f["E"][comp])
+    drift = max(np.abs(r - res[0]).max() for r in res)
+    assert drift < 1e-12, drift
+
+
+def test_sort_known_answer_gpu():
+    """tests/test_particles_dev.cpp:115-144 on the GPU: 1800 survive."""
+    p = apt.SimParams(N=[400, 400, 1], guard=[3, 3, 0], max_ptc_number=4000)
+    env = apt.sim_environment(p)
+    d = apt.sim_data(env)
+    rng = np.random.default_rng(0)
+    d0 = env.dims[0]
+    cells = [oc.MAX_CELL if i % 10 == 0 else
+             int(rng.integers(200, 301)) + int(rng.integers(200, 301)) * d0 for i in range(2000)]
+    cells += [oc.MAX_CELL] * 100 + [1 + int(rng.integers(200, 301)) * d0 for _ in range(900)]
+    n = len(cells)
+    z = np.zeros(n)
+    d.set_particles({"cell": np.array(cells, np.uint32), "x1": rng.random(n), "x2": rng.random(n),
+                     "x3": z, "p1": z, "p2": z, "p3": z, "w": np.ones(n),
+                     "flag": np.zeros(n, np.uint32)})
+    assert d.num_particles() == 3000
+    d.clear_guard_cells()
+    d.sort_particles()
+    assert d.num_particles() == 1800
+
+
+def test_energy_histogram_matches_oracle():
+    c = Case((64,), 50, theta=(0.3, 0.3))
+    o = c.oracle()
+    env, d = c.gpu(precision=apt.APT_PREC_FP64)
+    for nb, emax in ((64, 10.0), (17, 3.0)):
+        assert np.array_equal(d.energy_histogram(nb, emax), o.energy_histogram(nb, emax))
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_current_filter_matches_oracle(dim):
+    N = {1: (64,), 2: (12, 12), 3: (6, 6, 6)}[dim]
+    c = Case(N, 3, theta=(0.2, 0.2))
+    o = c.oracle()
+    p = apt.SimParams(N=c.N, guard=c.guard, periodic_boundary=[True] * 3,
+                      max_ptc_number=c.nptc + 8, current_smoothing=2)
+    env = apt.sim_environment(p)
+    d = apt.sim_data(env)
+    E, B = c.fields()
+    for comp in range(3):
+        d.set_field(apt.FIELD_E, comp, E[comp])
+        d.set_field(apt.FIELD_B, comp, B[comp])
+        d.set_field(apt.FIELD_BBG, comp, B[comp])
+    d.set_particles(o.particles())
+    apt.ptc_updater(env).update_particles(d, c.dt, 0)
+    o.ptc_update(c.dt, 0)
+    for comp in range(3):
+        for _ in range(2):
+            o.filter_current(o.f["J"][comp])
+        assert np.array_equal(d.get_field(apt.FIELD_J, comp), o.f["J"][comp])
+
+
+def test_nan_momentum_is_counted_not_trapped():
+    c = Case((32,), 1)
+    env, d = c.gpu(precision=apt.APT_PREC_FP64, load=False)
+    d.set_particles({"cell": np.array([10], np.uint32), "x1": [0.5], "x2": [0.0], "x3": [0.0],
+                     "p1": [np.nan], "p2": [0.0], "p3": [0.0], "w": [1.0],
+                     "flag": np.array([1 << 29], np.uint32)})
+    apt.ptc_updater(env).update_particles(d, c.dt, 0)
+    assert d.counters()["nan_momenta"] == 1
+
+
+def test_cpp_host_driver_matches_oracle(tmp_path):
+    """The C++ mirror of ptc_updater / field_solver / sim_data
+    (pic_step_driver, a lab-style main) against the oracle."""
+    exe = os.path.join(os.path.dirname(apt.LIB_PATH), "pic_step_driver")
+    out = tmp_path / "drv.bin"
+    r = subprocess.run([exe, str(out), "1", "128", "8", "45", "0", "0.1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = open(out, "rb").read()
+    n = int(np.frombuffer(raw[:8], np.uint64)[0])
+    off = 8
+    cell = np.frombuffer(raw[off:off + 4 * n], np.uint32); off += 4 * n
+    arrs = []
+    for _ in range(4):
+        arrs.append(np.frombuffer(raw[off:off + 8 * n], np.float64)); off += 8 * n
+    ne = int(np.frombuffer(raw[off:off + 8], np.uint64)[0]); off += 8
+    e1 = np.frombuffer(raw[off:off + 8 * ne], np.float64)
+    c = Case((128,), 8, theta=(0.1, 0.1), Bbg=(1.0, 0.0, 0.0), data_interval=1000000)
+    o = c.oracle()
+    o.sort()  # fill_multiplicity sorts after loading
+    for step in range(45):
+        o.ptc_update(c.dt, step)
+        o.field_update(c.dt, step)
+        if step % 20 == 0 and step != 0:
+            o.sort()
+    po = o.particles()
+    assert n == o.number
+    assert np.array_equal(cell, po["cell"])
+    for a, k in zip(arrs, ("x1", "p1", "p2", "p3")):
+        assert np.array_equal(a, po[k]), k
+    assert np.array_equal(e1, o.f["E"][0])
+
+
+# ---- mixed form ---------------------------------------------------------
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_mixed_within_fp32_tolerance(name):
+    """FP32 push/fields + int64 fixed-point J against the FP64 oracle for
+    12 steps: fields and momenta within 1e-5 of the field/momentum scale,
+    every cell index identical."""
+    c = Case(theta=(0.1, 0.1), drift=(0.2, -0.2), **CASES[name])
+    o = c.oracle()
+    env, d = c.gpu(precision=apt.APT_PREC_MIXED, load=False)
+    d.set_particles(o.particles())
+    run_both(c, o, env, d, 12)
+    pg, po = d.get_particles(), o.particles()
+    assert np.mean(pg["cell"] == po["cell"]) > 0.999
+    for k in ("p1", "p2", "p3"):
+        assert np.abs(pg[k] - po[k]).max() <= 1e-5 * np.abs(po[k]).max(), k
+    for comp in range(3):
+        for which, key in ((apt.FIELD_E, "E"), (apt.FIELD_B, "B")):
+            a, b = d.get_field(which, comp), o.f[key][comp]
+            assert np.abs(a - b).max() <= 1e-5 * max(np.abs(b).max(), 1e-30), (key, comp)
+
+
+def test_mixed_charge_conservation():
+    """Mixed form: div E - (dt/dx) rho drifts only by FP32 round-off (the
+    fixed-point deposit is charge conserving; E is FP32)."""
+    c = Case((24, 24), 4, theta=(0.1, 0.1), Bbg=(0.0, 0.0, 1.0), data_interval=1)
+    o = c.oracle()
+    env, d = c.gpu(precision=apt.APT_PREC_MIXED, load=False)
+    d.set_particles(o.particles())
+    pu, fs = apt.ptc_updater(env), apt.field_solver(env)
+    kappa = c.dt / c.delta[0]
+    res = []
+    for step in range(40):
+        pu.update_particles(d, c.dt, step)
+        fs.update_fields(d, c.dt, step)
+        rho = d.get_field(apt.FIELD_RHO, 0) + d.get_field(apt.FIELD_RHO, 1)
+        res.append(d.get_field(apt.FIELD_DIVE, 0) - kappa * rho)
+    scale = np.abs(d.get_field(apt.FIELD_DIVE, 0)).max()
+    drift = max(np.abs(r - res[1]).max() for r in res[1:])
+    assert drift < 1e-5 * scale, (drift, scale)
