@@ -88,6 +88,8 @@ SIGNATURES = {
     "apt_sort_particles": [_P],
     "apt_clear_guard_cells": [_P],
     "apt_energy_histogram": [_P, C.POINTER(C.c_uint32), C.c_int, C.c_double, C.c_int],
+    "apt_tile_offsets": [_P, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint32),
+                         C.POINTER(C.c_int)],
     "apt_sync": [_P],
     "apt_get_counters": [_P, C.POINTER(AptCounters)],
     "apt_reset_counters": [_P],
@@ -321,6 +323,21 @@ class sim_data:
             self.ctx, h.ctypes.data_as(C.POINTER(C.c_uint32)), nbins, E_max, flag_bit),
             "apt_energy_histogram")
         return h
+
+    def tile_offsets(self):
+        """(offsets, tile extent) of the mixed form's tile binning, or
+        (None, extent) when the particles are not binned."""
+        nt = C.c_uint32()
+        ext = (C.c_int * 3)()
+        self._check(self.lib.apt_tile_offsets(self.ctx, None, 0, C.byref(nt), ext),
+                    "apt_tile_offsets")
+        if nt.value == 0:
+            return None, tuple(ext)
+        off = np.zeros(nt.value + 1, np.uint32)
+        self._check(self.lib.apt_tile_offsets(
+            self.ctx, off.ctypes.data_as(C.POINTER(C.c_uint32)), off.size, C.byref(nt), ext),
+            "apt_tile_offsets")
+        return off, tuple(ext)
 
     # ---- control -----------------------------------------------------------
     def jexp(self):

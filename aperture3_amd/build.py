@@ -28,6 +28,8 @@ HIP_SOURCES = [
     "kern_field.hip",
     "kern_init.hip",
     "kern_sort.hip",
+    "kern_tile_mixed.hip",
+    "kern_tile_sort.hip",
     "apt_context.hip",
 ]
 HOST_SOURCES = ["host/aperture_host.cpp"]

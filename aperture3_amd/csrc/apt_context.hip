@@ -16,6 +16,7 @@
 #include "apt_common.hpp"
 #include "apt_kernels.hpp"
 #include "apt_sort.hpp"
+#include "apt_tiles.hpp"
 
 using namespace apt;
 
@@ -51,9 +52,10 @@ struct apt_ctx {
 
   // mixed-form tiling
   uint32_t *tile_off = nullptr;
-  uint32_t *tile_key_lut = nullptr;
   TileInfo tiles{};
-  bool tiles_valid = false;
+  bool tiling = false;       // shape set up for this mesh
+  bool tiles_valid = false;  // particles are binned by tile
+  TileSortWork tsort;
 
   std::string err;
   bool timing = false;
@@ -100,10 +102,11 @@ static int ceil_log2(double x) {
 
 // fixed-point exponent for the current from the largest |q w|: FP64 keeps
 // |J| < 2^41 wmax in the high limb; the mixed form rounds every
-// contribution to 2^-H and keeps |J| < 2^(63-H).
+// contribution (|c| <= |q w|) to an int32 multiple of 2^-H, H = 30 - e,
+// and sums in int64, so |J| < 2^33 wmax.
 static void update_scale(apt_ctx *ctx) {
   int e = ceil_log2(ctx->wmax > 0 ? ctx->wmax : 1.0);
-  ctx->H = (ctx->prec == APT_PREC_FP64) ? 22 - e : 40 - e;
+  ctx->H = (ctx->prec == APT_PREC_FP64) ? 22 - e : 30 - e;
 }
 
 static apt_status alloc_ptc(apt_ctx *ctx, bool second) {
@@ -149,6 +152,41 @@ static void free_ptc32(PtcF32 &p) {
   (void)hipFree(p.u1); (void)hipFree(p.u2); (void)hipFree(p.u3); (void)hipFree(p.w);
   (void)hipFree(p.flag);
   p = PtcF32{};
+}
+
+// Tiling of the mixed form: every tiled dimension periodic and divisible by
+// the tile extent; defaults 16x16 (halo 6) in 2D and 8^3 (halo 3) in 3D.
+static apt_status setup_tiles(apt_ctx *ctx) {
+  const Mesh &m = ctx->mesh;
+  ctx->tiling = false;
+  if (m.dim < 2) return APT_OK;
+  int T[3] = {1, 1, 1};
+  int halo;
+  if (m.dim == 2) {
+    T[0] = ctx->params.tile[0] > 0 ? ctx->params.tile[0] : 16;
+    T[1] = ctx->params.tile[1] > 0 ? ctx->params.tile[1] : 16;
+    halo = 6;
+    if (T[0] == 8) halo = 4;
+  } else {
+    T[0] = T[1] = T[2] = ctx->params.tile[0] > 0 ? ctx->params.tile[0] : 8;
+    halo = 3;
+  }
+  if (!tile_shape_supported(m.dim, T, halo)) return APT_OK;
+  uint64_t ntiles = 1;
+  for (int d = 0; d < m.dim; d++) {
+    if (!m.periodic[d] || m.N[d] % T[d] != 0) return APT_OK;
+    ctx->tiles.nt[d] = m.N[d] / T[d];
+    ntiles *= (uint64_t)ctx->tiles.nt[d];
+  }
+  for (int d = m.dim; d < 3; d++) ctx->tiles.nt[d] = 1;
+  for (int d = 0; d < 3; d++) ctx->tiles.T[d] = T[d];
+  ctx->tiles.halo = halo;
+  ctx->tiles.ntiles = (uint32_t)ntiles;
+  HIPCHK(hipMalloc(&ctx->tile_off, (ntiles + 1) * 4));
+  ctx->tiles.tile_off = ctx->tile_off;
+  ctx->tiles.tiled_end = 0;
+  ctx->tiling = true;
+  return APT_OK;
 }
 
 // ---- timing helpers ----------------------------------------------------
@@ -279,6 +317,13 @@ apt_status apt_create(const apt_grid *grid, const apt_params *params,
     apt_destroy(ctx);
     return st;
   }
+  if (ctx->prec == APT_PREC_MIXED) {
+    st = setup_tiles(ctx);
+    if (st != APT_OK) {
+      apt_destroy(ctx);
+      return st;
+    }
+  }
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
     apt_destroy(ctx);
     return APT_ERR_HIP;
@@ -302,7 +347,8 @@ apt_status apt_destroy(apt_ctx *ctx) {
   }
   (void)hipFree(ctx->divE); (void)hipFree(ctx->divB); (void)hipFree(ctx->jtmp);
   (void)hipFree(ctx->d_counters);
-  (void)hipFree(ctx->tile_off); (void)hipFree(ctx->tile_key_lut);
+  (void)hipFree(ctx->tile_off);
+  ctx->tsort.release();
   free_ptc64(ctx->p64); free_ptc64(ctx->p64b);
   free_ptc32(ctx->p32); free_ptc32(ctx->p32b);
   ctx->sortw.release();
@@ -551,14 +597,28 @@ apt_status apt_ptc_load_uniform(apt_ctx *ctx, int ppc_pairs,
   double qw = fabs(ctx->params.q_e * weight);
   if (ctx->number == 0 || qw > ctx->wmax) ctx->wmax = qw;
   update_scale(ctx);
+  bool tiled_load = ctx->prec == APT_PREC_MIXED && ctx->tiling && base == 0;
   if (ctx->prec == APT_PREC_FP64)
     HIPCHK(launch_load_uniform_f64(m, lp, ctx->p64, ctx->d_counters,
                                    ctx->stream));
+  else if (tiled_load)
+    HIPCHK(launch_load_tiled_f32(m, lp, ctx->tiles, ctx->p32, ctx->d_counters,
+                                 ctx->stream));
   else
     HIPCHK(launch_load_uniform_f32(m, lp, ctx->p32, ctx->d_counters,
                                    ctx->stream));
   ctx->number = base + add;
   ctx->tiles_valid = false;
+  if (tiled_load) {
+    const uint64_t per = (uint64_t)ctx->tiles.T[0] * ctx->tiles.T[1] *
+                         ctx->tiles.T[2] * 2 * (uint64_t)ppc_pairs;
+    std::vector<uint32_t> off(ctx->tiles.ntiles + 1);
+    for (uint64_t t = 0; t <= ctx->tiles.ntiles; t++) off[t] = (uint32_t)(t * per);
+    HIPCHK(hipMemcpyAsync(ctx->tile_off, off.data(), off.size() * 4,
+                          hipMemcpyHostToDevice, ctx->stream));
+    ctx->tiles.tiled_end = add;
+    ctx->tiles_valid = true;
+  }
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return APT_OK;
 }
@@ -645,6 +705,7 @@ apt_status apt_ptc_update(apt_ctx *ctx, double dt, uint32_t step) {
     if (!ctx->tiles_valid) {
       ti.tile_off = nullptr;
       ti.ntiles = 0;
+      ti.tiled_end = 0;
     }
     AccMixed a;
     for (int c = 0; c < 3; c++) a.J[c] = ctx->Jacc[c][0];
@@ -750,6 +811,69 @@ static apt_status ensure_second_buffer(apt_ctx *ctx) {
   return APT_OK;
 }
 
+static apt_status gather_all32(apt_ctx *ctx, uint64_t n) {
+  PtcF32 &a = ctx->p32, &b = ctx->p32b;
+  HIPCHK(gather_field<uint32_t>(ctx->sortw, a.cell, b.cell, n, ctx->stream));
+  HIPCHK(gather_field<float>(ctx->sortw, a.x1, b.x1, n, ctx->stream));
+  HIPCHK(gather_field<float>(ctx->sortw, a.x2, b.x2, n, ctx->stream));
+  HIPCHK(gather_field<float>(ctx->sortw, a.x3, b.x3, n, ctx->stream));
+  HIPCHK(gather_field<float>(ctx->sortw, a.u1, b.u1, n, ctx->stream));
+  HIPCHK(gather_field<float>(ctx->sortw, a.u2, b.u2, n, ctx->stream));
+  HIPCHK(gather_field<float>(ctx->sortw, a.u3, b.u3, n, ctx->stream));
+  HIPCHK(gather_field<float>(ctx->sortw, a.w, b.w, n, ctx->stream));
+  HIPCHK(gather_field<uint32_t>(ctx->sortw, a.flag, b.flag, n, ctx->stream));
+  std::swap(ctx->p32, ctx->p32b);
+  return APT_OK;
+}
+
+// Mixed form with tiling: the deterministic tile-binning sort; when a
+// particle moved further than one tile (or slots were appended past the
+// tiled range) a stable radix sort by tile key instead.  Both leave the
+// particles binned by tile with the tile offsets on the device.
+static apt_status sort_tiles(apt_ctx *ctx) {
+  const uint64_t n = ctx->number;
+  TileInfo &ti = ctx->tiles;
+  time_begin(ctx, APT_KERNEL_SORT);
+  unsigned long long far = 1;
+  if (ctx->tiles_valid) {
+    HIPCHK(tile_sort_count(ctx->mesh, ti, ctx->p32.cell, n, ctx->tsort,
+                           ctx->stream));
+    HIPCHK(hipMemcpyAsync(&far, ctx->tsort.far, 8, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  if (far == 0) {
+    HIPCHK(tile_sort_offsets(ctx->mesh, ti, ctx->tsort, ctx->stream));
+    HIPCHK(tile_sort_scatter(ctx->mesh, ti, ctx->p32, ctx->p32b, n,
+                             ctx->tsort, ctx->stream));
+    std::swap(ctx->p32, ctx->p32b);
+    HIPCHK(hipMemcpyAsync(ctx->tile_off, ctx->tsort.newoff,
+                          (ti.ntiles + 1) * 4ull, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+  } else {
+    HIPCHK(ctx->sortw.ensure(n));
+    HIPCHK(tile_keys(ctx->mesh, ti, ctx->p32.cell, n, ctx->sortw.keys_in,
+                     ctx->sortw.perm_in, ctx->stream));
+    HIPCHK(hipcub_sort_pairs(ctx->sortw, n, key_bits(ti.ntiles), ctx->stream));
+    apt_status st = gather_all32(ctx, n);
+    if (st != APT_OK) return st;
+    HIPCHK(tile_offsets_from_sorted(ctx->sortw.keys_out, n, ti.ntiles,
+                                    ctx->tile_off, ctx->stream));
+  }
+  time_end(ctx, APT_KERNEL_SORT);
+  uint32_t kept = 0;
+  HIPCHK(hipMemcpyAsync(&kept, ctx->tile_off + ti.ntiles, 4,
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (kept < n)
+    HIPCHK(launch_fill<uint32_t>(ctx->p32.cell + kept, MAX_CELL, n - kept,
+                                 ctx->stream));
+  ctx->number = kept;
+  ti.tiled_end = kept;
+  ctx->tiles_valid = true;
+  return APT_OK;
+}
+
 apt_status apt_sort_particles(apt_ctx *ctx) {
   CHECK_CTX(ctx);
   uint64_t n = ctx->number;
@@ -757,6 +881,7 @@ apt_status apt_sort_particles(apt_ctx *ctx) {
   apt_status st = ensure_second_buffer(ctx);
   if (st != APT_OK) return st;
   const uint32_t empty_key = (uint32_t)ctx->ncell;
+  if (ctx->prec == APT_PREC_MIXED && ctx->tiling) return sort_tiles(ctx);
   time_begin(ctx, APT_KERNEL_SORT);
   if (ctx->prec == APT_PREC_FP64) {
     PtcF64 &a = ctx->p64, &b = ctx->p64b;
@@ -828,6 +953,25 @@ apt_status apt_energy_histogram(apt_ctx *ctx, uint32_t *hist, int nbins,
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   (void)hipFree(d);
   HIPCHK(e);
+  return APT_OK;
+}
+
+apt_status apt_tile_offsets(apt_ctx *ctx, uint32_t *off, uint64_t cap,
+                            uint32_t *ntiles, int tile_extent[3]) {
+  CHECK_CTX(ctx);
+  if (!ntiles) return APT_ERR_INVALID_ARG;
+  *ntiles = 0;
+  if (tile_extent)
+    for (int d = 0; d < 3; d++) tile_extent[d] = ctx->tiles.T[d];
+  if (!ctx->tiling || !ctx->tiles_valid) return APT_OK;
+  *ntiles = ctx->tiles.ntiles;
+  if (!off) return APT_OK;
+  if (cap < (uint64_t)ctx->tiles.ntiles + 1)
+    return ctx->fail(APT_ERR_CAPACITY, "tile_offsets: need %u entries",
+                     ctx->tiles.ntiles + 1);
+  HIPCHK(hipMemcpyAsync(off, ctx->tile_off, (ctx->tiles.ntiles + 1) * 4ull,
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   return APT_OK;
 }
 

@@ -67,6 +67,7 @@ struct TileInfo {
   int halo;       // cells of LDS halo around a tile
   const uint32_t *tile_off;
   uint32_t ntiles;
+  uint64_t tiled_end;  // tile_off[ntiles], known on the host
 };
 
 // int64 accumulators of the mixed form: J[c] always, R[s] when rho != 0

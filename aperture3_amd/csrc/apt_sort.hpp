@@ -24,6 +24,10 @@ hipError_t sort_keys(SortWork &w, const uint32_t *cell, uint64_t n,
                      uint32_t empty_key, const uint32_t *tile_key_lut,
                      hipStream_t st);
 
+// radix sort of (keys_in, perm_in) -> (keys_out, perm_out) on `bits` bits
+hipError_t hipcub_sort_pairs(SortWork &w, uint64_t n, int bits,
+                             hipStream_t st);
+
 // dst[i] = src[perm_out[i]]
 template <typename T>
 hipError_t gather_field(SortWork &w, const T *src, T *dst, uint64_t n,

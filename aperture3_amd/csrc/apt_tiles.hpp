@@ -1,0 +1,70 @@
+// apt_tiles.hpp -- tiling of the mixed form: the tiled push, the tile-major
+// loader and the deterministic tile-binning sort.
+#pragma once
+#include "apt_common.hpp"
+#include "apt_kernels.hpp"
+
+namespace apt {
+
+bool tile_shape_supported(int dim, const int T[3], int halo);
+
+hipError_t launch_push_deposit_tiled(const Mesh &m, const Species &sp,
+                                     float dt, int S, const PtcF32 &p,
+                                     const FieldsF32 &f, const AccMixed &a,
+                                     const TileInfo &ti,
+                                     unsigned long long *counters,
+                                     hipStream_t st);
+
+// untiled push for the slots [first, first + n)
+hipError_t launch_push_deposit_mixed_range(const Mesh &m, const Species &sp,
+                                           float dt, int S, const PtcF32 &p,
+                                           const FieldsF32 &f,
+                                           const AccMixed &a,
+                                           unsigned long long *counters,
+                                           uint64_t first, uint64_t n,
+                                           hipStream_t st);
+
+// loader writing tile-major, cell-interleaved slots; tile t holds
+// cells_per_tile * 2 * ppc particles starting at t * that
+hipError_t launch_load_tiled_f32(const Mesh &m, const LoadParams &lp,
+                                 const TileInfo &ti, const PtcF32 &p,
+                                 unsigned long long *counters,
+                                 hipStream_t st);
+
+// Scratch of the tile-binning sort.
+struct TileSortWork {
+  uint32_t *cnt = nullptr;       // [nsrc][KD] particles per (source, dir)
+  uint32_t *base = nullptr;      // [nsrc][KD] first slot of each stream
+  uint32_t *newcnt = nullptr;    // [ntiles + 1]
+  uint32_t *newoff = nullptr;    // [ntiles + 1]
+  unsigned long long *far = nullptr;  // particles more than one tile away
+  void *scan_tmp = nullptr;
+  size_t scan_bytes = 0;
+  uint32_t ntiles = 0;
+  hipError_t ensure(uint32_t ntiles, int dim);
+  void release();
+};
+
+// Pass 1 of the tile sort: counts per (source tile, direction) and the
+// number of particles that moved further than one tile.  Fills ws.far.
+hipError_t tile_sort_count(const Mesh &m, const TileInfo &ti,
+                           const uint32_t *cell, uint64_t tail_end,
+                           TileSortWork &ws, hipStream_t st);
+// Pass 2: new tile offsets (ws.newoff, ntiles + 1 entries) and stream bases.
+hipError_t tile_sort_offsets(const Mesh &m, const TileInfo &ti,
+                             TileSortWork &ws, hipStream_t st);
+// Pass 3: ordered scatter of every SoA field from src to dst.
+hipError_t tile_sort_scatter(const Mesh &m, const TileInfo &ti,
+                             const PtcF32 &src, const PtcF32 &dst,
+                             uint64_t tail_end, TileSortWork &ws,
+                             hipStream_t st);
+// key = tile of the cell (empty: ntiles), for the stable fallback sort
+hipError_t tile_keys(const Mesh &m, const TileInfo &ti, const uint32_t *cell,
+                     uint64_t n, uint32_t *keys, uint32_t *perm,
+                     hipStream_t st);
+// tile offsets from sorted keys (off has ntiles + 1 entries)
+hipError_t tile_offsets_from_sorted(const uint32_t *keys, uint64_t n,
+                                    uint32_t ntiles, uint32_t *off,
+                                    hipStream_t st);
+
+}  // namespace apt

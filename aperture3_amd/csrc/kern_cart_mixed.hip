@@ -7,6 +7,7 @@
 // deterministic and charge conserving to the round-off of the FP32 shapes.
 #include "apt_common.hpp"
 #include "apt_kernels.hpp"
+#include "apt_tiles.hpp"
 
 namespace apt {
 
@@ -86,7 +87,7 @@ __device__ __forceinline__ void vay_pushf(float &p1, float &p2, float &p3,
 
 __device__ __forceinline__ void acc_fx(long long *__restrict__ acc,
                                        uint64_t o, float c, float scale) {
-  long long v = __float2ll_rn(c * scale);
+  long long v = (long long)__float2int_rn(c * scale);
   if (v != 0)
     __hip_atomic_fetch_add((unsigned long long *)(acc + o),
                            (unsigned long long)v, __ATOMIC_RELAXED,
@@ -113,12 +114,13 @@ __global__ void __launch_bounds__(256)
     k_push_deposit_mixed(Mesh m, Species sp, float dt, float scale, PtcF32 p,
                          FieldsF32 f, AccMixed a,
                          unsigned long long *__restrict__ counters,
-                         uint64_t n) {
+                         uint64_t first, uint64_t n) {
   long long *__restrict__ J0 = a.J[0];
   long long *__restrict__ J1 = a.J[1];
   long long *__restrict__ J2 = a.J[2];
-  uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint64_t idx = first + t;
   uint32_t c = p.cell[idx];
   if (c == MAX_CELL) return;
   int c1, c2, c3;
@@ -294,26 +296,46 @@ __global__ void k_finalize_mixed(long long *__restrict__ acc,
   acc[i] = 0;
 }
 
+hipError_t launch_push_deposit_mixed_range(const Mesh &m, const Species &sp,
+                                           float dt, int S, const PtcF32 &p,
+                                           const FieldsF32 &f,
+                                           const AccMixed &a,
+                                           unsigned long long *counters,
+                                           uint64_t first, uint64_t n,
+                                           hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  float scale = ldexpf(1.0f, S);
+  dim3 block(256), grid((unsigned)((n + 255) / 256));
+  if (m.dim == 1)
+    k_push_deposit_mixed<1><<<grid, block, 0, st>>>(m, sp, dt, scale, p, f, a,
+                                                    counters, first, n);
+  else if (m.dim == 2)
+    k_push_deposit_mixed<2><<<grid, block, 0, st>>>(m, sp, dt, scale, p, f, a,
+                                                    counters, first, n);
+  else
+    k_push_deposit_mixed<3><<<grid, block, 0, st>>>(m, sp, dt, scale, p, f, a,
+                                                    counters, first, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_push_deposit_mixed(const Mesh &m, const Species &sp,
                                      float dt, int S, const PtcF32 &p,
                                      const FieldsF32 &f, const AccMixed &a,
                                      const TileInfo &tiles,
                                      unsigned long long *counters,
                                      uint64_t n, hipStream_t st) {
-  (void)tiles;
-  if (n == 0) return hipSuccess;
-  float scale = ldexpf(1.0f, S);
-  dim3 block(256), grid((unsigned)((n + 255) / 256));
-  if (m.dim == 1)
-    k_push_deposit_mixed<1><<<grid, block, 0, st>>>(m, sp, dt, scale, p, f, a,
-                                                    counters, n);
-  else if (m.dim == 2)
-    k_push_deposit_mixed<2><<<grid, block, 0, st>>>(m, sp, dt, scale, p, f, a,
-                                                    counters, n);
-  else
-    k_push_deposit_mixed<3><<<grid, block, 0, st>>>(m, sp, dt, scale, p, f, a,
-                                                    counters, n);
-  return hipGetLastError();
+  if (tiles.tile_off == nullptr || tiles.ntiles == 0)
+    return launch_push_deposit_mixed_range(m, sp, dt, S, p, f, a, counters, 0,
+                                           n, st);
+  hipError_t e = launch_push_deposit_tiled(m, sp, dt, S, p, f, a, tiles,
+                                           counters, st);
+  if (e != hipSuccess) return e;
+  // slots past the tiled range (appended since the last sort)
+  if (n > tiles.tiled_end)
+    e = launch_push_deposit_mixed_range(m, sp, dt, S, p, f, a, counters,
+                                        tiles.tiled_end, n - tiles.tiled_end,
+                                        st);
+  return e;
 }
 
 hipError_t launch_finalize_mixed(long long *acc, float *out, uint64_t n,

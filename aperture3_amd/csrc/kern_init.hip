@@ -7,6 +7,7 @@
 // FP64 loader reproduces the CPU restatement bit for bit.
 #include "apt_common.hpp"
 #include "apt_kernels.hpp"
+#include "apt_tiles.hpp"
 
 namespace apt {
 
@@ -163,6 +164,57 @@ __global__ void k_load_f32(Mesh m, LoadParams lp, PtcF32 p,
   p.u3[o] = (float)q.u[2];
   p.w[o] = (float)lp.weight;
   p.flag[o] = q.flag;
+}
+
+// tile-major loader: slot t -> (tile, cell of the tile, k-th particle of
+// the cell); the particle written is the same one load_one produces for
+// the cell-major index, so both loaders draw the same particle set.
+__global__ void k_load_tiled_f32(Mesh m, LoadParams lp, TileInfo ti, PtcF32 p,
+                                 unsigned long long *counters, uint64_t n) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint64_t cpt = (uint64_t)ti.T[0] * ti.T[1] * ti.T[2];
+  const uint64_t per_tile = cpt * 2 * (uint64_t)lp.ppc;
+  const uint64_t tile = t / per_tile, r = t % per_tile;
+  const uint64_t cl = r % cpt, k = r / cpt;
+  const int tx = (int)(tile % (uint64_t)ti.nt[0]);
+  const int ty = (int)((tile / (uint64_t)ti.nt[0]) % (uint64_t)ti.nt[1]);
+  const int tz = (int)(tile / ((uint64_t)ti.nt[0] * ti.nt[1]));
+  const int lx = (int)(cl % (uint64_t)ti.T[0]);
+  const int ly = (int)((cl / (uint64_t)ti.T[0]) % (uint64_t)ti.T[1]);
+  const int lz = (int)(cl / ((uint64_t)ti.T[0] * ti.T[1]));
+  const uint64_t i = (uint64_t)tx * ti.T[0] + lx;
+  const uint64_t j = (uint64_t)ty * ti.T[1] + ly;
+  const uint64_t kk = (uint64_t)tz * ti.T[2] + lz;
+  const uint64_t rank = i + (uint64_t)m.N[0] * (j + (uint64_t)m.N[1] * kk);
+  const uint64_t cell_major = 2 * (rank * (uint64_t)lp.ppc + k / 2) + (k & 1);
+  LoadedPtc q;
+  if (!load_one(m, lp, cell_major, q)) atomicAdd(&counters[CTR_OVERFLOW], 1ull);
+  uint64_t o = lp.base + t;
+  p.cell[o] = q.cell;
+  p.x1[o] = (float)q.x[0];
+  p.x2[o] = (float)q.x[1];
+  p.x3[o] = (float)q.x[2];
+  if (p.x1[o] >= 1.0f) p.x1[o] = 0x1.fffffep-1f;
+  if (p.x2[o] >= 1.0f) p.x2[o] = 0x1.fffffep-1f;
+  if (p.x3[o] >= 1.0f) p.x3[o] = 0x1.fffffep-1f;
+  p.u1[o] = (float)q.u[0];
+  p.u2[o] = (float)q.u[1];
+  p.u3[o] = (float)q.u[2];
+  p.w[o] = (float)lp.weight;
+  p.flag[o] = q.flag;
+}
+
+hipError_t launch_load_tiled_f32(const Mesh &m, const LoadParams &lp,
+                                 const TileInfo &ti, const PtcF32 &p,
+                                 unsigned long long *counters,
+                                 hipStream_t st) {
+  uint64_t n = (uint64_t)m.N[0] * (m.dims[1] - 2 * m.guard[1]) *
+               (m.dims[2] - 2 * m.guard[2]) * (uint64_t)lp.ppc * 2;
+  if (n == 0) return hipSuccess;
+  k_load_tiled_f32<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(m, lp, ti, p,
+                                                               counters, n);
+  return hipGetLastError();
 }
 
 static uint64_t bulk_cells(const Mesh &m) {

@@ -112,6 +112,14 @@ hipError_t sort_keys(SortWork &w, const uint32_t *cell, uint64_t n,
   return hipGetLastError();
 }
 
+hipError_t hipcub_sort_pairs(SortWork &w, uint64_t n, int bits,
+                             hipStream_t st) {
+  size_t bytes = w.temp_bytes;
+  return hipcub::DeviceRadixSort::SortPairs(w.temp, bytes, w.keys_in,
+                                            w.keys_out, w.perm_in,
+                                            w.perm_out, (int)n, 0, bits, st);
+}
+
 template <typename T>
 hipError_t gather_field(SortWork &w, const T *src, T *dst, uint64_t n,
                         hipStream_t st) {

@@ -1,0 +1,515 @@
+// kern_tile_mixed.hip -- the tiled mixed-precision particle update and the
+// deterministic tile-binning sort that feeds it.
+//
+// Particles are binned by tile (T1 x T2 [x T3] cells).  One workgroup owns
+// one tile: it stages the E/B components of the tile plus an H-cell halo in
+// LDS, zeroes an LDS current window of the same extent, pushes every particle
+// binned to the tile (push, move, Esirkepov deposit as in
+// ptc_updater.cu:21-499), accumulates the current with LDS int64 atomics and
+// finally flushes the window with one global int64 atomic per non-zero cell
+// and component.  Particles that drifted out of the window since the last
+// sort take the global path (global gathers and atomics) and are counted.
+//
+// Because every contribution is rounded to 2^-S and summed as an integer,
+// the current -- and so the whole state -- is bit-reproducible whatever the
+// order of particles or workgroups.
+//
+// Workgroup -> tile mapping is XCD aware: workgroups are dispatched
+// round-robin over the 8 XCDs, so block b runs on XCD b % 8; giving XCD x the
+// contiguous tile range [x n/8, (x+1) n/8) keeps the halo reads of
+// neighbouring tiles in the same L2.
+#include <hipcub/hipcub.hpp>
+
+#include "apt_common.hpp"
+#include "apt_kernels.hpp"
+#include "apt_tiles.hpp"
+
+namespace apt {
+
+#pragma clang fp contract(fast)
+
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t ntiles) {
+  if ((ntiles & 7u) != 0) return b;
+  uint32_t per = ntiles >> 3;
+  return (b & 7u) * per + (b >> 3);
+}
+
+// index of a possibly out-of-range mesh coordinate folded into the bulk
+__device__ __forceinline__ int fold(int i, int g, int N) {
+  int r = (i - g) % N;
+  if (r < 0) r += N;
+  return r + g;
+}
+
+__device__ __forceinline__ void tapsf(float x, int st, int &o, float &w0,
+                                      float &w1) {
+  if (st) {
+    o = -1;
+    w0 = 1.0f - x;
+    w1 = x;
+  } else if (x < 0.5f) {
+    o = -1;
+    w0 = 0.5f - x;
+    w1 = 0.5f + x;
+  } else {
+    o = 0;
+    w0 = 1.5f - x;
+    w1 = x - 0.5f;
+  }
+}
+
+// 2D CIC gather from a row-major array with row stride `ld` at (i, j)
+__device__ __forceinline__ float gather2(const float *f, int ld, int i, int j,
+                                         float x1, float x2, int st) {
+  int o1, o2;
+  float a0, a1, b0, b1;
+  tapsf(x1, st & 1, o1, a0, a1);
+  tapsf(x2, (st >> 1) & 1, o2, b0, b1);
+  const float *q = f + (j + o2) * ld + (i + o1);
+  return (q[0] * a0 + q[1] * a1) * b0 + (q[ld] * a0 + q[ld + 1] * a1) * b1;
+}
+
+__device__ __forceinline__ float gather3g(const float *__restrict__ f,
+                                          int64_t d0, int64_t d01, int i,
+                                          int j, int k, float x1, float x2,
+                                          float x3, int st) {
+  int o1, o2, o3;
+  float a0, a1, b0, b1, g0, g1;
+  tapsf(x1, st & 1, o1, a0, a1);
+  tapsf(x2, (st >> 1) & 1, o2, b0, b1);
+  tapsf(x3, (st >> 2) & 1, o3, g0, g1);
+  const float *p = f + (int64_t)(k + o3) * d01 + (int64_t)(j + o2) * d0 +
+                   (i + o1);
+  float lo =
+      (p[0] * a0 + p[1] * a1) * b0 + (p[d0] * a0 + p[d0 + 1] * a1) * b1;
+  const float *q = p + d01;
+  float hi =
+      (q[0] * a0 + q[1] * a1) * b0 + (q[d0] * a0 + q[d0 + 1] * a1) * b1;
+  return lo * g0 + hi * g1;
+}
+
+__device__ __forceinline__ void vay_push_f(float &p1, float &p2, float &p3,
+                                           float gamma, float E1, float E2,
+                                           float E3, float B1, float B2,
+                                           float B3) {
+  float ig = 1.0f / gamma;
+  float up1 = p1 + 2.0f * E1 + (p2 * B3 - p3 * B2) * ig;
+  float up2 = p2 + 2.0f * E2 + (p3 * B1 - p1 * B3) * ig;
+  float up3 = p3 + 2.0f * E3 + (p1 * B2 - p2 * B1) * ig;
+  float tt = B1 * B1 + B2 * B2 + B3 * B3;
+  float ut = up1 * B1 + up2 * B2 + up3 * B3;
+  float sigma = 1.0f + up1 * up1 + up2 * up2 + up3 * up3 - tt;
+  float inv_gamma2 =
+      2.0f / (sigma + sqrtf(sigma * sigma + 4.0f * (tt + ut * ut)));
+  float s = 1.0f / (1.0f + inv_gamma2 * tt);
+  float ign = sqrtf(inv_gamma2);
+  p1 = (up1 + B1 * ut * inv_gamma2 + (up2 * B3 - up3 * B2) * ign) * s;
+  p2 = (up2 + B2 * ut * inv_gamma2 + (up3 * B1 - up1 * B3) * ign) * s;
+  p3 = (up3 + B3 * ut * inv_gamma2 + (up1 * B2 - up2 * B1) * ign) * s;
+}
+
+__device__ __forceinline__ long long fxq(float c, float scale) {
+  return (long long)__float2int_rn(c * scale);
+}
+
+__device__ __forceinline__ void lds_add(unsigned long long *a, long long v) {
+  if (v != 0)
+    __hip_atomic_fetch_add(a, (unsigned long long)v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void glb_add(long long *a, long long v) {
+  if (v != 0)
+    __hip_atomic_fetch_add((unsigned long long *)a, (unsigned long long)v,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------
+// 2D tiled push + deposit
+// ---------------------------------------------------------------------
+template <int T1, int T2, int H, int NT, bool RHO>
+__global__ void __launch_bounds__(NT)
+    k_push_deposit_tile2d(Mesh m, Species sp, float dt, float scale,
+                          PtcF32 p, FieldsF32 f, AccMixed a, TileInfo ti,
+                          unsigned long long *__restrict__ counters) {
+  constexpr int W1 = T1 + 2 * H, W2 = T2 + 2 * H, WN = W1 * W2;
+  __shared__ float sF[6][WN];
+  __shared__ unsigned long long sJ[3][WN];
+  __shared__ unsigned long long sR[RHO ? 2 : 1][RHO ? WN : 1];
+
+  const uint32_t t = xcd_tile(blockIdx.x, ti.ntiles);
+  const int tx = (int)(t % (uint32_t)ti.nt[0]);
+  const int ty = (int)(t / (uint32_t)ti.nt[0]);
+  const int g0 = m.guard[0], g1 = m.guard[1];
+  const int N0 = m.N[0], N1 = m.N[1], d0 = m.dims[0];
+  const int w0 = g0 + tx * T1 - H;  // window origin (mesh index)
+  const int w1 = g1 + ty * T2 - H;
+  constexpr bool rho = RHO;
+
+  for (int i = threadIdx.x; i < WN; i += NT) {
+    int l1 = i % W1, l2 = i / W1;
+    int64_t o = (int64_t)fold(w0 + l1, g0, N0) +
+                (int64_t)fold(w1 + l2, g1, N1) * d0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      sF[c][i] = f.E[c][o];
+      sF[3 + c][i] = f.B[c][o];
+      sJ[c][i] = 0ull;
+    }
+    if (rho) {
+      sR[0][i] = 0ull;
+      sR[1][i] = 0ull;
+    }
+  }
+  __syncthreads();
+
+  const uint32_t beg = ti.tile_off[t], end = ti.tile_off[t + 1];
+  const float idx0 = (float)m.inv_delta[0] * dt;
+  const float idx1 = (float)m.inv_delta[1] * dt;
+  unsigned long long outside = 0;
+  for (uint32_t idx = beg + threadIdx.x; idx < end; idx += NT) {
+    uint32_t c = p.cell[idx];
+    if (c == MAX_CELL) continue;
+    int c1 = (int)(c % (uint32_t)d0);
+    int c2 = (int)(c / (uint32_t)d0);
+    uint32_t flag = p.flag[idx];
+    int s = ptc_species(flag);
+    float x1 = p.x1[idx], x2 = p.x2[idx], x3 = p.x3[idx];
+    float u1 = p.u1[idx], u2 = p.u2[idx], u3 = p.u3[idx];
+    float w = p.w[idx];
+    // local coordinates in the window, minimal image for periodic wrap
+    int l1 = c1 - w0, l2 = c2 - w1;
+    if (m.periodic[0]) {
+      if (l1 < -H) l1 += N0;
+      else if (l1 >= N0 - H) l1 -= N0;
+    }
+    if (m.periodic[1]) {
+      if (l2 < -H) l2 += N1;
+      else if (l2 >= N1 - H) l2 -= N1;
+    }
+    const bool in = (l1 >= 2) & (l1 <= W1 - 3) & (l2 >= 2) & (l2 <= W2 - 3);
+    outside += in ? 0 : 1;
+    float gamma = sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+    if (!(flag & FLAG_IGNORE_EM)) {
+      float qm = (float)(0.5 * sp.q[s] / sp.m[s]) * dt;
+      float F[6];
+      if (in) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+          const int st = k == 0 ? ST_E1 : k == 1 ? ST_E2 : k == 2 ? ST_E3
+                       : k == 3 ? ST_B1 : k == 4 ? ST_B2 : ST_B3;
+          F[k] = gather2(sF[k], W1, l1, l2, x1, x2, st) * qm;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+          const int st = k == 0 ? ST_E1 : k == 1 ? ST_E2 : k == 2 ? ST_E3
+                       : k == 3 ? ST_B1 : k == 4 ? ST_B2 : ST_B3;
+          const float *src = k < 3 ? f.E[k] : f.B[k - 3];
+          F[k] = gather2(src, d0, c1, c2, x1, x2, st) * qm;
+        }
+      }
+      vay_push_f(u1, u2, u3, gamma, F[0], F[1], F[2], F[3], F[4], F[5]);
+      gamma = sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+      p.u1[idx] = u1;
+      p.u2[idx] = u2;
+      p.u3[idx] = u3;
+    }
+    const float ig = 1.0f / gamma;
+    const float v1 = u1 * ig, v2 = u2 * ig, v3 = u3 * ig;
+    float n1 = x1 + v1 * idx0;
+    float n2 = x2 + v2 * idx1;
+    int dc1 = (int)floorf(n1);
+    int dc2 = (int)floorf(n2);
+    n1 -= (float)dc1;
+    n2 -= (float)dc2;
+    if (n1 >= 1.0f) { n1 -= 1.0f; dc1 += 1; }
+    if (n2 >= 1.0f) { n2 -= 1.0f; dc2 += 1; }
+    if (dc1 > 1 || dc1 < -1 || dc2 > 1 || dc2 < -1) {
+      atomicAdd(&counters[CTR_MULTICELL], 1ull);
+      dc1 = max(-1, min(1, dc1));
+      dc2 = max(-1, min(1, dc2));
+    }
+    p.cell[idx] =
+        (uint32_t)m.idx(m.wrap(0, c1 + dc1), m.wrap(1, c2 + dc2), 0);
+    p.x1[idx] = n1;
+    p.x2[idx] = n2;
+    p.x3[idx] = x3 + v3 * dt;
+    if (flag & FLAG_IGNORE_CURRENT) continue;
+
+    // Esirkepov weights over node offsets -2..1 of the old cell
+    float sx0[4] = {0.f, 1.0f - x1, x1, 0.f};
+    float sy0[4] = {0.f, 1.0f - x2, x2, 0.f};
+    float sx1[4] = {0.f, 0.f, 0.f, 0.f}, sy1[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      sx1[q] = (q == 1 + dc1) ? 1.0f - n1 : ((q == 2 + dc1) ? n1 : 0.0f);
+      sy1[q] = (q == 1 + dc2) ? 1.0f - n2 : ((q == 2 + dc2) ? n2 : 0.0f);
+    }
+    const float qw = (float)sp.q[s] * w;
+    float djy[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int tj = 0; tj < 4; tj++) {
+      const float ay = 0.5f * (sy0[tj] + sy1[tj]);
+      const float dsy = sy1[tj] - sy0[tj];
+      float djx = 0.0f;
+#pragma unroll
+      for (int tk = 0; tk < 4; tk++) {
+        const int di = tk - 2, dj = tj - 2;
+        djx += (sx1[tk] - sx0[tk]) * ay;
+        djy[tk] += dsy * 0.5f * (sx0[tk] + sx1[tk]);
+        const float cen = (2.0f * sx1[tk] * sy1[tj] + sx0[tk] * sy1[tj] +
+                           sx1[tk] * sy0[tj] + 2.0f * sx0[tk] * sy0[tj]) *
+                          (1.0f / 6.0f);
+        // J1 at (i+1, j), J2 at (i, j+1), J3 at (i, j); reference signs
+        // (ptc_updater.cu:345-376): J = q w x (movement)
+        const long long v1q = tk < 3 ? fxq(qw * djx, scale) : 0;
+        const long long v2q = tj < 3 ? fxq(qw * djy[tk], scale) : 0;
+        const long long v3q = fxq(qw * v3 * cen, scale);
+        const long long vr = rho ? fxq(qw * sx1[tk] * sy1[tj], scale) : 0;
+        if (in) {
+          const int b = (l1 + di) + (l2 + dj) * W1;
+          lds_add(&sJ[0][b + 1], v1q);
+          lds_add(&sJ[1][b + W1], v2q);
+          lds_add(&sJ[2][b], v3q);
+          if (rho) {
+            if (s < 2)
+              lds_add(&sR[s][b], vr);
+            else
+              glb_add(a.R[s] + m.widx(c1 + di, c2 + dj, 0), vr);
+          }
+        } else {
+          glb_add(a.J[0] + m.widx(c1 + di + 1, c2 + dj, 0), v1q);
+          glb_add(a.J[1] + m.widx(c1 + di, c2 + dj + 1, 0), v2q);
+          const uint64_t o = m.widx(c1 + di, c2 + dj, 0);
+          glb_add(a.J[2] + o, v3q);
+          if (rho) glb_add(a.R[s] + o, vr);
+        }
+      }
+    }
+  }
+  if (outside) atomicAdd(&counters[CTR_OUTSIDE], outside);
+  __syncthreads();
+  for (int i = threadIdx.x; i < WN; i += NT) {
+    int l1 = i % W1, l2 = i / W1;
+    int64_t o = (int64_t)fold(w0 + l1, g0, N0) +
+                (int64_t)fold(w1 + l2, g1, N1) * d0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) glb_add(a.J[c] + o, (long long)sJ[c][i]);
+    if (rho) {
+      glb_add(a.R[0] + o, (long long)sR[0][i]);
+      glb_add(a.R[1] + o, (long long)sR[1][i]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// 3D tiled push + deposit: fields gathered through L1/L2, current in an
+// LDS int64 window.
+// ---------------------------------------------------------------------
+template <int T, int H, int NT>
+__global__ void __launch_bounds__(NT)
+    k_push_deposit_tile3d(Mesh m, Species sp, float dt, float scale,
+                          PtcF32 p, FieldsF32 f, AccMixed a, TileInfo ti,
+                          unsigned long long *__restrict__ counters) {
+  constexpr int W = T + 2 * H, WN = W * W * W;
+  __shared__ unsigned long long sJ[3][WN];
+
+  const uint32_t t = xcd_tile(blockIdx.x, ti.ntiles);
+  const int tx = (int)(t % (uint32_t)ti.nt[0]);
+  const int ty = (int)((t / (uint32_t)ti.nt[0]) % (uint32_t)ti.nt[1]);
+  const int tz = (int)(t / ((uint32_t)ti.nt[0] * (uint32_t)ti.nt[1]));
+  const int g0 = m.guard[0], g1 = m.guard[1], g2 = m.guard[2];
+  const int N0 = m.N[0], N1 = m.N[1], N2 = m.N[2];
+  const int64_t d0 = m.dims[0], d01 = (int64_t)m.dims[0] * m.dims[1];
+  const int w0 = g0 + tx * T - H, w1 = g1 + ty * T - H, w2 = g2 + tz * T - H;
+
+  for (int i = threadIdx.x; i < WN; i += NT) {
+    sJ[0][i] = 0ull;
+    sJ[1][i] = 0ull;
+    sJ[2][i] = 0ull;
+  }
+  __syncthreads();
+
+  const uint32_t beg = ti.tile_off[t], end = ti.tile_off[t + 1];
+  const float idx0 = (float)m.inv_delta[0] * dt;
+  const float idx1 = (float)m.inv_delta[1] * dt;
+  const float idx2 = (float)m.inv_delta[2] * dt;
+  unsigned long long outside = 0;
+  for (uint32_t idx = beg + threadIdx.x; idx < end; idx += NT) {
+    uint32_t c = p.cell[idx];
+    if (c == MAX_CELL) continue;
+    int c1 = (int)(c % (uint32_t)d0);
+    int c2 = (int)((c / (uint32_t)d0) % (uint32_t)m.dims[1]);
+    int c3 = (int)(c / (uint32_t)d01);
+    uint32_t flag = p.flag[idx];
+    int s = ptc_species(flag);
+    float x1 = p.x1[idx], x2 = p.x2[idx], x3 = p.x3[idx];
+    float u1 = p.u1[idx], u2 = p.u2[idx], u3 = p.u3[idx];
+    float w = p.w[idx];
+    int l1 = c1 - w0, l2 = c2 - w1, l3 = c3 - w2;
+    if (m.periodic[0]) { if (l1 < -H) l1 += N0; else if (l1 >= N0 - H) l1 -= N0; }
+    if (m.periodic[1]) { if (l2 < -H) l2 += N1; else if (l2 >= N1 - H) l2 -= N1; }
+    if (m.periodic[2]) { if (l3 < -H) l3 += N2; else if (l3 >= N2 - H) l3 -= N2; }
+    const bool in = (l1 >= 2) & (l1 <= W - 3) & (l2 >= 2) & (l2 <= W - 3) &
+                    (l3 >= 2) & (l3 <= W - 3);
+    outside += in ? 0 : 1;
+    float gamma = sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+    if (!(flag & FLAG_IGNORE_EM)) {
+      float qm = (float)(0.5 * sp.q[s] / sp.m[s]) * dt;
+      float E1 = gather3g(f.E[0], d0, d01, c1, c2, c3, x1, x2, x3, ST_E1) * qm;
+      float E2 = gather3g(f.E[1], d0, d01, c1, c2, c3, x1, x2, x3, ST_E2) * qm;
+      float E3 = gather3g(f.E[2], d0, d01, c1, c2, c3, x1, x2, x3, ST_E3) * qm;
+      float B1 = gather3g(f.B[0], d0, d01, c1, c2, c3, x1, x2, x3, ST_B1) * qm;
+      float B2 = gather3g(f.B[1], d0, d01, c1, c2, c3, x1, x2, x3, ST_B2) * qm;
+      float B3 = gather3g(f.B[2], d0, d01, c1, c2, c3, x1, x2, x3, ST_B3) * qm;
+      vay_push_f(u1, u2, u3, gamma, E1, E2, E3, B1, B2, B3);
+      gamma = sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+      p.u1[idx] = u1;
+      p.u2[idx] = u2;
+      p.u3[idx] = u3;
+    }
+    const float ig = 1.0f / gamma;
+    const float v1 = u1 * ig, v2 = u2 * ig, v3 = u3 * ig;
+    float n1 = x1 + v1 * idx0, n2 = x2 + v2 * idx1, n3 = x3 + v3 * idx2;
+    int dc1 = (int)floorf(n1), dc2 = (int)floorf(n2), dc3 = (int)floorf(n3);
+    n1 -= (float)dc1;
+    n2 -= (float)dc2;
+    n3 -= (float)dc3;
+    if (n1 >= 1.0f) { n1 -= 1.0f; dc1 += 1; }
+    if (n2 >= 1.0f) { n2 -= 1.0f; dc2 += 1; }
+    if (n3 >= 1.0f) { n3 -= 1.0f; dc3 += 1; }
+    if (dc1 > 1 || dc1 < -1 || dc2 > 1 || dc2 < -1 || dc3 > 1 || dc3 < -1) {
+      atomicAdd(&counters[CTR_MULTICELL], 1ull);
+      dc1 = max(-1, min(1, dc1));
+      dc2 = max(-1, min(1, dc2));
+      dc3 = max(-1, min(1, dc3));
+    }
+    p.cell[idx] = (uint32_t)m.idx(m.wrap(0, c1 + dc1), m.wrap(1, c2 + dc2),
+                                  m.wrap(2, c3 + dc3));
+    p.x1[idx] = n1;
+    p.x2[idx] = n2;
+    p.x3[idx] = n3;
+    if (flag & FLAG_IGNORE_CURRENT) continue;
+
+    float sx0[4] = {0.f, 1.0f - x1, x1, 0.f};
+    float sy0[4] = {0.f, 1.0f - x2, x2, 0.f};
+    float sz0[4] = {0.f, 1.0f - x3, x3, 0.f};
+    float sx1[4], sy1[4], sz1[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      sx1[q] = (q == 1 + dc1) ? 1.0f - n1 : ((q == 2 + dc1) ? n1 : 0.0f);
+      sy1[q] = (q == 1 + dc2) ? 1.0f - n2 : ((q == 2 + dc2) ? n2 : 0.0f);
+      sz1[q] = (q == 1 + dc3) ? 1.0f - n3 : ((q == 2 + dc3) ? n3 : 0.0f);
+    }
+    const float qw = (float)sp.q[s] * w;
+    float djz[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) djz[q][r] = 0.0f;
+#pragma unroll
+    for (int tk = 0; tk < 4; tk++) {
+      float djy[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int tj = 0; tj < 4; tj++) {
+        const float cyz = (2.0f * sy1[tj] * sz1[tk] + sy0[tj] * sz1[tk] +
+                           sy1[tj] * sz0[tk] + 2.0f * sy0[tj] * sz0[tk]) *
+                          (1.0f / 6.0f);
+        float djx = 0.0f;
+#pragma unroll
+        for (int ti2 = 0; ti2 < 4; ti2++) {
+          const int di = ti2 - 2, dj = tj - 2, dk = tk - 2;
+          djx += (sx1[ti2] - sx0[ti2]) * cyz;
+          const float cxz = (2.0f * sx1[ti2] * sz1[tk] + sx0[ti2] * sz1[tk] +
+                             sx1[ti2] * sz0[tk] + 2.0f * sx0[ti2] * sz0[tk]) *
+                            (1.0f / 6.0f);
+          djy[ti2] += (sy1[tj] - sy0[tj]) * cxz;
+          const float cxy = (2.0f * sx1[ti2] * sy1[tj] + sx0[ti2] * sy1[tj] +
+                             sx1[ti2] * sy0[tj] + 2.0f * sx0[ti2] * sy0[tj]) *
+                            (1.0f / 6.0f);
+          djz[tj][ti2] += (sz1[tk] - sz0[tk]) * cxy;
+          const long long q1 = ti2 < 3 ? fxq(qw * djx, scale) : 0;
+          const long long q2 = tj < 3 ? fxq(qw * djy[ti2], scale) : 0;
+          const long long q3 = tk < 3 ? fxq(qw * djz[tj][ti2], scale) : 0;
+          if (in) {
+            const int b = (l1 + di) + ((l2 + dj) + (l3 + dk) * W) * W;
+            lds_add(&sJ[0][b + 1], q1);
+            lds_add(&sJ[1][b + W], q2);
+            lds_add(&sJ[2][b + W * W], q3);
+          } else {
+            glb_add(a.J[0] + m.widx(c1 + di + 1, c2 + dj, c3 + dk), q1);
+            glb_add(a.J[1] + m.widx(c1 + di, c2 + dj + 1, c3 + dk), q2);
+            glb_add(a.J[2] + m.widx(c1 + di, c2 + dj, c3 + dk + 1), q3);
+          }
+        }
+      }
+    }
+  }
+  if (outside) atomicAdd(&counters[CTR_OUTSIDE], outside);
+  __syncthreads();
+  for (int i = threadIdx.x; i < WN; i += NT) {
+    int l1 = i % W, l2 = (i / W) % W, l3 = i / (W * W);
+    int64_t o = (int64_t)fold(w0 + l1, g0, N0) +
+                (int64_t)fold(w1 + l2, g1, N1) * d0 +
+                (int64_t)fold(w2 + l3, g2, N2) * d01;
+#pragma unroll
+    for (int c = 0; c < 3; c++) glb_add(a.J[c] + o, (long long)sJ[c][i]);
+  }
+}
+
+hipError_t launch_push_deposit_tiled(const Mesh &m, const Species &sp,
+                                     float dt, int S, const PtcF32 &p,
+                                     const FieldsF32 &f, const AccMixed &a,
+                                     const TileInfo &ti,
+                                     unsigned long long *counters,
+                                     hipStream_t st) {
+  float scale = ldexpf(1.0f, S);
+  if (m.dim == 2) {
+#define LAUNCH2(A, B, HH, NTH)                                              \
+  if (ti.T[0] == A && ti.T[1] == B && ti.halo == HH) {                      \
+    if (a.rho)                                                              \
+      k_push_deposit_tile2d<A, B, HH, NTH, true><<<ti.ntiles, NTH, 0, st>>>( \
+          m, sp, dt, scale, p, f, a, ti, counters);                         \
+    else                                                                    \
+      k_push_deposit_tile2d<A, B, HH, NTH, false><<<ti.ntiles, NTH, 0, st>>>(\
+          m, sp, dt, scale, p, f, a, ti, counters);                         \
+    return hipGetLastError();                                               \
+  }
+    LAUNCH2(16, 16, 6, 256)
+    LAUNCH2(32, 16, 6, 512)
+    LAUNCH2(16, 16, 4, 256)
+    LAUNCH2(8, 8, 4, 256)
+#undef LAUNCH2
+    return hipErrorInvalidValue;
+  } else if (m.dim == 3) {
+    if (ti.T[0] == 8 && ti.halo == 3)
+      k_push_deposit_tile3d<8, 3, 256><<<ti.ntiles, 256, 0, st>>>(
+          m, sp, dt, scale, p, f, a, ti, counters);
+    else if (ti.T[0] == 8 && ti.halo == 4)
+      k_push_deposit_tile3d<8, 4, 256><<<ti.ntiles, 256, 0, st>>>(
+          m, sp, dt, scale, p, f, a, ti, counters);
+    else if (ti.T[0] == 4 && ti.halo == 3)
+      k_push_deposit_tile3d<4, 3, 256><<<ti.ntiles, 256, 0, st>>>(
+          m, sp, dt, scale, p, f, a, ti, counters);
+    else
+      return hipErrorInvalidValue;
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+bool tile_shape_supported(int dim, const int T[3], int halo) {
+  if (dim == 2)
+    return (T[0] == 16 && T[1] == 16 && (halo == 6 || halo == 4)) ||
+           (T[0] == 32 && T[1] == 16 && halo == 6) ||
+           (T[0] == 8 && T[1] == 8 && halo == 4);
+  if (dim == 3)
+    return T[0] == T[1] && T[1] == T[2] &&
+           ((T[0] == 8 && (halo == 3 || halo == 4)) ||
+            (T[0] == 4 && halo == 3));
+  return false;
+}
+
+}  // namespace apt

@@ -1,0 +1,377 @@
+// kern_tile_sort.hip -- deterministic tile-binning sort of the mixed form
+// (the particle_base::sort_by_cell role, cuda/core/particle_base_impl.cu:
+// 435-464, at tile granularity) and the tile-major loader.
+//
+// Between sorts a particle moves less than a tile, so every particle goes
+// from its source tile s to s + dir with dir in {-1,0,1}^d.  Pass 1 counts
+// (source, dir) pairs; pass 2 lays every destination tile out as the
+// concatenation of its incoming streams (staying particles first, then the
+// 3^d - 1 neighbours in a fixed order) and scans the tile sizes; pass 3
+// scatters each source tile's particles in their original order, ranking
+// lanes with wavefront ballots.  The result is a pure function of the input
+// array -- no atomic decides a slot -- and costs one 4-byte read plus one
+// 36-byte read and write per particle.  Empty slots and particles outside the
+// bulk are dropped (compaction).  A particle that moved further than one tile
+// is counted in `far`; the caller then falls back to a stable radix sort by
+// tile key.
+#include <hipcub/hipcub.hpp>
+
+#include "apt_common.hpp"
+#include "apt_kernels.hpp"
+#include "apt_tiles.hpp"
+
+namespace apt {
+
+constexpr int KD_MAX = 27;
+constexpr uint32_t K_NONE = 0xffu;
+constexpr uint32_t K_FAR = 0xfeu;
+
+__device__ __forceinline__ int tile_dir(int d, int nt) {
+  // d = (dest - src) mod nt in [0, nt); -> -1, 0, 1, or 2 for "far"
+  if (d == 0) return 0;
+  if (d == 1) return 1;
+  if (d == nt - 1) return -1;
+  return 2;
+}
+
+template <int DIM>
+__device__ __forceinline__ uint32_t tile_of_cell(const Mesh &m,
+                                                 const TileInfo &ti,
+                                                 uint32_t c, int tc[3]) {
+  int c1, c2, c3;
+  m.split(c, c1, c2, c3);
+  if (!m.in_bulk(c1, c2, c3)) return 0xffffffffu;
+  tc[0] = (c1 - m.guard[0]) / ti.T[0];
+  tc[1] = DIM > 1 ? (c2 - m.guard[1]) / ti.T[1] : 0;
+  tc[2] = DIM > 2 ? (c3 - m.guard[2]) / ti.T[2] : 0;
+  return (uint32_t)(tc[0] + ti.nt[0] * (tc[1] + ti.nt[1] * tc[2]));
+}
+
+// direction code of a particle in source tile src (src == ntiles: the
+// unsorted tail, always far)
+template <int DIM>
+__device__ __forceinline__ uint32_t dir_code(const Mesh &m,
+                                             const TileInfo &ti, uint32_t src,
+                                             uint32_t c) {
+  if (c == MAX_CELL) return K_NONE;
+  int tc[3];
+  uint32_t dst = tile_of_cell<DIM>(m, ti, c, tc);
+  if (dst == 0xffffffffu) return K_NONE;
+  if (src >= ti.ntiles) return K_FAR;
+  int s0 = (int)(src % (uint32_t)ti.nt[0]);
+  int s1 = (int)((src / (uint32_t)ti.nt[0]) % (uint32_t)ti.nt[1]);
+  int s2 = (int)(src / ((uint32_t)ti.nt[0] * (uint32_t)ti.nt[1]));
+  int k = 0, mul = 1;
+  const int ss[3] = {s0, s1, s2};
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    int diff = (tc[d] - ss[d]) % ti.nt[d];
+    if (diff < 0) diff += ti.nt[d];
+    int dd = tile_dir(diff, ti.nt[d]);
+    if (dd == 2) return K_FAR;
+    k += (dd + 1) * mul;
+    mul *= 3;
+  }
+  return (uint32_t)k;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256)
+    k_tile_count(Mesh m, TileInfo ti, const uint32_t *__restrict__ cell,
+                 uint64_t tail_end, uint32_t *__restrict__ cnt,
+                 unsigned long long *__restrict__ far) {
+  constexpr int KD = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
+  __shared__ uint32_t sc[KD + 1];
+  const uint32_t src = blockIdx.x;
+  for (int k = threadIdx.x; k <= KD; k += blockDim.x) sc[k] = 0;
+  __syncthreads();
+  const uint64_t beg = ti.tile_off[src < ti.ntiles ? src : ti.ntiles];
+  const uint64_t end = src < ti.ntiles ? ti.tile_off[src + 1] : tail_end;
+  const int lane = threadIdx.x & 63;
+  for (uint64_t base = beg; base < end; base += blockDim.x) {
+    uint64_t i = base + threadIdx.x;
+    uint32_t k = i < end ? dir_code<DIM>(m, ti, src, cell[i]) : K_NONE;
+    uint32_t kk = k == K_FAR ? (uint32_t)KD : k;
+    // wavefront-aggregated LDS counting (most lanes share one direction)
+    unsigned long long todo = __ballot(kk != K_NONE);
+    while (todo) {
+      int leader = __ffsll((long long)todo) - 1;
+      uint32_t kl = __shfl(kk, leader);
+      unsigned long long mask = __ballot(kk == kl);
+      if (lane == leader) atomicAdd(&sc[kl], (uint32_t)__popcll(mask));
+      todo &= ~mask;
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < KD; k += blockDim.x)
+    cnt[(uint64_t)src * KD + k] = sc[k];
+  if (threadIdx.x == 0 && sc[KD]) atomicAdd(far, (unsigned long long)sc[KD]);
+}
+
+// neighbour tile of t in direction code k (inverse of dir_code)
+template <int DIM>
+__device__ __forceinline__ uint32_t tile_step(const TileInfo &ti, uint32_t t,
+                                              int k, int sign) {
+  int s[3] = {(int)(t % (uint32_t)ti.nt[0]),
+              (int)((t / (uint32_t)ti.nt[0]) % (uint32_t)ti.nt[1]),
+              (int)(t / ((uint32_t)ti.nt[0] * (uint32_t)ti.nt[1]))};
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    int dd = (k % 3) - 1;
+    k /= 3;
+    int v = (s[d] + sign * dd) % ti.nt[d];
+    if (v < 0) v += ti.nt[d];
+    s[d] = v;
+  }
+  return (uint32_t)(s[0] + ti.nt[0] * (s[1] + ti.nt[1] * s[2]));
+}
+
+// stream order inside a destination tile: the staying stream first, then
+// the other directions in code order
+template <int DIM>
+__device__ __forceinline__ int stream_code(int r) {
+  constexpr int KD = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
+  constexpr int center = (KD - 1) / 2;
+  if (r == 0) return center;
+  return r <= center ? r - 1 : r;
+}
+
+// per destination tile: size and relative base of each incoming stream
+template <int DIM>
+__global__ void k_tile_layout(TileInfo ti, const uint32_t *__restrict__ cnt,
+                              uint32_t *__restrict__ base,
+                              uint32_t *__restrict__ newcnt) {
+  constexpr int KD = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
+  uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d > ti.ntiles) return;
+  if (d == ti.ntiles) {
+    newcnt[d] = 0;
+    return;
+  }
+  uint32_t run = 0;
+  for (int r = 0; r < KD; r++) {
+    int k = stream_code<DIM>(r);
+    // the stream arriving at d in direction k left source d - dir(k)
+    uint32_t s = tile_step<DIM>(ti, d, k, -1);
+    base[(uint64_t)s * KD + k] = run;
+    run += cnt[(uint64_t)s * KD + k];
+  }
+  newcnt[d] = run;
+}
+
+template <int DIM>
+__global__ void k_tile_base_abs(TileInfo ti, const uint32_t *__restrict__ off,
+                                uint32_t *__restrict__ base) {
+  constexpr int KD = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)ti.ntiles * KD) return;
+  uint32_t s = (uint32_t)(i / KD);
+  int k = (int)(i % KD);
+  base[i] += off[tile_step<DIM>(ti, s, k, +1)];
+}
+
+template <int DIM, int NT>
+__global__ void __launch_bounds__(NT)
+    k_tile_scatter(Mesh m, TileInfo ti, PtcF32 src, PtcF32 dst,
+                   const uint32_t *__restrict__ base) {
+  constexpr int KD = DIM == 1 ? 3 : DIM == 2 ? 9 : 27;
+  constexpr int NW = NT / 64;
+  __shared__ uint32_t s_run[KD];
+  __shared__ uint32_t s_wcnt[NW][KD];
+  __shared__ uint32_t s_base[KD];
+  const uint32_t t = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int k = threadIdx.x; k < KD; k += NT) {
+    s_run[k] = 0;
+    s_base[k] = base[(uint64_t)t * KD + k];
+  }
+  const uint64_t beg = ti.tile_off[t], end = ti.tile_off[t + 1];
+  const unsigned long long lt = __lanemask_lt();
+  for (uint64_t b0 = beg; b0 < end; b0 += NT) {
+    for (int q = threadIdx.x; q < NW * KD; q += NT) (&s_wcnt[0][0])[q] = 0;
+    __syncthreads();
+    const uint64_t i = b0 + threadIdx.x;
+    uint32_t c = i < end ? src.cell[i] : MAX_CELL;
+    uint32_t k = i < end ? dir_code<DIM>(m, ti, t, c) : K_NONE;
+    if (k == K_FAR) k = K_NONE;  // only reached when the caller checked far == 0
+    uint32_t rank = 0;
+    unsigned long long todo = __ballot(k != K_NONE);
+    while (todo) {
+      int leader = __ffsll((long long)todo) - 1;
+      uint32_t kl = __shfl(k, leader);
+      unsigned long long mask = __ballot(k == kl);
+      if (k == kl) rank = (uint32_t)__popcll(mask & lt);
+      if (lane == leader) s_wcnt[wave][kl] = (uint32_t)__popcll(mask);
+      todo &= ~mask;
+    }
+    __syncthreads();
+    if (threadIdx.x < KD) {
+      uint32_t run = s_run[threadIdx.x];
+      for (int w = 0; w < NW; w++) {
+        uint32_t cw = s_wcnt[w][threadIdx.x];
+        s_wcnt[w][threadIdx.x] = run;
+        run += cw;
+      }
+      s_run[threadIdx.x] = run;
+    }
+    __syncthreads();
+    if (k != K_NONE) {
+      uint64_t o = (uint64_t)s_base[k] + s_wcnt[wave][k] + rank;
+      dst.cell[o] = c;
+      dst.x1[o] = src.x1[i];
+      dst.x2[o] = src.x2[i];
+      dst.x3[o] = src.x3[i];
+      dst.u1[o] = src.u1[i];
+      dst.u2[o] = src.u2[i];
+      dst.u3[o] = src.u3[i];
+      dst.w[o] = src.w[i];
+      dst.flag[o] = src.flag[i];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_tile_keys(Mesh m, TileInfo ti, const uint32_t *cell,
+                            uint64_t n, uint32_t *keys, uint32_t *perm) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c = cell[i];
+  uint32_t k = ti.ntiles;
+  if (c != MAX_CELL) {
+    int tc[3];
+    uint32_t t = m.dim == 3 ? tile_of_cell<3>(m, ti, c, tc)
+                            : m.dim == 2 ? tile_of_cell<2>(m, ti, c, tc)
+                                         : tile_of_cell<1>(m, ti, c, tc);
+    if (t != 0xffffffffu) k = t;
+  }
+  keys[i] = k;
+  perm[i] = (uint32_t)i;
+}
+
+// off[t] = first index with key >= t, for t in [0, ntiles]
+__global__ void k_offsets_from_sorted(const uint32_t *keys, uint64_t n,
+                                      uint32_t ntiles, uint32_t *off) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  uint32_t prev = i == 0 ? 0 : (keys[i - 1] > ntiles ? ntiles : keys[i - 1]);
+  uint32_t cur = i == n ? ntiles : (keys[i] > ntiles ? ntiles : keys[i]);
+  if (i == 0) {
+    for (uint32_t t = 0; t <= cur; t++) off[t] = 0;
+  } else {
+    for (uint32_t t = prev + 1; t <= cur; t++) off[t] = (uint32_t)i;
+  }
+}
+
+hipError_t TileSortWork::ensure(uint32_t nt, int dim) {
+  if (nt == ntiles && cnt) return hipSuccess;
+  release();
+  const int KD = dim == 1 ? 3 : dim == 2 ? 9 : 27;
+  const uint64_t nsrc = (uint64_t)nt + 1;
+  hipError_t e;
+  if ((e = hipMalloc(&cnt, nsrc * KD * 4)) != hipSuccess) return e;
+  if ((e = hipMalloc(&base, nsrc * KD * 4)) != hipSuccess) return e;
+  if ((e = hipMalloc(&newcnt, (nt + 1) * 4ull)) != hipSuccess) return e;
+  if ((e = hipMalloc(&newoff, (nt + 1) * 4ull)) != hipSuccess) return e;
+  if ((e = hipMalloc(&far, 8)) != hipSuccess) return e;
+  size_t bytes = 0;
+  e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, newcnt, newoff,
+                                       (int)(nt + 1));
+  if (e != hipSuccess) return e;
+  if ((e = hipMalloc(&scan_tmp, bytes)) != hipSuccess) return e;
+  scan_bytes = bytes;
+  ntiles = nt;
+  return hipSuccess;
+}
+
+void TileSortWork::release() {
+  (void)hipFree(cnt);
+  (void)hipFree(base);
+  (void)hipFree(newcnt);
+  (void)hipFree(newoff);
+  (void)hipFree(far);
+  (void)hipFree(scan_tmp);
+  cnt = base = newcnt = newoff = nullptr;
+  far = nullptr;
+  scan_tmp = nullptr;
+  scan_bytes = 0;
+  ntiles = 0;
+}
+
+template <typename T>
+__global__ void k_zero(T *a, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+hipError_t tile_sort_count(const Mesh &m, const TileInfo &ti,
+                           const uint32_t *cell, uint64_t tail_end,
+                           TileSortWork &ws, hipStream_t st) {
+  hipError_t e = ws.ensure(ti.ntiles, m.dim);
+  if (e != hipSuccess) return e;
+  k_zero<unsigned long long><<<1, 64, 0, st>>>(ws.far, 1);
+  const unsigned nsrc = ti.ntiles + 1;
+  if (m.dim == 2)
+    k_tile_count<2><<<nsrc, 256, 0, st>>>(m, ti, cell, tail_end, ws.cnt, ws.far);
+  else if (m.dim == 3)
+    k_tile_count<3><<<nsrc, 256, 0, st>>>(m, ti, cell, tail_end, ws.cnt, ws.far);
+  else
+    k_tile_count<1><<<nsrc, 256, 0, st>>>(m, ti, cell, tail_end, ws.cnt, ws.far);
+  return hipGetLastError();
+}
+
+hipError_t tile_sort_offsets(const Mesh &m, const TileInfo &ti,
+                             TileSortWork &ws, hipStream_t st) {
+  const unsigned nb = (ti.ntiles + 1 + 255) / 256;
+  if (m.dim == 2)
+    k_tile_layout<2><<<nb, 256, 0, st>>>(ti, ws.cnt, ws.base, ws.newcnt);
+  else if (m.dim == 3)
+    k_tile_layout<3><<<nb, 256, 0, st>>>(ti, ws.cnt, ws.base, ws.newcnt);
+  else
+    k_tile_layout<1><<<nb, 256, 0, st>>>(ti, ws.cnt, ws.base, ws.newcnt);
+  size_t bytes = ws.scan_bytes;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(
+      ws.scan_tmp, bytes, ws.newcnt, ws.newoff, (int)(ti.ntiles + 1), st);
+  if (e != hipSuccess) return e;
+  const int KD = m.dim == 1 ? 3 : m.dim == 2 ? 9 : 27;
+  const uint64_t nb2 = ((uint64_t)ti.ntiles * KD + 255) / 256;
+  if (m.dim == 2)
+    k_tile_base_abs<2><<<(unsigned)nb2, 256, 0, st>>>(ti, ws.newoff, ws.base);
+  else if (m.dim == 3)
+    k_tile_base_abs<3><<<(unsigned)nb2, 256, 0, st>>>(ti, ws.newoff, ws.base);
+  else
+    k_tile_base_abs<1><<<(unsigned)nb2, 256, 0, st>>>(ti, ws.newoff, ws.base);
+  return hipGetLastError();
+}
+
+hipError_t tile_sort_scatter(const Mesh &m, const TileInfo &ti,
+                             const PtcF32 &src, const PtcF32 &dst,
+                             uint64_t tail_end, TileSortWork &ws,
+                             hipStream_t st) {
+  (void)tail_end;
+  if (m.dim == 2)
+    k_tile_scatter<2, 256><<<ti.ntiles, 256, 0, st>>>(m, ti, src, dst, ws.base);
+  else if (m.dim == 3)
+    k_tile_scatter<3, 256><<<ti.ntiles, 256, 0, st>>>(m, ti, src, dst, ws.base);
+  else
+    k_tile_scatter<1, 256><<<ti.ntiles, 256, 0, st>>>(m, ti, src, dst, ws.base);
+  return hipGetLastError();
+}
+
+hipError_t tile_keys(const Mesh &m, const TileInfo &ti, const uint32_t *cell,
+                     uint64_t n, uint32_t *keys, uint32_t *perm,
+                     hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  k_tile_keys<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(m, ti, cell, n,
+                                                          keys, perm);
+  return hipGetLastError();
+}
+
+hipError_t tile_offsets_from_sorted(const uint32_t *keys, uint64_t n,
+                                    uint32_t ntiles, uint32_t *off,
+                                    hipStream_t st) {
+  k_offsets_from_sorted<<<(unsigned)((n + 1 + 255) / 256), 256, 0, st>>>(
+      keys, n, ntiles, off);
+  return hipGetLastError();
+}
+
+}  // namespace apt

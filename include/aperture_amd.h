@@ -172,6 +172,13 @@ apt_status apt_clear_guard_cells(apt_ctx *ctx);
 apt_status apt_energy_histogram(apt_ctx *ctx, uint32_t *hist, int nbins,
                                 double E_max, int flag_bit);
 
+/* Mixed form: tile binning of the particle array (no reference
+ * counterpart; the reference sorts by cell).  Writes ntiles + 1 offsets:
+ * tile t owns slots [off[t], off[t+1]); *ntiles = 0 when the particles are
+ * not binned (untiled mesh, or not sorted since the last apt_ptc_set). */
+apt_status apt_tile_offsets(apt_ctx *ctx, uint32_t *off, uint64_t cap,
+                            uint32_t *ntiles, int tile_extent[3]);
+
 /* ---- control -------------------------------------------------------- */
 apt_status apt_sync(apt_ctx *ctx);
 apt_status apt_get_counters(apt_ctx *ctx, apt_counters *out);

@@ -1,0 +1,171 @@
+"""GPU tests of the tiled mixed form: tile-major loader, tiled push/deposit,
+deterministic tile-binning sort (and its radix-sort fallback), against the
+FP64 oracle.
+
+Particles are matched to the oracle's through a tag stored in unused flag
+bits (10..28), which every kernel carries through unchanged."""
+import numpy as np
+import pytest
+
+import aperture3_amd as apt
+from cases import Case
+
+pytestmark = pytest.mark.gpu
+
+TAG_SHIFT = 10
+TAG_MASK = (1 << 19) - 1
+
+
+def tagged(po):
+    p = dict(po)
+    n = len(p["cell"])
+    assert n <= TAG_MASK
+    p["flag"] = (p["flag"] & np.uint32(0xE0000000)) | (np.arange(n, dtype=np.uint32) << TAG_SHIFT)
+    return p
+
+
+def by_tag(p):
+    tag = (p["flag"] >> TAG_SHIFT) & TAG_MASK
+    order = np.argsort(tag, kind="stable")
+    return {k: v[order] for k, v in p.items()}, tag[order]
+
+
+def tile_of(cells, env, T):
+    d0 = env.dims[0]
+    g = env.params().guard
+    c1 = cells % d0 - g[0]
+    if env.dim == 2:
+        c2 = cells // d0 - g[1]
+        nt0 = env.params().N[0] // T[0]
+        return c1 // T[0] + nt0 * (c2 // T[1])
+    d1 = env.dims[1]
+    c2 = (cells // d0) % d1 - g[1]
+    c3 = cells // (d0 * d1) - g[2]
+    nt0 = env.params().N[0] // T[0]
+    nt1 = env.params().N[1] // T[1]
+    return c1 // T[0] + nt0 * (c2 // T[1] + nt1 * (c3 // T[2]))
+
+
+def check_binning(env, d):
+    off, T = d.tile_offsets()
+    assert off is not None
+    p = d.get_particles()
+    assert off[-1] == d.num_particles()
+    t = tile_of(p["cell"].astype(np.int64), env, T)
+    owner = np.repeat(np.arange(len(off) - 1), np.diff(off.astype(np.int64)))
+    assert np.array_equal(t, owner)
+
+
+def test_tiled_loader_draws_the_fp64_particle_set():
+    c = Case((64, 64), 4, theta=(0.05, 0.05), drift=(0.3, -0.3), Bbg=(0, 0, 1.0))
+    o = c.oracle()
+    env, d = c.gpu(precision=apt.APT_PREC_MIXED)
+    off, T = d.tile_offsets()
+    assert off is not None and T[:2] == (16, 16)
+    pg, po = d.get_particles(), o.particles()
+    assert d.num_particles() == o.number
+    key_o = np.lexsort((po["p1"].astype(np.float32), po["x2"].astype(np.float32),
+                        po["x1"].astype(np.float32), po["cell"]))
+    key_g = np.lexsort((pg["p1"], pg["x2"], pg["x1"], pg["cell"]))
+    for k in ("cell", "x1", "x2", "p1", "p2", "p3"):
+        a = pg[k][key_g]
+        b = po[k][key_o]
+        if k != "cell":
+            b = b.astype(np.float32).astype(np.float64)
+        assert np.array_equal(a, b), k
+    check_binning(env, d)
+
+
+@pytest.mark.parametrize("N,ppc,tile", [((64, 64), 4, (16, 16)), ((32, 32), 3, (8, 8)),
+                                        ((16, 16, 16), 2, (8, 8, 8))])
+def test_tiled_step_vs_oracle(N, ppc, tile):
+    """Tiled mixed steps with tile sorts against the FP64 oracle: every
+    particle (matched by tag) keeps the oracle's cell and agrees to FP32
+    tolerance; E and B within 1e-5 of their scale."""
+    c = Case(N, ppc, theta=(0.1, 0.1), drift=(0.2, -0.2), Bbg=(0, 0, 1.0))
+    o = c.oracle()
+    po0 = tagged(o.particles())
+    o.set_particles(po0)
+    import aperture3_amd as a
+    p = a.SimParams(N=c.N, guard=c.guard, periodic_boundary=[True] * 3,
+                    max_ptc_number=c.nptc + 64, precision=a.APT_PREC_MIXED,
+                    tile=list(tile) + [0] * (3 - len(tile)), data_interval=100)
+    env = a.sim_environment(p)
+    d = a.sim_data(env)
+    E, B = c.fields()
+    for comp in range(3):
+        d.set_field(a.FIELD_E, comp, E[comp])
+        d.set_field(a.FIELD_B, comp, B[comp])
+        d.set_field(a.FIELD_BBG, comp, B[comp])
+    d.set_particles(po0)
+    d.sort_particles()  # radix fallback (not binned yet)
+    check_binning(env, d)
+    pu, fs = a.ptc_updater(env), a.field_solver(env)
+    for step in range(16):
+        o.ptc_update(c.dt, step)
+        o.field_update(c.dt, step)
+        pu.update_particles(d, c.dt, step)
+        fs.update_fields(d, c.dt, step)
+        if step in (5, 11):
+            o.sort()
+            d.sort_particles()  # tile-binning sort
+            check_binning(env, d)
+    pg, tg = by_tag(d.get_particles())
+    ps, ts = by_tag(o.particles())
+    assert np.array_equal(tg, ts)
+    assert np.mean(pg["cell"] == ps["cell"]) > 0.999
+    same = pg["cell"] == ps["cell"]
+    for k in ("x1", "x2"):
+        assert np.abs(pg[k][same] - ps[k][same]).max() < 1e-4, k
+    for k in ("p1", "p2", "p3"):
+        assert np.abs(pg[k] - ps[k]).max() <= 1e-5 * np.abs(ps[k]).max(), k
+    for comp in range(3):
+        for which, key in ((a.FIELD_E, "E"), (a.FIELD_B, "B")):
+            x, y = d.get_field(which, comp), o.f[key][comp]
+            assert np.abs(x - y).max() <= 1e-5 * max(np.abs(y).max(), 1e-30), (key, comp)
+
+
+def _run(c, steps, sorts):
+    env, d = c.gpu(precision=apt.APT_PREC_MIXED)
+    pu, fs = apt.ptc_updater(env), apt.field_solver(env)
+    for step in range(steps):
+        pu.update_particles(d, c.dt, step)
+        fs.update_fields(d, c.dt, step)
+        if step in sorts:
+            d.sort_particles()
+    out = d.get_particles()
+    fields = [d.get_field(apt.FIELD_E, k) for k in range(3)] + \
+             [d.get_field(apt.FIELD_J, k) for k in range(3)]
+    return out, fields, d.counters()
+
+
+def test_tiled_run_is_bit_reproducible():
+    c = Case((64, 64), 6, theta=(0.1, 0.1), drift=(0.3, -0.3), Bbg=(0, 0, 1.0))
+    a, fa, ca = _run(c, 25, (8, 16))
+    b, fb, cb = _run(c, 25, (8, 16))
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    for x, y in zip(fa, fb):
+        assert np.array_equal(x, y)
+
+
+def test_far_movers_fall_back_to_radix_sort():
+    """A particle that crosses more than one tile between sorts (8x8 tiles,
+    |v| ~ c for 24 steps of dt = dx/2 -> 12 cells) is still binned right."""
+    c = Case((32, 32), 1, theta=(0.0, 0.0), Bbg=(0, 0, 0.0), E1=lambda x, y, z: 0.0 * x)
+    p = apt.SimParams(N=c.N, guard=c.guard, periodic_boundary=[True] * 3,
+                      max_ptc_number=64, precision=apt.APT_PREC_MIXED, tile=[8, 8, 0])
+    env = apt.sim_environment(p)
+    d = apt.sim_data(env)
+    cells = np.array([2 + 3 + (2 + 5) * 36, 2 + 20 + (2 + 9) * 36], np.uint32)
+    d.set_particles({"cell": cells, "x1": [0.5, 0.5], "x2": [0.5, 0.5], "x3": [0.0, 0.0],
+                     "p1": [30.0, 0.1], "p2": [0.0, 0.0], "p3": [0.0, 0.0], "w": [1.0, 1.0],
+                     "flag": np.array([1 << 29, 0], np.uint32)})
+    d.sort_particles()
+    pu, fs = apt.ptc_updater(env), apt.field_solver(env)
+    for step in range(24):
+        pu.update_particles(d, c.dt, step)
+        fs.update_fields(d, c.dt, step)
+    d.sort_particles()
+    check_binning(env, d)
+    assert d.num_particles() == 2
