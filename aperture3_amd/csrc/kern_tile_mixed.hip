@@ -172,6 +172,10 @@ __global__ void __launch_bounds__(NT)
     if (c == MAX_CELL) continue;
     int c1 = (int)(c % (uint32_t)d0);
     int c2 = (int)(c / (uint32_t)d0);
+    if (!m.in_bulk(c1, c2, 0)) {
+      p.cell[idx] = MAX_CELL;
+      continue;
+    }
     uint32_t flag = p.flag[idx];
     int s = ptc_species(flag);
     float x1 = p.x1[idx], x2 = p.x2[idx], x3 = p.x3[idx];
@@ -342,6 +346,10 @@ __global__ void __launch_bounds__(NT)
     int c1 = (int)(c % (uint32_t)d0);
     int c2 = (int)((c / (uint32_t)d0) % (uint32_t)m.dims[1]);
     int c3 = (int)(c / (uint32_t)d01);
+    if (!m.in_bulk(c1, c2, c3)) {
+      p.cell[idx] = MAX_CELL;
+      continue;
+    }
     uint32_t flag = p.flag[idx];
     int s = ptc_species(flag);
     float x1 = p.x1[idx], x2 = p.x2[idx], x3 = p.x3[idx];
