@@ -265,10 +265,11 @@ __global__ void __launch_bounds__(NT)
         const float cen = (2.0f * sx1[tk] * sy1[tj] + sx0[tk] * sy1[tj] +
                            sx1[tk] * sy0[tj] + 2.0f * sx0[tk] * sy0[tj]) *
                           (1.0f / 6.0f);
-        // J1 at (i+1, j), J2 at (i, j+1), J3 at (i, j); reference signs
-        // (ptc_updater.cu:345-376): J = q w x (movement)
-        const long long v1q = tk < 3 ? fxq(qw * djx, scale) : 0;
-        const long long v2q = tj < 3 ? fxq(qw * djy[tk], scale) : 0;
+        // J1 at (i+1, j), J2 at (i, j+1), J3 at (i, j) with the reference's
+        // signs (ptc_updater.cu:345-376): J1,2 = -q w (shape flux), J3 =
+        // q w v3 (centred shape)
+        const long long v1q = tk < 3 ? fxq(-qw * djx, scale) : 0;
+        const long long v2q = tj < 3 ? fxq(-qw * djy[tk], scale) : 0;
         const long long v3q = fxq(qw * v3 * cen, scale);
         const long long vr = rho ? fxq(qw * sx1[tk] * sy1[tj], scale) : 0;
         if (in) {
@@ -437,9 +438,10 @@ __global__ void __launch_bounds__(NT)
                              sx1[ti2] * sy0[tj] + 2.0f * sx0[ti2] * sy0[tj]) *
                             (1.0f / 6.0f);
           djz[tj][ti2] += (sz1[tk] - sz0[tk]) * cxy;
-          const long long q1 = ti2 < 3 ? fxq(qw * djx, scale) : 0;
-          const long long q2 = tj < 3 ? fxq(qw * djy[ti2], scale) : 0;
-          const long long q3 = tk < 3 ? fxq(qw * djz[tj][ti2], scale) : 0;
+          // reference signs (ptc_updater.cu:445-487): J = -q w (shape flux)
+          const long long q1 = ti2 < 3 ? fxq(-qw * djx, scale) : 0;
+          const long long q2 = tj < 3 ? fxq(-qw * djy[ti2], scale) : 0;
+          const long long q3 = tk < 3 ? fxq(-qw * djz[tj][ti2], scale) : 0;
           if (in) {
             const int b = (l1 + di) + ((l2 + dj) + (l3 + dk) * W) * W;
             lds_add(&sJ[0][b + 1], q1);
