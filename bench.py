@@ -126,7 +126,20 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (read 1 GiB + write 1 GiB), "
+                         "the reference point for the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 28  # 2^28 floats = 1 GiB
+        x = torch.ones(n, device="cuda", dtype=torch.float32)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the PIC step is "
                           "not sharded (see the sharding design); use --gpus 1"}))
