@@ -108,6 +108,12 @@ __device__ __forceinline__ void vay_push_f(float &p1, float &p2, float &p3,
   p3 = (up3 + B3 * ut * inv_gamma2 + (up1 * B2 - up2 * B1) * ign) * s;
 }
 
+// per-species constants in FP32: qm = dt q / (2 m), q
+struct SpeciesF32 {
+  float qm[8];
+  float q[8];
+};
+
 __device__ __forceinline__ long long fxq(float c, float scale) {
   return (long long)__float2int_rn(c * scale);
 }
@@ -126,16 +132,193 @@ __device__ __forceinline__ void glb_add(long long *a, long long v) {
 
 // ---------------------------------------------------------------------
 // 2D tiled push + deposit
+//
+// Per particle: 9 coalesced 4-byte loads (the next particle's are issued
+// before the current one is processed), 24 LDS reads for the six CIC
+// gathers, the Vay push, the move, and the Esirkepov current on the 3x3
+// node window that contains both the old and the new shape (the 4x4
+// reference loop minus its always-zero row and column), i.e. at most
+// 6 + 6 + 9 int64 LDS atomics, zeros skipped.  Particles whose stencil
+// leaves the LDS window are queued in LDS and pushed afterwards through
+// global memory, so the common path carries no divergent global code.
 // ---------------------------------------------------------------------
+struct PtcRegs {
+  uint32_t c, flag;
+  float x1, x2, x3, u1, u2, u3, w;
+};
+
+__device__ __forceinline__ void load_ptc(const PtcF32 &p, uint32_t i,
+                                         PtcRegs &r) {
+  r.c = p.cell[i];
+  r.flag = p.flag[i];
+  r.x1 = p.x1[i];
+  r.x2 = p.x2[i];
+  r.x3 = p.x3[i];
+  r.u1 = p.u1[i];
+  r.u2 = p.u2[i];
+  r.u3 = p.u3[i];
+  r.w = p.w[i];
+}
+
+// old/new CIC shapes on the 3-node window starting at node offset
+// min(dc, 0) - 1 of the old cell (Esirkepov, ptc_updater.cu:347-384)
+__device__ __forceinline__ void shapes3(float xo, float xn, int dc, float a[3],
+                                        float b[3]) {
+  const float ao0 = 1.0f - xo, an0 = 1.0f - xn;
+  a[0] = dc < 0 ? 0.0f : ao0;
+  a[1] = dc < 0 ? ao0 : xo;
+  a[2] = dc < 0 ? xo : 0.0f;
+  b[0] = dc > 0 ? 0.0f : an0;
+  b[1] = dc > 0 ? an0 : xn;
+  b[2] = dc > 0 ? xn : 0.0f;
+}
+
+// One particle: push, move, write back, return the current contributions
+// (already in fixed point) on the 3x3 window.
+struct Deposit2 {
+  long long j1[2][3];  // [face t][row j]
+  long long j2[3][2];  // [col t][face j]
+  long long j3[3][3];
+  long long r[3][3];
+  int di0, dj0;        // window origin relative to the old cell
+  int s;
+  bool cur;            // deposits current
+};
+
+template <bool RHO, typename Gather>
+__device__ __forceinline__ void push_move_2d(
+    const Mesh &m, const float *qm_tab, const float *q_tab, float dt,
+    float idx0, float idx1, float scale, const PtcF32 &p, uint32_t idx,
+    const PtcRegs &r, int c1, int c2, Gather gather,
+    unsigned long long *counters, Deposit2 &D) {
+  const int s = ptc_species(r.flag);
+  float u1 = r.u1, u2 = r.u2, u3 = r.u3;
+  float gamma = sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+  if (!(r.flag & FLAG_IGNORE_EM)) {
+    const float qm = qm_tab[s];
+    float F[6];
+    gather(r.x1, r.x2, F);
+    vay_push_f(u1, u2, u3, gamma, F[0] * qm, F[1] * qm, F[2] * qm,
+               F[3] * qm, F[4] * qm, F[5] * qm);
+    gamma = sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+    p.u1[idx] = u1;
+    p.u2[idx] = u2;
+    p.u3[idx] = u3;
+  }
+  const float ig = 1.0f / gamma;
+  const float v1 = u1 * ig, v2 = u2 * ig, v3 = u3 * ig;
+  float n1 = r.x1 + v1 * idx0;
+  float n2 = r.x2 + v2 * idx1;
+  int dc1 = (int)floorf(n1);
+  int dc2 = (int)floorf(n2);
+  n1 -= (float)dc1;
+  n2 -= (float)dc2;
+  if (n1 >= 1.0f) { n1 -= 1.0f; dc1 += 1; }
+  if (n2 >= 1.0f) { n2 -= 1.0f; dc2 += 1; }
+  if (dc1 > 1 || dc1 < -1 || dc2 > 1 || dc2 < -1) {
+    atomicAdd(&counters[CTR_MULTICELL], 1ull);
+    dc1 = max(-1, min(1, dc1));
+    dc2 = max(-1, min(1, dc2));
+  }
+  p.cell[idx] = (uint32_t)m.idx(m.wrap(0, c1 + dc1), m.wrap(1, c2 + dc2), 0);
+  p.x1[idx] = n1;
+  p.x2[idx] = n2;
+  p.x3[idx] = r.x3 + v3 * dt;
+  D.s = s;
+  D.cur = !(r.flag & FLAG_IGNORE_CURRENT);
+  D.di0 = min(dc1, 0) - 1;
+  D.dj0 = min(dc2, 0) - 1;
+  float a[3], b[3], c[3], d[3];  // x old/new, y old/new
+  shapes3(r.x1, n1, dc1, a, b);
+  shapes3(r.x2, n2, dc2, c, d);
+  const float qw = q_tab[s] * r.w;
+  const float mqw = -qw;
+  // in-plane fluxes (reference weight -q w, ptc_updater.cu:345-372)
+  const float wx0 = b[0] - a[0], wx1 = wx0 + (b[1] - a[1]);
+  const float wy0 = d[0] - c[0], wy1 = wy0 + (d[1] - c[1]);
+  float ax[3], ay[3];
+#pragma unroll
+  for (int t = 0; t < 3; t++) {
+    ax[t] = 0.5f * (a[t] + b[t]);
+    ay[t] = 0.5f * (c[t] + d[t]);
+  }
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    D.j1[0][j] = fxq(mqw * wx0 * ay[j], scale);
+    D.j1[1][j] = fxq(mqw * wx1 * ay[j], scale);
+    D.j2[j][0] = fxq(mqw * wy0 * ax[j], scale);
+    D.j2[j][1] = fxq(mqw * wy1 * ax[j], scale);
+  }
+  const float k3 = qw * v3 * (1.0f / 6.0f);
+#pragma unroll
+  for (int t = 0; t < 3; t++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      D.j3[t][j] = fxq(k3 * (2.0f * b[t] * d[j] + a[t] * d[j] + b[t] * c[j] +
+                             2.0f * a[t] * c[j]),
+                       scale);
+      if (RHO) D.r[t][j] = fxq(qw * b[t] * d[j], scale);
+    }
+}
+
+// the whole update of one particle through global memory (used for the
+// rare particles that drifted out of their tile's LDS window)
+template <bool RHO>
+__device__ __forceinline__ void push_global_2d(const Mesh &m, const float *qm_tab,
+                                            const float *q_tab, float dt,
+                                            float idx0, float idx1,
+                                            float scale, const PtcF32 &p,
+                                            const FieldsF32 &f,
+                                            const AccMixed &a, uint32_t i,
+                                            unsigned long long *counters) {
+  PtcRegs r;
+  load_ptc(p, i, r);
+  const int d0 = m.dims[0];
+  const int c1 = (int)(r.c % (uint32_t)d0);
+  const int c2 = (int)(r.c / (uint32_t)d0);
+  Deposit2 D;
+  push_move_2d<RHO>(
+      m, qm_tab, q_tab, dt, idx0, idx1, scale, p, i, r, c1, c2,
+      [&](float x1, float x2, float F[6]) {
+        F[0] = gather2(f.E[0], d0, c1, c2, x1, x2, ST_E1);
+        F[1] = gather2(f.E[1], d0, c1, c2, x1, x2, ST_E2);
+        F[2] = gather2(f.E[2], d0, c1, c2, x1, x2, ST_E3);
+        F[3] = gather2(f.B[0], d0, c1, c2, x1, x2, ST_B1);
+        F[4] = gather2(f.B[1], d0, c1, c2, x1, x2, ST_B2);
+        F[5] = gather2(f.B[2], d0, c1, c2, x1, x2, ST_B3);
+      },
+      counters, D);
+  if (!D.cur) return;
+  const int i0 = c1 + D.di0, j0 = c2 + D.dj0;
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    glb_add(a.J[0] + m.widx(i0 + 1, j0 + j, 0), D.j1[0][j]);
+    glb_add(a.J[0] + m.widx(i0 + 2, j0 + j, 0), D.j1[1][j]);
+    glb_add(a.J[1] + m.widx(i0 + j, j0 + 1, 0), D.j2[j][0]);
+    glb_add(a.J[1] + m.widx(i0 + j, j0 + 2, 0), D.j2[j][1]);
+  }
+#pragma unroll
+  for (int tt = 0; tt < 3; tt++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      glb_add(a.J[2] + m.widx(i0 + tt, j0 + j, 0), D.j3[tt][j]);
+      if (RHO) glb_add(a.R[D.s] + m.widx(i0 + tt, j0 + j, 0), D.r[tt][j]);
+    }
+}
+
 template <int T1, int T2, int H, int NT, bool RHO>
 __global__ void __launch_bounds__(NT)
-    k_push_deposit_tile2d(Mesh m, Species sp, float dt, float scale,
+    k_push_deposit_tile2d(Mesh m, SpeciesF32 spf, float dt, float scale,
                           PtcF32 p, FieldsF32 f, AccMixed a, TileInfo ti,
                           unsigned long long *__restrict__ counters) {
   constexpr int W1 = T1 + 2 * H, W2 = T2 + 2 * H, WN = W1 * W2;
+  constexpr int QCAP = 512;
   __shared__ float sF[6][WN];
   __shared__ unsigned long long sJ[3][WN];
   __shared__ unsigned long long sR[RHO ? 2 : 1][RHO ? WN : 1];
+  __shared__ float s_qm[8], s_q[8];
+  __shared__ uint32_t s_queue[QCAP];
+  __shared__ uint32_t s_nq;
 
   const uint32_t t = xcd_tile(blockIdx.x, ti.ntiles);
   const int tx = (int)(t % (uint32_t)ti.nt[0]);
@@ -144,8 +327,12 @@ __global__ void __launch_bounds__(NT)
   const int N0 = m.N[0], N1 = m.N[1], d0 = m.dims[0];
   const int w0 = g0 + tx * T1 - H;  // window origin (mesh index)
   const int w1 = g1 + ty * T2 - H;
-  constexpr bool rho = RHO;
 
+  if (threadIdx.x < 8) {
+    s_qm[threadIdx.x] = spf.qm[threadIdx.x];
+    s_q[threadIdx.x] = spf.q[threadIdx.x];
+  }
+  if (threadIdx.x == 0) s_nq = 0;
   for (int i = threadIdx.x; i < WN; i += NT) {
     int l1 = i % W1, l2 = i / W1;
     int64_t o = (int64_t)fold(w0 + l1, g0, N0) +
@@ -156,7 +343,7 @@ __global__ void __launch_bounds__(NT)
       sF[3 + c][i] = f.B[c][o];
       sJ[c][i] = 0ull;
     }
-    if (rho) {
+    if (RHO) {
       sR[0][i] = 0ull;
       sR[1][i] = 0ull;
     }
@@ -167,132 +354,77 @@ __global__ void __launch_bounds__(NT)
   const float idx0 = (float)m.inv_delta[0] * dt;
   const float idx1 = (float)m.inv_delta[1] * dt;
   unsigned long long outside = 0;
-  for (uint32_t idx = beg + threadIdx.x; idx < end; idx += NT) {
-    uint32_t c = p.cell[idx];
-    if (c == MAX_CELL) continue;
-    int c1 = (int)(c % (uint32_t)d0);
-    int c2 = (int)(c / (uint32_t)d0);
+  PtcRegs nxt;
+  uint32_t idx = beg + threadIdx.x;
+  if (idx < end) load_ptc(p, idx, nxt);
+  for (; idx < end; idx += NT) {
+    const PtcRegs r = nxt;
+    if (idx + NT < end) load_ptc(p, idx + NT, nxt);
+    if (r.c == MAX_CELL) continue;
+    const int c1 = (int)(r.c % (uint32_t)d0);
+    const int c2 = (int)(r.c / (uint32_t)d0);
     if (!m.in_bulk(c1, c2, 0)) {
       p.cell[idx] = MAX_CELL;
       continue;
     }
-    uint32_t flag = p.flag[idx];
-    int s = ptc_species(flag);
-    float x1 = p.x1[idx], x2 = p.x2[idx], x3 = p.x3[idx];
-    float u1 = p.u1[idx], u2 = p.u2[idx], u3 = p.u3[idx];
-    float w = p.w[idx];
     // local coordinates in the window, minimal image for periodic wrap
     int l1 = c1 - w0, l2 = c2 - w1;
-    if (m.periodic[0]) {
-      if (l1 < -H) l1 += N0;
-      else if (l1 >= N0 - H) l1 -= N0;
-    }
-    if (m.periodic[1]) {
-      if (l2 < -H) l2 += N1;
-      else if (l2 >= N1 - H) l2 -= N1;
-    }
+    if (l1 < -H) l1 += N0;
+    else if (l1 >= N0 - H) l1 -= N0;
+    if (l2 < -H) l2 += N1;
+    else if (l2 >= N1 - H) l2 -= N1;
     const bool in = (l1 >= 2) & (l1 <= W1 - 3) & (l2 >= 2) & (l2 <= W2 - 3);
-    outside += in ? 0 : 1;
-    float gamma = sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
-    if (!(flag & FLAG_IGNORE_EM)) {
-      float qm = (float)(0.5 * sp.q[s] / sp.m[s]) * dt;
-      float F[6];
-      if (in) {
-#pragma unroll
-        for (int k = 0; k < 6; k++) {
-          const int st = k == 0 ? ST_E1 : k == 1 ? ST_E2 : k == 2 ? ST_E3
-                       : k == 3 ? ST_B1 : k == 4 ? ST_B2 : ST_B3;
-          F[k] = gather2(sF[k], W1, l1, l2, x1, x2, st) * qm;
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < 6; k++) {
-          const int st = k == 0 ? ST_E1 : k == 1 ? ST_E2 : k == 2 ? ST_E3
-                       : k == 3 ? ST_B1 : k == 4 ? ST_B2 : ST_B3;
-          const float *src = k < 3 ? f.E[k] : f.B[k - 3];
-          F[k] = gather2(src, d0, c1, c2, x1, x2, st) * qm;
-        }
-      }
-      vay_push_f(u1, u2, u3, gamma, F[0], F[1], F[2], F[3], F[4], F[5]);
-      gamma = sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
-      p.u1[idx] = u1;
-      p.u2[idx] = u2;
-      p.u3[idx] = u3;
+    if (!in) {
+      uint32_t q = atomicAdd(&s_nq, 1u);
+      if (q < QCAP)
+        s_queue[q] = idx;
+      else
+        push_global_2d<RHO>(m, s_qm, s_q, dt, idx0, idx1, scale, p, f, a, idx,
+                            counters);
+      outside++;
+      continue;
     }
-    const float ig = 1.0f / gamma;
-    const float v1 = u1 * ig, v2 = u2 * ig, v3 = u3 * ig;
-    float n1 = x1 + v1 * idx0;
-    float n2 = x2 + v2 * idx1;
-    int dc1 = (int)floorf(n1);
-    int dc2 = (int)floorf(n2);
-    n1 -= (float)dc1;
-    n2 -= (float)dc2;
-    if (n1 >= 1.0f) { n1 -= 1.0f; dc1 += 1; }
-    if (n2 >= 1.0f) { n2 -= 1.0f; dc2 += 1; }
-    if (dc1 > 1 || dc1 < -1 || dc2 > 1 || dc2 < -1) {
-      atomicAdd(&counters[CTR_MULTICELL], 1ull);
-      dc1 = max(-1, min(1, dc1));
-      dc2 = max(-1, min(1, dc2));
+    Deposit2 D;
+    push_move_2d<RHO>(
+        m, s_qm, s_q, dt, idx0, idx1, scale, p, idx, r, c1, c2,
+        [&](float x1, float x2, float F[6]) {
+          F[0] = gather2(sF[0], W1, l1, l2, x1, x2, ST_E1);
+          F[1] = gather2(sF[1], W1, l1, l2, x1, x2, ST_E2);
+          F[2] = gather2(sF[2], W1, l1, l2, x1, x2, ST_E3);
+          F[3] = gather2(sF[3], W1, l1, l2, x1, x2, ST_B1);
+          F[4] = gather2(sF[4], W1, l1, l2, x1, x2, ST_B2);
+          F[5] = gather2(sF[5], W1, l1, l2, x1, x2, ST_B3);
+        },
+        counters, D);
+    if (!D.cur) continue;
+    const int b0 = (l1 + D.di0) + (l2 + D.dj0) * W1;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      lds_add(&sJ[0][b0 + 1 + j * W1], D.j1[0][j]);
+      lds_add(&sJ[0][b0 + 2 + j * W1], D.j1[1][j]);
+      lds_add(&sJ[1][b0 + j + W1], D.j2[j][0]);
+      lds_add(&sJ[1][b0 + j + 2 * W1], D.j2[j][1]);
     }
-    p.cell[idx] =
-        (uint32_t)m.idx(m.wrap(0, c1 + dc1), m.wrap(1, c2 + dc2), 0);
-    p.x1[idx] = n1;
-    p.x2[idx] = n2;
-    p.x3[idx] = x3 + v3 * dt;
-    if (flag & FLAG_IGNORE_CURRENT) continue;
-
-    // Esirkepov weights over node offsets -2..1 of the old cell
-    float sx0[4] = {0.f, 1.0f - x1, x1, 0.f};
-    float sy0[4] = {0.f, 1.0f - x2, x2, 0.f};
-    float sx1[4] = {0.f, 0.f, 0.f, 0.f}, sy1[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      sx1[q] = (q == 1 + dc1) ? 1.0f - n1 : ((q == 2 + dc1) ? n1 : 0.0f);
-      sy1[q] = (q == 1 + dc2) ? 1.0f - n2 : ((q == 2 + dc2) ? n2 : 0.0f);
-    }
-    const float qw = (float)sp.q[s] * w;
-    float djy[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int tt = 0; tt < 3; tt++)
 #pragma unroll
-    for (int tj = 0; tj < 4; tj++) {
-      const float ay = 0.5f * (sy0[tj] + sy1[tj]);
-      const float dsy = sy1[tj] - sy0[tj];
-      float djx = 0.0f;
-#pragma unroll
-      for (int tk = 0; tk < 4; tk++) {
-        const int di = tk - 2, dj = tj - 2;
-        djx += (sx1[tk] - sx0[tk]) * ay;
-        djy[tk] += dsy * 0.5f * (sx0[tk] + sx1[tk]);
-        const float cen = (2.0f * sx1[tk] * sy1[tj] + sx0[tk] * sy1[tj] +
-                           sx1[tk] * sy0[tj] + 2.0f * sx0[tk] * sy0[tj]) *
-                          (1.0f / 6.0f);
-        // J1 at (i+1, j), J2 at (i, j+1), J3 at (i, j) with the reference's
-        // signs (ptc_updater.cu:345-376): J1,2 = -q w (shape flux), J3 =
-        // q w v3 (centred shape)
-        const long long v1q = tk < 3 ? fxq(-qw * djx, scale) : 0;
-        const long long v2q = tj < 3 ? fxq(-qw * djy[tk], scale) : 0;
-        const long long v3q = fxq(qw * v3 * cen, scale);
-        const long long vr = rho ? fxq(qw * sx1[tk] * sy1[tj], scale) : 0;
-        if (in) {
-          const int b = (l1 + di) + (l2 + dj) * W1;
-          lds_add(&sJ[0][b + 1], v1q);
-          lds_add(&sJ[1][b + W1], v2q);
-          lds_add(&sJ[2][b], v3q);
-          if (rho) {
-            if (s < 2)
-              lds_add(&sR[s][b], vr);
-            else
-              glb_add(a.R[s] + m.widx(c1 + di, c2 + dj, 0), vr);
-          }
-        } else {
-          glb_add(a.J[0] + m.widx(c1 + di + 1, c2 + dj, 0), v1q);
-          glb_add(a.J[1] + m.widx(c1 + di, c2 + dj + 1, 0), v2q);
-          const uint64_t o = m.widx(c1 + di, c2 + dj, 0);
-          glb_add(a.J[2] + o, v3q);
-          if (rho) glb_add(a.R[s] + o, vr);
+      for (int j = 0; j < 3; j++) {
+        lds_add(&sJ[2][b0 + tt + j * W1], D.j3[tt][j]);
+        if (RHO) {
+          if (D.s < 2)
+            lds_add(&sR[D.s][b0 + tt + j * W1], D.r[tt][j]);
+          else
+            glb_add(a.R[D.s] + m.widx(c1 + D.di0 + tt, c2 + D.dj0 + j, 0),
+                    D.r[tt][j]);
         }
       }
-    }
   }
+  __syncthreads();
+  // particles outside the window: global gathers and atomics
+  const uint32_t nq = min(s_nq, (uint32_t)QCAP);
+  for (uint32_t k = threadIdx.x; k < nq; k += NT)
+    push_global_2d<RHO>(m, s_qm, s_q, dt, idx0, idx1, scale, p, f, a,
+                        s_queue[k], counters);
   if (outside) atomicAdd(&counters[CTR_OUTSIDE], outside);
   __syncthreads();
   for (int i = threadIdx.x; i < WN; i += NT) {
@@ -301,7 +433,7 @@ __global__ void __launch_bounds__(NT)
                 (int64_t)fold(w1 + l2, g1, N1) * d0;
 #pragma unroll
     for (int c = 0; c < 3; c++) glb_add(a.J[c] + o, (long long)sJ[c][i]);
-    if (rho) {
+    if (RHO) {
       glb_add(a.R[0] + o, (long long)sR[0][i]);
       glb_add(a.R[1] + o, (long long)sR[1][i]);
     }
@@ -475,15 +607,20 @@ hipError_t launch_push_deposit_tiled(const Mesh &m, const Species &sp,
                                      unsigned long long *counters,
                                      hipStream_t st) {
   float scale = ldexpf(1.0f, S);
+  SpeciesF32 spf;
+  for (int i = 0; i < 8; i++) {
+    spf.qm[i] = (float)(0.5 * sp.q[i] / sp.m[i]) * dt;
+    spf.q[i] = (float)sp.q[i];
+  }
   if (m.dim == 2) {
 #define LAUNCH2(A, B, HH, NTH)                                              \
   if (ti.T[0] == A && ti.T[1] == B && ti.halo == HH) {                      \
     if (a.rho)                                                              \
       k_push_deposit_tile2d<A, B, HH, NTH, true><<<ti.ntiles, NTH, 0, st>>>( \
-          m, sp, dt, scale, p, f, a, ti, counters);                         \
+          m, spf, dt, scale, p, f, a, ti, counters);                        \
     else                                                                    \
       k_push_deposit_tile2d<A, B, HH, NTH, false><<<ti.ntiles, NTH, 0, st>>>(\
-          m, sp, dt, scale, p, f, a, ti, counters);                         \
+          m, spf, dt, scale, p, f, a, ti, counters);                        \
     return hipGetLastError();                                               \
   }
     LAUNCH2(16, 16, 6, 256)
