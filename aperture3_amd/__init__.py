@@ -47,7 +47,8 @@ class AptParams(C.Structure):
                 ("q_e", C.c_double), ("ion_mass", C.c_double),
                 ("data_interval", C.c_int), ("current_smoothing", C.c_int),
                 ("fix_3d_filter", C.c_int), ("max_ptc_number", C.c_uint64),
-                ("random_seed", C.c_uint64), ("tile", C.c_int * 3)]
+                ("random_seed", C.c_uint64), ("tile", C.c_int * 3),
+                ("tile_halo", C.c_int), ("tile_threads", C.c_int)]
 
 
 class AptParticles(C.Structure):
@@ -149,6 +150,8 @@ class SimParams:
         self.size = [1.0, 1.0, 1.0]
         self.precision = APT_PREC_FP64
         self.tile = [0, 0, 0]
+        self.tile_halo = 0
+        self.tile_threads = 0
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise AttributeError(f"unknown parameter {k}")
@@ -205,6 +208,8 @@ class sim_data:
         q.fix_3d_filter = 1 if p.fix_3d_filter else 0
         q.max_ptc_number = p.max_ptc_number
         q.random_seed = p.random_seed
+        q.tile_halo = p.tile_halo
+        q.tile_threads = p.tile_threads
         h = C.c_void_p()
         st = self.lib.apt_create(C.byref(g), C.byref(q), C.byref(h))
         if st != APT_OK:

@@ -67,6 +67,14 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
 
 
+# The mixed-precision kernels use the hardware FP32 reciprocal/square root
+# (<= 1 ulp) instead of the correctly rounded expansions.
+FILE_FLAGS = {
+    "kern_tile_mixed.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "kern_cart_mixed.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+}
+
+
 def build(verbose: bool = False, jobs: int | None = None) -> str:
     hipcc = _hipcc()
     os.makedirs(BUILD, exist_ok=True)
@@ -75,7 +83,8 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     for src in HIP_SOURCES + HOST_SOURCES:
         obj = os.path.join(BUILD, src.replace("/", "_") + ".o")
         objs.append(obj)
-        cmds.append([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([hipcc, *HIPCC_FLAGS, *FILE_FLAGS.get(src, []), "-c",
+                     os.path.join(CSRC, src), "-o", obj])
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         for cmd, _ in zip(cmds, ex.map(_run, cmds)):

@@ -161,17 +161,17 @@ static apt_status setup_tiles(apt_ctx *ctx) {
   ctx->tiling = false;
   if (m.dim < 2) return APT_OK;
   int T[3] = {1, 1, 1};
-  int halo;
+  int halo, threads = ctx->params.tile_threads > 0 ? ctx->params.tile_threads : 256;
   if (m.dim == 2) {
     T[0] = ctx->params.tile[0] > 0 ? ctx->params.tile[0] : 16;
     T[1] = ctx->params.tile[1] > 0 ? ctx->params.tile[1] : 16;
-    halo = 6;
-    if (T[0] == 8) halo = 4;
+    halo = T[0] == 8 ? 4 : 6;
   } else {
     T[0] = T[1] = T[2] = ctx->params.tile[0] > 0 ? ctx->params.tile[0] : 8;
     halo = 3;
   }
-  if (!tile_shape_supported(m.dim, T, halo)) return APT_OK;
+  if (ctx->params.tile_halo > 0) halo = ctx->params.tile_halo;
+  if (!tile_shape_supported(m.dim, T, halo, threads)) return APT_OK;
   uint64_t ntiles = 1;
   for (int d = 0; d < m.dim; d++) {
     if (!m.periodic[d] || m.N[d] % T[d] != 0) return APT_OK;
@@ -181,6 +181,7 @@ static apt_status setup_tiles(apt_ctx *ctx) {
   for (int d = m.dim; d < 3; d++) ctx->tiles.nt[d] = 1;
   for (int d = 0; d < 3; d++) ctx->tiles.T[d] = T[d];
   ctx->tiles.halo = halo;
+  ctx->tiles.threads = threads;
   ctx->tiles.ntiles = (uint32_t)ntiles;
   HIPCHK(hipMalloc(&ctx->tile_off, (ntiles + 1) * 4));
   ctx->tiles.tile_off = ctx->tile_off;

@@ -65,6 +65,7 @@ struct TileInfo {
   int T[3];       // tile extent in cells
   int nt[3];      // tiles per dimension
   int halo;       // cells of LDS halo around a tile
+  int threads;    // threads per tile workgroup
   const uint32_t *tile_off;
   uint32_t ntiles;
   uint64_t tiled_end;  // tile_off[ntiles], known on the host

@@ -6,7 +6,7 @@
 
 namespace apt {
 
-bool tile_shape_supported(int dim, const int T[3], int halo);
+bool tile_shape_supported(int dim, const int T[3], int halo, int threads);
 
 hipError_t launch_push_deposit_tiled(const Mesh &m, const Species &sp,
                                      float dt, int S, const PtcF32 &p,

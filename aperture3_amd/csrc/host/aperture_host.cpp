@@ -117,6 +117,8 @@ sim_data::sim_data(sim_environment &e)
   q.current_smoothing = p.current_smoothing;
   q.max_ptc_number = p.max_ptc_number;
   q.random_seed = (uint64_t)p.random_seed;
+  q.tile_halo = p.tile_halo;
+  q.tile_threads = p.tile_threads;
   check(nullptr, apt_create(&g, &q, &ctx), "apt_create");
 }
 

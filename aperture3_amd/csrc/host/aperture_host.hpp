@@ -38,6 +38,8 @@ struct SimParams {
   // not in the reference: numeric form and mixed-form tile
   int precision = APT_PREC_FP64;
   int tile[3] = {0, 0, 0};
+  int tile_halo = 0;
+  int tile_threads = 0;
 };
 
 void check(apt_ctx *ctx, apt_status st, const char *what);

@@ -88,7 +88,8 @@ __device__ __forceinline__ float gather3g(const float *__restrict__ f,
   return lo * g0 + hi * g1;
 }
 
-__device__ __forceinline__ void vay_push_f(float &p1, float &p2, float &p3,
+// Vay push (vay_push.cuh:8-30) in FP32; returns 1/gamma of the new momentum
+__device__ __forceinline__ float vay_push_f(float &p1, float &p2, float &p3,
                                            float gamma, float E1, float E2,
                                            float E3, float B1, float B2,
                                            float B3) {
@@ -106,6 +107,7 @@ __device__ __forceinline__ void vay_push_f(float &p1, float &p2, float &p3,
   p1 = (up1 + B1 * ut * inv_gamma2 + (up2 * B3 - up3 * B2) * ign) * s;
   p2 = (up2 + B2 * ut * inv_gamma2 + (up3 * B1 - up1 * B3) * ign) * s;
   p3 = (up3 + B3 * ut * inv_gamma2 + (up1 * B2 - up2 * B1) * ign) * s;
+  return ign;
 }
 
 // per-species constants in FP32: qm = dt q / (2 m), q
@@ -193,19 +195,20 @@ __device__ __forceinline__ void push_move_2d(
     unsigned long long *counters, Deposit2 &D) {
   const int s = ptc_species(r.flag);
   float u1 = r.u1, u2 = r.u2, u3 = r.u3;
-  float gamma = sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+  const float gamma = sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+  float ig;
   if (!(r.flag & FLAG_IGNORE_EM)) {
     const float qm = qm_tab[s];
     float F[6];
     gather(r.x1, r.x2, F);
-    vay_push_f(u1, u2, u3, gamma, F[0] * qm, F[1] * qm, F[2] * qm,
-               F[3] * qm, F[4] * qm, F[5] * qm);
-    gamma = sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+    ig = vay_push_f(u1, u2, u3, gamma, F[0] * qm, F[1] * qm, F[2] * qm,
+                    F[3] * qm, F[4] * qm, F[5] * qm);
     p.u1[idx] = u1;
     p.u2[idx] = u2;
     p.u3[idx] = u3;
+  } else {
+    ig = 1.0f / gamma;
   }
-  const float ig = 1.0f / gamma;
   const float v1 = u1 * ig, v2 = u2 * ig, v3 = u3 * ig;
   float n1 = r.x1 + v1 * idx0;
   float n2 = r.x2 + v2 * idx1;
@@ -234,8 +237,10 @@ __device__ __forceinline__ void push_move_2d(
   const float qw = q_tab[s] * r.w;
   const float mqw = -qw;
   // in-plane fluxes (reference weight -q w, ptc_updater.cu:345-372)
-  const float wx0 = b[0] - a[0], wx1 = wx0 + (b[1] - a[1]);
-  const float wy0 = d[0] - c[0], wy1 = wy0 + (d[1] - c[1]);
+  // prefix sums of the shape change; the second one is written as minus
+  // the change of the last node, so it is exactly zero without a crossing
+  const float wx0 = b[0] - a[0], wx1 = a[2] - b[2];
+  const float wy0 = d[0] - c[0], wy1 = c[2] - d[2];
   float ax[3], ay[3];
 #pragma unroll
   for (int t = 0; t < 3; t++) {
@@ -504,8 +509,7 @@ __global__ void __launch_bounds__(NT)
       float B1 = gather3g(f.B[0], d0, d01, c1, c2, c3, x1, x2, x3, ST_B1) * qm;
       float B2 = gather3g(f.B[1], d0, d01, c1, c2, c3, x1, x2, x3, ST_B2) * qm;
       float B3 = gather3g(f.B[2], d0, d01, c1, c2, c3, x1, x2, x3, ST_B3) * qm;
-      vay_push_f(u1, u2, u3, gamma, E1, E2, E3, B1, B2, B3);
-      gamma = sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+      gamma = 1.0f / vay_push_f(u1, u2, u3, gamma, E1, E2, E3, B1, B2, B3);
       p.u1[idx] = u1;
       p.u2[idx] = u2;
       p.u3[idx] = u3;
@@ -600,6 +604,16 @@ __global__ void __launch_bounds__(NT)
   }
 }
 
+// 2D tile shapes compiled in: (T1, T2, halo, threads)
+#define TILE2D_SHAPES(X) \
+  X(16, 16, 6, 256)      \
+  X(16, 16, 6, 512)      \
+  X(16, 16, 5, 256)      \
+  X(16, 16, 4, 256)      \
+  X(32, 16, 6, 512)      \
+  X(32, 32, 6, 1024)     \
+  X(8, 8, 4, 256)
+
 hipError_t launch_push_deposit_tiled(const Mesh &m, const Species &sp,
                                      float dt, int S, const PtcF32 &p,
                                      const FieldsF32 &f, const AccMixed &a,
@@ -614,7 +628,7 @@ hipError_t launch_push_deposit_tiled(const Mesh &m, const Species &sp,
   }
   if (m.dim == 2) {
 #define LAUNCH2(A, B, HH, NTH)                                              \
-  if (ti.T[0] == A && ti.T[1] == B && ti.halo == HH) {                      \
+  if (ti.T[0] == A && ti.T[1] == B && ti.halo == HH && ti.threads == NTH) { \
     if (a.rho)                                                              \
       k_push_deposit_tile2d<A, B, HH, NTH, true><<<ti.ntiles, NTH, 0, st>>>( \
           m, spf, dt, scale, p, f, a, ti, counters);                        \
@@ -623,10 +637,7 @@ hipError_t launch_push_deposit_tiled(const Mesh &m, const Species &sp,
           m, spf, dt, scale, p, f, a, ti, counters);                        \
     return hipGetLastError();                                               \
   }
-    LAUNCH2(16, 16, 6, 256)
-    LAUNCH2(32, 16, 6, 512)
-    LAUNCH2(16, 16, 4, 256)
-    LAUNCH2(8, 8, 4, 256)
+    TILE2D_SHAPES(LAUNCH2)
 #undef LAUNCH2
     return hipErrorInvalidValue;
   } else if (m.dim == 3) {
@@ -647,13 +658,16 @@ hipError_t launch_push_deposit_tiled(const Mesh &m, const Species &sp,
   return hipGetLastError();
 }
 
-bool tile_shape_supported(int dim, const int T[3], int halo) {
-  if (dim == 2)
-    return (T[0] == 16 && T[1] == 16 && (halo == 6 || halo == 4)) ||
-           (T[0] == 32 && T[1] == 16 && halo == 6) ||
-           (T[0] == 8 && T[1] == 8 && halo == 4);
+bool tile_shape_supported(int dim, const int T[3], int halo, int threads) {
+  if (dim == 2) {
+#define CHECK2(A, B, HH, NTH) \
+  if (T[0] == A && T[1] == B && halo == HH && threads == NTH) return true;
+    TILE2D_SHAPES(CHECK2)
+#undef CHECK2
+    return false;
+  }
   if (dim == 3)
-    return T[0] == T[1] && T[1] == T[2] &&
+    return threads == 256 && T[0] == T[1] && T[1] == T[2] &&
            ((T[0] == 8 && (halo == 3 || halo == 4)) ||
             (T[0] == 4 && halo == 3));
   return false;

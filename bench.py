@@ -42,11 +42,16 @@ BYTES_PER_CELL = {"mixed": 60, "fp64": 120}   # Yee: read E,B,J; write E,B
 
 
 def make_sim(apt, precision, n_cells=N_CELLS, ppc=PPC):
+    from aperture3_amd import autotune
+    tuned = autotune.lookup(autotune.key(2, n_cells, ppc, THETA, DRIFT, SORT_INTERVAL))
+    tile = {}
+    if tuned and precision == apt.APT_PREC_MIXED:
+        tile = dict(tile=tuned["tile"], tile_halo=tuned["halo"], tile_threads=tuned["threads"])
     p = apt.SimParams(N=[n_cells, n_cells, 1], guard=[2, 2, 0],
                       size=[1.0, 1.0, 1.0], periodic_boundary=[True, True, True],
                       q_e=1.0, max_ptc_number=n_cells * n_cells * ppc * 2 + 1024,
                       precision=precision, data_interval=10 ** 9,
-                      sort_interval=SORT_INTERVAL, random_seed=43210)
+                      sort_interval=SORT_INTERVAL, random_seed=43210, **tile)
     env = apt.sim_environment(p)
     data = apt.sim_data(env)
     data.init_bg_B_field(2, lambda x, y, z: 1.0 + 0.0 * x)
@@ -78,6 +83,7 @@ def measure(apt, precision, steps, warmup):
     field_ms, field_n = data.timing_get(apt.KERNEL_FIELD)
     sort_ms, sort_n = data.timing_get(apt.KERNEL_SORT)
     counters = data.counters()
+    _, tile_ext = data.tile_offsets()
     data.timing_enable(False)
     data.close()
     el = t1 - t0
@@ -90,6 +96,8 @@ def measure(apt, precision, steps, warmup):
         "sort_ms": sort_ms / max(sort_n, 1) if sort_n else None,
         "sorts": sort_n,
         "counters": counters,
+        "tile": {"extent": list(tile_ext), "halo": p.tile_halo or None,
+                 "threads": p.tile_threads or None},
     }
 
 
@@ -188,6 +196,7 @@ def main():
             "avg_launch_ms": mixed["push_kernel_ms"],
         },
         "field_cell_updates_per_s": cells / (mixed["field_ms"] * 1e-3),
+        "tile": mixed.get("tile"),
         "sort_ms": mixed["sort_ms"],
         "mixed": mixed,
     }

@@ -88,6 +88,9 @@ typedef struct {
   uint64_t max_ptc_number;  /* max_ptc_number (:22) */
   uint64_t random_seed;     /* random_seed (:116) */
   int tile[3];              /* mixed form: tile extent in cells (0 = auto) */
+  int tile_halo;            /* mixed form: LDS halo in cells (0 = auto) */
+  int tile_threads;         /* mixed form: threads per tile workgroup
+                               (0 = auto) */
 } apt_params;
 
 /* Host view of particle_data (src/data/particle_data.h:50-61) in FP64.
