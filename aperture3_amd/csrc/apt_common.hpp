@@ -29,6 +29,33 @@ __host__ __device__ inline int ptc_species(uint32_t flag) {
 constexpr int ST_E1 = 0x6, ST_E2 = 0x5, ST_E3 = 0x3;
 constexpr int ST_B1 = 0x1, ST_B2 = 0x2, ST_B3 = 0x4;
 
+// Unsigned 32-bit division by a run-time constant without a divide
+// instruction (Granlund & Montgomery 1994, "Division by invariant integers
+// using multiplication", round-up method): q = (t + ((n - t) >> s1)) >> s2
+// with t = umulhi(n, m).  Exact for every 32-bit n and 1 <= d < 2^32.
+struct FastDiv {
+  uint32_t d, m, s1, s2;
+  __host__ __device__ inline uint32_t div(uint32_t n) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t t = __umulhi(n, m);
+#else
+    uint32_t t = (uint32_t)(((uint64_t)n * m) >> 32);
+#endif
+    return (t + ((n - t) >> s1)) >> s2;
+  }
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv f;
+  f.d = d;
+  uint32_t l = 0;
+  while (l < 32 && (1ull << l) < d) l++;  // l = ceil(log2 d)
+  f.m = (uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+  f.s1 = l < 1 ? l : 1;
+  f.s2 = l > 0 ? l - 1 : 0;
+  return f;
+}
+
 // Mesh with guard cells, x fastest (Quadmesh, quadmesh.h:16-402).
 struct Mesh {
   int dim;
@@ -39,6 +66,8 @@ struct Mesh {
   double delta[3];
   double inv_delta[3];
   double lower[3];
+  FastDiv div0;   // by dims[0]
+  FastDiv div01;  // by dims[0] * dims[1]
 
   __host__ __device__ inline uint64_t size() const {
     return (uint64_t)dims[0] * dims[1] * dims[2];
@@ -50,9 +79,11 @@ struct Mesh {
   }
   __host__ __device__ inline void split(uint32_t c, int &c1, int &c2,
                                         int &c3) const {
-    c1 = (int)(c % (uint32_t)dims[0]);
-    c2 = (int)((c / (uint32_t)dims[0]) % (uint32_t)dims[1]);
-    c3 = (int)(c / ((uint32_t)dims[0] * (uint32_t)dims[1]));
+    const uint32_t r = div0.div(c);   // c / dims[0]
+    c1 = (int)(c - r * (uint32_t)dims[0]);
+    const uint32_t k = div01.div(c);  // c / (dims[0] dims[1])
+    c2 = (int)(r - k * (uint32_t)dims[1]);
+    c3 = (int)k;
   }
   // periodic wrap of an index (replaces the MPI self-exchange fold and
   // particle migration of a single periodic rank)

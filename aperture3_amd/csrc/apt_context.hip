@@ -243,6 +243,8 @@ apt_status apt_create(const apt_grid *grid, const apt_params *params,
     if (grid->N[d] > 1) m.dim = d + 1;
   }
   if (m.dim == 0) m.dim = 1;
+  m.div0 = make_fastdiv((uint32_t)m.dims[0]);
+  m.div01 = make_fastdiv((uint32_t)m.dims[0] * (uint32_t)m.dims[1]);
   for (int d = 0; d < m.dim; d++)
     if (m.guard[d] < 2) {
       delete ctx;

@@ -88,12 +88,17 @@ __device__ __forceinline__ float gather3g(const float *__restrict__ f,
   return lo * g0 + hi * g1;
 }
 
+// hardware FP32 square root and reciprocal (<= 1 ulp); arguments here are
+// >= 1 or well inside the normal range
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Vay push (vay_push.cuh:8-30) in FP32; returns 1/gamma of the new momentum
 __device__ __forceinline__ float vay_push_f(float &p1, float &p2, float &p3,
                                            float gamma, float E1, float E2,
                                            float E3, float B1, float B2,
                                            float B3) {
-  float ig = 1.0f / gamma;
+  float ig = hw_rcp(gamma);
   float up1 = p1 + 2.0f * E1 + (p2 * B3 - p3 * B2) * ig;
   float up2 = p2 + 2.0f * E2 + (p3 * B1 - p1 * B3) * ig;
   float up3 = p3 + 2.0f * E3 + (p1 * B2 - p2 * B1) * ig;
@@ -101,9 +106,9 @@ __device__ __forceinline__ float vay_push_f(float &p1, float &p2, float &p3,
   float ut = up1 * B1 + up2 * B2 + up3 * B3;
   float sigma = 1.0f + up1 * up1 + up2 * up2 + up3 * up3 - tt;
   float inv_gamma2 =
-      2.0f / (sigma + sqrtf(sigma * sigma + 4.0f * (tt + ut * ut)));
-  float s = 1.0f / (1.0f + inv_gamma2 * tt);
-  float ign = sqrtf(inv_gamma2);
+      2.0f * hw_rcp(sigma + hw_sqrt(sigma * sigma + 4.0f * (tt + ut * ut)));
+  float s = hw_rcp(1.0f + inv_gamma2 * tt);
+  float ign = hw_sqrt(inv_gamma2);
   p1 = (up1 + B1 * ut * inv_gamma2 + (up2 * B3 - up3 * B2) * ign) * s;
   p2 = (up2 + B2 * ut * inv_gamma2 + (up3 * B1 - up1 * B3) * ign) * s;
   p3 = (up3 + B3 * ut * inv_gamma2 + (up1 * B2 - up2 * B1) * ign) * s;
@@ -120,11 +125,17 @@ __device__ __forceinline__ long long fxq(float c, float scale) {
   return (long long)__float2int_rn(c * scale);
 }
 
+// LDS atomics are issued unconditionally: an add of zero costs less than
+// the exec-mask branch that would skip it.
 __device__ __forceinline__ void lds_add(unsigned long long *a, long long v) {
-  if (v != 0)
-    __hip_atomic_fetch_add(a, (unsigned long long)v, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_WORKGROUP);
+  __hip_atomic_fetch_add(a, (unsigned long long)v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+
+// integer part (toward zero) of an already scaled contribution: one
+// conversion instruction; the truncation error, below 2^-S |q w|, is
+// symmetric about zero and far below the FP32 rounding of the contribution
+__device__ __forceinline__ long long fxi(float c) { return (long long)(int)c; }
 
 __device__ __forceinline__ void glb_add(long long *a, long long v) {
   if (v != 0)
@@ -195,7 +206,7 @@ __device__ __forceinline__ void push_move_2d(
     unsigned long long *counters, Deposit2 &D) {
   const int s = ptc_species(r.flag);
   float u1 = r.u1, u2 = r.u2, u3 = r.u3;
-  const float gamma = sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+  const float gamma = hw_sqrt(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
   float ig;
   if (!(r.flag & FLAG_IGNORE_EM)) {
     const float qm = qm_tab[s];
@@ -207,7 +218,7 @@ __device__ __forceinline__ void push_move_2d(
     p.u2[idx] = u2;
     p.u3[idx] = u3;
   } else {
-    ig = 1.0f / gamma;
+    ig = hw_rcp(gamma);
   }
   const float v1 = u1 * ig, v2 = u2 * ig, v3 = u3 * ig;
   float n1 = r.x1 + v1 * idx0;
@@ -223,7 +234,8 @@ __device__ __forceinline__ void push_move_2d(
     dc1 = max(-1, min(1, dc1));
     dc2 = max(-1, min(1, dc2));
   }
-  p.cell[idx] = (uint32_t)m.idx(m.wrap(0, c1 + dc1), m.wrap(1, c2 + dc2), 0);
+  p.cell[idx] = (uint32_t)m.wrap(0, c1 + dc1) +
+                (uint32_t)m.wrap(1, c2 + dc2) * (uint32_t)m.dims[0];
   p.x1[idx] = n1;
   p.x2[idx] = n2;
   p.x3[idx] = r.x3 + v3 * dt;
@@ -235,8 +247,8 @@ __device__ __forceinline__ void push_move_2d(
   shapes3(r.x1, n1, dc1, a, b);
   shapes3(r.x2, n2, dc2, c, d);
   const float qw = q_tab[s] * r.w;
-  const float mqw = -qw;
-  // in-plane fluxes (reference weight -q w, ptc_updater.cu:345-372)
+  // fixed-point scale folded into the per-particle factors
+  const float mqw = -qw * scale * 0.5f;
   // prefix sums of the shape change; the second one is written as minus
   // the change of the last node, so it is exactly zero without a crossing
   const float wx0 = b[0] - a[0], wx1 = a[2] - b[2];
@@ -244,30 +256,37 @@ __device__ __forceinline__ void push_move_2d(
   float ax[3], ay[3];
 #pragma unroll
   for (int t = 0; t < 3; t++) {
-    ax[t] = 0.5f * (a[t] + b[t]);
-    ay[t] = 0.5f * (c[t] + d[t]);
+    ax[t] = a[t] + b[t];  // twice the mean shape; the 1/2 is in mqw
+    ay[t] = c[t] + d[t];
   }
+  // in-plane fluxes (reference weight -q w, ptc_updater.cu:345-372)
+  const float fx0 = mqw * wx0, fx1 = mqw * wx1;
+  const float fy0 = mqw * wy0, fy1 = mqw * wy1;
 #pragma unroll
   for (int j = 0; j < 3; j++) {
-    D.j1[0][j] = fxq(mqw * wx0 * ay[j], scale);
-    D.j1[1][j] = fxq(mqw * wx1 * ay[j], scale);
-    D.j2[j][0] = fxq(mqw * wy0 * ax[j], scale);
-    D.j2[j][1] = fxq(mqw * wy1 * ax[j], scale);
+    D.j1[0][j] = fxi(fx0 * ay[j]);
+    D.j1[1][j] = fxi(fx1 * ay[j]);
+    D.j2[j][0] = fxi(fy0 * ax[j]);
+    D.j2[j][1] = fxi(fy1 * ax[j]);
   }
-  const float k3 = qw * v3 * (1.0f / 6.0f);
+  // out-of-plane: q w v3 (2 b d + a d + b c + 2 a c) / 6, factored
+  const float k3 = qw * v3 * (scale * (1.0f / 6.0f));
+  float e[3], g[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    e[j] = k3 * (2.0f * d[j] + c[j]);
+    g[j] = k3 * (d[j] + 2.0f * c[j]);
+  }
+  const float kr = qw * scale;
 #pragma unroll
   for (int t = 0; t < 3; t++)
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-      D.j3[t][j] = fxq(k3 * (2.0f * b[t] * d[j] + a[t] * d[j] + b[t] * c[j] +
-                             2.0f * a[t] * c[j]),
-                       scale);
-      if (RHO) D.r[t][j] = fxq(qw * b[t] * d[j], scale);
+      D.j3[t][j] = fxi(b[t] * e[j] + a[t] * g[j]);
+      if (RHO) D.r[t][j] = fxi(kr * b[t] * d[j]);
     }
 }
 
-// the whole update of one particle through global memory (used for the
-// rare particles that drifted out of their tile's LDS window)
 template <bool RHO>
 __device__ __forceinline__ void push_global_2d(const Mesh &m, const float *qm_tab,
                                             const float *q_tab, float dt,
@@ -279,8 +298,8 @@ __device__ __forceinline__ void push_global_2d(const Mesh &m, const float *qm_ta
   PtcRegs r;
   load_ptc(p, i, r);
   const int d0 = m.dims[0];
-  const int c1 = (int)(r.c % (uint32_t)d0);
-  const int c2 = (int)(r.c / (uint32_t)d0);
+  const int c2 = (int)m.div0.div(r.c);
+  const int c1 = (int)(r.c - (uint32_t)c2 * (uint32_t)d0);
   Deposit2 D;
   push_move_2d<RHO>(
       m, qm_tab, q_tab, dt, idx0, idx1, scale, p, i, r, c1, c2,
@@ -359,17 +378,24 @@ __global__ void __launch_bounds__(NT)
   const float idx0 = (float)m.inv_delta[0] * dt;
   const float idx1 = (float)m.inv_delta[1] * dt;
   unsigned long long outside = 0;
+  // tile-relative arrays: 32-bit offsets from scalar bases
+  PtcF32 pt = p;
+  pt.cell += beg; pt.flag += beg;
+  pt.x1 += beg; pt.x2 += beg; pt.x3 += beg;
+  pt.u1 += beg; pt.u2 += beg; pt.u3 += beg; pt.w += beg;
+  const uint32_t cnt = end - beg;
   PtcRegs nxt;
-  uint32_t idx = beg + threadIdx.x;
-  if (idx < end) load_ptc(p, idx, nxt);
-  for (; idx < end; idx += NT) {
+  uint32_t li = threadIdx.x;
+  if (li < cnt) load_ptc(pt, li, nxt);
+  for (; li < cnt; li += NT) {
+    const uint32_t idx = beg + li;
     const PtcRegs r = nxt;
-    if (idx + NT < end) load_ptc(p, idx + NT, nxt);
+    if (li + NT < cnt) load_ptc(pt, li + NT, nxt);
     if (r.c == MAX_CELL) continue;
-    const int c1 = (int)(r.c % (uint32_t)d0);
-    const int c2 = (int)(r.c / (uint32_t)d0);
+    const int c2 = (int)m.div0.div(r.c);
+    const int c1 = (int)(r.c - (uint32_t)c2 * (uint32_t)d0);
     if (!m.in_bulk(c1, c2, 0)) {
-      p.cell[idx] = MAX_CELL;
+      pt.cell[li] = MAX_CELL;
       continue;
     }
     // local coordinates in the window, minimal image for periodic wrap
@@ -391,7 +417,7 @@ __global__ void __launch_bounds__(NT)
     }
     Deposit2 D;
     push_move_2d<RHO>(
-        m, s_qm, s_q, dt, idx0, idx1, scale, p, idx, r, c1, c2,
+        m, s_qm, s_q, dt, idx0, idx1, scale, pt, li, r, c1, c2,
         [&](float x1, float x2, float F[6]) {
           F[0] = gather2(sF[0], W1, l1, l2, x1, x2, ST_E1);
           F[1] = gather2(sF[1], W1, l1, l2, x1, x2, ST_E2);
@@ -481,9 +507,8 @@ __global__ void __launch_bounds__(NT)
   for (uint32_t idx = beg + threadIdx.x; idx < end; idx += NT) {
     uint32_t c = p.cell[idx];
     if (c == MAX_CELL) continue;
-    int c1 = (int)(c % (uint32_t)d0);
-    int c2 = (int)((c / (uint32_t)d0) % (uint32_t)m.dims[1]);
-    int c3 = (int)(c / (uint32_t)d01);
+    int c1, c2, c3;
+    m.split(c, c1, c2, c3);
     if (!m.in_bulk(c1, c2, c3)) {
       p.cell[idx] = MAX_CELL;
       continue;
