@@ -91,6 +91,7 @@ SIGNATURES = {
     "apt_energy_histogram": [_P, C.POINTER(C.c_uint32), C.c_int, C.c_double, C.c_int],
     "apt_tile_offsets": [_P, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint32),
                          C.POINTER(C.c_int)],
+    "apt_stream_probe": [_P, C.c_uint64, C.c_int, C.POINTER(C.c_double)],
     "apt_sync": [_P],
     "apt_get_counters": [_P, C.POINTER(AptCounters)],
     "apt_reset_counters": [_P],
@@ -343,6 +344,13 @@ class sim_data:
             self.ctx, off.ctypes.data_as(C.POINTER(C.c_uint32)), off.size, C.byref(nt), ext),
             "apt_tile_offsets")
         return off, tuple(ext)
+
+    def stream_probe(self, n=None, reps=5):
+        """ms per launch of the push's particle stream with no arithmetic."""
+        ms = C.c_double()
+        n = self.num_particles() if n is None else n
+        self._check(self.lib.apt_stream_probe(self.ctx, n, reps, C.byref(ms)), "apt_stream_probe")
+        return ms.value
 
     # ---- control -----------------------------------------------------------
     def jexp(self):

@@ -978,6 +978,27 @@ apt_status apt_tile_offsets(apt_ctx *ctx, uint32_t *off, uint64_t cap,
   return APT_OK;
 }
 
+apt_status apt_stream_probe(apt_ctx *ctx, uint64_t n, int reps, double *ms) {
+  CHECK_CTX(ctx);
+  if (!ms || reps < 1 || ctx->prec != APT_PREC_MIXED || n > ctx->cap)
+    return ctx->fail(APT_ERR_INVALID_ARG, "stream_probe: mixed form, n <= cap");
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  HIPCHK(launch_stream_probe(ctx->p32, n, ctx->stream));  // warm-up
+  HIPCHK(hipEventRecord(a, ctx->stream));
+  for (int r = 0; r < reps; r++)
+    HIPCHK(launch_stream_probe(ctx->p32, n, ctx->stream));
+  HIPCHK(hipEventRecord(b, ctx->stream));
+  HIPCHK(hipEventSynchronize(b));
+  float t = 0.0f;
+  HIPCHK(hipEventElapsedTime(&t, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms = t / reps;
+  return APT_OK;
+}
+
 // ---- control ------------------------------------------------------------
 apt_status apt_sync(apt_ctx *ctx) {
   CHECK_CTX(ctx);

@@ -67,4 +67,6 @@ hipError_t tile_offsets_from_sorted(const uint32_t *keys, uint64_t n,
                                     uint32_t ntiles, uint32_t *off,
                                     hipStream_t st);
 
+hipError_t launch_stream_probe(const PtcF32 &p, uint64_t n, hipStream_t st);
+
 }  // namespace apt

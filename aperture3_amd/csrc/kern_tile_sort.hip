@@ -374,4 +374,31 @@ hipError_t tile_offsets_from_sorted(const uint32_t *keys, uint64_t n,
   return hipGetLastError();
 }
 
+// Speed-of-light probe: the push's 36-byte read and 28-byte write per
+// particle, values written back unchanged (x1 gets x1 + 0 * w, so the loads
+// cannot be elided).
+__global__ void __launch_bounds__(256)
+    k_stream_probe(PtcF32 p, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c = p.cell[i];
+  uint32_t fl = p.flag[i];
+  float x1 = p.x1[i], x2 = p.x2[i], x3 = p.x3[i];
+  float u1 = p.u1[i], u2 = p.u2[i], u3 = p.u3[i], w = p.w[i];
+  float z = (fl == 0xffffffffu) ? w : 0.0f;
+  p.cell[i] = c;
+  p.x1[i] = x1 + z;
+  p.x2[i] = x2;
+  p.x3[i] = x3;
+  p.u1[i] = u1;
+  p.u2[i] = u2;
+  p.u3[i] = u3;
+}
+
+hipError_t launch_stream_probe(const PtcF32 &p, uint64_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  k_stream_probe<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(p, n);
+  return hipGetLastError();
+}
+
 }  // namespace apt

@@ -84,6 +84,7 @@ def measure(apt, precision, steps, warmup):
     sort_ms, sort_n = data.timing_get(apt.KERNEL_SORT)
     counters = data.counters()
     _, tile_ext = data.tile_offsets()
+    stream_ms = data.stream_probe() if precision == apt.APT_PREC_MIXED else None
     data.timing_enable(False)
     data.close()
     el = t1 - t0
@@ -96,6 +97,7 @@ def measure(apt, precision, steps, warmup):
         "sort_ms": sort_ms / max(sort_n, 1) if sort_n else None,
         "sorts": sort_n,
         "counters": counters,
+        "stream_probe_ms": stream_ms,
         "tile": {"extent": list(tile_ext), "halo": p.tile_halo or None,
                  "threads": p.tile_threads or None},
     }
@@ -194,6 +196,7 @@ def main():
             "traffic": None,
             "bytes_per_push": BYTES_PER_PUSH["mixed"],
             "avg_launch_ms": mixed["push_kernel_ms"],
+            "stream_ceiling_GBps": n * BYTES_PER_PUSH["mixed"] / (mixed["stream_probe_ms"] * 1e-3) / 1e9,
         },
         "field_cell_updates_per_s": cells / (mixed["field_ms"] * 1e-3),
         "tile": mixed.get("tile"),

@@ -182,6 +182,13 @@ apt_status apt_energy_histogram(apt_ctx *ctx, uint32_t *hist, int nbins,
 apt_status apt_tile_offsets(apt_ctx *ctx, uint32_t *off, uint64_t cap,
                             uint32_t *ntiles, int tile_extent[3]);
 
+/* Measurement (no reference counterpart): the speed of light of the mixed
+ * form's particle stream.  Runs `reps` launches of a kernel that reads the
+ * 9 particle arrays of the first n particles and writes 7 of them back
+ * (unchanged) -- the push's HBM traffic with no arithmetic -- and returns
+ * the average milliseconds per launch. */
+apt_status apt_stream_probe(apt_ctx *ctx, uint64_t n, int reps, double *ms);
+
 /* ---- control -------------------------------------------------------- */
 apt_status apt_sync(apt_ctx *ctx);
 apt_status apt_get_counters(apt_ctx *ctx, apt_counters *out);
