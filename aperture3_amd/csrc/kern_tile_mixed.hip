@@ -194,6 +194,7 @@ struct Deposit2 {
   long long j3[3][3];
   long long r[3][3];
   int di0, dj0;        // window origin relative to the old cell
+  int dc1, dc2;        // cell crossing in x and y
   int s;
   bool cur;            // deposits current
 };
@@ -243,6 +244,8 @@ __device__ __forceinline__ void push_move_2d(
   D.cur = !(r.flag & FLAG_IGNORE_CURRENT);
   D.di0 = min(dc1, 0) - 1;
   D.dj0 = min(dc2, 0) - 1;
+  D.dc1 = dc1;
+  D.dc2 = dc2;
   float a[3], b[3], c[3], d[3];  // x old/new, y old/new
   shapes3(r.x1, n1, dc1, a, b);
   shapes3(r.x2, n2, dc2, c, d);
@@ -429,26 +432,50 @@ __global__ void __launch_bounds__(NT)
         counters, D);
     if (!D.cur) continue;
     const int b0 = (l1 + D.di0) + (l2 + D.dj0) * W1;
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      lds_add(&sJ[0][b0 + 1 + j * W1], D.j1[0][j]);
-      lds_add(&sJ[0][b0 + 2 + j * W1], D.j1[1][j]);
-      lds_add(&sJ[1][b0 + j + W1], D.j2[j][0]);
-      lds_add(&sJ[1][b0 + j + 2 * W1], D.j2[j][1]);
+    unsigned long long *J1 = &sJ[0][b0], *J2 = &sJ[1][b0], *J3 = &sJ[2][b0];
+    // the 2x2 core is non-zero for every particle; the third column (row)
+    // only when the particle crossed in x (y) -- masked lanes cost no LDS
+    // bandwidth
+    lds_add(J1 + 1, D.j1[0][0]);
+    lds_add(J1 + 1 + W1, D.j1[0][1]);
+    lds_add(J2 + W1, D.j2[0][0]);
+    lds_add(J2 + 1 + W1, D.j2[1][0]);
+    lds_add(J3, D.j3[0][0]);
+    lds_add(J3 + 1, D.j3[1][0]);
+    lds_add(J3 + W1, D.j3[0][1]);
+    lds_add(J3 + 1 + W1, D.j3[1][1]);
+    const bool xc = D.dc1 != 0, yc = D.dc2 != 0;
+    if (xc) {
+      lds_add(J1 + 2, D.j1[1][0]);
+      lds_add(J1 + 2 + W1, D.j1[1][1]);
+      lds_add(J2 + 2 + W1, D.j2[2][0]);
+      lds_add(J3 + 2, D.j3[2][0]);
+      lds_add(J3 + 2 + W1, D.j3[2][1]);
     }
+    if (yc) {
+      lds_add(J1 + 1 + 2 * W1, D.j1[0][2]);
+      lds_add(J2 + 2 * W1, D.j2[0][1]);
+      lds_add(J2 + 1 + 2 * W1, D.j2[1][1]);
+      lds_add(J3 + 2 * W1, D.j3[0][2]);
+      lds_add(J3 + 1 + 2 * W1, D.j3[1][2]);
+    }
+    if (xc && yc) {
+      lds_add(J1 + 2 + 2 * W1, D.j1[1][2]);
+      lds_add(J2 + 2 + 2 * W1, D.j2[2][1]);
+      lds_add(J3 + 2 + 2 * W1, D.j3[2][2]);
+    }
+    if (RHO) {
 #pragma unroll
-    for (int tt = 0; tt < 3; tt++)
+      for (int tt = 0; tt < 3; tt++)
 #pragma unroll
-      for (int j = 0; j < 3; j++) {
-        lds_add(&sJ[2][b0 + tt + j * W1], D.j3[tt][j]);
-        if (RHO) {
+        for (int j = 0; j < 3; j++) {
           if (D.s < 2)
             lds_add(&sR[D.s][b0 + tt + j * W1], D.r[tt][j]);
           else
             glb_add(a.R[D.s] + m.widx(c1 + D.di0 + tt, c2 + D.dj0 + j, 0),
                     D.r[tt][j]);
         }
-      }
+    }
   }
   __syncthreads();
   // particles outside the window: global gathers and atomics
