@@ -103,6 +103,19 @@ def measure(apt, precision, steps, warmup):
     }
 
 
+def profiled_traffic():
+    """HBM bytes per launch of the push kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "push_deposit_c2_mixed.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline():
     """The FP64 oracle (a C port of the reference CUDA path, one core) on a
     bounded sample of the same workload: 512 x 512 cells at 32 ppc per
@@ -193,7 +206,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic(),
+            "traffic_source": "profiles/push_deposit_c2_mixed.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1)",
             "bytes_per_push": BYTES_PER_PUSH["mixed"],
             "avg_launch_ms": mixed["push_kernel_ms"],
             "stream_ceiling_GBps": n * BYTES_PER_PUSH["mixed"] / (mixed["stream_probe_ms"] * 1e-3) / 1e9,
