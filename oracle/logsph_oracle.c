@@ -1,0 +1,660 @@
+/*
+ * logsph_oracle.c -- CPU FP64 restatement of Aperture's 2D log-spherical
+ * (ln r, theta) particle and field update.  TEST INFRASTRUCTURE ONLY (the
+ * checker, never the product): see pic_oracle.h.
+ *
+ * Reference files restated (relative to the Aperture3 tree):
+ *   src/grids/grid_log_sph.cpp, grid_log_sph.h   metric coefficients
+ *   src/cuda/algorithms/user_push_2d_logsph.cuh  push (Vay + gravity +
+ *                                                perpendicular cooling)
+ *   src/cuda/algorithms/gravity.cuh, sync_cooling.cuh
+ *   src/cuda/algorithms/ptc_updater_logsph.cu    move + deposit, process_j,
+ *                                                filter, injection, axis and
+ *                                                outflow particle boundaries
+ *   src/cuda/algorithms/field_solver_logsph.cu   integral-form Yee, divs,
+ *                                                star / outflow / axis BCs
+ *
+ * Deviations shared with the HIP product (documented in the design):
+ *   - current and charge go through the two-limb int64 fixed point;
+ *   - J3 is accumulated as q w v3 S and divided by the cell volume once per
+ *     cell in process_j (the reference divides every contribution);
+ *   - exp/sin/cos/acos/atan2/log on the particle path are the orc_math.c
+ *     versions, identical on the device;
+ *   - a move of more than one cell removes the particle (the reference
+ *     prints a debug message and writes out of its stencil);
+ *   - injected particles go to compact slots, one block per injected theta
+ *     row (the reference indexes by absolute row and over-runs its count).
+ */
+#include <math.h>
+#include <string.h>
+
+#include "orc_internal.h"
+
+#define LS_PI 3.14159265358979323846
+#define FLAG_PRIMARY (1u << 6)
+#define FLAG_TRACKED (1u << 1)
+
+/* Quadmesh::pos(i, n, pos_in_cell) (quadmesh.h:144-150) */
+static inline double lpos(const orc_mesh *m, int d, int n, double x) {
+  return m->lower[d] + m->delta[d] * ((double)(n - m->guard[d]) + x);
+}
+static inline double spos(const orc_mesh *m, int d, int n, int stagger) {
+  return lpos(m, d, n, stagger ? 1.0 : 0.5);
+}
+
+/* Grid_LogSph::l1 / A2 / V3 (grid_log_sph.h:25-39) */
+static double g_l1(double r, double rs) {
+  double a = sqrt(1.0 - rs / r);
+  return r * a + 0.5 * rs * log(2.0 * r * (1.0 + a) - rs);
+}
+static double g_A2(double r, double rs) {
+  double a = sqrt(1.0 - rs / r);
+  return 0.25 * r * a * (2.0 * r + 3.0 * rs) +
+         0.375 * rs * rs * log(2.0 * r * (1.0 + a) - rs);
+}
+static double g_V3(double r, double rs) {
+  double a = sqrt(1.0 - rs / r);
+  return r * a * (8.0 * r * r + 10.0 * r * rs + 15.0 * rs * rs) / 24.0 +
+         0.3125 * rs * rs * rs * log(2.0 * r * (1.0 + a) - rs);
+}
+
+/* Grid_LogSph::compute_coef (grid_log_sph.cpp:10-104), r_g = 0 as at :14.
+ * Host-side FP64 with the C library, like the reference. */
+void orc_logsph_compute_coef(const orc_mesh *m, orc_logsph_coef *c) {
+  const double r_g = 0.0;
+  const int d0 = m->dims[0];
+  const double D0 = m->delta[0], D1 = m->delta[1];
+  for (int j = 0; j < m->dims[1]; j++) {
+    double x2 = spos(m, 1, j, 0);
+    double x2s = spos(m, 1, j, 1);
+    for (int i = 0; i < d0; i++) {
+      size_t o = (size_t)i + (size_t)j * d0;
+      double x1 = spos(m, 0, i, 0);
+      double x1s = spos(m, 0, i, 1);
+      double r_plus = exp(x1 + D0);
+      double r = exp(x1);
+      double rs = exp(x1s);
+      double rs_minus = exp(x1s - D0);
+      int axis = fabs(x2s) < 1.0e-5 || fabs(x2s - LS_PI) < 1.0e-5;
+      c->a[ORC_L1E][o] = g_l1(rs, r_g) - g_l1(rs_minus, r_g);
+      c->a[ORC_L2E][o] = rs * D1;
+      c->a[ORC_L3E][o] = rs * sin(x2s);
+      c->a[ORC_L1B][o] = g_l1(r_plus, r_g) - g_l1(r, r_g);
+      c->a[ORC_L2B][o] = r * D1;
+      c->a[ORC_L3B][o] = r * sin(x2);
+      c->a[ORC_A1E][o] = r * r * (cos(x2) - cos(x2 + D1));
+      if (axis) c->a[ORC_A1E][o] = r * r * 2.0 * (1.0 - cos(0.5 * D1));
+      c->a[ORC_A2E][o] = (g_A2(r_plus, r_g) - g_A2(r, r_g)) * sin(x2);
+      c->a[ORC_A3E][o] = (g_A2(r_plus, r_g) - g_A2(r, r_g)) * D1;
+      c->a[ORC_A1B][o] = rs * rs * (cos(x2s - D1) - cos(x2s));
+      c->a[ORC_A2B][o] = (g_A2(rs, r_g) - g_A2(rs_minus, r_g)) * sin(x2s);
+      c->a[ORC_A3B][o] = (g_A2(rs, r_g) - g_A2(rs_minus, r_g)) * D1;
+      c->a[ORC_DV][o] = (g_V3(r_plus, r_g) - g_V3(r, r_g)) *
+                        (cos(x2) - cos(x2 + D1)) / (D0 * D1);
+      if (axis)
+        c->a[ORC_DV][o] = (g_V3(r_plus, r_g) - g_V3(r, r_g)) * 2.0 *
+                          (1.0 - cos(0.5 * D1)) / (D0 * D1);
+    }
+  }
+}
+
+/* beta_phi (field_solver_logsph.cu:16-20); alpha_gr == 1 (:22-26). */
+static inline double beta_phi(const orc_logsph_params *lp, double r,
+                              double theta) {
+  return -0.4 * lp->compactness * lp->omega * orc_sin(theta) / (r * r);
+}
+
+/* logsph2cart / cart2logsph (ptc_updater_logsph.cu:26-44) */
+static inline void logsph2cart(double *v1, double *v2, double *v3, double x2,
+                               double x3) {
+  double v1n = *v1, v2n = *v2, v3n = *v3;
+  double s2, c2, s3, c3;
+  orc_sincos(x2, &s2, &c2);
+  orc_sincos(x3, &s3, &c3);
+  *v1 = v1n * s2 * c3 + v2n * c2 * c3 - v3n * s3;
+  *v2 = v1n * s2 * s3 + v2n * c2 * s3 + v3n * c3;
+  *v3 = v1n * c2 - v2n * s2;
+}
+static inline void cart2logsph(double *v1, double *v2, double *v3, double x2,
+                               double x3) {
+  double v1n = *v1, v2n = *v2, v3n = *v3;
+  double s2, c2, s3, c3;
+  orc_sincos(x2, &s2, &c2);
+  orc_sincos(x3, &s3, &c3);
+  *v1 = v1n * s2 * c3 + v2n * s2 * s3 + v3n * c2;
+  *v2 = v1n * c2 * c3 + v2n * c2 * s3 - v3n * s2;
+  *v3 = -v1n * s3 + v2n * c3;
+}
+
+/* One particle: user_push_2d_logsph<1> (user_push_2d_logsph.cuh:14-114)
+ * then deposit_and_move_2d_log_sph<1> (ptc_updater_logsph.cu:134-282). */
+static void logsph_push_one(const orc_mesh *m, const orc_logsph_params *lp,
+                            orc_fields *f, orc_ptc *p, const orc_species *sp,
+                            double dt, size_t idx, uint64_t *multi) {
+  uint32_t c = p->cell[idx];
+  if (c == ORC_MAX_CELL) return;
+  if (!is_in_bulk(m, c)) {
+    p->cell[idx] = ORC_MAX_CELL;
+    return;
+  }
+  const int d0 = m->dims[0];
+  int c1 = (int)(c % (uint32_t)d0);
+  int c2 = (int)(c / (uint32_t)d0);
+  uint32_t flag = p->flag[idx];
+  int s = (int)(flag >> 29);
+  double old_x1 = p->x1[idx], old_x2 = p->x2[idx], old_x3 = p->x3[idx];
+  double p1 = p->p1[idx], p2 = p->p2[idx], p3 = p->p3[idx];
+  double gamma;
+  double r = orc_exp(lpos(m, 0, c1, old_x1));
+  double q_over_m = dt * 0.5 * sp->q[s] / sp->m[s];
+  gamma = sqrt(1.0 + p1 * p1 + p2 * p2 + p3 * p3);
+  if (!(flag & FLAG_IGNORE_EM)) {
+    /* alpha_gr == 1: alpha * interp is the interpolated value itself */
+    double E1 = gather2d(f->E[0], d0, old_x1, old_x2, c1, c2, ST_E1) * q_over_m;
+    double E2 = gather2d(f->E[1], d0, old_x1, old_x2, c1, c2, ST_E2) * q_over_m;
+    double E3 = gather2d(f->E[2], d0, old_x1, old_x2, c1, c2, ST_E3) * q_over_m;
+    double B1 = gather2d(f->B[0], d0, old_x1, old_x2, c1, c2, ST_B1) * q_over_m;
+    double B2 = gather2d(f->B[1], d0, old_x1, old_x2, c1, c2, ST_B2) * q_over_m;
+    double B3 = gather2d(f->B[2], d0, old_x1, old_x2, c1, c2, ST_B3) * q_over_m;
+    vay_push(&p1, &p2, &p3, &gamma, E1, E2, E3, B1, B2, B3);
+    /* gravity (gravity.cuh:13-26): p1 -= dt g m / (r^2 |q|), gamma anew */
+    double g = lp->gravity_on ? lp->gravity : 0.0;
+    p1 -= dt * g * sp->m[s] / (r * r * fabs(sp->q[s]));
+    gamma = sqrt(1.0 + p1 * p1 + p2 * p2 + p3 * p3);
+    /* sync_kill_perp (sync_cooling.cuh:41-68), leptons only */
+    if (lp->rad_cooling_on && s != 2) {
+      B1 /= q_over_m;
+      B2 /= q_over_m;
+      B3 /= q_over_m;
+      double B_sqr = B1 * B1 + B2 * B2 + B3 * B3;
+      double pdotB = (p1 * B1 + p2 * B2 + p3 * B3);
+      double dp1 = -lp->rad_cooling_coef * (p1 - B1 * pdotB / B_sqr);
+      double dp2 = -lp->rad_cooling_coef * (p2 - B2 * pdotB / B_sqr);
+      double dp3 = -lp->rad_cooling_coef * (p3 - B3 * pdotB / B_sqr);
+      double fB = sqrt(B_sqr) / lp->B0;
+      p1 += dp1 * fB;
+      p2 += dp2 * fB;
+      p3 += dp3 * fB;
+      double pp = sqrt(p1 * p1 + p2 * p2 + p3 * p3);
+      gamma = sqrt(1.0 + pp * pp);
+    }
+    p->p1[idx] = p1;
+    p->p2[idx] = p2;
+    p->p3[idx] = p3;
+    p->E[idx] = gamma;
+  }
+
+  /* ---- move in Cartesian, back to (ln r, theta, phi), deposit ---- */
+  double v1 = p->p1[idx], v2 = p->p2[idx], v3 = p->p3[idx];
+  gamma = p->E[idx];
+  double w = p->w[idx];
+  double r1 = lpos(m, 0, c1, old_x1);
+  double exp_r1 = orc_exp(r1);
+  double r2 = lpos(m, 1, c2, old_x2);
+  v1 = v1 / gamma;
+  v2 = v2 / gamma;
+  v3 = v3 / gamma;
+  double v3_gr = v3 - beta_phi(lp, exp_r1, r2);
+  double s2, c2v, s3, c3;
+  orc_sincos(r2, &s2, &c2v);
+  orc_sincos(old_x3, &s3, &c3);
+  double x = exp_r1 * s2 * c3;
+  double y = exp_r1 * s2 * s3;
+  double z = exp_r1 * c2v;
+  logsph2cart(&v1, &v2, &v3_gr, r2, old_x3);
+  x += v1 * dt;
+  y += v2 * dt;
+  z += v3_gr * dt;
+  double r1p = sqrt(x * x + y * y + z * z);
+  double r2p = orc_acos(z / r1p);
+  double exp_r1p = r1p;
+  r1p = orc_log(r1p);
+  double r3p = orc_atan2(y, x);
+  cart2logsph(&v1, &v2, &v3_gr, r2p, r3p);
+  double np1 = v1 * gamma, np2 = v2 * gamma;
+  double np3 = (v3_gr + beta_phi(lp, exp_r1p, r2p)) * gamma;
+  double new_x1 = old_x1 + (r1p - r1) / m->delta[0];
+  double new_x2 = old_x2 + (r2p - r2) / m->delta[1];
+  int dc1 = (int)floor(new_x1);
+  int dc2 = (int)floor(new_x2);
+  if (dc1 < -1 || dc1 > 1 || dc2 < -1 || dc2 > 1) {
+    if (multi) (*multi)++;
+    p->cell[idx] = ORC_MAX_CELL;
+    return;
+  }
+  new_x1 -= (double)dc1;
+  new_x2 -= (double)dc2;
+  if (lpos(m, 1, c2 + dc2, new_x2) < 0.0) {
+    dc2 += 1;
+    new_x2 = 1.0 - new_x2;
+    np2 = -np2;
+    np3 = -np3;
+  }
+  if (lpos(m, 1, c2 + dc2, new_x2) >= LS_PI) {
+    dc2 -= 1;
+    new_x2 = 1.0 - new_x2;
+    np2 = -np2;
+    np3 = -np3;
+  }
+  p->p1[idx] = np1;
+  p->p2[idx] = np2;
+  p->p3[idx] = np3;
+  p->cell[idx] = (uint32_t)((c1 + dc1) + (c2 + dc2) * d0);
+  p->x1[idx] = new_x1;
+  p->x2[idx] = new_x2;
+  p->x3[idx] = r3p;
+
+  if (flag & FLAG_IGNORE_CURRENT) return;
+  const int H = sp->jexp;
+  double weight = -sp->q[s] * w;
+  int j0 = (dc2 == -1 ? -2 : -1), j1 = (dc2 == 1 ? 1 : 0);
+  int i0 = (dc1 == -1 ? -2 : -1), i1 = (dc1 == 1 ? 1 : 0);
+  double djy[3] = {0.0, 0.0, 0.0};
+  for (int j = j0; j <= j1; j++) {
+    double sy0 = cic(-old_x2 + j + 1);
+    double sy1 = cic(-new_x2 + (j + 1 - dc2));
+    double djx = 0.0;
+    for (int i = i0; i <= i1; i++) {
+      double sx0 = cic(-old_x1 + i + 1);
+      double sx1 = cic(-new_x1 + (i + 1 - dc1));
+      size_t o = (size_t)(i + c1) + (size_t)(j + c2) * d0;
+      djx += movement2d(sy0, sy1, sx0, sx1);
+      acc_add(f->Jacc[0], o + 1, weight * djx, H);
+      djy[i - i0] += movement2d(sx0, sx1, sy0, sy1);
+      acc_add(f->Jacc[1], o + d0, weight * djy[i - i0], H);
+      double val2 = center2d(sx0, sx1, sy0, sy1);
+      acc_add(f->Jacc[2], o, -weight * v3_gr * val2, H);
+      if (sp->deposit_rho) {
+        double s1 = sx1 * sy1;
+        acc_add(f->Racc[s], o, -weight * s1, H);
+      }
+    }
+  }
+}
+
+/* filter_current_logsph (ptc_updater_logsph.cu:616-666) on a single-rank
+ * domain (all four boundary flags set), then the whole-array copy back
+ * (:752): the guard ring takes tmp's, which no pass writes. */
+static void logsph_filter(const orc_mesh *m, double *j, double *tmp,
+                          const double *A) {
+  const int d0 = m->dims[0];
+  for (int n2 = m->guard[1]; n2 < m->dims[1] - m->guard[1]; n2++)
+    for (int n1 = m->guard[0]; n1 < m->dims[0] - m->guard[0]; n1++) {
+      size_t o = (size_t)n1 + (size_t)n2 * d0;
+      size_t drp = n1 == m->dims[0] - m->guard[0] - 1 ? 0 : 1;
+      size_t drm = n1 == m->guard[0] ? 0 : 1;
+      size_t dtp = n2 == m->dims[1] - m->guard[1] - 1 ? 0 : (size_t)d0;
+      size_t dtm = n2 == m->guard[1] ? 0 : (size_t)d0;
+      double t = 0.25 * j[o] * A[o];
+      t += 0.125 * j[o + drp] * A[o + drp];
+      t += 0.125 * j[o - drm] * A[o - drm];
+      t += 0.125 * j[o + dtp] * A[o + dtp];
+      t += 0.125 * j[o - dtm] * A[o - dtm];
+      t += 0.0625 * j[o + drp + dtp] * A[o + drp + dtp];
+      t += 0.0625 * j[o - drm + dtp] * A[o - drm + dtp];
+      t += 0.0625 * j[o + drp - dtm] * A[o + drp - dtm];
+      t += 0.0625 * j[o - drm - dtm] * A[o - drm - dtm];
+      t /= A[o];
+      tmp[o] = t;
+    }
+  memcpy(j, tmp, sizeof(double) * (size_t)m->dims[0] * m->dims[1]);
+}
+
+/* ptc_updater_logsph::update_particles (ptc_updater_logsph.cu:681-810) and
+ * apply_boundary (:812-846) on one rank: zero J/Rho, push + move + deposit,
+ * process_j (:284-299), smoothing, clear guard cells, axis J3 (:414-438),
+ * outflow flag (:397-412).  jtmp needs dims0*dims1 doubles, guards zero. */
+uint64_t orc_logsph_ptc_update(const orc_mesh *m, const orc_logsph_coef *c,
+                               const orc_logsph_params *lp, orc_fields *f,
+                               orc_ptc *p, const orc_species *sp,
+                               double *jtmp, double dt, uint32_t step) {
+  (void)step;
+  const int d0 = m->dims[0], d1 = m->dims[1];
+  const size_t n = (size_t)d0 * d1;
+  uint64_t multi = 0;
+  for (int k = 0; k < 3; k++) {
+    memset(f->Jacc[k][0], 0, n * sizeof(int64_t));
+    memset(f->Jacc[k][1], 0, n * sizeof(int64_t));
+  }
+  for (int s = 0; s < sp->num_species; s++) {
+    memset(f->Racc[s][0], 0, n * sizeof(int64_t));
+    memset(f->Racc[s][1], 0, n * sizeof(int64_t));
+  }
+  for (uint64_t i = 0; i < p->number; i++)
+    logsph_push_one(m, lp, f, p, sp, dt, i, &multi);
+  /* process_j: J1,J2 *= (D0 D1/dt)/A; J3 and Rho /= dV, every cell */
+  const double wj = m->delta[0] * m->delta[1] / dt;
+  for (size_t o = 0; o < n; o++) {
+    double j1 = orc_fx_value(f->Jacc[0][0][o], f->Jacc[0][1][o], sp->jexp);
+    double j2 = orc_fx_value(f->Jacc[1][0][o], f->Jacc[1][1][o], sp->jexp);
+    double j3 = orc_fx_value(f->Jacc[2][0][o], f->Jacc[2][1][o], sp->jexp);
+    f->J[0][o] = j1 * (wj / c->a[ORC_A1E][o]);
+    f->J[1][o] = j2 * (wj / c->a[ORC_A2E][o]);
+    f->J[2][o] = j3 / c->a[ORC_DV][o];
+    for (int s = 0; s < sp->num_species; s++) {
+      double rho = orc_fx_value(f->Racc[s][0][o], f->Racc[s][1][o], sp->jexp);
+      f->Rho[s][o] = rho / c->a[ORC_DV][o];
+    }
+  }
+  for (int pass = 0; pass < lp->current_smoothing; pass++) {
+    logsph_filter(m, f->J[0], jtmp, c->a[ORC_A1E]);
+    logsph_filter(m, f->J[1], jtmp, c->a[ORC_A2E]);
+    logsph_filter(m, f->J[2], jtmp, c->a[ORC_A3E]);
+    if (sp->deposit_rho)
+      for (int s = 0; s < sp->num_species; s++)
+        logsph_filter(m, f->Rho[s], jtmp, c->a[ORC_DV]);
+  }
+  orc_clear_guard_cells(m, p);
+  const int g1 = m->guard[1], jl = d1 - m->guard[1];
+  for (int i = 0; i < d0; i++) {
+    f->J[2][(size_t)i + (size_t)(g1 - 1) * d0] = 0.0;
+    f->J[2][(size_t)i + (size_t)g1 * d0] = 0.0;
+    f->J[2][(size_t)i + (size_t)jl * d0] = 0.0;
+    f->J[2][(size_t)i + (size_t)(jl - 1) * d0] = 0.0;
+  }
+  for (uint64_t i = 0; i < p->number; i++) {
+    uint32_t cc = p->cell[i];
+    if (cc == ORC_MAX_CELL) continue;
+    if (p->flag[i] & FLAG_IGNORE_EM) continue;
+    int c1 = (int)(cc % (uint32_t)d0);
+    if (c1 > d0 - lp->damping_length + 2) p->flag[i] |= FLAG_IGNORE_EM;
+  }
+  return multi;
+}
+
+#define F2(a, i, j) (a)[(size_t)(i) + (size_t)(j) * d0]
+
+/* compute_b_update_logsph (field_solver_logsph.cu:117-185), alpha_gr = 1 */
+static void logsph_update_b(const orc_mesh *m, const orc_logsph_coef *c,
+                            const orc_logsph_params *lp, orc_fields *f,
+                            double dt) {
+  const int d0 = m->dims[0];
+  double *E1 = f->E[0], *E2 = f->E[1], *E3 = f->E[2];
+  double *B1 = f->B[0], *B2 = f->B[1], *B3 = f->B[2];
+  const double *G1 = f->Ebg[0], *G2 = f->Ebg[1], *G3 = f->Ebg[2];
+  const double *H1 = f->Bbg[0], *H2 = f->Bbg[1];
+  const double *l1e = c->a[ORC_L1E], *l2e = c->a[ORC_L2E], *l3e = c->a[ORC_L3E];
+  const double *A1b = c->a[ORC_A1B], *A2b = c->a[ORC_A2B], *A3b = c->a[ORC_A3B];
+  for (int j = m->guard[1]; j < m->dims[1] - m->guard[1]; j++)
+    for (int i = m->guard[0]; i < m->dims[0] - m->guard[0]; i++) {
+      size_t o = (size_t)i + (size_t)j * d0;
+      double r1 = orc_exp(spos(m, 0, i, 1));
+      double r0 = orc_exp(spos(m, 0, i - 1, 1));
+      double r = orc_exp(spos(m, 0, i, 0));
+      B1[o] += -dt *
+               ((F2(E3, i, j) - F2(G3, i, j)) * F2(l3e, i, j) -
+                (F2(E3, i, j - 1) - F2(G3, i, j - 1)) * F2(l3e, i, j - 1)) /
+               A1b[o];
+      B2[o] += -dt *
+               ((F2(E3, i - 1, j) - F2(G3, i - 1, j)) * F2(l3e, i - 1, j) -
+                (F2(E3, i, j) - F2(G3, i, j)) * F2(l3e, i, j)) /
+               A2b[o];
+      double th0 = spos(m, 1, j, 0);
+      B3[o] += -dt *
+               ((((F2(E2, i, j) - F2(G2, i, j)) +
+                  F2(H1, i, j) * beta_phi(lp, r1, th0)) *
+                     F2(l2e, i, j) -
+                 ((F2(E2, i - 1, j) - F2(G2, i - 1, j)) +
+                  F2(H1, i - 1, j) * beta_phi(lp, r0, th0)) *
+                     F2(l2e, i - 1, j) +
+                 ((F2(E1, i, j - 1) - F2(G1, i, j - 1)) -
+                  F2(H2, i, j - 1) * beta_phi(lp, r, spos(m, 1, j - 1, 1))) *
+                     F2(l1e, i, j - 1) -
+                 ((F2(E1, i, j) - F2(G1, i, j)) -
+                  F2(H2, i, j) * beta_phi(lp, r, spos(m, 1, j, 1))) *
+                     F2(l1e, i, j)) /
+                A3b[o]);
+    }
+  /* theta = 0 axis row (:177-184) */
+  for (int i = m->guard[0]; i < m->dims[0] - m->guard[0]; i++) {
+    size_t o = (size_t)i + (size_t)(m->guard[1] - 1) * d0;
+    B2[o] = f->Bbg[1][o];
+  }
+}
+
+/* compute_e_update_logsph (field_solver_logsph.cu:28-114), alpha_gr = 1 */
+static void logsph_update_e(const orc_mesh *m, const orc_logsph_coef *c,
+                            orc_fields *f, double dt) {
+  const int d0 = m->dims[0];
+  double *E1 = f->E[0], *E2 = f->E[1], *E3 = f->E[2];
+  const double *B1 = f->B[0], *B2 = f->B[1], *B3 = f->B[2];
+  const double *H1 = f->Bbg[0], *H2 = f->Bbg[1], *H3 = f->Bbg[2];
+  const double *J1 = f->J[0], *J2 = f->J[1], *J3 = f->J[2];
+  const double *l1b = c->a[ORC_L1B], *l2b = c->a[ORC_L2B], *l3b = c->a[ORC_L3B];
+  const double *A1e = c->a[ORC_A1E], *A2e = c->a[ORC_A2E], *A3e = c->a[ORC_A3E];
+  const double D1 = m->delta[1];
+  for (int j = m->guard[1]; j < m->dims[1] - m->guard[1]; j++)
+    for (int i = m->guard[0]; i < m->dims[0] - m->guard[0]; i++) {
+      size_t o = (size_t)i + (size_t)j * d0;
+      double r0 = orc_exp(spos(m, 0, i, 0));
+      if (fabs(spos(m, 1, j, 1) - LS_PI) < 1.0e-5) {
+        E1[o] += dt * (-4.0 * (B3[o] - H3[o]) / (D1 * r0) - J1[o]);
+      } else {
+        E1[o] += dt * (((F2(B3, i, j + 1) - F2(H3, i, j + 1)) *
+                            F2(l3b, i, j + 1) -
+                        (F2(B3, i, j) - F2(H3, i, j)) * F2(l3b, i, j)) /
+                           A1e[o] -
+                       J1[o]);
+      }
+      E2[o] += dt * (((F2(B3, i, j) - F2(H3, i, j)) * F2(l3b, i, j) -
+                      (F2(B3, i + 1, j) - F2(H3, i + 1, j)) *
+                          F2(l3b, i + 1, j)) /
+                         A2e[o] -
+                     J2[o]);
+      E3[o] += dt * (((F2(B2, i + 1, j) - F2(H2, i + 1, j)) *
+                          F2(l2b, i + 1, j) -
+                      (F2(B2, i, j) - F2(H2, i, j)) * F2(l2b, i, j) +
+                      (F2(B1, i, j) - F2(H1, i, j)) * F2(l1b, i, j) -
+                      (F2(B1, i, j + 1) - F2(H1, i, j + 1)) *
+                          F2(l1b, i, j + 1)) /
+                         A3e[o] -
+                     J3[o]);
+    }
+  /* theta = 0 axis row (:100-113) */
+  const int g = m->guard[1];
+  for (int i = m->guard[0]; i < m->dims[0] - m->guard[0]; i++) {
+    size_t o = (size_t)i + (size_t)(g - 1) * d0;
+    double r0 = orc_exp(spos(m, 0, i, 0));
+    E3[o] = f->Ebg[2][o];
+    E1[o] += dt * (4.0 * (F2(B3, i, g) - F2(H3, i, g)) / (D1 * r0) - J1[o]);
+  }
+}
+
+/* compute_divs_logsph (field_solver_logsph.cu:187-250) */
+static void logsph_divs(const orc_mesh *m, const orc_logsph_coef *c,
+                        orc_fields *f) {
+  const int d0 = m->dims[0];
+  const double *E1 = f->E[0], *E2 = f->E[1];
+  const double *B1 = f->B[0], *B2 = f->B[1];
+  const double *G1 = f->Ebg[0], *G2 = f->Ebg[1];
+  const double *H1 = f->Bbg[0], *H2 = f->Bbg[1];
+  const double *A1e = c->a[ORC_A1E], *A2e = c->a[ORC_A2E];
+  const double *A1b = c->a[ORC_A1B], *A2b = c->a[ORC_A2B], *dV = c->a[ORC_DV];
+  const double D0 = m->delta[0], D1 = m->delta[1];
+  const double size1 = m->delta[1] * (double)(m->dims[1] - 2 * m->guard[1]);
+  for (int j = m->guard[1]; j < m->dims[1] - m->guard[1]; j++)
+    for (int i = m->guard[0]; i < m->dims[0] - m->guard[0]; i++) {
+      size_t o = (size_t)i + (size_t)j * d0;
+      if (spos(m, 0, i, 1) > D0) {
+        f->divE[o] = ((F2(E1, i + 1, j) - F2(G1, i + 1, j)) * F2(A1e, i + 1, j) -
+                      (F2(E1, i, j) - F2(G1, i, j)) * F2(A1e, i, j) +
+                      (F2(E2, i, j + 1) - F2(G2, i, j + 1)) * F2(A2e, i, j + 1) -
+                      (F2(E2, i, j) - F2(G2, i, j)) * F2(A2e, i, j)) /
+                     (dV[o] * D0 * D1);
+        if (fabs(spos(m, 1, j, 1) - size1 + m->lower[1]) < 1.0e-5)
+          f->divE[o] = ((F2(E1, i + 1, j) - F2(G1, i + 1, j)) * F2(A1e, i + 1, j) -
+                        (F2(E1, i, j) - F2(G1, i, j)) * F2(A1e, i, j) -
+                        2.0 * (F2(E2, i, j) - F2(G2, i, j)) * F2(A2e, i, j)) /
+                       (dV[o] * D0 * D1);
+      }
+      f->divB[o] = ((F2(B1, i, j) - F2(H1, i, j)) * F2(A1b, i, j) -
+                    (F2(B1, i - 1, j) - F2(H1, i - 1, j)) * F2(A1b, i - 1, j) +
+                    (F2(B2, i, j) - F2(H2, i, j)) * F2(A2b, i, j) -
+                    (F2(B2, i, j - 1) - F2(H2, i, j - 1)) * F2(A2b, i, j - 1)) /
+                   (dV[o] * D0 * D1);
+    }
+  const int g = m->guard[1];
+  for (int i = m->guard[0]; i < m->dims[0] - m->guard[0]; i++) {
+    size_t o = (size_t)i + (size_t)(g - 1) * d0;
+    f->divE[o] = ((F2(E1, i + 1, g - 1) - F2(G1, i + 1, g - 1)) *
+                      F2(A1e, i + 1, g - 1) -
+                  (F2(E1, i, g - 1) - F2(G1, i, g - 1)) * F2(A1e, i, g - 1) +
+                  2.0 * (F2(E2, i, g) - F2(G2, i, g)) * F2(A2e, i, g)) /
+                 (dV[o] * D0 * D1);
+  }
+}
+
+/* field_solver_logsph::update_fields (field_solver_logsph.cu:354-398):
+ * B, then E, then the divergences (guard exchanges are no-ops on one
+ * rank with these non-periodic boundaries). */
+void orc_logsph_field_update(const orc_mesh *m, const orc_logsph_coef *c,
+                             const orc_logsph_params *lp, orc_fields *f,
+                             double dt) {
+  logsph_update_b(m, c, lp, f, dt);
+  logsph_update_e(m, c, f, dt);
+  logsph_divs(m, c, f);
+}
+
+/* field_solver_logsph::apply_boundary (field_solver_logsph.cu:403-428):
+ * stellar_boundary (:252-275), outflow_boundary_sph (:322-345),
+ * axis_boundary_lower/upper (:277-320), in that order. */
+void orc_logsph_field_boundary(const orc_mesh *m, const orc_logsph_params *lp,
+                               orc_fields *f, double omega) {
+  const int d0 = m->dims[0], d1 = m->dims[1];
+  double *E1 = f->E[0], *E2 = f->E[1], *E3 = f->E[2];
+  double *B1 = f->B[0], *B2 = f->B[1], *B3 = f->B[2];
+  const double *G1 = f->Ebg[0], *G2 = f->Ebg[1], *G3 = f->Ebg[2];
+  const double *H1 = f->Bbg[0], *H2 = f->Bbg[1], *H3 = f->Bbg[2];
+  for (int j = 0; j < d1; j++) {
+    double theta_s = spos(m, 1, j, 1);
+    double theta = spos(m, 1, j, 0);
+    for (int i = 0; i < m->guard[0]; i++) {
+      size_t o = (size_t)i + (size_t)j * d0;
+      double r_s = orc_exp(spos(m, 0, i, 1));
+      double r = orc_exp(spos(m, 0, i, 0));
+      double omega_LT = 0.4 * omega * lp->compactness;
+      B1[o] = H1[o];
+      B3[o] = H3[o];
+      E2[o] = -(omega - omega_LT) * orc_sin(theta) * H1[o] / r_s / r_s + G2[o];
+      E1[o] = (omega - omega_LT) * orc_sin(theta_s) * H2[o] / r / r + G1[o];
+      B2[o] = H2[o];
+      B3[o] = H3[o];
+    }
+  }
+  const int L = lp->damping_length;
+  for (int j = 0; j < d1; j++)
+    for (int i = 0; i < L; i++) {
+      size_t o = (size_t)(d0 - L + i) + (size_t)j * d0;
+      double x = (double)i / L;
+      double lambda = 1.0 - lp->damping_coef * (x * x);
+      E1[o] = lambda * (E1[o] - G1[o]) + G1[o];
+      E2[o] = lambda * (E2[o] - G2[o]) + G2[o];
+      E3[o] = lambda * (E3[o] - G3[o]) + G3[o];
+      B3[o] = lambda * (B3[o] - H3[o]) + H3[o];
+    }
+  const int g = m->guard[1], jl = d1 - m->guard[1];
+  for (int i = 0; i < d0; i++) {
+    size_t a = (size_t)i + (size_t)(g - 1) * d0, b = (size_t)i + (size_t)g * d0;
+    E3[a] = G3[a];
+    E2[a] = -(E2[b] - G2[b]) + G2[a];
+    B3[a] = H3[a];
+    B3[b] = H3[b];
+    B2[a] = H2[a];
+    B1[a] = (B1[b] - H1[b]) + H1[a];
+  }
+  for (int i = 0; i < d0; i++) {
+    size_t a = (size_t)i + (size_t)(jl - 1) * d0, b = (size_t)i + (size_t)jl * d0;
+    E3[a] = G3[a];
+    E2[b] = -(E2[a] - G2[a]) + G2[b];
+    B3[b] = H3[b];
+    B3[a] = H3[a];
+    B2[a] = H2[a];
+    B1[b] = (B1[a] - H1[a]) + H1[b];
+  }
+}
+#undef F2
+
+/* measure_surface_density + inject_ptc (ptc_updater_logsph.cu:301-395,
+ * 440-493, host side :848-870).  The surface densities go through the same
+ * fixed point as the current (scale 2^-Hs) so the injection gate is
+ * reproducible.  Rows guard1+1 .. dims1-guard1-2 get inj_per_cell e-/e+
+ * (or e-/ion) pairs at r-cell guard0+2, written to slots
+ * number + 2*inj*(row - guard1 - 1) + 2n (+1); a gated row leaves its slots
+ * empty.  Returns the new particle count. */
+uint64_t orc_logsph_inject(const orc_mesh *m, const orc_logsph_params *lp,
+                           orc_ptc *p, int inj_per_cell, double p1, double p2,
+                           double p3, double w, double omega, double q_e,
+                           uint64_t seed, uint32_t epoch, int Hs,
+                           uint64_t id_base, int64_t *surf /* 4*N1 */) {
+  (void)p2; (void)p3; (void)omega;
+  const int d0 = m->dims[0], g0 = m->guard[0], g1 = m->guard[1];
+  const int N1 = m->dims[1] - 2 * g1;
+  memset(surf, 0, sizeof(int64_t) * 4 * (size_t)N1);
+  int64_t *se[2] = {surf, surf + N1}, *spp[2] = {surf + 2 * N1, surf + 3 * N1};
+  const int inject_cell = g0 + 2, sum_cells = 3;
+  for (uint64_t idx = 0; idx < p->number; idx++) {
+    uint32_t c = p->cell[idx];
+    if (c == ORC_MAX_CELL) continue;
+    int c1 = (int)(c % (uint32_t)d0), c2 = (int)(c / (uint32_t)d0);
+    if (!(c1 >= inject_cell - 1 && c1 <= inject_cell - 1 + sum_cells)) continue;
+    int s = (int)(p->flag[idx] >> 29);
+    int64_t **dst = s == 0 ? se : (s == 2 ? spp : NULL);
+    if (!dst) continue;
+    double wp = p->w[idx];
+    int k[5] = {c2 - g1 - 2, c2 - g1 - 1, c2 - g1, c2 - g1 + 1, c2 - g1 + 2};
+    const double coef[5] = {1.0, 4.0, 6.0, 4.0, 1.0};
+    for (int t = 0; t < 5; t++) {
+      int kk = k[t] < 0 ? 0 : (k[t] > N1 - 1 ? N1 - 1 : k[t]);
+      int64_t hi, lo;
+      orc_fx_split(coef[t] * wp / (double)sum_cells / 16.0, Hs, &hi, &lo);
+      dst[0][kk] += hi;
+      dst[1][kk] += lo;
+    }
+  }
+  const double sgate = (1.0 / m->delta[1]) * (1.0 / m->delta[1]);
+  uint64_t base = p->number;
+  for (int i = g1 + 1; i < m->dims[1] - g1 - 1; i++) {
+    uint64_t off = base + (uint64_t)(i - g1 - 1) * inj_per_cell * 2;
+    double de = orc_fx_value(se[0][i - g1], se[1][i - g1], Hs);
+    double dp = orc_fx_value(spp[0][i - g1], spp[1][i - g1], Hs);
+    double dens = de > dp ? de : dp;
+    double sin_theta = orc_sin(lpos(m, 1, i, 0.5));
+    int gated = q_e * dens > 1.5 * sgate * sin_theta;
+    for (int n = 0; n < inj_per_cell; n++) {
+      for (int s = 0; s < 2; s++) {
+        uint64_t o = off + 2 * (uint64_t)n + s;
+        p->cell[o] = ORC_MAX_CELL;
+        p->id[o] = id_base + (o - base);
+      }
+      if (gated) continue;
+      double u[2];
+      orc_uniform2(seed, ((uint64_t)epoch << 32) | (uint32_t)i, (uint32_t)n, 4,
+                   u);
+      double x2 = 1.0 - u[0];
+      double theta = lpos(m, 1, i, x2);
+      double st, ct;
+      orc_sincos(theta, &st, &ct);
+      double vphi = 0.0;
+      double w_ptc = w * st;
+      double gamma = sqrt(1.0 + p1 * p1 + vphi * vphi);
+      double sg = (double)((0.0 < ct) - (ct < 0.0));
+      uint32_t fl = FLAG_PRIMARY | (u[1] < lp->track_percent ? FLAG_TRACKED : 0u);
+      for (int s = 0; s < 2; s++) {
+        uint64_t o = off + 2 * (uint64_t)n + s;
+        p->x1[o] = 0.5;
+        p->x2[o] = x2;
+        p->x3[o] = 0.0;
+        p->p1[o] = p1 * 2.0 * fabs(ct);
+        p->p2[o] = p1 * st * sg;
+        p->p3[o] = vphi;
+        p->E[o] = gamma;
+        p->cell[o] = (uint32_t)(inject_cell + i * d0);
+        p->w[o] = w_ptc;
+        uint32_t type = s == 0 ? 0u : (lp->inject_ions ? 2u : 1u);
+        p->flag[o] = fl | (type << 29);
+      }
+    }
+  }
+  p->number = base + (uint64_t)(N1 - 2) * inj_per_cell * 2;
+  return p->number;
+}

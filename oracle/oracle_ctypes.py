@@ -37,6 +37,14 @@ def load():
     lib.orc_sample_mj.restype = C.c_int
     lib.orc_sample_mj.argtypes = [C.c_double, C.c_double, C.c_uint64, C.c_uint64,
                                   C.POINTER(C.c_double)]
+    for name in ("orc_exp", "orc_sin", "orc_cos", "orc_acos"):
+        getattr(lib, name).restype = C.c_double
+        getattr(lib, name).argtypes = [C.c_double]
+    lib.orc_atan2.restype = C.c_double
+    lib.orc_atan2.argtypes = [C.c_double, C.c_double]
+    for name in ("orc_logsph_ptc_update", "orc_logsph_inject", "orc_emit_photons",
+                 "orc_produce_pairs", "orc_sort_photons"):
+        getattr(lib, name).restype = C.c_uint64
     return lib
 
 
@@ -68,6 +76,37 @@ class OrcSpecies(C.Structure):
                 ("jexp", C.c_int), ("deposit_rho", C.c_int)]
 
 
+class OrcLogsphCoef(C.Structure):
+    _fields_ = [("a", _PD * 13)]
+
+
+class OrcLogsphParams(C.Structure):
+    _fields_ = [("compactness", C.c_double), ("omega", C.c_double),
+                ("gravity_on", C.c_int), ("gravity", C.c_double),
+                ("rad_cooling_on", C.c_int), ("rad_cooling_coef", C.c_double),
+                ("B0", C.c_double), ("damping_length", C.c_int),
+                ("damping_coef", C.c_double), ("inject_ions", C.c_int),
+                ("track_percent", C.c_double), ("current_smoothing", C.c_int)]
+
+
+class OrcPhotons(C.Structure):
+    _fields_ = [("cell", C.POINTER(C.c_uint32)), ("x1", _PD), ("x2", _PD), ("x3", _PD),
+                ("p1", _PD), ("p2", _PD), ("p3", _PD), ("E", _PD), ("w", _PD),
+                ("path", _PD), ("flag", C.POINTER(C.c_uint32)),
+                ("id", C.POINTER(C.c_uint64)), ("number", C.c_uint64)]
+
+
+class OrcRtParams(C.Structure):
+    _fields_ = [("model", C.c_int), ("gamma_thr", C.c_double), ("photon_path", C.c_double),
+                ("E_secondary", C.c_double), ("r_cutoff", C.c_double), ("e_frac", C.c_double),
+                ("max_ptc", C.c_uint64), ("max_photon", C.c_uint64)]
+
+
+# names of the 13 log-spherical coefficient arrays, in oracle order
+COEF_NAMES = ("l1_e", "l2_e", "l3_e", "l1_b", "l2_b", "l3_b", "A1_e", "A2_e", "A3_e",
+              "A1_b", "A2_b", "A3_b", "dV")
+
+
 def _d(a):
     return a.ctypes.data_as(_PD)
 
@@ -80,6 +119,10 @@ class OracleSim:
                   ("x3", np.float64), ("p1", np.float64), ("p2", np.float64),
                   ("p3", np.float64), ("E", np.float64), ("w", np.float64),
                   ("flag", np.uint32), ("id", np.uint64))
+    PH_FIELDS = (("cell", np.uint32), ("x1", np.float64), ("x2", np.float64),
+                 ("x3", np.float64), ("p1", np.float64), ("p2", np.float64),
+                 ("p3", np.float64), ("E", np.float64), ("w", np.float64),
+                 ("path", np.float64), ("flag", np.uint32), ("id", np.uint64))
 
     def __init__(self, N, guard, size=(1.0, 1.0, 1.0), lower=(0.0, 0.0, 0.0),
                  periodic=(True, True, True), q_e=1.0, ion_mass=1.0,
@@ -121,6 +164,21 @@ class OracleSim:
         self.ptc = {k: np.zeros(capacity, dt) for k, dt in self.PTC_FIELDS}
         self.ptc["cell"][:] = MAX_CELL
         self.number = 0
+        self.q_e = q_e
+        self.seed = 43210
+        self.id_next = 0
+        # photons / pair creation state
+        self.ph_capacity = 0
+        self.ph = {k: np.zeros(0, dt) for k, dt in self.PH_FIELDS}
+        self.ph_number = 0
+        self.ph_id_next = 0
+        self.rt = None
+        self.rt_epoch = 0
+        self.overflow = 0
+        # log-spherical state
+        self.logsph = False
+        self.coef = None
+        self.inject_epoch = 0
         self._bind()
 
     # ---- structs -------------------------------------------------------------
@@ -168,6 +226,7 @@ class OracleSim:
                 self.ptc["id"][:n] = np.arange(n, dtype=np.uint64)
         self.ptc["cell"][n:] = MAX_CELL
         self.number = n
+        self.id_next = int(self.ptc["id"][:n].max()) + 1 if (n and "id" in ptc) else n
 
     def particles(self):
         return {k: self.ptc[k][: self.number].copy() for k, _ in self.PTC_FIELDS}
@@ -187,6 +246,7 @@ class OracleSim:
                                         C.c_double(drift[0]), C.c_double(drift[1]),
                                         C.c_double(weight), C.c_uint64(seed))
         self.number = int(got)
+        self.id_next = max(self.id_next, self.number)
         return self.number
 
     # ---- step ----------------------------------------------------------------
@@ -240,6 +300,182 @@ class OracleSim:
         self.lib.orc_energy_histogram(C.byref(P), h.ctypes.data_as(C.POINTER(C.c_uint32)),
                                       C.c_int(nbins), C.c_double(E_max), C.c_int(flag_bit))
         return h
+
+
+class LogSphOracle(OracleSim):
+    """2D log-spherical (ln r, theta) state: the Cartesian state plus the 13
+    Grid_LogSph coefficient arrays and the log-spherical step functions."""
+
+    def __init__(self, N, guard, size, lower=(0.0, 0.0, 0.0), q_e=1.0, ion_mass=1.0,
+                 num_species=3, capacity=0, jexp=22, data_interval=100, **lp):
+        super().__init__(N, guard, size=size, lower=lower, periodic=(False, False, False),
+                         q_e=q_e, ion_mass=ion_mass, num_species=num_species,
+                         capacity=capacity, jexp=jexp, data_interval=data_interval)
+        self.logsph = True
+        self.lp = OrcLogsphParams()
+        defaults = dict(compactness=0.0, omega=0.0, gravity_on=0, gravity=0.0,
+                        rad_cooling_on=0, rad_cooling_coef=1.0, B0=1.0,
+                        damping_length=30, damping_coef=0.002, inject_ions=1,
+                        track_percent=0.0, current_smoothing=0)
+        defaults.update(lp)
+        for k, v in defaults.items():
+            setattr(self.lp, k, v)
+        self.coef = [np.zeros(self.ncell) for _ in range(13)]
+        self.C = OrcLogsphCoef()
+        for i in range(13):
+            self.C.a[i] = _d(self.coef[i])
+        self.lib.orc_logsph_compute_coef(C.byref(self.mesh), C.byref(self.C))
+        self.jtmp = np.zeros(self.ncell)
+        self.multi = 0
+
+    def coefficient(self, name):
+        return self.coef[COEF_NAMES.index(name)]
+
+    def ptc_update(self, dt, step, deposit_rho=None):
+        if deposit_rho is None:
+            deposit_rho = ((step + 1) % self.data_interval) == 0
+        self.sp.deposit_rho = 1 if deposit_rho else 0
+        P = self._ptc_struct()
+        self.multi += int(self.lib.orc_logsph_ptc_update(
+            C.byref(self.mesh), C.byref(self.C), C.byref(self.lp), C.byref(self.F),
+            C.byref(P), C.byref(self.sp), _d(self.jtmp), C.c_double(dt), C.c_uint32(step)))
+        if self.ph_number:
+            self.move_photons(dt)
+            self.clear_photon_guard_cells()
+
+    def field_update(self, dt, step=0):
+        self.lib.orc_logsph_field_update(C.byref(self.mesh), C.byref(self.C),
+                                         C.byref(self.lp), C.byref(self.F), C.c_double(dt))
+
+    def field_boundary(self, omega):
+        self.lib.orc_logsph_field_boundary(C.byref(self.mesh), C.byref(self.lp),
+                                           C.byref(self.F), C.c_double(omega))
+
+    def inject(self, inj_per_cell, p1, p2=0.0, p3=0.0, w=1.0, omega=0.0, Hs=None):
+        N1 = self.dims[1] - 2 * self.mesh.guard[1]
+        add = (N1 - 2) * inj_per_cell * 2
+        need = self.number + add
+        if need > self.capacity:
+            self._grow(need)
+        if Hs is None:
+            Hs = self.sp.jexp
+        surf = np.zeros(4 * N1, np.int64)
+        P = self._ptc_struct()
+        base = self.number
+        n = self.lib.orc_logsph_inject(
+            C.byref(self.mesh), C.byref(self.lp), C.byref(P), C.c_int(inj_per_cell),
+            C.c_double(p1), C.c_double(p2), C.c_double(p3), C.c_double(w), C.c_double(omega),
+            C.c_double(self.q_e), C.c_uint64(self.seed), C.c_uint32(self.inject_epoch),
+            C.c_int(Hs), C.c_uint64(self.id_next), surf.ctypes.data_as(_PI64))
+        self.inject_epoch += 1
+        self.number = int(n)
+        self.id_next += self.number - base
+        return self.number
+
+
+def _grow_ptc(self, n):
+    old = self.ptc
+    self.ptc = {k: np.zeros(n, dt) for k, dt in self.PTC_FIELDS}
+    self.ptc["cell"][:] = MAX_CELL
+    for k in self.ptc:
+        self.ptc[k][: self.capacity] = old[k][: self.capacity]
+    self.capacity = n
+
+
+def _ph_struct(self):
+    P = OrcPhotons()
+    P.cell = self.ph["cell"].ctypes.data_as(C.POINTER(C.c_uint32))
+    for k in ("x1", "x2", "x3", "p1", "p2", "p3", "E", "w", "path"):
+        setattr(P, k, _d(self.ph[k]))
+    P.flag = self.ph["flag"].ctypes.data_as(C.POINTER(C.c_uint32))
+    P.id = self.ph["id"].ctypes.data_as(C.POINTER(C.c_uint64))
+    P.number = self.ph_number
+    return P
+
+
+def _set_rt(self, model=0, gamma_thr=20.0, photon_path=1.0, E_secondary=4.0,
+            r_cutoff=4.0, e_frac=0.2, max_photon=0, max_ptc=None):
+    """Configure pair creation (SimParams gamma_thr, photon_path,
+    E_secondary, r_cutoff; sim_params.h:52-79) and size the photon store."""
+    self.rt = OrcRtParams()
+    self.rt.model = model
+    self.rt.gamma_thr = gamma_thr
+    self.rt.photon_path = photon_path
+    self.rt.E_secondary = E_secondary
+    self.rt.r_cutoff = r_cutoff
+    self.rt.e_frac = e_frac
+    self.rt.max_ptc = self.capacity if max_ptc is None else max_ptc
+    self.rt.max_photon = max_photon
+    self.ph_capacity = max_photon
+    self.ph = {k: np.zeros(max_photon, dt) for k, dt in self.PH_FIELDS}
+    self.ph["cell"][:] = MAX_CELL
+    self.ph_number = 0
+
+
+def _emit_photons(self):
+    ov = C.c_uint64(0)
+    P, Ph = self._ptc_struct(), self._ph_struct()
+    k = self.lib.orc_emit_photons(C.byref(self.mesh), C.byref(self.rt), C.byref(P),
+                                  C.byref(Ph), C.c_uint64(self.seed),
+                                  C.c_uint32(self.rt_epoch), C.c_uint64(self.ph_id_next),
+                                  C.byref(ov))
+    self.rt_epoch += 1
+    self.ph_number = int(Ph.number)
+    self.ph_id_next += int(k)
+    self.overflow += int(ov.value)
+    return int(k)
+
+
+def _produce_pairs(self):
+    ov = C.c_uint64(0)
+    P, Ph = self._ptc_struct(), self._ph_struct()
+    k = self.lib.orc_produce_pairs(C.byref(self.mesh), C.byref(self.rt), C.byref(P),
+                                   C.byref(Ph), C.c_uint64(self.id_next), C.byref(ov))
+    self.number = int(P.number)
+    self.id_next += 2 * int(k)
+    self.overflow += int(ov.value)
+    return int(k)
+
+
+def _move_photons(self, dt):
+    Ph = self._ph_struct()
+    r_cut = self.rt.r_cutoff if self.rt is not None else 1e300
+    self.lib.orc_move_photons(C.byref(self.mesh), C.c_int(1 if self.logsph else 0),
+                              C.c_double(r_cut), C.byref(Ph), C.c_double(dt))
+
+
+def _clear_photon_guard_cells(self):
+    Ph = self._ph_struct()
+    self.lib.orc_clear_photon_guard_cells(C.byref(self.mesh), C.byref(Ph))
+
+
+def _sort_photons(self):
+    scratch = np.zeros(max(self.ph_number, 1), np.uint64)
+    Ph = self._ph_struct()
+    kept = self.lib.orc_sort_photons(C.byref(self.mesh), C.byref(Ph),
+                                     scratch.ctypes.data_as(C.POINTER(C.c_uint64)))
+    self.ph_number = int(kept)
+    return self.ph_number
+
+
+def _photons(self):
+    return {k: self.ph[k][: self.ph_number].copy() for k, _ in self.PH_FIELDS}
+
+
+OracleSim._grow = _grow_ptc
+OracleSim._ph_struct = _ph_struct
+OracleSim.set_rt = _set_rt
+OracleSim.emit_photons = _emit_photons
+OracleSim.produce_pairs = _produce_pairs
+OracleSim.move_photons = _move_photons
+OracleSim.clear_photon_guard_cells = _clear_photon_guard_cells
+OracleSim.sort_photons = _sort_photons
+OracleSim.photons = _photons
+
+
+def math_fn(name, *args):
+    """One of the oracle's shared elementary functions (orc_math.c)."""
+    return getattr(load(), "orc_" + name)(*args)
 
 
 def philox(ctr, key):

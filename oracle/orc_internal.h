@@ -1,0 +1,120 @@
+/*
+ * orc_internal.h -- helpers shared by the oracle's translation units
+ * (Cartesian step, log-spherical step, radiative transfer).  TEST
+ * INFRASTRUCTURE ONLY, like the rest of oracle/.
+ */
+#ifndef ORC_INTERNAL_H
+#define ORC_INTERNAL_H
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pic_oracle.h"
+
+/* Two uniforms in (0,1] from one Philox block: 53 bits each. */
+static inline void orc_uniform2(uint64_t seed, uint64_t pid, uint32_t draw,
+                         uint32_t purpose, double u[2]) {
+  uint32_t ctr[4] = {(uint32_t)pid, (uint32_t)(pid >> 32), draw, purpose};
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t r[4];
+  orc_philox4x32_10(ctr, key, r);
+  uint64_t a = (((uint64_t)r[0] << 32) | r[1]) >> 11;
+  uint64_t b = (((uint64_t)r[2] << 32) | r[3]) >> 11;
+  u[0] = (double)(a + 1) * 0x1p-53;
+  u[1] = (double)(b + 1) * 0x1p-53;
+}
+
+static inline int is_in_bulk(const orc_mesh *m, uint32_t c) {
+  uint32_t c1 = c % (uint32_t)m->dims[0];
+  uint32_t c2 = (c / (uint32_t)m->dims[0]) % (uint32_t)m->dims[1];
+  uint32_t c3 = c / ((uint32_t)m->dims[0] * (uint32_t)m->dims[1]);
+  return ((int)c1 >= m->guard[0] && (int)c1 < m->dims[0] - m->guard[0]) &&
+         ((int)c2 >= m->guard[1] && (int)c2 < m->dims[1] - m->guard[1]) &&
+         ((int)c3 >= m->guard[2] && (int)c3 < m->dims[2] - m->guard[2]);
+}
+
+/* ------------------------------------------------------------------ */
+/* Interpolation: spline_t<1> (interpolation.cuh:28-37) and interp_cell */
+/* (interpolation.cuh:79-87).                                            */
+/* ------------------------------------------------------------------ */
+static inline double cic(double dx) {
+  double a = 1.0 - fabs(dx);
+  return a > 0.0 ? a : 0.0;
+}
+
+static inline double interp_cell(double rel, int c, int t, int stagger) {
+  double x = ((double)t + (stagger == 0 ? 0.5 : 1.0)) - (rel + (double)c);
+  return cic(x);
+}
+
+/* Interpolator2D (interpolation.cuh:146-180) */
+static inline double gather2d(const double *f, int d0, double x1, double x2, int c1,
+                       int c2, int st) {
+  double r = 0.0;
+  for (int j = 0; j <= 2; j++) {
+    int jj = j + c2 - 1;
+    for (int i = 0; i <= 2; i++) {
+      int ii = i + c1 - 1;
+      r += f[ii + jj * d0] * interp_cell(x1, c1, ii, st & 1) *
+           interp_cell(x2, c2, jj, (st >> 1) & 1);
+    }
+  }
+  return r;
+}
+
+/* vay_push (vay_push.cuh:8-30); E, B already scaled by dt q/(2m). */
+static inline void vay_push(double *p1, double *p2, double *p3, double *gamma,
+                     double E1, double E2, double E3, double B1, double B2,
+                     double B3) {
+  double g = *gamma;
+  double up1 = *p1 + 2.0 * E1 + (*p2 * B3 - *p3 * B2) / g;
+  double up2 = *p2 + 2.0 * E2 + (*p3 * B1 - *p1 * B3) / g;
+  double up3 = *p3 + 2.0 * E3 + (*p1 * B2 - *p2 * B1) / g;
+  double tt = B1 * B1 + B2 * B2 + B3 * B3;
+  double ut = up1 * B1 + up2 * B2 + up3 * B3;
+  double sigma = 1.0 + up1 * up1 + up2 * up2 + up3 * up3 - tt;
+  double inv_gamma2 =
+      2.0 / (sigma + sqrt(sigma * sigma + 4.0 * (tt + ut * ut)));
+  double s = 1.0 / (1.0 + inv_gamma2 * tt);
+  g = 1.0 / sqrt(inv_gamma2);
+  *p1 = (up1 + B1 * ut * inv_gamma2 + (up2 * B3 - up3 * B2) / g) * s;
+  *p2 = (up2 + B2 * ut * inv_gamma2 + (up3 * B1 - up1 * B3) / g) * s;
+  *p3 = (up3 + B3 * ut * inv_gamma2 + (up1 * B2 - up2 * B1) / g) * s;
+  *gamma = g;
+}
+
+/* Stagger codes, fields_impl.hpp:85-94 */
+#define ST_E1 0x6
+#define ST_E2 0x5
+#define ST_E3 0x3
+#define ST_B1 0x1
+#define ST_B2 0x2
+#define ST_B3 0x4
+
+#define FLAG_IGNORE_CURRENT (1u << 3)
+#define FLAG_IGNORE_EM (1u << 4)
+
+/* Exact-order helpers, ptc_updater_helper.cuh:62-77 (1/6 exact, see the
+ * deviation note in the design: the reference writes 0.1666667f). */
+static inline double center2d(double sx0, double sx1, double sy0,
+                              double sy1) {
+  return (2.0 * sx1 * sy1 + sx0 * sy1 + sx1 * sy0 + 2.0 * sx0 * sy0) *
+         (1.0 / 6.0);
+}
+static inline double movement3d(double sx0, double sx1, double sy0,
+                                double sy1, double sz0, double sz1) {
+  return (sz1 - sz0) * center2d(sx0, sx1, sy0, sy1);
+}
+static inline double movement2d(double sx0, double sx1, double sy0,
+                                double sy1) {
+  return (sy1 - sy0) * 0.5 * (sx0 + sx1);
+}
+
+static inline void acc_add(int64_t *acc[2], size_t o, double c, int H) {
+  int64_t hi, lo;
+  orc_fx_split(c, H, &hi, &lo);
+  acc[0][o] += hi;
+  acc[1][o] += lo;
+}
+
+#endif

@@ -72,6 +72,14 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
                        uint32_t out[4]);
 double orc_log(double x);
 
+/* --- elementary functions shared bit-for-bit with the device (orc_math.c) --- */
+double orc_exp(double x);
+void orc_sincos(double x, double *s, double *c);
+double orc_sin(double x);
+double orc_cos(double x);
+double orc_atan2(double y, double x);
+double orc_acos(double t);
+
 /* --- fixed point helpers --- */
 void orc_fx_split(double c, int H, int64_t *hi, int64_t *lo);
 double orc_fx_value(int64_t hi, int64_t lo, int H);
@@ -98,6 +106,79 @@ void orc_filter_current(const orc_mesh *m, double *j, double *tmp,
 /* --- sort / compaction --- */
 void orc_clear_guard_cells(const orc_mesh *m, orc_ptc *p);
 uint64_t orc_sort_by_cell(const orc_mesh *m, orc_ptc *p, uint64_t *scratch);
+
+/* --- 2D log-spherical grid (logsph_oracle.c) --- */
+enum {
+  ORC_L1E, ORC_L2E, ORC_L3E, ORC_L1B, ORC_L2B, ORC_L3B,
+  ORC_A1E, ORC_A2E, ORC_A3E, ORC_A1B, ORC_A2B, ORC_A3B, ORC_DV,
+  ORC_NCOEF
+};
+/* the 13 coefficient arrays of Grid_LogSph (grid_log_sph.h:45-49), each
+ * dims0*dims1 doubles */
+typedef struct {
+  double *a[ORC_NCOEF];
+} orc_logsph_coef;
+
+/* the SimParams entries the log-spherical path reads (sim_params.h:31-64) */
+typedef struct {
+  double compactness, omega;
+  int gravity_on;
+  double gravity;
+  int rad_cooling_on;
+  double rad_cooling_coef, B0;
+  int damping_length;
+  double damping_coef;
+  int inject_ions;
+  double track_percent;
+  int current_smoothing;
+} orc_logsph_params;
+
+void orc_logsph_compute_coef(const orc_mesh *m, orc_logsph_coef *c);
+uint64_t orc_logsph_ptc_update(const orc_mesh *m, const orc_logsph_coef *c,
+                               const orc_logsph_params *lp, orc_fields *f,
+                               orc_ptc *p, const orc_species *sp,
+                               double *jtmp, double dt, uint32_t step);
+void orc_logsph_field_update(const orc_mesh *m, const orc_logsph_coef *c,
+                             const orc_logsph_params *lp, orc_fields *f,
+                             double dt);
+void orc_logsph_field_boundary(const orc_mesh *m, const orc_logsph_params *lp,
+                               orc_fields *f, double omega);
+uint64_t orc_logsph_inject(const orc_mesh *m, const orc_logsph_params *lp,
+                           orc_ptc *p, int inj_per_cell, double p1, double p2,
+                           double p3, double w, double omega, double q_e,
+                           uint64_t seed, uint32_t epoch, int Hs,
+                           uint64_t id_base, int64_t *surf);
+
+/* --- photons and pair creation (rt_oracle.c) --- */
+/* photon_data SoA (src/data/particle_data.h:68-81), FP64 form */
+typedef struct {
+  uint32_t *cell;
+  double *x1, *x2, *x3;
+  double *p1, *p2, *p3;
+  double *E, *w, *path;
+  uint32_t *flag;
+  uint64_t *id;
+  uint64_t number;
+} orc_photons;
+
+typedef struct {
+  int model; /* 0: reference default hooks; 1: exponential free path */
+  double gamma_thr, photon_path, E_secondary, r_cutoff, e_frac;
+  uint64_t max_ptc, max_photon;
+} orc_rt_params;
+
+uint64_t orc_emit_photons(const orc_mesh *m, const orc_rt_params *rt,
+                          orc_ptc *p, orc_photons *ph, uint64_t seed,
+                          uint32_t epoch, uint64_t id_base,
+                          uint64_t *overflow);
+uint64_t orc_produce_pairs(const orc_mesh *m, const orc_rt_params *rt,
+                           orc_ptc *p, orc_photons *ph, uint64_t id_base,
+                           uint64_t *overflow);
+void orc_move_photons(const orc_mesh *m, int logsph, double r_cutoff,
+                      orc_photons *ph, double dt);
+void orc_clear_photon_guard_cells(const orc_mesh *m, orc_photons *ph);
+uint64_t orc_sort_photons(const orc_mesh *m, orc_photons *ph,
+                          uint64_t *scratch);
 
 /* --- diagnostics --- */
 void orc_energy_histogram(const orc_ptc *p, uint32_t *hist, int nbins,

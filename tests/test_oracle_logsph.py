@@ -1,0 +1,276 @@
+"""Pins the oracle's 2D log-spherical step (oracle/logsph_oracle.c), which
+restates Grid_LogSph::compute_coef (grid_log_sph.cpp:10-104), the push and
+mover of ptc_updater_logsph.cu:46-282, process_j (:284-299) and the
+integral-form Yee solver of field_solver_logsph.cu:28-345.
+
+What pins it independently of its own formulas:
+  * coefficients against numerical quadrature of the spherical line, area
+    and volume elements (scipy.integrate), including the axis cells;
+  * the mover against closed-form straight-line motion (radial, polar and
+    azimuthal) and the axis reflection;
+  * div B unchanged by the B update (discrete Stokes: edge lengths against
+    face areas and every stencil offset);
+  * Gauss's law: div E - rho conserved at round-off through the particle
+    step + process_j + E update (continuity of the deposit, the
+    Delta1 Delta2/(dt A) scaling and the divergence's area weights);
+  * a vacuum static dipole stays exactly static through update and
+    boundaries;
+  * gravity and perpendicular cooling against their closed forms.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+import oracle_ctypes as oc
+
+LNR = math.log(30.0)
+
+
+def make(N=(32, 32), guard=3, cap=4096, **kw):
+    return oc.LogSphOracle((N[0], N[1], 1), (guard, guard, 0), (LNR, math.pi, 1.0),
+                           capacity=cap, **kw)
+
+
+def ij(o, i, j):
+    return i + j * o.dims[0]
+
+
+def pos(o, d, n, x):
+    m = o.mesh
+    return m.lower[d] + m.delta[d] * ((n - m.guard[d]) + x)
+
+
+def test_coefficients_against_quadrature():
+    o = make((16, 24))
+    D0, D1 = o.mesh.delta[0], o.mesh.delta[1]
+    rng = np.random.default_rng(3)
+    for _ in range(40):
+        i = int(rng.integers(0, o.dims[0]))
+        j = int(rng.integers(3, o.dims[1] - 3))
+        x1, x1s = pos(o, 0, i, 0.5), pos(o, 0, i, 1.0)
+        x2, x2s = pos(o, 1, j, 0.5), pos(o, 1, j, 1.0)
+        r, rp, rs, rsm = math.exp(x1), math.exp(x1 + D0), math.exp(x1s), math.exp(x1s - D0)
+        k = ij(o, i, j)
+        # lengths: l1 = integral dr, l2 = r dtheta, l3 = r sin(theta)
+        assert o.coefficient("l1_e")[k] == pytest.approx(integrate.quad(lambda t: 1.0, rsm, rs)[0], rel=1e-12)
+        assert o.coefficient("l1_b")[k] == pytest.approx(integrate.quad(lambda t: 1.0, r, rp)[0], rel=1e-12)
+        # areas: A1 = r^2 int sin dtheta, A2 = sin(theta) int r dr, A3 = dtheta int r dr
+        a1 = r * r * integrate.quad(math.sin, x2, x2 + D1)[0]
+        if abs(x2s) > 1e-5 and abs(x2s - math.pi) > 1e-5:
+            assert o.coefficient("A1_e")[k] == pytest.approx(a1, rel=1e-12)
+        assert o.coefficient("A2_e")[k] == pytest.approx(
+            math.sin(x2) * integrate.quad(lambda t: t, r, rp)[0], rel=1e-11)
+        assert o.coefficient("A3_e")[k] == pytest.approx(
+            D1 * integrate.quad(lambda t: t, r, rp)[0], rel=1e-11)
+        assert o.coefficient("A1_b")[k] == pytest.approx(
+            rs * rs * integrate.quad(math.sin, x2s - D1, x2s)[0], rel=1e-11)
+        # volume per (D0 D1): int r^2 dr int sin dtheta / (D0 D1)
+        vol = integrate.quad(lambda t: t * t, r, rp)[0] * integrate.quad(math.sin, x2, x2 + D1)[0]
+        if abs(x2s) > 1e-5 and abs(x2s - math.pi) > 1e-5:
+            assert o.coefficient("dV")[k] == pytest.approx(vol / (D0 * D1), rel=1e-11)
+    # axis cells: the cap of half-angle D1/2 (area 2(1 - cos(D1/2)))
+    g = o.mesh.guard[1]
+    for j in (g - 1, o.dims[1] - g - 1):
+        k = ij(o, 5, j)
+        r = math.exp(pos(o, 0, 5, 0.5))
+        cap = integrate.quad(math.sin, 0.0, 0.5 * D1)[0]
+        assert o.coefficient("A1_e")[k] == pytest.approx(r * r * 2.0 * cap, rel=1e-12)
+
+
+def one_particle(o, c1, c2, x1, x2, p, flag=0, x3=0.0):
+    o.set_particles({"cell": [ij(o, c1, c2)], "x1": [x1], "x2": [x2], "x3": [x3],
+                     "p1": [p[0]], "p2": [p[1]], "p3": [p[2]], "w": [1.0],
+                     "flag": [flag]})
+
+
+IGNORE_EM = 1 << 4
+
+
+def test_radial_move_closed_form():
+    o = make()
+    dt = 0.01
+    c1, c2, x1, x2 = 10, 12, 0.3, 0.4
+    one_particle(o, c1, c2, x1, x2, (2.0, 0.0, 0.0), flag=IGNORE_EM)
+    r0, th0 = math.exp(pos(o, 0, c1, x1)), pos(o, 1, c2, x2)
+    v = 2.0 / math.sqrt(5.0)
+    o.ptc_update(dt, 0)
+    P = o.particles()
+    c = int(P["cell"][0])
+    r1 = math.exp(pos(o, 0, c % o.dims[0], P["x1"][0]))
+    th1 = pos(o, 1, c // o.dims[0], P["x2"][0])
+    assert r1 == pytest.approx(r0 + v * dt, rel=1e-13)
+    assert th1 == pytest.approx(th0, abs=1e-14)
+    assert P["p1"][0] == pytest.approx(2.0, rel=1e-13)
+    assert abs(P["p2"][0]) < 1e-13 and abs(P["p3"][0]) < 1e-13
+
+
+def test_polar_and_azimuthal_move_closed_form():
+    o = make()
+    dt = 0.01
+    c1, c2 = 10, 19  # theta near pi/2
+    for comp in (1, 2):
+        p = [0.0, 0.0, 0.0]
+        p[comp] = 1.0
+        one_particle(o, c1, c2, 0.5, 0.25, p, flag=IGNORE_EM)
+        r0, th0 = math.exp(pos(o, 0, c1, 0.5)), pos(o, 1, c2, 0.25)
+        o.ptc_update(dt, 0)
+        P = o.particles()
+        c = int(P["cell"][0])
+        r1 = math.exp(pos(o, 0, c % o.dims[0], P["x1"][0]))
+        th1 = pos(o, 1, c // o.dims[0], P["x2"][0])
+        d = dt / math.sqrt(2.0)
+        assert r1 == pytest.approx(math.hypot(r0, d), rel=1e-13)
+        if comp == 1:
+            assert th1 == pytest.approx(th0 + math.atan2(d, r0), abs=1e-13)
+            assert P["x3"][0] == pytest.approx(0.0, abs=1e-15)
+        else:
+            rho = math.hypot(r0 * math.sin(th0), d)
+            assert th1 == pytest.approx(math.atan2(rho, r0 * math.cos(th0)), abs=1e-13)
+            assert P["x3"][0] == pytest.approx(math.atan2(d, r0 * math.sin(th0)), abs=1e-13)
+        pm = math.sqrt(P["p1"][0] ** 2 + P["p2"][0] ** 2 + P["p3"][0] ** 2)
+        assert pm == pytest.approx(1.0, rel=1e-13)
+
+
+def test_axis_reflection():
+    o = make()
+    g = 3
+    one_particle(o, 10, g, 0.5, 0.01, (0.0, -0.5, 0.3), flag=IGNORE_EM)
+    o.ptc_update(0.05, 0)
+    P = o.particles()
+    assert int(P["cell"][0]) // o.dims[0] == g          # stays in the first row
+    assert 0.0 <= P["x2"][0] < 1.0
+    assert P["p2"][0] > 0.0                               # theta momentum flipped
+    pm = math.sqrt(P["p1"][0] ** 2 + P["p2"][0] ** 2 + P["p3"][0] ** 2)
+    assert pm == pytest.approx(math.hypot(0.5, 0.3), rel=1e-13)
+
+
+def test_divb_preserved_by_b_update():
+    o = make((24, 20))
+    rng = np.random.default_rng(11)
+    for c in range(3):
+        o.f["E"][c][:] = rng.normal(size=o.ncell)
+    o.lp.omega = 0.0
+    o.field_update(0.0)           # divB of the initial (zero) B
+    o.f["B"][0][:] = rng.normal(size=o.ncell)
+    o.f["B"][1][:] = rng.normal(size=o.ncell)
+    o.field_update(0.0)
+    before = o.divB.copy()
+    o.field_update(0.01)
+    g = 3
+    v = (o.divB - before).reshape(o.dims[1], o.dims[0])[g + 1:-g - 1, g + 1:-g - 1]
+    assert np.abs(v).max() < 1e-12 * np.abs(before).max()
+
+
+def test_gauss_law_conserved():
+    o = make((32, 32), cap=20000, data_interval=1)
+    rng = np.random.default_rng(5)
+    g = 3
+    n = 3000
+    c1 = rng.integers(g + 8, 32 + g - 8, n)
+    c2 = rng.integers(g + 8, 32 + g - 8, n)
+    sp = rng.integers(0, 2, n)
+    o.set_particles({"cell": c1 + c2 * o.dims[0], "x1": rng.random(n), "x2": rng.random(n),
+                     "x3": rng.uniform(-3, 3, n), "p1": rng.normal(0, 0.3, n),
+                     "p2": rng.normal(0, 0.3, n), "p3": rng.normal(0, 0.3, n),
+                     "w": rng.uniform(0.5, 1.5, n), "flag": (sp << 29).astype(np.uint32)})
+    for c in range(3):
+        o.f["Bbg"][c][:] = rng.normal(0, 0.1, o.ncell)
+        o.f["B"][c][:] = o.f["Bbg"][c] + rng.normal(0, 0.05, o.ncell)
+    dt = 0.002
+    res = []
+    for step in range(6):
+        o.ptc_update(dt, step)
+        o.field_update(dt)
+        rho = o.rho[0] + o.rho[1]
+        v = (o.divE - rho).reshape(o.dims[1], o.dims[0])[g + 2:-g - 2, g + 2:-g - 2]
+        res.append(v.copy())
+    scale = np.abs(o.rho[0].reshape(o.dims[1], o.dims[0])).max()
+    drift = max(np.abs(r - res[0]).max() for r in res[1:])
+    assert drift < 1e-11 * scale, (drift, scale)
+
+
+def dipole(o):
+    mu = 1.0e2
+    d0 = o.dims[0]
+    for c, st, fn in ((0, 0x1, lambda r, t: 2 * mu * np.cos(t) / r ** 3),
+                      (1, 0x2, lambda r, t: mu * np.sin(t) / r ** 3)):
+        i = np.arange(o.dims[0])
+        j = np.arange(o.dims[1])
+        x1 = pos(o, 0, i, 1.0 if st & 1 else 0.5)
+        x2 = pos(o, 1, j, 1.0 if st & 2 else 0.5)
+        T, R = np.meshgrid(x2, np.exp(x1), indexing="ij")
+        v = fn(R, T).ravel()
+        o.f["Bbg"][c][:] = v
+        o.f["B"][c][:] = v
+
+
+def test_static_dipole_stays_static():
+    o = make((32, 32))
+    dipole(o)
+    B0 = [b.copy() for b in o.f["B"]]
+    for _ in range(5):
+        o.ptc_update(0.01, 0)
+        o.field_update(0.01)
+        o.field_boundary(0.0)
+    for c in range(3):
+        assert np.array_equal(o.f["B"][c], B0[c])
+        assert not np.any(o.f["E"][c])
+
+
+def test_gravity_and_cooling_closed_form():
+    o = make(gravity_on=1, gravity=0.7)
+    dt = 0.01
+    one_particle(o, 10, 16, 0.5, 0.5, (0.5, 0.2, -0.1))
+    r = math.exp(pos(o, 0, 10, 0.5))
+    o.ptc_update(dt, 0)
+    P = o.particles()
+    # E = B = 0: Vay leaves u unchanged; gravity subtracts dt g m/(r^2 |q|)
+    # (the mover then rotates u into the new local frame; |u| is what to pin)
+    p1 = 0.5 - dt * 0.7 / (r * r)
+    pm = math.sqrt(P["p1"][0] ** 2 + P["p2"][0] ** 2 + P["p3"][0] ** 2)
+    assert pm == pytest.approx(math.sqrt(p1 ** 2 + 0.2 ** 2 + 0.1 ** 2), rel=1e-13)
+    # cooling: B along r removes (coef |B|/B0) of u_perp, keeps u_par
+    o2 = make(rad_cooling_on=1, rad_cooling_coef=0.25, B0=2.0)
+    o2.f["B"][0][:] = 1.0
+    one_particle(o2, 10, 16, 0.5, 0.5, (0.5, 0.2, -0.1), flag=0)
+    o2.set_particles({**{k: v for k, v in o2.particles().items()}, "flag": [1 << 29]})
+    o2.sp.q[1] = 1e-30  # negligible Lorentz rotation, same q/m sign path
+    o2.ptc_update(1e-9, 0)
+    P = o2.particles()
+    # tiny dt: the move barely rotates the frame
+    assert P["p1"][0] == pytest.approx(0.5, rel=1e-6)
+    assert P["p2"][0] == pytest.approx(0.2 * (1 - 0.25 * 0.5), rel=1e-6)
+    assert P["p3"][0] == pytest.approx(-0.1 * (1 - 0.25 * 0.5), rel=1e-6)
+
+
+def test_injection_layout_and_gate():
+    o = make((32, 32), cap=10000, track_percent=0.0, inject_ions=1)
+    n = o.inject(2, 0.01, w=1.0)
+    g = 3
+    N1 = 32
+    assert n == 2 * 2 * (N1 - 2)
+    P = o.particles()
+    cells = P["cell"]
+    assert np.all(cells % o.dims[0] == g + 2)
+    rows = cells // o.dims[0]
+    assert rows.min() == g + 1 and rows.max() == g + N1 - 2
+    theta = pos(o, 1, rows, P["x2"])
+    np.testing.assert_allclose(P["w"], np.sin(theta), rtol=1e-14)
+    np.testing.assert_allclose(P["p1"], 0.02 * np.abs(np.cos(theta)), rtol=1e-14)
+    assert np.all((P["flag"] >> 29)[0::2] == 0) and np.all((P["flag"] >> 29)[1::2] == 2)
+    assert np.all(P["flag"] & (1 << 6))
+    # dense surface: a row whose surface density exceeds the limit is skipped
+    o2 = make((32, 32), cap=20000)
+    row = g + 10
+    k = 400
+    o2.set_particles({"cell": np.full(k, (g + 2) + row * o2.dims[0]), "x1": np.full(k, 0.5),
+                      "x2": np.full(k, 0.5), "x3": np.zeros(k), "p1": np.zeros(k),
+                      "p2": np.zeros(k), "p3": np.zeros(k), "w": np.full(k, 5.0),
+                      "flag": np.zeros(k, np.uint32)})
+    o2.inject(1, 0.01)
+    P = o2.particles()
+    new = P["cell"][k:]
+    got_rows = set((new[new != oc.MAX_CELL] // o2.dims[0]).tolist())
+    assert row not in got_rows and (row - 5) in got_rows

@@ -1,0 +1,65 @@
+"""Pins the oracle's elementary functions (oracle/orc_math.c), which the
+log-spherical path shares bit-for-bit with the device, against numpy's
+libm to a few ulp, plus their exact special values."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_ctypes as oc
+
+RNG = np.random.default_rng(7)
+
+
+def f(name, *a):
+    return oc.math_fn(name, *a)
+
+
+def test_exp_ulps():
+    xs = np.concatenate([RNG.uniform(-8, 8, 4000), RNG.uniform(-0.4, 0.4, 1000),
+                         np.linspace(-3.5, 3.5, 701)])
+    for x in xs:
+        ref = math.exp(x)
+        assert abs(f("exp", x) - ref) <= 2 * math.ulp(ref), x
+    assert f("exp", 0.0) == 1.0
+
+
+def test_sin_cos():
+    xs = np.concatenate([RNG.uniform(-7, 7, 4000), np.linspace(-math.pi, math.pi, 1001)])
+    for x in xs:
+        assert abs(f("sin", x) - math.sin(x)) <= 3e-16 * max(1.0, abs(x)), x
+        assert abs(f("cos", x) - math.cos(x)) <= 3e-16 * max(1.0, abs(x)), x
+    assert f("sin", 0.0) == 0.0 and f("cos", 0.0) == 1.0
+    # relative accuracy near zero
+    for x in RNG.uniform(1e-9, 1e-3, 200):
+        assert abs(f("sin", x) - math.sin(x)) <= 2 * math.ulp(math.sin(x))
+
+
+@pytest.mark.parametrize("quad", [(1, 1), (1, -1), (-1, 1), (-1, -1)])
+def test_atan2_quadrants(quad):
+    ys = quad[0] * RNG.uniform(0, 10, 2000)
+    xs = quad[1] * RNG.uniform(0, 10, 2000)
+    for y, x in zip(ys, xs):
+        assert abs(f("atan2", y, x) - math.atan2(y, x)) <= 6e-16, (y, x)
+    for y, x in zip(quad[0] * RNG.uniform(0, 1e-6, 200), quad[1] * RNG.uniform(1, 2, 200)):
+        ref = math.atan2(y, x)
+        assert abs(f("atan2", y, x) - ref) <= 4 * math.ulp(ref) + 1e-300
+
+
+def test_atan2_axes():
+    assert f("atan2", 0.0, 1.0) == 0.0
+    assert abs(f("atan2", 1.0, 0.0) - math.pi / 2) <= 2 * math.ulp(math.pi / 2)
+    assert abs(f("atan2", 1.0, -1e-30) - math.pi / 2) <= 2 * math.ulp(math.pi / 2)
+    assert abs(f("atan2", 0.0, -1.0) - math.pi) <= 2 * math.ulp(math.pi)
+
+
+def test_acos():
+    ts = np.concatenate([RNG.uniform(-1, 1, 4000), 1 - RNG.uniform(0, 1e-8, 200),
+                         -1 + RNG.uniform(0, 1e-8, 200)])
+    for t in ts:
+        ref = math.acos(t)
+        assert abs(f("acos", t) - ref) <= 4 * math.ulp(max(ref, 0.5)), t
+    assert f("acos", 1.0) == 0.0
+    assert abs(f("acos", -1.0) - math.pi) <= 2 * math.ulp(math.pi)
+    # slightly outside [-1, 1] by rounding: clamped, not NaN
+    assert f("acos", 1.0 + 2e-16) == 0.0
