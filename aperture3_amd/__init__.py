@@ -27,6 +27,11 @@ KERNEL_PUSH_DEPOSIT, KERNEL_FIELD, KERNEL_SORT = range(3)
 MAX_CELL = 0xFFFFFFFF
 FLAG_IGNORE_CURRENT = 1 << 3
 FLAG_IGNORE_EM = 1 << 4
+COORD_CARTESIAN, COORD_LOGSPH = 0, 1
+RT_DEFAULT, RT_EXP_PATH = 0, 1
+# the 13 Grid_LogSph coefficient arrays (grid_log_sph.h:45-49), ABI order
+COEF_NAMES = ("l1_e", "l2_e", "l3_e", "l1_b", "l2_b", "l3_b", "A1_e", "A2_e", "A3_e",
+              "A1_b", "A2_b", "A3_b", "dV")
 
 # stagger of each component (fields_impl.hpp:85-94, sim_data_impl.hpp:51-65)
 STAGGER = {
@@ -48,7 +53,22 @@ class AptParams(C.Structure):
                 ("data_interval", C.c_int), ("current_smoothing", C.c_int),
                 ("fix_3d_filter", C.c_int), ("max_ptc_number", C.c_uint64),
                 ("random_seed", C.c_uint64), ("tile", C.c_int * 3),
-                ("tile_halo", C.c_int), ("tile_threads", C.c_int)]
+                ("tile_halo", C.c_int), ("tile_threads", C.c_int),
+                ("coord", C.c_int), ("compactness", C.c_double), ("omega", C.c_double),
+                ("gravity_on", C.c_int), ("gravity", C.c_double),
+                ("rad_cooling_on", C.c_int), ("rad_cooling_coef", C.c_double),
+                ("B0", C.c_double), ("damping_length", C.c_int),
+                ("damping_coef", C.c_double), ("inject_ions", C.c_int),
+                ("track_percent", C.c_double), ("rt_model", C.c_int),
+                ("gamma_thr", C.c_double), ("photon_path", C.c_double),
+                ("E_secondary", C.c_double), ("r_cutoff", C.c_double),
+                ("e_frac", C.c_double), ("max_photon_number", C.c_uint64)]
+
+# SimParams names copied into AptParams one to one
+_PARAM_FIELDS = ("compactness", "omega", "gravity_on", "gravity", "rad_cooling_on",
+                 "rad_cooling_coef", "B0", "damping_length", "damping_coef", "inject_ions",
+                 "track_percent", "rt_model", "gamma_thr", "photon_path", "E_secondary",
+                 "r_cutoff", "e_frac", "max_photon_number")
 
 
 class AptParticles(C.Structure):
@@ -59,6 +79,23 @@ class AptParticles(C.Structure):
                 ("p3", C.POINTER(C.c_double)),
                 ("E", C.POINTER(C.c_double)), ("weight", C.POINTER(C.c_double)),
                 ("flag", C.POINTER(C.c_uint32)), ("id", C.POINTER(C.c_uint64))]
+
+
+class AptPhotons(C.Structure):
+    _fields_ = [("cell", C.POINTER(C.c_uint32)),
+                ("x1", C.POINTER(C.c_double)), ("x2", C.POINTER(C.c_double)),
+                ("x3", C.POINTER(C.c_double)),
+                ("p1", C.POINTER(C.c_double)), ("p2", C.POINTER(C.c_double)),
+                ("p3", C.POINTER(C.c_double)),
+                ("E", C.POINTER(C.c_double)), ("weight", C.POINTER(C.c_double)),
+                ("path_left", C.POINTER(C.c_double)),
+                ("flag", C.POINTER(C.c_uint32)), ("id", C.POINTER(C.c_uint64))]
+
+
+class AptInfo(C.Structure):
+    _fields_ = [("number", C.c_uint64), ("ph_number", C.c_uint64), ("H", C.c_int),
+                ("Hs", C.c_int), ("id_next", C.c_uint64), ("ph_id_next", C.c_uint64),
+                ("rt_epoch", C.c_uint32), ("inject_epoch", C.c_uint32)]
 
 
 class AptCounters(C.Structure):
@@ -74,6 +111,15 @@ SIGNATURES = {
     "apt_last_error": [_P],
     "apt_get_jexp": [_P, C.POINTER(C.c_int)],
     "apt_get_mesh": [_P, C.POINTER(C.c_int), C.POINTER(C.c_double)],
+    "apt_get_info": [_P, C.POINTER(AptInfo)],
+    "apt_get_coef": [_P, C.c_int, C.POINTER(C.c_double), C.c_uint64],
+    "apt_ph_set": [_P, C.POINTER(AptPhotons), C.c_uint64],
+    "apt_ph_get": [_P, C.POINTER(AptPhotons), C.c_uint64, C.POINTER(C.c_uint64)],
+    "apt_field_boundary": [_P, C.c_double, C.c_double],
+    "apt_inject_ptc": [_P, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                       C.c_double],
+    "apt_emit_photons": [_P, C.POINTER(C.c_uint64)],
+    "apt_produce_pairs": [_P, C.POINTER(C.c_uint64)],
     "apt_field_set": [_P, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_uint64],
     "apt_field_get": [_P, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_uint64],
     "apt_ptc_set": [_P, C.POINTER(AptParticles), C.c_uint64],
@@ -153,6 +199,27 @@ class SimParams:
         self.tile = [0, 0, 0]
         self.tile_halo = 0
         self.tile_threads = 0
+        # log-spherical path (sim_params.h:31-64, 101)
+        self.coord_system = "Cartesian"
+        self.compactness = 0.5
+        self.omega = 0.2
+        self.gravity_on = False
+        self.gravity = 0.0
+        self.rad_cooling_on = False
+        self.rad_cooling_coef = 1.0
+        self.B0 = 1.0
+        self.damping_length = 30
+        self.damping_coef = 0.002
+        self.inject_ions = True
+        self.track_percent = 0.2
+        # pair creation (sim_params.h:23, 52-79)
+        self.rt_model = RT_DEFAULT
+        self.gamma_thr = 20.0
+        self.photon_path = 1.0
+        self.E_secondary = 4.0
+        self.r_cutoff = 4.0
+        self.e_frac = 0.2
+        self.max_photon_number = 0
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise AttributeError(f"unknown parameter {k}")
@@ -211,6 +278,10 @@ class sim_data:
         q.random_seed = p.random_seed
         q.tile_halo = p.tile_halo
         q.tile_threads = p.tile_threads
+        q.coord = COORD_LOGSPH if p.coord_system == "LogSpherical" else COORD_CARTESIAN
+        for k in _PARAM_FIELDS:
+            v = getattr(p, k)
+            setattr(q, k, int(v) if isinstance(v, bool) else v)
         h = C.c_void_p()
         st = self.lib.apt_create(C.byref(g), C.byref(q), C.byref(h))
         if st != APT_OK:
@@ -352,6 +423,49 @@ class sim_data:
         self._check(self.lib.apt_stream_probe(self.ctx, n, reps, C.byref(ms)), "apt_stream_probe")
         return ms.value
 
+    # ---- photons -------------------------------------------------------------
+    PH_FIELDS = (("cell", np.uint32), ("x1", np.float64), ("x2", np.float64),
+                 ("x3", np.float64), ("p1", np.float64), ("p2", np.float64),
+                 ("p3", np.float64), ("E", np.float64), ("w", np.float64),
+                 ("path", np.float64), ("flag", np.uint32), ("id", np.uint64))
+
+    def _ph_struct(self, arrays):
+        v = AptPhotons()
+        v.cell = arrays["cell"].ctypes.data_as(C.POINTER(C.c_uint32))
+        for f in ("x1", "x2", "x3", "p1", "p2", "p3", "E"):
+            setattr(v, f, _dptr(arrays[f]) if f in arrays else None)
+        v.weight = _dptr(arrays["w"])
+        v.path_left = _dptr(arrays["path"])
+        v.flag = arrays["flag"].ctypes.data_as(C.POINTER(C.c_uint32)) if "flag" in arrays else None
+        v.id = arrays["id"].ctypes.data_as(C.POINTER(C.c_uint64)) if "id" in arrays else None
+        return v
+
+    def set_photons(self, ph: dict):
+        n = len(ph["cell"])
+        arrays = {k: np.ascontiguousarray(ph[k], dtype=dt) for k, dt in self.PH_FIELDS if k in ph}
+        self._check(self.lib.apt_ph_set(self.ctx, C.byref(self._ph_struct(arrays)), n),
+                    "apt_ph_set")
+
+    def get_photons(self):
+        n = self.info()["ph_number"]
+        out = {k: np.empty(n, dt) for k, dt in self.PH_FIELDS}
+        got = C.c_uint64()
+        self._check(self.lib.apt_ph_get(self.ctx, C.byref(self._ph_struct(out)), n,
+                                        C.byref(got)), "apt_ph_get")
+        return out
+
+    def coefficient(self, name):
+        """A Grid_LogSph coefficient array (grid_log_sph.h:45-49)."""
+        a = np.empty(self.env.ncell(), np.float64)
+        self._check(self.lib.apt_get_coef(self.ctx, COEF_NAMES.index(name), _dptr(a), a.size),
+                    "apt_get_coef")
+        return a
+
+    def info(self):
+        i = AptInfo()
+        self._check(self.lib.apt_get_info(self.ctx, C.byref(i)), "apt_get_info")
+        return {k: getattr(i, k) for k, _ in AptInfo._fields_}
+
     # ---- control -----------------------------------------------------------
     def jexp(self):
         h = C.c_int()
@@ -413,3 +527,47 @@ class field_solver:
 
     def apply_outflow_boundary(self, data: sim_data, time: float = 0.0):
         """Empty in the reference (field_solver.cu:451-452)."""
+
+
+class ptc_updater_logsph(ptc_updater):
+    """ptc_updater_logsph (src/algorithms/ptc_updater_logsph.h:13-28): the
+    same update_particles entry (the context knows its coordinates) plus
+    inject_ptc."""
+
+    def inject_ptc(self, data: sim_data, inj_per_cell: int, p1: float, p2: float,
+                   p3: float, w: float, omega: float):
+        data._check(data.lib.apt_inject_ptc(data.ctx, inj_per_cell, p1, p2, p3, w, omega),
+                    "apt_inject_ptc")
+
+
+class field_solver_logsph:
+    """field_solver_logsph (src/algorithms/field_solver_logsph.h:10-20)."""
+
+    def __init__(self, env: sim_environment):
+        self.m_env = env
+
+    def update_fields(self, data: sim_data, dt: float, time: float = 0.0):
+        data._check(data.lib.apt_field_update(data.ctx, dt, 0), "apt_field_update")
+
+    def apply_boundary(self, data: sim_data, omega: float, time: float = 0.0):
+        data._check(data.lib.apt_field_boundary(data.ctx, omega, time), "apt_field_boundary")
+
+
+class radiative_transfer:
+    """radiative_transfer (src/radiation/radiative_transfer.h:11-28)."""
+
+    def __init__(self, env: sim_environment):
+        self.m_env = env
+
+    def initialize(self):
+        """user_rt_init is empty for the built-in hook models."""
+
+    def emit_photons(self, data: sim_data) -> int:
+        k = C.c_uint64()
+        data._check(data.lib.apt_emit_photons(data.ctx, C.byref(k)), "apt_emit_photons")
+        return k.value
+
+    def produce_pairs(self, data: sim_data) -> int:
+        k = C.c_uint64()
+        data._check(data.lib.apt_produce_pairs(data.ctx, C.byref(k)), "apt_produce_pairs")
+        return k.value

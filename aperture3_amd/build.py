@@ -30,6 +30,8 @@ HIP_SOURCES = [
     "kern_sort.hip",
     "kern_tile_mixed.hip",
     "kern_tile_sort.hip",
+    "kern_logsph.hip",
+    "kern_rt.hip",
     "apt_context.hip",
 ]
 HOST_SOURCES = ["host/aperture_host.cpp"]

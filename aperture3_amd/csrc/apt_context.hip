@@ -57,6 +57,25 @@ struct apt_ctx {
   bool tiles_valid = false;  // particles are binned by tile
   TileSortWork tsort;
 
+  // log-spherical path
+  bool logsph = false;
+  LsParams lp{};
+  LsCoef coef{};
+  double *coef_mem[LS_NCOEF] = {};
+  long long *surf = nullptr;  // 4 N1 int64: surface density e-, ion (2 limbs)
+  uint32_t inject_epoch = 0;
+  double wmaxw = 1.0;  // largest particle weight: surface-density scale
+
+  // photons and pair creation
+  RtParams rt{};
+  RtWork rtw;
+  uint32_t rt_epoch = 0;
+  uint64_t ph_cap = 0, ph_number = 0;
+  PhoF64 ph64{}, ph64b{};
+  PhoF32 ph32{}, ph32b{};
+  bool ph_have_b = false;
+  uint64_t id_next = 0, ph_id_next = 0;
+
   std::string err;
   bool timing = false;
   std::vector<EventPair> ev[APT_KERNEL_NUM];
@@ -109,6 +128,12 @@ static void update_scale(apt_ctx *ctx) {
   ctx->H = (ctx->prec == APT_PREC_FP64) ? 22 - e : 30 - e;
 }
 
+// surface-density fixed point (injection gate): 2^-Hs with the largest
+// particle weight below 2^(22-Hs)
+static int surface_scale(const apt_ctx *ctx) {
+  return 22 - ceil_log2(ctx->wmaxw > 1.0 ? ctx->wmaxw : 1.0);
+}
+
 static apt_status alloc_ptc(apt_ctx *ctx, bool second) {
   uint64_t n = ctx->cap;
   if (ctx->prec == APT_PREC_FP64) {
@@ -152,6 +177,38 @@ static void free_ptc32(PtcF32 &p) {
   (void)hipFree(p.u1); (void)hipFree(p.u2); (void)hipFree(p.u3); (void)hipFree(p.w);
   (void)hipFree(p.flag);
   p = PtcF32{};
+}
+
+static apt_status alloc_photons(apt_ctx *ctx, bool second) {
+  uint64_t n = ctx->ph_cap;
+  if (ctx->prec == APT_PREC_FP64) {
+    PhoF64 &p = second ? ctx->ph64b : ctx->ph64;
+    HIPCHK(hipMalloc(&p.cell, n * 4));
+    double **d[] = {&p.x1, &p.x2, &p.x3, &p.p1, &p.p2, &p.p3, &p.E, &p.w, &p.path};
+    for (double **q : d) HIPCHK(hipMalloc(q, n * 8));
+    HIPCHK(hipMalloc(&p.flag, n * 4));
+    HIPCHK(hipMalloc(&p.id, n * 8));
+    HIPCHK(launch_fill<uint32_t>(p.cell, MAX_CELL, n, ctx->stream));
+  } else {
+    PhoF32 &p = second ? ctx->ph32b : ctx->ph32;
+    HIPCHK(hipMalloc(&p.cell, n * 4));
+    float **d[] = {&p.x1, &p.x2, &p.x3, &p.p1, &p.p2, &p.p3, &p.w, &p.path};
+    for (float **q : d) HIPCHK(hipMalloc(q, n * 4));
+    HIPCHK(hipMalloc(&p.flag, n * 4));
+    HIPCHK(launch_fill<uint32_t>(p.cell, MAX_CELL, n, ctx->stream));
+  }
+  return APT_OK;
+}
+
+static void free_ph64(PhoF64 &p) {
+  void *a[] = {p.cell, p.x1, p.x2, p.x3, p.p1, p.p2, p.p3, p.E, p.w, p.path, p.flag, p.id};
+  for (void *q : a) (void)hipFree(q);
+  p = PhoF64{};
+}
+static void free_ph32(PhoF32 &p) {
+  void *a[] = {p.cell, p.x1, p.x2, p.x3, p.p1, p.p2, p.p3, p.w, p.path, p.flag};
+  for (void *q : a) (void)hipFree(q);
+  p = PhoF32{};
 }
 
 // Tiling of the mixed form: every tiled dimension periodic and divisible by
@@ -289,7 +346,9 @@ apt_status apt_create(const apt_grid *grid, const apt_params *params,
   int ns = params->num_species < 1 ? 1 : params->num_species;
   if (ns > 4) ns = 4;
   ctx->params.num_species = ns;
-  for (int s = 0; s < ns && ok; s++) ok = alloc_field(&ctx->Rho[s]);
+  // charge arrays for all four species slots: injected ions and pair
+  // secondaries never index a missing array, whatever num_species says
+  for (int s = 0; s < 4 && ok; s++) ok = alloc_field(&ctx->Rho[s]);
   ok = ok && alloc_field(&ctx->divE) && alloc_field(&ctx->divB) &&
        alloc_field(&ctx->jtmp);
   int limbs = ctx->prec == APT_PREC_FP64 ? 2 : 1;
@@ -298,7 +357,7 @@ apt_status apt_create(const apt_grid *grid, const apt_params *params,
       ok = hipMalloc(&ctx->Jacc[c][l], nc * 8) == hipSuccess &&
            launch_fill<long long>(ctx->Jacc[c][l], 0, nc, ctx->stream) ==
                hipSuccess;
-  for (int s = 0; s < ns && ok; s++)
+  for (int s = 0; s < 4 && ok; s++)
     for (int l = 0; l < limbs && ok; l++)
       ok = hipMalloc(&ctx->Racc[s][l], nc * 8) == hipSuccess &&
            launch_fill<long long>(ctx->Racc[s][l], 0, nc, ctx->stream) ==
@@ -320,7 +379,64 @@ apt_status apt_create(const apt_grid *grid, const apt_params *params,
     apt_destroy(ctx);
     return st;
   }
-  if (ctx->prec == APT_PREC_MIXED) {
+  ctx->logsph = params->coord == APT_COORD_LOGSPH;
+  if (params->coord != APT_COORD_CARTESIAN && !ctx->logsph) {
+    apt_destroy(ctx);
+    return APT_ERR_INVALID_ARG;
+  }
+  if (ctx->logsph) {
+    // Grid_LogSph is 2D only (field_solver_logsph.cu:357-358)
+    if (m.dim != 2 || m.guard[0] < 2 || m.guard[1] < 2 ||
+        params->damping_length < 0 || params->damping_length > m.dims[0]) {
+      apt_destroy(ctx);
+      return APT_ERR_INVALID_ARG;
+    }
+    LsParams &lp = ctx->lp;
+    lp.compactness = params->compactness;
+    lp.omega = params->omega;
+    lp.gravity = params->gravity_on ? params->gravity : 0.0;
+    lp.rad_cooling_on = params->rad_cooling_on;
+    lp.rad_cooling_coef = params->rad_cooling_coef;
+    lp.B0 = params->B0;
+    lp.damping_length = params->damping_length;
+    lp.damping_coef = params->damping_coef;
+    lp.inject_ions = params->inject_ions;
+    lp.track_percent = params->track_percent;
+    std::vector<double> c[LS_NCOEF];
+    logsph_coef_host(m, c);
+    bool okc = true;
+    for (int k = 0; k < LS_NCOEF && okc; k++) {
+      okc = hipMalloc(&ctx->coef_mem[k], nc * 8) == hipSuccess &&
+            hipMemcpy(ctx->coef_mem[k], c[k].data(), nc * 8,
+                      hipMemcpyHostToDevice) == hipSuccess;
+      ctx->coef.a[k] = ctx->coef_mem[k];
+    }
+    const uint64_t N1 = (uint64_t)m.N[1];
+    okc = okc && hipMalloc(&ctx->surf, 4 * N1 * 8) == hipSuccess;
+    if (!okc) {
+      apt_destroy(ctx);
+      return APT_ERR_HIP;
+    }
+  }
+  RtParams &rt = ctx->rt;
+  rt.model = params->rt_model;
+  rt.gamma_thr = params->gamma_thr;
+  rt.photon_path = params->photon_path;
+  rt.E_secondary = params->E_secondary;
+  rt.r_cutoff = params->r_cutoff;
+  rt.e_frac = params->e_frac;
+  rt.max_ptc = ctx->cap;
+  rt.max_photon = params->max_photon_number;
+  ctx->ph_cap = params->max_photon_number;
+  if (ctx->ph_cap > 0xffffffffull) {
+    apt_destroy(ctx);
+    return APT_ERR_INVALID_ARG;
+  }
+  if (ctx->ph_cap > 0 && (st = alloc_photons(ctx, false)) != APT_OK) {
+    apt_destroy(ctx);
+    return st;
+  }
+  if (ctx->prec == APT_PREC_MIXED && !ctx->logsph) {
     st = setup_tiles(ctx);
     if (st != APT_OK) {
       apt_destroy(ctx);
@@ -350,6 +466,11 @@ apt_status apt_destroy(apt_ctx *ctx) {
   }
   (void)hipFree(ctx->divE); (void)hipFree(ctx->divB); (void)hipFree(ctx->jtmp);
   (void)hipFree(ctx->d_counters);
+  for (int k = 0; k < LS_NCOEF; k++) (void)hipFree(ctx->coef_mem[k]);
+  (void)hipFree(ctx->surf);
+  free_ph64(ctx->ph64); free_ph64(ctx->ph64b);
+  free_ph32(ctx->ph32); free_ph32(ctx->ph32b);
+  ctx->rtw.release();
   (void)hipFree(ctx->tile_off);
   ctx->tsort.release();
   free_ptc64(ctx->p64); free_ptc64(ctx->p64b);
@@ -382,6 +503,31 @@ apt_status apt_get_mesh(const apt_ctx *ctx, int dims[3], double delta[3]) {
     if (dims) dims[d] = ctx->mesh.dims[d];
     if (delta) delta[d] = ctx->mesh.delta[d];
   }
+  return APT_OK;
+}
+
+apt_status apt_get_info(const apt_ctx *ctx, apt_info *out) {
+  CHECK_CTX(ctx);
+  if (!out) return APT_ERR_INVALID_ARG;
+  out->number = ctx->number;
+  out->ph_number = ctx->ph_number;
+  out->H = ctx->H;
+  out->Hs = surface_scale(ctx);
+  out->id_next = ctx->id_next;
+  out->ph_id_next = ctx->ph_id_next;
+  out->rt_epoch = ctx->rt_epoch;
+  out->inject_epoch = ctx->inject_epoch;
+  return APT_OK;
+}
+
+apt_status apt_get_coef(apt_ctx *ctx, int which, double *dst, uint64_t count) {
+  CHECK_CTX(ctx);
+  if (!ctx->logsph)
+    return ctx->fail(APT_ERR_UNSUPPORTED, "get_coef: not a log-spherical context");
+  if (which < 0 || which >= LS_NCOEF || !dst || count != ctx->ncell)
+    return ctx->fail(APT_ERR_INVALID_ARG, "get_coef: bad coefficient/size");
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(dst, ctx->coef_mem[which], count * 8, hipMemcpyDeviceToHost));
   return APT_OK;
 }
 
@@ -444,14 +590,23 @@ apt_status apt_ptc_set(apt_ctx *ctx, const apt_particles *src, uint64_t n) {
     return ctx->fail(APT_ERR_CAPACITY, "ptc_set: %llu > capacity %llu",
                      (unsigned long long)n, (unsigned long long)ctx->cap);
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  double wmax = fabs(ctx->params.q_e);
+  double wmax = fabs(ctx->params.q_e), wmaxw = 1.0;
   for (uint64_t i = 0; i < n; i++) {
     int s = (int)(src->flag[i] >> 29);
+    if (s > 3)
+      return ctx->fail(APT_ERR_INVALID_ARG, "ptc_set: species %d > 3 at %llu", s,
+                       (unsigned long long)i);
     double qw = fabs(ctx->sp.q[s] * src->weight[i]);
     if (qw > wmax) wmax = qw;
+    if (fabs(src->weight[i]) > wmaxw) wmaxw = fabs(src->weight[i]);
   }
   ctx->wmax = wmax;
+  ctx->wmaxw = wmaxw;
   update_scale(ctx);
+  ctx->id_next = n;
+  if (src->id)
+    for (uint64_t i = 0; i < n; i++)
+      if (src->id[i] + 1 > ctx->id_next) ctx->id_next = src->id[i] + 1;
   if (ctx->prec == APT_PREC_FP64) {
     PtcF64 &p = ctx->p64;
     HIPCHK(hipMemcpy(p.cell, src->cell, n * 4, hipMemcpyHostToDevice));
@@ -575,6 +730,106 @@ apt_status apt_ptc_number(apt_ctx *ctx, uint64_t *n) {
   return APT_OK;
 }
 
+apt_status apt_ph_set(apt_ctx *ctx, const apt_photons *src, uint64_t n) {
+  CHECK_CTX(ctx);
+  if (!src || !src->cell || !src->x1 || !src->x2 || !src->x3 || !src->p1 ||
+      !src->p2 || !src->p3 || !src->weight || !src->path_left)
+    return ctx->fail(APT_ERR_INVALID_ARG, "ph_set: missing arrays");
+  if (n > ctx->ph_cap)
+    return ctx->fail(APT_ERR_CAPACITY, "ph_set: %llu > photon capacity %llu",
+                     (unsigned long long)n, (unsigned long long)ctx->ph_cap);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  std::vector<uint32_t> fl(n, 0u);
+  if (src->flag) fl.assign(src->flag, src->flag + n);
+  if (ctx->prec == APT_PREC_FP64) {
+    PhoF64 &p = ctx->ph64;
+    HIPCHK(hipMemcpy(p.cell, src->cell, n * 4, hipMemcpyHostToDevice));
+    const double *hs[] = {src->x1, src->x2, src->x3, src->p1, src->p2,
+                          src->p3, src->weight, src->path_left};
+    double *ds[] = {p.x1, p.x2, p.x3, p.p1, p.p2, p.p3, p.w, p.path};
+    for (int k = 0; k < 8; k++)
+      HIPCHK(hipMemcpy(ds[k], hs[k], n * 8, hipMemcpyHostToDevice));
+    std::vector<double> e(n);
+    for (uint64_t i = 0; i < n; i++)
+      e[i] = src->E ? src->E[i]
+                    : sqrt(src->p1[i] * src->p1[i] + src->p2[i] * src->p2[i] +
+                           src->p3[i] * src->p3[i]);
+    HIPCHK(hipMemcpy(p.E, e.data(), n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p.flag, fl.data(), n * 4, hipMemcpyHostToDevice));
+    std::vector<uint64_t> ids(n);
+    for (uint64_t i = 0; i < n; i++) ids[i] = src->id ? src->id[i] : i;
+    HIPCHK(hipMemcpy(p.id, ids.data(), n * 8, hipMemcpyHostToDevice));
+  } else {
+    PhoF32 &p = ctx->ph32;
+    HIPCHK(hipMemcpy(p.cell, src->cell, n * 4, hipMemcpyHostToDevice));
+    const double *hs[] = {src->x1, src->x2, src->x3, src->p1, src->p2,
+                          src->p3, src->weight, src->path_left};
+    float *ds[] = {p.x1, p.x2, p.x3, p.p1, p.p2, p.p3, p.w, p.path};
+    std::vector<float> f(n);
+    for (int k = 0; k < 8; k++) {
+      for (uint64_t i = 0; i < n; i++) f[i] = (float)hs[k][i];
+      HIPCHK(hipMemcpy(ds[k], f.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(p.flag, fl.data(), n * 4, hipMemcpyHostToDevice));
+  }
+  uint32_t *cell = ctx->prec == APT_PREC_FP64 ? ctx->ph64.cell : ctx->ph32.cell;
+  if (n < ctx->ph_number)
+    HIPCHK(launch_fill<uint32_t>(cell + n, MAX_CELL, ctx->ph_number - n,
+                                 ctx->stream));
+  ctx->ph_number = n;
+  ctx->ph_id_next = n;
+  if (src->id)
+    for (uint64_t i = 0; i < n; i++)
+      if (src->id[i] + 1 > ctx->ph_id_next) ctx->ph_id_next = src->id[i] + 1;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return APT_OK;
+}
+
+apt_status apt_ph_get(apt_ctx *ctx, apt_photons *dst, uint64_t cap,
+                      uint64_t *n) {
+  CHECK_CTX(ctx);
+  if (!dst) return APT_ERR_INVALID_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const uint64_t num = ctx->ph_number;
+  if (n) *n = num;
+  if (num > cap)
+    return ctx->fail(APT_ERR_CAPACITY, "ph_get: %llu photons > cap %llu",
+                     (unsigned long long)num, (unsigned long long)cap);
+  if (num == 0) return APT_OK;
+  if (ctx->prec == APT_PREC_FP64) {
+    PhoF64 &p = ctx->ph64;
+    double *hs[] = {dst->x1, dst->x2, dst->x3, dst->p1, dst->p2, dst->p3,
+                    dst->E, dst->weight, dst->path_left};
+    double *ds[] = {p.x1, p.x2, p.x3, p.p1, p.p2, p.p3, p.E, p.w, p.path};
+    for (int k = 0; k < 9; k++)
+      if (hs[k]) HIPCHK(hipMemcpy(hs[k], ds[k], num * 8, hipMemcpyDeviceToHost));
+    if (dst->id) HIPCHK(hipMemcpy(dst->id, p.id, num * 8, hipMemcpyDeviceToHost));
+    if (dst->cell) HIPCHK(hipMemcpy(dst->cell, p.cell, num * 4, hipMemcpyDeviceToHost));
+    if (dst->flag) HIPCHK(hipMemcpy(dst->flag, p.flag, num * 4, hipMemcpyDeviceToHost));
+  } else {
+    PhoF32 &p = ctx->ph32;
+    double *hs[] = {dst->x1, dst->x2, dst->x3, dst->p1, dst->p2, dst->p3,
+                    dst->weight, dst->path_left};
+    float *ds[] = {p.x1, p.x2, p.x3, p.p1, p.p2, p.p3, p.w, p.path};
+    std::vector<float> f(num), pp[3];
+    for (int k = 0; k < 8; k++) {
+      HIPCHK(hipMemcpy(f.data(), ds[k], num * 4, hipMemcpyDeviceToHost));
+      if (k >= 3 && k < 6) pp[k - 3] = f;
+      if (hs[k])
+        for (uint64_t i = 0; i < num; i++) hs[k][i] = f[i];
+    }
+    if (dst->E)
+      for (uint64_t i = 0; i < num; i++)
+        dst->E[i] = sqrt((double)pp[0][i] * pp[0][i] + (double)pp[1][i] * pp[1][i] +
+                         (double)pp[2][i] * pp[2][i]);
+    if (dst->id)
+      for (uint64_t i = 0; i < num; i++) dst->id[i] = i;
+    if (dst->cell) HIPCHK(hipMemcpy(dst->cell, p.cell, num * 4, hipMemcpyDeviceToHost));
+    if (dst->flag) HIPCHK(hipMemcpy(dst->flag, p.flag, num * 4, hipMemcpyDeviceToHost));
+  }
+  return APT_OK;
+}
+
 apt_status apt_ptc_load_uniform(apt_ctx *ctx, int ppc_pairs,
                                 const double theta[2], const double drift[2],
                                 double weight) {
@@ -611,6 +866,8 @@ apt_status apt_ptc_load_uniform(apt_ctx *ctx, int ppc_pairs,
     HIPCHK(launch_load_uniform_f32(m, lp, ctx->p32, ctx->d_counters,
                                    ctx->stream));
   ctx->number = base + add;
+  if (ctx->id_next < ctx->number) ctx->id_next = ctx->number;
+  if (fabs(weight) > ctx->wmaxw) ctx->wmaxw = fabs(weight);
   ctx->tiles_valid = false;
   if (tiled_load) {
     const uint64_t per = (uint64_t)ctx->tiles.T[0] * ctx->tiles.T[1] *
@@ -638,7 +895,27 @@ static apt_status filter_pass(apt_ctx *ctx, void *arr) {
   return APT_OK;
 }
 
+template <typename T>
+static apt_status logsph_smooth(apt_ctx *ctx, int rho) {
+  const double *A[3] = {ctx->coef.a[LS_A1E], ctx->coef.a[LS_A2E],
+                        ctx->coef.a[LS_A3E]};
+  for (int pass = 0; pass < ctx->params.current_smoothing; pass++) {
+    for (int c = 0; c < 3; c++)
+      HIPCHK(launch_logsph_filter<T>(ctx->mesh, (T *)ctx->J[c], (T *)ctx->jtmp,
+                                     A[c], ctx->stream));
+    if (rho)
+      for (int s = 0; s < ctx->params.num_species; s++)
+        HIPCHK(launch_logsph_filter<T>(ctx->mesh, (T *)ctx->Rho[s],
+                                       (T *)ctx->jtmp, ctx->coef.a[LS_DV],
+                                       ctx->stream));
+  }
+  return APT_OK;
+}
+
 static apt_status smooth(apt_ctx *ctx, int rho) {
+  if (ctx->logsph)
+    return ctx->prec == APT_PREC_FP64 ? logsph_smooth<double>(ctx, rho)
+                                      : logsph_smooth<float>(ctx, rho);
   for (int pass = 0; pass < ctx->params.current_smoothing; pass++) {
     for (int c = 0; c < 3; c++) {
       apt_status st = filter_pass(ctx, ctx->J[c]);
@@ -659,12 +936,101 @@ apt_status apt_smooth_current(apt_ctx *ctx, uint32_t step) {
   return smooth(ctx, ((step + 1) % (uint32_t)di) == 0);
 }
 
+static apt_status move_photons(apt_ctx *ctx, double dt) {
+  if (ctx->ph_number == 0) return APT_OK;
+  const int ls = ctx->logsph ? 1 : 0;
+  if (ctx->prec == APT_PREC_FP64)
+    HIPCHK(rt_move_photons<double>(ctx->mesh, ls, ctx->rt.r_cutoff,
+                                   view_of(ctx->ph64), ctx->ph_number, dt,
+                                   ctx->stream));
+  else
+    HIPCHK(rt_move_photons<float>(ctx->mesh, ls, ctx->rt.r_cutoff,
+                                  view_of(ctx->ph32), ctx->ph_number, dt,
+                                  ctx->stream));
+  if (ctx->logsph) {
+    uint32_t *cell = ctx->prec == APT_PREC_FP64 ? ctx->ph64.cell : ctx->ph32.cell;
+    HIPCHK(launch_erase_guard(ctx->mesh, cell, ctx->ph_number, ctx->stream));
+  }
+  return APT_OK;
+}
+
+// ptc_updater_logsph::update_particles + apply_boundary
+// (ptc_updater_logsph.cu:681-846): push + move + deposit, process_j,
+// smoothing, guard erase, axis J3, outflow flag, photon move.
+static apt_status logsph_ptc_update(apt_ctx *ctx, double dt, int rho) {
+  const int ns = 4;  // every allocated charge slot (see apt_create)
+  uint32_t *cell, *flag;
+  time_begin(ctx, APT_KERNEL_PUSH_DEPOSIT);
+  if (ctx->prec == APT_PREC_FP64) {
+    FieldsF64 f;
+    AccF64 a;
+    for (int c = 0; c < 3; c++) {
+      f.E[c] = (const double *)ctx->E[c];
+      f.B[c] = (const double *)ctx->B[c];
+      a.J[c][0] = ctx->Jacc[c][0];
+      a.J[c][1] = ctx->Jacc[c][1];
+    }
+    for (int s = 0; s < 4; s++) {
+      a.R[s][0] = ctx->Racc[s][0];
+      a.R[s][1] = ctx->Racc[s][1];
+    }
+    HIPCHK(launch_logsph_push_f64(ctx->mesh, ctx->sp, ctx->lp, dt, ctx->H, rho,
+                                  ctx->p64, f, a, ctx->d_counters, ctx->number,
+                                  ctx->stream));
+    cell = ctx->p64.cell;
+    flag = ctx->p64.flag;
+  } else {
+    FieldsF32 f;
+    AccMixed a;
+    for (int c = 0; c < 3; c++) {
+      f.E[c] = (const float *)ctx->E[c];
+      f.B[c] = (const float *)ctx->B[c];
+      a.J[c] = ctx->Jacc[c][0];
+    }
+    for (int s = 0; s < 4; s++) a.R[s] = ctx->Racc[s][0];
+    a.rho = rho;
+    HIPCHK(launch_logsph_push_f32(ctx->mesh, ctx->sp, ctx->lp, (float)dt,
+                                  ctx->H, ctx->p32, f, a, ctx->d_counters,
+                                  ctx->number, ctx->stream));
+    cell = ctx->p32.cell;
+    flag = ctx->p32.flag;
+  }
+  time_end(ctx, APT_KERNEL_PUSH_DEPOSIT);
+  const int limbs = ctx->prec == APT_PREC_FP64 ? 2 : 1;
+  if (ctx->prec == APT_PREC_FP64)
+    HIPCHK(launch_logsph_process_j<double>(ctx->mesh, ctx->coef, dt, ctx->H, limbs,
+                                           ctx->Jacc, ctx->Racc, ns,
+                                           (double *const *)ctx->J,
+                                           (double *const *)ctx->Rho, ctx->stream));
+  else
+    HIPCHK(launch_logsph_process_j<float>(ctx->mesh, ctx->coef, dt, ctx->H, limbs,
+                                          ctx->Jacc, ctx->Racc, ns,
+                                          (float *const *)ctx->J,
+                                          (float *const *)ctx->Rho, ctx->stream));
+  apt_status st = smooth(ctx, rho);
+  if (st != APT_OK) return st;
+  if (ctx->prec == APT_PREC_FP64)
+    HIPCHK(launch_logsph_ptc_boundary<double>(ctx->mesh, ctx->lp,
+                                              (double *)ctx->J[2], cell, flag,
+                                              ctx->number, ctx->stream));
+  else
+    HIPCHK(launch_logsph_ptc_boundary<float>(ctx->mesh, ctx->lp,
+                                             (float *)ctx->J[2], cell, flag,
+                                             ctx->number, ctx->stream));
+  return move_photons(ctx, dt);
+}
+
 apt_status apt_ptc_update(apt_ctx *ctx, double dt, uint32_t step) {
   CHECK_CTX(ctx);
   const int di = ctx->params.data_interval > 0 ? ctx->params.data_interval : 1;
   const int rho = ((step + 1) % (uint32_t)di) == 0;
   const uint64_t nc = ctx->ncell;
-  const int ns = ctx->params.num_species;
+  const int ns = 4;  // every allocated charge slot (see apt_create)
+  if (ctx->logsph) {
+    if (!(dt > 0.0))
+      return ctx->fail(APT_ERR_INVALID_ARG, "ptc_update: dt must be > 0");
+    return logsph_ptc_update(ctx, dt, rho);
+  }
   if (ctx->prec == APT_PREC_FP64) {
     FieldsF64 f;
     for (int c = 0; c < 3; c++) {
@@ -730,7 +1096,9 @@ apt_status apt_ptc_update(apt_ctx *ctx, double dt, uint32_t step) {
         HIPCHK(launch_fill<float>((float *)ctx->Rho[s], 0.0f, nc, ctx->stream));
     }
   }
-  return smooth(ctx, rho);
+  apt_status st = smooth(ctx, rho);
+  if (st != APT_OK) return st;
+  return move_photons(ctx, dt);
 }
 
 // ---- fields --------------------------------------------------------------
@@ -784,10 +1152,123 @@ static apt_status field_update_t(apt_ctx *ctx, double dt, uint32_t step) {
   return APT_OK;
 }
 
+template <typename T>
+static apt_status logsph_field_update_t(apt_ctx *ctx, double dt) {
+  time_begin(ctx, APT_KERNEL_FIELD);
+  HIPCHK(launch_logsph_field_update<T>(ctx->mesh, ctx->coef, ctx->lp,
+                                       field_set<T>(ctx), dt, ctx->stream));
+  time_end(ctx, APT_KERNEL_FIELD);
+  return APT_OK;
+}
+
 apt_status apt_field_update(apt_ctx *ctx, double dt, uint32_t step) {
   CHECK_CTX(ctx);
+  if (ctx->logsph)
+    return ctx->prec == APT_PREC_FP64 ? logsph_field_update_t<double>(ctx, dt)
+                                      : logsph_field_update_t<float>(ctx, dt);
   return ctx->prec == APT_PREC_FP64 ? field_update_t<double>(ctx, dt, step)
                                     : field_update_t<float>(ctx, dt, step);
+}
+
+apt_status apt_field_boundary(apt_ctx *ctx, double omega, double time) {
+  CHECK_CTX(ctx);
+  (void)time;
+  if (!ctx->logsph) return APT_OK;
+  if (ctx->prec == APT_PREC_FP64)
+    HIPCHK(launch_logsph_field_boundary<double>(ctx->mesh, ctx->lp,
+                                                field_set<double>(ctx), omega,
+                                                ctx->stream));
+  else
+    HIPCHK(launch_logsph_field_boundary<float>(ctx->mesh, ctx->lp,
+                                               field_set<float>(ctx), omega,
+                                               ctx->stream));
+  return APT_OK;
+}
+
+apt_status apt_inject_ptc(apt_ctx *ctx, int inj_per_cell, double p1,
+                          double p2, double p3, double w, double omega) {
+  CHECK_CTX(ctx);
+  (void)p2;
+  (void)p3;
+  (void)omega;
+  if (!ctx->logsph)
+    return ctx->fail(APT_ERR_UNSUPPORTED, "inject_ptc: log-spherical contexts only");
+  if (inj_per_cell < 1)
+    return ctx->fail(APT_ERR_INVALID_ARG, "inject_ptc: inj_per_cell < 1");
+  const uint64_t rows = (uint64_t)ctx->mesh.N[1] - 2;
+  const uint64_t add = rows * (uint64_t)inj_per_cell * 2;
+  if (ctx->number + add > ctx->cap)
+    return ctx->fail(APT_ERR_CAPACITY, "inject_ptc: %llu particles > capacity",
+                     (unsigned long long)(ctx->number + add));
+  if (fabs(w) > ctx->wmaxw) ctx->wmaxw = fabs(w);
+  double qw = fabs(ctx->params.q_e * w);
+  if (qw > ctx->wmax) {
+    ctx->wmax = qw;
+    update_scale(ctx);
+  }
+  const int Hs = surface_scale(ctx);
+  if (ctx->prec == APT_PREC_FP64)
+    HIPCHK(launch_logsph_inject<double>(ctx->mesh, ctx->lp, view_of(ctx->p64),
+                                        ctx->number, inj_per_cell, p1, w,
+                                        ctx->params.q_e, ctx->params.random_seed,
+                                        ctx->inject_epoch, Hs, ctx->id_next,
+                                        ctx->surf, ctx->stream));
+  else
+    HIPCHK(launch_logsph_inject<float>(ctx->mesh, ctx->lp, view_of(ctx->p32),
+                                       ctx->number, inj_per_cell, p1, w,
+                                       ctx->params.q_e, ctx->params.random_seed,
+                                       ctx->inject_epoch, Hs, ctx->id_next,
+                                       ctx->surf, ctx->stream));
+  ctx->inject_epoch++;
+  ctx->number += add;
+  ctx->id_next += add;
+  return APT_OK;
+}
+
+apt_status apt_emit_photons(apt_ctx *ctx, uint64_t *emitted) {
+  CHECK_CTX(ctx);
+  if (ctx->ph_cap == 0)
+    return ctx->fail(APT_ERR_STATE, "emit_photons: max_photon_number is 0");
+  uint64_t k = 0;
+  const uint64_t seed = ctx->params.random_seed;
+  if (ctx->prec == APT_PREC_FP64)
+    HIPCHK(rt_emit<double>(ctx->mesh, ctx->rt, view_of(ctx->p64), ctx->number,
+                           view_of(ctx->ph64), ctx->ph_number, seed,
+                           ctx->rt_epoch, ctx->ph_id_next, ctx->rtw,
+                           ctx->d_counters, ctx->stream, &k));
+  else
+    HIPCHK(rt_emit<float>(ctx->mesh, ctx->rt, view_of(ctx->p32), ctx->number,
+                          view_of(ctx->ph32), ctx->ph_number, seed,
+                          ctx->rt_epoch, ctx->ph_id_next, ctx->rtw,
+                          ctx->d_counters, ctx->stream, &k));
+  ctx->rt_epoch++;
+  ctx->ph_number += k;
+  ctx->ph_id_next += k;
+  if (emitted) *emitted = k;
+  return APT_OK;
+}
+
+apt_status apt_produce_pairs(apt_ctx *ctx, uint64_t *pairs) {
+  CHECK_CTX(ctx);
+  if (ctx->ph_cap == 0)
+    return ctx->fail(APT_ERR_STATE, "produce_pairs: max_photon_number is 0");
+  uint64_t k = 0;
+  if (ctx->prec == APT_PREC_FP64)
+    HIPCHK(rt_pairs<double>(ctx->mesh, ctx->rt, view_of(ctx->p64), ctx->number,
+                            view_of(ctx->ph64), ctx->ph_number, ctx->id_next,
+                            ctx->rtw, ctx->d_counters, ctx->stream, &k));
+  else
+    HIPCHK(rt_pairs<float>(ctx->mesh, ctx->rt, view_of(ctx->p32), ctx->number,
+                           view_of(ctx->ph32), ctx->ph_number, ctx->id_next,
+                           ctx->rtw, ctx->d_counters, ctx->stream, &k));
+  if (k > 0) {
+    // secondaries carry weights from the photons (<= any parent weight),
+    // so the fixed-point scale stays valid
+    ctx->number += 2 * k;
+    ctx->id_next += 2 * k;
+  }
+  if (pairs) *pairs = k;
+  return APT_OK;
 }
 apt_status apt_field_update_e(apt_ctx *ctx, double dt) {
   CHECK_CTX(ctx);
@@ -877,8 +1358,60 @@ static apt_status sort_tiles(apt_ctx *ctx) {
   return APT_OK;
 }
 
+// photons.sort_by_cell (sim_data_impl.hpp:80): stable sort by cell,
+// empty slots dropped
+static apt_status sort_photons(apt_ctx *ctx) {
+  const uint64_t n = ctx->ph_number;
+  if (n == 0) return APT_OK;
+  if (!ctx->ph_have_b) {
+    apt_status st = alloc_photons(ctx, true);
+    if (st != APT_OK) return st;
+    ctx->ph_have_b = true;
+  }
+  const uint32_t empty_key = (uint32_t)ctx->ncell;
+  SortWork &w = ctx->sortw;
+  if (ctx->prec == APT_PREC_FP64) {
+    PhoF64 &a = ctx->ph64, &b = ctx->ph64b;
+    HIPCHK(sort_keys(w, a.cell, n, empty_key, nullptr, ctx->stream));
+    HIPCHK(gather_field<uint32_t>(w, a.cell, b.cell, n, ctx->stream));
+    double *sa[] = {a.x1, a.x2, a.x3, a.p1, a.p2, a.p3, a.E, a.w, a.path};
+    double *sb[] = {b.x1, b.x2, b.x3, b.p1, b.p2, b.p3, b.E, b.w, b.path};
+    for (int k = 0; k < 9; k++)
+      HIPCHK(gather_field<double>(w, sa[k], sb[k], n, ctx->stream));
+    HIPCHK(gather_field<uint32_t>(w, a.flag, b.flag, n, ctx->stream));
+    HIPCHK(gather_field<uint64_t>(w, a.id, b.id, n, ctx->stream));
+    std::swap(ctx->ph64, ctx->ph64b);
+  } else {
+    PhoF32 &a = ctx->ph32, &b = ctx->ph32b;
+    HIPCHK(sort_keys(w, a.cell, n, empty_key, nullptr, ctx->stream));
+    HIPCHK(gather_field<uint32_t>(w, a.cell, b.cell, n, ctx->stream));
+    float *sa[] = {a.x1, a.x2, a.x3, a.p1, a.p2, a.p3, a.w, a.path};
+    float *sb[] = {b.x1, b.x2, b.x3, b.p1, b.p2, b.p3, b.w, b.path};
+    for (int k = 0; k < 8; k++)
+      HIPCHK(gather_field<float>(w, sa[k], sb[k], n, ctx->stream));
+    HIPCHK(gather_field<uint32_t>(w, a.flag, b.flag, n, ctx->stream));
+    std::swap(ctx->ph32, ctx->ph32b);
+  }
+  unsigned long long kept = 0;
+  HIPCHK(hipMemcpyAsync(&kept, w.count, 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  uint32_t *cell = ctx->prec == APT_PREC_FP64 ? ctx->ph64.cell : ctx->ph32.cell;
+  if (kept < n)
+    HIPCHK(launch_fill<uint32_t>(cell + kept, MAX_CELL, n - kept, ctx->stream));
+  ctx->ph_number = kept;
+  return APT_OK;
+}
+
+static apt_status sort_particles_only(apt_ctx *ctx);
+
 apt_status apt_sort_particles(apt_ctx *ctx) {
   CHECK_CTX(ctx);
+  apt_status st = sort_particles_only(ctx);
+  if (st != APT_OK) return st;
+  return sort_photons(ctx);
+}
+
+static apt_status sort_particles_only(apt_ctx *ctx) {
   uint64_t n = ctx->number;
   if (n == 0) return APT_OK;
   apt_status st = ensure_second_buffer(ctx);

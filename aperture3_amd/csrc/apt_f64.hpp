@@ -1,0 +1,123 @@
+// apt_f64.hpp -- FP64 device building blocks shared by the Cartesian and
+// log-spherical kernels: CIC weights and gathers (interpolation.cuh:28-209),
+// the Vay pusher (vay_push.cuh:8-30), the Esirkepov helpers
+// (ptc_updater_helper.cuh:62-77) and the two-limb fixed-point atomic add.
+// Every expression keeps the reference's operation order; with contraction
+// off they round exactly like the CPU restatement.
+#pragma once
+#include "apt_common.hpp"
+
+namespace apt {
+
+__device__ inline double cic(double dx) {
+  double a = 1.0 - fabs(dx);
+  return a > 0.0 ? a : 0.0;
+}
+
+// interp_cell (interpolation.cuh:79-87)
+__device__ inline double interp_cell(double rel, int c, int t, int st) {
+  double x = ((double)t + (st == 0 ? 0.5 : 1.0)) - (rel + (double)c);
+  return cic(x);
+}
+
+// Interpolator1D/2D/3D (interpolation.cuh:91-209): all taps, same order.
+__device__ inline double gather1(const double *__restrict__ f, double x1,
+                                 int c1, int st) {
+  double r = 0.0;
+#pragma unroll
+  for (int i = 0; i <= 2; i++) {
+    int ii = i + c1 - 1;
+    r += f[ii] * interp_cell(x1, c1, ii, st & 1);
+  }
+  return r;
+}
+
+__device__ inline double gather2(const double *__restrict__ f, int d0,
+                                 double x1, double x2, int c1, int c2,
+                                 int st) {
+  double r = 0.0;
+#pragma unroll
+  for (int j = 0; j <= 2; j++) {
+    int jj = j + c2 - 1;
+#pragma unroll
+    for (int i = 0; i <= 2; i++) {
+      int ii = i + c1 - 1;
+      r += f[ii + (int64_t)jj * d0] * interp_cell(x1, c1, ii, st & 1) *
+           interp_cell(x2, c2, jj, (st >> 1) & 1);
+    }
+  }
+  return r;
+}
+
+__device__ inline double gather3(const double *__restrict__ f, int d0,
+                                 int d1, double x1, double x2, double x3,
+                                 int c1, int c2, int c3, int st) {
+  double r = 0.0;
+#pragma unroll
+  for (int k = -1; k <= 1; k++) {
+    double w3 = interp_cell(x3, c3, k + c3, (st >> 2) & 1);
+#pragma unroll
+    for (int j = -1; j <= 1; j++) {
+      double w2 = interp_cell(x2, c2, j + c2, (st >> 1) & 1);
+#pragma unroll
+      for (int i = -1; i <= 1; i++) {
+        int64_t o = (int64_t)(i + c1) + (int64_t)(j + c2) * d0 +
+                    (int64_t)(k + c3) * d0 * d1;
+        r += f[o] * interp_cell(x1, c1, i + c1, st & 1) * w2 * w3;
+      }
+    }
+  }
+  return r;
+}
+
+// vay_push (vay_push.cuh:8-30), fields pre-scaled by dt q / 2m.
+__device__ inline void vay_push(double &p1, double &p2, double &p3,
+                                double &gamma, double E1, double E2,
+                                double E3, double B1, double B2, double B3) {
+  double up1 = p1 + 2.0 * E1 + (p2 * B3 - p3 * B2) / gamma;
+  double up2 = p2 + 2.0 * E2 + (p3 * B1 - p1 * B3) / gamma;
+  double up3 = p3 + 2.0 * E3 + (p1 * B2 - p2 * B1) / gamma;
+  double tt = B1 * B1 + B2 * B2 + B3 * B3;
+  double ut = up1 * B1 + up2 * B2 + up3 * B3;
+  double sigma = 1.0 + up1 * up1 + up2 * up2 + up3 * up3 - tt;
+  double inv_gamma2 =
+      2.0 / (sigma + sqrt(sigma * sigma + 4.0 * (tt + ut * ut)));
+  double s = 1.0 / (1.0 + inv_gamma2 * tt);
+  gamma = 1.0 / sqrt(inv_gamma2);
+  p1 = (up1 + B1 * ut * inv_gamma2 + (up2 * B3 - up3 * B2) / gamma) * s;
+  p2 = (up2 + B2 * ut * inv_gamma2 + (up3 * B1 - up1 * B3) / gamma) * s;
+  p3 = (up3 + B3 * ut * inv_gamma2 + (up1 * B2 - up2 * B1) / gamma) * s;
+}
+
+// center2d / movement2d / movement3d (ptc_updater_helper.cuh:62-77), with
+// an exact 1/6 where the reference writes 0.1666667f.
+__device__ inline double center2d(double sx0, double sx1, double sy0,
+                                  double sy1) {
+  return (2.0 * sx1 * sy1 + sx0 * sy1 + sx1 * sy0 + 2.0 * sx0 * sy0) *
+         (1.0 / 6.0);
+}
+__device__ inline double movement3d(double sx0, double sx1, double sy0,
+                                    double sy1, double sz0, double sz1) {
+  return (sz1 - sz0) * center2d(sx0, sx1, sy0, sy1);
+}
+__device__ inline double movement2d(double sx0, double sx1, double sy0,
+                                    double sy1) {
+  return (sy1 - sy0) * 0.5 * (sx0 + sx1);
+}
+
+__device__ inline void acc_add(long long *__restrict__ hi,
+                               long long *__restrict__ lo, uint64_t o,
+                               double c, int H) {
+  long long h, l;
+  fx_split(c, H, h, l);
+  if (h != 0)
+    __hip_atomic_fetch_add((unsigned long long *)(hi + o),
+                           (unsigned long long)h, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  if (l != 0)
+    __hip_atomic_fetch_add((unsigned long long *)(lo + o),
+                           (unsigned long long)l, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace apt

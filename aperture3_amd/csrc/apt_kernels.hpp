@@ -2,6 +2,8 @@
 // kernel in the library (the role data_ptrs plays in the reference,
 // cuda/data_ptrs.h:11-29, but passed per launch and split by precision).
 #pragma once
+#include <vector>
+
 #include "apt_common.hpp"
 
 namespace apt {
@@ -87,6 +89,27 @@ hipError_t launch_push_deposit_mixed(const Mesh &m, const Species &sp,
 hipError_t launch_finalize_mixed(long long *acc, float *out, uint64_t n,
                                  int S, hipStream_t st);
 
+// ---- particle views shared by the forms -------------------------------
+// E (stored gamma) and id exist in the FP64 form only; the mixed form
+// recomputes gamma from u and carries no id.
+template <typename R>
+struct PtcView {
+  uint32_t *cell;
+  R *x1, *x2, *x3;
+  R *p1, *p2, *p3;
+  R *E;
+  R *w;
+  uint32_t *flag;
+  uint64_t *id;
+};
+inline PtcView<double> view_of(const PtcF64 &p) {
+  return {p.cell, p.x1, p.x2, p.x3, p.p1, p.p2, p.p3, p.E, p.w, p.flag, p.id};
+}
+inline PtcView<float> view_of(const PtcF32 &p) {
+  return {p.cell, p.x1, p.x2, p.x3, p.u1, p.u2, p.u3, nullptr, p.w, p.flag,
+          nullptr};
+}
+
 // ---- fields (T = double: FP64 form, T = float: mixed form) ----------
 template <typename T>
 struct FieldSet {
@@ -111,6 +134,136 @@ hipError_t launch_filter(const Mesh &m, T *j, T *tmp, int fix3d,
                          hipStream_t st);
 template <typename T>
 hipError_t launch_fill(T *a, T v, uint64_t n, hipStream_t st);
+
+// ---- 2D log-spherical path (kern_logsph.hip) --------------------------
+// the 13 Grid_LogSph coefficient arrays (grid_log_sph.h:45-49), FP64
+enum LsCoefIdx {
+  LS_L1E, LS_L2E, LS_L3E, LS_L1B, LS_L2B, LS_L3B,
+  LS_A1E, LS_A2E, LS_A3E, LS_A1B, LS_A2B, LS_A3B, LS_DV,
+  LS_NCOEF
+};
+struct LsCoef {
+  const double *a[LS_NCOEF];
+};
+// the SimParams entries of the log-spherical path (sim_params.h:31-64);
+// gravity is 0 when gravity_on is off
+struct LsParams {
+  double compactness, omega, gravity, rad_cooling_coef, B0, damping_coef,
+      track_percent;
+  int rad_cooling_on, damping_length, inject_ions;
+};
+
+void logsph_coef_host(const Mesh &m, std::vector<double> (&c)[LS_NCOEF]);
+hipError_t launch_logsph_push_f64(const Mesh &m, const Species &sp,
+                                  const LsParams &lp, double dt, int H,
+                                  int rho, const PtcF64 &p,
+                                  const FieldsF64 &f, const AccF64 &a,
+                                  unsigned long long *counters, uint64_t n,
+                                  hipStream_t st);
+hipError_t launch_logsph_push_f32(const Mesh &m, const Species &sp,
+                                  const LsParams &lp, float dt, int S,
+                                  const PtcF32 &p, const FieldsF32 &f,
+                                  const AccMixed &a,
+                                  unsigned long long *counters, uint64_t n,
+                                  hipStream_t st);
+template <typename T>
+hipError_t launch_logsph_process_j(const Mesh &m, const LsCoef &cf, double dt,
+                                   int H, int limbs, long long *const (*J)[2],
+                                   long long *const (*R)[2], int ns,
+                                   T *const *Jout, T *const *Rout,
+                                   hipStream_t st);
+template <typename T>
+hipError_t launch_logsph_filter(const Mesh &m, T *j, T *tmp, const double *A,
+                                hipStream_t st);
+template <typename T>
+hipError_t launch_logsph_ptc_boundary(const Mesh &m, const LsParams &lp,
+                                      T *J3, uint32_t *cell, uint32_t *flag,
+                                      uint64_t n, hipStream_t st);
+template <typename T>
+hipError_t launch_logsph_field_update(const Mesh &m, const LsCoef &cf,
+                                      const LsParams &lp,
+                                      const FieldSet<T> &f, double dt,
+                                      hipStream_t st);
+template <typename T>
+hipError_t launch_logsph_field_boundary(const Mesh &m, const LsParams &lp,
+                                        const FieldSet<T> &f, double omega,
+                                        hipStream_t st);
+template <typename R>
+hipError_t launch_logsph_inject(const Mesh &m, const LsParams &lp,
+                                const PtcView<R> &p, uint64_t number, int inj,
+                                double p1, double w, double q_e, uint64_t seed,
+                                uint32_t epoch, int Hs, uint64_t id_base,
+                                long long *surf, hipStream_t st);
+
+// ---- photons and pair creation (kern_rt.hip) ----------------------------
+// photon_data (src/data/particle_data.h:68-81); the mixed form recomputes
+// the photon energy from |p| and carries no id
+struct PhoF64 {
+  uint32_t *cell;
+  double *x1, *x2, *x3;
+  double *p1, *p2, *p3;
+  double *E, *w, *path;
+  uint32_t *flag;
+  uint64_t *id;
+};
+struct PhoF32 {
+  uint32_t *cell;
+  float *x1, *x2, *x3;
+  float *p1, *p2, *p3;
+  float *w, *path;
+  uint32_t *flag;
+};
+template <typename R>
+struct PhoView {
+  uint32_t *cell;
+  R *x1, *x2, *x3;
+  R *p1, *p2, *p3;
+  R *E, *w, *path;
+  uint32_t *flag;
+  uint64_t *id;
+};
+inline PhoView<double> view_of(const PhoF64 &p) {
+  return {p.cell, p.x1, p.x2, p.x3, p.p1, p.p2, p.p3, p.E, p.w, p.path, p.flag,
+          p.id};
+}
+inline PhoView<float> view_of(const PhoF32 &p) {
+  return {p.cell, p.x1, p.x2, p.x3, p.p1, p.p2, p.p3, nullptr, p.w, p.path,
+          p.flag, nullptr};
+}
+
+// model 0: the reference's default hooks; model 1: exponential free path
+struct RtParams {
+  int model;
+  double gamma_thr, photon_path, E_secondary, r_cutoff, e_frac;
+  uint64_t max_ptc, max_photon;
+};
+
+// scan scratch of the emit / pair passes
+struct RtWork {
+  uint32_t *flags = nullptr, *rank = nullptr;
+  unsigned long long *total = nullptr;
+  void *temp = nullptr;
+  size_t temp_bytes = 0;
+  uint64_t cap = 0;
+  hipError_t ensure(uint64_t n);
+  void release();
+};
+
+template <typename R>
+hipError_t rt_emit(const Mesh &m, const RtParams &rt, const PtcView<R> &p,
+                   uint64_t n, const PhoView<R> &ph, uint64_t ph_base,
+                   uint64_t seed, uint32_t epoch, uint64_t id_base,
+                   RtWork &w, unsigned long long *counters, hipStream_t st,
+                   uint64_t *emitted);
+template <typename R>
+hipError_t rt_pairs(const Mesh &m, const RtParams &rt, const PtcView<R> &p,
+                    uint64_t p_base, const PhoView<R> &ph, uint64_t nph,
+                    uint64_t id_base, RtWork &w, unsigned long long *counters,
+                    hipStream_t st, uint64_t *made);
+template <typename R>
+hipError_t rt_move_photons(const Mesh &m, int logsph, double r_cutoff,
+                           const PhoView<R> &ph, uint64_t n, double dt,
+                           hipStream_t st);
 
 // ---- loading, sort, diagnostics -------------------------------------
 struct LoadParams {

@@ -1,0 +1,935 @@
+// kern_logsph.hip -- the 2D log-spherical (ln r, theta) path:
+//   * Grid_LogSph::compute_coef (grid_log_sph.cpp:10-104) on the host, FP64;
+//   * fused push + move + deposit: user_push_2d_logsph<1>
+//     (user_push_2d_logsph.cuh:14-114: Vay, gravity, perpendicular cooling)
+//     then deposit_and_move_2d_log_sph<1> (ptc_updater_logsph.cu:134-282:
+//     straight line in Cartesian, back to (ln r, theta, phi), axis
+//     reflection, Esirkepov in index space), one thread per particle;
+//   * process_j (:284-299), filter_current_logsph (:616-666), the particle
+//     boundaries of apply_boundary (:812-846);
+//   * the integral-form Yee update, divergences and field boundaries of
+//     field_solver_logsph.cu:28-345;
+//   * measure_surface_density + inject_ptc (:301-395, 440-493).
+// R = double is the FP64 form: contraction off, the shared elementary
+// functions of apt_math.hpp, all nine interpolation taps in reference
+// order, two-limb fixed-point current -- bit-identical to the CPU
+// restatement.  R = float is the mixed form: FP32 arithmetic and library
+// functions, two-tap gathers, single-limb int64 current; the metric
+// coefficients stay FP64.
+#include <type_traits>
+#include <vector>
+
+#include "apt_common.hpp"
+#include "apt_f32.hpp"
+#include "apt_f64.hpp"
+#include "apt_kernels.hpp"
+#include "apt_math.hpp"
+
+namespace apt {
+
+constexpr double LS_PI = 3.14159265358979323846;
+
+// Quadmesh::pos(i, n, pos_in_cell) (quadmesh.h:144-150)
+__host__ __device__ inline double lpos(const Mesh &m, int d, int n, double x) {
+  return m.lower[d] + m.delta[d] * ((double)(n - m.guard[d]) + x);
+}
+__host__ __device__ inline double spos(const Mesh &m, int d, int n, int st) {
+  return lpos(m, d, n, st ? 1.0 : 0.5);
+}
+
+// ---------------------------------------------------------------------
+// metric coefficients (host, FP64, C library -- like the reference)
+// ---------------------------------------------------------------------
+static double g_l1(double r, double rs) {
+  double a = sqrt(1.0 - rs / r);
+  return r * a + 0.5 * rs * log(2.0 * r * (1.0 + a) - rs);
+}
+static double g_A2(double r, double rs) {
+  double a = sqrt(1.0 - rs / r);
+  return 0.25 * r * a * (2.0 * r + 3.0 * rs) +
+         0.375 * rs * rs * log(2.0 * r * (1.0 + a) - rs);
+}
+static double g_V3(double r, double rs) {
+  double a = sqrt(1.0 - rs / r);
+  return r * a * (8.0 * r * r + 10.0 * r * rs + 15.0 * rs * rs) / 24.0 +
+         0.3125 * rs * rs * rs * log(2.0 * r * (1.0 + a) - rs);
+}
+
+void logsph_coef_host(const Mesh &m, std::vector<double> (&c)[LS_NCOEF]) {
+  const double r_g = 0.0;  // grid_log_sph.cpp:14
+  const int d0 = m.dims[0];
+  const size_t n = (size_t)m.dims[0] * m.dims[1];
+  for (int k = 0; k < LS_NCOEF; k++) c[k].assign(n, 0.0);
+  const double D0 = m.delta[0], D1 = m.delta[1];
+  for (int j = 0; j < m.dims[1]; j++) {
+    double x2 = spos(m, 1, j, 0), x2s = spos(m, 1, j, 1);
+    for (int i = 0; i < d0; i++) {
+      size_t o = (size_t)i + (size_t)j * d0;
+      double x1 = spos(m, 0, i, 0), x1s = spos(m, 0, i, 1);
+      double r_plus = exp(x1 + D0), r = exp(x1), rs = exp(x1s),
+             rs_minus = exp(x1s - D0);
+      bool axis = fabs(x2s) < 1.0e-5 || fabs(x2s - LS_PI) < 1.0e-5;
+      c[LS_L1E][o] = g_l1(rs, r_g) - g_l1(rs_minus, r_g);
+      c[LS_L2E][o] = rs * D1;
+      c[LS_L3E][o] = rs * sin(x2s);
+      c[LS_L1B][o] = g_l1(r_plus, r_g) - g_l1(r, r_g);
+      c[LS_L2B][o] = r * D1;
+      c[LS_L3B][o] = r * sin(x2);
+      c[LS_A1E][o] = r * r * (cos(x2) - cos(x2 + D1));
+      if (axis) c[LS_A1E][o] = r * r * 2.0 * (1.0 - cos(0.5 * D1));
+      c[LS_A2E][o] = (g_A2(r_plus, r_g) - g_A2(r, r_g)) * sin(x2);
+      c[LS_A3E][o] = (g_A2(r_plus, r_g) - g_A2(r, r_g)) * D1;
+      c[LS_A1B][o] = rs * rs * (cos(x2s - D1) - cos(x2s));
+      c[LS_A2B][o] = (g_A2(rs, r_g) - g_A2(rs_minus, r_g)) * sin(x2s);
+      c[LS_A3B][o] = (g_A2(rs, r_g) - g_A2(rs_minus, r_g)) * D1;
+      c[LS_DV][o] = (g_V3(r_plus, r_g) - g_V3(r, r_g)) *
+                    (cos(x2) - cos(x2 + D1)) / (D0 * D1);
+      if (axis)
+        c[LS_DV][o] = (g_V3(r_plus, r_g) - g_V3(r, r_g)) * 2.0 *
+                      (1.0 - cos(0.5 * D1)) / (D0 * D1);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// elementary functions per form
+// ---------------------------------------------------------------------
+template <typename R>
+struct LsMath;
+template <>
+struct LsMath<double> {
+  static __device__ double exp(double x) { return apt_exp(x); }
+  static __device__ void sincos(double x, double &s, double &c) {
+    apt_sincos(x, s, c);
+  }
+  static __device__ double sin(double x) { return apt_sin(x); }
+  static __device__ double acos(double t) { return apt_acos(t); }
+  static __device__ double atan2(double y, double x) { return apt_atan2(y, x); }
+  static __device__ double log(double x) { return apt_log(x); }
+};
+template <>
+struct LsMath<float> {
+  static __device__ float exp(float x) { return expf(x); }
+  static __device__ void sincos(float x, float &s, float &c) {
+    sincosf(x, &s, &c);
+  }
+  static __device__ float sin(float x) { return sinf(x); }
+  static __device__ float acos(float t) {
+    return acosf(fminf(fmaxf(t, -1.0f), 1.0f));
+  }
+  static __device__ float atan2(float y, float x) { return atan2f(y, x); }
+  static __device__ float log(float x) { return logf(x); }
+};
+
+// beta_phi (field_solver_logsph.cu:16-20); alpha_gr == 1 (:22-26)
+template <typename R>
+__device__ inline R beta_phi(const LsParams &lp, R r, R theta) {
+  return (R)(-0.4) * (R)lp.compactness * (R)lp.omega * LsMath<R>::sin(theta) /
+         (r * r);
+}
+
+// current accumulation per form
+struct LsAccF64 {
+  long long *J[3][2];
+  long long *R[4][2];
+  int H;
+  __device__ void add(int c, uint64_t o, double v) const {
+    acc_add(J[c][0], J[c][1], o, v, H);
+  }
+  __device__ void add_rho(int s, uint64_t o, double v) const {
+    // species without a charge array (s >= num_species) are not recorded
+    if (s < 4 && R[s][0]) acc_add(R[s][0], R[s][1], o, v, H);
+  }
+};
+struct LsAccF32 {
+  long long *J[3];
+  long long *R[4];
+  float scale;
+  __device__ void add(int c, uint64_t o, float v) const {
+    acc_fx(J[c], o, v, scale);
+  }
+  __device__ void add_rho(int s, uint64_t o, float v) const {
+    if (s < 4 && R[s]) acc_fx(R[s], o, v, scale);
+  }
+};
+
+template <typename R>
+__device__ inline R cicR(R d) {
+  R a = (R)1 - fabs(d);
+  return a > (R)0 ? a : (R)0;
+}
+template <typename R>
+__device__ inline R center2dR(R sx0, R sx1, R sy0, R sy1) {
+  return ((R)2 * sx1 * sy1 + sx0 * sy1 + sx1 * sy0 + (R)2 * sx0 * sy0) *
+         ((R)1 / (R)6);
+}
+template <typename R>
+__device__ inline R movement2dR(R sx0, R sx1, R sy0, R sy1) {
+  return (sy1 - sy0) * (R)0.5 * (sx0 + sx1);
+}
+
+template <typename R>
+__device__ inline R gatherR(const R *__restrict__ f, const Mesh &m, int c1,
+                            int c2, R x1, R x2, int st) {
+  if constexpr (std::is_same<R, double>::value)
+    return gather2(f, m.dims[0], x1, x2, c1, c2, st);
+  else
+    return gatherf<2>(f, m, c1, c2, 0, x1, x2, 0.0f, st);
+}
+
+// ---------------------------------------------------------------------
+// push + move + deposit (one thread per particle)
+// ---------------------------------------------------------------------
+template <typename R, typename Acc>
+__global__ void __launch_bounds__(256)
+    k_logsph_push(Mesh m, Species sp, LsParams lp, R dt, int rho,
+                  PtcView<R> p, const R *__restrict__ E1f,
+                  const R *__restrict__ E2f, const R *__restrict__ E3f,
+                  const R *__restrict__ B1f, const R *__restrict__ B2f,
+                  const R *__restrict__ B3f, Acc acc,
+                  unsigned long long *__restrict__ counters, uint64_t n) {
+  using M = LsMath<R>;
+  uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  uint32_t c = p.cell[idx];
+  if (c == MAX_CELL) return;
+  const int d0 = m.dims[0];
+  int c1 = (int)(c % (uint32_t)d0), c2 = (int)(c / (uint32_t)d0);
+  if (!(c1 >= m.guard[0] && c1 < d0 - m.guard[0] && c2 >= m.guard[1] &&
+        c2 < m.dims[1] - m.guard[1])) {
+    p.cell[idx] = MAX_CELL;
+    return;
+  }
+  const uint32_t flag = p.flag[idx];
+  const int s = ptc_species(flag);
+  const R old_x1 = p.x1[idx], old_x2 = p.x2[idx], old_x3 = p.x3[idx];
+  R p1 = p.p1[idx], p2 = p.p2[idx], p3 = p.p3[idx];
+  if (p1 != p1 || p2 != p2 || p3 != p3) atomicAdd(&counters[CTR_NAN], 1ull);
+  const R r = M::exp((R)lpos(m, 0, c1, (double)old_x1));
+  const R q_over_m = (R)(dt * (R)0.5 * (R)sp.q[s] / (R)sp.m[s]);
+  R gamma = sqrt((R)1 + p1 * p1 + p2 * p2 + p3 * p3);
+  if (!(flag & FLAG_IGNORE_EM)) {
+    R E1 = gatherR(E1f, m, c1, c2, old_x1, old_x2, ST_E1) * q_over_m;
+    R E2 = gatherR(E2f, m, c1, c2, old_x1, old_x2, ST_E2) * q_over_m;
+    R E3 = gatherR(E3f, m, c1, c2, old_x1, old_x2, ST_E3) * q_over_m;
+    R B1 = gatherR(B1f, m, c1, c2, old_x1, old_x2, ST_B1) * q_over_m;
+    R B2 = gatherR(B2f, m, c1, c2, old_x1, old_x2, ST_B2) * q_over_m;
+    R B3 = gatherR(B3f, m, c1, c2, old_x1, old_x2, ST_B3) * q_over_m;
+    if constexpr (std::is_same<R, double>::value)
+      vay_push(p1, p2, p3, gamma, E1, E2, E3, B1, B2, B3);
+    else
+      vay_pushf(p1, p2, p3, gamma, E1, E2, E3, B1, B2, B3);
+    // gravity (gravity.cuh:13-26)
+    p1 -= dt * (R)lp.gravity * (R)sp.m[s] / (r * r * (R)fabs(sp.q[s]));
+    gamma = sqrt((R)1 + p1 * p1 + p2 * p2 + p3 * p3);
+    // sync_kill_perp (sync_cooling.cuh:41-68), leptons only
+    if (lp.rad_cooling_on && s != 2) {
+      B1 /= q_over_m;
+      B2 /= q_over_m;
+      B3 /= q_over_m;
+      R B_sqr = B1 * B1 + B2 * B2 + B3 * B3;
+      R pdotB = (p1 * B1 + p2 * B2 + p3 * B3);
+      R k = (R)lp.rad_cooling_coef;
+      R dp1 = -k * (p1 - B1 * pdotB / B_sqr);
+      R dp2 = -k * (p2 - B2 * pdotB / B_sqr);
+      R dp3 = -k * (p3 - B3 * pdotB / B_sqr);
+      R fB = sqrt(B_sqr) / (R)lp.B0;
+      p1 += dp1 * fB;
+      p2 += dp2 * fB;
+      p3 += dp3 * fB;
+      R pp = sqrt(p1 * p1 + p2 * p2 + p3 * p3);
+      gamma = sqrt((R)1 + pp * pp);
+    }
+    if (p.E) p.E[idx] = gamma;
+  } else if (p.E) {
+    gamma = p.E[idx];
+  }
+
+  // ---- move in Cartesian and back (ptc_updater_logsph.cu:163-230) ----
+  const R w = p.w[idx];
+  const R r1 = (R)lpos(m, 0, c1, (double)old_x1);
+  const R exp_r1 = M::exp(r1);
+  const R r2 = (R)lpos(m, 1, c2, (double)old_x2);
+  R v1 = p1 / gamma, v2 = p2 / gamma, v3 = p3 / gamma;
+  R v3_gr = v3 - beta_phi<R>(lp, exp_r1, r2);
+  R s2, cs2, s3, cs3;
+  M::sincos(r2, s2, cs2);
+  M::sincos(old_x3, s3, cs3);
+  R x = exp_r1 * s2 * cs3;
+  R y = exp_r1 * s2 * s3;
+  R z = exp_r1 * cs2;
+  {
+    R a = v1, b = v2, cc = v3_gr;
+    v1 = a * s2 * cs3 + b * cs2 * cs3 - cc * s3;
+    v2 = a * s2 * s3 + b * cs2 * s3 + cc * cs3;
+    v3_gr = a * cs2 - b * s2;
+  }
+  x += v1 * dt;
+  y += v2 * dt;
+  z += v3_gr * dt;
+  R r1p = sqrt(x * x + y * y + z * z);
+  // FP64: acos(z / r) as the reference; FP32: the well-conditioned
+  // atan2(sqrt(x^2 + y^2), z) -- acos loses sqrt(ulp) near the axis
+  R r2p;
+  if constexpr (std::is_same<R, double>::value)
+    r2p = M::acos(z / r1p);
+  else
+    r2p = M::atan2(sqrt(x * x + y * y), z);
+  const R exp_r1p = r1p;
+  r1p = M::log(r1p);
+  const R r3p = M::atan2(y, x);
+  {
+    R ps2, pc2, ps3, pc3;
+    M::sincos(r2p, ps2, pc2);
+    M::sincos(r3p, ps3, pc3);
+    R a = v1, b = v2, cc = v3_gr;
+    v1 = a * ps2 * pc3 + b * ps2 * ps3 + cc * pc2;
+    v2 = a * pc2 * pc3 + b * pc2 * ps3 - cc * ps2;
+    v3_gr = -a * ps3 + b * pc3;
+  }
+  R np1 = v1 * gamma, np2 = v2 * gamma;
+  R np3 = (v3_gr + beta_phi<R>(lp, exp_r1p, r2p)) * gamma;
+  R new_x1 = old_x1 + (r1p - r1) / (R)m.delta[0];
+  R new_x2 = old_x2 + (r2p - r2) / (R)m.delta[1];
+  int dc1 = (int)floor(new_x1);
+  int dc2 = (int)floor(new_x2);
+  if (dc1 < -1 || dc1 > 1 || dc2 < -1 || dc2 > 1) {
+    atomicAdd(&counters[CTR_MULTICELL], 1ull);
+    p.cell[idx] = MAX_CELL;
+    return;
+  }
+  new_x1 -= (R)dc1;
+  new_x2 -= (R)dc2;
+  if (lpos(m, 1, c2 + dc2, (double)new_x2) < 0.0) {
+    dc2 += 1;
+    new_x2 = (R)1 - new_x2;
+    np2 = -np2;
+    np3 = -np3;
+  }
+  if (lpos(m, 1, c2 + dc2, (double)new_x2) >= LS_PI) {
+    dc2 -= 1;
+    new_x2 = (R)1 - new_x2;
+    np2 = -np2;
+    np3 = -np3;
+  }
+  if constexpr (!std::is_same<R, double>::value) {
+    // keep FP32 offsets inside [0, 1)
+    if (new_x1 >= 1.0f) new_x1 = 0x1.fffffep-1f;
+    if (new_x2 >= 1.0f) new_x2 = 0x1.fffffep-1f;
+  }
+  p.p1[idx] = np1;
+  p.p2[idx] = np2;
+  p.p3[idx] = np3;
+  p.cell[idx] = (uint32_t)((c1 + dc1) + (c2 + dc2) * d0);
+  p.x1[idx] = new_x1;
+  p.x2[idx] = new_x2;
+  p.x3[idx] = r3p;
+
+  if (flag & FLAG_IGNORE_CURRENT) return;
+  const R weight = -(R)sp.q[s] * w;
+  const int j0 = (dc2 == -1 ? -2 : -1), j1 = (dc2 == 1 ? 1 : 0);
+  const int i0 = (dc1 == -1 ? -2 : -1), i1 = (dc1 == 1 ? 1 : 0);
+  R djy[3] = {(R)0, (R)0, (R)0};
+  for (int j = j0; j <= j1; j++) {
+    R sy0 = cicR<R>(-old_x2 + (R)j + (R)1);
+    R sy1 = cicR<R>(-new_x2 + (R)(j + 1 - dc2));
+    R djx = (R)0;
+    for (int i = i0; i <= i1; i++) {
+      R sx0 = cicR<R>(-old_x1 + (R)i + (R)1);
+      R sx1 = cicR<R>(-new_x1 + (R)(i + 1 - dc1));
+      uint64_t o = (uint64_t)(i + c1) + (uint64_t)(j + c2) * d0;
+      djx += movement2dR<R>(sy0, sy1, sx0, sx1);
+      acc.add(0, o + 1, weight * djx);
+      djy[i - i0] += movement2dR<R>(sx0, sx1, sy0, sy1);
+      acc.add(1, o + d0, weight * djy[i - i0]);
+      R val2 = center2dR<R>(sx0, sx1, sy0, sy1);
+      acc.add(2, o, -weight * v3_gr * val2);
+      if (rho) {
+        R s1 = sx1 * sy1;
+        acc.add_rho(s, o, -weight * s1);
+      }
+    }
+  }
+}
+
+// process_j (ptc_updater_logsph.cu:284-299): J1,J2 *= (D0 D1/dt)/A,
+// J3 and Rho /= dV over every cell; clears the accumulators.
+template <typename T>
+__global__ void k_logsph_process_j(LsCoef cf, double wj, int H, int limbs,
+                                   long long *j0h, long long *j0l,
+                                   long long *j1h, long long *j1l,
+                                   long long *j2h, long long *j2l, T *J1,
+                                   T *J2, T *J3, uint64_t n) {
+  uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n) return;
+  double a, b, c;
+  if (limbs == 2) {
+    a = fx_value(j0h[o], j0l[o], H);
+    b = fx_value(j1h[o], j1l[o], H);
+    c = fx_value(j2h[o], j2l[o], H);
+    j0l[o] = 0;
+    j1l[o] = 0;
+    j2l[o] = 0;
+  } else {
+    a = ldexp((double)j0h[o], -H);
+    b = ldexp((double)j1h[o], -H);
+    c = ldexp((double)j2h[o], -H);
+  }
+  j0h[o] = 0;
+  j1h[o] = 0;
+  j2h[o] = 0;
+  J1[o] = (T)(a * (wj / cf.a[LS_A1E][o]));
+  J2[o] = (T)(b * (wj / cf.a[LS_A2E][o]));
+  J3[o] = (T)(c / cf.a[LS_DV][o]);
+}
+
+template <typename T>
+__global__ void k_logsph_process_rho(LsCoef cf, int H, int limbs,
+                                     long long *hi, long long *lo, T *Rho,
+                                     uint64_t n) {
+  uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n) return;
+  double v = limbs == 2 ? fx_value(hi[o], lo[o], H) : ldexp((double)hi[o], -H);
+  hi[o] = 0;
+  if (limbs == 2) lo[o] = 0;
+  Rho[o] = (T)(v / cf.a[LS_DV][o]);
+}
+
+// filter_current_logsph (ptc_updater_logsph.cu:616-666), one rank: all
+// four boundary offsets collapse at the domain edge.
+template <typename T>
+__global__ void k_logsph_filter(Mesh m, const T *__restrict__ j,
+                                T *__restrict__ tmp,
+                                const double *__restrict__ A) {
+  const int N0 = m.dims[0] - 2 * m.guard[0], N1 = m.dims[1] - 2 * m.guard[1];
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)N0 * N1) return;
+  const int d0 = m.dims[0];
+  int n1 = m.guard[0] + (int)(t % N0), n2 = m.guard[1] + (int)(t / N0);
+  uint64_t o = (uint64_t)n1 + (uint64_t)n2 * d0;
+  uint64_t drp = n1 == m.dims[0] - m.guard[0] - 1 ? 0 : 1;
+  uint64_t drm = n1 == m.guard[0] ? 0 : 1;
+  uint64_t dtp = n2 == m.dims[1] - m.guard[1] - 1 ? 0 : (uint64_t)d0;
+  uint64_t dtm = n2 == m.guard[1] ? 0 : (uint64_t)d0;
+  T r = (T)0.25 * j[o] * (T)A[o];
+  r += (T)0.125 * j[o + drp] * (T)A[o + drp];
+  r += (T)0.125 * j[o - drm] * (T)A[o - drm];
+  r += (T)0.125 * j[o + dtp] * (T)A[o + dtp];
+  r += (T)0.125 * j[o - dtm] * (T)A[o - dtm];
+  r += (T)0.0625 * j[o + drp + dtp] * (T)A[o + drp + dtp];
+  r += (T)0.0625 * j[o - drm + dtp] * (T)A[o - drm + dtp];
+  r += (T)0.0625 * j[o + drp - dtm] * (T)A[o + drp - dtm];
+  r += (T)0.0625 * j[o - drm - dtm] * (T)A[o - drm - dtm];
+  r /= (T)A[o];
+  tmp[o] = r;
+}
+
+// axis_rho_lower/upper (ptc_updater_logsph.cu:414-438)
+template <typename T>
+__global__ void k_logsph_axis_j3(Mesh m, T *J3) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.dims[0]) return;
+  const int d0 = m.dims[0], g = m.guard[1], jl = m.dims[1] - m.guard[1];
+  J3[i + (uint64_t)(g - 1) * d0] = (T)0;
+  J3[i + (uint64_t)g * d0] = (T)0;
+  J3[i + (uint64_t)jl * d0] = (T)0;
+  J3[i + (uint64_t)(jl - 1) * d0] = (T)0;
+}
+
+// ptc_outflow (ptc_updater_logsph.cu:397-412)
+__global__ void k_logsph_outflow_flag(Mesh m, int damping_length,
+                                      const uint32_t *__restrict__ cell,
+                                      uint32_t *__restrict__ flag,
+                                      uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c = cell[i];
+  if (c == MAX_CELL) return;
+  uint32_t f = flag[i];
+  if (f & FLAG_IGNORE_EM) return;
+  int c1 = (int)(c % (uint32_t)m.dims[0]);
+  if (c1 > m.dims[0] - damping_length + 2) flag[i] = f | FLAG_IGNORE_EM;
+}
+
+// ---------------------------------------------------------------------
+// field solver (field_solver_logsph.cu:28-345), alpha_gr = 1
+// ---------------------------------------------------------------------
+#define F2(a, i, j) (a)[(uint64_t)(i) + (uint64_t)(j) * d0]
+#define C2(k, i, j) ((T)cf.a[k][(uint64_t)(i) + (uint64_t)(j) * d0])
+
+template <typename T>
+__device__ inline void bulk_ij(const Mesh &m, uint64_t t, int &i, int &j) {
+  const int N0 = m.dims[0] - 2 * m.guard[0];
+  i = m.guard[0] + (int)(t % N0);
+  j = m.guard[1] + (int)(t / N0);
+}
+
+template <typename T>
+__global__ void k_logsph_update_b(Mesh m, LsCoef cf, LsParams lp,
+                                  FieldSet<T> f, T dt, uint64_t nb) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb) return;
+  int i, j;
+  bulk_ij<T>(m, t, i, j);
+  const int d0 = m.dims[0];
+  const uint64_t o = (uint64_t)i + (uint64_t)j * d0;
+  const T *E1 = f.E[0], *E2 = f.E[1], *E3 = f.E[2];
+  const T *G1 = f.Ebg[0], *G2 = f.Ebg[1], *G3 = f.Ebg[2];
+  const T *H1 = f.Bbg[0], *H2 = f.Bbg[1];
+  const double r1 = apt_exp(spos(m, 0, i, 1));
+  const double r0 = apt_exp(spos(m, 0, i - 1, 1));
+  const double r = apt_exp(spos(m, 0, i, 0));
+  const double th0 = spos(m, 1, j, 0);
+  const T b_r1 = (T)beta_phi<double>(lp, r1, th0);
+  const T b_r0 = (T)beta_phi<double>(lp, r0, th0);
+  const T b_m = (T)beta_phi<double>(lp, r, spos(m, 1, j - 1, 1));
+  const T b_c = (T)beta_phi<double>(lp, r, spos(m, 1, j, 1));
+  f.B[0][o] += -dt *
+               ((F2(E3, i, j) - F2(G3, i, j)) * C2(LS_L3E, i, j) -
+                (F2(E3, i, j - 1) - F2(G3, i, j - 1)) * C2(LS_L3E, i, j - 1)) /
+               C2(LS_A1B, i, j);
+  f.B[1][o] += -dt *
+               ((F2(E3, i - 1, j) - F2(G3, i - 1, j)) * C2(LS_L3E, i - 1, j) -
+                (F2(E3, i, j) - F2(G3, i, j)) * C2(LS_L3E, i, j)) /
+               C2(LS_A2B, i, j);
+  f.B[2][o] += -dt * ((((F2(E2, i, j) - F2(G2, i, j)) + F2(H1, i, j) * b_r1) *
+                           C2(LS_L2E, i, j) -
+                       ((F2(E2, i - 1, j) - F2(G2, i - 1, j)) +
+                        F2(H1, i - 1, j) * b_r0) *
+                           C2(LS_L2E, i - 1, j) +
+                       ((F2(E1, i, j - 1) - F2(G1, i, j - 1)) -
+                        F2(H2, i, j - 1) * b_m) *
+                           C2(LS_L1E, i, j - 1) -
+                       ((F2(E1, i, j) - F2(G1, i, j)) - F2(H2, i, j) * b_c) *
+                           C2(LS_L1E, i, j)) /
+                      C2(LS_A3B, i, j));
+}
+
+template <typename T>
+__global__ void k_logsph_update_e(Mesh m, LsCoef cf, FieldSet<T> f, T dt,
+                                  uint64_t nb) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb) return;
+  int i, j;
+  bulk_ij<T>(m, t, i, j);
+  const int d0 = m.dims[0];
+  const uint64_t o = (uint64_t)i + (uint64_t)j * d0;
+  const T *B1 = f.B[0], *B2 = f.B[1], *B3 = f.B[2];
+  const T *H1 = f.Bbg[0], *H2 = f.Bbg[1], *H3 = f.Bbg[2];
+  const T D1 = (T)m.delta[1];
+  if (fabs(spos(m, 1, j, 1) - LS_PI) < 1.0e-5) {
+    const T r0 = (T)apt_exp(spos(m, 0, i, 0));
+    f.E[0][o] += dt * ((T)(-4.0) * (B3[o] - H3[o]) / (D1 * r0) - f.J[0][o]);
+  } else {
+    f.E[0][o] += dt * (((F2(B3, i, j + 1) - F2(H3, i, j + 1)) *
+                            C2(LS_L3B, i, j + 1) -
+                        (F2(B3, i, j) - F2(H3, i, j)) * C2(LS_L3B, i, j)) /
+                           C2(LS_A1E, i, j) -
+                       f.J[0][o]);
+  }
+  f.E[1][o] += dt * (((F2(B3, i, j) - F2(H3, i, j)) * C2(LS_L3B, i, j) -
+                      (F2(B3, i + 1, j) - F2(H3, i + 1, j)) *
+                          C2(LS_L3B, i + 1, j)) /
+                         C2(LS_A2E, i, j) -
+                     f.J[1][o]);
+  f.E[2][o] += dt * (((F2(B2, i + 1, j) - F2(H2, i + 1, j)) *
+                          C2(LS_L2B, i + 1, j) -
+                      (F2(B2, i, j) - F2(H2, i, j)) * C2(LS_L2B, i, j) +
+                      (F2(B1, i, j) - F2(H1, i, j)) * C2(LS_L1B, i, j) -
+                      (F2(B1, i, j + 1) - F2(H1, i, j + 1)) *
+                          C2(LS_L1B, i, j + 1)) /
+                         C2(LS_A3E, i, j) -
+                     f.J[2][o]);
+}
+
+// the theta = 0 rows written after the bulk updates
+// (field_solver_logsph.cu:100-113 and :177-184)
+template <typename T>
+__global__ void k_logsph_axis_e(Mesh m, FieldSet<T> f, T dt) {
+  int i = m.guard[0] + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.dims[0] - m.guard[0]) return;
+  const int d0 = m.dims[0], g = m.guard[1];
+  const uint64_t o = (uint64_t)i + (uint64_t)(g - 1) * d0;
+  const T r0 = (T)apt_exp(spos(m, 0, i, 0));
+  const T D1 = (T)m.delta[1];
+  f.E[2][o] = f.Ebg[2][o];
+  f.E[0][o] += dt * ((T)4.0 * (F2(f.B[2], i, g) - F2(f.Bbg[2], i, g)) /
+                         (D1 * r0) -
+                     f.J[0][o]);
+}
+template <typename T>
+__global__ void k_logsph_axis_b(Mesh m, FieldSet<T> f) {
+  int i = m.guard[0] + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.dims[0] - m.guard[0]) return;
+  const uint64_t o = (uint64_t)i + (uint64_t)(m.guard[1] - 1) * m.dims[0];
+  f.B[1][o] = f.Bbg[1][o];
+}
+
+template <typename T>
+__global__ void k_logsph_divs(Mesh m, LsCoef cf, FieldSet<T> f, uint64_t nb) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb) return;
+  int i, j;
+  bulk_ij<T>(m, t, i, j);
+  const int d0 = m.dims[0];
+  const uint64_t o = (uint64_t)i + (uint64_t)j * d0;
+  const T *E1 = f.E[0], *E2 = f.E[1], *B1 = f.B[0], *B2 = f.B[1];
+  const T *G1 = f.Ebg[0], *G2 = f.Ebg[1], *H1 = f.Bbg[0], *H2 = f.Bbg[1];
+  const T den = C2(LS_DV, i, j) * (T)m.delta[0] * (T)m.delta[1];
+  const double size1 = m.delta[1] * (double)(m.dims[1] - 2 * m.guard[1]);
+  if (spos(m, 0, i, 1) > m.delta[0]) {
+    if (fabs(spos(m, 1, j, 1) - size1 + m.lower[1]) < 1.0e-5)
+      f.divE[o] = ((F2(E1, i + 1, j) - F2(G1, i + 1, j)) * C2(LS_A1E, i + 1, j) -
+                   (F2(E1, i, j) - F2(G1, i, j)) * C2(LS_A1E, i, j) -
+                   (T)2.0 * (F2(E2, i, j) - F2(G2, i, j)) * C2(LS_A2E, i, j)) /
+                  den;
+    else
+      f.divE[o] = ((F2(E1, i + 1, j) - F2(G1, i + 1, j)) * C2(LS_A1E, i + 1, j) -
+                   (F2(E1, i, j) - F2(G1, i, j)) * C2(LS_A1E, i, j) +
+                   (F2(E2, i, j + 1) - F2(G2, i, j + 1)) * C2(LS_A2E, i, j + 1) -
+                   (F2(E2, i, j) - F2(G2, i, j)) * C2(LS_A2E, i, j)) /
+                  den;
+  }
+  f.divB[o] = ((F2(B1, i, j) - F2(H1, i, j)) * C2(LS_A1B, i, j) -
+               (F2(B1, i - 1, j) - F2(H1, i - 1, j)) * C2(LS_A1B, i - 1, j) +
+               (F2(B2, i, j) - F2(H2, i, j)) * C2(LS_A2B, i, j) -
+               (F2(B2, i, j - 1) - F2(H2, i, j - 1)) * C2(LS_A2B, i, j - 1)) /
+              den;
+}
+
+template <typename T>
+__global__ void k_logsph_axis_dive(Mesh m, LsCoef cf, FieldSet<T> f) {
+  int i = m.guard[0] + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.dims[0] - m.guard[0]) return;
+  const int d0 = m.dims[0], g = m.guard[1];
+  const uint64_t o = (uint64_t)i + (uint64_t)(g - 1) * d0;
+  const T *E1 = f.E[0], *E2 = f.E[1], *G1 = f.Ebg[0], *G2 = f.Ebg[1];
+  f.divE[o] = ((F2(E1, i + 1, g - 1) - F2(G1, i + 1, g - 1)) *
+                   C2(LS_A1E, i + 1, g - 1) -
+               (F2(E1, i, g - 1) - F2(G1, i, g - 1)) * C2(LS_A1E, i, g - 1) +
+               (T)2.0 * (F2(E2, i, g) - F2(G2, i, g)) * C2(LS_A2E, i, g)) /
+              (C2(LS_DV, i, g - 1) * (T)m.delta[0] * (T)m.delta[1]);
+}
+
+// stellar_boundary (field_solver_logsph.cu:252-275): i < guard0, every j
+template <typename T>
+__global__ void k_logsph_stellar(Mesh m, LsParams lp, FieldSet<T> f,
+                                 double omega) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m.dims[1] * m.guard[0]) return;
+  int j = t / m.guard[0], i = t % m.guard[0];
+  const uint64_t o = (uint64_t)i + (uint64_t)j * m.dims[0];
+  const double theta_s = spos(m, 1, j, 1), theta = spos(m, 1, j, 0);
+  const double r_s = apt_exp(spos(m, 0, i, 1)), r = apt_exp(spos(m, 0, i, 0));
+  const double omega_LT = 0.4 * omega * lp.compactness;
+  f.B[0][o] = f.Bbg[0][o];
+  f.B[2][o] = f.Bbg[2][o];
+  if constexpr (std::is_same<T, double>::value) {
+    f.E[1][o] = -(omega - omega_LT) * apt_sin(theta) * f.Bbg[0][o] / r_s / r_s +
+                f.Ebg[1][o];
+    f.E[0][o] = (omega - omega_LT) * apt_sin(theta_s) * f.Bbg[1][o] / r / r +
+                f.Ebg[0][o];
+  } else {
+    f.E[1][o] = (T)(-(omega - omega_LT) * apt_sin(theta) * (double)f.Bbg[0][o] /
+                    r_s / r_s) +
+                f.Ebg[1][o];
+    f.E[0][o] = (T)((omega - omega_LT) * apt_sin(theta_s) *
+                    (double)f.Bbg[1][o] / r / r) +
+                f.Ebg[0][o];
+  }
+  f.B[1][o] = f.Bbg[1][o];
+  f.B[2][o] = f.Bbg[2][o];
+}
+
+// outflow_boundary_sph (field_solver_logsph.cu:322-345)
+template <typename T>
+__global__ void k_logsph_damping(Mesh m, LsParams lp, FieldSet<T> f) {
+  const int L = lp.damping_length;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m.dims[1] * L) return;
+  int j = t / L, i = t % L;
+  const uint64_t o = (uint64_t)(m.dims[0] - L + i) + (uint64_t)j * m.dims[0];
+  const double x = (double)i / L;
+  const T lambda = (T)(1.0 - lp.damping_coef * (x * x));
+  for (int c = 0; c < 3; c++)
+    f.E[c][o] = lambda * (f.E[c][o] - f.Ebg[c][o]) + f.Ebg[c][o];
+  f.B[2][o] = lambda * (f.B[2][o] - f.Bbg[2][o]) + f.Bbg[2][o];
+}
+
+// axis_boundary_lower / upper (field_solver_logsph.cu:277-320)
+template <typename T>
+__global__ void k_logsph_axis_fields(Mesh m, FieldSet<T> f, int upper) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.dims[0]) return;
+  const int d0 = m.dims[0];
+  T *E2 = f.E[1], *E3 = f.E[2], *B1 = f.B[0], *B2 = f.B[1], *B3 = f.B[2];
+  const T *G2 = f.Ebg[1], *G3 = f.Ebg[2], *H1 = f.Bbg[0], *H2 = f.Bbg[1],
+          *H3 = f.Bbg[2];
+  if (!upper) {
+    const int g = m.guard[1];
+    uint64_t a = (uint64_t)i + (uint64_t)(g - 1) * d0, b = (uint64_t)i + (uint64_t)g * d0;
+    E3[a] = G3[a];
+    E2[a] = -(E2[b] - G2[b]) + G2[a];
+    B3[a] = H3[a];
+    B3[b] = H3[b];
+    B2[a] = H2[a];
+    B1[a] = (B1[b] - H1[b]) + H1[a];
+  } else {
+    const int jl = m.dims[1] - m.guard[1];
+    uint64_t a = (uint64_t)i + (uint64_t)(jl - 1) * d0, b = (uint64_t)i + (uint64_t)jl * d0;
+    E3[a] = G3[a];
+    E2[b] = -(E2[a] - G2[a]) + G2[b];
+    B3[b] = H3[b];
+    B3[a] = H3[a];
+    B2[a] = H2[a];
+    B1[b] = (B1[a] - H1[a]) + H1[b];
+  }
+}
+#undef F2
+#undef C2
+
+// ---------------------------------------------------------------------
+// injection (ptc_updater_logsph.cu:301-395, 440-493)
+// ---------------------------------------------------------------------
+template <typename R>
+__global__ void k_logsph_surface(Mesh m, const uint32_t *__restrict__ cell,
+                                 const uint32_t *__restrict__ flag,
+                                 const R *__restrict__ w, uint64_t n,
+                                 long long *surf, int Hs) {
+  uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  uint32_t c = cell[idx];
+  if (c == MAX_CELL) return;
+  const int d0 = m.dims[0], g0 = m.guard[0], g1 = m.guard[1];
+  const int N1 = m.dims[1] - 2 * g1;
+  int c1 = (int)(c % (uint32_t)d0), c2 = (int)(c / (uint32_t)d0);
+  const int inject_cell = g0 + 2, sum_cells = 3;
+  if (!(c1 >= inject_cell - 1 && c1 <= inject_cell - 1 + sum_cells)) return;
+  int s = ptc_species(flag[idx]);
+  if (s != 0 && s != 2) return;
+  long long *hi = surf + (s == 0 ? 0 : 2 * N1), *lo = hi + N1;
+  const double wp = (double)w[idx];
+  const double coef[5] = {1.0, 4.0, 6.0, 4.0, 1.0};
+  for (int t = 0; t < 5; t++) {
+    int k = c2 - g1 - 2 + t;
+    k = k < 0 ? 0 : (k > N1 - 1 ? N1 - 1 : k);
+    acc_add(hi, lo, (uint64_t)k, coef[t] * wp / (double)sum_cells / 16.0, Hs);
+  }
+}
+
+template <typename R>
+__global__ void k_logsph_inject(Mesh m, LsParams lp, PtcView<R> p,
+                                uint64_t base, int inj, double p1, double w,
+                                double q_e, uint64_t seed, uint32_t epoch,
+                                int Hs, uint64_t id_base,
+                                const long long *__restrict__ surf) {
+  const int g0 = m.guard[0], g1 = m.guard[1], d0 = m.dims[0];
+  const int N1 = m.dims[1] - 2 * g1;
+  const int rows = N1 - 2;
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)rows * inj) return;
+  const int i = g1 + 1 + (int)(t / inj), nn = (int)(t % inj);
+  const uint64_t off = base + (uint64_t)(i - g1 - 1) * inj * 2 + 2 * (uint64_t)nn;
+  const long long *seh = surf, *sel = surf + N1, *sph = surf + 2 * N1,
+                  *spl = surf + 3 * N1;
+  const double de = fx_value(seh[i - g1], sel[i - g1], Hs);
+  const double dp = fx_value(sph[i - g1], spl[i - g1], Hs);
+  const double dens = de > dp ? de : dp;
+  const double sgate = (1.0 / m.delta[1]) * (1.0 / m.delta[1]);
+  const double sin_theta = apt_sin(lpos(m, 1, i, 0.5));
+  const bool gated = q_e * dens > 1.5 * sgate * sin_theta;
+  for (int s = 0; s < 2; s++) {
+    p.cell[off + s] = MAX_CELL;
+    if (p.id) p.id[off + s] = id_base + (off + s - base);
+  }
+  if (gated) return;
+  double u0, u1;
+  uniform2(seed, ((uint64_t)epoch << 32) | (uint32_t)i, (uint32_t)nn, 4, u0, u1);
+  const double x2 = 1.0 - u0;
+  const double theta = lpos(m, 1, i, x2);
+  double st, ct;
+  apt_sincos(theta, st, ct);
+  const double vphi = 0.0;
+  const double w_ptc = w * st;
+  const double gamma = sqrt(1.0 + p1 * p1 + vphi * vphi);
+  const double sg = (double)((0.0 < ct) - (ct < 0.0));
+  const uint32_t fl = FLAG_PRIMARY | (u1 < lp.track_percent ? FLAG_TRACKED : 0u);
+  for (int s = 0; s < 2; s++) {
+    const uint64_t o = off + s;
+    R x2r = (R)x2;
+    if constexpr (!std::is_same<R, double>::value)
+      if (x2r >= 1.0f) x2r = 0x1.fffffep-1f;
+    p.x1[o] = (R)0.5;
+    p.x2[o] = x2r;
+    p.x3[o] = (R)0;
+    p.p1[o] = (R)(p1 * 2.0 * fabs(ct));
+    p.p2[o] = (R)(p1 * st * sg);
+    p.p3[o] = (R)vphi;
+    if (p.E) p.E[o] = (R)gamma;
+    p.w[o] = (R)w_ptc;
+    const uint32_t type = s == 0 ? 0u : (lp.inject_ions ? 2u : 1u);
+    p.flag[o] = fl | (type << 29);
+    p.cell[o] = (uint32_t)(g0 + 2 + i * d0);
+  }
+}
+
+// ---------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------
+static inline dim3 grid1(uint64_t n, int b = 256) {
+  return dim3((unsigned)((n + b - 1) / b));
+}
+
+hipError_t launch_logsph_push_f64(const Mesh &m, const Species &sp,
+                                  const LsParams &lp, double dt, int H,
+                                  int rho, const PtcF64 &p,
+                                  const FieldsF64 &f, const AccF64 &a,
+                                  unsigned long long *counters, uint64_t n,
+                                  hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  LsAccF64 acc;
+  for (int c = 0; c < 3; c++)
+    for (int l = 0; l < 2; l++) acc.J[c][l] = a.J[c][l];
+  for (int s = 0; s < 4; s++)
+    for (int l = 0; l < 2; l++) acc.R[s][l] = a.R[s][l];
+  acc.H = H;
+  k_logsph_push<double, LsAccF64><<<grid1(n), 256, 0, st>>>(
+      m, sp, lp, dt, rho, view_of(p), f.E[0], f.E[1], f.E[2], f.B[0], f.B[1],
+      f.B[2], acc, counters, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_logsph_push_f32(const Mesh &m, const Species &sp,
+                                  const LsParams &lp, float dt, int S,
+                                  const PtcF32 &p, const FieldsF32 &f,
+                                  const AccMixed &a,
+                                  unsigned long long *counters, uint64_t n,
+                                  hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  LsAccF32 acc;
+  for (int c = 0; c < 3; c++) acc.J[c] = a.J[c];
+  for (int s = 0; s < 4; s++) acc.R[s] = a.R[s];
+  acc.scale = ldexpf(1.0f, S);
+  k_logsph_push<float, LsAccF32><<<grid1(n), 256, 0, st>>>(
+      m, sp, lp, dt, a.rho, view_of(p), f.E[0], f.E[1], f.E[2], f.B[0],
+      f.B[1], f.B[2], acc, counters, n);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_logsph_process_j(const Mesh &m, const LsCoef &cf, double dt,
+                                   int H, int limbs, long long *const (*J)[2],
+                                   long long *const (*Rr)[2], int ns, T *const *Jout,
+                                   T *const *Rout, hipStream_t st) {
+  const uint64_t n = m.size();
+  const double wj = m.delta[0] * m.delta[1] / dt;
+  k_logsph_process_j<T><<<grid1(n), 256, 0, st>>>(
+      cf, wj, H, limbs, J[0][0], J[0][1], J[1][0], J[1][1], J[2][0], J[2][1],
+      Jout[0], Jout[1], Jout[2], n);
+  for (int s = 0; s < ns; s++)
+    k_logsph_process_rho<T><<<grid1(n), 256, 0, st>>>(cf, H, limbs, Rr[s][0],
+                                                      Rr[s][1], Rout[s], n);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_logsph_filter(const Mesh &m, T *j, T *tmp, const double *A,
+                                hipStream_t st) {
+  const uint64_t nb = (uint64_t)(m.dims[0] - 2 * m.guard[0]) *
+                      (m.dims[1] - 2 * m.guard[1]);
+  k_logsph_filter<T><<<grid1(nb), 256, 0, st>>>(m, j, tmp, A);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // whole-array copy back (ptc_updater_logsph.cu:752): guards take tmp's
+  return hipMemcpyAsync(j, tmp, m.size() * sizeof(T), hipMemcpyDeviceToDevice,
+                        st);
+}
+
+template <typename T>
+hipError_t launch_logsph_ptc_boundary(const Mesh &m, const LsParams &lp,
+                                      T *J3, uint32_t *cell, uint32_t *flag,
+                                      uint64_t n, hipStream_t st) {
+  hipError_t e = launch_erase_guard(m, cell, n, st);
+  if (e != hipSuccess) return e;
+  k_logsph_axis_j3<T><<<grid1(m.dims[0]), 256, 0, st>>>(m, J3);
+  if (n > 0)
+    k_logsph_outflow_flag<<<grid1(n), 256, 0, st>>>(m, lp.damping_length, cell,
+                                                    flag, n);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_logsph_field_update(const Mesh &m, const LsCoef &cf,
+                                      const LsParams &lp, const FieldSet<T> &f,
+                                      double dt, hipStream_t st) {
+  const uint64_t nb = (uint64_t)(m.dims[0] - 2 * m.guard[0]) *
+                      (m.dims[1] - 2 * m.guard[1]);
+  const int nr = m.dims[0] - 2 * m.guard[0];
+  k_logsph_update_b<T><<<grid1(nb), 256, 0, st>>>(m, cf, lp, f, (T)dt, nb);
+  k_logsph_axis_b<T><<<grid1(nr), 256, 0, st>>>(m, f);
+  k_logsph_update_e<T><<<grid1(nb), 256, 0, st>>>(m, cf, f, (T)dt, nb);
+  k_logsph_axis_e<T><<<grid1(nr), 256, 0, st>>>(m, f, (T)dt);
+  k_logsph_divs<T><<<grid1(nb), 256, 0, st>>>(m, cf, f, nb);
+  k_logsph_axis_dive<T><<<grid1(nr), 256, 0, st>>>(m, cf, f);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_logsph_field_boundary(const Mesh &m, const LsParams &lp,
+                                        const FieldSet<T> &f, double omega,
+                                        hipStream_t st) {
+  if (m.guard[0] > 0)
+    k_logsph_stellar<T><<<grid1((uint64_t)m.dims[1] * m.guard[0]), 256, 0, st>>>(
+        m, lp, f, omega);
+  if (lp.damping_length > 0)
+    k_logsph_damping<T><<<grid1((uint64_t)m.dims[1] * lp.damping_length), 256, 0,
+                          st>>>(m, lp, f);
+  k_logsph_axis_fields<T><<<grid1(m.dims[0]), 256, 0, st>>>(m, f, 0);
+  k_logsph_axis_fields<T><<<grid1(m.dims[0]), 256, 0, st>>>(m, f, 1);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_logsph_inject(const Mesh &m, const LsParams &lp,
+                                const PtcView<R> &p, uint64_t number, int inj,
+                                double p1, double w, double q_e, uint64_t seed,
+                                uint32_t epoch, int Hs, uint64_t id_base,
+                                long long *surf, hipStream_t st) {
+  const int N1 = m.dims[1] - 2 * m.guard[1];
+  hipError_t e = launch_fill<long long>(surf, 0, 4 * (uint64_t)N1, st);
+  if (e != hipSuccess) return e;
+  if (number > 0)
+    k_logsph_surface<R><<<grid1(number), 256, 0, st>>>(m, p.cell, p.flag, p.w,
+                                                       number, surf, Hs);
+  const uint64_t nt = (uint64_t)(N1 - 2) * inj;
+  if (nt > 0)
+    k_logsph_inject<R><<<grid1(nt), 256, 0, st>>>(m, lp, p, number, inj, p1, w,
+                                                  q_e, seed, epoch, Hs, id_base,
+                                                  surf);
+  return hipGetLastError();
+}
+
+#define INST(T)                                                                \
+  template hipError_t launch_logsph_process_j<T>(                              \
+      const Mesh &, const LsCoef &, double, int, int, long long *const (*)[2], \
+      long long *const (*)[2], int, T *const *, T *const *, hipStream_t);     \
+  template hipError_t launch_logsph_filter<T>(const Mesh &, T *, T *,          \
+                                              const double *, hipStream_t);    \
+  template hipError_t launch_logsph_ptc_boundary<T>(                           \
+      const Mesh &, const LsParams &, T *, uint32_t *, uint32_t *, uint64_t,   \
+      hipStream_t);                                                            \
+  template hipError_t launch_logsph_field_update<T>(                           \
+      const Mesh &, const LsCoef &, const LsParams &, const FieldSet<T> &,     \
+      double, hipStream_t);                                                    \
+  template hipError_t launch_logsph_field_boundary<T>(                         \
+      const Mesh &, const LsParams &, const FieldSet<T> &, double,             \
+      hipStream_t);                                                            \
+  template hipError_t launch_logsph_inject<T>(                                 \
+      const Mesh &, const LsParams &, const PtcView<T> &, uint64_t, int,       \
+      double, double, double, uint64_t, uint32_t, int, uint64_t, long long *,  \
+      hipStream_t);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace apt

@@ -1,0 +1,467 @@
+// kern_rt.hip -- pair creation and photon transport:
+//   * radiative_transfer::emit_photons / produce_pairs
+//     (radiative_transfer.cu:28-301) with the reference's default hooks
+//     (user_radiative_functions.cu:13-125) and an exponential-free-path
+//     model with a fixed photon energy fraction;
+//   * move_photons_cart_{1,2,3}d (ptc_updater.cu:627-741) and the
+//     log-spherical move_photons (ptc_updater_logsph.cu:58-132).
+// Decide -> exclusive scan -> write: the k-th emitting particle (slot order)
+// fills photon slot number + k and the k-th converting photon fills
+// particle slots number + 2k, 2k+1, replacing the reference's per-block
+// atomic counters, so events and their storage order do not depend on the
+// launch shape.  Random draws are Philox4x32-10 keyed on (seed, particle
+// slot, call epoch).  The one host read per call is the event count, as
+// in the reference (radiative_transfer.cu:225-228, 279-282).
+#include <hipcub/hipcub.hpp>
+#include <type_traits>
+
+#include "apt_common.hpp"
+#include "apt_kernels.hpp"
+#include "apt_math.hpp"
+
+namespace apt {
+
+constexpr double RT_PI = 3.14159265358979323846;
+
+template <typename R>
+struct RtMath;
+template <>
+struct RtMath<double> {
+  static __device__ double sin(double x) { return apt_sin(x); }
+  static __device__ double log(double x) { return apt_log(x); }
+  static __device__ double exp(double x) { return apt_exp(x); }
+  static __device__ void sincos(double x, double &s, double &c) {
+    apt_sincos(x, s, c);
+  }
+  static __device__ double acos(double t) { return apt_acos(t); }
+  static __device__ double atan2(double y, double x) { return apt_atan2(y, x); }
+};
+template <>
+struct RtMath<float> {
+  static __device__ float sin(float x) { return sinf(x); }
+  static __device__ float log(float x) { return logf(x); }
+  static __device__ float exp(float x) { return expf(x); }
+  static __device__ void sincos(float x, float &s, float &c) {
+    sincosf(x, &s, &c);
+  }
+  static __device__ float acos(float t) {
+    return acosf(fminf(fmaxf(t, -1.0f), 1.0f));
+  }
+  static __device__ float atan2(float y, float x) { return atan2f(y, x); }
+};
+
+// Quadmesh::pos (quadmesh.h:144-150), including the collapsed dimensions
+__device__ inline double rt_pos(const Mesh &m, int d, int n, double x) {
+  if (d < m.dim) return m.lower[d] + m.delta[d] * ((double)(n - m.guard[d]) + x);
+  return x * m.delta[d];
+}
+
+template <typename R>
+__device__ inline R ptc_gamma(const PtcView<R> &p, uint64_t i) {
+  if (p.E) return p.E[i];
+  R a = p.p1[i], b = p.p2[i], c = p.p3[i];
+  return sqrt((R)1 + a * a + b * b + c * c);
+}
+
+// count_photon_produced: the emit decision (check_emit_photon)
+template <typename R>
+__global__ void k_emit_flag(Mesh m, RtParams rt, PtcView<R> p, uint64_t n,
+                            uint32_t *__restrict__ flags) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c = p.cell[i], f = 0;
+  if (c != MAX_CELL) {
+    int c1, c2, c3;
+    m.split(c, c1, c2, c3);
+    if (m.in_bulk(c1, c2, c3) && ptc_species(p.flag[i]) != 2 &&
+        ptc_gamma(p, i) > (R)rt.gamma_thr)
+      f = 1;
+  }
+  flags[i] = f;
+}
+
+// produce_photons + emit_photon
+template <typename R>
+__global__ void k_emit(Mesh m, RtParams rt, PtcView<R> p, PhoView<R> ph,
+                       uint64_t n, const uint32_t *__restrict__ flags,
+                       const uint32_t *__restrict__ rank, uint64_t base,
+                       uint64_t seed, uint32_t epoch, uint64_t id_base,
+                       unsigned long long *__restrict__ counters) {
+  using M = RtMath<R>;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flags[i]) return;
+  const uint64_t slot = base + rank[i];
+  if (slot >= rt.max_photon) {
+    atomicAdd(&counters[CTR_OVERFLOW], 1ull);
+    return;
+  }
+  ph.cell[slot] = MAX_CELL;
+  if (ph.id) ph.id[slot] = id_base + (slot - base);
+  double u0, u1;
+  uniform2(seed, i, epoch, 5, u0, u1);
+  const uint32_t c = p.cell[i];
+  const R gamma = ptc_gamma(p, i);
+  const R p1 = p.p1[i], p2 = p.p2[i], p3 = p.p3[i];
+  const R pi = sqrt(gamma * gamma - (R)1);
+  R Eph;
+  if (rt.model == 0)
+    Eph = (R)2.5 + (R)u0 * ((R)rt.E_secondary - (R)1) * (R)2;
+  else
+    Eph = (R)rt.e_frac * gamma;
+  const R ge = gamma - Eph;
+  const R pf2 = ge * ge - (R)1;
+  const R pf = sqrt(pf2 > (R)0 ? pf2 : (R)0);
+  p.p1[i] = p1 * pf / pi;
+  p.p2[i] = p2 * pf / pi;
+  p.p3[i] = p3 * pf / pi;
+  if (p.E) p.E[i] = ge;
+  R path;
+  if (rt.model == 0) {
+    int c1, c2, c3;
+    m.split(c, c1, c2, c3);
+    const R theta = (R)rt_pos(m, 1, c2, (double)p.x2[i]);
+    R lph = ((R)1 / M::sin(theta) - (R)1) * (R)rt.photon_path;
+    if (lph > (R)10) lph = (R)10;
+    if (theta < (R)0.005 || theta > (R)(RT_PI - 0.005)) return;
+    path = lph * ((R)0.5 + (R)0.5 * (R)u1);
+  } else {
+    path = -(R)rt.photon_path * M::log((R)u1);
+  }
+  if (path > (R)rt.r_cutoff) return;
+  ph.x1[slot] = p.x1[i];
+  ph.x2[slot] = p.x2[i];
+  ph.x3[slot] = p.x3[i];
+  ph.p1[slot] = Eph * p1 / pi;
+  ph.p2[slot] = Eph * p2 / pi;
+  ph.p3[slot] = Eph * p3 / pi;
+  if (ph.E) ph.E[slot] = Eph;
+  ph.w[slot] = p.w[i];
+  ph.path[slot] = path;
+  ph.flag[slot] = 0;
+  ph.cell[slot] = c;
+}
+
+// count_pairs_produced: check_produce_pair
+template <typename R>
+__global__ void k_pair_flag(Mesh m, PhoView<R> ph, uint64_t n,
+                            uint32_t *__restrict__ flags) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  uint32_t c = ph.cell[t], f = 0;
+  if (c != MAX_CELL) {
+    int c1, c2, c3;
+    m.split(c, c1, c2, c3);
+    if (m.in_bulk(c1, c2, c3) && ph.path[t] <= (R)0) f = 1;
+  }
+  flags[t] = f;
+}
+
+// produce_pairs + produce_pair
+template <typename R>
+__global__ void k_pair(RtParams rt, PtcView<R> p, PhoView<R> ph, uint64_t n,
+                       const uint32_t *__restrict__ flags,
+                       const uint32_t *__restrict__ rank, uint64_t base,
+                       uint64_t id_base,
+                       unsigned long long *__restrict__ counters) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || !flags[t]) return;
+  const uint64_t k = rank[t];
+  const uint64_t off = base + 2 * k;
+  if (off + 1 >= rt.max_ptc) {
+    atomicAdd(&counters[CTR_OVERFLOW], 1ull);
+    ph.cell[t] = MAX_CELL;
+    return;
+  }
+  const R p1 = ph.p1[t], p2 = ph.p2[t], p3 = ph.p3[t];
+  R E_ph2 = p1 * p1 + p2 * p2 + p3 * p3;
+  if (E_ph2 <= (R)4.01) E_ph2 = (R)4.01;
+  const R ratio = sqrt((R)0.25 - (R)1 / E_ph2);
+  const R gamma = sqrt((R)1 + ratio * ratio * E_ph2);
+  const uint32_t c = ph.cell[t];
+  for (int s = 0; s < 2; s++) {
+    const uint64_t o = off + s;
+    p.x1[o] = ph.x1[t];
+    p.x2[o] = ph.x2[t];
+    p.x3[o] = ph.x3[t];
+    p.p1[o] = ratio * p1;
+    p.p2[o] = ratio * p2;
+    p.p3[o] = ratio * p3;
+    if (p.E) p.E[o] = gamma;
+    p.w[o] = ph.w[t];
+    p.flag[o] = FLAG_SECONDARY | ((uint32_t)s << 29);
+    if (p.id) p.id[o] = id_base + 2 * k + s;
+    p.cell[o] = c;
+  }
+  ph.cell[t] = MAX_CELL;
+}
+
+// events with rank below the capacity limit: rank[n-1] + flags[n-1]
+__global__ void k_event_total(const uint32_t *__restrict__ flags,
+                              const uint32_t *__restrict__ rank, uint64_t n,
+                              unsigned long long *__restrict__ out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    *out = n ? (unsigned long long)rank[n - 1] + flags[n - 1] : 0ull;
+}
+
+// move_photons_cart_{1,2,3}d (ptc_updater.cu:627-741) + periodic wrap
+template <typename R>
+__global__ void k_move_photons_cart(Mesh m, PhoView<R> ph, uint64_t n, R dt) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t c = ph.cell[t];
+  if (c == MAX_CELL) return;
+  const R p1 = ph.p1[t], p2 = ph.p2[t], p3 = ph.p3[t];
+  const R E = sqrt(p1 * p1 + p2 * p2 + p3 * p3);
+  const R x1 = ph.x1[t], x2 = ph.x2[t], x3 = ph.x3[t];
+  int c1, c2, c3;
+  m.split(c, c1, c2, c3);
+  int dc[3] = {0, 0, 0};
+  R nx[3];
+  if (m.dim == 1) {
+    nx[0] = x1 + p1 / E * dt * (R)m.inv_delta[0];
+    dc[0] = (int)floor(nx[0]);
+    nx[0] -= (R)dc[0];
+    nx[1] = x2 + p2 * dt / E;
+    nx[2] = x3 + p3 * dt / E;
+  } else {
+    const R v1 = p1 / E, v2 = p2 / E, v3 = p3 / E;
+    nx[0] = x1 + v1 * dt * (R)m.inv_delta[0];
+    nx[1] = x2 + v2 * dt * (R)m.inv_delta[1];
+    dc[0] = (int)floor(nx[0]);
+    dc[1] = (int)floor(nx[1]);
+    nx[0] -= (R)dc[0];
+    nx[1] -= (R)dc[1];
+    if (m.dim == 2) {
+      nx[2] = x3 + v3 * dt;
+    } else {
+      nx[2] = x3 + v3 * dt * (R)m.inv_delta[2];
+      dc[2] = (int)floor(nx[2]);
+      nx[2] -= (R)dc[2];
+    }
+  }
+  if (dc[0] < -1 || dc[0] > 1 || dc[1] < -1 || dc[1] > 1 || dc[2] < -1 ||
+      dc[2] > 1) {
+    ph.cell[t] = MAX_CELL;
+    return;
+  }
+  int n1 = c1 + dc[0], n2 = c2 + dc[1], n3 = c3 + dc[2];
+  if (m.dim >= 1) n1 = m.wrap(0, n1);
+  if (m.dim >= 2) n2 = m.wrap(1, n2);
+  if (m.dim >= 3) n3 = m.wrap(2, n3);
+  ph.cell[t] = (uint32_t)m.idx(n1, n2, n3);
+  ph.x1[t] = nx[0];
+  ph.x2[t] = nx[1];
+  ph.x3[t] = nx[2];
+  ph.path[t] -= dt;
+}
+
+// move_photons (ptc_updater_logsph.cu:58-132); as in the reference the
+// cell is taken before the axis reflection
+template <typename R>
+__global__ void k_move_photons_logsph(Mesh m, PhoView<R> ph, uint64_t n, R dt,
+                                      double r_cutoff) {
+  using M = RtMath<R>;
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t c = ph.cell[t];
+  if (c == MAX_CELL) return;
+  const int d0 = m.dims[0];
+  const int c1 = (int)(c % (uint32_t)d0), c2 = (int)(c / (uint32_t)d0);
+  R v1 = ph.p1[t], v2 = ph.p2[t], v3 = ph.p3[t];
+  const R E = sqrt(v1 * v1 + v2 * v2 + v3 * v3);
+  v1 = v1 / E;
+  v2 = v2 / E;
+  v3 = v3 / E;
+  const R old_x1 = ph.x1[t], old_x2 = ph.x2[t], old_x3 = ph.x3[t];
+  const R r1 = (R)rt_pos(m, 0, c1, (double)old_x1);
+  const R exp_r1 = M::exp(r1);
+  if (exp_r1 > (R)r_cutoff || exp_r1 < (R)1.02) {
+    ph.cell[t] = MAX_CELL;
+    return;
+  }
+  const R r2 = (R)rt_pos(m, 1, c2, (double)old_x2);
+  R s2, cs2, s3, cs3;
+  M::sincos(r2, s2, cs2);
+  M::sincos(old_x3, s3, cs3);
+  R x = exp_r1 * s2 * cs3;
+  R y = exp_r1 * s2 * s3;
+  R z = exp_r1 * cs2;
+  {
+    R a = v1, b = v2, cc = v3;
+    v1 = a * s2 * cs3 + b * cs2 * cs3 - cc * s3;
+    v2 = a * s2 * s3 + b * cs2 * s3 + cc * cs3;
+    v3 = a * cs2 - b * s2;
+  }
+  x += v1 * dt;
+  y += v2 * dt;
+  z += v3 * dt;
+  R r1p = sqrt(x * x + y * y + z * z);
+  R r2p;  // FP32: atan2 form, as in the particle mover
+  if constexpr (std::is_same<R, double>::value)
+    r2p = M::acos(z / r1p);
+  else
+    r2p = M::atan2(sqrt(x * x + y * y), z);
+  r1p = M::log(r1p);
+  const R r3p = M::atan2(y, x);
+  {
+    R ps2, pc2, ps3, pc3;
+    M::sincos(r2p, ps2, pc2);
+    M::sincos(r3p, ps3, pc3);
+    R a = v1, b = v2, cc = v3;
+    v1 = a * ps2 * pc3 + b * ps2 * ps3 + cc * pc2;
+    v2 = a * pc2 * pc3 + b * pc2 * ps3 - cc * ps2;
+    v3 = -a * ps3 + b * pc3;
+  }
+  ph.p1[t] = v1 * E;
+  ph.p2[t] = v2 * E;
+  ph.p3[t] = v3 * E;
+  R new_x1 = old_x1 + (r1p - r1) / (R)m.delta[0];
+  R new_x2 = old_x2 + (r2p - r2) / (R)m.delta[1];
+  const int dc1 = (int)floor(new_x1);
+  const int dc2 = (int)floor(new_x2);
+  if (dc1 < -1 || dc1 > 1 || dc2 < -1 || dc2 > 1) {
+    ph.cell[t] = MAX_CELL;
+    return;
+  }
+  ph.cell[t] = (uint32_t)((c1 + dc1) + (c2 + dc2) * d0);
+  new_x1 -= (R)dc1;
+  new_x2 -= (R)dc2;
+  if (rt_pos(m, 1, c2 + dc2, (double)new_x2) < 0.0)
+    new_x2 = (R)1 - new_x2;
+  else if (rt_pos(m, 1, c2 + dc2, (double)new_x2) >= RT_PI)
+    new_x2 = (R)1 - new_x2;
+  ph.x1[t] = new_x1;
+  ph.x2[t] = new_x2;
+  ph.x3[t] = r3p;
+  ph.path[t] -= dt;
+}
+
+// ---------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------
+static inline unsigned rt_blocks(uint64_t n) {
+  return (unsigned)((n + 255) / 256);
+}
+
+hipError_t RtWork::ensure(uint64_t n) {
+  if (n <= cap && total) return hipSuccess;
+  release();
+  hipError_t e;
+  uint64_t c = n < 1024 ? 1024 : n;
+  if ((e = hipMalloc(&flags, c * 4)) != hipSuccess) return e;
+  if ((e = hipMalloc(&rank, c * 4)) != hipSuccess) return e;
+  if ((e = hipMalloc(&total, sizeof(unsigned long long))) != hipSuccess)
+    return e;
+  size_t bytes = 0;
+  e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, flags, rank, (int)c);
+  if (e != hipSuccess) return e;
+  if ((e = hipMalloc(&temp, bytes)) != hipSuccess) return e;
+  temp_bytes = bytes;
+  cap = c;
+  return hipSuccess;
+}
+
+void RtWork::release() {
+  (void)hipFree(flags);
+  (void)hipFree(rank);
+  (void)hipFree(total);
+  (void)hipFree(temp);
+  flags = rank = nullptr;
+  total = nullptr;
+  temp = nullptr;
+  temp_bytes = 0;
+  cap = 0;
+}
+
+static hipError_t scan_events(RtWork &w, uint64_t n, hipStream_t st,
+                              uint64_t *count) {
+  size_t bytes = w.temp_bytes;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(w.temp, bytes, w.flags,
+                                                  w.rank, (int)n, st);
+  if (e != hipSuccess) return e;
+  k_event_total<<<1, 1, 0, st>>>(w.flags, w.rank, n, w.total);
+  unsigned long long h = 0;
+  if ((e = hipMemcpyAsync(&h, w.total, 8, hipMemcpyDeviceToHost, st)) !=
+      hipSuccess)
+    return e;
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+  *count = h;
+  return hipSuccess;
+}
+
+template <typename R>
+hipError_t rt_emit(const Mesh &m, const RtParams &rt, const PtcView<R> &p,
+                   uint64_t n, const PhoView<R> &ph, uint64_t ph_base,
+                   uint64_t seed, uint32_t epoch, uint64_t id_base,
+                   RtWork &w, unsigned long long *counters, hipStream_t st,
+                   uint64_t *emitted) {
+  *emitted = 0;
+  if (n == 0) return hipSuccess;
+  hipError_t e = w.ensure(n);
+  if (e != hipSuccess) return e;
+  k_emit_flag<R><<<rt_blocks(n), 256, 0, st>>>(m, rt, p, n, w.flags);
+  uint64_t total = 0;
+  if ((e = scan_events(w, n, st, &total)) != hipSuccess) return e;
+  if (total == 0) return hipSuccess;
+  k_emit<R><<<rt_blocks(n), 256, 0, st>>>(m, rt, p, ph, n, w.flags, w.rank,
+                                          ph_base, seed, epoch, id_base,
+                                          counters);
+  uint64_t room = rt.max_photon > ph_base ? rt.max_photon - ph_base : 0;
+  *emitted = total < room ? total : room;
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t rt_pairs(const Mesh &m, const RtParams &rt, const PtcView<R> &p,
+                    uint64_t p_base, const PhoView<R> &ph, uint64_t nph,
+                    uint64_t id_base, RtWork &w, unsigned long long *counters,
+                    hipStream_t st, uint64_t *made) {
+  *made = 0;
+  if (nph == 0) return hipSuccess;
+  hipError_t e = w.ensure(nph);
+  if (e != hipSuccess) return e;
+  k_pair_flag<R><<<rt_blocks(nph), 256, 0, st>>>(m, ph, nph, w.flags);
+  uint64_t total = 0;
+  if ((e = scan_events(w, nph, st, &total)) != hipSuccess) return e;
+  if (total == 0) return hipSuccess;
+  k_pair<R><<<rt_blocks(nph), 256, 0, st>>>(rt, p, ph, nph, w.flags, w.rank,
+                                            p_base, id_base, counters);
+  // pairs whose second slot fits: base + 2k + 1 < max_ptc, k < L/2
+  const uint64_t L = rt.max_ptc > p_base ? rt.max_ptc - p_base : 0;
+  *made = total < L / 2 ? total : L / 2;
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t rt_move_photons(const Mesh &m, int logsph, double r_cutoff,
+                           const PhoView<R> &ph, uint64_t n, double dt,
+                           hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  if (logsph)
+    k_move_photons_logsph<R><<<rt_blocks(n), 256, 0, st>>>(m, ph, n, (R)dt,
+                                                           r_cutoff);
+  else
+    k_move_photons_cart<R><<<rt_blocks(n), 256, 0, st>>>(m, ph, n, (R)dt);
+  return hipGetLastError();
+}
+
+#define INST(R)                                                               \
+  template hipError_t rt_emit<R>(const Mesh &, const RtParams &,              \
+                                 const PtcView<R> &, uint64_t,                \
+                                 const PhoView<R> &, uint64_t, uint64_t,      \
+                                 uint32_t, uint64_t, RtWork &,                \
+                                 unsigned long long *, hipStream_t,           \
+                                 uint64_t *);                                 \
+  template hipError_t rt_pairs<R>(const Mesh &, const RtParams &,             \
+                                  const PtcView<R> &, uint64_t,               \
+                                  const PhoView<R> &, uint64_t, uint64_t,     \
+                                  RtWork &, unsigned long long *,             \
+                                  hipStream_t, uint64_t *);                   \
+  template hipError_t rt_move_photons<R>(const Mesh &, int, double,           \
+                                         const PhoView<R> &, uint64_t,        \
+                                         double, hipStream_t);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace apt

@@ -43,6 +43,19 @@ enum {
                          int64 fixed-point current */
 };
 
+/* Coordinate systems (SimParams::coord_system, sim_params.h:101). */
+enum {
+  APT_COORD_CARTESIAN = 0,
+  APT_COORD_LOGSPH = 1 /* 2D (ln r, theta), Grid_LogSph (grid_log_sph.h) */
+};
+
+/* Pair-creation models (radiative_transfer hooks). */
+enum {
+  APT_RT_DEFAULT = 0, /* user_radiative_functions.cu:13-125 */
+  APT_RT_EXP_PATH = 1 /* photon energy e_frac gamma, free path
+                         -photon_path ln u */
+};
+
 /* Field identifiers (the members of sim_data, src/sim_data.h:40-60). */
 enum {
   APT_FIELD_E = 0,
@@ -91,6 +104,27 @@ typedef struct {
   int tile_halo;            /* mixed form: LDS halo in cells (0 = auto) */
   int tile_threads;         /* mixed form: threads per tile workgroup
                                (0 = auto) */
+  /* ---- log-spherical path (sim_params.h:31-64) ---- */
+  int coord;                /* APT_COORD_* (coord_system, :101) */
+  double compactness;       /* compactness (:33): beta_phi, omega_LT */
+  double omega;             /* omega (:62): beta_phi's rotation rate */
+  int gravity_on;           /* gravity_on (:31) */
+  double gravity;           /* gravity (:32) */
+  int rad_cooling_on;       /* rad_cooling_on (:34) */
+  double rad_cooling_coef;  /* rad_cooling_coef (:35) */
+  double B0;                /* B0 (:58), the cooling field scale */
+  int damping_length;       /* damping_length (:47), cells */
+  double damping_coef;      /* damping_coef (:48) */
+  int inject_ions;          /* inject_ions (:64) */
+  double track_percent;     /* track_percent (:56) */
+  /* ---- pair creation (sim_params.h:52-79) ---- */
+  int rt_model;             /* APT_RT_* */
+  double gamma_thr;         /* gamma_thr (:52) */
+  double photon_path;       /* photon_path (:73) */
+  double E_secondary;       /* E_secondary (:78) */
+  double r_cutoff;          /* r_cutoff (:79) */
+  double e_frac;            /* APT_RT_EXP_PATH: photon energy / gamma */
+  uint64_t max_photon_number; /* max_photon_number (:23) */
 } apt_params;
 
 /* Host view of particle_data (src/data/particle_data.h:50-61) in FP64.
@@ -104,6 +138,31 @@ typedef struct {
   uint32_t *flag;
   uint64_t *id;
 } apt_particles;
+
+/* Host view of photon_data (src/data/particle_data.h:68-81) in FP64. */
+typedef struct {
+  uint32_t *cell;
+  double *x1, *x2, *x3;
+  double *p1, *p2, *p3;
+  double *E;
+  double *weight;
+  double *path_left;
+  uint32_t *flag;
+  uint64_t *id;
+} apt_photons;
+
+/* Bookkeeping of a context (no reference counterpart; for tests and
+ * tools): counts, fixed-point exponents, id and random-stream epochs. */
+typedef struct {
+  uint64_t number;       /* particles (including empty slots) */
+  uint64_t ph_number;    /* photons */
+  int H;                 /* current fixed-point exponent */
+  int Hs;                /* surface-density fixed-point exponent */
+  uint64_t id_next;      /* next particle id (FP64 form) */
+  uint64_t ph_id_next;   /* next photon id (FP64 form) */
+  uint32_t rt_epoch;     /* emit_photons calls so far */
+  uint32_t inject_epoch; /* inject calls so far */
+} apt_info;
 
 /* Device-side event counters that replace the reference's trap/printf
  * diagnostics (ptc_updater.cu:43-50, 251-255). */
@@ -125,6 +184,11 @@ const char *apt_last_error(const apt_ctx *ctx);
  * the FP64 form, J = acc 2^-H in the mixed form) */
 apt_status apt_get_jexp(const apt_ctx *ctx, int *H);
 apt_status apt_get_mesh(const apt_ctx *ctx, int dims[3], double delta[3]);
+apt_status apt_get_info(const apt_ctx *ctx, apt_info *out);
+/* Log-spherical metric coefficient `which` (0..12 in the order l1_e, l2_e,
+ * l3_e, l1_b, l2_b, l3_b, A1_e, A2_e, A3_e, A1_b, A2_b, A3_b, dV;
+ * Grid_LogSph members, grid_log_sph.h:45-49), dims0*dims1 doubles. */
+apt_status apt_get_coef(apt_ctx *ctx, int which, double *dst, uint64_t count);
 
 /* ---- data movement (multi_array copy_to/from_device, fields.h) ------- */
 /* field<N,T>::initialize / copy_to_device (core/fields.h:111-143) */
@@ -147,6 +211,11 @@ apt_status apt_ptc_load_uniform(apt_ctx *ctx, int ppc_pairs,
                                 const double theta[2], const double drift[2],
                                 double weight);
 
+/* photons: particles_t<photon_data>::copy_from / copy_to_host / number */
+apt_status apt_ph_set(apt_ctx *ctx, const apt_photons *src, uint64_t n);
+apt_status apt_ph_get(apt_ctx *ctx, apt_photons *dst, uint64_t cap,
+                      uint64_t *n);
+
 /* ---- the time step -------------------------------------------------- */
 /* ptc_updater::update_particles(sim_data&, double dt, uint32_t step)
  * (src/algorithms/ptc_updater.h:11-24, cuda/algorithms/ptc_updater.cu:
@@ -158,15 +227,33 @@ apt_status apt_ptc_update(apt_ctx *ctx, double dt, uint32_t step);
  * Rho on deposit steps); apt_ptc_update already runs them once. */
 apt_status apt_smooth_current(apt_ctx *ctx, uint32_t step);
 /* field_solver::update_fields(sim_data&, double dt, uint32_t step)
- * (src/algorithms/field_solver.h:11-25, field_solver.cu:334-358) */
+ * (src/algorithms/field_solver.h:11-25, field_solver.cu:334-358); in a
+ * log-spherical context field_solver_logsph::update_fields(sim_data&,
+ * double dt, double time) (field_solver_logsph.h:10-20,
+ * field_solver_logsph.cu:354-398): B, E, divergences, no half step. */
 apt_status apt_field_update(apt_ctx *ctx, double dt, uint32_t step);
+/* field_solver_logsph::apply_boundary(sim_data&, double omega, double time)
+ * (field_solver_logsph.cu:403-428): rotating conductor, absorbing layer,
+ * axis.  A no-op in a Cartesian context, like
+ * field_solver::apply_outflow_boundary (field_solver.cu:451-452). */
+apt_status apt_field_boundary(apt_ctx *ctx, double omega, double time);
+/* ptc_updater_logsph::inject_ptc(sim_data&, int inj_per_cell, Scalar p1,
+ * Scalar p2, Scalar p3, Scalar w, Scalar omega) (ptc_updater_logsph.h:13-28,
+ * ptc_updater_logsph.cu:848-870); log-spherical contexts only. */
+apt_status apt_inject_ptc(apt_ctx *ctx, int inj_per_cell, double p1,
+                          double p2, double p3, double w, double omega);
+/* radiative_transfer::emit_photons(sim_data&) / produce_pairs(sim_data&)
+ * (radiative_transfer.h:11-28, radiative_transfer.cu:198-301); the
+ * optional outputs return the number of events. */
+apt_status apt_emit_photons(apt_ctx *ctx, uint64_t *emitted);
+apt_status apt_produce_pairs(apt_ctx *ctx, uint64_t *pairs);
 /* field_solver::update_e_field / update_b_field / compute_divs
  * (field_solver.cu:360-504) */
 apt_status apt_field_update_e(apt_ctx *ctx, double dt);
 apt_status apt_field_update_b(apt_ctx *ctx, double dt);
 apt_status apt_field_compute_divs(apt_ctx *ctx);
-/* sim_data::sort_particles -> particle_base::sort_by_cell
- * (sim_data_impl.hpp:78-89, cuda/core/particle_base_impl.cu:435-464) */
+/* sim_data::sort_particles -> particle_base::sort_by_cell for particles and
+ * photons (sim_data_impl.hpp:78-89, cuda/core/particle_base_impl.cu:435-464) */
 apt_status apt_sort_particles(apt_ctx *ctx);
 /* particle_base::clear_guard_cells (particle_base_impl.cu:489-493) */
 apt_status apt_clear_guard_cells(apt_ctx *ctx);

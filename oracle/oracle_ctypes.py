@@ -260,6 +260,8 @@ class OracleSim:
         if not deposit_rho:
             for s in range(4):
                 self.rho[s][:] = 0.0
+        if self.ph_number:
+            self.move_photons(dt)
 
     def field_update(self, dt, step):
         self.lib.orc_field_update(C.byref(self.mesh), C.byref(self.F), C.c_double(dt),

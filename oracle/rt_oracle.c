@@ -310,19 +310,19 @@ void orc_clear_photon_guard_cells(const orc_mesh *m, orc_photons *ph) {
 }
 
 /* photons.sort_by_cell (sim_data_impl.hpp:80, particle_base_impl.cu:435-464):
- * stable counting sort by cell, empty and out-of-bulk slots last and
- * dropped.  scratch: number uint64. */
+ * stable counting sort by cell, empty slots last and dropped (the guard
+ * clear removes out-of-bulk photons beforehand).  scratch: number uint64. */
 uint64_t orc_sort_photons(const orc_mesh *m, orc_photons *ph,
                           uint64_t *scratch) {
   uint64_t n = ph->number, kept = 0;
   for (uint64_t t = 0; t < n; t++)
-    if (ph->cell[t] != ORC_MAX_CELL && is_in_bulk(m, ph->cell[t])) kept++;
+    if (ph->cell[t] != ORC_MAX_CELL) kept++;
   /* order: stable by cell */
   size_t nc = (size_t)m->dims[0] * m->dims[1] * m->dims[2];
   uint64_t *cnt = (uint64_t *)calloc(nc + 1, sizeof(uint64_t));
   for (uint64_t t = 0; t < n; t++) {
     uint32_t c = ph->cell[t];
-    size_t key = (c != ORC_MAX_CELL && is_in_bulk(m, c)) ? c : nc;
+    size_t key = c != ORC_MAX_CELL ? c : nc;
     cnt[key]++;
   }
   uint64_t acc = 0;
@@ -333,7 +333,7 @@ uint64_t orc_sort_photons(const orc_mesh *m, orc_photons *ph,
   }
   for (uint64_t t = 0; t < n; t++) {
     uint32_t c = ph->cell[t];
-    size_t key = (c != ORC_MAX_CELL && is_in_bulk(m, c)) ? c : nc;
+    size_t key = c != ORC_MAX_CELL ? c : nc;
     scratch[cnt[key]++] = t;
   }
   free(cnt);
