@@ -375,6 +375,55 @@ class LogSphOracle(OracleSim):
         return self.number
 
 
+class OrcFFEParams(C.Structure):
+    _fields_ = [("damping_length", C.c_int), ("damping_coef", C.c_double)]
+
+
+class FFEOracle(OracleSim):
+    """3D log-spherical (ln r, theta, phi) field state for the force-free /
+    vacuum solver: phi periodic, the 2D Grid_LogSph coefficients shared by
+    every phi plane."""
+
+    def __init__(self, N, guard, size, damping_length=0, damping_coef=0.0):
+        super().__init__(N, guard, size=size, periodic=(False, False, True),
+                         capacity=0, num_species=1)
+        self.coef = [np.zeros(self.ncell) for _ in range(13)]
+        self.C = OrcLogsphCoef()
+        for i in range(13):
+            self.C.a[i] = _d(self.coef[i])
+        self.lib.orc_logsph_compute_coef(C.byref(self.mesh), C.byref(self.C))
+        self.fp = OrcFFEParams(damping_length, damping_coef)
+        self.scratch = np.zeros(12 * self.ncell)
+        self.dE = np.zeros(3 * self.ncell)
+
+    def coefficient(self, name):
+        n = self.dims[0] * self.dims[1]
+        return self.coef[COEF_NAMES.index(name)][:n]
+
+    def update(self, dt, omega=0.0, time=0.0):
+        self.lib.orc_ffe_update(C.byref(self.mesh), C.byref(self.C), C.byref(self.fp),
+                                C.byref(self.F), C.c_double(dt), C.c_double(omega),
+                                C.c_double(time), _d(self.scratch))
+
+    def boundary(self, omega, time=0.0):
+        self.lib.orc_ffe_boundary(C.byref(self.mesh), C.byref(self.fp), C.byref(self.F),
+                                  C.c_double(omega), C.c_double(time))
+
+    def rk_push(self, dt):
+        n = self.ncell
+        dE = [np.zeros(n) for _ in range(3)]
+        dB = [np.zeros(n) for _ in range(3)]
+        self.lib.orc_ffe_rk_push(C.byref(self.mesh), C.byref(self.C), C.byref(self.F),
+                                 *[_d(a) for a in dE + dB], C.c_double(dt))
+        return dE, dB
+
+    def clean_epar(self):
+        self.lib.orc_ffe_clean_epar(C.byref(self.mesh), C.byref(self.F), _d(self.dE))
+
+    def check_egtb(self):
+        self.lib.orc_ffe_check_egtb(C.byref(self.mesh), C.byref(self.F), _d(self.dE))
+
+
 def _grow_ptc(self, n):
     old = self.ptc
     self.ptc = {k: np.zeros(n, dt) for k, dt in self.PTC_FIELDS}

@@ -149,6 +149,23 @@ uint64_t orc_logsph_inject(const orc_mesh *m, const orc_logsph_params *lp,
                            uint64_t seed, uint32_t epoch, int Hs,
                            uint64_t id_base, int64_t *surf);
 
+/* --- 3D log-spherical force-free / vacuum solver (ffe_oracle.c) --- */
+typedef struct {
+  int damping_length;
+  double damping_coef;
+} orc_ffe_params;
+
+void orc_ffe_update(const orc_mesh *m, const orc_logsph_coef *c,
+                    const orc_ffe_params *fp, orc_fields *f, double dt,
+                    double omega, double time, double *scratch);
+void orc_ffe_boundary(const orc_mesh *m, const orc_ffe_params *fp,
+                      orc_fields *f, double omega, double t);
+void orc_ffe_rk_push(const orc_mesh *m, const orc_logsph_coef *c,
+                     orc_fields *f, double *dE0, double *dE1, double *dE2,
+                     double *dB0, double *dB1, double *dB2, double dt);
+void orc_ffe_clean_epar(const orc_mesh *m, orc_fields *f, double *dE);
+void orc_ffe_check_egtb(const orc_mesh *m, orc_fields *f, double *dE);
+
 /* --- photons and pair creation (rt_oracle.c) --- */
 /* photon_data SoA (src/data/particle_data.h:68-81), FP64 form */
 typedef struct {
