@@ -119,6 +119,10 @@ SIGNATURES = {
     "apt_inject_ptc": [_P, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                        C.c_double],
     "apt_emit_photons": [_P, C.POINTER(C.c_uint64)],
+    "apt_ffe_update": [_P, C.c_double, C.c_double, C.c_double],
+    "apt_ffe_boundary": [_P, C.c_double, C.c_double],
+    "apt_ffe_clean_epar": [_P],
+    "apt_ffe_check_egtb": [_P],
     "apt_produce_pairs": [_P, C.POINTER(C.c_uint64)],
     "apt_field_set": [_P, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_uint64],
     "apt_field_get": [_P, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_uint64],
@@ -455,8 +459,8 @@ class sim_data:
         return out
 
     def coefficient(self, name):
-        """A Grid_LogSph coefficient array (grid_log_sph.h:45-49)."""
-        a = np.empty(self.env.ncell(), np.float64)
+        """A Grid_LogSph coefficient array (grid_log_sph.h:45-49), dims0 x dims1."""
+        a = np.empty(self.env.dims[0] * self.env.dims[1], np.float64)
         self._check(self.lib.apt_get_coef(self.ctx, COEF_NAMES.index(name), _dptr(a), a.size),
                     "apt_get_coef")
         return a
@@ -571,3 +575,23 @@ class radiative_transfer:
         k = C.c_uint64()
         data._check(data.lib.apt_produce_pairs(data.ctx, C.byref(k)), "apt_produce_pairs")
         return k.value
+
+
+class ffe_solver_logsph:
+    """ffe_solver_logsph (src/algorithms/ffe_solver_logsph.h:12-33) on a 3D
+    log-spherical context."""
+
+    def __init__(self, env: sim_environment):
+        self.m_env = env
+
+    def update_fields(self, data: sim_data, dt: float, omega: float, time: float = 0.0):
+        data._check(data.lib.apt_ffe_update(data.ctx, dt, omega, time), "apt_ffe_update")
+
+    def apply_boundary(self, data: sim_data, omega: float, time: float = 0.0):
+        data._check(data.lib.apt_ffe_boundary(data.ctx, omega, time), "apt_ffe_boundary")
+
+    def clean_epar(self, data: sim_data):
+        data._check(data.lib.apt_ffe_clean_epar(data.ctx), "apt_ffe_clean_epar")
+
+    def check_eGTb(self, data: sim_data):
+        data._check(data.lib.apt_ffe_check_egtb(data.ctx), "apt_ffe_check_egtb")

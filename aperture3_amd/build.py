@@ -32,6 +32,7 @@ HIP_SOURCES = [
     "kern_tile_sort.hip",
     "kern_logsph.hip",
     "kern_rt.hip",
+    "kern_ffe.hip",
     "apt_context.hip",
 ]
 HOST_SOURCES = ["host/aperture_host.cpp"]

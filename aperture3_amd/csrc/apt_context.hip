@@ -66,6 +66,9 @@ struct apt_ctx {
   uint32_t inject_epoch = 0;
   double wmaxw = 1.0;  // largest particle weight: surface-density scale
 
+  // force-free / vacuum solver scratch (3D log-spherical contexts)
+  void *ffe[12] = {};  // En[3], Bn[3], dE[3], dB[3]
+
   // photons and pair creation
   RtParams rt{};
   RtWork rtw;
@@ -385,9 +388,13 @@ apt_status apt_create(const apt_grid *grid, const apt_params *params,
     return APT_ERR_INVALID_ARG;
   }
   if (ctx->logsph) {
-    // Grid_LogSph is 2D only (field_solver_logsph.cu:357-358)
-    if (m.dim != 2 || m.guard[0] < 2 || m.guard[1] < 2 ||
-        params->damping_length < 0 || params->damping_length > m.dims[0]) {
+    // 2D: the particle path and Yee solver (field_solver_logsph.cu:357-358);
+    // 3D: the force-free / vacuum solver, which updates two guard layers
+    // (ffe_solver_logsph.cu:18) and needs guard >= 3
+    bool ok2 = m.dim == 2 && m.guard[0] >= 2 && m.guard[1] >= 2;
+    bool ok3 = m.dim == 3 && m.guard[0] >= 3 && m.guard[1] >= 3 && m.guard[2] >= 3;
+    if (!(ok2 || ok3) || params->damping_length < 0 ||
+        params->damping_length > m.dims[0]) {
       apt_destroy(ctx);
       return APT_ERR_INVALID_ARG;
     }
@@ -405,9 +412,10 @@ apt_status apt_create(const apt_grid *grid, const apt_params *params,
     std::vector<double> c[LS_NCOEF];
     logsph_coef_host(m, c);
     bool okc = true;
+    const uint64_t n2 = (uint64_t)m.dims[0] * m.dims[1];  // 2D arrays
     for (int k = 0; k < LS_NCOEF && okc; k++) {
-      okc = hipMalloc(&ctx->coef_mem[k], nc * 8) == hipSuccess &&
-            hipMemcpy(ctx->coef_mem[k], c[k].data(), nc * 8,
+      okc = hipMalloc(&ctx->coef_mem[k], n2 * 8) == hipSuccess &&
+            hipMemcpy(ctx->coef_mem[k], c[k].data(), n2 * 8,
                       hipMemcpyHostToDevice) == hipSuccess;
       ctx->coef.a[k] = ctx->coef_mem[k];
     }
@@ -467,6 +475,7 @@ apt_status apt_destroy(apt_ctx *ctx) {
   (void)hipFree(ctx->divE); (void)hipFree(ctx->divB); (void)hipFree(ctx->jtmp);
   (void)hipFree(ctx->d_counters);
   for (int k = 0; k < LS_NCOEF; k++) (void)hipFree(ctx->coef_mem[k]);
+  for (int k = 0; k < 12; k++) (void)hipFree(ctx->ffe[k]);
   (void)hipFree(ctx->surf);
   free_ph64(ctx->ph64); free_ph64(ctx->ph64b);
   free_ph32(ctx->ph32); free_ph32(ctx->ph32b);
@@ -524,7 +533,8 @@ apt_status apt_get_coef(apt_ctx *ctx, int which, double *dst, uint64_t count) {
   CHECK_CTX(ctx);
   if (!ctx->logsph)
     return ctx->fail(APT_ERR_UNSUPPORTED, "get_coef: not a log-spherical context");
-  if (which < 0 || which >= LS_NCOEF || !dst || count != ctx->ncell)
+  const uint64_t n2 = (uint64_t)ctx->mesh.dims[0] * ctx->mesh.dims[1];
+  if (which < 0 || which >= LS_NCOEF || !dst || count != n2)
     return ctx->fail(APT_ERR_INVALID_ARG, "get_coef: bad coefficient/size");
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipMemcpy(dst, ctx->coef_mem[which], count * 8, hipMemcpyDeviceToHost));
@@ -1027,6 +1037,9 @@ apt_status apt_ptc_update(apt_ctx *ctx, double dt, uint32_t step) {
   const uint64_t nc = ctx->ncell;
   const int ns = 4;  // every allocated charge slot (see apt_create)
   if (ctx->logsph) {
+    if (ctx->mesh.dim != 2)
+      return ctx->fail(APT_ERR_UNSUPPORTED, "ptc_update: 3D log-spherical contexts "
+                                            "carry the force-free solver only");
     if (!(dt > 0.0))
       return ctx->fail(APT_ERR_INVALID_ARG, "ptc_update: dt must be > 0");
     return logsph_ptc_update(ctx, dt, rho);
@@ -1163,6 +1176,8 @@ static apt_status logsph_field_update_t(apt_ctx *ctx, double dt) {
 
 apt_status apt_field_update(apt_ctx *ctx, double dt, uint32_t step) {
   CHECK_CTX(ctx);
+  if (ctx->logsph && ctx->mesh.dim != 2)
+    return ctx->fail(APT_ERR_UNSUPPORTED, "field_update: use apt_ffe_update in 3D");
   if (ctx->logsph)
     return ctx->prec == APT_PREC_FP64 ? logsph_field_update_t<double>(ctx, dt)
                                       : logsph_field_update_t<float>(ctx, dt);
@@ -1174,6 +1189,8 @@ apt_status apt_field_boundary(apt_ctx *ctx, double omega, double time) {
   CHECK_CTX(ctx);
   (void)time;
   if (!ctx->logsph) return APT_OK;
+  if (ctx->mesh.dim != 2)
+    return ctx->fail(APT_ERR_UNSUPPORTED, "field_boundary: use apt_ffe_boundary in 3D");
   if (ctx->prec == APT_PREC_FP64)
     HIPCHK(launch_logsph_field_boundary<double>(ctx->mesh, ctx->lp,
                                                 field_set<double>(ctx), omega,
@@ -1191,8 +1208,8 @@ apt_status apt_inject_ptc(apt_ctx *ctx, int inj_per_cell, double p1,
   (void)p2;
   (void)p3;
   (void)omega;
-  if (!ctx->logsph)
-    return ctx->fail(APT_ERR_UNSUPPORTED, "inject_ptc: log-spherical contexts only");
+  if (!ctx->logsph || ctx->mesh.dim != 2)
+    return ctx->fail(APT_ERR_UNSUPPORTED, "inject_ptc: 2D log-spherical contexts only");
   if (inj_per_cell < 1)
     return ctx->fail(APT_ERR_INVALID_ARG, "inject_ptc: inj_per_cell < 1");
   const uint64_t rows = (uint64_t)ctx->mesh.N[1] - 2;
@@ -1223,6 +1240,93 @@ apt_status apt_inject_ptc(apt_ctx *ctx, int inj_per_cell, double p1,
   ctx->number += add;
   ctx->id_next += add;
   return APT_OK;
+}
+
+// ---- force-free / vacuum solver (3D log-spherical) ---------------------
+template <typename T>
+static apt_status ffe_buffers(apt_ctx *ctx, FfeBuf<T> &b) {
+  for (int k = 0; k < 12; k++) {
+    if (!ctx->ffe[k]) {
+      HIPCHK(hipMalloc(&ctx->ffe[k], ctx->ncell * sizeof(T)));
+      HIPCHK(launch_fill<T>((T *)ctx->ffe[k], (T)0, ctx->ncell, ctx->stream));
+    }
+  }
+  for (int q = 0; q < 3; q++) {
+    b.En[q] = (T *)ctx->ffe[q];
+    b.Bn[q] = (T *)ctx->ffe[3 + q];
+    b.dE[q] = (T *)ctx->ffe[6 + q];
+    b.dB[q] = (T *)ctx->ffe[9 + q];
+  }
+  return APT_OK;
+}
+
+static apt_status ffe_check(apt_ctx *ctx, const char *what) {
+  if (!ctx->logsph || ctx->mesh.dim != 3)
+    return ctx->fail(APT_ERR_UNSUPPORTED, "%s: 3D log-spherical contexts only", what);
+  return APT_OK;
+}
+
+template <typename T>
+static apt_status ffe_update_t(apt_ctx *ctx, double dt, double omega,
+                               double time) {
+  FfeBuf<T> b;
+  apt_status st = ffe_buffers<T>(ctx, b);
+  if (st != APT_OK) return st;
+  time_begin(ctx, APT_KERNEL_FIELD);
+  HIPCHK(launch_ffe_update<T>(ctx->mesh, ctx->coef, ctx->lp.damping_length,
+                              ctx->lp.damping_coef, field_set<T>(ctx), b, dt,
+                              omega, time, ctx->stream));
+  time_end(ctx, APT_KERNEL_FIELD);
+  return APT_OK;
+}
+
+apt_status apt_ffe_update(apt_ctx *ctx, double dt, double omega, double time) {
+  CHECK_CTX(ctx);
+  apt_status st = ffe_check(ctx, "ffe_update");
+  if (st != APT_OK) return st;
+  return ctx->prec == APT_PREC_FP64 ? ffe_update_t<double>(ctx, dt, omega, time)
+                                    : ffe_update_t<float>(ctx, dt, omega, time);
+}
+
+apt_status apt_ffe_boundary(apt_ctx *ctx, double omega, double time) {
+  CHECK_CTX(ctx);
+  (void)time;
+  apt_status st = ffe_check(ctx, "ffe_boundary");
+  if (st != APT_OK) return st;
+  if (ctx->prec == APT_PREC_FP64)
+    HIPCHK(launch_ffe_boundary<double>(ctx->mesh, ctx->lp.damping_length,
+                                       ctx->lp.damping_coef,
+                                       field_set<double>(ctx), omega, ctx->stream));
+  else
+    HIPCHK(launch_ffe_boundary<float>(ctx->mesh, ctx->lp.damping_length,
+                                      ctx->lp.damping_coef,
+                                      field_set<float>(ctx), omega, ctx->stream));
+  return APT_OK;
+}
+
+template <typename T>
+static apt_status ffe_clean_t(apt_ctx *ctx, int mode) {
+  FfeBuf<T> b;
+  apt_status st = ffe_buffers<T>(ctx, b);
+  if (st != APT_OK) return st;
+  HIPCHK(launch_ffe_clean<T>(ctx->mesh, field_set<T>(ctx), b, mode, ctx->stream));
+  return APT_OK;
+}
+
+apt_status apt_ffe_clean_epar(apt_ctx *ctx) {
+  CHECK_CTX(ctx);
+  apt_status st = ffe_check(ctx, "ffe_clean_epar");
+  if (st != APT_OK) return st;
+  return ctx->prec == APT_PREC_FP64 ? ffe_clean_t<double>(ctx, 0)
+                                    : ffe_clean_t<float>(ctx, 0);
+}
+
+apt_status apt_ffe_check_egtb(apt_ctx *ctx) {
+  CHECK_CTX(ctx);
+  apt_status st = ffe_check(ctx, "ffe_check_egtb");
+  if (st != APT_OK) return st;
+  return ctx->prec == APT_PREC_FP64 ? ffe_clean_t<double>(ctx, 1)
+                                    : ffe_clean_t<float>(ctx, 1);
 }
 
 apt_status apt_emit_photons(apt_ctx *ctx, uint64_t *emitted) {

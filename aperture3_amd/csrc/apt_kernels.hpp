@@ -195,6 +195,25 @@ hipError_t launch_logsph_inject(const Mesh &m, const LsParams &lp,
                                 uint32_t epoch, int Hs, uint64_t id_base,
                                 long long *surf, hipStream_t st);
 
+// ---- 3D log-spherical force-free / vacuum solver (kern_ffe.hip) -------
+template <typename T>
+struct FfeBuf {
+  T *En[3], *Bn[3], *dE[3], *dB[3];
+};
+template <typename T>
+hipError_t launch_ffe_update(const Mesh &m, const LsCoef &cf, int damping_length,
+                             double damping_coef, const FieldSet<T> &f,
+                             const FfeBuf<T> &b, double dt, double omega,
+                             double time, hipStream_t st);
+template <typename T>
+hipError_t launch_ffe_boundary(const Mesh &m, int damping_length,
+                               double damping_coef, const FieldSet<T> &f,
+                               double omega, hipStream_t st);
+// mode 0: clean_epar (E -> dE), mode 1: check_eGTb (dE -> E)
+template <typename T>
+hipError_t launch_ffe_clean(const Mesh &m, const FieldSet<T> &f,
+                            const FfeBuf<T> &b, int mode, hipStream_t st);
+
 // ---- photons and pair creation (kern_rt.hip) ----------------------------
 // photon_data (src/data/particle_data.h:68-81); the mixed form recomputes
 // the photon energy from |p| and carries no id

@@ -242,6 +242,18 @@ apt_status apt_field_boundary(apt_ctx *ctx, double omega, double time);
  * ptc_updater_logsph.cu:848-870); log-spherical contexts only. */
 apt_status apt_inject_ptc(apt_ctx *ctx, int inj_per_cell, double p1,
                           double p2, double p3, double w, double omega);
+/* ffe_solver_logsph::update_fields(sim_data&, double dt, double omega,
+ * double time) (src/algorithms/ffe_solver_logsph.h:12-33,
+ * ffe_solver_logsph.cu:880-930): SSP-RK3 vacuum / force-free step of a 3D
+ * log-spherical context (N = r, theta, phi; guard >= 3; phi periodic). */
+apt_status apt_ffe_update(apt_ctx *ctx, double dt, double omega, double time);
+/* ffe_solver_logsph::apply_boundary (ffe_solver_logsph.cu:932-966) */
+apt_status apt_ffe_boundary(apt_ctx *ctx, double omega, double time);
+/* ffe_solver_logsph::clean_epar (:1029-1037): the solver's dE copy of E
+ * minus its component along B; check_eGTb (:1039-1047): E := that copy,
+ * scaled down where |E| > |B|. */
+apt_status apt_ffe_clean_epar(apt_ctx *ctx);
+apt_status apt_ffe_check_egtb(apt_ctx *ctx);
 /* radiative_transfer::emit_photons(sim_data&) / produce_pairs(sim_data&)
  * (radiative_transfer.h:11-28, radiative_transfer.cu:198-301); the
  * optional outputs return the number of events. */

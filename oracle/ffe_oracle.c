@@ -251,14 +251,19 @@ static double stag_interp(const double *a, size_t idx, int in, int out,
 
 static const int ST_E[3] = {0x6, 0x5, 0x3}, ST_B[3] = {0x1, 0x2, 0x4};
 
+/* The midpoint interpolation reaches two cells down in each dimension;
+ * the reference's range (guard - 2) reads before the array when guard = 3,
+ * so clean_epar / check_eGTb start at max(guard - 2, 2). */
+#define FIX_LO(m, d) ((m)->guard[d] - FFE_SHIFT > 2 ? (m)->guard[d] - FFE_SHIFT : 2)
+
 /* ffe_clean_epar (:540-612): dE := E - (E.B) B / (B^2 + TINY), each
  * component at its own location (dE: 3 x ncell). */
 void orc_ffe_clean_epar(const orc_mesh *m, orc_fields *f, double *dE) {
   const int d0 = m->dims[0], d1 = m->dims[1];
   const size_t n = (size_t)m->dims[0] * m->dims[1] * m->dims[2];
-  for (int k = m->guard[2] - FFE_SHIFT; k < m->dims[2] - m->guard[2] + FFE_SHIFT; k++)
-    for (int j = m->guard[1] - FFE_SHIFT; j < m->dims[1] - m->guard[1] + FFE_SHIFT; j++)
-      for (int i = m->guard[0] - FFE_SHIFT; i < m->dims[0] - m->guard[0] + FFE_SHIFT; i++) {
+  for (int k = FIX_LO(m, 2); k < m->dims[2] - m->guard[2] + FFE_SHIFT; k++)
+    for (int j = FIX_LO(m, 1); j < m->dims[1] - m->guard[1] + FFE_SHIFT; j++)
+      for (int i = FIX_LO(m, 0); i < m->dims[0] - m->guard[0] + FFE_SHIFT; i++) {
         size_t ijk = IDX(i, j, k);
         for (int q = 0; q < 3; q++) {
           double ex = stag_interp(f->E[0], ijk, ST_E[0], ST_E[q], d0, d1);
@@ -283,9 +288,9 @@ void orc_ffe_clean_epar(const orc_mesh *m, orc_fields *f, double *dE) {
 void orc_ffe_check_egtb(const orc_mesh *m, orc_fields *f, double *dE) {
   const int d0 = m->dims[0], d1 = m->dims[1];
   const size_t n = (size_t)m->dims[0] * m->dims[1] * m->dims[2];
-  for (int k = m->guard[2] - FFE_SHIFT; k < m->dims[2] - m->guard[2] + FFE_SHIFT; k++)
-    for (int j = m->guard[1] - FFE_SHIFT; j < m->dims[1] - m->guard[1] + FFE_SHIFT; j++)
-      for (int i = m->guard[0] - FFE_SHIFT; i < m->dims[0] - m->guard[0] + FFE_SHIFT; i++) {
+  for (int k = FIX_LO(m, 2); k < m->dims[2] - m->guard[2] + FFE_SHIFT; k++)
+    for (int j = FIX_LO(m, 1); j < m->dims[1] - m->guard[1] + FFE_SHIFT; j++)
+      for (int i = FIX_LO(m, 0); i < m->dims[0] - m->guard[0] + FFE_SHIFT; i++) {
         size_t ijk = IDX(i, j, k);
         for (int q = 0; q < 3; q++) {
           double ex = stag_interp(f->E[0], ijk, ST_E[0], ST_E[q], d0, d1);
@@ -300,9 +305,9 @@ void orc_ffe_check_egtb(const orc_mesh *m, orc_fields *f, double *dE) {
           dE[(size_t)q * n + ijk] = temp * dE[(size_t)q * n + ijk];
         }
       }
-  for (int k = m->guard[2] - FFE_SHIFT; k < m->dims[2] - m->guard[2] + FFE_SHIFT; k++)
-    for (int j = m->guard[1] - FFE_SHIFT; j < m->dims[1] - m->guard[1] + FFE_SHIFT; j++)
-      for (int i = m->guard[0] - FFE_SHIFT; i < m->dims[0] - m->guard[0] + FFE_SHIFT; i++) {
+  for (int k = FIX_LO(m, 2); k < m->dims[2] - m->guard[2] + FFE_SHIFT; k++)
+    for (int j = FIX_LO(m, 1); j < m->dims[1] - m->guard[1] + FFE_SHIFT; j++)
+      for (int i = FIX_LO(m, 0); i < m->dims[0] - m->guard[0] + FFE_SHIFT; i++) {
         size_t ijk = IDX(i, j, k);
         for (int q = 0; q < 3; q++) f->E[q][ijk] = dE[(size_t)q * n + ijk];
       }

@@ -43,7 +43,7 @@ def load():
     lib.orc_atan2.restype = C.c_double
     lib.orc_atan2.argtypes = [C.c_double, C.c_double]
     for name in ("orc_logsph_ptc_update", "orc_logsph_inject", "orc_emit_photons",
-                 "orc_produce_pairs", "orc_sort_photons"):
+                 "orc_produce_pairs", "orc_sort_photons", "orc_inject_left"):
         getattr(lib, name).restype = C.c_uint64
     return lib
 
@@ -513,6 +513,23 @@ def _photons(self):
     return {k: self.ph[k][: self.ph_number].copy() for k, _ in self.PH_FIELDS}
 
 
+def _inject_left(self, n_pairs, theta, weight=1.0, epoch=0):
+    """Wall injection of the batched 1D ensemble (rt_oracle.c)."""
+    ov = C.c_uint64(0)
+    P = self._ptc_struct()
+    base = self.number
+    n = self.lib.orc_inject_left(C.byref(self.mesh), C.byref(P), C.c_int(n_pairs),
+                                 C.c_double(theta), C.c_double(weight),
+                                 C.c_uint64(self.seed), C.c_uint32(epoch),
+                                 C.c_uint64(self.id_next), C.c_uint64(self.capacity),
+                                 C.byref(ov))
+    self.number = int(n)
+    self.id_next += self.number - base
+    self.overflow += int(ov.value)
+    return self.number
+
+
+OracleSim.inject_left = _inject_left
 OracleSim._grow = _grow_ptc
 OracleSim._ph_struct = _ph_struct
 OracleSim.set_rt = _set_rt

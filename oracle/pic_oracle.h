@@ -196,6 +196,10 @@ void orc_move_photons(const orc_mesh *m, int logsph, double r_cutoff,
 void orc_clear_photon_guard_cells(const orc_mesh *m, orc_photons *ph);
 uint64_t orc_sort_photons(const orc_mesh *m, orc_photons *ph,
                           uint64_t *scratch);
+uint64_t orc_inject_left(const orc_mesh *m, orc_ptc *p, int n_pairs,
+                         double theta, double weight, uint64_t seed,
+                         uint32_t epoch, uint64_t id_base, uint64_t cap,
+                         uint64_t *overflow);
 
 /* --- diagnostics --- */
 void orc_energy_histogram(const orc_ptc *p, uint32_t *hist, int nbins,

@@ -361,3 +361,44 @@ uint64_t orc_sort_photons(const orc_mesh *m, orc_photons *ph,
   ph->number = kept;
   return kept;
 }
+
+/* Wall injection of the batched 1D ensemble (no reference counterpart; the
+ * workload's open left wall): n_pairs co-located e-/e+ pairs in the first
+ * bulk cell, offset 1 - u (Philox purpose 7, key (epoch << 16) | n),
+ * Maxwell-Juttner momenta at temperature theta (key 2^63 | (pair << 1) | s)
+ * with p1 turned inward.  Pairs that do not fit below `cap` are dropped and
+ * counted.  Returns the new particle count. */
+uint64_t orc_inject_left(const orc_mesh *m, orc_ptc *p, int n_pairs,
+                         double theta, double weight, uint64_t seed,
+                         uint32_t epoch, uint64_t id_base, uint64_t cap,
+                         uint64_t *overflow) {
+  uint64_t base = p->number, made = 0;
+  for (int n = 0; n < n_pairs; n++) {
+    if (base + 2 * made + 2 > cap) {
+      if (overflow) *overflow += (uint64_t)(n_pairs - n);
+      break;
+    }
+    uint64_t pair = ((uint64_t)epoch << 16) | (uint64_t)n;
+    double a[2];
+    orc_uniform2(seed, pair, 0, 7, a);
+    for (int s = 0; s < 2; s++) {
+      uint64_t o = base + 2 * made + s;
+      double u[3];
+      orc_sample_mj(theta, 0.0, seed, (1ull << 63) | (pair << 1) | (uint64_t)s, u);
+      p->cell[o] = (uint32_t)m->guard[0];
+      p->x1[o] = 1.0 - a[0];
+      p->x2[o] = 0.0;
+      p->x3[o] = 0.0;
+      p->p1[o] = fabs(u[0]);
+      p->p2[o] = u[1];
+      p->p3[o] = u[2];
+      p->E[o] = sqrt(1.0 + p->p1[o] * p->p1[o] + u[1] * u[1] + u[2] * u[2]);
+      p->w[o] = weight;
+      p->flag[o] = (uint32_t)s << 29;
+      p->id[o] = id_base + 2 * made + s;
+    }
+    made++;
+  }
+  p->number = base + 2 * made;
+  return p->number;
+}

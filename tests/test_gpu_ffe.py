@@ -1,0 +1,75 @@
+"""GPU parity of the 3D log-spherical force-free / vacuum solver
+(ffe_solver_logsph.cu) against the CPU oracle through the C ABI.
+
+Bar: FP64 bit-exact E, B and B0 after three SSP-RK3 steps with a rotating
+star, the absorbing layer and the axis conditions, and after clean_epar +
+check_eGTb; the mixed form (FP32 fields, FP64 coefficients) within a
+stated relative tolerance."""
+import math
+
+import numpy as np
+import pytest
+
+import aperture3_amd as apt
+import oracle_ctypes as oc
+
+pytestmark = pytest.mark.gpu
+
+N = (24, 20, 16)
+SIZE = (math.log(30.0), math.pi, 2.0 * math.pi)
+
+
+def setup(precision):
+    o = oc.FFEOracle(N, (3, 3, 3), SIZE, damping_length=6, damping_coef=0.05)
+    p = apt.SimParams(N=list(N), guard=[3, 3, 3], size=list(SIZE),
+                      periodic_boundary=[False, False, True], precision=precision,
+                      coord_system="LogSpherical", max_ptc_number=0,
+                      damping_length=6, damping_coef=0.05, num_species=1)
+    env = apt.sim_environment(p)
+    d = apt.sim_data(env)
+    rng = np.random.default_rng(21)
+    n = o.ncell
+    for c in range(3):
+        bg = rng.normal(size=n)
+        b = bg + rng.normal(0, 0.1, n)
+        e = rng.normal(0, 0.1, n)
+        for arr, key, which in ((bg, "Bbg", apt.FIELD_BBG), (b, "B", apt.FIELD_B),
+                                (e, "E", apt.FIELD_E)):
+            o.f[key][c][:] = arr
+            d.set_field(which, c, arr)
+    return o, env, d
+
+
+def test_ffe_fp64_bit_exact():
+    o, env, d = setup(apt.APT_PREC_FP64)
+    solver = apt.ffe_solver_logsph(env)
+    for s in range(3):
+        o.update(0.004, omega=0.1, time=s * 0.004)
+        solver.update_fields(d, 0.004, 0.1, s * 0.004)
+    for which, key in ((apt.FIELD_E, "E"), (apt.FIELD_B, "B"), (apt.FIELD_BBG, "Bbg")):
+        for c in range(3):
+            assert np.array_equal(d.get_field(which, c), o.f[key][c]), (key, c)
+    o.clean_epar()
+    o.check_egtb()
+    solver.clean_epar(d)
+    solver.check_eGTb(d)
+    for c in range(3):
+        assert np.array_equal(d.get_field(apt.FIELD_E, c), o.f["E"][c]), ("E clean", c)
+    for name in apt.COEF_NAMES:
+        assert np.array_equal(d.coefficient(name), o.coefficient(name)), name
+    d.close()
+
+
+def test_ffe_mixed_tolerance():
+    o, env, d = setup(apt.APT_PREC_MIXED)
+    solver = apt.ffe_solver_logsph(env)
+    for s in range(3):
+        o.update(0.004, omega=0.1, time=s * 0.004)
+        solver.update_fields(d, 0.004, 0.1, s * 0.004)
+    for which, key in ((apt.FIELD_E, "E"), (apt.FIELD_B, "B")):
+        for c in range(3):
+            ref, got = o.f[key][c], d.get_field(which, c)
+            fin = np.isfinite(ref)
+            scale = np.abs(ref[fin]).max()
+            assert np.abs(got[fin] - ref[fin]).max() <= 1e-5 * scale, (key, c)
+    d.close()
