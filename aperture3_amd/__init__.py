@@ -103,6 +103,20 @@ class AptCounters(C.Structure):
                 ("capacity_overflow", C.c_uint64), ("outside_tile", C.c_uint64)]
 
 
+class AptBatchInst(C.Structure):
+    _fields_ = [("N", C.c_int), ("guard", C.c_int), ("ppc", C.c_int), ("inj", C.c_int),
+                ("cap", C.c_uint64), ("ph_cap", C.c_uint64), ("cfl", C.c_double),
+                ("weight", C.c_double), ("theta", C.c_double), ("gamma_thr", C.c_double),
+                ("Bx", C.c_double)]
+
+
+class AptBatchParams(C.Structure):
+    _fields_ = [("precision", C.c_int), ("q_e", C.c_double), ("ion_mass", C.c_double),
+                ("seed", C.c_uint64), ("chunk", C.c_int), ("grid_mult", C.c_int),
+                ("sort_interval", C.c_int), ("photon_path", C.c_double),
+                ("e_frac", C.c_double), ("r_cutoff", C.c_double)]
+
+
 # every entry point declared in include/aperture_amd.h: name -> argtypes
 _P = C.c_void_p
 SIGNATURES = {
@@ -147,6 +161,31 @@ SIGNATURES = {
     "apt_reset_counters": [_P],
     "apt_timing_enable": [_P, C.c_int],
     "apt_timing_get": [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)],
+    "apt_batch_create": [C.POINTER(AptBatchInst), C.c_int, C.POINTER(AptBatchParams),
+                         C.POINTER(_P)],
+    "apt_batch_destroy": [_P],
+    "apt_batch_last_error": [_P],
+    "apt_batch_get_jexp": [_P, C.POINTER(C.c_int)],
+    "apt_batch_inst_seed": [_P, C.c_int, C.POINTER(C.c_uint64)],
+    "apt_batch_load": [_P, C.POINTER(AptBatchInst)],
+    "apt_batch_step": [_P, C.c_int],
+    "apt_batch_graph": [_P, C.c_int],
+    "apt_batch_sort": [_P],
+    "apt_batch_sync": [_P],
+    "apt_batch_counts": [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+    "apt_batch_get_counters": [_P, C.POINTER(AptCounters)],
+    "apt_batch_timing": [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                         C.POINTER(C.c_uint64)],
+    "apt_batch_get_particles": [_P, C.c_int, C.POINTER(AptParticles), C.c_uint64,
+                                C.POINTER(C.c_uint64)],
+    "apt_batch_set_particles": [_P, C.c_int, C.POINTER(AptParticles), C.c_uint64],
+    "apt_batch_get_photons": [_P, C.c_int, C.POINTER(AptPhotons), C.c_uint64,
+                              C.POINTER(C.c_uint64)],
+    "apt_batch_get_field": [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                            C.c_uint64],
+    "apt_batch_set_field": [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                            C.c_uint64],
+    "apt_batch_sample": [C.c_uint64, C.c_int, C.POINTER(AptBatchInst)],
 }
 
 _lib = None
@@ -165,7 +204,7 @@ def load_library(path: str = LIB_PATH):
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = args
-        fn.restype = C.c_char_p if name == "apt_last_error" else C.c_int
+        fn.restype = C.c_char_p if name.endswith("last_error") else C.c_int
     _lib = lib
     return lib
 
@@ -595,3 +634,172 @@ class ffe_solver_logsph:
 
     def check_eGTb(self, data: sim_data):
         data._check(data.lib.apt_ffe_check_egtb(data.ctx), "apt_ffe_check_egtb")
+
+
+# ---------------------------------------------------------------------------
+# batched ensemble of 1D gap boxes (no reference counterpart: the reference
+# runs one box per process, labs/1dgap/src/main.cpp:61-98)
+BATCH_INST_FIELDS = tuple(f for f, _ in AptBatchInst._fields_)
+
+
+def sample_batch(n: int, seed: int = 1):
+    """The ensemble workload draw (apt_batch_sample) as a list of dicts."""
+    lib = load_library()
+    arr = (AptBatchInst * n)()
+    st = lib.apt_batch_sample(seed, n, arr)
+    if st != 0:
+        raise AptError(f"apt_batch_sample: status {st}")
+    return [{f: getattr(a, f) for f in BATCH_INST_FIELDS} for a in arr]
+
+
+class batch_ensemble:
+    """n independent 1D open gap boxes advanced together on the GPU.
+
+    Each box runs the 1dgap loop (update_particles, clear_guard_cells,
+    update_fields, emit_photons, produce_pairs, sort every sort_interval)
+    with wall injection; box k uses its own seed (inst_seed(k))."""
+
+    def __init__(self, insts, precision=APT_PREC_MIXED, q_e=1.0, ion_mass=1.0, seed=1,
+                 chunk=256, grid_mult=8, sort_interval=20, photon_path=0.2,
+                 e_frac=0.1, r_cutoff=1e30, load=True):
+        self.lib = load_library()
+        self.n = len(insts)
+        self.insts = [dict(i) for i in insts]
+        self._specs = (AptBatchInst * self.n)()
+        for a, d in zip(self._specs, self.insts):
+            for f in BATCH_INST_FIELDS:
+                setattr(a, f, d[f])
+        self.params = AptBatchParams(precision=precision, q_e=q_e, ion_mass=ion_mass,
+                                     seed=seed, chunk=chunk, grid_mult=grid_mult,
+                                     sort_interval=sort_interval, photon_path=photon_path,
+                                     e_frac=e_frac, r_cutoff=r_cutoff)
+        self.precision = precision
+        h = _P()
+        st = self.lib.apt_batch_create(self._specs, self.n, C.byref(self.params), C.byref(h))
+        if st != 0:
+            raise AptError(f"apt_batch_create: status {st}")
+        self.b = h
+        if load:
+            self._check(self.lib.apt_batch_load(self.b, self._specs), "apt_batch_load")
+
+    def _check(self, st, what):
+        if st != 0:
+            msg = self.lib.apt_batch_last_error(self.b)
+            raise AptError(f"{what}: status {st}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.apt_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, nsteps=1):
+        self._check(self.lib.apt_batch_step(self.b, nsteps), "apt_batch_step")
+
+    def graph(self, on=True):
+        self._check(self.lib.apt_batch_graph(self.b, 1 if on else 0), "apt_batch_graph")
+
+    def sort(self):
+        self._check(self.lib.apt_batch_sort(self.b), "apt_batch_sort")
+
+    def sync(self):
+        self._check(self.lib.apt_batch_sync(self.b), "apt_batch_sync")
+
+    def jexp(self):
+        h = C.c_int()
+        self._check(self.lib.apt_batch_get_jexp(self.b, C.byref(h)), "apt_batch_get_jexp")
+        return h.value
+
+    def inst_seed(self, k):
+        s = C.c_uint64()
+        self._check(self.lib.apt_batch_inst_seed(self.b, k, C.byref(s)), "apt_batch_inst_seed")
+        return s.value
+
+    def counts(self):
+        a = np.zeros(self.n, np.uint64)
+        b = np.zeros(self.n, np.uint64)
+        self._check(self.lib.apt_batch_counts(self.b, a.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                              b.ctypes.data_as(C.POINTER(C.c_uint64))),
+                    "apt_batch_counts")
+        return a, b
+
+    def counters(self):
+        c = AptCounters()
+        self._check(self.lib.apt_batch_get_counters(self.b, C.byref(c)), "apt_batch_get_counters")
+        return {f: getattr(c, f) for f, _ in AptCounters._fields_}
+
+    def timing(self, on=-1):
+        sm, pm, n = C.c_double(), C.c_double(), C.c_uint64()
+        self._check(self.lib.apt_batch_timing(self.b, on, C.byref(sm), C.byref(pm),
+                                              C.byref(n)), "apt_batch_timing")
+        return sm.value, pm.value, n.value
+
+    def get_particles(self, k):
+        n = int(self.counts()[0][k])
+        out = {"cell": np.empty(n, np.uint32), "flag": np.empty(n, np.uint32),
+               "id": np.empty(n, np.uint64)}
+        for f in ("x1", "x2", "x3", "p1", "p2", "p3", "E", "w"):
+            out[f] = np.empty(n, np.float64)
+        v = AptParticles()
+        v.cell = out["cell"].ctypes.data_as(C.POINTER(C.c_uint32))
+        v.flag = out["flag"].ctypes.data_as(C.POINTER(C.c_uint32))
+        v.id = out["id"].ctypes.data_as(C.POINTER(C.c_uint64))
+        for f in ("x1", "x2", "x3", "p1", "p2", "p3", "E"):
+            setattr(v, f, _dptr(out[f]))
+        v.weight = _dptr(out["w"])
+        got = C.c_uint64()
+        self._check(self.lib.apt_batch_get_particles(self.b, k, C.byref(v), n, C.byref(got)),
+                    "apt_batch_get_particles")
+        return out
+
+    def set_particles(self, k, ptc: dict):
+        n = len(ptc["cell"])
+        keep = {}
+
+        def arr(name, dt, ct):
+            a = np.ascontiguousarray(ptc[name], dtype=dt)
+            keep[name] = a
+            return a.ctypes.data_as(C.POINTER(ct))
+        v = AptParticles()
+        v.cell = arr("cell", np.uint32, C.c_uint32)
+        for f in ("x1", "x2", "x3", "p1", "p2", "p3"):
+            setattr(v, f, arr(f, np.float64, C.c_double))
+        v.E = arr("E", np.float64, C.c_double) if "E" in ptc else None
+        v.weight = arr("w", np.float64, C.c_double)
+        v.flag = arr("flag", np.uint32, C.c_uint32)
+        v.id = arr("id", np.uint64, C.c_uint64) if "id" in ptc else None
+        self._check(self.lib.apt_batch_set_particles(self.b, k, C.byref(v), n),
+                    "apt_batch_set_particles")
+
+    def get_photons(self, k):
+        n = int(self.counts()[1][k])
+        out = {k2: np.empty(n, dt) for k2, dt in sim_data.PH_FIELDS}
+        v = AptPhotons()
+        v.cell = out["cell"].ctypes.data_as(C.POINTER(C.c_uint32))
+        for f in ("x1", "x2", "x3", "p1", "p2", "p3", "E"):
+            setattr(v, f, _dptr(out[f]))
+        v.weight = _dptr(out["w"])
+        v.path_left = _dptr(out["path"])
+        v.flag = out["flag"].ctypes.data_as(C.POINTER(C.c_uint32))
+        v.id = out["id"].ctypes.data_as(C.POINTER(C.c_uint64))
+        got = C.c_uint64()
+        self._check(self.lib.apt_batch_get_photons(self.b, k, C.byref(v), n, C.byref(got)),
+                    "apt_batch_get_photons")
+        return out
+
+    def get_field(self, k, which, comp):
+        n = self.insts[k]["N"] + 2 * self.insts[k]["guard"]
+        a = np.empty(n, np.float64)
+        self._check(self.lib.apt_batch_get_field(self.b, k, which, comp, _dptr(a), n),
+                    "apt_batch_get_field")
+        return a
+
+    def set_field(self, k, which, comp, arr):
+        a = np.ascontiguousarray(arr, np.float64)
+        self._check(self.lib.apt_batch_set_field(self.b, k, which, comp, _dptr(a), a.size),
+                    "apt_batch_set_field")

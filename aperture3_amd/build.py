@@ -33,6 +33,7 @@ HIP_SOURCES = [
     "kern_logsph.hip",
     "kern_rt.hip",
     "kern_ffe.hip",
+    "kern_batch.hip",
     "apt_context.hip",
 ]
 HOST_SOURCES = ["host/aperture_host.cpp"]
@@ -75,6 +76,7 @@ def _run(cmd: list[str]) -> None:
 FILE_FLAGS = {
     "kern_tile_mixed.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "kern_cart_mixed.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "kern_batch.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }
 
 

@@ -1,0 +1,1593 @@
+// kern_batch.hip -- the batched ensemble: many independent 1D gap boxes of
+// uneven sizes advanced together (the loop of labs/1dgap/src/main.cpp:61-98
+// with the Cartesian 1D physics of ptc_updater.cu:21-499 and
+// field_solver.cu:14-358, open walls, wall injection and pair creation).
+//
+// Layout: every instance owns a contiguous slot range of one particle SoA
+// (base, cap), of one photon SoA and of the field arrays (fbase, dims).
+// Per step, all instances at once:
+//   plan      one workgroup turns the per-instance particle counts (device
+//             memory) into a chunk table: CH particles per chunk;
+//   push      ONE persistent launch: workgroups pull chunks from a device
+//             work counter until the table is exhausted (every wave reaches
+//             the exit); push + move + deposit per particle, open-wall
+//             removal, and the emit decision with a per-chunk count;
+//   fields    finalize J, B update, E update over every instance's cells;
+//   inject    wall pairs appended at each instance's count;
+//   emit      chunk counts -> 3-kernel scan -> photons written in slot order;
+//   photons   move, open-wall removal, convert decision -> scan -> pairs.
+// No step reads a count back to the host, no memset is issued (fills are
+// kernels), so the step is captured once in a hipGraph and replayed.
+// R = double reproduces the single-box CPU restatement bit for bit; R = float
+// is the mixed form.
+#include <hipcub/hipcub.hpp>
+#include <string.h>
+
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/aperture_amd.h"
+#include "apt_common.hpp"
+#include "apt_f32.hpp"
+#include "apt_f64.hpp"
+#include "apt_kernels.hpp"
+#include "apt_math.hpp"
+#include "apt_mj.hpp"
+#include "apt_sort.hpp"
+
+namespace apt {
+
+struct BInst {
+  Mesh m;
+  double dt;
+  uint64_t base, cap;        // particle slots
+  uint64_t ph_base, ph_cap;  // photon slots
+  uint64_t fbase;            // first field cell
+  uint64_t boff;             // first bulk cell in the ensemble's bulk list
+  uint64_t seed;
+  double gamma_thr, theta, weight;
+  int inj;
+};
+
+// per-instance device counters
+struct BCount {
+  unsigned long long number, ph_number, id_next, ph_id_next;
+};
+
+struct BPlan {
+  uint64_t *chunk_off;     // n + 1: first chunk of each instance (particles)
+  uint64_t *ph_chunk_off;  // n + 1: same for photons
+  unsigned long long *work;  // [0] particle chunks, [1] photon chunks
+};
+
+__device__ inline int find_inst(const uint64_t *off, int n, uint64_t x) {
+  int lo = 0, hi = n;  // off[lo] <= x < off[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (off[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// chunk tables from the device counts; resets the work counters
+__global__ void k_b_plan(const BCount *cnt, int n, uint32_t CH, BPlan plan) {
+  if (threadIdx.x != 0) return;
+  uint64_t a = 0, b = 0;
+  for (int i = 0; i < n; i++) {
+    plan.chunk_off[i] = a;
+    plan.ph_chunk_off[i] = b;
+    a += (cnt[i].number + CH - 1) / CH;
+    b += (cnt[i].ph_number + CH - 1) / CH;
+  }
+  plan.chunk_off[n] = a;
+  plan.ph_chunk_off[n] = b;
+  plan.work[0] = 0;
+  plan.work[1] = 0;
+}
+
+template <typename R>
+struct BFields {
+  R *E[3], *B[3];
+  const R *Ebg[3], *Bbg[3];
+  R *J[3];
+  long long *acc[3][2];  // FP64: two limbs; FP32: [c][0]
+};
+
+template <typename R>
+__device__ inline R gather1R(const R *__restrict__ f, int c, R x, int st) {
+  if constexpr (std::is_same<R, double>::value) {
+    return gather1(f, x, c, st);
+  } else {
+    int o;
+    float w0, w1;
+    taps(x, st & 1, o, w0, w1);
+    return f[c + o] * w0 + f[c + o + 1] * w1;
+  }
+}
+
+// ---------------------------------------------------------------------
+// the persistent push (ptc_push_cart_1d + deposit_current_cart_1d)
+// ---------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(1024)
+    k_b_push(const BInst *__restrict__ inst, int n, const BCount *__restrict__ cnt,
+             BPlan plan, Species sp, int H, PtcView<R> p, BFields<R> f,
+             uint8_t *__restrict__ eflag, uint32_t *__restrict__ echunk,
+             unsigned long long *__restrict__ counters) {
+  __shared__ unsigned long long s_chunk;
+  __shared__ uint32_t s_cnt;
+  const uint64_t total = plan.chunk_off[n];
+  const uint32_t CH = blockDim.x;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      s_chunk = atomicAdd(&plan.work[0], 1ull);
+      s_cnt = 0;
+    }
+    __syncthreads();
+    const uint64_t chunk = s_chunk;
+    if (chunk >= total) break;  // uniform: every wave of the group exits
+    const int k = find_inst(plan.chunk_off, n, chunk);
+    const BInst &I = inst[k];
+    const uint64_t local = (chunk - plan.chunk_off[k]) * CH + threadIdx.x;
+    int emit = 0;
+    if (local < cnt[k].number) {
+      const uint64_t o = I.base + local;
+      const Mesh &m = I.m;
+      const uint32_t c = p.cell[o];
+      eflag[o] = 0;
+      if (c != MAX_CELL) {
+        const int c1 = (int)c;
+        if (!(c1 >= m.guard[0] && c1 < m.dims[0] - m.guard[0])) {
+          p.cell[o] = MAX_CELL;
+        } else {
+          const uint32_t flag = p.flag[o];
+          const int s = ptc_species(flag);
+          const R dt = (R)I.dt;
+          R x1 = p.x1[o], x2 = p.x2[o], x3 = p.x3[o];
+          R p1 = p.p1[o], p2 = p.p2[o], p3 = p.p3[o];
+          if (p1 != p1 || p2 != p2 || p3 != p3) atomicAdd(&counters[CTR_NAN], 1ull);
+          const R q_over_m = (R)(I.dt * 0.5 * sp.q[s] / sp.m[s]);
+          R gamma = sqrt((R)1 + p1 * p1 + p2 * p2 + p3 * p3);
+          const R *E1 = f.E[0] + I.fbase, *E2 = f.E[1] + I.fbase, *E3 = f.E[2] + I.fbase;
+          const R *B1 = f.B[0] + I.fbase, *B2 = f.B[1] + I.fbase, *B3 = f.B[2] + I.fbase;
+          if (!(flag & FLAG_IGNORE_EM)) {
+            R e1 = gather1R(E1, c1, x1, ST_E1) * q_over_m;
+            R e2 = gather1R(E2, c1, x1, ST_E2) * q_over_m;
+            R e3 = gather1R(E3, c1, x1, ST_E3) * q_over_m;
+            R b1 = gather1R(B1, c1, x1, ST_B1) * q_over_m;
+            R b2 = gather1R(B2, c1, x1, ST_B2) * q_over_m;
+            R b3 = gather1R(B3, c1, x1, ST_B3) * q_over_m;
+            if constexpr (std::is_same<R, double>::value) {
+              vay_push(p1, p2, p3, gamma, e1, e2, e3, b1, b2, b3);
+            } else {
+              vay_pushf(p1, p2, p3, gamma, e1, e2, e3, b1, b2, b3);
+              gamma = sqrtf(1.0f + p1 * p1 + p2 * p2 + p3 * p3);
+            }
+            p.p1[o] = p1;
+            p.p2[o] = p2;
+            p.p3[o] = p3;
+            if (p.E) p.E[o] = gamma;
+          } else if (p.E) {
+            gamma = p.E[o];
+          }
+          const R v1 = p1 / gamma, v2 = p2 / gamma, v3 = p3 / gamma;
+          const R w = p.w[o];
+          const R old_x1 = x1;
+          R new_x1 = old_x1 + (v1 * dt) / (R)m.delta[0];
+          int dc1 = (int)floor(new_x1);
+          if (dc1 > 1 || dc1 < -1) {
+            atomicAdd(&counters[CTR_MULTICELL], 1ull);
+            dc1 = dc1 > 1 ? 1 : -1;
+          }
+          new_x1 -= (R)dc1;
+          if constexpr (!std::is_same<R, double>::value)
+            if (new_x1 >= 1.0f) new_x1 = 0x1.fffffep-1f;
+          const int nc = c1 + dc1;
+          p.cell[o] = (uint32_t)nc;
+          p.x1[o] = new_x1;
+          p.x2[o] = x2 + v2 * dt;
+          p.x3[o] = x3 + v3 * dt;
+          if (!(flag & FLAG_IGNORE_CURRENT)) {
+            const R weight = (R)sp.q[s] * w;
+            const int i0 = (dc1 == -1 ? -2 : -1), i1 = (dc1 == 1 ? 1 : 0);
+            R djx = (R)0;
+            const uint64_t fb = I.fbase;
+            for (int i = i0; i <= i1; i++) {
+              R sx0, sx1;
+              if constexpr (std::is_same<R, double>::value) {
+                sx0 = cic(-old_x1 + i + 1);
+                sx1 = cic(-new_x1 + (i + 1 - dc1));
+              } else {
+                sx0 = cicf(-old_x1 + (float)(i + 1));
+                sx1 = cicf(-new_x1 + (float)(i + 1 - dc1));
+              }
+              djx += sx1 - sx0;
+              const uint64_t o1 = fb + (uint64_t)(i + c1 + 1), o0 = fb + (uint64_t)(i + c1);
+              const R val1 = (R)0.5 * (sx0 + sx1);
+              if constexpr (std::is_same<R, double>::value) {
+                acc_add(f.acc[0][0], f.acc[0][1], o1, -weight * djx, H);
+                acc_add(f.acc[1][0], f.acc[1][1], o0, weight * v2 * val1, H);
+                acc_add(f.acc[2][0], f.acc[2][1], o0, weight * v3 * val1, H);
+              } else {
+                const float scale = ldexpf(1.0f, H);
+                acc_fx(f.acc[0][0], o1, -weight * djx, scale);
+                acc_fx(f.acc[1][0], o0, weight * v2 * val1, scale);
+                acc_fx(f.acc[2][0], o0, weight * v3 * val1, scale);
+              }
+            }
+          }
+          // open walls: a particle that left the bulk is removed (the
+          // clear_guard_cells after the update, labs/1dgap main.cpp:68)
+          if (!(nc >= m.guard[0] && nc < m.dims[0] - m.guard[0])) {
+            p.cell[o] = MAX_CELL;
+          } else if (s != 2 && gamma > (R)I.gamma_thr) {
+            eflag[o] = 1;
+            emit = 1;
+          }
+        }
+      }
+    }
+    // per-chunk emitter count (slot order inside the chunk is recovered
+    // by the emit pass)
+    unsigned long long bal = __ballot(emit);
+    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(&s_cnt, (uint32_t)__popcll(bal));
+    __syncthreads();
+    if (threadIdx.x == 0) echunk[chunk] = s_cnt;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------
+// scan of chunk counts (3 passes, no memsets; exclusive, in place into out)
+// ---------------------------------------------------------------------
+constexpr int SCAN_B = 1024;
+
+__global__ void k_scan_blocks(const uint32_t *__restrict__ in, uint64_t n,
+                              uint64_t *__restrict__ out,
+                              uint64_t *__restrict__ bsum) {
+  __shared__ uint64_t s[SCAN_B];
+  const uint64_t i = (uint64_t)blockIdx.x * SCAN_B + threadIdx.x;
+  s[threadIdx.x] = i < n ? in[i] : 0;
+  __syncthreads();
+  for (int d = 1; d < SCAN_B; d <<= 1) {
+    uint64_t v = threadIdx.x >= (unsigned)d ? s[threadIdx.x - d] : 0;
+    __syncthreads();
+    s[threadIdx.x] += v;
+    __syncthreads();
+  }
+  if (i < n) out[i] = s[threadIdx.x] - (i < n ? in[i] : 0);
+  if (threadIdx.x == SCAN_B - 1) bsum[blockIdx.x] = s[SCAN_B - 1];
+}
+
+// exclusive scan of the nb block sums in one workgroup of SCAN_B threads:
+// thread t owns a contiguous run of ceil(nb / SCAN_B) sums; bsum[nb] = total
+__global__ void __launch_bounds__(SCAN_B)
+    k_scan_sums(uint64_t *__restrict__ bsum, uint64_t nb) {
+  __shared__ uint64_t s[SCAN_B];
+  const uint64_t per = (nb + SCAN_B - 1) / SCAN_B;
+  const uint64_t lo = threadIdx.x * per;
+  const uint64_t hi = lo + per < nb ? lo + per : nb;
+  uint64_t run = 0;
+  for (uint64_t b = lo; b < hi; b++) run += bsum[b];
+  s[threadIdx.x] = run;
+  __syncthreads();
+  for (int d = 1; d < SCAN_B; d <<= 1) {
+    uint64_t v = threadIdx.x >= (unsigned)d ? s[threadIdx.x - d] : 0;
+    __syncthreads();
+    s[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint64_t a = s[threadIdx.x] - run;  // exclusive prefix of this run
+  for (uint64_t b = lo; b < hi; b++) {
+    const uint64_t v = bsum[b];
+    bsum[b] = a;
+    a += v;
+  }
+  if (threadIdx.x == SCAN_B - 1) bsum[nb] = s[SCAN_B - 1];
+}
+
+__global__ void k_scan_add(uint64_t *__restrict__ out, uint64_t n,
+                           const uint64_t *__restrict__ bsum) {
+  const uint64_t i = (uint64_t)blockIdx.x * SCAN_B + threadIdx.x;
+  if (i < n) out[i] += bsum[blockIdx.x];
+}
+
+// ---------------------------------------------------------------------
+// fields: finalize J, then the 1D Yee updates (field_solver.cu:14-235)
+// ---------------------------------------------------------------------
+template <typename R>
+__global__ void k_b_finalize(BFields<R> f, uint64_t ncell, int H, int limbs) {
+  const uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= ncell) return;
+  for (int c = 0; c < 3; c++) {
+    long long *hi = f.acc[c][0];
+    if (limbs == 2) {
+      long long *lo = f.acc[c][1];
+      f.J[c][o] = (R)fx_value(hi[o], lo[o], H);
+      lo[o] = 0;
+    } else {
+      f.J[c][o] = (R)ldexp((double)hi[o], -H);
+    }
+    hi[o] = 0;
+  }
+}
+
+template <typename R>
+__global__ void k_b_update_e(const BInst *__restrict__ inst, int n,
+                             const uint64_t *__restrict__ boff, uint64_t nbulk,
+                             BFields<R> f, double factor) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbulk) return;
+  const int k = find_inst(boff, n, t);
+  const BInst &I = inst[k];
+  const int i = I.m.guard[0] + (int)(t - boff[k]);
+  const uint64_t o = I.fbase + (uint64_t)i;
+  const R dt = (R)(factor * I.dt);
+  const R i0 = (R)I.m.inv_delta[0];
+  const R *b2 = f.B[1], *b3 = f.B[2], *g2 = f.Bbg[1], *g3 = f.Bbg[2];
+  f.E[0][o] -= dt * f.J[0][o];
+  f.E[1][o] += dt * ((b3[o] - b3[o + 1] - g3[o] + g3[o + 1]) * i0 - f.J[1][o]);
+  f.E[2][o] += dt * ((b2[o + 1] - b2[o] - g2[o + 1] + g2[o]) * i0 - f.J[2][o]);
+}
+
+template <typename R>
+__global__ void k_b_update_b(const BInst *__restrict__ inst, int n,
+                             const uint64_t *__restrict__ boff, uint64_t nbulk,
+                             BFields<R> f) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbulk) return;
+  const int k = find_inst(boff, n, t);
+  const BInst &I = inst[k];
+  const int i = I.m.guard[0] + (int)(t - boff[k]);
+  const uint64_t o = I.fbase + (uint64_t)i;
+  const R dt = (R)I.dt;
+  const R i0 = (R)I.m.inv_delta[0];
+  const R *e2 = f.E[1], *e3 = f.E[2], *g2 = f.Ebg[1], *g3 = f.Ebg[2];
+  f.B[1][o] -= dt * (e3[o - 1] - e3[o] - g3[o - 1] + g3[o]) * i0;
+  f.B[2][o] -= dt * (e2[o] - e2[o - 1] - g2[o] + g2[o - 1]) * i0;
+}
+
+// ---------------------------------------------------------------------
+// wall injection (the ensemble's open left wall)
+// ---------------------------------------------------------------------
+template <typename R>
+__global__ void k_b_inject(const BInst *__restrict__ inst, int n,
+                           const uint64_t *__restrict__ ioff, uint64_t ntot,
+                           const BCount *__restrict__ cnt, PtcView<R> p,
+                           const uint32_t *__restrict__ d_epoch) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntot) return;
+  const uint32_t epoch = *d_epoch;
+  const int k = find_inst(ioff, n, t);
+  const BInst &I = inst[k];
+  const int nn = (int)(t - ioff[k]);
+  const uint64_t base = cnt[k].number;
+  if (base + 2 * (uint64_t)nn + 2 > I.cap) return;  // counted on update
+  const uint64_t pair = ((uint64_t)epoch << 16) | (uint64_t)nn;
+  double a0, a1;
+  uniform2(I.seed, pair, 0, 7, a0, a1);
+  for (int s = 0; s < 2; s++) {
+    const uint64_t o = I.base + base + 2 * (uint64_t)nn + s;
+    double u0, u1, u2;
+    sample_mj(I.theta, 0.0, I.seed, (1ull << 63) | (pair << 1) | (uint64_t)s, u0, u1, u2);
+    const double up1 = fabs(u0);
+    R x1 = (R)(1.0 - a0);
+    if constexpr (!std::is_same<R, double>::value)
+      if (x1 >= 1.0f) x1 = 0x1.fffffep-1f;
+    p.cell[o] = (uint32_t)I.m.guard[0];
+    p.x1[o] = x1;
+    p.x2[o] = (R)0;
+    p.x3[o] = (R)0;
+    p.p1[o] = (R)up1;
+    p.p2[o] = (R)u1;
+    p.p3[o] = (R)u2;
+    if (p.E) p.E[o] = (R)sqrt(1.0 + up1 * up1 + u1 * u1 + u2 * u2);
+    p.w[o] = (R)I.weight;
+    p.flag[o] = (uint32_t)s << 29;
+    if (p.id) p.id[o] = cnt[k].id_next + 2 * (uint64_t)nn + s;
+  }
+}
+
+__global__ void k_b_inject_count(const BInst *__restrict__ inst, int n,
+                                 BCount *__restrict__ cnt,
+                                 unsigned long long *__restrict__ counters) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const BInst &I = inst[k];
+  const uint64_t room = I.cap > cnt[k].number ? (I.cap - cnt[k].number) / 2 : 0;
+  const uint64_t made = (uint64_t)I.inj < room ? (uint64_t)I.inj : room;
+  if (made < (uint64_t)I.inj)
+    atomicAdd(&counters[CTR_OVERFLOW], (unsigned long long)(I.inj - made));
+  cnt[k].number += 2 * made;
+  cnt[k].id_next += 2 * made;
+}
+
+// ---------------------------------------------------------------------
+// emission (radiative_transfer.cu:28-96, exponential free-path model)
+// ---------------------------------------------------------------------
+template <typename R>
+__global__ void k_b_emit(const BInst *__restrict__ inst, int n,
+                         const BCount *__restrict__ cnt, BPlan plan,
+                         const uint8_t *__restrict__ eflag,
+                         const uint64_t *__restrict__ epre, PtcView<R> p,
+                         PhoView<R> ph, double photon_path, double e_frac,
+                         double r_cutoff, const uint32_t *__restrict__ d_epoch,
+                         unsigned long long *__restrict__ counters) {
+  const uint32_t epoch = *d_epoch;
+  const uint64_t chunk = blockIdx.x;
+  const uint64_t total = plan.chunk_off[n];
+  if (chunk >= total) return;
+  const int k = find_inst(plan.chunk_off, n, chunk);
+  const BInst &I = inst[k];
+  const uint64_t local = (chunk - plan.chunk_off[k]) * blockDim.x + threadIdx.x;
+  const bool mine = local < cnt[k].number && eflag[I.base + local];
+  // rank inside the chunk: ballot prefix over the 64-lane waves
+  __shared__ uint32_t s_wave[16];
+  const unsigned long long bal = __ballot(mine);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) s_wave[wv] = (uint32_t)__popcll(bal);
+  __syncthreads();
+  uint32_t before = 0;
+  for (int q = 0; q < wv; q++) before += s_wave[q];
+  if (!mine) return;
+  before += (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+  const uint64_t krank = epre[chunk] - epre[plan.chunk_off[k]] + before;
+  const uint64_t slot_local = cnt[k].ph_number + krank;
+  if (slot_local >= I.ph_cap) {
+    atomicAdd(&counters[CTR_OVERFLOW], 1ull);
+    return;
+  }
+  const uint64_t o = I.base + local, so = I.ph_base + slot_local;
+  ph.cell[so] = MAX_CELL;
+  if (ph.id) ph.id[so] = cnt[k].ph_id_next + krank;
+  double u0, u1;
+  uniform2(I.seed, local, epoch, 5, u0, u1);
+  const R gamma = p.E ? p.E[o]
+                      : sqrt((R)1 + p.p1[o] * p.p1[o] + p.p2[o] * p.p2[o] +
+                             p.p3[o] * p.p3[o]);
+  const R p1 = p.p1[o], p2 = p.p2[o], p3 = p.p3[o];
+  const R pi = sqrt(gamma * gamma - (R)1);
+  const R Eph = (R)e_frac * gamma;
+  const R ge = gamma - Eph;
+  const R pf2 = ge * ge - (R)1;
+  const R pf = sqrt(pf2 > (R)0 ? pf2 : (R)0);
+  p.p1[o] = p1 * pf / pi;
+  p.p2[o] = p2 * pf / pi;
+  p.p3[o] = p3 * pf / pi;
+  if (p.E) p.E[o] = ge;
+  R path;
+  if constexpr (std::is_same<R, double>::value)
+    path = -(R)photon_path * apt_log(u1);
+  else
+    path = -(R)photon_path * logf((float)u1);
+  if (path > (R)r_cutoff) return;
+  ph.x1[so] = p.x1[o];
+  ph.x2[so] = p.x2[o];
+  ph.x3[so] = p.x3[o];
+  ph.p1[so] = Eph * p1 / pi;
+  ph.p2[so] = Eph * p2 / pi;
+  ph.p3[so] = Eph * p3 / pi;
+  if (ph.E) ph.E[so] = Eph;
+  ph.w[so] = p.w[o];
+  ph.path[so] = path;
+  ph.flag[so] = 0;
+  ph.cell[so] = p.cell[o];
+}
+
+// per-instance event totals from the chunk prefix -> counts
+__global__ void k_b_emit_count(const BInst *__restrict__ inst, int n,
+                               BCount *__restrict__ cnt, const uint64_t *off,
+                               const uint64_t *__restrict__ pre, int photons) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const BInst &I = inst[k];
+  const uint64_t ev = pre[off[k + 1]] - pre[off[k]];
+  if (!photons) {
+    const uint64_t room = I.ph_cap > cnt[k].ph_number ? I.ph_cap - cnt[k].ph_number : 0;
+    const uint64_t made = ev < room ? ev : room;
+    cnt[k].ph_number += made;
+    cnt[k].ph_id_next += made;
+  } else {
+    const uint64_t L = I.cap > cnt[k].number ? I.cap - cnt[k].number : 0;
+    const uint64_t made = ev < L / 2 ? ev : L / 2;
+    cnt[k].number += 2 * made;
+    cnt[k].id_next += 2 * made;
+  }
+}
+
+// ---------------------------------------------------------------------
+// photons: move (move_photons_cart_1d, ptc_updater.cu:627-656), open-wall
+// removal, convert decision with per-chunk counts, then the pairs
+// ---------------------------------------------------------------------
+template <typename R>
+__global__ void k_b_photon_move(const BInst *__restrict__ inst, int n,
+                                const BCount *__restrict__ cnt, BPlan plan,
+                                PhoView<R> ph) {
+  const uint64_t chunk = blockIdx.x;
+  if (chunk >= plan.ph_chunk_off[n]) return;
+  const int k = find_inst(plan.ph_chunk_off, n, chunk);
+  const BInst &I = inst[k];
+  const uint64_t local = (chunk - plan.ph_chunk_off[k]) * blockDim.x + threadIdx.x;
+  if (local >= cnt[k].ph_number) return;
+  const uint64_t t = I.ph_base + local;
+  const uint32_t c = ph.cell[t];
+  if (c == MAX_CELL) return;
+  const Mesh &m = I.m;
+  const R dt = (R)I.dt;
+  const R p1 = ph.p1[t], p2 = ph.p2[t], p3 = ph.p3[t];
+  const R E = sqrt(p1 * p1 + p2 * p2 + p3 * p3);
+  R nx = ph.x1[t] + p1 / E * dt * (R)m.inv_delta[0];
+  const int dc = (int)floor(nx);
+  nx -= (R)dc;
+  const int nc = (int)c + dc;
+  // a photon leaving the bulk is removed (photons.clear_guard_cells)
+  if (dc < -1 || dc > 1 || !(nc >= m.guard[0] && nc < m.dims[0] - m.guard[0])) {
+    ph.cell[t] = MAX_CELL;
+    return;
+  }
+  if constexpr (!std::is_same<R, double>::value)
+    if (nx >= 1.0f) nx = 0x1.fffffep-1f;
+  ph.cell[t] = (uint32_t)nc;
+  ph.x1[t] = nx;
+  ph.x2[t] = ph.x2[t] + p2 * dt / E;
+  ph.x3[t] = ph.x3[t] + p3 * dt / E;
+  ph.path[t] -= dt;
+}
+
+// check_produce_pair with per-chunk counts
+template <typename R>
+__global__ void __launch_bounds__(1024)
+    k_b_pair_decide(const BInst *__restrict__ inst, int n,
+                    const BCount *__restrict__ cnt, BPlan plan, PhoView<R> ph,
+                    uint8_t *__restrict__ pflag, uint32_t *__restrict__ pchunk) {
+  __shared__ uint32_t s_cnt;
+  const uint64_t chunk = blockIdx.x;
+  if (chunk >= plan.ph_chunk_off[n]) return;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  const int k = find_inst(plan.ph_chunk_off, n, chunk);
+  const BInst &I = inst[k];
+  const uint64_t local = (chunk - plan.ph_chunk_off[k]) * blockDim.x + threadIdx.x;
+  int conv = 0;
+  if (local < cnt[k].ph_number) {
+    const uint64_t t = I.ph_base + local;
+    const uint32_t c = ph.cell[t];
+    conv = c != MAX_CELL && (int)c >= I.m.guard[0] &&
+           (int)c < I.m.dims[0] - I.m.guard[0] && ph.path[t] <= (R)0;
+    pflag[t] = (uint8_t)conv;
+  }
+  unsigned long long bal = __ballot(conv);
+  if ((threadIdx.x & 63) == 0 && bal) atomicAdd(&s_cnt, (uint32_t)__popcll(bal));
+  __syncthreads();
+  if (threadIdx.x == 0) pchunk[chunk] = s_cnt;
+}
+
+__global__ void k_b_tick(uint32_t *d_epoch) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d_epoch += 1u;
+}
+
+template <typename R>
+__global__ void k_b_pairs(const BInst *__restrict__ inst, int n,
+                          const BCount *__restrict__ cnt, BPlan plan,
+                          const uint8_t *__restrict__ pflag,
+                          const uint64_t *__restrict__ ppre, PtcView<R> p,
+                          PhoView<R> ph, unsigned long long *__restrict__ counters) {
+  const uint64_t chunk = blockIdx.x;
+  if (chunk >= plan.ph_chunk_off[n]) return;
+  const int k = find_inst(plan.ph_chunk_off, n, chunk);
+  const BInst &I = inst[k];
+  const uint64_t local = (chunk - plan.ph_chunk_off[k]) * blockDim.x + threadIdx.x;
+  const bool mine = local < cnt[k].ph_number && pflag[I.ph_base + local];
+  __shared__ uint32_t s_wave[16];
+  const unsigned long long bal = __ballot(mine);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) s_wave[wv] = (uint32_t)__popcll(bal);
+  __syncthreads();
+  uint32_t before = 0;
+  for (int q = 0; q < wv; q++) before += s_wave[q];
+  if (!mine) return;
+  before += (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+  const uint64_t kr = ppre[chunk] - ppre[plan.ph_chunk_off[k]] + before;
+  const uint64_t t = I.ph_base + local;
+  const uint64_t off = cnt[k].number + 2 * kr;
+  if (off + 1 >= I.cap) {
+    atomicAdd(&counters[CTR_OVERFLOW], 1ull);
+    ph.cell[t] = MAX_CELL;
+    return;
+  }
+  const R p1 = ph.p1[t], p2 = ph.p2[t], p3 = ph.p3[t];
+  R E_ph2 = p1 * p1 + p2 * p2 + p3 * p3;
+  if (E_ph2 <= (R)4.01) E_ph2 = (R)4.01;
+  const R ratio = sqrt((R)0.25 - (R)1 / E_ph2);
+  const R gamma = sqrt((R)1 + ratio * ratio * E_ph2);
+  for (int s = 0; s < 2; s++) {
+    const uint64_t o = I.base + off + s;
+    p.x1[o] = ph.x1[t];
+    p.x2[o] = ph.x2[t];
+    p.x3[o] = ph.x3[t];
+    p.p1[o] = ratio * p1;
+    p.p2[o] = ratio * p2;
+    p.p3[o] = ratio * p3;
+    if (p.E) p.E[o] = gamma;
+    p.w[o] = ph.w[t];
+    p.flag[o] = FLAG_SECONDARY | ((uint32_t)s << 29);
+    if (p.id) p.id[o] = cnt[k].id_next + 2 * kr + s;
+    p.cell[o] = ph.cell[t];
+  }
+  ph.cell[t] = MAX_CELL;
+}
+
+// ---------------------------------------------------------------------
+// sort with compaction per instance: key = (instance << cbits) | cell
+// ---------------------------------------------------------------------
+__global__ void k_b_keys(const BInst *__restrict__ inst, int n,
+                         const uint64_t *__restrict__ slot_off,
+                         const BCount *__restrict__ cnt, int photons,
+                         const uint32_t *__restrict__ cell, uint64_t total,
+                         int cbits, uint32_t *__restrict__ keys,
+                         uint32_t *__restrict__ perm) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int k = find_inst(slot_off, n, t);
+  const uint64_t local = t - slot_off[k];
+  const uint64_t num = photons ? cnt[k].ph_number : cnt[k].number;
+  uint32_t c = local < num ? cell[t] : MAX_CELL;
+  const uint32_t empty = (1u << cbits) - 1u;
+  keys[t] = ((uint32_t)k << cbits) | (c == MAX_CELL ? empty : c);
+  perm[t] = (uint32_t)t;
+}
+
+__global__ void k_b_count_sorted(const BInst *__restrict__ inst, int n,
+                                 const uint64_t *__restrict__ slot_off,
+                                 BCount *__restrict__ cnt, int photons,
+                                 const uint32_t *__restrict__ keys, int cbits) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t empty = ((uint32_t)k << cbits) | ((1u << cbits) - 1u);
+  uint64_t lo = slot_off[k], hi = slot_off[k + 1];  // first key >= empty
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < empty) lo = mid + 1;
+    else hi = mid;
+  }
+  const uint64_t num = lo - slot_off[k];
+  if (photons) cnt[k].ph_number = num;
+  else cnt[k].number = num;
+}
+
+__global__ void k_b_clear_tail(int n, const uint64_t *__restrict__ slot_off,
+                               const BCount *__restrict__ cnt, int photons,
+                               uint32_t *__restrict__ cell, uint64_t total) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int k = find_inst(slot_off, n, t);
+  const uint64_t num = photons ? cnt[k].ph_number : cnt[k].number;
+  if (t - slot_off[k] >= num) cell[t] = MAX_CELL;
+}
+
+__global__ void k_u8_fill(uint8_t *a, uint64_t n) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = 0;
+}
+
+}  // namespace apt
+
+// ======================================================================
+// host object and C entry points
+// ======================================================================
+using namespace apt;
+
+struct apt_batch {
+  int n = 0;
+  int prec = APT_PREC_MIXED;
+  apt_batch_params params{};
+  std::vector<BInst> hinst;
+  BInst *d_inst = nullptr;
+  BCount *d_cnt = nullptr;
+  BPlan plan{};
+  uint64_t *d_boff = nullptr, *d_ioff = nullptr;     // bulk cells, injections
+  uint64_t *d_slot_off = nullptr, *d_ph_off = nullptr;  // particle / photon slots
+  uint64_t nbulk = 0, ninj = 0, ncell = 0, cap = 0, ph_cap = 0;
+  uint64_t max_chunks = 0, max_ph_chunks = 0;
+  Species sp{};
+  uint32_t *d_epoch = nullptr;  // step counter read by the kernels
+  unsigned push_grid = 0;       // persistent push workgroups
+  int H = 22;
+  int cbits = 15;
+  uint32_t step = 0;
+  hipStream_t stream = nullptr;
+  unsigned long long *d_counters = nullptr;
+  // particles / photons (second buffers for the sort)
+  PtcF64 p64{}, p64b{};
+  PtcF32 p32{}, p32b{};
+  PhoF64 h64{}, h64b{};
+  PhoF32 h32{}, h32b{};
+  // fields (R) and accumulators
+  void *E[3] = {}, *B[3] = {}, *Ebg[3] = {}, *Bbg[3] = {}, *J[3] = {};
+  long long *acc[3][2] = {};
+  // emission / pair bookkeeping
+  uint8_t *eflag = nullptr, *pflag = nullptr;
+  uint32_t *echunk = nullptr, *pchunk = nullptr;
+  uint64_t *epre = nullptr, *ppre = nullptr, *bsum = nullptr;
+  SortWork sortw;
+  // graphs: one per buffer parity (the sort swaps the particle and photon
+  // buffers, and a graph holds the pointers it was captured with)
+  bool use_graph = false;
+  int parity = 0;
+  hipGraph_t graph[2] = {};
+  hipGraphExec_t gexec[2] = {};
+  // timing: whole step (ev0, ev1) and the persistent push (evp0, evp1)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr;
+  double step_ms = 0.0, push_ms = 0.0;
+  uint64_t steps_timed = 0;
+  bool timing = false;
+  std::string err;
+
+  apt_status fail(apt_status code, const char *msg) {
+    err = msg;
+    return code;
+  }
+};
+
+#define BCHK(call)                                                         \
+  do {                                                                     \
+    hipError_t e_ = (call);                                                \
+    if (e_ != hipSuccess) {                                                \
+      b->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
+      return APT_ERR_HIP;                                                  \
+    }                                                                      \
+  } while (0)
+
+// blocks of t threads covering n items; at least one (a kernel guards its
+// own range, and a zero-block launch is an error)
+static inline unsigned bblocks(uint64_t n, int t = 256) {
+  const uint64_t g = (n + t - 1) / t;
+  return (unsigned)(g ? g : 1);
+}
+static inline unsigned at_least_one(uint64_t g) { return (unsigned)(g ? g : 1); }
+
+template <typename R>
+static BFields<R> bfields(apt_batch *b) {
+  BFields<R> f;
+  for (int c = 0; c < 3; c++) {
+    f.E[c] = (R *)b->E[c];
+    f.B[c] = (R *)b->B[c];
+    f.Ebg[c] = (const R *)b->Ebg[c];
+    f.Bbg[c] = (const R *)b->Bbg[c];
+    f.J[c] = (R *)b->J[c];
+    f.acc[c][0] = b->acc[c][0];
+    f.acc[c][1] = b->acc[c][1];
+  }
+  return f;
+}
+
+static apt_status scan_chunks(apt_batch *b, const uint32_t *in, uint64_t nmax,
+                              uint64_t *out) {
+  const uint64_t nb = (nmax + SCAN_B) / SCAN_B;  // covers index nmax
+  k_scan_blocks<<<(unsigned)nb, SCAN_B, 0, b->stream>>>(in, nmax + 1, out, b->bsum);
+  k_scan_sums<<<1, SCAN_B, 0, b->stream>>>(b->bsum, nb);
+  k_scan_add<<<(unsigned)nb, SCAN_B, 0, b->stream>>>(out, nmax + 1, b->bsum);
+  BCHK(hipGetLastError());
+  return APT_OK;
+}
+
+template <typename R>
+static apt_status batch_step_t(apt_batch *b, PtcView<R> p, PhoView<R> ph,
+                               bool first) {
+  const int n = b->n;
+  const uint32_t CH = (uint32_t)b->params.chunk;
+  const BFields<R> f = bfields<R>(b);
+  const int limbs = std::is_same<R, double>::value ? 2 : 1;
+  hipStream_t st = b->stream;
+  // 1. wall injection (before the push, as inject_ptc in the lab loops)
+  if (b->ninj > 0) {
+    k_b_inject<R><<<bblocks(b->ninj), 256, 0, st>>>(b->d_inst, n, b->d_ioff, b->ninj,
+                                                   b->d_cnt, p, b->d_epoch);
+    k_b_inject_count<<<bblocks(n), 256, 0, st>>>(b->d_inst, n, b->d_cnt, b->d_counters);
+  }
+  // 2. plan + the persistent push, then the photons' move
+  k_b_plan<<<1, 64, 0, st>>>(b->d_cnt, n, CH, b->plan);
+  if (b->timing) BCHK(hipEventRecord(b->evp0, st));
+  k_b_push<R><<<b->push_grid, CH, 0, st>>>(b->d_inst, n, b->d_cnt, b->plan, b->sp, b->H,
+                                           p, f, b->eflag, b->echunk, b->d_counters);
+  if (b->timing) BCHK(hipEventRecord(b->evp1, st));
+  k_b_photon_move<R><<<at_least_one(b->max_ph_chunks), CH, 0, st>>>(b->d_inst, n, b->d_cnt,
+                                                                b->plan, ph);
+  // 3. fields (field_solver::update_fields, step-0 half step first)
+  k_b_finalize<R><<<bblocks(b->ncell), 256, 0, st>>>(f, b->ncell, b->H, limbs);
+  if (first)
+    k_b_update_e<R><<<bblocks(b->nbulk), 256, 0, st>>>(b->d_inst, n, b->d_boff,
+                                                      b->nbulk, f, 0.5);
+  k_b_update_b<R><<<bblocks(b->nbulk), 256, 0, st>>>(b->d_inst, n, b->d_boff, b->nbulk,
+                                                    f);
+  k_b_update_e<R><<<bblocks(b->nbulk), 256, 0, st>>>(b->d_inst, n, b->d_boff, b->nbulk,
+                                                    f, 1.0);
+  // 4. emission: chunk counts -> prefix -> photons in slot order
+  apt_status s = scan_chunks(b, b->echunk, b->max_chunks, b->epre);
+  if (s != APT_OK) return s;
+  k_b_emit<R><<<at_least_one(b->max_chunks), CH, 0, st>>>(
+      b->d_inst, n, b->d_cnt, b->plan, b->eflag, b->epre, p, ph, b->params.photon_path,
+      b->params.e_frac, b->params.r_cutoff, b->d_epoch, b->d_counters);
+  k_b_emit_count<<<bblocks(n), 256, 0, st>>>(b->d_inst, n, b->d_cnt, b->plan.chunk_off,
+                                            b->epre, 0);
+  // 5. pairs from every photon with no path left (new ones included)
+  k_b_plan<<<1, 64, 0, st>>>(b->d_cnt, n, CH, b->plan);
+  k_b_pair_decide<R><<<at_least_one(b->max_ph_chunks), CH, 0, st>>>(b->d_inst, n, b->d_cnt,
+                                                                b->plan, ph, b->pflag,
+                                                                b->pchunk);
+  s = scan_chunks(b, b->pchunk, b->max_ph_chunks, b->ppre);
+  if (s != APT_OK) return s;
+  k_b_pairs<R><<<at_least_one(b->max_ph_chunks), CH, 0, st>>>(
+      b->d_inst, n, b->d_cnt, b->plan, b->pflag, b->ppre, p, ph, b->d_counters);
+  k_b_emit_count<<<bblocks(n), 256, 0, st>>>(b->d_inst, n, b->d_cnt,
+                                            b->plan.ph_chunk_off, b->ppre, 1);
+  k_b_tick<<<1, 1, 0, st>>>(b->d_epoch);
+  BCHK(hipGetLastError());
+  return APT_OK;
+}
+
+static apt_status batch_run_step(apt_batch *b, bool first) {
+  if (b->prec == APT_PREC_FP64)
+    return batch_step_t<double>(b, view_of(b->p64), view_of(b->h64), first);
+  return batch_step_t<float>(b, view_of(b->p32), view_of(b->h32), first);
+}
+
+template <typename R, typename PT>
+static apt_status sort_gather(apt_batch *b, R **a, R **bb, int nr, uint32_t *ca,
+                              uint32_t *cb, uint32_t *fa, uint32_t *fb, uint64_t *ia,
+                              uint64_t *ib, uint64_t total) {
+  SortWork &w = b->sortw;
+  BCHK(gather_field<uint32_t>(w, ca, cb, total, b->stream));
+  for (int q = 0; q < nr; q++) BCHK(gather_field<R>(w, a[q], bb[q], total, b->stream));
+  BCHK(gather_field<uint32_t>(w, fa, fb, total, b->stream));
+  if (ia) BCHK(gather_field<uint64_t>(w, ia, ib, total, b->stream));
+  return APT_OK;
+}
+
+static apt_status batch_sort(apt_batch *b, int photons) {
+  const uint64_t total = photons ? b->ph_cap : b->cap;
+  if (total == 0) return APT_OK;
+  SortWork &w = b->sortw;
+  BCHK(w.ensure(total));
+  const uint32_t *cell = photons ? (b->prec == APT_PREC_FP64 ? b->h64.cell : b->h32.cell)
+                                 : (b->prec == APT_PREC_FP64 ? b->p64.cell : b->p32.cell);
+  const uint64_t *off = photons ? b->d_ph_off : b->d_slot_off;
+  k_b_keys<<<bblocks(total), 256, 0, b->stream>>>(b->d_inst, b->n, off, b->d_cnt, photons,
+                                                 cell, total, b->cbits, w.keys_in,
+                                                 w.perm_in);
+  int ibits = 1;
+  while ((1 << ibits) < b->n) ibits++;
+  BCHK(hipcub_sort_pairs(w, total, b->cbits + ibits, b->stream));
+  apt_status st;
+  if (!photons && b->prec == APT_PREC_FP64) {
+    PtcF64 &a = b->p64, &c = b->p64b;
+    double *sa[] = {a.x1, a.x2, a.x3, a.p1, a.p2, a.p3, a.E, a.w};
+    double *sc[] = {c.x1, c.x2, c.x3, c.p1, c.p2, c.p3, c.E, c.w};
+    st = sort_gather<double, PtcF64>(b, sa, sc, 8, a.cell, c.cell, a.flag, c.flag, a.id,
+                                     c.id, total);
+    std::swap(b->p64, b->p64b);
+  } else if (!photons) {
+    PtcF32 &a = b->p32, &c = b->p32b;
+    float *sa[] = {a.x1, a.x2, a.x3, a.u1, a.u2, a.u3, a.w};
+    float *sc[] = {c.x1, c.x2, c.x3, c.u1, c.u2, c.u3, c.w};
+    st = sort_gather<float, PtcF32>(b, sa, sc, 7, a.cell, c.cell, a.flag, c.flag, nullptr,
+                                    nullptr, total);
+    std::swap(b->p32, b->p32b);
+  } else if (b->prec == APT_PREC_FP64) {
+    PhoF64 &a = b->h64, &c = b->h64b;
+    double *sa[] = {a.x1, a.x2, a.x3, a.p1, a.p2, a.p3, a.E, a.w, a.path};
+    double *sc[] = {c.x1, c.x2, c.x3, c.p1, c.p2, c.p3, c.E, c.w, c.path};
+    st = sort_gather<double, PhoF64>(b, sa, sc, 9, a.cell, c.cell, a.flag, c.flag, a.id,
+                                     c.id, total);
+    std::swap(b->h64, b->h64b);
+  } else {
+    PhoF32 &a = b->h32, &c = b->h32b;
+    float *sa[] = {a.x1, a.x2, a.x3, a.p1, a.p2, a.p3, a.w, a.path};
+    float *sc[] = {c.x1, c.x2, c.x3, c.p1, c.p2, c.p3, c.w, c.path};
+    st = sort_gather<float, PhoF32>(b, sa, sc, 8, a.cell, c.cell, a.flag, c.flag, nullptr,
+                                    nullptr, total);
+    std::swap(b->h32, b->h32b);
+  }
+  if (st != APT_OK) return st;
+  k_b_count_sorted<<<bblocks(b->n), 256, 0, b->stream>>>(b->d_inst, b->n, off, b->d_cnt,
+                                                        photons, w.keys_out, b->cbits);
+  uint32_t *ncell = photons ? (b->prec == APT_PREC_FP64 ? b->h64.cell : b->h32.cell)
+                            : (b->prec == APT_PREC_FP64 ? b->p64.cell : b->p32.cell);
+  k_b_clear_tail<<<bblocks(total), 256, 0, b->stream>>>(b->n, off, b->d_cnt, photons,
+                                                       ncell, total);
+  BCHK(hipGetLastError());
+  return APT_OK;
+}
+
+
+// ---- allocation helpers ----------------------------------------------
+template <typename T>
+static bool dalloc(T **p, uint64_t n) {
+  return hipMalloc((void **)p, (n ? n : 1) * sizeof(T)) == hipSuccess;
+}
+
+static bool alloc_particles(apt_batch *b, bool second) {
+  const uint64_t n = b->cap;
+  if (b->prec == APT_PREC_FP64) {
+    PtcF64 &p = second ? b->p64b : b->p64;
+    return dalloc(&p.cell, n) && dalloc(&p.x1, n) && dalloc(&p.x2, n) &&
+           dalloc(&p.x3, n) && dalloc(&p.p1, n) && dalloc(&p.p2, n) &&
+           dalloc(&p.p3, n) && dalloc(&p.E, n) && dalloc(&p.w, n) &&
+           dalloc(&p.flag, n) && dalloc(&p.id, n) &&
+           launch_fill<uint32_t>(p.cell, MAX_CELL, n, b->stream) == hipSuccess;
+  }
+  PtcF32 &p = second ? b->p32b : b->p32;
+  return dalloc(&p.cell, n) && dalloc(&p.x1, n) && dalloc(&p.x2, n) &&
+         dalloc(&p.x3, n) && dalloc(&p.u1, n) && dalloc(&p.u2, n) &&
+         dalloc(&p.u3, n) && dalloc(&p.w, n) && dalloc(&p.flag, n) &&
+         launch_fill<uint32_t>(p.cell, MAX_CELL, n, b->stream) == hipSuccess;
+}
+
+static bool alloc_photons(apt_batch *b, bool second) {
+  const uint64_t n = b->ph_cap;
+  if (b->prec == APT_PREC_FP64) {
+    PhoF64 &p = second ? b->h64b : b->h64;
+    return dalloc(&p.cell, n) && dalloc(&p.x1, n) && dalloc(&p.x2, n) &&
+           dalloc(&p.x3, n) && dalloc(&p.p1, n) && dalloc(&p.p2, n) &&
+           dalloc(&p.p3, n) && dalloc(&p.E, n) && dalloc(&p.w, n) &&
+           dalloc(&p.path, n) && dalloc(&p.flag, n) && dalloc(&p.id, n) &&
+           launch_fill<uint32_t>(p.cell, MAX_CELL, n, b->stream) == hipSuccess;
+  }
+  PhoF32 &p = second ? b->h32b : b->h32;
+  return dalloc(&p.cell, n) && dalloc(&p.x1, n) && dalloc(&p.x2, n) &&
+         dalloc(&p.x3, n) && dalloc(&p.p1, n) && dalloc(&p.p2, n) &&
+         dalloc(&p.p3, n) && dalloc(&p.w, n) && dalloc(&p.path, n) &&
+         dalloc(&p.flag, n) &&
+         launch_fill<uint32_t>(p.cell, MAX_CELL, n, b->stream) == hipSuccess;
+}
+
+static void drop_graphs(apt_batch *b) {
+  for (int q = 0; q < 2; q++) {
+    if (b->gexec[q]) (void)hipGraphExecDestroy(b->gexec[q]);
+    if (b->graph[q]) (void)hipGraphDestroy(b->graph[q]);
+    b->gexec[q] = nullptr;
+    b->graph[q] = nullptr;
+  }
+}
+
+static void free_all(apt_batch *b) {
+  auto F = [](void *p) { (void)hipFree(p); };
+  F(b->d_inst); F(b->d_cnt); F(b->plan.chunk_off); F(b->plan.ph_chunk_off);
+  F(b->plan.work); F(b->d_boff); F(b->d_ioff); F(b->d_slot_off); F(b->d_ph_off);
+  F(b->d_counters);
+  F(b->d_epoch);
+  for (PtcF64 *p : {&b->p64, &b->p64b}) {
+    void *a[] = {p->cell, p->x1, p->x2, p->x3, p->p1, p->p2, p->p3, p->E, p->w, p->flag, p->id};
+    for (void *q : a) F(q);
+  }
+  for (PtcF32 *p : {&b->p32, &b->p32b}) {
+    void *a[] = {p->cell, p->x1, p->x2, p->x3, p->u1, p->u2, p->u3, p->w, p->flag};
+    for (void *q : a) F(q);
+  }
+  for (PhoF64 *p : {&b->h64, &b->h64b}) {
+    void *a[] = {p->cell, p->x1, p->x2, p->x3, p->p1, p->p2, p->p3, p->E, p->w, p->path,
+                 p->flag, p->id};
+    for (void *q : a) F(q);
+  }
+  for (PhoF32 *p : {&b->h32, &b->h32b}) {
+    void *a[] = {p->cell, p->x1, p->x2, p->x3, p->p1, p->p2, p->p3, p->w, p->path, p->flag};
+    for (void *q : a) F(q);
+  }
+  for (int c = 0; c < 3; c++) {
+    F(b->E[c]); F(b->B[c]); F(b->Ebg[c]); F(b->Bbg[c]); F(b->J[c]);
+    F(b->acc[c][0]); F(b->acc[c][1]);
+  }
+  F(b->eflag); F(b->pflag); F(b->echunk); F(b->pchunk); F(b->epre); F(b->ppre); F(b->bsum);
+  b->sortw.release();
+  drop_graphs(b);
+  for (hipEvent_t e : {b->ev0, b->ev1, b->evp0, b->evp1})
+    if (e) (void)hipEventDestroy(e);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+static int ceil_log2_d(double x) {
+  int e = 0;
+  double v = 1.0;
+  while (v < x && e < 60) { v *= 2.0; e++; }
+  while (v / 2.0 >= x && e > -60) { v /= 2.0; e--; }
+  return e;
+}
+
+static apt_status check_inst(apt_batch *b, int inst) {
+  if (!b || inst < 0 || inst >= b->n) return APT_ERR_INVALID_ARG;
+  return APT_OK;
+}
+
+template <typename T>
+static hipError_t copy_out(double *dst, const T *src, uint64_t n) {
+  if (!dst || n == 0) return hipSuccess;
+  if (std::is_same<T, double>::value)
+    return hipMemcpy(dst, src, n * 8, hipMemcpyDeviceToHost);
+  std::vector<T> t(n);
+  hipError_t e = hipMemcpy(t.data(), src, n * sizeof(T), hipMemcpyDeviceToHost);
+  for (uint64_t i = 0; i < n; i++) dst[i] = (double)t[i];
+  return e;
+}
+template <typename T>
+static hipError_t copy_in(T *dst, const double *src, uint64_t n) {
+  if (n == 0) return hipSuccess;
+  if (std::is_same<T, double>::value)
+    return hipMemcpy(dst, src, n * 8, hipMemcpyHostToDevice);
+  std::vector<T> t(n);
+  for (uint64_t i = 0; i < n; i++) t[i] = (T)src[i];
+  return hipMemcpy(dst, t.data(), n * sizeof(T), hipMemcpyHostToDevice);
+}
+
+extern "C" {
+
+apt_status apt_batch_create(const apt_batch_inst *specs, int n,
+                            const apt_batch_params *params, apt_batch **out) {
+  if (!specs || !params || !out || n < 1 || n > 65536) return APT_ERR_INVALID_ARG;
+  *out = nullptr;
+  int devs = 0;
+  if (hipGetDeviceCount(&devs) != hipSuccess || devs < 1) return APT_ERR_HIP;
+  if (params->precision != APT_PREC_FP64 && params->precision != APT_PREC_MIXED)
+    return APT_ERR_INVALID_ARG;
+  const int CH = params->chunk;
+  if (CH != 256 && CH != 512 && CH != 1024) return APT_ERR_INVALID_ARG;
+  if (params->grid_mult < 1 || params->grid_mult > 64) return APT_ERR_INVALID_ARG;
+  apt_batch *b = new apt_batch();
+  b->n = n;
+  b->prec = params->precision;
+  b->params = *params;
+  b->hinst.resize(n);
+  uint64_t base = 0, ph_base = 0, fbase = 0, boff = 0, ioff = 0;
+  int maxdims = 0;
+  double wmax = fabs(params->q_e);
+  std::vector<uint64_t> boffs(n + 1), ioffs(n + 1), soffs(n + 1), poffs(n + 1);
+  for (int k = 0; k < n; k++) {
+    const apt_batch_inst &s = specs[k];
+    if (s.N < 4 || s.ppc < 0 || s.inj < 0 || !(s.weight > 0.0) || s.guard < 2 ||
+        s.cap < 2 || !(s.cfl > 0.0) || !(s.cfl < 1.0)) {
+      delete b;
+      return APT_ERR_INVALID_ARG;
+    }
+    BInst &I = b->hinst[k];
+    memset(&I, 0, sizeof(I));
+    Mesh &m = I.m;
+    m.dim = 1;
+    m.N[0] = s.N;
+    m.N[1] = m.N[2] = 1;
+    m.guard[0] = s.guard;
+    m.dims[0] = s.N + 2 * s.guard;
+    m.dims[1] = m.dims[2] = 1;
+    m.periodic[0] = m.periodic[1] = m.periodic[2] = 0;
+    m.lower[0] = 0.0;
+    m.delta[0] = 1.0 / s.N;
+    m.inv_delta[0] = 1.0 / m.delta[0];
+    for (int d = 1; d < 3; d++) {
+      m.delta[d] = 1.0;
+      m.inv_delta[d] = 1.0;
+    }
+    m.div0 = make_fastdiv((uint32_t)m.dims[0]);
+    m.div01 = make_fastdiv((uint32_t)m.dims[0]);
+    I.dt = s.cfl * m.delta[0];
+    I.base = base;
+    I.cap = s.cap;
+    I.ph_base = ph_base;
+    I.ph_cap = s.ph_cap;
+    I.fbase = fbase;
+    I.boff = boff;
+    I.seed = params->seed + 0x9E3779B97F4A7C15ull * (uint64_t)(k + 1);
+    I.gamma_thr = s.gamma_thr;
+    I.theta = s.theta;
+    I.weight = s.weight;
+    I.inj = s.inj;
+    boffs[k] = boff;
+    ioffs[k] = ioff;
+    soffs[k] = base;
+    poffs[k] = ph_base;
+    base += s.cap;
+    ph_base += s.ph_cap;
+    fbase += (uint64_t)m.dims[0];
+    boff += (uint64_t)s.N;
+    ioff += (uint64_t)s.inj;
+    if (m.dims[0] > maxdims) maxdims = m.dims[0];
+    double qw = fabs(params->q_e * s.weight);
+    if (qw > wmax) wmax = qw;
+  }
+  boffs[n] = boff;
+  ioffs[n] = ioff;
+  soffs[n] = base;
+  poffs[n] = ph_base;
+  b->cap = base;
+  b->ph_cap = ph_base;
+  b->ncell = fbase;
+  b->nbulk = boff;
+  b->ninj = ioff;
+  if (b->cap > 0xffffffffull || b->ph_cap > 0xffffffffull) {
+    delete b;
+    return APT_ERR_INVALID_ARG;  // 32-bit sort permutation
+  }
+  b->cbits = 1;
+  while ((1 << b->cbits) <= maxdims) b->cbits++;
+  int ibits = 1;
+  while ((1 << ibits) < n) ibits++;
+  if (b->cbits + ibits > 32) {
+    delete b;
+    return APT_ERR_INVALID_ARG;
+  }
+  b->H = (b->prec == APT_PREC_FP64 ? 22 : 30) - ceil_log2_d(wmax);
+  for (int i = 0; i < 8; i++) {
+    b->sp.q[i] = params->q_e;
+    b->sp.m[i] = params->q_e;
+  }
+  b->sp.q[0] = -params->q_e;
+  b->sp.m[2] = params->q_e * params->ion_mass;
+  for (int k = 0; k < n; k++) {
+    b->max_chunks += (specs[k].cap + CH - 1) / CH;
+    b->max_ph_chunks += (specs[k].ph_cap + CH - 1) / CH;
+  }
+  bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
+  ok = ok && dalloc(&b->d_inst, n) &&
+       hipMemcpy(b->d_inst, b->hinst.data(), n * sizeof(BInst), hipMemcpyHostToDevice) ==
+           hipSuccess;
+  std::vector<BCount> cnt(n);
+  for (int k = 0; k < n; k++) cnt[k] = BCount{0, 0, 0, 0};
+  ok = ok && dalloc(&b->d_cnt, n) &&
+       hipMemcpy(b->d_cnt, cnt.data(), n * sizeof(BCount), hipMemcpyHostToDevice) ==
+           hipSuccess;
+  ok = ok && dalloc(&b->plan.chunk_off, n + 1) && dalloc(&b->plan.ph_chunk_off, n + 1) &&
+       dalloc(&b->plan.work, 2);
+  auto up = [&](uint64_t **d, std::vector<uint64_t> &h) {
+    return dalloc(d, h.size()) &&
+           hipMemcpy(*d, h.data(), h.size() * 8, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  ok = ok && up(&b->d_boff, boffs) && up(&b->d_ioff, ioffs) && up(&b->d_slot_off, soffs) &&
+       up(&b->d_ph_off, poffs);
+  ok = ok && dalloc(&b->d_counters, CTR_NUM) &&
+       launch_fill<long long>((long long *)b->d_counters, 0, CTR_NUM, b->stream) ==
+           hipSuccess;
+  ok = ok && alloc_particles(b, false) && alloc_particles(b, true) &&
+       alloc_photons(b, false) && alloc_photons(b, true);
+  const size_t es = b->prec == APT_PREC_FP64 ? 8 : 4;
+  for (int c = 0; c < 3 && ok; c++) {
+    void **arrs[] = {&b->E[c], &b->B[c], &b->Ebg[c], &b->Bbg[c], &b->J[c]};
+    for (void **a : arrs) {
+      ok = ok && hipMalloc(a, b->ncell * es) == hipSuccess &&
+           hipMemsetAsync(*a, 0, b->ncell * es, b->stream) == hipSuccess;
+    }
+    for (int l = 0; l < (b->prec == APT_PREC_FP64 ? 2 : 1); l++)
+      ok = ok && dalloc(&b->acc[c][l], b->ncell) &&
+           launch_fill<long long>(b->acc[c][l], 0, b->ncell, b->stream) == hipSuccess;
+  }
+  ok = ok && dalloc(&b->eflag, b->cap) && dalloc(&b->pflag, b->ph_cap) &&
+       dalloc(&b->echunk, b->max_chunks + 1) && dalloc(&b->pchunk, b->max_ph_chunks + 1) &&
+       dalloc(&b->epre, b->max_chunks + 1) && dalloc(&b->ppre, b->max_ph_chunks + 1);
+  const uint64_t nbs = ((b->max_chunks > b->max_ph_chunks ? b->max_chunks : b->max_ph_chunks) +
+                        SCAN_B) / SCAN_B + 1;
+  ok = ok && dalloc(&b->bsum, nbs);
+  if (ok) {
+    k_u8_fill<<<bblocks(b->cap), 256, 0, b->stream>>>(b->eflag, b->cap);
+    k_u8_fill<<<bblocks(b->ph_cap), 256, 0, b->stream>>>(b->pflag, b->ph_cap);
+    ok = launch_fill<uint32_t>(b->echunk, 0u, b->max_chunks + 1, b->stream) == hipSuccess &&
+         launch_fill<uint32_t>(b->pchunk, 0u, b->max_ph_chunks + 1, b->stream) == hipSuccess;
+  }
+  ok = ok && dalloc(&b->d_epoch, 1) &&
+       launch_fill<uint32_t>(b->d_epoch, 0u, 1, b->stream) == hipSuccess;
+  for (hipEvent_t *e : {&b->ev0, &b->ev1, &b->evp0, &b->evp1})
+    ok = ok && hipEventCreate(e) == hipSuccess;
+  // persistent push: grid_mult workgroups per CU, never more than chunks
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  uint64_t pg = (uint64_t)ncu * (uint64_t)params->grid_mult;
+  if (pg > b->max_chunks) pg = b->max_chunks;
+  b->push_grid = (unsigned)(pg ? pg : 1);
+  ok = ok && hipStreamSynchronize(b->stream) == hipSuccess;
+  if (!ok) {
+    free_all(b);
+    delete b;
+    return APT_ERR_HIP;
+  }
+  *out = b;
+  return APT_OK;
+}
+
+apt_status apt_batch_destroy(apt_batch *b) {
+  if (!b) return APT_ERR_INVALID_ARG;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  free_all(b);
+  delete b;
+  return APT_OK;
+}
+
+const char *apt_batch_last_error(const apt_batch *b) {
+  return b ? b->err.c_str() : "null batch";
+}
+
+apt_status apt_batch_get_jexp(const apt_batch *b, int *H) {
+  if (!b || !H) return APT_ERR_INVALID_ARG;
+  *H = b->H;
+  return APT_OK;
+}
+
+// ppc e-/e+ pairs per cell of every instance at its temperature (the
+// loader of apt_ptc_load_uniform on the instance's slot range), B_x in B
+// and Bbg (sim_data.h:23-29), E = 0
+apt_status apt_batch_load(apt_batch *b, const apt_batch_inst *specs) {
+  if (!b || !specs) return APT_ERR_INVALID_ARG;
+  std::vector<BCount> cnt(b->n);
+  BCHK(hipStreamSynchronize(b->stream));
+  BCHK(hipMemcpy(cnt.data(), b->d_cnt, b->n * sizeof(BCount), hipMemcpyDeviceToHost));
+  const size_t es = b->prec == APT_PREC_FP64 ? 8 : 4;
+  for (int k = 0; k < b->n; k++) {
+    const BInst &I = b->hinst[k];
+    const uint64_t add = 2 * (uint64_t)specs[k].ppc * (uint64_t)specs[k].N;
+    if (add > I.cap) return b->fail(APT_ERR_CAPACITY, "batch_load: ppc N over capacity");
+    LoadParams lp;
+    lp.ppc = specs[k].ppc;
+    lp.theta[0] = lp.theta[1] = specs[k].theta;
+    lp.drift[0] = lp.drift[1] = 0.0;
+    lp.weight = specs[k].weight;
+    lp.seed = I.seed;
+    lp.base = 0;
+    if (add > 0) {
+      if (b->prec == APT_PREC_FP64) {
+        PtcF64 q = b->p64;
+        q.cell += I.base; q.x1 += I.base; q.x2 += I.base; q.x3 += I.base;
+        q.p1 += I.base; q.p2 += I.base; q.p3 += I.base; q.E += I.base;
+        q.w += I.base; q.flag += I.base; q.id += I.base;
+        BCHK(launch_load_uniform_f64(I.m, lp, q, b->d_counters, b->stream));
+      } else {
+        PtcF32 q = b->p32;
+        q.cell += I.base; q.x1 += I.base; q.x2 += I.base; q.x3 += I.base;
+        q.u1 += I.base; q.u2 += I.base; q.u3 += I.base; q.w += I.base;
+        q.flag += I.base;
+        BCHK(launch_load_uniform_f32(I.m, lp, q, b->d_counters, b->stream));
+      }
+    }
+    cnt[k].number = add;
+    cnt[k].id_next = add;
+    const uint64_t nd = (uint64_t)I.m.dims[0];
+    std::vector<double> bx(nd, specs[k].Bx);
+    if (b->prec == APT_PREC_FP64) {
+      BCHK(hipMemcpy((double *)b->B[0] + I.fbase, bx.data(), nd * es, hipMemcpyHostToDevice));
+      BCHK(hipMemcpy((double *)b->Bbg[0] + I.fbase, bx.data(), nd * es, hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> fx(nd, (float)specs[k].Bx);
+      BCHK(hipMemcpy((float *)b->B[0] + I.fbase, fx.data(), nd * es, hipMemcpyHostToDevice));
+      BCHK(hipMemcpy((float *)b->Bbg[0] + I.fbase, fx.data(), nd * es, hipMemcpyHostToDevice));
+    }
+  }
+  BCHK(hipMemcpy(b->d_cnt, cnt.data(), b->n * sizeof(BCount), hipMemcpyHostToDevice));
+  BCHK(hipStreamSynchronize(b->stream));
+  return APT_OK;
+}
+
+static apt_status sort_both(apt_batch *b) {
+  apt_status st = batch_sort(b, 0);
+  if (st == APT_OK) st = batch_sort(b, 1);
+  if (st == APT_OK) b->parity ^= 1;
+  return st;
+}
+
+// one or more ensemble steps; steps after the first are replayed from a
+// hipGraph when enabled (step 0 carries the E half step); the sort runs
+// after every step s with s % sort_interval == 0, s != 0 (1dgap main.cpp:91)
+apt_status apt_batch_step(apt_batch *b, int nsteps) {
+  if (!b || nsteps < 0) return APT_ERR_INVALID_ARG;
+  for (int s = 0; s < nsteps; s++) {
+    const bool first = b->step == 0;
+    if (b->timing) BCHK(hipEventRecord(b->ev0, b->stream));
+    if (b->use_graph && !first) {
+      const int q = b->parity;
+      if (!b->gexec[q]) {
+        BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        apt_status st = batch_run_step(b, false);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (st != APT_OK) {
+          if (g) (void)hipGraphDestroy(g);
+          return st;
+        }
+        BCHK(e);
+        b->graph[q] = g;
+        BCHK(hipGraphInstantiate(&b->gexec[q], g, nullptr, nullptr, 0));
+      }
+      BCHK(hipGraphLaunch(b->gexec[q], b->stream));
+    } else {
+      apt_status st = batch_run_step(b, first);
+      if (st != APT_OK) return st;
+    }
+    if (b->timing) {
+      BCHK(hipEventRecord(b->ev1, b->stream));
+      BCHK(hipEventSynchronize(b->ev1));
+      float ms = 0.0f, pms = 0.0f;
+      BCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+      BCHK(hipEventElapsedTime(&pms, b->evp0, b->evp1));
+      b->step_ms += ms;
+      b->push_ms += pms;
+      b->steps_timed++;
+    }
+    const int si = b->params.sort_interval;
+    if (si > 0 && b->step % (uint32_t)si == 0 && b->step != 0) {
+      apt_status st = sort_both(b);
+      if (st != APT_OK) return st;
+    }
+    b->step++;
+  }
+  return APT_OK;
+}
+
+apt_status apt_batch_sort(apt_batch *b) {
+  if (!b) return APT_ERR_INVALID_ARG;
+  return sort_both(b);
+}
+
+apt_status apt_batch_graph(apt_batch *b, int on) {
+  if (!b) return APT_ERR_INVALID_ARG;
+  b->use_graph = on != 0;
+  return APT_OK;
+}
+
+apt_status apt_batch_sync(apt_batch *b) {
+  if (!b) return APT_ERR_INVALID_ARG;
+  BCHK(hipStreamSynchronize(b->stream));
+  return APT_OK;
+}
+
+apt_status apt_batch_counts(apt_batch *b, uint64_t *number, uint64_t *ph_number) {
+  if (!b) return APT_ERR_INVALID_ARG;
+  std::vector<BCount> cnt(b->n);
+  BCHK(hipStreamSynchronize(b->stream));
+  BCHK(hipMemcpy(cnt.data(), b->d_cnt, b->n * sizeof(BCount), hipMemcpyDeviceToHost));
+  for (int k = 0; k < b->n; k++) {
+    if (number) number[k] = cnt[k].number;
+    if (ph_number) ph_number[k] = cnt[k].ph_number;
+  }
+  return APT_OK;
+}
+
+apt_status apt_batch_get_counters(apt_batch *b, apt_counters *out) {
+  if (!b || !out) return APT_ERR_INVALID_ARG;
+  unsigned long long h[CTR_NUM];
+  BCHK(hipStreamSynchronize(b->stream));
+  BCHK(hipMemcpy(h, b->d_counters, sizeof(h), hipMemcpyDeviceToHost));
+  out->nan_momenta = h[CTR_NAN];
+  out->multi_cell_moves = h[CTR_MULTICELL];
+  out->capacity_overflow = h[CTR_OVERFLOW];
+  out->outside_tile = h[CTR_OUTSIDE];
+  return APT_OK;
+}
+
+// on = 1: time every step (step and push durations, one sync per step);
+// on = 0: stop; on = -1: only read.  Reading returns the sums since the
+// last switch.  Graphs are re-captured, since the push events are part of
+// the captured step.
+apt_status apt_batch_timing(apt_batch *b, int on, double *step_ms, double *push_ms,
+                            uint64_t *steps) {
+  if (!b) return APT_ERR_INVALID_ARG;
+  if (step_ms) *step_ms = b->step_ms;
+  if (push_ms) *push_ms = b->push_ms;
+  if (steps) *steps = b->steps_timed;
+  if (on >= 0) {
+    BCHK(hipStreamSynchronize(b->stream));
+    if ((on != 0) != b->timing) drop_graphs(b);
+    b->timing = on != 0;
+    b->step_ms = 0.0;
+    b->push_ms = 0.0;
+    b->steps_timed = 0;
+  }
+  return APT_OK;
+}
+
+apt_status apt_batch_get_particles(apt_batch *b, int inst, apt_particles *dst,
+                                   uint64_t cap, uint64_t *n) {
+  if (check_inst(b, inst) != APT_OK || !dst) return APT_ERR_INVALID_ARG;
+  std::vector<uint64_t> num(b->n);
+  apt_status st = apt_batch_counts(b, num.data(), nullptr);
+  if (st != APT_OK) return st;
+  const uint64_t m = num[inst], o = b->hinst[inst].base;
+  if (n) *n = m;
+  if (m > cap) return b->fail(APT_ERR_CAPACITY, "batch_get_particles: cap too small");
+  if (b->prec == APT_PREC_FP64) {
+    PtcF64 &p = b->p64;
+    if (dst->cell) BCHK(hipMemcpy(dst->cell, p.cell + o, m * 4, hipMemcpyDeviceToHost));
+    BCHK(copy_out(dst->x1, p.x1 + o, m));
+    BCHK(copy_out(dst->x2, p.x2 + o, m));
+    BCHK(copy_out(dst->x3, p.x3 + o, m));
+    BCHK(copy_out(dst->p1, p.p1 + o, m));
+    BCHK(copy_out(dst->p2, p.p2 + o, m));
+    BCHK(copy_out(dst->p3, p.p3 + o, m));
+    BCHK(copy_out(dst->E, p.E + o, m));
+    BCHK(copy_out(dst->weight, p.w + o, m));
+    if (dst->flag) BCHK(hipMemcpy(dst->flag, p.flag + o, m * 4, hipMemcpyDeviceToHost));
+    if (dst->id) BCHK(hipMemcpy(dst->id, p.id + o, m * 8, hipMemcpyDeviceToHost));
+  } else {
+    PtcF32 &p = b->p32;
+    if (dst->cell) BCHK(hipMemcpy(dst->cell, p.cell + o, m * 4, hipMemcpyDeviceToHost));
+    BCHK(copy_out(dst->x1, p.x1 + o, m));
+    BCHK(copy_out(dst->x2, p.x2 + o, m));
+    BCHK(copy_out(dst->x3, p.x3 + o, m));
+    BCHK(copy_out(dst->p1, p.u1 + o, m));
+    BCHK(copy_out(dst->p2, p.u2 + o, m));
+    BCHK(copy_out(dst->p3, p.u3 + o, m));
+    BCHK(copy_out(dst->weight, p.w + o, m));
+    if (dst->flag) BCHK(hipMemcpy(dst->flag, p.flag + o, m * 4, hipMemcpyDeviceToHost));
+    if (dst->E && dst->p1 && dst->p2 && dst->p3)
+      for (uint64_t i = 0; i < m; i++)
+        dst->E[i] = sqrt(1.0 + dst->p1[i] * dst->p1[i] + dst->p2[i] * dst->p2[i] +
+                         dst->p3[i] * dst->p3[i]);
+    if (dst->id)
+      for (uint64_t i = 0; i < m; i++) dst->id[i] = i;
+  }
+  return APT_OK;
+}
+
+apt_status apt_batch_get_photons(apt_batch *b, int inst, apt_photons *dst,
+                                 uint64_t cap, uint64_t *n) {
+  if (check_inst(b, inst) != APT_OK || !dst) return APT_ERR_INVALID_ARG;
+  std::vector<uint64_t> num(b->n);
+  apt_status st = apt_batch_counts(b, nullptr, num.data());
+  if (st != APT_OK) return st;
+  const uint64_t m = num[inst], o = b->hinst[inst].ph_base;
+  if (n) *n = m;
+  if (m > cap) return b->fail(APT_ERR_CAPACITY, "batch_get_photons: cap too small");
+  if (b->prec == APT_PREC_FP64) {
+    PhoF64 &p = b->h64;
+    if (dst->cell) BCHK(hipMemcpy(dst->cell, p.cell + o, m * 4, hipMemcpyDeviceToHost));
+    BCHK(copy_out(dst->x1, p.x1 + o, m));
+    BCHK(copy_out(dst->x2, p.x2 + o, m));
+    BCHK(copy_out(dst->x3, p.x3 + o, m));
+    BCHK(copy_out(dst->p1, p.p1 + o, m));
+    BCHK(copy_out(dst->p2, p.p2 + o, m));
+    BCHK(copy_out(dst->p3, p.p3 + o, m));
+    BCHK(copy_out(dst->E, p.E + o, m));
+    BCHK(copy_out(dst->weight, p.w + o, m));
+    BCHK(copy_out(dst->path_left, p.path + o, m));
+    if (dst->flag) BCHK(hipMemcpy(dst->flag, p.flag + o, m * 4, hipMemcpyDeviceToHost));
+    if (dst->id) BCHK(hipMemcpy(dst->id, p.id + o, m * 8, hipMemcpyDeviceToHost));
+  } else {
+    PhoF32 &p = b->h32;
+    if (dst->cell) BCHK(hipMemcpy(dst->cell, p.cell + o, m * 4, hipMemcpyDeviceToHost));
+    BCHK(copy_out(dst->x1, p.x1 + o, m));
+    BCHK(copy_out(dst->x2, p.x2 + o, m));
+    BCHK(copy_out(dst->x3, p.x3 + o, m));
+    BCHK(copy_out(dst->p1, p.p1 + o, m));
+    BCHK(copy_out(dst->p2, p.p2 + o, m));
+    BCHK(copy_out(dst->p3, p.p3 + o, m));
+    BCHK(copy_out(dst->weight, p.w + o, m));
+    BCHK(copy_out(dst->path_left, p.path + o, m));
+    if (dst->flag) BCHK(hipMemcpy(dst->flag, p.flag + o, m * 4, hipMemcpyDeviceToHost));
+  }
+  return APT_OK;
+}
+
+apt_status apt_batch_set_particles(apt_batch *b, int inst, const apt_particles *src,
+                                   uint64_t m) {
+  if (check_inst(b, inst) != APT_OK || !src || !src->cell) return APT_ERR_INVALID_ARG;
+  const BInst &I = b->hinst[inst];
+  if (m > I.cap) return b->fail(APT_ERR_CAPACITY, "batch_set_particles: over capacity");
+  const uint64_t o = I.base;
+  BCHK(hipStreamSynchronize(b->stream));
+  std::vector<uint32_t> cells(I.cap, MAX_CELL);
+  for (uint64_t i = 0; i < m; i++) cells[i] = src->cell[i];
+  std::vector<uint32_t> fl(m, 0u);
+  if (src->flag) fl.assign(src->flag, src->flag + m);
+  if (b->prec == APT_PREC_FP64) {
+    PtcF64 &p = b->p64;
+    BCHK(hipMemcpy(p.cell + o, cells.data(), I.cap * 4, hipMemcpyHostToDevice));
+    BCHK(copy_in(p.x1 + o, src->x1, m));
+    BCHK(copy_in(p.x2 + o, src->x2, m));
+    BCHK(copy_in(p.x3 + o, src->x3, m));
+    BCHK(copy_in(p.p1 + o, src->p1, m));
+    BCHK(copy_in(p.p2 + o, src->p2, m));
+    BCHK(copy_in(p.p3 + o, src->p3, m));
+    std::vector<double> g(m);
+    for (uint64_t i = 0; i < m; i++)
+      g[i] = src->E ? src->E[i]
+                    : sqrt(1.0 + src->p1[i] * src->p1[i] + src->p2[i] * src->p2[i] +
+                           src->p3[i] * src->p3[i]);
+    BCHK(copy_in(p.E + o, g.data(), m));
+    BCHK(copy_in(p.w + o, src->weight, m));
+    BCHK(hipMemcpy(p.flag + o, fl.data(), m * 4, hipMemcpyHostToDevice));
+    std::vector<uint64_t> ids(m);
+    for (uint64_t i = 0; i < m; i++) ids[i] = src->id ? src->id[i] : i;
+    BCHK(hipMemcpy(p.id + o, ids.data(), m * 8, hipMemcpyHostToDevice));
+  } else {
+    PtcF32 &p = b->p32;
+    BCHK(hipMemcpy(p.cell + o, cells.data(), I.cap * 4, hipMemcpyHostToDevice));
+    BCHK(copy_in(p.x1 + o, src->x1, m));
+    BCHK(copy_in(p.x2 + o, src->x2, m));
+    BCHK(copy_in(p.x3 + o, src->x3, m));
+    BCHK(copy_in(p.u1 + o, src->p1, m));
+    BCHK(copy_in(p.u2 + o, src->p2, m));
+    BCHK(copy_in(p.u3 + o, src->p3, m));
+    BCHK(copy_in(p.w + o, src->weight, m));
+    BCHK(hipMemcpy(p.flag + o, fl.data(), m * 4, hipMemcpyHostToDevice));
+  }
+  std::vector<BCount> cnt(b->n);
+  BCHK(hipMemcpy(cnt.data(), b->d_cnt, b->n * sizeof(BCount), hipMemcpyDeviceToHost));
+  cnt[inst].number = m;
+  cnt[inst].id_next = m;
+  if (src->id)
+    for (uint64_t i = 0; i < m; i++)
+      if (src->id[i] + 1 > cnt[inst].id_next) cnt[inst].id_next = src->id[i] + 1;
+  BCHK(hipMemcpy(b->d_cnt, cnt.data(), b->n * sizeof(BCount), hipMemcpyHostToDevice));
+  return APT_OK;
+}
+
+static void *bfield_ptr(apt_batch *b, int which, int comp) {
+  if (comp < 0 || comp > 2) return nullptr;
+  switch (which) {
+    case APT_FIELD_E: return b->E[comp];
+    case APT_FIELD_B: return b->B[comp];
+    case APT_FIELD_EBG: return b->Ebg[comp];
+    case APT_FIELD_BBG: return b->Bbg[comp];
+    case APT_FIELD_J: return b->J[comp];
+  }
+  return nullptr;
+}
+
+apt_status apt_batch_get_field(apt_batch *b, int inst, int which, int comp, double *dst,
+                               uint64_t count) {
+  if (check_inst(b, inst) != APT_OK || !dst) return APT_ERR_INVALID_ARG;
+  void *a = bfield_ptr(b, which, comp);
+  const BInst &I = b->hinst[inst];
+  if (!a || count != (uint64_t)I.m.dims[0]) return APT_ERR_INVALID_ARG;
+  BCHK(hipStreamSynchronize(b->stream));
+  if (b->prec == APT_PREC_FP64)
+    BCHK(copy_out(dst, (double *)a + I.fbase, count));
+  else
+    BCHK(copy_out(dst, (float *)a + I.fbase, count));
+  return APT_OK;
+}
+
+apt_status apt_batch_set_field(apt_batch *b, int inst, int which, int comp,
+                               const double *src, uint64_t count) {
+  if (check_inst(b, inst) != APT_OK || !src) return APT_ERR_INVALID_ARG;
+  void *a = bfield_ptr(b, which, comp);
+  const BInst &I = b->hinst[inst];
+  if (!a || count != (uint64_t)I.m.dims[0]) return APT_ERR_INVALID_ARG;
+  BCHK(hipStreamSynchronize(b->stream));
+  if (b->prec == APT_PREC_FP64)
+    BCHK(copy_in((double *)a + I.fbase, src, count));
+  else
+    BCHK(copy_in((float *)a + I.fbase, src, count));
+  return APT_OK;
+}
+
+apt_status apt_batch_inst_seed(const apt_batch *b, int inst, uint64_t *seed) {
+  if (!b || inst < 0 || inst >= b->n || !seed) return APT_ERR_INVALID_ARG;
+  *seed = b->hinst[inst].seed;
+  return APT_OK;
+}
+
+// The batched-ensemble workload draw: for instance k, six uniforms from
+// Philox4x32-10 keyed (seed, k, draw, purpose 8) give N = exp(U[ln 1024,
+// ln 16384]) (rounded), ppc = U[50, 400] per species, B_x = U[0.5, 2],
+// theta = U[0.01, 0.3], gamma_thr = U[1e2, 1e4].
+apt_status apt_batch_sample(uint64_t seed, int n, apt_batch_inst *out) {
+  if (!out || n < 1) return APT_ERR_INVALID_ARG;
+  for (int k = 0; k < n; k++) {
+    double a0, a1, b0, b1, c0, c1;
+    uniform2(seed, (uint64_t)k, 0, 8, a0, a1);
+    uniform2(seed, (uint64_t)k, 1, 8, b0, b1);
+    uniform2(seed, (uint64_t)k, 2, 8, c0, c1);
+    apt_batch_inst &s = out[k];
+    memset(&s, 0, sizeof(s));
+    s.N = (int)llround(exp(log(1024.0) + a0 * (log(16384.0) - log(1024.0))));
+    s.ppc = 50 + (int)floor(a1 * 351.0);
+    if (s.ppc > 400) s.ppc = 400;
+    s.Bx = 0.5 + 1.5 * b0;
+    s.theta = 0.01 + 0.29 * b1;
+    s.gamma_thr = 1e2 + (1e4 - 1e2) * c0;
+    s.guard = 2;
+    s.cfl = 0.5;
+    s.weight = 1.0;
+    s.inj = s.ppc / 8 > 0 ? s.ppc / 8 : 1;
+    s.cap = (uint64_t)(2.0 * s.ppc * s.N * 1.25) + 64;
+    s.ph_cap = s.cap / 8 + 64;
+  }
+  return APT_OK;
+}
+
+}  // extern "C"

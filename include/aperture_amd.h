@@ -299,6 +299,83 @@ apt_status apt_timing_enable(apt_ctx *ctx, int on);
 apt_status apt_timing_get(apt_ctx *ctx, int kernel, double *total_ms,
                           uint64_t *launches);
 
+/* ---- batched ensemble of 1D gap boxes ----------------------------------
+ * Many independent 1D open boxes of uneven sizes advanced together: the
+ * loop of labs/1dgap/src/main.cpp:61-98 (update_particles, clear_guard_cells
+ * of particles and photons, update_fields, emit_photons, produce_pairs,
+ * sort_by_cell every sort_interval steps) per box, with wall injection of
+ * `inj` e-/e+ pairs per step at the left wall before the push.  The
+ * reference runs one box per process; there is no batched counterpart.
+ * Physics per box: ptc_updater.cu:21-499 (1D Cartesian Vay push + deposit),
+ * field_solver.cu:14-235 (1D Yee, half E step at step 0),
+ * radiative_transfer.cu:28-301 with the exponential free-path model
+ * (E_ph = e_frac gamma, path = -photon_path ln u).  Box k runs with its own
+ * seed (apt_batch_inst_seed) and reproduces the single-box FP64 path. */
+typedef struct apt_batch apt_batch;
+
+typedef struct {
+  int N;              /* bulk cells; the box is [0, 1), delta = 1/N */
+  int guard;          /* guard cells per side, >= 2 */
+  int ppc;            /* initial particles per cell per species */
+  int inj;            /* wall-injected pairs per step */
+  uint64_t cap;       /* particle slots of this box */
+  uint64_t ph_cap;    /* photon slots of this box */
+  double cfl;         /* dt = cfl * delta, 0 < cfl < 1 */
+  double weight;      /* particle weight */
+  double theta;       /* Maxwell-Juttner temperature (load and injection) */
+  double gamma_thr;   /* emission threshold Lorentz factor */
+  double Bx;          /* uniform B_x (and background B_x) */
+} apt_batch_inst;
+
+typedef struct {
+  int precision;        /* APT_PREC_* */
+  double q_e;           /* charge unit (electron -q_e) */
+  double ion_mass;
+  uint64_t seed;        /* box k uses seed + 0x9E3779B97F4A7C15 (k + 1) */
+  int chunk;            /* particles per workgroup: 256, 512 or 1024 */
+  int grid_mult;        /* persistent push workgroups per CU, 1..64 */
+  int sort_interval;    /* 0: never */
+  double photon_path;   /* mean free path */
+  double e_frac;        /* photon energy / parent gamma */
+  double r_cutoff;      /* photons with a longer path are dropped */
+} apt_batch_params;
+
+apt_status apt_batch_create(const apt_batch_inst *specs, int n,
+                            const apt_batch_params *params, apt_batch **out);
+apt_status apt_batch_destroy(apt_batch *b);
+const char *apt_batch_last_error(const apt_batch *b);
+/* the fixed-point exponent of the current accumulators */
+apt_status apt_batch_get_jexp(const apt_batch *b, int *H);
+apt_status apt_batch_inst_seed(const apt_batch *b, int inst, uint64_t *seed);
+/* initial state: 2 ppc N particles per box (the apt_ptc_load_uniform
+ * loader on the box's slots with its seed and theta), B_x = Bbg_x = Bx */
+apt_status apt_batch_load(apt_batch *b, const apt_batch_inst *specs);
+/* nsteps steps of every box (graph replays after step 0 when enabled) */
+apt_status apt_batch_step(apt_batch *b, int nsteps);
+apt_status apt_batch_graph(apt_batch *b, int on);
+apt_status apt_batch_sort(apt_batch *b);
+apt_status apt_batch_sync(apt_batch *b);
+apt_status apt_batch_counts(apt_batch *b, uint64_t *number, uint64_t *ph_number);
+apt_status apt_batch_get_counters(apt_batch *b, apt_counters *out);
+/* on = 1 start, 0 stop, -1 read only; sums since the last switch */
+apt_status apt_batch_timing(apt_batch *b, int on, double *step_ms, double *push_ms,
+                            uint64_t *steps);
+apt_status apt_batch_get_particles(apt_batch *b, int inst, apt_particles *dst,
+                                   uint64_t cap, uint64_t *n);
+apt_status apt_batch_set_particles(apt_batch *b, int inst, const apt_particles *src,
+                                   uint64_t n);
+apt_status apt_batch_get_photons(apt_batch *b, int inst, apt_photons *dst,
+                                 uint64_t cap, uint64_t *n);
+/* which = APT_FIELD_E/B/EBG/BBG/J, count = N + 2 guard */
+apt_status apt_batch_get_field(apt_batch *b, int inst, int which, int comp,
+                               double *dst, uint64_t count);
+apt_status apt_batch_set_field(apt_batch *b, int inst, int which, int comp,
+                               const double *src, uint64_t count);
+/* the ensemble workload draw: per box, Philox4x32-10 keyed (seed, box)
+ * gives N = exp(U[ln 1024, ln 16384]), ppc = U[50, 400], Bx = U[0.5, 2],
+ * theta = U[0.01, 0.3], gamma_thr = U[1e2, 1e4] */
+apt_status apt_batch_sample(uint64_t seed, int n, apt_batch_inst *out);
+
 #ifdef __cplusplus
 }
 #endif

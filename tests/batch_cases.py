@@ -1,0 +1,79 @@
+"""Shared set-up of the batched-ensemble parity cases: a few ragged 1D open
+gap boxes, built once for the HIP batch and once per box for the oracle,
+which replays the 1dgap loop (labs/1dgap/src/main.cpp:61-98) box by box:
+
+    inject_left -> update_particles (push, deposit, photon move)
+    -> clear_guard_cells (particles, photons) -> update_fields
+    -> emit_photons -> produce_pairs -> sort every sort_interval steps
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle_ctypes as oc
+
+# three uneven boxes; a low gamma threshold and a short free path make the
+# emission / conversion branches fire every step
+INSTS = [
+    dict(N=64, guard=2, ppc=3, inj=2, cap=2000, ph_cap=1500, cfl=0.5, weight=1.0,
+         theta=0.2, gamma_thr=1.15, Bx=1.0),
+    dict(N=97, guard=2, ppc=5, inj=3, cap=4000, ph_cap=3000, cfl=0.45, weight=0.5,
+         theta=0.1, gamma_thr=1.05, Bx=0.7),
+    dict(N=37, guard=3, ppc=2, inj=0, cap=1500, ph_cap=600, cfl=0.5, weight=2.0,
+         theta=0.3, gamma_thr=1.3, Bx=1.8),
+]
+PARAMS = dict(q_e=1.0, ion_mass=1.0, seed=7, chunk=256, grid_mult=2, sort_interval=3,
+              photon_path=0.03, e_frac=0.1, r_cutoff=1e30)
+
+
+def product(precision, insts=INSTS, graph=False, **kw):
+    import aperture3_amd as apt
+    p = dict(PARAMS)
+    p.update(kw)
+    b = apt.batch_ensemble(insts, precision=precision, **p)
+    b.graph(graph)
+    return b
+
+
+class OracleBox:
+    """One box of the ensemble on the oracle."""
+
+    def __init__(self, inst, seed, H, params=PARAMS):
+        self.inst = inst
+        self.params = params
+        N, g = inst["N"], inst["guard"]
+        self.sim = oc.OracleSim([N], [g], size=(1.0, 1.0, 1.0),
+                                periodic=(False, False, False), q_e=params["q_e"],
+                                ion_mass=params["ion_mass"], num_species=2,
+                                capacity=inst["cap"], jexp=H)
+        self.sim.seed = seed
+        self.dt = inst["cfl"] * (1.0 / N)
+        self.sim.load_uniform(inst["ppc"], theta=(inst["theta"], inst["theta"]),
+                              drift=(0.0, 0.0), weight=inst["weight"], seed=seed)
+        self.sim.f["B"][0][:] = inst["Bx"]
+        self.sim.f["Bbg"][0][:] = inst["Bx"]
+        self.sim.set_rt(model=1, gamma_thr=inst["gamma_thr"],
+                        photon_path=params["photon_path"], r_cutoff=params["r_cutoff"],
+                        e_frac=params["e_frac"], max_photon=inst["ph_cap"],
+                        max_ptc=inst["cap"])
+        self.step_no = 0
+
+    def step(self):
+        s, sim, inst = self.step_no, self.sim, self.inst
+        sim.inject_left(inst["inj"], inst["theta"], inst["weight"], epoch=s)
+        sim.ptc_update(self.dt, s, deposit_rho=False)
+        sim.clear_guard_cells()
+        sim.clear_photon_guard_cells()
+        sim.field_update(self.dt, s)
+        sim.emit_photons()
+        sim.produce_pairs()
+        si = self.params["sort_interval"]
+        if si > 0 and s % si == 0 and s != 0:
+            sim.sort()
+            sim.sort_photons()
+        self.step_no += 1
+
+
+def oracle_boxes(b, insts=INSTS, params=PARAMS):
+    H = b.jexp()
+    return [OracleBox(inst, b.inst_seed(k), H, params) for k, inst in enumerate(insts)]

@@ -1,0 +1,154 @@
+"""Parity of the batched ensemble of 1D open gap boxes (kern_batch.hip)
+against the oracle run box by box (labs/1dgap/src/main.cpp:61-98 with wall
+injection): FP64 bit-exact particles, photons, counts and fields per box;
+hipGraph replays identical to eager steps; the mixed form within stated
+tolerances of the FP64 oracle."""
+import numpy as np
+import pytest
+
+import batch_cases as bc
+
+pytestmark = pytest.mark.gpu
+
+PTC_KEYS = ("x1", "x2", "x3", "p1", "p2", "p3", "E", "w", "flag", "id")
+PH_KEYS = ("x1", "x2", "x3", "p1", "p2", "p3", "E", "w", "path", "flag", "id")
+
+
+def _apt():
+    import aperture3_amd as apt
+    return apt
+
+
+def _compare_box(b, k, box, exact=True):
+    apt = _apt()
+    sim = box.sim
+    num, phn = b.counts()
+    assert int(num[k]) == sim.number, f"box {k}: particle count"
+    assert int(phn[k]) == sim.ph_number, f"box {k}: photon count"
+    got = b.get_particles(k)
+    ref = sim.particles()
+    np.testing.assert_array_equal(got["cell"], ref["cell"], err_msg=f"box {k}: cells")
+    live = ref["cell"] != oc_max()
+    for f in PTC_KEYS:
+        np.testing.assert_array_equal(got[f][live], ref[f][live], err_msg=f"box {k}: {f}")
+    gph = b.get_photons(k)
+    rph = sim.photons()
+    np.testing.assert_array_equal(gph["cell"], rph["cell"], err_msg=f"box {k}: photon cells")
+    live = rph["cell"] != oc_max()
+    for f in PH_KEYS:
+        np.testing.assert_array_equal(gph[f][live], rph[f][live],
+                                      err_msg=f"box {k}: photon {f}")
+    for name, which in (("E", apt.FIELD_E), ("B", apt.FIELD_B), ("J", apt.FIELD_J)):
+        for c in range(3):
+            np.testing.assert_array_equal(b.get_field(k, which, c), sim.f[name][c],
+                                          err_msg=f"box {k}: {name}{c + 1}")
+
+
+def oc_max():
+    return np.uint32(0xFFFFFFFF)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_fp64_bit_exact(graph):
+    apt = _apt()
+    b = bc.product(apt.APT_PREC_FP64, graph=graph)
+    boxes = bc.oracle_boxes(b)
+    for k, box in enumerate(boxes):
+        _compare_box(b, k, box)  # the loaded state
+    events = 0
+    for s in range(8):  # sorts after steps 3 and 6
+        b.step(1)
+        for box in boxes:
+            box.step()
+        for k, box in enumerate(boxes):
+            _compare_box(b, k, box)
+        events += sum(box.sim.ph_number for box in boxes)
+    assert events > 0, "the case must create photons"
+    assert any(box.sim.id_next > box.sim.number for box in boxes)
+    c = b.counters()
+    assert c["nan_momenta"] == 0 and c["capacity_overflow"] == 0
+
+
+def test_batch_graph_replay_equals_eager():
+    apt = _apt()
+    a = bc.product(apt.APT_PREC_FP64, graph=False, sort_interval=2)
+    g = bc.product(apt.APT_PREC_FP64, graph=True, sort_interval=2)
+    for _ in range(3):
+        a.step(3)
+        g.step(3)
+        for k in range(len(bc.INSTS)):
+            pa, pg = a.get_particles(k), g.get_particles(k)
+            for f in ("cell",) + PTC_KEYS:
+                np.testing.assert_array_equal(pa[f], pg[f])
+            ha, hg = a.get_photons(k), g.get_photons(k)
+            np.testing.assert_array_equal(ha["cell"], hg["cell"])
+            for c in range(3):
+                np.testing.assert_array_equal(a.get_field(k, apt.FIELD_E, c),
+                                              g.get_field(k, apt.FIELD_E, c))
+
+
+def test_batch_capacity_overflow_counted():
+    """A box whose slots run out counts the lost injections and pairs and
+    keeps running; the oracle drops the same ones."""
+    apt = _apt()
+    insts = [dict(bc.INSTS[0], cap=64 * 3 * 2 + 6, ph_cap=40)]
+    b = bc.product(apt.APT_PREC_FP64, insts=insts)
+    boxes = bc.oracle_boxes(b, insts=insts)
+    for _ in range(4):
+        b.step(1)
+        boxes[0].step()
+        _compare_box(b, 0, boxes[0])
+    assert b.counters()["capacity_overflow"] == boxes[0].sim.overflow > 0
+
+
+def test_batch_timing_and_sample():
+    apt = _apt()
+    insts = apt.sample_batch(4, seed=3)
+    for i in insts:  # shrink the draw to a test size, keep its parameters
+        i["N"] = max(16, i["N"] // 64)
+        i["ppc"] = max(2, i["ppc"] // 50)
+        i["inj"] = 1
+        i["cap"] = 4 * i["N"] * i["ppc"] + 256
+        i["ph_cap"] = i["cap"] // 4
+    b = apt.batch_ensemble(insts, precision=apt.APT_PREC_MIXED)
+    b.graph(True)
+    b.step(1)
+    b.timing(1)
+    b.step(5)
+    step_ms, push_ms, n = b.timing(0)
+    assert n == 5 and step_ms > 0 and 0 < push_ms <= step_ms
+
+
+def test_batch_mixed_against_fp64_oracle():
+    """Mixed form: FP32 push/fields with fixed-point current.  Emission is
+    switched off (gamma_thr huge) so slot order stays aligned; after 6
+    steps positions (cell + offset) agree to 2e-4 cells, momenta to 1e-4
+    of their scale, E/B/J to 1e-3 of the field scale."""
+    apt = _apt()
+    insts = [dict(i, gamma_thr=1e30) for i in bc.INSTS]
+    b = bc.product(apt.APT_PREC_MIXED, insts=insts, sort_interval=0)
+    boxes = bc.oracle_boxes(b, insts=insts, params=dict(bc.PARAMS, sort_interval=0))
+    b.step(6)
+    for box in boxes:
+        for _ in range(6):
+            box.step()
+    num, _ = b.counts()
+    for k, box in enumerate(boxes):
+        sim = box.sim
+        assert int(num[k]) == sim.number
+        got, ref = b.get_particles(k), sim.particles()
+        both = (got["cell"] != oc_max()) & (ref["cell"] != oc_max())
+        # a particle at a wall may leave in one form and not the other
+        assert np.count_nonzero(got["cell"] != ref["cell"]) <= max(2, len(ref["cell"]) // 500)
+        xg = got["cell"][both].astype(np.float64) + got["x1"][both]
+        xr = ref["cell"][both].astype(np.float64) + ref["x1"][both]
+        assert np.max(np.abs(xg - xr)) <= 2e-4
+        for f in ("p1", "p2", "p3"):
+            scale = max(1.0, np.max(np.abs(ref[f][both])))
+            assert np.max(np.abs(got[f][both] - ref[f][both])) <= 1e-4 * scale, f
+        for name, which in (("E", apt.FIELD_E), ("B", apt.FIELD_B), ("J", apt.FIELD_J)):
+            for c in range(3):
+                r = sim.f[name][c]
+                g = b.get_field(k, which, c)
+                scale = max(np.max(np.abs(r)), 1e-30)
+                assert np.max(np.abs(g - r)) <= 1e-3 * scale + 1e-7, (k, name, c)
