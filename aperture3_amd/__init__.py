@@ -166,6 +166,7 @@ SIGNATURES = {
     "apt_batch_destroy": [_P],
     "apt_batch_last_error": [_P],
     "apt_batch_get_jexp": [_P, C.POINTER(C.c_int)],
+    "apt_batch_size": [_P, C.POINTER(C.c_int)],
     "apt_batch_inst_seed": [_P, C.c_int, C.POINTER(C.c_uint64)],
     "apt_batch_load": [_P, C.POINTER(AptBatchInst)],
     "apt_batch_step": [_P, C.c_int],
@@ -174,6 +175,7 @@ SIGNATURES = {
     "apt_batch_sync": [_P],
     "apt_batch_counts": [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
     "apt_batch_get_counters": [_P, C.POINTER(AptCounters)],
+    "apt_batch_pushes": [_P, C.POINTER(C.c_uint64)],
     "apt_batch_timing": [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                          C.POINTER(C.c_uint64)],
     "apt_batch_get_particles": [_P, C.c_int, C.POINTER(AptParticles), C.c_uint64,
@@ -732,6 +734,11 @@ class batch_ensemble:
         c = AptCounters()
         self._check(self.lib.apt_batch_get_counters(self.b, C.byref(c)), "apt_batch_get_counters")
         return {f: getattr(c, f) for f, _ in AptCounters._fields_}
+
+    def pushes(self):
+        t = C.c_uint64()
+        self._check(self.lib.apt_batch_pushes(self.b, C.byref(t)), "apt_batch_pushes")
+        return t.value
 
     def timing(self, on=-1):
         sm, pm, n = C.c_double(), C.c_double(), C.c_uint64()

@@ -1,4 +1,5 @@
-"""Autotuner for the mixed form's tile shape, LDS halo and workgroup size.
+"""Autotuner for the mixed form's tile shape, LDS halo and workgroup size,
+and for the batched ensemble's chunk size and persistent grid.
 
 For a workload key (dimension, grid, particles per cell, drift) it builds the
 real configuration for every compiled candidate, runs a few warm-up steps,
@@ -119,8 +120,57 @@ def tune(dim, n, ppc, theta=0.05, drift=0.3, sort_interval=20, steps=12, verbose
     return entry
 
 
+# batched ensemble: particles per workgroup (chunk) x persistent
+# workgroups per CU
+BATCH_CANDIDATES = [(c, g) for c in (256, 512, 1024) for g in (2, 4, 8, 16)]
+
+
+def batch_key(n_inst: int, seed: int, sort_interval: int) -> str:
+    return f"batch_n{n_inst}_seed{seed}_sort{sort_interval}"
+
+
+def measure_batch(apt, insts, chunk, grid_mult, steps=8, sort_interval=20, seed=1):
+    """Average ms per graph-replayed ensemble step and per persistent push
+    for one (chunk, grid_mult)."""
+    b = apt.batch_ensemble(insts, precision=apt.APT_PREC_MIXED, seed=seed, chunk=chunk,
+                           grid_mult=grid_mult, sort_interval=sort_interval)
+    try:
+        b.graph(True)
+        b.step(2)  # step 0 (eager, E half step) and the graph capture
+        b.timing(1)
+        b.step(steps)
+        step_ms, push_ms, n = b.timing(0)
+        return {"step_ms": step_ms / n, "push_ms": push_ms / n}
+    finally:
+        b.close()
+
+
+def tune_batch(n_inst=256, seed=1, sort_interval=20, steps=8, verbose=True):
+    import aperture3_amd as apt
+    apt.load_library()
+    insts = apt.sample_batch(n_inst, seed)
+    results = []
+    for chunk, gm in BATCH_CANDIDATES:
+        t0 = time.time()
+        r = measure_batch(apt, insts, chunk, gm, steps=steps, sort_interval=sort_interval,
+                          seed=seed)
+        r.update({"chunk": chunk, "grid_mult": gm})
+        results.append(r)
+        if verbose:
+            print(f"chunk={chunk} grid_mult={gm}: step {r['step_ms']:.3f} ms, push "
+                  f"{r['push_ms']:.3f} ms ({time.time() - t0:.1f} s)", flush=True)
+    best = min(results, key=lambda r: r["step_ms"])
+    entry = dict(best)
+    entry["candidates"] = results
+    store(batch_key(n_inst, seed, sort_interval), entry)
+    return entry
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--batch", type=int, default=0,
+                    help="tune the batched ensemble of this many boxes instead")
+    ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--dim", type=int, default=2)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--ppc", type=int, default=32)
@@ -129,6 +179,10 @@ def main():
     ap.add_argument("--sort-interval", type=int, default=20)
     ap.add_argument("--steps", type=int, default=12)
     a = ap.parse_args()
+    if a.batch:
+        best = tune_batch(a.batch, a.seed, a.sort_interval, a.steps)
+        print(json.dumps({k: v for k, v in best.items() if k != "candidates"}))
+        return
     best = tune(a.dim, a.n, a.ppc, a.theta, a.drift, a.sort_interval, a.steps)
     print(json.dumps({k: v for k, v in best.items() if k != "candidates"}))
 

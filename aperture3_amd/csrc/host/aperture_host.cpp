@@ -119,6 +119,31 @@ sim_data::sim_data(sim_environment &e)
   q.random_seed = (uint64_t)p.random_seed;
   q.tile_halo = p.tile_halo;
   q.tile_threads = p.tile_threads;
+  if (p.coord_system == "LogSpherical")
+    q.coord = APT_COORD_LOGSPH;
+  else if (p.coord_system == "Cartesian")
+    q.coord = APT_COORD_CARTESIAN;
+  else
+    throw std::runtime_error("coord_system: " + p.coord_system +
+                             " is not built (Cartesian, LogSpherical)");
+  q.compactness = p.compactness;
+  q.omega = p.omega;
+  q.gravity_on = p.gravity_on ? 1 : 0;
+  q.gravity = p.gravity;
+  q.rad_cooling_on = p.rad_cooling_on ? 1 : 0;
+  q.rad_cooling_coef = p.rad_cooling_coef;
+  q.B0 = p.B0;
+  q.damping_length = p.damping_length;
+  q.damping_coef = p.damping_coef;
+  q.inject_ions = p.inject_ions ? 1 : 0;
+  q.track_percent = p.track_percent;
+  q.rt_model = p.rt_model;
+  q.gamma_thr = p.gamma_thr;
+  q.photon_path = p.photon_path;
+  q.E_secondary = p.E_secondary;
+  q.r_cutoff = p.r_cutoff;
+  q.e_frac = p.e_frac;
+  q.max_photon_number = p.max_photon_number;
   check(nullptr, apt_create(&g, &q, &ctx), "apt_create");
 }
 
@@ -190,6 +215,105 @@ void field_solver::update_e_field(sim_data &data, double dt) {
 
 void field_solver::update_b_field(sim_data &data, double dt) {
   check(data.ctx, apt_field_update_b(data.ctx, dt), "apt_field_update_b");
+}
+
+void ptc_updater_logsph::update_particles(sim_data &data, double dt,
+                                          uint32_t step) {
+  check(data.ctx, apt_ptc_update(data.ctx, dt, step), "apt_ptc_update");
+}
+
+void ptc_updater_logsph::apply_boundary(sim_data &, double, uint32_t) {
+  // applied inside update_particles (guard erase, axis J3, outflow flag)
+}
+
+void ptc_updater_logsph::inject_ptc(sim_data &data, int inj_per_cell,
+                                    Scalar p1, Scalar p2, Scalar p3, Scalar w,
+                                    Scalar omega) {
+  check(data.ctx, apt_inject_ptc(data.ctx, inj_per_cell, p1, p2, p3, w, omega),
+        "apt_inject_ptc");
+}
+
+void field_solver_logsph::update_fields(sim_data &data, double dt, double) {
+  check(data.ctx, apt_field_update(data.ctx, dt, 0), "apt_field_update");
+}
+
+void field_solver_logsph::apply_boundary(sim_data &data, double omega,
+                                         double time) {
+  check(data.ctx, apt_field_boundary(data.ctx, omega, time),
+        "apt_field_boundary");
+}
+
+uint64_t radiative_transfer::emit_photons(sim_data &data) {
+  uint64_t k = 0;
+  check(data.ctx, apt_emit_photons(data.ctx, &k), "apt_emit_photons");
+  return k;
+}
+
+uint64_t radiative_transfer::produce_pairs(sim_data &data) {
+  uint64_t k = 0;
+  check(data.ctx, apt_produce_pairs(data.ctx, &k), "apt_produce_pairs");
+  return k;
+}
+
+void ffe_solver_logsph::update_fields(sim_data &data, double dt, double omega,
+                                      double time) {
+  check(data.ctx, apt_ffe_update(data.ctx, dt, omega, time), "apt_ffe_update");
+}
+
+void ffe_solver_logsph::apply_boundary(sim_data &data, double omega,
+                                       double time) {
+  check(data.ctx, apt_ffe_boundary(data.ctx, omega, time), "apt_ffe_boundary");
+}
+
+void ffe_solver_logsph::clean_epar(sim_data &data) {
+  check(data.ctx, apt_ffe_clean_epar(data.ctx), "apt_ffe_clean_epar");
+}
+
+void ffe_solver_logsph::check_eGTb(sim_data &data) {
+  check(data.ctx, apt_ffe_check_egtb(data.ctx), "apt_ffe_check_egtb");
+}
+
+void batch_ensemble::check(apt_status st, const char *what) {
+  if (st == APT_OK) return;
+  std::string msg = std::string(what) + " failed (status " +
+                    std::to_string(st) + ")";
+  if (m_b) msg += ": " + std::string(apt_batch_last_error(m_b));
+  throw std::runtime_error(msg);
+}
+
+batch_ensemble::batch_ensemble(const std::vector<apt_batch_inst> &boxes,
+                               const apt_batch_params &params, bool load) {
+  check(apt_batch_create(boxes.data(), (int)boxes.size(), &params, &m_b),
+        "apt_batch_create");
+  if (load) check(apt_batch_load(m_b, boxes.data()), "apt_batch_load");
+}
+
+batch_ensemble::~batch_ensemble() {
+  if (m_b) apt_batch_destroy(m_b);
+}
+
+void batch_ensemble::step(int nsteps) {
+  check(apt_batch_step(m_b, nsteps), "apt_batch_step");
+}
+
+void batch_ensemble::use_graph(bool on) {
+  check(apt_batch_graph(m_b, on ? 1 : 0), "apt_batch_graph");
+}
+
+void batch_ensemble::sort() { check(apt_batch_sort(m_b), "apt_batch_sort"); }
+
+std::vector<uint64_t> batch_ensemble::num_particles() {
+  int n = 0;
+  check(apt_batch_size(m_b, &n), "apt_batch_size");
+  std::vector<uint64_t> v((size_t)n);
+  check(apt_batch_counts(m_b, v.data(), nullptr), "apt_batch_counts");
+  return v;
+}
+
+uint64_t batch_ensemble::pushes() {
+  uint64_t t = 0;
+  check(apt_batch_pushes(m_b, &t), "apt_batch_pushes");
+  return t;
 }
 
 }  // namespace Aperture

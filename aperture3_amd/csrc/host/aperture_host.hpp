@@ -35,11 +35,34 @@ struct SimParams {
   int guard[3] = {0, 0, 0};
   Scalar lower[3] = {0.0, 0.0, 0.0};
   Scalar size[3] = {1.0, 1.0, 1.0};
-  // not in the reference: numeric form and mixed-form tile
+  // log-spherical path (sim_params.h:31-64, coord_system :101)
+  std::string coord_system = "Cartesian";  // or "LogSpherical"
+  Scalar compactness = 0.5;
+  Scalar omega = 0.2;
+  bool gravity_on = false;
+  Scalar gravity = 0.0;
+  bool rad_cooling_on = false;
+  Scalar rad_cooling_coef = 1.0;
+  Scalar B0 = 1.0;
+  int damping_length = 30;
+  Scalar damping_coef = 0.002;
+  bool inject_ions = true;
+  Scalar track_percent = 0.2;
+  // pair creation (sim_params.h:23, 52-79)
+  uint64_t max_photon_number = 0;
+  Scalar gamma_thr = 20.0;
+  Scalar photon_path = 1.0;
+  Scalar E_secondary = 4.0;
+  Scalar r_cutoff = 4.0;
+  // not in the reference: numeric form, mixed-form tile, pair-creation
+  // model (APT_RT_DEFAULT: the default hooks; APT_RT_EXP_PATH: exponential
+  // free path, photon energy e_frac gamma)
   int precision = APT_PREC_FP64;
   int tile[3] = {0, 0, 0};
   int tile_halo = 0;
   int tile_threads = 0;
+  int rt_model = APT_RT_DEFAULT;
+  Scalar e_frac = 0.2;
 };
 
 void check(apt_ctx *ctx, apt_status st, const char *what);
@@ -120,6 +143,82 @@ class field_solver {
 
  private:
   sim_environment &m_env;
+};
+
+// ptc_updater_logsph (algorithms/ptc_updater_logsph.h:13-28) on a 2D
+// log-spherical context.  update_particles already applies the particle
+// boundary (guard erase, axis J3, outflow flag), as the reference's
+// update_particles does before returning.
+class ptc_updater_logsph {
+ public:
+  explicit ptc_updater_logsph(sim_environment &env) : m_env(env) {}
+  void update_particles(sim_data &data, double dt, uint32_t step = 0);
+  void apply_boundary(sim_data &data, double dt, uint32_t step = 0);
+  void inject_ptc(sim_data &data, int inj_per_cell, Scalar p1, Scalar p2,
+                  Scalar p3, Scalar w, Scalar omega);
+
+ protected:
+  sim_environment &m_env;
+};
+
+// field_solver_logsph (algorithms/field_solver_logsph.h:10-20)
+class field_solver_logsph {
+ public:
+  explicit field_solver_logsph(sim_environment &env) : m_env(env) {}
+  void update_fields(sim_data &data, double dt, double time = 0.0);
+  void apply_boundary(sim_data &data, double omega, double time = 0.0);
+
+ private:
+  sim_environment &m_env;
+};
+
+// radiative_transfer (radiation/radiative_transfer.h:11-28)
+class radiative_transfer {
+ public:
+  explicit radiative_transfer(sim_environment &env) : m_env(env) {}
+  void initialize() {}  // the built-in hook models keep no state
+  // the reference returns void; the counts are returned for convenience
+  uint64_t emit_photons(sim_data &data);
+  uint64_t produce_pairs(sim_data &data);
+
+ protected:
+  sim_environment &m_env;
+};
+
+// ffe_solver_logsph (algorithms/ffe_solver_logsph.h:12-33) on a 3D
+// log-spherical context
+class ffe_solver_logsph {
+ public:
+  explicit ffe_solver_logsph(sim_environment &env) : m_env(env) {}
+  void update_fields(sim_data &data, double dt, double omega,
+                     double time = 0.0);
+  void apply_boundary(sim_data &data, double omega, double time = 0.0);
+  void clean_epar(sim_data &data);
+  void check_eGTb(sim_data &data);
+
+ private:
+  sim_environment &m_env;
+};
+
+// The batched ensemble of 1D open gap boxes (no reference class: the
+// reference runs one labs/1dgap box per process).  RAII over apt_batch_*.
+class batch_ensemble {
+ public:
+  batch_ensemble(const std::vector<apt_batch_inst> &boxes,
+                 const apt_batch_params &params, bool load = true);
+  ~batch_ensemble();
+  batch_ensemble(const batch_ensemble &) = delete;
+  batch_ensemble &operator=(const batch_ensemble &) = delete;
+  void step(int nsteps = 1);
+  void use_graph(bool on);
+  void sort();
+  std::vector<uint64_t> num_particles();
+  uint64_t pushes();
+  apt_batch *handle() { return m_b; }
+
+ private:
+  void check(apt_status st, const char *what);
+  apt_batch *m_b = nullptr;
 };
 
 }  // namespace Aperture

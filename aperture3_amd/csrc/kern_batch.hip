@@ -115,11 +115,14 @@ __global__ void __launch_bounds__(1024)
     k_b_push(const BInst *__restrict__ inst, int n, const BCount *__restrict__ cnt,
              BPlan plan, Species sp, int H, PtcView<R> p, BFields<R> f,
              uint8_t *__restrict__ eflag, uint32_t *__restrict__ echunk,
-             unsigned long long *__restrict__ counters) {
+             unsigned long long *__restrict__ counters,
+             unsigned long long *__restrict__ pushed) {
   __shared__ unsigned long long s_chunk;
   __shared__ uint32_t s_cnt;
+  __shared__ unsigned long long s_live;
   const uint64_t total = plan.chunk_off[n];
   const uint32_t CH = blockDim.x;
+  if (threadIdx.x == 0) s_live = 0;
   for (;;) {
     if (threadIdx.x == 0) {
       s_chunk = atomicAdd(&plan.work[0], 1ull);
@@ -131,7 +134,7 @@ __global__ void __launch_bounds__(1024)
     const int k = find_inst(plan.chunk_off, n, chunk);
     const BInst &I = inst[k];
     const uint64_t local = (chunk - plan.chunk_off[k]) * CH + threadIdx.x;
-    int emit = 0;
+    int emit = 0, live = 0;
     if (local < cnt[k].number) {
       const uint64_t o = I.base + local;
       const Mesh &m = I.m;
@@ -142,6 +145,7 @@ __global__ void __launch_bounds__(1024)
         if (!(c1 >= m.guard[0] && c1 < m.dims[0] - m.guard[0])) {
           p.cell[o] = MAX_CELL;
         } else {
+          live = 1;
           const uint32_t flag = p.flag[o];
           const int s = ptc_species(flag);
           const R dt = (R)I.dt;
@@ -232,11 +236,17 @@ __global__ void __launch_bounds__(1024)
     // per-chunk emitter count (slot order inside the chunk is recovered
     // by the emit pass)
     unsigned long long bal = __ballot(emit);
-    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(&s_cnt, (uint32_t)__popcll(bal));
+    const unsigned long long lbal = __ballot(live);
+    if ((threadIdx.x & 63) == 0) {
+      if (bal) atomicAdd(&s_cnt, (uint32_t)__popcll(bal));
+      if (lbal) atomicAdd(&s_live, (unsigned long long)__popcll(lbal));
+    }
     __syncthreads();
     if (threadIdx.x == 0) echunk[chunk] = s_cnt;
     __syncthreads();
   }
+  // particles pushed by this workgroup (one device atomic per workgroup)
+  if (threadIdx.x == 0 && s_live) atomicAdd(pushed, s_live);
 }
 
 // ---------------------------------------------------------------------
@@ -693,6 +703,7 @@ struct apt_batch {
   uint64_t max_chunks = 0, max_ph_chunks = 0;
   Species sp{};
   uint32_t *d_epoch = nullptr;  // step counter read by the kernels
+  unsigned long long *d_pushed = nullptr;  // particles pushed, all steps
   unsigned push_grid = 0;       // persistent push workgroups
   int H = 22;
   int cbits = 15;
@@ -791,7 +802,8 @@ static apt_status batch_step_t(apt_batch *b, PtcView<R> p, PhoView<R> ph,
   k_b_plan<<<1, 64, 0, st>>>(b->d_cnt, n, CH, b->plan);
   if (b->timing) BCHK(hipEventRecord(b->evp0, st));
   k_b_push<R><<<b->push_grid, CH, 0, st>>>(b->d_inst, n, b->d_cnt, b->plan, b->sp, b->H,
-                                           p, f, b->eflag, b->echunk, b->d_counters);
+                                           p, f, b->eflag, b->echunk, b->d_counters,
+                                           b->d_pushed);
   if (b->timing) BCHK(hipEventRecord(b->evp1, st));
   k_b_photon_move<R><<<at_least_one(b->max_ph_chunks), CH, 0, st>>>(b->d_inst, n, b->d_cnt,
                                                                 b->plan, ph);
@@ -958,6 +970,7 @@ static void free_all(apt_batch *b) {
   F(b->plan.work); F(b->d_boff); F(b->d_ioff); F(b->d_slot_off); F(b->d_ph_off);
   F(b->d_counters);
   F(b->d_epoch);
+  F(b->d_pushed);
   for (PtcF64 *p : {&b->p64, &b->p64b}) {
     void *a[] = {p->cell, p->x1, p->x2, p->x3, p->p1, p->p2, p->p3, p->E, p->w, p->flag, p->id};
     for (void *q : a) F(q);
@@ -1172,6 +1185,8 @@ apt_status apt_batch_create(const apt_batch_inst *specs, int n,
   }
   ok = ok && dalloc(&b->d_epoch, 1) &&
        launch_fill<uint32_t>(b->d_epoch, 0u, 1, b->stream) == hipSuccess;
+  ok = ok && dalloc(&b->d_pushed, 1) &&
+       launch_fill<long long>((long long *)b->d_pushed, 0, 1, b->stream) == hipSuccess;
   for (hipEvent_t *e : {&b->ev0, &b->ev1, &b->evp0, &b->evp1})
     ok = ok && hipEventCreate(e) == hipSuccess;
   // persistent push: grid_mult workgroups per CU, never more than chunks
@@ -1201,6 +1216,12 @@ apt_status apt_batch_destroy(apt_batch *b) {
 
 const char *apt_batch_last_error(const apt_batch *b) {
   return b ? b->err.c_str() : "null batch";
+}
+
+apt_status apt_batch_size(const apt_batch *b, int *n) {
+  if (!b || !n) return APT_ERR_INVALID_ARG;
+  *n = b->n;
+  return APT_OK;
 }
 
 apt_status apt_batch_get_jexp(const apt_batch *b, int *H) {
@@ -1343,6 +1364,15 @@ apt_status apt_batch_counts(apt_batch *b, uint64_t *number, uint64_t *ph_number)
     if (number) number[k] = cnt[k].number;
     if (ph_number) ph_number[k] = cnt[k].ph_number;
   }
+  return APT_OK;
+}
+
+apt_status apt_batch_pushes(apt_batch *b, uint64_t *total) {
+  if (!b || !total) return APT_ERR_INVALID_ARG;
+  unsigned long long h = 0;
+  BCHK(hipStreamSynchronize(b->stream));
+  BCHK(hipMemcpy(&h, b->d_pushed, sizeof(h), hipMemcpyDeviceToHost));
+  *total = h;
   return APT_OK;
 }
 

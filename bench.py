@@ -103,6 +103,65 @@ def measure(apt, precision, steps, warmup):
     }
 
 
+C5_BOXES = 256
+C5_SEED = 1
+C5_SORT = 20
+C5_BYTES_PER_PUSH = 65  # read 36 B (cell, x1-3, u1-3, w, flag), write 28 B + emit flag
+
+
+def measure_c5(apt, steps, warmup):
+    """The batched ensemble: 256 1D open gap boxes drawn by apt_batch_sample
+    (cells log-uniform in [1024, 16384], ppc U[50, 400] per species, B_x
+    U[0.5, 2], theta U[0.01, 0.3], gamma_thr U[1e2, 1e4]), wall injection of
+    ppc/8 pairs per step, exponential free path 0.2 with photon energy 0.1
+    gamma, mixed form, one graph replay per step, sort every 20 steps."""
+    from aperture3_amd import autotune
+    insts = apt.sample_batch(C5_BOXES, C5_SEED)
+    tuned = autotune.lookup(autotune.batch_key(C5_BOXES, C5_SEED, C5_SORT)) or {}
+    chunk, gm = tuned.get("chunk", 256), tuned.get("grid_mult", 8)
+    b = apt.batch_ensemble(insts, precision=apt.APT_PREC_MIXED, seed=C5_SEED, chunk=chunk,
+                           grid_mult=gm, sort_interval=C5_SORT, photon_path=0.2,
+                           e_frac=0.1, r_cutoff=1e30)
+    try:
+        b.graph(True)
+        n0 = int(b.counts()[0].sum())
+        b.step(max(warmup, 2))
+        b.sync()
+        p0 = b.pushes()
+        b.timing(1)
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        t1 = time.perf_counter()
+        step_ms, push_ms, n = b.timing(0)
+        pushed = b.pushes() - p0
+        num, phn = b.counts()
+        counters = b.counters()
+    finally:
+        b.close()
+    el = t1 - t0
+    per_step = pushed / steps
+    achieved = per_step * C5_BYTES_PER_PUSH / (push_ms / n * 1e-3) / 1e9
+    return {
+        "workload": "256 independent 1D open gap boxes (apt_batch_sample seed 1): cells "
+                    "log-uniform [1024, 16384], ppc U[50,400] per species, B_x U[0.5,2], "
+                    "theta U[0.01,0.3], gamma_thr U[1e2,1e4]; left-wall injection ppc/8 "
+                    "pairs/step, photon path Exp(0.2), E_ph = 0.1 gamma; mixed form; "
+                    "graph-replayed step; sort every 20",
+        "boxes": C5_BOXES, "cells": int(sum(i["N"] for i in insts)),
+        "particles_initial": n0, "particles_final": int(num.sum()),
+        "photons_final": int(phn.sum()),
+        "steps": steps, "warmup": max(warmup, 2),
+        "pushes_per_s": pushed / el, "ms_per_step": 1e3 * el / steps,
+        "step_ms_events": step_ms / n, "push_kernel_ms": push_ms / n,
+        "chunk": chunk, "grid_mult": gm, "tuned": bool(tuned),
+        "roofline": {"bound": "hbm", "kernel": "persistent batch push (mixed)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "bytes_per_push": C5_BYTES_PER_PUSH},
+        "counters": counters,
+    }
+
+
 def profiled_traffic():
     """HBM bytes per launch of the push kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
@@ -149,6 +208,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--c5-steps", type=int, default=1000,
+                    help="steps of the batched-ensemble measurement (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (read 1 GiB + write 1 GiB), "
                          "the reference point for the FETCH_SIZE/WRITE_SIZE counters")
@@ -223,6 +284,8 @@ def main():
         line["fp64"] = fp64
         line["fp64_roofline_frac"] = (fp64["particles"] * BYTES_PER_PUSH["fp64"]
                                       / (fp64["push_kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+    if args.c5_steps > 0:
+        line["batched_ensemble"] = measure_c5(apt, args.c5_steps, args.warmup)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line))

@@ -344,6 +344,8 @@ apt_status apt_batch_create(const apt_batch_inst *specs, int n,
                             const apt_batch_params *params, apt_batch **out);
 apt_status apt_batch_destroy(apt_batch *b);
 const char *apt_batch_last_error(const apt_batch *b);
+/* number of boxes */
+apt_status apt_batch_size(const apt_batch *b, int *n);
 /* the fixed-point exponent of the current accumulators */
 apt_status apt_batch_get_jexp(const apt_batch *b, int *H);
 apt_status apt_batch_inst_seed(const apt_batch *b, int inst, uint64_t *seed);
@@ -357,6 +359,8 @@ apt_status apt_batch_sort(apt_batch *b);
 apt_status apt_batch_sync(apt_batch *b);
 apt_status apt_batch_counts(apt_batch *b, uint64_t *number, uint64_t *ph_number);
 apt_status apt_batch_get_counters(apt_batch *b, apt_counters *out);
+/* particles pushed (live at push time) over all steps so far */
+apt_status apt_batch_pushes(apt_batch *b, uint64_t *total);
 /* on = 1 start, 0 stop, -1 read only; sums since the last switch */
 apt_status apt_batch_timing(apt_batch *b, int on, double *step_ms, double *push_ms,
                             uint64_t *steps);
