@@ -89,4 +89,34 @@ __device__ __forceinline__ void acc_fx(long long *__restrict__ acc,
                            __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// hardware FP32 square root and reciprocal (<= 1 ulp); arguments here are
+// >= 1 or well inside the normal range
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Vay push (vay_push.cuh:8-30) in FP32 with the hardware reciprocal and
+// square root; returns 1/gamma of the new momentum.  Contraction follows
+// the including file or block.
+__device__ __forceinline__ float vay_push_f(float &p1, float &p2, float &p3,
+                                           float gamma, float E1, float E2,
+                                           float E3, float B1, float B2,
+                                           float B3) {
+#pragma clang fp contract(fast)
+  float ig = hw_rcp(gamma);
+  float up1 = p1 + 2.0f * E1 + (p2 * B3 - p3 * B2) * ig;
+  float up2 = p2 + 2.0f * E2 + (p3 * B1 - p1 * B3) * ig;
+  float up3 = p3 + 2.0f * E3 + (p1 * B2 - p2 * B1) * ig;
+  float tt = B1 * B1 + B2 * B2 + B3 * B3;
+  float ut = up1 * B1 + up2 * B2 + up3 * B3;
+  float sigma = 1.0f + up1 * up1 + up2 * up2 + up3 * up3 - tt;
+  float inv_gamma2 =
+      2.0f * hw_rcp(sigma + hw_sqrt(sigma * sigma + 4.0f * (tt + ut * ut)));
+  float s = hw_rcp(1.0f + inv_gamma2 * tt);
+  float ign = hw_sqrt(inv_gamma2);
+  p1 = (up1 + B1 * ut * inv_gamma2 + (up2 * B3 - up3 * B2) * ign) * s;
+  p2 = (up2 + B2 * ut * inv_gamma2 + (up3 * B1 - up1 * B3) * ign) * s;
+  p3 = (up3 + B3 * ut * inv_gamma2 + (up1 * B2 - up2 * B1) * ign) * s;
+  return ign;
+}
+
 }  // namespace apt

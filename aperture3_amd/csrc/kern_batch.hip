@@ -8,10 +8,12 @@
 // Per step, all instances at once:
 //   plan      one workgroup turns the per-instance particle counts (device
 //             memory) into a chunk table: CH particles per chunk;
-//   push      ONE persistent launch: workgroups pull chunks from a device
-//             work counter until the table is exhausted (every wave reaches
-//             the exit); push + move + deposit per particle, open-wall
-//             removal, and the emit decision with a per-chunk count;
+//   push      ONE persistent launch: each workgroup walks a contiguous run
+//             of the chunk table (bounds computed from the device counts,
+//             so every wave reaches the exit); push + move + deposit per
+//             particle (current summed per wavefront, then in an LDS
+//             window per chunk), open-wall removal, and the emit decision
+//             with a per-chunk count;
 //   fields    finalize J, B update, E update over every instance's cells;
 //   inject    wall pairs appended at each instance's count;
 //   emit      chunk counts -> 3-kernel scan -> photons written in slot order;
@@ -21,6 +23,7 @@
 // R = double reproduces the single-box CPU restatement bit for bit; R = float
 // is the mixed form.
 #include <hipcub/hipcub.hpp>
+#include <limits.h>
 #include <string.h>
 
 #include <string>
@@ -47,6 +50,7 @@ struct BInst {
   uint64_t boff;             // first bulk cell in the ensemble's bulk list
   uint64_t seed;
   double gamma_thr, theta, weight;
+  double qm[8];  // dt q / (2 m) per species, FP64 (ptc_push_cart_1d order)
   int inj;
 };
 
@@ -58,7 +62,6 @@ struct BCount {
 struct BPlan {
   uint64_t *chunk_off;     // n + 1: first chunk of each instance (particles)
   uint64_t *ph_chunk_off;  // n + 1: same for photons
-  unsigned long long *work;  // [0] particle chunks, [1] photon chunks
 };
 
 __device__ inline int find_inst(const uint64_t *off, int n, uint64_t x) {
@@ -71,7 +74,7 @@ __device__ inline int find_inst(const uint64_t *off, int n, uint64_t x) {
   return lo;
 }
 
-// chunk tables from the device counts; resets the work counters
+// chunk tables from the device counts
 __global__ void k_b_plan(const BCount *cnt, int n, uint32_t CH, BPlan plan) {
   if (threadIdx.x != 0) return;
   uint64_t a = 0, b = 0;
@@ -83,8 +86,6 @@ __global__ void k_b_plan(const BCount *cnt, int n, uint32_t CH, BPlan plan) {
   }
   plan.chunk_off[n] = a;
   plan.ph_chunk_off[n] = b;
-  plan.work[0] = 0;
-  plan.work[1] = 0;
 }
 
 template <typename R>
@@ -108,61 +109,326 @@ __device__ inline R gather1R(const R *__restrict__ f, int c, R x, int st) {
 }
 
 // ---------------------------------------------------------------------
+// wave-level helper (64 lanes, every lane active)
+// ---------------------------------------------------------------------
+constexpr int LDS_W = 64;     // current window of a chunk, cells
+constexpr int NCOPY = 4;      // LDS copies of the current window
+
+__device__ __forceinline__ int wave_min_i32(int v) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) v = min(v, __shfl_xor(v, d, 64));
+  return v;
+}
+
+// Where a workgroup is in the chunk table: the instance of a chunk and
+// that instance's uniform data.  A workgroup walks a contiguous run of
+// chunks, so the instance changes rarely; the loads that follow a change
+// are the only metadata loads of the loop.
+struct ICur {
+  int k;
+  uint64_t off0, off1;  // chunk_off[k], chunk_off[k + 1]
+  uint64_t base, number, fbase;
+  double dt, delta, gamma_thr;
+  int lo, hi;  // bulk cells [lo, hi)
+};
+
+__device__ __forceinline__ void icur_set(ICur &c, int k, const uint64_t *__restrict__ off,
+                                         const BInst *__restrict__ inst,
+                                         const BCount *__restrict__ cnt) {
+  c.k = k;
+  c.off0 = off[k];
+  c.off1 = off[k + 1];
+  c.base = inst[k].base;
+  c.number = cnt[k].number;
+  c.fbase = inst[k].fbase;
+  c.dt = inst[k].dt;
+  c.delta = inst[k].m.delta[0];
+  c.gamma_thr = inst[k].gamma_thr;
+  c.lo = inst[k].m.guard[0];
+  c.hi = inst[k].m.dims[0] - inst[k].m.guard[0];
+}
+
+// move the cursor forward to the instance of `chunk` (uniform)
+__device__ __forceinline__ void icur_to(ICur &c, uint64_t chunk,
+                                        const uint64_t *__restrict__ off,
+                                        const BInst *__restrict__ inst,
+                                        const BCount *__restrict__ cnt) {
+  if (chunk < c.off1) return;
+  int k = c.k + 1;
+  while (off[k + 1] <= chunk) k++;  // skips instances without chunks
+  icur_set(c, k, off, inst, cnt);
+}
+
+// one particle per lane of a chunk, as loaded (the software pipeline's
+// register stage)
+template <typename R>
+struct PReg {
+  bool valid;     // the lane's slot is below the instance's count
+  uint64_t o;     // slot
+  uint32_t cell, flag;
+  R x1, x2, x3, p1, p2, p3, w;
+};
+
+template <typename R>
+__device__ __forceinline__ void load_chunk(const ICur &c, const PtcView<R> &p,
+                                           uint64_t chunk, uint32_t CH, PReg<R> &r) {
+  const uint64_t local = (chunk - c.off0) * CH + threadIdx.x;
+  r.valid = local < c.number;
+  r.o = c.base + local;
+  // straight-line loads (a lane past the count reads the instance's first
+  // slot), so the compiler can count them and wait only for what it uses
+  const uint64_t o = r.valid ? r.o : c.base;
+  r.cell = p.cell[o];
+  r.flag = p.flag[o];
+  r.x1 = p.x1[o];
+  r.x2 = p.x2[o];
+  r.x3 = p.x3[o];
+  r.p1 = p.p1[o];
+  r.p2 = p.p2[o];
+  r.p3 = p.p3[o];
+  r.w = p.w[o];
+  if (!r.valid) r.cell = MAX_CELL;
+}
+
+// The field taps of E1..3, B1..3 around a particle, loaded in one go and
+// evaluated after the next chunk's loads are issued.  FP64: the three taps
+// c-1, c, c+1 summed in the order of gather1 (Interpolator1D,
+// interpolation.cuh:91-120), so the result is bit-identical; FP32: the two
+// non-zero taps.
+template <typename R>
+struct FTaps {
+  static constexpr int NT = std::is_same<R, double>::value ? 3 : 2;
+  R e[3][NT], b[3][NT];
+  int c1;
+  __device__ __forceinline__ void load(const BFields<R> &f, uint64_t fb, int c, R x1) {
+    const int ste[3] = {ST_E1, ST_E2, ST_E3}, stb[3] = {ST_B1, ST_B2, ST_B3};
+    c1 = c;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      if constexpr (NT == 3) {
+        const R *E = f.E[q] + fb + (c - 1);
+        const R *B = f.B[q] + fb + (c - 1);
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+          e[q][t] = E[t];
+          b[q][t] = B[t];
+        }
+      } else {
+        // upper node (st = 1) or x < 1/2: taps c-1, c; else c, c+1
+        const int oe = ((ste[q] & 1) || x1 < (R)0.5) ? -1 : 0;
+        const int ob = ((stb[q] & 1) || x1 < (R)0.5) ? -1 : 0;
+        const R *E = f.E[q] + fb + (c + oe);
+        const R *B = f.B[q] + fb + (c + ob);
+        e[q][0] = E[0];
+        e[q][1] = E[1];
+        b[q][0] = B[0];
+        b[q][1] = B[1];
+      }
+    }
+  }
+  __device__ __forceinline__ R eval1(const R *v, R x1, int st) const {
+    if constexpr (NT == 3) {
+      R r = 0.0;
+#pragma unroll
+      for (int i = 0; i <= 2; i++) r += v[i] * interp_cell(x1, c1, i + c1 - 1, st);
+      return r;
+    } else {
+      int o;
+      float w0, w1;
+      taps(x1, st, o, w0, w1);
+      return v[0] * w0 + v[1] * w1;
+    }
+  }
+  __device__ __forceinline__ void eval(R x1, R qm, R &e1, R &e2, R &e3, R &b1, R &b2,
+                                       R &b3) const {
+    e1 = eval1(e[0], x1, ST_E1 & 1) * qm;
+    e2 = eval1(e[1], x1, ST_E2 & 1) * qm;
+    e3 = eval1(e[2], x1, ST_E3 & 1) * qm;
+    b1 = eval1(b[0], x1, ST_B1 & 1) * qm;
+    b2 = eval1(b[1], x1, ST_B2 & 1) * qm;
+    b3 = eval1(b[2], x1, ST_B3 & 1) * qm;
+  }
+};
+
+// The mixed form's particle update, straight-line, with contraction: the
+// Vay push with hardware reciprocal/square root (1/gamma comes out of the
+// push), the move in cell units, and the FP32 Esirkepov contributions
+// (J1 at cells c+i+1, J2/J3 at c+i, i = -2..1; zero outside [i0, i1]).
+struct MixedOut {
+  float x1, x2, x3, p1, p2, p3, ig;
+  int dc;
+  bool nan, multi;
+};
+
+__device__ __forceinline__ void push_mixed_1d(float x1, float x2, float x3, float p1,
+                                              float p2, float p3, float w,
+                                              const FTaps<float> &ft, bool em, float qm,
+                                              float q, float dt, float dt_dx, bool cur,
+                                              MixedOut &o, float *g1, float *g2,
+                                              float *g3) {
+#pragma clang fp contract(fast)
+  o.nan = p1 != p1 || p2 != p2 || p3 != p3;
+  const float gamma = hw_sqrt(1.0f + p1 * p1 + p2 * p2 + p3 * p3);
+  float ig;
+  if (em) {
+    float e1, e2, e3, b1, b2, b3;
+    ft.eval(x1, qm, e1, e2, e3, b1, b2, b3);
+    ig = vay_push_f(p1, p2, p3, gamma, e1, e2, e3, b1, b2, b3);
+  } else {
+    ig = hw_rcp(gamma);
+  }
+  const float v1 = p1 * ig, v2 = p2 * ig, v3 = p3 * ig;
+  float nx = x1 + v1 * dt_dx;
+  int dc = (int)floorf(nx);
+  o.multi = dc > 1 || dc < -1;
+  dc = dc > 1 ? 1 : (dc < -1 ? -1 : dc);
+  nx -= (float)dc;
+  nx = nx < 1.0f ? nx : 0x1.fffffep-1f;
+  o.x1 = nx;
+  o.x2 = x2 + v2 * dt;
+  o.x3 = x3 + v3 * dt;
+  o.p1 = p1;
+  o.p2 = p2;
+  o.p3 = p3;
+  o.ig = ig;
+  o.dc = dc;
+  const float wq = cur ? q * w : 0.0f;
+  const int i0 = (dc == -1 ? -2 : -1), i1 = (dc == 1 ? 1 : 0);
+  float djx = 0.0f;
+#pragma unroll
+  for (int i = -2; i <= 1; i++) {
+    const bool on = i >= i0 && i <= i1;
+    const float sx0 = cicf(-x1 + (float)(i + 1));
+    const float sx1 = cicf(-nx + (float)(i + 1 - dc));
+    djx += on ? sx1 - sx0 : 0.0f;
+    const float val1 = on ? 0.5f * (sx0 + sx1) : 0.0f;
+    g1[i + 2] = on ? -wq * djx : 0.0f;
+    g2[i + 2] = wq * v2 * val1;
+    g3[i + 2] = wq * v3 * val1;
+  }
+}
+
+// ---------------------------------------------------------------------
 // the persistent push (ptc_push_cart_1d + deposit_current_cart_1d)
 // ---------------------------------------------------------------------
+// FP64 keeps the register budget of 512-thread workgroups (no spills)
 template <typename R>
-__global__ void __launch_bounds__(1024)
+struct PushBounds {
+  static constexpr int value = std::is_same<R, double>::value ? 512 : 1024;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(PushBounds<R>::value)
     k_b_push(const BInst *__restrict__ inst, int n, const BCount *__restrict__ cnt,
-             BPlan plan, Species sp, int H, PtcView<R> p, BFields<R> f,
-             uint8_t *__restrict__ eflag, uint32_t *__restrict__ echunk,
+             const uint64_t *__restrict__ chunk_off, Species sp, int H, PtcView<R> p,
+             BFields<R> f, uint8_t *__restrict__ eflag, uint32_t *__restrict__ echunk,
              unsigned long long *__restrict__ counters,
              unsigned long long *__restrict__ pushed) {
-  __shared__ unsigned long long s_chunk;
+  constexpr int LIMBS = std::is_same<R, double>::value ? 2 : 1;
   __shared__ uint32_t s_cnt;
   __shared__ unsigned long long s_live;
-  const uint64_t total = plan.chunk_off[n];
+  __shared__ int s_cmin;
+  __shared__ long long s_acc[LIMBS][NCOPY][3 * LDS_W];  // J1..J3 over the window
+  __shared__ R s_qm[8], s_q[8];  // per species: dt q / 2m (instance), q
+  const uint64_t total = chunk_off[n];
   const uint32_t CH = blockDim.x;
   if (threadIdx.x == 0) s_live = 0;
-  for (;;) {
+  if (threadIdx.x < 8) s_q[threadIdx.x] = (R)sp.q[threadIdx.x];
+  for (int t = threadIdx.x; t < 3 * LDS_W * LIMBS * NCOPY; t += blockDim.x)
+    (&s_acc[0][0][0])[t] = 0;
+  // each workgroup owns a contiguous run of chunks: neighbouring chunks
+  // (which share boundary cells of the current) stay on one workgroup and
+  // one XCD, and no work counter is contended
+  const uint64_t per = (total + gridDim.x - 1) / gridDim.x;
+  const uint64_t c_lo = (uint64_t)blockIdx.x * per;
+  const uint64_t c_hi = c_lo + per < total ? c_lo + per : total;
+  // Software pipeline: the particle arrays of the next chunk are requested
+  // while the current one is pushed, so each chunk costs one memory round
+  // trip for its particles (plus the field taps, which sit in L2).
+  PReg<R> cur, nxt;
+  ICur ic{}, in{};  // cursors of the current and the prefetched chunk
+  int tab_k = -1;   // instance whose dt q / 2m is in s_qm
+  if (c_lo < c_hi) {
+    icur_set(ic, find_inst(chunk_off, n, c_lo), chunk_off, inst, cnt);
+    in = ic;
+    load_chunk(ic, p, c_lo, CH, cur);
+  }
+  for (uint64_t chunk = c_lo; chunk < c_hi; chunk++) {  // uniform bounds
+    const int k = ic.k;
     if (threadIdx.x == 0) {
-      s_chunk = atomicAdd(&plan.work[0], 1ull);
       s_cnt = 0;
+      s_cmin = INT_MAX;
+    }
+    if (k != tab_k) {  // uniform; the barrier below publishes the table
+      if (threadIdx.x < 8) s_qm[threadIdx.x] = (R)inst[k].qm[threadIdx.x];
+      tab_k = k;
     }
     __syncthreads();
-    const uint64_t chunk = s_chunk;
-    if (chunk >= total) break;  // uniform: every wave of the group exits
-    const int k = find_inst(plan.chunk_off, n, chunk);
-    const BInst &I = inst[k];
-    const uint64_t local = (chunk - plan.chunk_off[k]) * CH + threadIdx.x;
+    const uint64_t o = cur.o;
     int emit = 0, live = 0;
-    if (local < cnt[k].number) {
-      const uint64_t o = I.base + local;
-      const Mesh &m = I.m;
-      const uint32_t c = p.cell[o];
-      eflag[o] = 0;
-      if (c != MAX_CELL) {
-        const int c1 = (int)c;
-        if (!(c1 >= m.guard[0] && c1 < m.dims[0] - m.guard[0])) {
-          p.cell[o] = MAX_CELL;
-        } else {
+    bool nan_seen = false, multi_seen = false;
+    int dep = 0, cdep = 0;  // this lane deposits current around cell cdep
+    // FP64: fixed-point limbs per contribution; mixed: FP32 contributions,
+    // turned into fixed point after the wave sum
+    long long q1[4] = {0, 0, 0, 0}, q2[4] = {0, 0, 0, 0}, q3[4] = {0, 0, 0, 0};
+    long long r1[4] = {0, 0, 0, 0}, r2[4] = {0, 0, 0, 0}, r3[4] = {0, 0, 0, 0};
+    float g1[4] = {0.f, 0.f, 0.f, 0.f}, g2[4] = {0.f, 0.f, 0.f, 0.f},
+          g3[4] = {0.f, 0.f, 0.f, 0.f};
+    const int c1 = (int)cur.cell;
+    const bool in_bulk = cur.valid && cur.cell != MAX_CELL && c1 >= ic.lo && c1 < ic.hi;
+    const uint32_t flag = cur.flag;
+    const bool em = in_bulk && !(flag & FLAG_IGNORE_EM);
+    // field taps of the current chunk (issued before the next chunk's
+    // particle loads, so waiting for them does not wait for those)
+    FTaps<R> ft;
+    ft.load(f, ic.fbase, in_bulk ? c1 : ic.lo, cur.x1);  // every lane, in bounds
+    {
+      // the next chunk of the run (the last chunk re-reads itself)
+      const uint64_t cn = chunk + 1 < c_hi ? chunk + 1 : chunk;
+      icur_to(in, cn, chunk_off, inst, cnt);
+      load_chunk(in, p, cn, CH, nxt);
+    }
+    if (cur.valid && cur.cell != MAX_CELL && !in_bulk) p.cell[o] = MAX_CELL;
+    if constexpr (!std::is_same<R, double>::value) {
+      // mixed form: one straight-line FP32 update (contraction on)
+      if (in_bulk) {
+        live = 1;
+        const int s = ptc_species(flag);
+        MixedOut mo;
+        push_mixed_1d(cur.x1, cur.x2, cur.x3, cur.p1, cur.p2, cur.p3, cur.w, ft, em,
+                      s_qm[s], s_q[s], (float)ic.dt, (float)(ic.dt / ic.delta),
+                      !(flag & FLAG_IGNORE_CURRENT), mo, g1, g2, g3);
+        nan_seen = mo.nan;
+        multi_seen = mo.multi;
+        const int nc = c1 + mo.dc;
+        const bool stays = nc >= ic.lo && nc < ic.hi;
+        p.cell[o] = stays ? (uint32_t)nc : MAX_CELL;
+        p.x1[o] = mo.x1;
+        p.x2[o] = mo.x2;
+        p.x3[o] = mo.x3;
+        p.p1[o] = mo.p1;
+        p.p2[o] = mo.p2;
+        p.p3[o] = mo.p3;
+        dep = !(flag & FLAG_IGNORE_CURRENT);
+        cdep = c1;
+        // gamma > gamma_thr  <=>  1/gamma < 1/gamma_thr
+        emit = stays && s != 2 && mo.ig < (float)(1.0 / ic.gamma_thr);
+      }
+    } else {
+    if (in_bulk) {
+      {
+        {
           live = 1;
-          const uint32_t flag = p.flag[o];
           const int s = ptc_species(flag);
-          const R dt = (R)I.dt;
-          R x1 = p.x1[o], x2 = p.x2[o], x3 = p.x3[o];
-          R p1 = p.p1[o], p2 = p.p2[o], p3 = p.p3[o];
-          if (p1 != p1 || p2 != p2 || p3 != p3) atomicAdd(&counters[CTR_NAN], 1ull);
-          const R q_over_m = (R)(I.dt * 0.5 * sp.q[s] / sp.m[s]);
+          const R dt = (R)ic.dt;
+          R x1 = cur.x1, x2 = cur.x2, x3 = cur.x3;
+          R p1 = cur.p1, p2 = cur.p2, p3 = cur.p3;
+          nan_seen = p1 != p1 || p2 != p2 || p3 != p3;
           R gamma = sqrt((R)1 + p1 * p1 + p2 * p2 + p3 * p3);
-          const R *E1 = f.E[0] + I.fbase, *E2 = f.E[1] + I.fbase, *E3 = f.E[2] + I.fbase;
-          const R *B1 = f.B[0] + I.fbase, *B2 = f.B[1] + I.fbase, *B3 = f.B[2] + I.fbase;
-          if (!(flag & FLAG_IGNORE_EM)) {
-            R e1 = gather1R(E1, c1, x1, ST_E1) * q_over_m;
-            R e2 = gather1R(E2, c1, x1, ST_E2) * q_over_m;
-            R e3 = gather1R(E3, c1, x1, ST_E3) * q_over_m;
-            R b1 = gather1R(B1, c1, x1, ST_B1) * q_over_m;
-            R b2 = gather1R(B2, c1, x1, ST_B2) * q_over_m;
-            R b3 = gather1R(B3, c1, x1, ST_B3) * q_over_m;
+          if (em) {
+            const R q_over_m = s_qm[s];
+            R e1, e2, e3, b1, b2, b3;
+            ft.eval(x1, q_over_m, e1, e2, e3, b1, b2, b3);
             if constexpr (std::is_same<R, double>::value) {
               vay_push(p1, p2, p3, gamma, e1, e2, e3, b1, b2, b3);
             } else {
@@ -172,17 +438,17 @@ __global__ void __launch_bounds__(1024)
             p.p1[o] = p1;
             p.p2[o] = p2;
             p.p3[o] = p3;
-            if (p.E) p.E[o] = gamma;
-          } else if (p.E) {
-            gamma = p.E[o];
+            if constexpr (std::is_same<R, double>::value) p.E[o] = gamma;
+          } else if constexpr (std::is_same<R, double>::value) {
+            gamma = p.E[o];  // ignore_EM: the stored gamma
           }
           const R v1 = p1 / gamma, v2 = p2 / gamma, v3 = p3 / gamma;
-          const R w = p.w[o];
+          const R w = cur.w;
           const R old_x1 = x1;
-          R new_x1 = old_x1 + (v1 * dt) / (R)m.delta[0];
+          R new_x1 = old_x1 + (v1 * dt) / (R)ic.delta;
           int dc1 = (int)floor(new_x1);
           if (dc1 > 1 || dc1 < -1) {
-            atomicAdd(&counters[CTR_MULTICELL], 1ull);
+            multi_seen = true;
             dc1 = dc1 > 1 ? 1 : -1;
           }
           new_x1 -= (R)dc1;
@@ -194,11 +460,17 @@ __global__ void __launch_bounds__(1024)
           p.x2[o] = x2 + v2 * dt;
           p.x3[o] = x3 + v3 * dt;
           if (!(flag & FLAG_IGNORE_CURRENT)) {
-            const R weight = (R)sp.q[s] * w;
+            // Esirkepov 1D: J1 at cells c1+i+1, J2/J3 at c1+i for i in
+            // [i0, i1], as fixed-point integers (the same split as the
+            // single-box deposit); unrolled so the arrays stay in registers
+            const R weight = s_q[s] * w;
             const int i0 = (dc1 == -1 ? -2 : -1), i1 = (dc1 == 1 ? 1 : 0);
             R djx = (R)0;
-            const uint64_t fb = I.fbase;
-            for (int i = i0; i <= i1; i++) {
+            dep = 1;
+            cdep = c1;
+#pragma unroll
+            for (int i = -2; i <= 1; i++) {
+              if (i < i0 || i > i1) continue;
               R sx0, sx1;
               if constexpr (std::is_same<R, double>::value) {
                 sx0 = cic(-old_x1 + i + 1);
@@ -208,28 +480,102 @@ __global__ void __launch_bounds__(1024)
                 sx1 = cicf(-new_x1 + (float)(i + 1 - dc1));
               }
               djx += sx1 - sx0;
-              const uint64_t o1 = fb + (uint64_t)(i + c1 + 1), o0 = fb + (uint64_t)(i + c1);
               const R val1 = (R)0.5 * (sx0 + sx1);
               if constexpr (std::is_same<R, double>::value) {
-                acc_add(f.acc[0][0], f.acc[0][1], o1, -weight * djx, H);
-                acc_add(f.acc[1][0], f.acc[1][1], o0, weight * v2 * val1, H);
-                acc_add(f.acc[2][0], f.acc[2][1], o0, weight * v3 * val1, H);
+                fx_split(-weight * djx, H, q1[i + 2], r1[i + 2]);
+                fx_split(weight * v2 * val1, H, q2[i + 2], r2[i + 2]);
+                fx_split(weight * v3 * val1, H, q3[i + 2], r3[i + 2]);
               } else {
-                const float scale = ldexpf(1.0f, H);
-                acc_fx(f.acc[0][0], o1, -weight * djx, scale);
-                acc_fx(f.acc[1][0], o0, weight * v2 * val1, scale);
-                acc_fx(f.acc[2][0], o0, weight * v3 * val1, scale);
+                g1[i + 2] = -weight * djx;
+                g2[i + 2] = weight * v2 * val1;
+                g3[i + 2] = weight * v3 * val1;
               }
             }
           }
           // open walls: a particle that left the bulk is removed (the
           // clear_guard_cells after the update, labs/1dgap main.cpp:68)
-          if (!(nc >= m.guard[0] && nc < m.dims[0] - m.guard[0])) {
+          if (!(nc >= ic.lo && nc < ic.hi)) {
             p.cell[o] = MAX_CELL;
-          } else if (s != 2 && gamma > (R)I.gamma_thr) {
-            eflag[o] = 1;
+          } else if (s != 2 && gamma > (R)ic.gamma_thr) {
             emit = 1;
           }
+        }
+      }
+    }
+    }
+    if (cur.valid) eflag[o] = (uint8_t)emit;
+    // ---- current: LDS window of the chunk -> one flush per chunk ----
+    // A chunk's particles are sorted by cell (after the load and every
+    // sort) and so cover a few cells: each lane adds its fixed-point
+    // contributions into an LDS window at the chunk's lowest cell, in one
+    // of NCOPY copies (lane & (NCOPY-1)) so fewer lanes hit one address at
+    // once; the window is flushed with one global add per touched cell.
+    // Integer adds: the totals equal the per-particle atomics of the
+    // single-box deposit bit for bit, in any order.  A lane whose cells
+    // fall outside the window (unsorted tails far from the chunk's lowest
+    // cell) adds to global memory directly.
+    {
+      const float scale = ldexpf(1.0f, H);
+      const int wmin = wave_min_i32(dep ? cdep : INT_MAX);
+      if ((threadIdx.x & 63) == 0 && wmin != INT_MAX) atomicMin(&s_cmin, wmin);
+      __syncthreads();
+      const int cb = s_cmin - 2;  // window cell 0
+      const uint64_t fb = ic.fbase;
+      if (dep) {
+        const bool lds_ok = cdep - 2 >= cb && cdep + 3 <= cb + LDS_W;
+        const int cp = threadIdx.x & (NCOPY - 1);
+#pragma unroll
+        for (int i = -2; i <= 1; i++) {
+          const int w1 = cdep + i + 1 - cb, w0 = cdep + i - cb;
+          long long a[3], b2[3];
+          if constexpr (LIMBS == 2) {
+            a[0] = q1[i + 2]; a[1] = q2[i + 2]; a[2] = q3[i + 2];
+            b2[0] = r1[i + 2]; b2[1] = r2[i + 2]; b2[2] = r3[i + 2];
+          } else {
+            a[0] = (long long)__float2int_rn(g1[i + 2] * scale);
+            a[1] = (long long)__float2int_rn(g2[i + 2] * scale);
+            a[2] = (long long)__float2int_rn(g3[i + 2] * scale);
+            b2[0] = b2[1] = b2[2] = 0;
+          }
+#pragma unroll
+          for (int comp = 0; comp < 3; comp++) {
+            const int wc = comp == 0 ? w1 : w0;
+            if (lds_ok) {
+              const int w = comp * LDS_W + wc;
+              if (a[comp])
+                atomicAdd((unsigned long long *)&s_acc[0][cp][w], (unsigned long long)a[comp]);
+              if (LIMBS == 2 && b2[comp])
+                atomicAdd((unsigned long long *)&s_acc[LIMBS - 1][cp][w],
+                          (unsigned long long)b2[comp]);
+            } else {
+              const uint64_t o = fb + (uint64_t)(cb + wc);
+              if (a[comp])
+                __hip_atomic_fetch_add((unsigned long long *)(f.acc[comp][0] + o),
+                                       (unsigned long long)a[comp], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+              if (LIMBS == 2 && b2[comp])
+                __hip_atomic_fetch_add((unsigned long long *)(f.acc[comp][1] + o),
+                                       (unsigned long long)b2[comp], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+          }
+        }
+      }
+      __syncthreads();
+      // flush the window (sum of the copies; cleared for the next chunk)
+      for (int t = threadIdx.x; t < 3 * LDS_W * LIMBS; t += blockDim.x) {
+        const int l = t / (3 * LDS_W), w = t % (3 * LDS_W);
+        long long v = 0;
+#pragma unroll
+        for (int cp = 0; cp < NCOPY; cp++) {
+          v += s_acc[l][cp][w];
+          s_acc[l][cp][w] = 0;  // copies may cancel: clear every one
+        }
+        if (v) {
+          const uint64_t o = fb + (uint64_t)(cb + w % LDS_W);
+          __hip_atomic_fetch_add((unsigned long long *)(f.acc[w / LDS_W][l] + o),
+                                 (unsigned long long)v, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
         }
       }
     }
@@ -237,13 +583,18 @@ __global__ void __launch_bounds__(1024)
     // by the emit pass)
     unsigned long long bal = __ballot(emit);
     const unsigned long long lbal = __ballot(live);
+    const unsigned long long nbal = __ballot(nan_seen), mbal = __ballot(multi_seen);
     if ((threadIdx.x & 63) == 0) {
       if (bal) atomicAdd(&s_cnt, (uint32_t)__popcll(bal));
       if (lbal) atomicAdd(&s_live, (unsigned long long)__popcll(lbal));
+      if (nbal) atomicAdd(&counters[CTR_NAN], (unsigned long long)__popcll(nbal));
+      if (mbal) atomicAdd(&counters[CTR_MULTICELL], (unsigned long long)__popcll(mbal));
     }
     __syncthreads();
     if (threadIdx.x == 0) echunk[chunk] = s_cnt;
     __syncthreads();
+    cur = nxt;
+    ic = in;
   }
   // particles pushed by this workgroup (one device atomic per workgroup)
   if (threadIdx.x == 0 && s_live) atomicAdd(pushed, s_live);
@@ -418,72 +769,76 @@ __global__ void k_b_inject_count(const BInst *__restrict__ inst, int n,
 // emission (radiative_transfer.cu:28-96, exponential free-path model)
 // ---------------------------------------------------------------------
 template <typename R>
-__global__ void k_b_emit(const BInst *__restrict__ inst, int n,
-                         const BCount *__restrict__ cnt, BPlan plan,
-                         const uint8_t *__restrict__ eflag,
-                         const uint64_t *__restrict__ epre, PtcView<R> p,
-                         PhoView<R> ph, double photon_path, double e_frac,
-                         double r_cutoff, const uint32_t *__restrict__ d_epoch,
-                         unsigned long long *__restrict__ counters) {
+__global__ void __launch_bounds__(1024)
+    k_b_emit(const BInst *__restrict__ inst, int n, const BCount *__restrict__ cnt,
+             BPlan plan, const uint8_t *__restrict__ eflag,
+             const uint64_t *__restrict__ epre, PtcView<R> p, PhoView<R> ph,
+             double photon_path, double e_frac, double r_cutoff,
+             const uint32_t *__restrict__ d_epoch,
+             unsigned long long *__restrict__ counters) {
   const uint32_t epoch = *d_epoch;
-  const uint64_t chunk = blockIdx.x;
   const uint64_t total = plan.chunk_off[n];
-  if (chunk >= total) return;
-  const int k = find_inst(plan.chunk_off, n, chunk);
-  const BInst &I = inst[k];
-  const uint64_t local = (chunk - plan.chunk_off[k]) * blockDim.x + threadIdx.x;
-  const bool mine = local < cnt[k].number && eflag[I.base + local];
-  // rank inside the chunk: ballot prefix over the 64-lane waves
   __shared__ uint32_t s_wave[16];
-  const unsigned long long bal = __ballot(mine);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) s_wave[wv] = (uint32_t)__popcll(bal);
-  __syncthreads();
-  uint32_t before = 0;
-  for (int q = 0; q < wv; q++) before += s_wave[q];
-  if (!mine) return;
-  before += (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-  const uint64_t krank = epre[chunk] - epre[plan.chunk_off[k]] + before;
-  const uint64_t slot_local = cnt[k].ph_number + krank;
-  if (slot_local >= I.ph_cap) {
-    atomicAdd(&counters[CTR_OVERFLOW], 1ull);
-    return;
+  // grid-stride over the chunks; chunks without emitters are skipped on
+  // their prefix (most chunks, at high thresholds)
+  for (uint64_t chunk = blockIdx.x; chunk < total; chunk += gridDim.x) {
+    if (epre[chunk + 1] == epre[chunk]) continue;  // uniform
+    const int k = find_inst(plan.chunk_off, n, chunk);
+    const BInst &I = inst[k];
+    const uint64_t local = (chunk - plan.chunk_off[k]) * blockDim.x + threadIdx.x;
+    const bool mine = local < cnt[k].number && eflag[I.base + local];
+    // rank inside the chunk: ballot prefix over the 64-lane waves
+    const unsigned long long bal = __ballot(mine);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) s_wave[wv] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t before = 0;
+    for (int q = 0; q < wv; q++) before += s_wave[q];
+    __syncthreads();  // s_wave is reused by the next chunk
+    if (!mine) continue;
+    before += (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+    const uint64_t krank = epre[chunk] - epre[plan.chunk_off[k]] + before;
+    const uint64_t slot_local = cnt[k].ph_number + krank;
+    if (slot_local >= I.ph_cap) {
+      atomicAdd(&counters[CTR_OVERFLOW], 1ull);
+      continue;
+    }
+    const uint64_t o = I.base + local, so = I.ph_base + slot_local;
+    ph.cell[so] = MAX_CELL;
+    if (ph.id) ph.id[so] = cnt[k].ph_id_next + krank;
+    double u0, u1;
+    uniform2(I.seed, local, epoch, 5, u0, u1);
+    const R gamma = p.E ? p.E[o]
+                        : sqrt((R)1 + p.p1[o] * p.p1[o] + p.p2[o] * p.p2[o] +
+                               p.p3[o] * p.p3[o]);
+    const R p1 = p.p1[o], p2 = p.p2[o], p3 = p.p3[o];
+    const R pi = sqrt(gamma * gamma - (R)1);
+    const R Eph = (R)e_frac * gamma;
+    const R ge = gamma - Eph;
+    const R pf2 = ge * ge - (R)1;
+    const R pf = sqrt(pf2 > (R)0 ? pf2 : (R)0);
+    p.p1[o] = p1 * pf / pi;
+    p.p2[o] = p2 * pf / pi;
+    p.p3[o] = p3 * pf / pi;
+    if (p.E) p.E[o] = ge;
+    R path;
+    if constexpr (std::is_same<R, double>::value)
+      path = -(R)photon_path * apt_log(u1);
+    else
+      path = -(R)photon_path * logf((float)u1);
+    if (path > (R)r_cutoff) continue;
+    ph.x1[so] = p.x1[o];
+    ph.x2[so] = p.x2[o];
+    ph.x3[so] = p.x3[o];
+    ph.p1[so] = Eph * p1 / pi;
+    ph.p2[so] = Eph * p2 / pi;
+    ph.p3[so] = Eph * p3 / pi;
+    if (ph.E) ph.E[so] = Eph;
+    ph.w[so] = p.w[o];
+    ph.path[so] = path;
+    ph.flag[so] = 0;
+    ph.cell[so] = p.cell[o];
   }
-  const uint64_t o = I.base + local, so = I.ph_base + slot_local;
-  ph.cell[so] = MAX_CELL;
-  if (ph.id) ph.id[so] = cnt[k].ph_id_next + krank;
-  double u0, u1;
-  uniform2(I.seed, local, epoch, 5, u0, u1);
-  const R gamma = p.E ? p.E[o]
-                      : sqrt((R)1 + p.p1[o] * p.p1[o] + p.p2[o] * p.p2[o] +
-                             p.p3[o] * p.p3[o]);
-  const R p1 = p.p1[o], p2 = p.p2[o], p3 = p.p3[o];
-  const R pi = sqrt(gamma * gamma - (R)1);
-  const R Eph = (R)e_frac * gamma;
-  const R ge = gamma - Eph;
-  const R pf2 = ge * ge - (R)1;
-  const R pf = sqrt(pf2 > (R)0 ? pf2 : (R)0);
-  p.p1[o] = p1 * pf / pi;
-  p.p2[o] = p2 * pf / pi;
-  p.p3[o] = p3 * pf / pi;
-  if (p.E) p.E[o] = ge;
-  R path;
-  if constexpr (std::is_same<R, double>::value)
-    path = -(R)photon_path * apt_log(u1);
-  else
-    path = -(R)photon_path * logf((float)u1);
-  if (path > (R)r_cutoff) return;
-  ph.x1[so] = p.x1[o];
-  ph.x2[so] = p.x2[o];
-  ph.x3[so] = p.x3[o];
-  ph.p1[so] = Eph * p1 / pi;
-  ph.p2[so] = Eph * p2 / pi;
-  ph.p3[so] = Eph * p3 / pi;
-  if (ph.E) ph.E[so] = Eph;
-  ph.w[so] = p.w[o];
-  ph.path[so] = path;
-  ph.flag[so] = 0;
-  ph.cell[so] = p.cell[o];
 }
 
 // per-instance event totals from the chunk prefix -> counts
@@ -515,35 +870,36 @@ template <typename R>
 __global__ void k_b_photon_move(const BInst *__restrict__ inst, int n,
                                 const BCount *__restrict__ cnt, BPlan plan,
                                 PhoView<R> ph) {
-  const uint64_t chunk = blockIdx.x;
-  if (chunk >= plan.ph_chunk_off[n]) return;
-  const int k = find_inst(plan.ph_chunk_off, n, chunk);
-  const BInst &I = inst[k];
-  const uint64_t local = (chunk - plan.ph_chunk_off[k]) * blockDim.x + threadIdx.x;
-  if (local >= cnt[k].ph_number) return;
-  const uint64_t t = I.ph_base + local;
-  const uint32_t c = ph.cell[t];
-  if (c == MAX_CELL) return;
-  const Mesh &m = I.m;
-  const R dt = (R)I.dt;
-  const R p1 = ph.p1[t], p2 = ph.p2[t], p3 = ph.p3[t];
-  const R E = sqrt(p1 * p1 + p2 * p2 + p3 * p3);
-  R nx = ph.x1[t] + p1 / E * dt * (R)m.inv_delta[0];
-  const int dc = (int)floor(nx);
-  nx -= (R)dc;
-  const int nc = (int)c + dc;
-  // a photon leaving the bulk is removed (photons.clear_guard_cells)
-  if (dc < -1 || dc > 1 || !(nc >= m.guard[0] && nc < m.dims[0] - m.guard[0])) {
-    ph.cell[t] = MAX_CELL;
-    return;
+  const uint64_t total = plan.ph_chunk_off[n];
+  for (uint64_t chunk = blockIdx.x; chunk < total; chunk += gridDim.x) {
+    const int k = find_inst(plan.ph_chunk_off, n, chunk);
+    const BInst &I = inst[k];
+    const uint64_t local = (chunk - plan.ph_chunk_off[k]) * blockDim.x + threadIdx.x;
+    if (local >= cnt[k].ph_number) continue;
+    const uint64_t t = I.ph_base + local;
+    const uint32_t c = ph.cell[t];
+    if (c == MAX_CELL) continue;
+    const Mesh &m = I.m;
+    const R dt = (R)I.dt;
+    const R p1 = ph.p1[t], p2 = ph.p2[t], p3 = ph.p3[t];
+    const R E = sqrt(p1 * p1 + p2 * p2 + p3 * p3);
+    R nx = ph.x1[t] + p1 / E * dt * (R)m.inv_delta[0];
+    const int dc = (int)floor(nx);
+    nx -= (R)dc;
+    const int nc = (int)c + dc;
+    // a photon leaving the bulk is removed (photons.clear_guard_cells)
+    if (dc < -1 || dc > 1 || !(nc >= m.guard[0] && nc < m.dims[0] - m.guard[0])) {
+      ph.cell[t] = MAX_CELL;
+      continue;
+    }
+    if constexpr (!std::is_same<R, double>::value)
+      if (nx >= 1.0f) nx = 0x1.fffffep-1f;
+    ph.cell[t] = (uint32_t)nc;
+    ph.x1[t] = nx;
+    ph.x2[t] = ph.x2[t] + p2 * dt / E;
+    ph.x3[t] = ph.x3[t] + p3 * dt / E;
+    ph.path[t] -= dt;
   }
-  if constexpr (!std::is_same<R, double>::value)
-    if (nx >= 1.0f) nx = 0x1.fffffep-1f;
-  ph.cell[t] = (uint32_t)nc;
-  ph.x1[t] = nx;
-  ph.x2[t] = ph.x2[t] + p2 * dt / E;
-  ph.x3[t] = ph.x3[t] + p3 * dt / E;
-  ph.path[t] -= dt;
 }
 
 // check_produce_pair with per-chunk counts
@@ -553,25 +909,26 @@ __global__ void __launch_bounds__(1024)
                     const BCount *__restrict__ cnt, BPlan plan, PhoView<R> ph,
                     uint8_t *__restrict__ pflag, uint32_t *__restrict__ pchunk) {
   __shared__ uint32_t s_cnt;
-  const uint64_t chunk = blockIdx.x;
-  if (chunk >= plan.ph_chunk_off[n]) return;
-  if (threadIdx.x == 0) s_cnt = 0;
-  __syncthreads();
-  const int k = find_inst(plan.ph_chunk_off, n, chunk);
-  const BInst &I = inst[k];
-  const uint64_t local = (chunk - plan.ph_chunk_off[k]) * blockDim.x + threadIdx.x;
-  int conv = 0;
-  if (local < cnt[k].ph_number) {
-    const uint64_t t = I.ph_base + local;
-    const uint32_t c = ph.cell[t];
-    conv = c != MAX_CELL && (int)c >= I.m.guard[0] &&
-           (int)c < I.m.dims[0] - I.m.guard[0] && ph.path[t] <= (R)0;
-    pflag[t] = (uint8_t)conv;
+  const uint64_t total = plan.ph_chunk_off[n];
+  for (uint64_t chunk = blockIdx.x; chunk < total; chunk += gridDim.x) {
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    const int k = find_inst(plan.ph_chunk_off, n, chunk);
+    const BInst &I = inst[k];
+    const uint64_t local = (chunk - plan.ph_chunk_off[k]) * blockDim.x + threadIdx.x;
+    int conv = 0;
+    if (local < cnt[k].ph_number) {
+      const uint64_t t = I.ph_base + local;
+      const uint32_t c = ph.cell[t];
+      conv = c != MAX_CELL && (int)c >= I.m.guard[0] &&
+             (int)c < I.m.dims[0] - I.m.guard[0] && ph.path[t] <= (R)0;
+      pflag[t] = (uint8_t)conv;
+    }
+    unsigned long long bal = __ballot(conv);
+    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(&s_cnt, (uint32_t)__popcll(bal));
+    __syncthreads();
+    if (threadIdx.x == 0) pchunk[chunk] = s_cnt;
   }
-  unsigned long long bal = __ballot(conv);
-  if ((threadIdx.x & 63) == 0 && bal) atomicAdd(&s_cnt, (uint32_t)__popcll(bal));
-  __syncthreads();
-  if (threadIdx.x == 0) pchunk[chunk] = s_cnt;
 }
 
 __global__ void k_b_tick(uint32_t *d_epoch) {
@@ -579,54 +936,57 @@ __global__ void k_b_tick(uint32_t *d_epoch) {
 }
 
 template <typename R>
-__global__ void k_b_pairs(const BInst *__restrict__ inst, int n,
-                          const BCount *__restrict__ cnt, BPlan plan,
-                          const uint8_t *__restrict__ pflag,
-                          const uint64_t *__restrict__ ppre, PtcView<R> p,
-                          PhoView<R> ph, unsigned long long *__restrict__ counters) {
-  const uint64_t chunk = blockIdx.x;
-  if (chunk >= plan.ph_chunk_off[n]) return;
-  const int k = find_inst(plan.ph_chunk_off, n, chunk);
-  const BInst &I = inst[k];
-  const uint64_t local = (chunk - plan.ph_chunk_off[k]) * blockDim.x + threadIdx.x;
-  const bool mine = local < cnt[k].ph_number && pflag[I.ph_base + local];
+__global__ void __launch_bounds__(1024)
+    k_b_pairs(const BInst *__restrict__ inst, int n, const BCount *__restrict__ cnt,
+              BPlan plan, const uint8_t *__restrict__ pflag,
+              const uint64_t *__restrict__ ppre, PtcView<R> p, PhoView<R> ph,
+              unsigned long long *__restrict__ counters) {
+  const uint64_t total = plan.ph_chunk_off[n];
   __shared__ uint32_t s_wave[16];
-  const unsigned long long bal = __ballot(mine);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) s_wave[wv] = (uint32_t)__popcll(bal);
-  __syncthreads();
-  uint32_t before = 0;
-  for (int q = 0; q < wv; q++) before += s_wave[q];
-  if (!mine) return;
-  before += (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-  const uint64_t kr = ppre[chunk] - ppre[plan.ph_chunk_off[k]] + before;
-  const uint64_t t = I.ph_base + local;
-  const uint64_t off = cnt[k].number + 2 * kr;
-  if (off + 1 >= I.cap) {
-    atomicAdd(&counters[CTR_OVERFLOW], 1ull);
+  for (uint64_t chunk = blockIdx.x; chunk < total; chunk += gridDim.x) {
+    if (ppre[chunk + 1] == ppre[chunk]) continue;  // uniform: no conversions
+    const int k = find_inst(plan.ph_chunk_off, n, chunk);
+    const BInst &I = inst[k];
+    const uint64_t local = (chunk - plan.ph_chunk_off[k]) * blockDim.x + threadIdx.x;
+    const bool mine = local < cnt[k].ph_number && pflag[I.ph_base + local];
+    const unsigned long long bal = __ballot(mine);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) s_wave[wv] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t before = 0;
+    for (int q = 0; q < wv; q++) before += s_wave[q];
+    __syncthreads();  // s_wave is reused by the next chunk
+    if (!mine) continue;
+    before += (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+    const uint64_t kr = ppre[chunk] - ppre[plan.ph_chunk_off[k]] + before;
+    const uint64_t t = I.ph_base + local;
+    const uint64_t off = cnt[k].number + 2 * kr;
+    if (off + 1 >= I.cap) {
+      atomicAdd(&counters[CTR_OVERFLOW], 1ull);
+      ph.cell[t] = MAX_CELL;
+      continue;
+    }
+    const R p1 = ph.p1[t], p2 = ph.p2[t], p3 = ph.p3[t];
+    R E_ph2 = p1 * p1 + p2 * p2 + p3 * p3;
+    if (E_ph2 <= (R)4.01) E_ph2 = (R)4.01;
+    const R ratio = sqrt((R)0.25 - (R)1 / E_ph2);
+    const R gamma = sqrt((R)1 + ratio * ratio * E_ph2);
+    for (int s = 0; s < 2; s++) {
+      const uint64_t o = I.base + off + s;
+      p.x1[o] = ph.x1[t];
+      p.x2[o] = ph.x2[t];
+      p.x3[o] = ph.x3[t];
+      p.p1[o] = ratio * p1;
+      p.p2[o] = ratio * p2;
+      p.p3[o] = ratio * p3;
+      if (p.E) p.E[o] = gamma;
+      p.w[o] = ph.w[t];
+      p.flag[o] = FLAG_SECONDARY | ((uint32_t)s << 29);
+      if (p.id) p.id[o] = cnt[k].id_next + 2 * kr + s;
+      p.cell[o] = ph.cell[t];
+    }
     ph.cell[t] = MAX_CELL;
-    return;
   }
-  const R p1 = ph.p1[t], p2 = ph.p2[t], p3 = ph.p3[t];
-  R E_ph2 = p1 * p1 + p2 * p2 + p3 * p3;
-  if (E_ph2 <= (R)4.01) E_ph2 = (R)4.01;
-  const R ratio = sqrt((R)0.25 - (R)1 / E_ph2);
-  const R gamma = sqrt((R)1 + ratio * ratio * E_ph2);
-  for (int s = 0; s < 2; s++) {
-    const uint64_t o = I.base + off + s;
-    p.x1[o] = ph.x1[t];
-    p.x2[o] = ph.x2[t];
-    p.x3[o] = ph.x3[t];
-    p.p1[o] = ratio * p1;
-    p.p2[o] = ratio * p2;
-    p.p3[o] = ratio * p3;
-    if (p.E) p.E[o] = gamma;
-    p.w[o] = ph.w[t];
-    p.flag[o] = FLAG_SECONDARY | ((uint32_t)s << 29);
-    if (p.id) p.id[o] = cnt[k].id_next + 2 * kr + s;
-    p.cell[o] = ph.cell[t];
-  }
-  ph.cell[t] = MAX_CELL;
 }
 
 // ---------------------------------------------------------------------
@@ -705,6 +1065,7 @@ struct apt_batch {
   uint32_t *d_epoch = nullptr;  // step counter read by the kernels
   unsigned long long *d_pushed = nullptr;  // particles pushed, all steps
   unsigned push_grid = 0;       // persistent push workgroups
+  unsigned aux_grid = 0;        // grid-stride emission / photon kernels
   int H = 22;
   int cbits = 15;
   uint32_t step = 0;
@@ -757,7 +1118,6 @@ static inline unsigned bblocks(uint64_t n, int t = 256) {
   const uint64_t g = (n + t - 1) / t;
   return (unsigned)(g ? g : 1);
 }
-static inline unsigned at_least_one(uint64_t g) { return (unsigned)(g ? g : 1); }
 
 template <typename R>
 static BFields<R> bfields(apt_batch *b) {
@@ -806,14 +1166,14 @@ static apt_status batch_step_t(apt_batch *b, PtcView<R> p, PhoView<R> ph,
   // 2. plan + the persistent push, then the photons' move
   if (s0) k_b_plan<<<1, 64, 0, st>>>(b->d_cnt, n, CH, b->plan);
   if (s1)
-    k_b_push<R><<<b->push_grid, CH, 0, st>>>(b->d_inst, n, b->d_cnt, b->plan, b->sp, b->H,
+    k_b_push<R><<<b->push_grid, CH, 0, st>>>(b->d_inst, n, b->d_cnt, b->plan.chunk_off, b->sp, b->H,
                                              p, f, b->eflag, b->echunk, b->d_counters,
                                              b->d_pushed);
   if (!s2) {
     BCHK(hipGetLastError());
     return APT_OK;
   }
-  k_b_photon_move<R><<<at_least_one(b->max_ph_chunks), CH, 0, st>>>(b->d_inst, n, b->d_cnt,
+  k_b_photon_move<R><<<b->aux_grid, CH, 0, st>>>(b->d_inst, n, b->d_cnt,
                                                                 b->plan, ph);
   // 3. fields (field_solver::update_fields, step-0 half step first)
   k_b_finalize<R><<<bblocks(b->ncell), 256, 0, st>>>(f, b->ncell, b->H, limbs);
@@ -827,19 +1187,19 @@ static apt_status batch_step_t(apt_batch *b, PtcView<R> p, PhoView<R> ph,
   // 4. emission: chunk counts -> prefix -> photons in slot order
   apt_status s = scan_chunks(b, b->echunk, b->max_chunks, b->epre);
   if (s != APT_OK) return s;
-  k_b_emit<R><<<at_least_one(b->max_chunks), CH, 0, st>>>(
+  k_b_emit<R><<<b->aux_grid, CH, 0, st>>>(
       b->d_inst, n, b->d_cnt, b->plan, b->eflag, b->epre, p, ph, b->params.photon_path,
       b->params.e_frac, b->params.r_cutoff, b->d_epoch, b->d_counters);
   k_b_emit_count<<<bblocks(n), 256, 0, st>>>(b->d_inst, n, b->d_cnt, b->plan.chunk_off,
                                             b->epre, 0);
   // 5. pairs from every photon with no path left (new ones included)
   k_b_plan<<<1, 64, 0, st>>>(b->d_cnt, n, CH, b->plan);
-  k_b_pair_decide<R><<<at_least_one(b->max_ph_chunks), CH, 0, st>>>(b->d_inst, n, b->d_cnt,
+  k_b_pair_decide<R><<<b->aux_grid, CH, 0, st>>>(b->d_inst, n, b->d_cnt,
                                                                 b->plan, ph, b->pflag,
                                                                 b->pchunk);
   s = scan_chunks(b, b->pchunk, b->max_ph_chunks, b->ppre);
   if (s != APT_OK) return s;
-  k_b_pairs<R><<<at_least_one(b->max_ph_chunks), CH, 0, st>>>(
+  k_b_pairs<R><<<b->aux_grid, CH, 0, st>>>(
       b->d_inst, n, b->d_cnt, b->plan, b->pflag, b->ppre, p, ph, b->d_counters);
   k_b_emit_count<<<bblocks(n), 256, 0, st>>>(b->d_inst, n, b->d_cnt,
                                             b->plan.ph_chunk_off, b->ppre, 1);
@@ -1006,7 +1366,7 @@ static void drop_graphs(apt_batch *b) {
 static void free_all(apt_batch *b) {
   auto F = [](void *p) { (void)hipFree(p); };
   F(b->d_inst); F(b->d_cnt); F(b->plan.chunk_off); F(b->plan.ph_chunk_off);
-  F(b->plan.work); F(b->d_boff); F(b->d_ioff); F(b->d_slot_off); F(b->d_ph_off);
+  F(b->d_boff); F(b->d_ioff); F(b->d_slot_off); F(b->d_ph_off);
   F(b->d_counters);
   F(b->d_epoch);
   F(b->d_pushed);
@@ -1084,6 +1444,7 @@ apt_status apt_batch_create(const apt_batch_inst *specs, int n,
     return APT_ERR_INVALID_ARG;
   const int CH = params->chunk;
   if (CH != 256 && CH != 512 && CH != 1024) return APT_ERR_INVALID_ARG;
+  if (params->precision == APT_PREC_FP64 && CH > 512) return APT_ERR_INVALID_ARG;
   if (params->grid_mult < 1 || params->grid_mult > 64) return APT_ERR_INVALID_ARG;
   apt_batch *b = new apt_batch();
   b->n = n;
@@ -1173,6 +1534,9 @@ apt_status apt_batch_create(const apt_batch_inst *specs, int n,
   }
   b->sp.q[0] = -params->q_e;
   b->sp.m[2] = params->q_e * params->ion_mass;
+  for (int k = 0; k < n; k++)  // the push's dt q / (2 m), same FP64 order
+    for (int sp = 0; sp < 8; sp++)
+      b->hinst[k].qm[sp] = b->hinst[k].dt * 0.5 * b->sp.q[sp] / b->sp.m[sp];
   for (int k = 0; k < n; k++) {
     b->max_chunks += (specs[k].cap + CH - 1) / CH;
     b->max_ph_chunks += (specs[k].ph_cap + CH - 1) / CH;
@@ -1186,8 +1550,7 @@ apt_status apt_batch_create(const apt_batch_inst *specs, int n,
   ok = ok && dalloc(&b->d_cnt, n) &&
        hipMemcpy(b->d_cnt, cnt.data(), n * sizeof(BCount), hipMemcpyHostToDevice) ==
            hipSuccess;
-  ok = ok && dalloc(&b->plan.chunk_off, n + 1) && dalloc(&b->plan.ph_chunk_off, n + 1) &&
-       dalloc(&b->plan.work, 2);
+  ok = ok && dalloc(&b->plan.chunk_off, n + 1) && dalloc(&b->plan.ph_chunk_off, n + 1);
   auto up = [&](uint64_t **d, std::vector<uint64_t> &h) {
     return dalloc(d, h.size()) &&
            hipMemcpy(*d, h.data(), h.size() * 8, hipMemcpyHostToDevice) == hipSuccess;
@@ -1235,6 +1598,7 @@ apt_status apt_batch_create(const apt_batch_inst *specs, int n,
   uint64_t pg = (uint64_t)ncu * (uint64_t)params->grid_mult;
   if (pg > b->max_chunks) pg = b->max_chunks;
   b->push_grid = (unsigned)(pg ? pg : 1);
+  b->aux_grid = (unsigned)ncu * 4;
   ok = ok && hipStreamSynchronize(b->stream) == hipSuccess;
   if (!ok) {
     free_all(b);
@@ -1318,6 +1682,12 @@ apt_status apt_batch_load(apt_batch *b, const apt_batch_inst *specs) {
     }
   }
   BCHK(hipMemcpy(b->d_cnt, cnt.data(), b->n * sizeof(BCount), hipMemcpyHostToDevice));
+  // sorted by cell right after loading, as sim_data::fill_multiplicity
+  // does (cuda/sim_data.cu:290)
+  apt_status st = batch_sort(b, 0);
+  if (st == APT_OK) st = batch_sort(b, 1);
+  if (st != APT_OK) return st;
+  b->parity ^= 1;
   BCHK(hipStreamSynchronize(b->stream));
   return APT_OK;
 }

@@ -21,6 +21,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "apt_common.hpp"
+#include "apt_f32.hpp"
 #include "apt_kernels.hpp"
 #include "apt_tiles.hpp"
 
@@ -86,33 +87,6 @@ __device__ __forceinline__ float gather3g(const float *__restrict__ f,
   float hi =
       (q[0] * a0 + q[1] * a1) * b0 + (q[d0] * a0 + q[d0 + 1] * a1) * b1;
   return lo * g0 + hi * g1;
-}
-
-// hardware FP32 square root and reciprocal (<= 1 ulp); arguments here are
-// >= 1 or well inside the normal range
-__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-
-// Vay push (vay_push.cuh:8-30) in FP32; returns 1/gamma of the new momentum
-__device__ __forceinline__ float vay_push_f(float &p1, float &p2, float &p3,
-                                           float gamma, float E1, float E2,
-                                           float E3, float B1, float B2,
-                                           float B3) {
-  float ig = hw_rcp(gamma);
-  float up1 = p1 + 2.0f * E1 + (p2 * B3 - p3 * B2) * ig;
-  float up2 = p2 + 2.0f * E2 + (p3 * B1 - p1 * B3) * ig;
-  float up3 = p3 + 2.0f * E3 + (p1 * B2 - p2 * B1) * ig;
-  float tt = B1 * B1 + B2 * B2 + B3 * B3;
-  float ut = up1 * B1 + up2 * B2 + up3 * B3;
-  float sigma = 1.0f + up1 * up1 + up2 * up2 + up3 * up3 - tt;
-  float inv_gamma2 =
-      2.0f * hw_rcp(sigma + hw_sqrt(sigma * sigma + 4.0f * (tt + ut * ut)));
-  float s = hw_rcp(1.0f + inv_gamma2 * tt);
-  float ign = hw_sqrt(inv_gamma2);
-  p1 = (up1 + B1 * ut * inv_gamma2 + (up2 * B3 - up3 * B2) * ign) * s;
-  p2 = (up2 + B2 * ut * inv_gamma2 + (up3 * B1 - up1 * B3) * ign) * s;
-  p3 = (up3 + B3 * ut * inv_gamma2 + (up1 * B2 - up2 * B1) * ign) * s;
-  return ign;
 }
 
 // per-species constants in FP32: qm = dt q / (2 m), q

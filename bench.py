@@ -41,6 +41,12 @@ BYTES_PER_PUSH = {"mixed": 64, "fp64": 132}   # algorithmic particle stream
 BYTES_PER_CELL = {"mixed": 60, "fp64": 120}   # Yee: read E,B,J; write E,B
 
 
+def progress(msg):
+    """One progress line per phase on stderr (the JSON line stays alone on
+    stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def make_sim(apt, precision, n_cells=N_CELLS, ppc=PPC):
     from aperture3_amd import autotune
     tuned = autotune.lookup(autotune.key(2, n_cells, ppc, THETA, DRIFT, SORT_INTERVAL))
@@ -130,7 +136,13 @@ def measure_c5(apt, steps, warmup):
         p0 = b.pushes()
         b.timing(1)
         t0 = time.perf_counter()
-        b.step(steps)
+        done = 0
+        while done < steps:  # a progress line every 250 steps
+            k = min(250, steps - done)
+            b.step(k)
+            done += k
+            if done < steps:
+                progress(f"ensemble: {done}/{steps} steps")
         b.sync()
         t1 = time.perf_counter()
         step_ms, push_ms, n = b.timing(0)
@@ -210,6 +222,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--c5-steps", type=int, default=1000,
                     help="steps of the batched-ensemble measurement (0: skip)")
+    ap.add_argument("--c5-only", action="store_true",
+                    help="only the batched-ensemble measurement (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (read 1 GiB + write 1 GiB), "
                          "the reference point for the FETCH_SIZE/WRITE_SIZE counters")
@@ -230,9 +244,19 @@ def main():
         return 2
     import aperture3_amd as apt
     apt.load_library()
+    if args.c5_only:
+        progress("batched ensemble, 256 boxes")
+        print(json.dumps({"batched_ensemble": measure_c5(apt, args.c5_steps, args.warmup)}))
+        return 0
 
+    progress("mixed form, 2048^2 box")
     mixed = measure(apt, apt.APT_PREC_MIXED, args.steps, args.warmup)
-    fp64 = None if args.no_fp64 else measure(apt, apt.APT_PREC_FP64, args.fp64_steps, 2)
+    progress(f"mixed: {mixed['pushes_per_s']:.3e} pushes/s")
+    fp64 = None
+    if not args.no_fp64:
+        progress("FP64 form, 2048^2 box")
+        fp64 = measure(apt, apt.APT_PREC_FP64, args.fp64_steps, 2)
+        progress(f"fp64: {fp64['pushes_per_s']:.3e} pushes/s")
 
     n = mixed["particles"]
     avg_s = mixed["push_kernel_ms"] * 1e-3
@@ -285,8 +309,11 @@ def main():
         line["fp64_roofline_frac"] = (fp64["particles"] * BYTES_PER_PUSH["fp64"]
                                       / (fp64["push_kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
     if args.c5_steps > 0:
+        progress("batched ensemble, 256 boxes")
         line["batched_ensemble"] = measure_c5(apt, args.c5_steps, args.warmup)
+        progress(f"ensemble: {line['batched_ensemble']['pushes_per_s']:.3e} pushes/s")
     if not args.no_cpu:
+        progress("CPU baseline (FP64 oracle)")
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line))
     return 0

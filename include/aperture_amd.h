@@ -332,7 +332,8 @@ typedef struct {
   double q_e;           /* charge unit (electron -q_e) */
   double ion_mass;
   uint64_t seed;        /* box k uses seed + 0x9E3779B97F4A7C15 (k + 1) */
-  int chunk;            /* particles per workgroup: 256, 512 or 1024 */
+  int chunk;            /* particles per workgroup: 256, 512 or 1024
+                           (FP64: 256 or 512) */
   int grid_mult;        /* persistent push workgroups per CU, 1..64 */
   int sort_interval;    /* 0: never */
   double photon_path;   /* mean free path */

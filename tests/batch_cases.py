@@ -2,6 +2,7 @@
 gap boxes, built once for the HIP batch and once per box for the oracle,
 which replays the 1dgap loop (labs/1dgap/src/main.cpp:61-98) box by box:
 
+    load (sorted by cell), then per step:
     inject_left -> update_particles (push, deposit, photon move)
     -> clear_guard_cells (particles, photons) -> update_fields
     -> emit_photons -> produce_pairs -> sort every sort_interval steps
@@ -50,6 +51,7 @@ class OracleBox:
         self.dt = inst["cfl"] * (1.0 / N)
         self.sim.load_uniform(inst["ppc"], theta=(inst["theta"], inst["theta"]),
                               drift=(0.0, 0.0), weight=inst["weight"], seed=seed)
+        self.sim.sort()  # fill_multiplicity sorts after loading
         self.sim.f["B"][0][:] = inst["Bx"]
         self.sim.f["Bbg"][0][:] = inst["Bx"]
         self.sim.set_rt(model=1, gamma_thr=inst["gamma_thr"],

@@ -69,6 +69,32 @@ def test_batch_fp64_bit_exact(graph):
     assert c["nan_momenta"] == 0 and c["capacity_overflow"] == 0
 
 
+DENSE = [
+    dict(N=40, guard=2, ppc=100, inj=5, cap=20000, ph_cap=8000, cfl=0.5, weight=0.25,
+         theta=0.2, gamma_thr=1.2, Bx=1.0),
+    dict(N=150, guard=2, ppc=8, inj=2, cap=6000, ph_cap=3000, cfl=0.5, weight=1.0,
+         theta=0.15, gamma_thr=1.1, Bx=0.6),
+]
+
+
+@pytest.mark.parametrize("chunk", [256, 512])
+def test_batch_fp64_dense_boxes_bit_exact(chunk):
+    """Cell-ordered boxes where a wavefront spans at most a few cells, so
+    the current goes through the wave sums and the chunk's LDS window;
+    wider waves and unsorted injection and pair tails take the per-lane
+    paths.  The integer totals, and so J, E and B, stay bit-exact."""
+    apt = _apt()
+    b = bc.product(apt.APT_PREC_FP64, insts=DENSE, graph=True, chunk=chunk)
+    boxes = bc.oracle_boxes(b, insts=DENSE)
+    for s in range(7):
+        b.step(1)
+        for box in boxes:
+            box.step()
+        for k, box in enumerate(boxes):
+            _compare_box(b, k, box)
+    assert sum(box.sim.ph_number for box in boxes) > 0
+
+
 def test_batch_graph_replay_equals_eager():
     apt = _apt()
     a = bc.product(apt.APT_PREC_FP64, graph=False, sort_interval=2)
