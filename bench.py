@@ -37,7 +37,7 @@ THETA = 0.05
 DRIFT = 0.3
 SORT_INTERVAL = 20
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
-BYTES_PER_PUSH = {"mixed": 64, "fp64": 132}   # algorithmic particle stream
+BYTES_PER_PUSH = {"mixed": 64, "fp64": 132, "mixed3d": 64}  # algorithmic particle stream
 BYTES_PER_CELL = {"mixed": 60, "fp64": 120}   # Yee: read E,B,J; write E,B
 
 
@@ -174,6 +174,190 @@ def measure_c5(apt, steps, warmup):
     }
 
 
+def measure_c1(apt, steps):
+    """The 1D Cartesian gap in the plain FP64 form: 4096 cells, periodic,
+    e-/e+ at 100 ppc each (819,200 particles), MJ theta = 0.1, B_x = 1,
+    E_x = 1e-3 sin(2 pi 4 x), dt = dx/2, Vay + Esirkepov + 1D FDTD."""
+    import numpy as np
+    n = 4096
+    p = apt.SimParams(N=[n, 1, 1], guard=[2, 0, 0], size=[1.0, 1.0, 1.0],
+                      periodic_boundary=[True, False, False], q_e=1.0,
+                      max_ptc_number=n * 200 + 1024, precision=apt.APT_PREC_FP64,
+                      data_interval=10 ** 9, sort_interval=20, random_seed=43210)
+    env = apt.sim_environment(p)
+    data = apt.sim_data(env)
+    try:
+        data.init_bg_B_field(0, lambda x, y, z: 1.0 + 0.0 * x)
+        data.initialize_field(apt.FIELD_E, 0, lambda x, y, z: 1e-3 * np.sin(2 * np.pi * 4 * x))
+        data.fill_multiplicity(1.0, 100, (0.1, 0.1), (0.0, 0.0))
+        npart = data.num_particles()
+        dt = 0.5 / n
+        pu, fs = apt.ptc_updater(env), apt.field_solver(env)
+        data.sync()
+        t0 = time.perf_counter()
+        for step in range(steps):
+            pu.update_particles(data, dt, step)
+            fs.update_fields(data, dt, step)
+            if step % 20 == 0 and step != 0:
+                data.sort_particles()
+        data.sync()
+        el = time.perf_counter() - t0
+    finally:
+        data.close()
+    return {"workload": "1D Cartesian gap, 4096 cells periodic, 2 x 100 ppc (819,200 "
+                        "particles), MJ theta=0.1, B_x=1, E_x=1e-3 sin(8 pi x), plain FP64",
+            "steps": steps, "particles": npart, "pushes_per_s": npart * steps / el,
+            "ms_per_step": 1e3 * el / steps,
+            "field_cell_updates_per_s": n * steps / el}
+
+
+def measure_c3(apt, steps):
+    """The 2D log-spherical magnetosphere in the mixed form: 1024 x 1024
+    cells over r in [1, 30], theta in [0, pi]; dipole mu = 1e4 as B and
+    background; star rotating at Omega = 0.1 (the stellar boundary sets E
+    of the corotating conductor); injection of 1 pair per surface cell per
+    step with p1 = 0.01 (the pulsar lab's injection momentum); pair creation
+    with threshold gamma = 1e3, exponential free path of mean 0.2, photon
+    energy 0.1 gamma; 32-cell absorbing layer; sort every 20 steps.  Counts
+    grow with the injection and the cascade: pushes are summed over the
+    particles actually pushed each step."""
+    import math
+    import numpy as np
+    N, g = 1024, 2
+    mu, omega = 1.0e4, 0.1
+    size = [math.log(30.0), math.pi, 1.0]
+    p = apt.SimParams(N=[N, N, 1], guard=[g, g, 0], size=size,
+                      periodic_boundary=[False, False, False], q_e=1.0,
+                      max_ptc_number=48 * 1024 * 1024, max_photon_number=16 * 1024 * 1024,
+                      precision=apt.APT_PREC_MIXED, num_species=3, data_interval=10 ** 9,
+                      sort_interval=20, random_seed=43210, coord_system="LogSpherical",
+                      omega=omega, B0=mu, damping_length=32, damping_coef=0.002,
+                      rt_model=apt.RT_EXP_PATH, gamma_thr=1.0e3, photon_path=0.2,
+                      e_frac=0.1, r_cutoff=30.0)
+    env = apt.sim_environment(p)
+    data = apt.sim_data(env)
+    dims = [N + 2 * g, N + 2 * g]
+    delta = [size[0] / N, size[1] / N]
+
+    def coord(d, st):
+        return delta[d] * ((np.arange(dims[d]) - g) + (1.0 if st else 0.5))
+    try:
+        for c, (s1, s2) in ((0, (1, 0)), (1, (0, 1))):
+            T, R = np.meshgrid(coord(1, s2), np.exp(coord(0, s1)), indexing="ij")
+            f = (2 * mu * np.cos(T) / R ** 3 if c == 0 else mu * np.sin(T) / R ** 3).ravel()
+            data.set_field(apt.FIELD_BBG, c, f)
+            data.set_field(apt.FIELD_B, c, f)
+        dt = 0.3 * delta[0]
+        pu = apt.ptc_updater_logsph(env)
+        fs = apt.field_solver_logsph(env)
+        rad = apt.radiative_transfer(env)
+        pushed = 0
+        data.timing_enable(True)
+        data.sync()
+        t0 = time.perf_counter()
+        for step in range(steps):
+            pu.inject_ptc(data, 1, 0.01, 0.0, 0.0, 1.0, omega)
+            pushed += data.num_particles()
+            pu.update_particles(data, dt, step)
+            fs.update_fields(data, dt, step * dt)
+            fs.apply_boundary(data, omega, step * dt)
+            rad.emit_photons(data)
+            rad.produce_pairs(data)
+            if step % 20 == 0 and step != 0:
+                data.sort_particles()
+            if step % 500 == 499:
+                progress(f"log-spherical: {step + 1}/{steps} steps, "
+                         f"{data.num_particles()} particles")
+        data.sync()
+        el = time.perf_counter() - t0
+        push_ms, push_n = data.timing_get(apt.KERNEL_PUSH_DEPOSIT)
+        field_ms, field_n = data.timing_get(apt.KERNEL_FIELD)
+        info = data.info()
+        counters = data.counters()
+    finally:
+        data.close()
+    return {"workload": "2D log-spherical magnetosphere 1024x1024, r in [1,30], dipole "
+                        "mu=1e4, Omega=0.1, injection 1 pair/surface cell/step (p1=0.01), "
+                        "pairs: gamma_thr=1e3, Exp(0.2) path, E_ph=0.1 gamma, 32-cell "
+                        "absorber, mixed form",
+            "steps": steps, "pushes": pushed, "pushes_per_s": pushed / el,
+            "ms_per_step": 1e3 * el / steps,
+            "particles_final": info["number"], "photons_final": info["ph_number"],
+            "push_kernel_ms": push_ms / max(push_n, 1),
+            "field_ms": field_ms / max(field_n, 1),
+            "field_cell_updates_per_s": (N * N) / (field_ms / max(field_n, 1) * 1e-3)
+            if field_n else None,
+            "counters": counters}
+
+
+def measure_c4(apt, steps, warmup):
+    """The 3D Cartesian box in the mixed form: 512^3 periodic cells, e-/e+
+    at 4 ppc each (1,073,741,824 particles), MJ theta = 0.1, B = (0, 0, 1)
+    plus an Alfvenic perturbation along z (4 modes k = 1..4 of random phase,
+    0.05 each in B_x and B_y: divergence-free, |dB|_rms = 0.1), E = 0, Yee,
+    sort every 10 steps."""
+    import numpy as np
+    from aperture3_amd import autotune
+    n, ppc = 512, 4
+    tuned = autotune.lookup(autotune.key(3, n, ppc, 0.1, 0.0, 10))
+    tile = {}
+    if tuned:
+        tile = dict(tile=tuned["tile"], tile_halo=tuned["halo"], tile_threads=tuned["threads"])
+    p = apt.SimParams(N=[n, n, n], guard=[2, 2, 2], size=[1.0, 1.0, 1.0],
+                      periodic_boundary=[True, True, True], q_e=1.0,
+                      max_ptc_number=n ** 3 * ppc * 2 + 1024, precision=apt.APT_PREC_MIXED,
+                      data_interval=10 ** 9, sort_interval=10, random_seed=43210, **tile)
+    env = apt.sim_environment(p)
+    data = apt.sim_data(env)
+    rng = np.random.default_rng(4)
+    ph = rng.uniform(0, 2 * np.pi, (2, 4))
+    try:
+        data.init_bg_B_field(2, lambda x, y, z: 1.0 + 0.0 * x)
+        data.initialize_field(apt.FIELD_B, 0, lambda x, y, z: sum(
+            0.05 * np.cos(2 * np.pi * k * z + ph[0, k - 1]) for k in range(1, 5)) + 0.0 * x)
+        data.initialize_field(apt.FIELD_B, 1, lambda x, y, z: sum(
+            0.05 * np.sin(2 * np.pi * k * z + ph[1, k - 1]) for k in range(1, 5)) + 0.0 * x)
+        data.fill_multiplicity(1.0, ppc, (0.1, 0.1), (0.0, 0.0))
+        npart = data.num_particles()
+        dt = 0.5 / n
+        pu, fs = apt.ptc_updater(env), apt.field_solver(env)
+        for step in range(warmup):
+            pu.update_particles(data, dt, step)
+            fs.update_fields(data, dt, step)
+        data.sync()
+        data.timing_enable(True)
+        t0 = time.perf_counter()
+        for step in range(warmup, warmup + steps):
+            pu.update_particles(data, dt, step)
+            fs.update_fields(data, dt, step)
+            if step % 10 == 0 and step != 0:
+                data.sort_particles()
+        data.sync()
+        el = time.perf_counter() - t0
+        push_ms, push_n = data.timing_get(apt.KERNEL_PUSH_DEPOSIT)
+        field_ms, field_n = data.timing_get(apt.KERNEL_FIELD)
+        sort_ms, sort_n = data.timing_get(apt.KERNEL_SORT)
+        _, tile_ext = data.tile_offsets()
+        counters = data.counters()
+    finally:
+        data.close()
+    pk = push_ms / max(push_n, 1)
+    achieved = npart * BYTES_PER_PUSH["mixed3d"] / (pk * 1e-3) / 1e9
+    return {"workload": "3D Cartesian periodic 512^3, 2 x 4 ppc (1,073,741,824 particles), "
+                        "MJ theta=0.1, B=(0,0,1) + Alfvenic dB (k=1..4, rms 0.1), E=0, Yee, "
+                        "sort every 10, mixed form",
+            "steps": steps, "warmup": warmup, "particles": npart,
+            "pushes_per_s": npart * steps / el, "ms_per_step": 1e3 * el / steps,
+            "push_kernel_ms": pk, "field_ms": field_ms / max(field_n, 1),
+            "sort_ms": sort_ms / sort_n if sort_n else None,
+            "field_cell_updates_per_s": n ** 3 / (field_ms / max(field_n, 1) * 1e-3),
+            "tile": list(tile_ext), "tuned": bool(tuned),
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "bytes_per_push": BYTES_PER_PUSH["mixed3d"]},
+            "counters": counters}
+
+
 def profiled_traffic():
     """HBM bytes per launch of the push kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
@@ -224,6 +408,11 @@ def main():
                     help="steps of the batched-ensemble measurement (0: skip)")
     ap.add_argument("--c5-only", action="store_true",
                     help="only the batched-ensemble measurement (for profiling)")
+    ap.add_argument("--c1-steps", type=int, default=1000)
+    ap.add_argument("--c3-steps", type=int, default=2000)
+    ap.add_argument("--c4-steps", type=int, default=100)
+    ap.add_argument("--only", default="",
+                    help="comma list of c1,c3,c4,c5: run only those (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (read 1 GiB + write 1 GiB), "
                          "the reference point for the FETCH_SIZE/WRITE_SIZE counters")
@@ -245,8 +434,20 @@ def main():
     import aperture3_amd as apt
     apt.load_library()
     if args.c5_only:
-        progress("batched ensemble, 256 boxes")
-        print(json.dumps({"batched_ensemble": measure_c5(apt, args.c5_steps, args.warmup)}))
+        args.only = "c5"
+    if args.only:
+        out = {}
+        for name in args.only.split(","):
+            progress(f"{name} only")
+            if name == "c1":
+                out["c1_1d_fp64"] = measure_c1(apt, args.c1_steps)
+            elif name == "c3":
+                out["c3_logsph_mixed"] = measure_c3(apt, args.c3_steps)
+            elif name == "c4":
+                out["c4_3d_mixed"] = measure_c4(apt, args.c4_steps, 2)
+            elif name == "c5":
+                out["batched_ensemble"] = measure_c5(apt, args.c5_steps, args.warmup)
+        print(json.dumps(out))
         return 0
 
     progress("mixed form, 2048^2 box")
@@ -308,6 +509,17 @@ def main():
         line["fp64"] = fp64
         line["fp64_roofline_frac"] = (fp64["particles"] * BYTES_PER_PUSH["fp64"]
                                       / (fp64["push_kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+    configs = {}
+    if args.c1_steps > 0:
+        progress("1D gap, FP64")
+        configs["c1_1d_fp64"] = measure_c1(apt, args.c1_steps)
+    if args.c3_steps > 0:
+        progress("2D log-spherical magnetosphere, mixed")
+        configs["c3_logsph_mixed"] = measure_c3(apt, args.c3_steps)
+    if args.c4_steps > 0:
+        progress("3D 512^3 box, mixed")
+        configs["c4_3d_mixed"] = measure_c4(apt, args.c4_steps, 2)
+    line["configs"] = configs
     if args.c5_steps > 0:
         progress("batched ensemble, 256 boxes")
         line["batched_ensemble"] = measure_c5(apt, args.c5_steps, args.warmup)
