@@ -169,7 +169,10 @@ def measure_c5(apt, steps, warmup):
         "chunk": chunk, "grid_mult": gm, "tuned": bool(tuned),
         "roofline": {"bound": "hbm", "kernel": "persistent batch push (mixed)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "bytes_per_push": C5_BYTES_PER_PUSH},
+                     "frac": achieved / HBM_PEAK_GBS, "bytes_per_push": C5_BYTES_PER_PUSH,
+                     "traffic": profiled_traffic("batch_push_c5_mixed.md"),
+                     "traffic_source": "profiles/batch_push_c5_mixed.md (rocprofv3 "
+                                       "FETCH_SIZE x2 + WRITE_SIZE x1)"},
         "counters": counters,
     }
 
@@ -358,10 +361,10 @@ def measure_c4(apt, steps, warmup):
             "counters": counters}
 
 
-def profiled_traffic():
-    """HBM bytes per launch of the push kernel from the committed rocprofv3
+def profiled_traffic(name="push_deposit_c2_mixed.md"):
+    """HBM bytes per launch of a push kernel from its committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
-    path = os.path.join(ROOT, "profiles", "push_deposit_c2_mixed.md")
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
