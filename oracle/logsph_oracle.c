@@ -300,6 +300,11 @@ static void logsph_filter(const orc_mesh *m, double *j, double *tmp,
   memcpy(j, tmp, sizeof(double) * (size_t)m->dims[0] * m->dims[1]);
 }
 
+/* The filter alone, for the pinning tests. */
+void orc_logsph_filter(const orc_mesh *m, double *j, double *tmp, const double *A) {
+  logsph_filter(m, j, tmp, A);
+}
+
 /* ptc_updater_logsph::update_particles (ptc_updater_logsph.cu:681-810) and
  * apply_boundary (:812-846) on one rank: zero J/Rho, push + move + deposit,
  * process_j (:284-299), smoothing, clear guard cells, axis J3 (:414-438),

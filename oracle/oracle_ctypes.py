@@ -345,6 +345,11 @@ class LogSphOracle(OracleSim):
             self.move_photons(dt)
             self.clear_photon_guard_cells()
 
+    def filter(self, a, A):
+        """filter_current_logsph on one array with area/volume weights A."""
+        tmp = np.zeros_like(a)
+        self.lib.orc_logsph_filter(C.byref(self.mesh), _d(a), _d(tmp), _d(A))
+
     def field_update(self, dt, step=0):
         self.lib.orc_logsph_field_update(C.byref(self.mesh), C.byref(self.C),
                                          C.byref(self.lp), C.byref(self.F), C.c_double(dt))

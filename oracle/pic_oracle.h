@@ -100,6 +100,7 @@ void orc_update_e(const orc_mesh *m, orc_fields *f, double dt);
 void orc_update_b(const orc_mesh *m, orc_fields *f, double dt);
 void orc_compute_divs(const orc_mesh *m, orc_fields *f);
 void orc_guard_copy(const orc_mesh *m, double *a);
+void orc_logsph_filter(const orc_mesh *m, double *j, double *tmp, const double *A);
 void orc_filter_current(const orc_mesh *m, double *j, double *tmp,
                         int fix_3d_weight);
 
