@@ -563,34 +563,43 @@ __global__ void __launch_bounds__(NT)
     p.x3[idx] = n3;
     if (flag & FLAG_IGNORE_CURRENT) continue;
 
-    float sx0[4] = {0.f, 1.0f - x1, x1, 0.f};
-    float sy0[4] = {0.f, 1.0f - x2, x2, 0.f};
-    float sz0[4] = {0.f, 1.0f - x3, x3, 0.f};
-    float sx1[4], sy1[4], sz1[4];
+    // Esirkepov over the 3 positions per axis that can be non-zero: shape
+    // positions q = 0..3 hold offsets -2..1; with dc >= 0 position 0 is
+    // zero for both shapes, with dc == -1 position 3 is, so each axis walks
+    // q = b, b+1, b+2 (b = 0 if dc == -1 else 1): 27 points instead of 64,
+    // the same sums.  The last position's cumulative flux is the total
+    // (zero) and is not deposited, as in the 4-point form.
+    const int bx = dc1 == -1 ? 0 : 1, by = dc2 == -1 ? 0 : 1, bz = dc3 == -1 ? 0 : 1;
+    float sx0[3], sy0[3], sz0[3], sx1[3], sy1[3], sz1[3];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      sx1[q] = (q == 1 + dc1) ? 1.0f - n1 : ((q == 2 + dc1) ? n1 : 0.0f);
-      sy1[q] = (q == 1 + dc2) ? 1.0f - n2 : ((q == 2 + dc2) ? n2 : 0.0f);
-      sz1[q] = (q == 1 + dc3) ? 1.0f - n3 : ((q == 2 + dc3) ? n3 : 0.0f);
+    for (int t3 = 0; t3 < 3; t3++) {
+      const int qx = bx + t3, qy = by + t3, qz = bz + t3;
+      sx0[t3] = qx == 1 ? 1.0f - x1 : (qx == 2 ? x1 : 0.0f);
+      sy0[t3] = qy == 1 ? 1.0f - x2 : (qy == 2 ? x2 : 0.0f);
+      sz0[t3] = qz == 1 ? 1.0f - x3 : (qz == 2 ? x3 : 0.0f);
+      sx1[t3] = (qx == 1 + dc1) ? 1.0f - n1 : ((qx == 2 + dc1) ? n1 : 0.0f);
+      sy1[t3] = (qy == 1 + dc2) ? 1.0f - n2 : ((qy == 2 + dc2) ? n2 : 0.0f);
+      sz1[t3] = (qz == 1 + dc3) ? 1.0f - n3 : ((qz == 2 + dc3) ? n3 : 0.0f);
     }
-    const float qw = (float)sp.q[s] * w;
-    float djz[4][4];
+    const float qws = (float)sp.q[s] * w * scale;  // pre-scaled: one cvt per add
+    float djz[3][3];
 #pragma unroll
-    for (int q = 0; q < 4; q++)
+    for (int q = 0; q < 3; q++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) djz[q][r] = 0.0f;
+      for (int r = 0; r < 3; r++) djz[q][r] = 0.0f;
 #pragma unroll
-    for (int tk = 0; tk < 4; tk++) {
-      float djy[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int tk = 0; tk < 3; tk++) {
+      float djy[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-      for (int tj = 0; tj < 4; tj++) {
+      for (int tj = 0; tj < 3; tj++) {
         const float cyz = (2.0f * sy1[tj] * sz1[tk] + sy0[tj] * sz1[tk] +
                            sy1[tj] * sz0[tk] + 2.0f * sy0[tj] * sz0[tk]) *
                           (1.0f / 6.0f);
         float djx = 0.0f;
 #pragma unroll
-        for (int ti2 = 0; ti2 < 4; ti2++) {
-          const int di = ti2 - 2, dj = tj - 2, dk = tk - 2;
+        for (int ti2 = 0; ti2 < 3; ti2++) {
+          const int qx = bx + ti2, qy = by + tj, qz = bz + tk;
+          const int di = qx - 2, dj = qy - 2, dk = qz - 2;
           djx += (sx1[ti2] - sx0[ti2]) * cyz;
           const float cxz = (2.0f * sx1[ti2] * sz1[tk] + sx0[ti2] * sz1[tk] +
                              sx1[ti2] * sz0[tk] + 2.0f * sx0[ti2] * sz0[tk]) *
@@ -601,9 +610,9 @@ __global__ void __launch_bounds__(NT)
                             (1.0f / 6.0f);
           djz[tj][ti2] += (sz1[tk] - sz0[tk]) * cxy;
           // reference signs (ptc_updater.cu:445-487): J = -q w (shape flux)
-          const long long q1 = ti2 < 3 ? fxq(-qw * djx, scale) : 0;
-          const long long q2 = tj < 3 ? fxq(-qw * djy[ti2], scale) : 0;
-          const long long q3 = tk < 3 ? fxq(-qw * djz[tj][ti2], scale) : 0;
+          const long long q1 = qx < 3 ? fxi(-qws * djx) : 0;
+          const long long q2 = qy < 3 ? fxi(-qws * djy[ti2]) : 0;
+          const long long q3 = qz < 3 ? fxi(-qws * djz[tj][ti2]) : 0;
           if (in) {
             const int b = (l1 + di) + ((l2 + dj) + (l3 + dk) * W) * W;
             lds_add(&sJ[0][b + 1], q1);
@@ -629,6 +638,17 @@ __global__ void __launch_bounds__(NT)
     for (int c = 0; c < 3; c++) glb_add(a.J[c] + o, (long long)sJ[c][i]);
   }
 }
+
+// 3D tile shapes compiled in: (T, halo, threads).  The current window is
+// (T + 2 halo)^3 cells x 3 int64: 8/5 -> 137 KB, one workgroup per CU.
+#define TILE3D_SHAPES(X) \
+  X(8, 3, 256)           \
+  X(8, 4, 256)           \
+  X(4, 3, 256)           \
+  X(8, 4, 512)           \
+  X(8, 4, 1024)          \
+  X(8, 5, 512)           \
+  X(8, 5, 1024)
 
 // 2D tile shapes compiled in: (T1, T2, halo, threads)
 #define TILE2D_SHAPES(X) \
@@ -667,17 +687,15 @@ hipError_t launch_push_deposit_tiled(const Mesh &m, const Species &sp,
 #undef LAUNCH2
     return hipErrorInvalidValue;
   } else if (m.dim == 3) {
-    if (ti.T[0] == 8 && ti.halo == 3)
-      k_push_deposit_tile3d<8, 3, 256><<<ti.ntiles, 256, 0, st>>>(
-          m, sp, dt, scale, p, f, a, ti, counters);
-    else if (ti.T[0] == 8 && ti.halo == 4)
-      k_push_deposit_tile3d<8, 4, 256><<<ti.ntiles, 256, 0, st>>>(
-          m, sp, dt, scale, p, f, a, ti, counters);
-    else if (ti.T[0] == 4 && ti.halo == 3)
-      k_push_deposit_tile3d<4, 3, 256><<<ti.ntiles, 256, 0, st>>>(
-          m, sp, dt, scale, p, f, a, ti, counters);
-    else
-      return hipErrorInvalidValue;
+#define LAUNCH3(TT, HH, NTH)                                            \
+  if (ti.T[0] == TT && ti.halo == HH && ti.threads == NTH) {            \
+    k_push_deposit_tile3d<TT, HH, NTH><<<ti.ntiles, NTH, 0, st>>>(      \
+        m, sp, dt, scale, p, f, a, ti, counters);                       \
+    return hipGetLastError();                                           \
+  }
+    TILE3D_SHAPES(LAUNCH3)
+#undef LAUNCH3
+    return hipErrorInvalidValue;
   } else {
     return hipErrorInvalidValue;
   }
@@ -692,10 +710,13 @@ bool tile_shape_supported(int dim, const int T[3], int halo, int threads) {
 #undef CHECK2
     return false;
   }
-  if (dim == 3)
-    return threads == 256 && T[0] == T[1] && T[1] == T[2] &&
-           ((T[0] == 8 && (halo == 3 || halo == 4)) ||
-            (T[0] == 4 && halo == 3));
+  if (dim == 3) {
+    if (T[0] != T[1] || T[1] != T[2]) return false;
+#define CHECK3(TT, HH, NTH) \
+  if (T[0] == TT && halo == HH && threads == NTH) return true;
+    TILE3D_SHAPES(CHECK3)
+#undef CHECK3
+  }
   return false;
 }
 

@@ -76,9 +76,11 @@ def test_tiled_loader_draws_the_fp64_particle_set():
     check_binning(env, d)
 
 
-@pytest.mark.parametrize("N,ppc,tile", [((64, 64), 4, (16, 16)), ((32, 32), 3, (8, 8)),
-                                        ((16, 16, 16), 2, (8, 8, 8))])
-def test_tiled_step_vs_oracle(N, ppc, tile):
+@pytest.mark.parametrize("N,ppc,tile,halo,threads", [
+    ((64, 64), 4, (16, 16), 0, 0), ((32, 32), 3, (8, 8), 0, 0),
+    ((16, 16, 16), 2, (8, 8, 8), 0, 0), ((24, 24, 24), 2, (8, 8, 8), 4, 512),
+    ((24, 24, 24), 2, (8, 8, 8), 5, 1024)])
+def test_tiled_step_vs_oracle(N, ppc, tile, halo, threads):
     """Tiled mixed steps with tile sorts against the FP64 oracle: every
     particle (matched by tag) keeps the oracle's cell and agrees to FP32
     tolerance; E and B within 1e-5 of their scale."""
@@ -89,7 +91,8 @@ def test_tiled_step_vs_oracle(N, ppc, tile):
     import aperture3_amd as a
     p = a.SimParams(N=c.N, guard=c.guard, periodic_boundary=[True] * 3,
                     max_ptc_number=c.nptc + 64, precision=a.APT_PREC_MIXED,
-                    tile=list(tile) + [0] * (3 - len(tile)), data_interval=100)
+                    tile=list(tile) + [0] * (3 - len(tile)), tile_halo=halo,
+                    tile_threads=threads, data_interval=100)
     env = a.sim_environment(p)
     d = a.sim_data(env)
     E, B = c.fields()
