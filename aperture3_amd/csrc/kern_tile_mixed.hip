@@ -483,6 +483,9 @@ __global__ void __launch_bounds__(NT)
                           unsigned long long *__restrict__ counters) {
   constexpr int W = T + 2 * H, WN = W * W * W;
   __shared__ unsigned long long sJ[3][WN];
+  __shared__ float s_qm[8];  // dt q / (2 m) per species
+  if (threadIdx.x < 8)
+    s_qm[threadIdx.x] = (float)(0.5 * sp.q[threadIdx.x] / sp.m[threadIdx.x]) * dt;
 
   const uint32_t t = xcd_tile(blockIdx.x, ti.ntiles);
   const int tx = (int)(t % (uint32_t)ti.nt[0]);
@@ -504,21 +507,53 @@ __global__ void __launch_bounds__(NT)
   const float idx0 = (float)m.inv_delta[0] * dt;
   const float idx1 = (float)m.inv_delta[1] * dt;
   const float idx2 = (float)m.inv_delta[2] * dt;
-  unsigned long long outside = 0;
+  unsigned long long outside = 0, multi = 0;
+  // Software pipeline over the tile's particles: the next particle's
+  // arrays are requested before the current one's field taps are used.
+  // Loads are straight-line (a lane past the end re-reads the last slot).
+  struct P3 {
+    uint32_t cell, flag;
+    float x1, x2, x3, u1, u2, u3, w;
+  };
+  auto load = [&](uint32_t i, P3 &r) {
+    const uint32_t j = i < end ? i : end - 1;
+    r.cell = p.cell[j];
+    r.flag = p.flag[j];
+    r.x1 = p.x1[j];
+    r.x2 = p.x2[j];
+    r.x3 = p.x3[j];
+    r.u1 = p.u1[j];
+    r.u2 = p.u2[j];
+    r.u3 = p.u3[j];
+    r.w = p.w[j];
+  };
+  P3 cur, nxt;
+  if (beg + threadIdx.x < end) load(beg + threadIdx.x, cur);
   for (uint32_t idx = beg + threadIdx.x; idx < end; idx += NT) {
-    uint32_t c = p.cell[idx];
-    if (c == MAX_CELL) continue;
+    const uint32_t c = cur.cell;
     int c1, c2, c3;
-    m.split(c, c1, c2, c3);
-    if (!m.in_bulk(c1, c2, c3)) {
-      p.cell[idx] = MAX_CELL;
-      continue;
+    m.split(c == MAX_CELL ? (uint32_t)m.idx(g0, g1, g2) : c, c1, c2, c3);
+    const bool live = c != MAX_CELL && m.in_bulk(c1, c2, c3);
+    if (!live) {  // safe taps for a dead slot
+      c1 = g0;
+      c2 = g1;
+      c3 = g2;
     }
-    uint32_t flag = p.flag[idx];
-    int s = ptc_species(flag);
-    float x1 = p.x1[idx], x2 = p.x2[idx], x3 = p.x3[idx];
-    float u1 = p.u1[idx], u2 = p.u2[idx], u3 = p.u3[idx];
-    float w = p.w[idx];
+    const uint32_t flag = cur.flag;
+    const int s = ptc_species(flag);
+    const float x1 = cur.x1, x2 = cur.x2, x3 = cur.x3;
+    float u1 = cur.u1, u2 = cur.u2, u3 = cur.u3;
+    const float w = cur.w;
+    // field taps of the current particle (issued before the prefetch)
+    const float E1 = gather3g(f.E[0], d0, d01, c1, c2, c3, x1, x2, x3, ST_E1);
+    const float E2 = gather3g(f.E[1], d0, d01, c1, c2, c3, x1, x2, x3, ST_E2);
+    const float E3 = gather3g(f.E[2], d0, d01, c1, c2, c3, x1, x2, x3, ST_E3);
+    const float B1 = gather3g(f.B[0], d0, d01, c1, c2, c3, x1, x2, x3, ST_B1);
+    const float B2 = gather3g(f.B[1], d0, d01, c1, c2, c3, x1, x2, x3, ST_B2);
+    const float B3 = gather3g(f.B[2], d0, d01, c1, c2, c3, x1, x2, x3, ST_B3);
+    load(idx + NT, nxt);
+    if (c != MAX_CELL && !live) p.cell[idx] = MAX_CELL;
+    if (live) {
     int l1 = c1 - w0, l2 = c2 - w1, l3 = c3 - w2;
     if (m.periodic[0]) { if (l1 < -H) l1 += N0; else if (l1 >= N0 - H) l1 -= N0; }
     if (m.periodic[1]) { if (l2 < -H) l2 += N1; else if (l2 >= N1 - H) l2 -= N1; }
@@ -526,21 +561,18 @@ __global__ void __launch_bounds__(NT)
     const bool in = (l1 >= 2) & (l1 <= W - 3) & (l2 >= 2) & (l2 <= W - 3) &
                     (l3 >= 2) & (l3 <= W - 3);
     outside += in ? 0 : 1;
-    float gamma = sqrtf(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+    const float gamma = hw_sqrt(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+    float ig;
     if (!(flag & FLAG_IGNORE_EM)) {
-      float qm = (float)(0.5 * sp.q[s] / sp.m[s]) * dt;
-      float E1 = gather3g(f.E[0], d0, d01, c1, c2, c3, x1, x2, x3, ST_E1) * qm;
-      float E2 = gather3g(f.E[1], d0, d01, c1, c2, c3, x1, x2, x3, ST_E2) * qm;
-      float E3 = gather3g(f.E[2], d0, d01, c1, c2, c3, x1, x2, x3, ST_E3) * qm;
-      float B1 = gather3g(f.B[0], d0, d01, c1, c2, c3, x1, x2, x3, ST_B1) * qm;
-      float B2 = gather3g(f.B[1], d0, d01, c1, c2, c3, x1, x2, x3, ST_B2) * qm;
-      float B3 = gather3g(f.B[2], d0, d01, c1, c2, c3, x1, x2, x3, ST_B3) * qm;
-      gamma = 1.0f / vay_push_f(u1, u2, u3, gamma, E1, E2, E3, B1, B2, B3);
+      const float qm = s_qm[s];
+      ig = vay_push_f(u1, u2, u3, gamma, E1 * qm, E2 * qm, E3 * qm, B1 * qm, B2 * qm,
+                      B3 * qm);
       p.u1[idx] = u1;
       p.u2[idx] = u2;
       p.u3[idx] = u3;
+    } else {
+      ig = hw_rcp(gamma);
     }
-    const float ig = 1.0f / gamma;
     const float v1 = u1 * ig, v2 = u2 * ig, v3 = u3 * ig;
     float n1 = x1 + v1 * idx0, n2 = x2 + v2 * idx1, n3 = x3 + v3 * idx2;
     int dc1 = (int)floorf(n1), dc2 = (int)floorf(n2), dc3 = (int)floorf(n3);
@@ -551,7 +583,7 @@ __global__ void __launch_bounds__(NT)
     if (n2 >= 1.0f) { n2 -= 1.0f; dc2 += 1; }
     if (n3 >= 1.0f) { n3 -= 1.0f; dc3 += 1; }
     if (dc1 > 1 || dc1 < -1 || dc2 > 1 || dc2 < -1 || dc3 > 1 || dc3 < -1) {
-      atomicAdd(&counters[CTR_MULTICELL], 1ull);
+      multi++;
       dc1 = max(-1, min(1, dc1));
       dc2 = max(-1, min(1, dc2));
       dc3 = max(-1, min(1, dc3));
@@ -561,15 +593,19 @@ __global__ void __launch_bounds__(NT)
     p.x1[idx] = n1;
     p.x2[idx] = n2;
     p.x3[idx] = n3;
-    if (flag & FLAG_IGNORE_CURRENT) continue;
-
-    // Esirkepov over the 3 positions per axis that can be non-zero: shape
-    // positions q = 0..3 hold offsets -2..1; with dc >= 0 position 0 is
-    // zero for both shapes, with dc == -1 position 3 is, so each axis walks
-    // q = b, b+1, b+2 (b = 0 if dc == -1 else 1): 27 points instead of 64,
-    // the same sums.  The last position's cumulative flux is the total
-    // (zero) and is not deposited, as in the 4-point form.
+    if (!(flag & FLAG_IGNORE_CURRENT)) {
+    // Esirkepov in separable form.  Along each axis the shapes live on the
+    // 3 positions t = 0..2 starting at b = (dc == -1 ? 0 : 1) (offsets
+    // b + t - 2); the cumulative shape difference D(t) is non-zero only for
+    // t <= |dc|, and the transverse weights only for t < 2 + |dc|.  With
+    //   c_yz(j,k) = (sz1 (2 sy1 + sy0) + sz0 (sy1 + 2 sy0)) / 6 (and
+    //   likewise c_xz, c_xy),
+    // J1(t,j,k) = -q w Dx(t) c_yz(j,k), J2 = -q w Dy(t) c_xz(i,k),
+    // J3 = -q w Dz(t) c_xy(i,j) (ptc_updater.cu:445-487): the same sums as
+    // the cumulative loops, with 4 + 4 + 4 adds for a particle that stays
+    // in its cell (up to 18 each when it crosses).
     const int bx = dc1 == -1 ? 0 : 1, by = dc2 == -1 ? 0 : 1, bz = dc3 == -1 ? 0 : 1;
+    const int ax1 = dc1 != 0, ay1 = dc2 != 0, az1 = dc3 != 0;  // |dc|
     float sx0[3], sy0[3], sz0[3], sx1[3], sy1[3], sz1[3];
 #pragma unroll
     for (int t3 = 0; t3 < 3; t3++) {
@@ -581,52 +617,54 @@ __global__ void __launch_bounds__(NT)
       sy1[t3] = (qy == 1 + dc2) ? 1.0f - n2 : ((qy == 2 + dc2) ? n2 : 0.0f);
       sz1[t3] = (qz == 1 + dc3) ? 1.0f - n3 : ((qz == 2 + dc3) ? n3 : 0.0f);
     }
-    const float qws = (float)sp.q[s] * w * scale;  // pre-scaled: one cvt per add
-    float djz[3][3];
+    const float Dx[2] = {sx1[0] - sx0[0], sx1[0] - sx0[0] + (sx1[1] - sx0[1])};
+    const float Dy[2] = {sy1[0] - sy0[0], sy1[0] - sy0[0] + (sy1[1] - sy0[1])};
+    const float Dz[2] = {sz1[0] - sz0[0], sz1[0] - sz0[0] + (sz1[1] - sz0[1])};
+    float hx[3], gx[3], hy[3], gy[3];
 #pragma unroll
-    for (int q = 0; q < 3; q++)
+    for (int t3 = 0; t3 < 3; t3++) {
+      hx[t3] = 2.0f * sx1[t3] + sx0[t3];
+      gx[t3] = sx1[t3] + 2.0f * sx0[t3];
+      hy[t3] = 2.0f * sy1[t3] + sy0[t3];
+      gy[t3] = sy1[t3] + 2.0f * sy0[t3];
+    }
+    // -q w scale / 6, pre-scaled so each add is one multiply and one cvt
+    const float qs = (float)sp.q[s] * w * scale * (-1.0f / 6.0f);
+    const int b0 = (l1 + bx - 2) + ((l2 + by - 2) + (l3 + bz - 2) * W) * W;
+    const int gx0 = c1 + bx - 2, gy0 = c2 + by - 2, gz0 = c3 + bz - 2;
 #pragma unroll
-      for (int r = 0; r < 3; r++) djz[q][r] = 0.0f;
+    for (int u = 0; u < 3; u++) {
 #pragma unroll
-    for (int tk = 0; tk < 3; tk++) {
-      float djy[3] = {0.f, 0.f, 0.f};
+      for (int v = 0; v < 3; v++) {
+        // J1 at (t, j=u, k=v), J2 at (i=u, t, k=v), J3 at (i=u, j=v, t)
+        const float cyz = sz1[v] * hy[u] + sz0[v] * gy[u];
+        const float cxz = sz1[v] * hx[u] + sz0[v] * gx[u];
+        const float cxy = sy1[v] * hx[u] + sy0[v] * gx[u];
 #pragma unroll
-      for (int tj = 0; tj < 3; tj++) {
-        const float cyz = (2.0f * sy1[tj] * sz1[tk] + sy0[tj] * sz1[tk] +
-                           sy1[tj] * sz0[tk] + 2.0f * sy0[tj] * sz0[tk]) *
-                          (1.0f / 6.0f);
-        float djx = 0.0f;
-#pragma unroll
-        for (int ti2 = 0; ti2 < 3; ti2++) {
-          const int qx = bx + ti2, qy = by + tj, qz = bz + tk;
-          const int di = qx - 2, dj = qy - 2, dk = qz - 2;
-          djx += (sx1[ti2] - sx0[ti2]) * cyz;
-          const float cxz = (2.0f * sx1[ti2] * sz1[tk] + sx0[ti2] * sz1[tk] +
-                             sx1[ti2] * sz0[tk] + 2.0f * sx0[ti2] * sz0[tk]) *
-                            (1.0f / 6.0f);
-          djy[ti2] += (sy1[tj] - sy0[tj]) * cxz;
-          const float cxy = (2.0f * sx1[ti2] * sy1[tj] + sx0[ti2] * sy1[tj] +
-                             sx1[ti2] * sy0[tj] + 2.0f * sx0[ti2] * sy0[tj]) *
-                            (1.0f / 6.0f);
-          djz[tj][ti2] += (sz1[tk] - sz0[tk]) * cxy;
-          // reference signs (ptc_updater.cu:445-487): J = -q w (shape flux)
-          const long long q1 = qx < 3 ? fxi(-qws * djx) : 0;
-          const long long q2 = qy < 3 ? fxi(-qws * djy[ti2]) : 0;
-          const long long q3 = qz < 3 ? fxi(-qws * djz[tj][ti2]) : 0;
+        for (int t = 0; t < 2; t++) {
+          const bool on1 = (t <= ax1) & (u < 2 + ay1) & (v < 2 + az1);
+          const bool on2 = (t <= ay1) & (u < 2 + ax1) & (v < 2 + az1);
+          const bool on3 = (t <= az1) & (u < 2 + ax1) & (v < 2 + ay1);
+          const long long q1 = fxi(qs * Dx[t] * cyz);
+          const long long q2 = fxi(qs * Dy[t] * cxz);
+          const long long q3 = fxi(qs * Dz[t] * cxy);
           if (in) {
-            const int b = (l1 + di) + ((l2 + dj) + (l3 + dk) * W) * W;
-            lds_add(&sJ[0][b + 1], q1);
-            lds_add(&sJ[1][b + W], q2);
-            lds_add(&sJ[2][b + W * W], q3);
+            if (on1) lds_add(&sJ[0][b0 + t + (u + v * W) * W + 1], q1);
+            if (on2) lds_add(&sJ[1][b0 + u + (t + v * W) * W + W], q2);
+            if (on3) lds_add(&sJ[2][b0 + u + (v + t * W) * W + W * W], q3);
           } else {
-            glb_add(a.J[0] + m.widx(c1 + di + 1, c2 + dj, c3 + dk), q1);
-            glb_add(a.J[1] + m.widx(c1 + di, c2 + dj + 1, c3 + dk), q2);
-            glb_add(a.J[2] + m.widx(c1 + di, c2 + dj, c3 + dk + 1), q3);
+            if (on1) glb_add(a.J[0] + m.widx(gx0 + t + 1, gy0 + u, gz0 + v), q1);
+            if (on2) glb_add(a.J[1] + m.widx(gx0 + u, gy0 + t + 1, gz0 + v), q2);
+            if (on3) glb_add(a.J[2] + m.widx(gx0 + u, gy0 + v, gz0 + t + 1), q3);
           }
         }
       }
     }
+    }  // current
+    }  // live
+    cur = nxt;
   }
+  if (multi) atomicAdd(&counters[CTR_MULTICELL], multi);
   if (outside) atomicAdd(&counters[CTR_OUTSIDE], outside);
   __syncthreads();
   for (int i = threadIdx.x; i < WN; i += NT) {
