@@ -126,6 +126,7 @@ SIGNATURES = {
     "apt_get_jexp": [_P, C.POINTER(C.c_int)],
     "apt_get_mesh": [_P, C.POINTER(C.c_int), C.POINTER(C.c_double)],
     "apt_get_info": [_P, C.POINTER(AptInfo)],
+    "apt_set_counters": [_P, C.POINTER(AptInfo)],
     "apt_get_coef": [_P, C.c_int, C.POINTER(C.c_double), C.c_uint64],
     "apt_ph_set": [_P, C.POINTER(AptPhotons), C.c_uint64],
     "apt_ph_get": [_P, C.POINTER(AptPhotons), C.c_uint64, C.POINTER(C.c_uint64)],
@@ -241,6 +242,11 @@ class SimParams:
         self.lower = [0.0, 0.0, 0.0]
         self.size = [1.0, 1.0, 1.0]
         self.precision = APT_PREC_FP64
+        # output (sim_params.h:103-117, 26)
+        self.data_dir = "Data/"
+        self.conf_file = ""
+        self.downsample = 1
+        self.max_tracked = 100
         self.tile = [0, 0, 0]
         self.tile_halo = 0
         self.tile_threads = 0
@@ -332,6 +338,7 @@ class sim_data:
         if st != APT_OK:
             raise AptError(f"apt_create failed with status {st}")
         self.ctx = h
+        self.grid = g
 
     def _check(self, st, what):
         if st != APT_OK:
@@ -810,3 +817,6 @@ class batch_ensemble:
         a = np.ascontiguousarray(arr, np.float64)
         self._check(self.lib.apt_batch_set_field(self.b, k, which, comp, _dptr(a), a.size),
                     "apt_batch_set_field")
+
+
+from .exporter import data_exporter  # noqa: E402  (uses the classes above)

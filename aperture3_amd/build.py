@@ -104,7 +104,41 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
           os.path.join(CSRC, "host", "pic_step_driver.cpp"), "-o", drv + ".tmp",
           f"-L{HERE}", "-laperture_amd", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN"])
     os.replace(drv + ".tmp", drv)
+    build_io()
     return LIB
+
+
+# HDF5 output and snapshots link the HDF5 C library; the library is built
+# where the HDF5 headers are found and skipped (with a message) elsewhere.
+HDF5_ROOTS = [os.environ.get("APT_HDF5_ROOT"), "/opt/conda", "/usr"]
+IO_LIB = os.path.join(HERE, "libaperture_io.so")
+
+
+def hdf5_root() -> str | None:
+    for root in HDF5_ROOTS:
+        if root and os.path.exists(os.path.join(root, "include", "hdf5.h")):
+            return root
+    return None
+
+
+def build_io() -> str | None:
+    root = hdf5_root()
+    if root is None:
+        print("aperture3_amd: hdf5.h not found, libaperture_io.so not built",
+              file=sys.stderr)
+        return None
+    libdir = os.path.join(root, "lib")
+    tmp = IO_LIB + ".tmp"
+    # libstdc++ is linked statically: the HDF5 directory may hold an older
+    # libstdc++.so that the run path would otherwise hand to the HIP runtime
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+          "-static-libstdc++", "-static-libgcc",
+          f"-I{INCLUDE}", f"-I{os.path.join(root, 'include')}",
+          os.path.join(CSRC, "io", "apt_io.cpp"), "-o", tmp,
+          f"-L{HERE}", "-laperture_amd", f"-L{libdir}", "-lhdf5",
+          "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{libdir}"])
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
 
 
 if __name__ == "__main__":

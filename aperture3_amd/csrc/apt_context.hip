@@ -529,6 +529,27 @@ apt_status apt_get_info(const apt_ctx *ctx, apt_info *out) {
   return APT_OK;
 }
 
+// restores the id counters, random-stream epochs and fixed-point exponents
+// (a snapshot reload); counts come from the restored arrays.  Only the
+// exponents of the weight scales matter, so they are restored as powers of
+// two that give the same H and Hs.
+apt_status apt_set_counters(apt_ctx *ctx, const apt_info *in) {
+  CHECK_CTX(ctx);
+  if (!in) return APT_ERR_INVALID_ARG;
+  const int e = (ctx->prec == APT_PREC_FP64 ? 22 : 30) - in->H;
+  const int es = 22 - in->Hs;
+  if (e < -60 || e > 60 || es < 0 || es > 60)
+    return ctx->fail(APT_ERR_INVALID_ARG, "set_counters: exponents out of range");
+  ctx->id_next = in->id_next;
+  ctx->ph_id_next = in->ph_id_next;
+  ctx->rt_epoch = in->rt_epoch;
+  ctx->inject_epoch = in->inject_epoch;
+  ctx->wmax = ldexp(1.0, e);
+  update_scale(ctx);
+  ctx->wmaxw = ldexp(1.0, es);
+  return APT_OK;
+}
+
 apt_status apt_get_coef(apt_ctx *ctx, int which, double *dst, uint64_t count) {
   CHECK_CTX(ctx);
   if (!ctx->logsph)

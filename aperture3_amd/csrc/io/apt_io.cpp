@@ -1,0 +1,658 @@
+// apt_io.cpp -- HDF5 output and snapshots (the formats of the reference's
+// src/utils/data_exporter.cpp), host code over the HDF5 C library and the
+// context's C ABI.
+#include <hdf5.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "aperture_io.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+apt_status fail(apt_status st, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return st;
+}
+
+// an open HDF5 file closed on scope exit
+struct H5 {
+  hid_t id = -1;
+  ~H5() {
+    if (id >= 0) H5Fclose(id);
+  }
+};
+
+// dims of a mesh array in HDF5 order (slowest first), as H5File::
+// write_parallel orders an Extent (hdf_wrapper_impl.hpp:79-84)
+int h5_dims(int dim, const int ext[3], hsize_t out[3]) {
+  for (int i = 0; i < dim; i++) out[i] = (hsize_t)ext[dim - 1 - i];
+  return dim;
+}
+
+template <typename T>
+hid_t h5type();
+template <>
+hid_t h5type<double>() { return H5T_NATIVE_DOUBLE; }
+template <>
+hid_t h5type<float>() { return H5T_NATIVE_FLOAT; }
+template <>
+hid_t h5type<uint32_t>() { return H5T_NATIVE_UINT32; }
+template <>
+hid_t h5type<uint64_t>() { return H5T_NATIVE_UINT64; }
+template <>
+hid_t h5type<int>() { return H5T_NATIVE_INT; }
+
+template <typename T>
+bool write_array(hid_t file, const char *name, int rank, const hsize_t *dims,
+                 const T *data) {
+  hid_t space = H5Screate_simple(rank, dims, nullptr);
+  if (space < 0) return false;
+  hid_t ds = H5Dcreate2(file, name, h5type<T>(), space, H5P_DEFAULT, H5P_DEFAULT,
+                        H5P_DEFAULT);
+  bool ok = ds >= 0 &&
+            H5Dwrite(ds, h5type<T>(), H5S_ALL, H5S_ALL, H5P_DEFAULT, data) >= 0;
+  if (ds >= 0) H5Dclose(ds);
+  H5Sclose(space);
+  return ok;
+}
+
+template <typename T>
+bool write_vec(hid_t file, const std::string &name, const std::vector<T> &v) {
+  hsize_t n = v.size();
+  // a zero-length dataset is legal; give HDF5 a valid pointer
+  static const T dummy{};
+  return write_array<T>(file, name.c_str(), 1, &n, v.empty() ? &dummy : v.data());
+}
+
+template <typename T>
+bool write_scalar(hid_t file, const char *name, T v) {
+  hid_t space = H5Screate(H5S_SCALAR);
+  hid_t ds = H5Dcreate2(file, name, h5type<T>(), space, H5P_DEFAULT, H5P_DEFAULT,
+                        H5P_DEFAULT);
+  bool ok = ds >= 0 && H5Dwrite(ds, h5type<T>(), H5S_ALL, H5S_ALL, H5P_DEFAULT, &v) >= 0;
+  if (ds >= 0) H5Dclose(ds);
+  H5Sclose(space);
+  return ok;
+}
+
+// reads a whole dataset of `count` elements (checked) into out
+template <typename T>
+bool read_array(hid_t file, const char *name, T *out, uint64_t count) {
+  hid_t ds = H5Dopen2(file, name, H5P_DEFAULT);
+  if (ds < 0) return false;
+  hid_t space = H5Dget_space(ds);
+  hssize_t n = H5Sget_simple_extent_npoints(space);
+  bool ok = n >= 0 && (uint64_t)n == count;
+  if (ok && count > 0)
+    ok = H5Dread(ds, h5type<T>(), H5S_ALL, H5S_ALL, H5P_DEFAULT, out) >= 0;
+  H5Sclose(space);
+  H5Dclose(ds);
+  return ok;
+}
+
+template <typename T>
+bool read_scalar(hid_t file, const char *name, T *out) {
+  return read_array<T>(file, name, out, 1);
+}
+
+uint64_t dataset_size(hid_t file, const char *name) {
+  hid_t ds = H5Dopen2(file, name, H5P_DEFAULT);
+  if (ds < 0) return (uint64_t)-1;
+  hid_t space = H5Dget_space(ds);
+  hssize_t n = H5Sget_simple_extent_npoints(space);
+  H5Sclose(space);
+  H5Dclose(ds);
+  return n < 0 ? (uint64_t)-1 : (uint64_t)n;
+}
+
+struct MeshInfo {
+  int dims[3];
+  double delta[3];
+  int dim;
+  uint64_t ncell;
+};
+
+apt_status mesh_of(apt_ctx *ctx, MeshInfo &m) {
+  apt_status st = apt_get_mesh(ctx, m.dims, m.delta);
+  if (st != APT_OK) return fail(st, "apt_get_mesh failed");
+  m.dim = 1;
+  for (int d = 0; d < 3; d++)
+    if (m.dims[d] > 1) m.dim = d + 1;
+  m.ncell = (uint64_t)m.dims[0] * m.dims[1] * m.dims[2];
+  return APT_OK;
+}
+
+// stagger of each output field (fields_impl.hpp:85-94, sim_data_impl.hpp:51-65)
+int stagger_of(int which, int comp) {
+  static const int e[3] = {0x6, 0x5, 0x3}, b[3] = {0x1, 0x2, 0x4};
+  switch (which) {
+    case APT_FIELD_E:
+    case APT_FIELD_EBG:
+    case APT_FIELD_J:
+      return e[comp];
+    case APT_FIELD_B:
+    case APT_FIELD_BBG:
+      return b[comp];
+    case APT_FIELD_RHO:
+    case APT_FIELD_DIVE:
+      return 0x7;
+    default:
+      return 0;
+  }
+}
+
+void out_extent(const apt_grid *g, int dim, int d, int ext[3]) {
+  for (int i = 0; i < 3; i++) {
+    ext[i] = i < dim ? g->N[i] : 1;
+    if (ext[i] > d) ext[i] /= d;
+  }
+}
+
+// Quadmesh::pos(i, n, stagger) (quadmesh.h:119-150) with stagger 0
+double pos(const apt_grid *g, int dim, int i, int n) {
+  const double delta = g->size[i] / g->N[i];
+  if (i < dim) return g->lower[i] + delta * (n - g->guard[i] + 0.5);
+  return 0.5 * delta;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *apt_io_last_error(void) { return g_err.c_str(); }
+
+apt_status apt_io_downsample(const double *f, const int dims[3], const int guard[3],
+                             int dim, int st, int d, float *out) {
+  if (!f || !dims || !guard || !out || d < 1 || dim < 1 || dim > 3)
+    return fail(APT_ERR_INVALID_ARG, "downsample: bad arguments");
+  int ext[3];
+  for (int i = 0; i < 3; i++) {
+    ext[i] = i < dim ? dims[i] - 2 * guard[i] : 1;
+    if (ext[i] > d) ext[i] /= d;
+  }
+  // neighbour offsets toward the node stagger 0b111 (interpolation.cuh:
+  // 213-262): -1 - out and -out where the input stagger differs
+  int om[3], op[3];
+  for (int a = 0; a < 3; a++) {
+    const bool same = ((st >> a) & 1) == 1;
+    om[a] = same ? 0 : -2;
+    op[a] = same ? 0 : -1;
+  }
+  const int64_t d0 = dims[0], d01 = (int64_t)dims[0] * dims[1];
+  for (int k = 0; k < ext[2]; k++)
+    for (int j = 0; j < ext[1]; j++)
+      for (int i = 0; i < ext[0]; i++) {
+        const int64_t o = (int64_t)(i * d + guard[0]) + (int64_t)(j * d + guard[1]) * d0 +
+                          (int64_t)(k * d + guard[2]) * d01;
+        // every neighbour index must stay in the array
+        for (int a = 0; a < dim; a++) {
+          const int c = (a == 0 ? i : (a == 1 ? j : k)) * d + guard[a];
+          if (c + om[a] < 0 || c + op[a] >= dims[a])
+            return fail(APT_ERR_INVALID_ARG, "downsample: guard too small");
+        }
+        double v;
+        if (dim == 1) {
+          const double f1 = f[o + op[0]], f0 = f[o + om[0]];
+          v = 0.5 * (f1 + f0);
+        } else if (dim == 2) {
+          const double f1 = 0.5 * (f[o + op[0] + op[1] * d0] + f[o + op[0] + om[1] * d0]);
+          const double f0 = 0.5 * (f[o + om[0] + op[1] * d0] + f[o + om[0] + om[1] * d0]);
+          v = 0.5 * (f1 + f0);
+        } else {
+          auto at = [&](int x, int y, int z) { return f[o + x + y * d0 + z * d01]; };
+          const double f11 = 0.5 * (at(op[0], op[1], om[2]) + at(op[0], op[1], op[2]));
+          const double f10 = 0.5 * (at(op[0], om[1], om[2]) + at(op[0], om[1], op[2]));
+          const double f01 = 0.5 * (at(om[0], op[1], om[2]) + at(om[0], op[1], op[2]));
+          const double f00 = 0.5 * (at(om[0], om[1], om[2]) + at(om[0], om[1], op[2]));
+          const double f1 = 0.5 * (f11 + f10), f0 = 0.5 * (f01 + f00);
+          v = 0.5 * (f1 + f0);
+        }
+        out[i + (int64_t)j * ext[0] + (int64_t)k * ext[0] * ext[1]] = (float)v;
+      }
+  return APT_OK;
+}
+
+apt_status apt_io_save_snapshot(apt_ctx *ctx, const char *filename, uint32_t step,
+                                double time, int data_interval, int output_num,
+                                uint64_t rng_seed) {
+  if (!ctx || !filename) return fail(APT_ERR_INVALID_ARG, "save_snapshot: null argument");
+  MeshInfo m;
+  apt_status st = mesh_of(ctx, m);
+  if (st != APT_OK) return st;
+  apt_info info;
+  if ((st = apt_get_info(ctx, &info)) != APT_OK) return fail(st, "apt_get_info failed");
+  H5 file;
+  file.id = H5Fcreate(filename, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+  if (file.id < 0) return fail(APT_ERR_INVALID_ARG, "cannot create %s", filename);
+  hsize_t hd[3];
+  const int rank = h5_dims(m.dim, m.dims, hd);
+  std::vector<double> buf(m.ncell);
+  static const char *names[4][3] = {
+      {"Ex", "Ey", "Ez"}, {"Bx", "By", "Bz"}, {"E0x", "E0y", "E0z"}, {"B0x", "B0y", "B0z"}};
+  const int which[4] = {APT_FIELD_E, APT_FIELD_B, APT_FIELD_EBG, APT_FIELD_BBG};
+  for (int q = 0; q < 4; q++)
+    for (int c = 0; c < 3; c++) {
+      if ((st = apt_field_get(ctx, which[q], c, buf.data(), m.ncell)) != APT_OK)
+        return fail(st, "apt_field_get(%d, %d): %s", which[q], c, apt_last_error(ctx));
+      if (!write_array<double>(file.id, names[q][c], rank, hd, buf.data()))
+        return fail(APT_ERR_HIP, "writing %s", names[q][c]);
+    }
+  // particles (particle_data.h:50-61 member names, prefix ptc_)
+  const uint64_t n = info.number;
+  std::vector<double> x1(n), x2(n), x3(n), p1(n), p2(n), p3(n), E(n), w(n);
+  std::vector<uint32_t> cell(n), flag(n);
+  std::vector<uint64_t> id(n);
+  apt_particles pv{cell.data(), x1.data(), x2.data(), x3.data(), p1.data(), p2.data(),
+                   p3.data(),   E.data(),  w.data(),  flag.data(), id.data()};
+  uint64_t got = 0;
+  if (n > 0 && (st = apt_ptc_get(ctx, &pv, n, &got)) != APT_OK)
+    return fail(st, "apt_ptc_get: %s", apt_last_error(ctx));
+  bool ok = write_vec(file.id, "ptc_x1", x1) && write_vec(file.id, "ptc_x2", x2) &&
+            write_vec(file.id, "ptc_x3", x3) && write_vec(file.id, "ptc_p1", p1) &&
+            write_vec(file.id, "ptc_p2", p2) && write_vec(file.id, "ptc_p3", p3) &&
+            write_vec(file.id, "ptc_E", E) && write_vec(file.id, "ptc_weight", w) &&
+            write_vec(file.id, "ptc_cell", cell) && write_vec(file.id, "ptc_id", id) &&
+            write_vec(file.id, "ptc_flag", flag);
+  // photons (photon_data, particle_data.h:68-81, prefix ph_)
+  const uint64_t nph = info.ph_number;
+  std::vector<double> hx1(nph), hx2(nph), hx3(nph), hp1(nph), hp2(nph), hp3(nph), hE(nph),
+      hw(nph), hpath(nph);
+  std::vector<uint32_t> hcell(nph), hflag(nph);
+  std::vector<uint64_t> hid(nph);
+  apt_photons hv{hcell.data(), hx1.data(), hx2.data(), hx3.data(), hp1.data(), hp2.data(),
+                 hp3.data(),   hE.data(),  hw.data(),  hpath.data(), hflag.data(),
+                 hid.data()};
+  if (nph > 0 && (st = apt_ph_get(ctx, &hv, nph, &got)) != APT_OK)
+    return fail(st, "apt_ph_get: %s", apt_last_error(ctx));
+  ok = ok && write_vec(file.id, "ph_x1", hx1) && write_vec(file.id, "ph_x2", hx2) &&
+       write_vec(file.id, "ph_x3", hx3) && write_vec(file.id, "ph_p1", hp1) &&
+       write_vec(file.id, "ph_p2", hp2) && write_vec(file.id, "ph_p3", hp3) &&
+       write_vec(file.id, "ph_E", hE) && write_vec(file.id, "ph_weight", hw) &&
+       write_vec(file.id, "ph_path_left", hpath) && write_vec(file.id, "ph_cell", hcell) &&
+       write_vec(file.id, "ph_id", hid) && write_vec(file.id, "ph_flag", hflag);
+  // counts per rank and the run scalars (data_exporter.cpp:470-480)
+  ok = ok && write_vec(file.id, "ptc_num", std::vector<uint64_t>{n}) &&
+       write_vec(file.id, "ph_num", std::vector<uint64_t>{nph}) &&
+       write_scalar<uint32_t>(file.id, "step", step) &&
+       write_scalar<double>(file.id, "time", time) &&
+       write_scalar<uint64_t>(file.id, "data_interval", (uint64_t)data_interval) &&
+       write_scalar<int>(file.id, "num_ranks", 1) &&
+       write_scalar<int>(file.id, "output_num", output_num);
+  // the counter-based random streams and id counters (replace rand_states)
+  ok = ok && write_scalar<uint64_t>(file.id, "rng_seed", rng_seed) &&
+       write_scalar<uint32_t>(file.id, "rt_epoch", info.rt_epoch) &&
+       write_scalar<uint32_t>(file.id, "inject_epoch", info.inject_epoch) &&
+       write_scalar<uint64_t>(file.id, "ptc_id_next", info.id_next) &&
+       write_scalar<uint64_t>(file.id, "ph_id_next", info.ph_id_next) &&
+       write_scalar<int>(file.id, "jexp", info.H) &&
+       write_scalar<int>(file.id, "surface_exp", info.Hs);
+  if (!ok) return fail(APT_ERR_HIP, "writing particle arrays of %s", filename);
+  return APT_OK;
+}
+
+apt_status apt_io_load_snapshot(apt_ctx *ctx, const char *filename, uint32_t *step,
+                                double *time, int *output_num) {
+  if (!ctx || !filename) return fail(APT_ERR_INVALID_ARG, "load_snapshot: null argument");
+  MeshInfo m;
+  apt_status st = mesh_of(ctx, m);
+  if (st != APT_OK) return st;
+  H5 file;
+  file.id = H5Fopen(filename, H5F_ACC_RDONLY, H5P_DEFAULT);
+  if (file.id < 0) return fail(APT_ERR_INVALID_ARG, "cannot open %s", filename);
+  int num_ranks = 0;
+  if (!read_scalar<int>(file.id, "num_ranks", &num_ranks) || num_ranks != 1)
+    return fail(APT_ERR_INVALID_ARG, "%s: restarting from %d ranks on one", filename,
+                num_ranks);
+  std::vector<double> buf(m.ncell);
+  static const char *names[4][3] = {
+      {"Ex", "Ey", "Ez"}, {"Bx", "By", "Bz"}, {"E0x", "E0y", "E0z"}, {"B0x", "B0y", "B0z"}};
+  const int which[4] = {APT_FIELD_E, APT_FIELD_B, APT_FIELD_EBG, APT_FIELD_BBG};
+  for (int q = 0; q < 4; q++)
+    for (int c = 0; c < 3; c++) {
+      if (!read_array<double>(file.id, names[q][c], buf.data(), m.ncell))
+        return fail(APT_ERR_INVALID_ARG, "%s: %s missing or of another grid", filename,
+                    names[q][c]);
+      if ((st = apt_field_set(ctx, which[q], c, buf.data(), m.ncell)) != APT_OK)
+        return fail(st, "apt_field_set: %s", apt_last_error(ctx));
+    }
+  uint64_t n = 0, nph = 0;
+  if (!read_array<uint64_t>(file.id, "ptc_num", &n, 1) ||
+      !read_array<uint64_t>(file.id, "ph_num", &nph, 1))
+    return fail(APT_ERR_INVALID_ARG, "%s: ptc_num / ph_num missing", filename);
+  // at least one element each, so that an empty set passes valid pointers
+  const uint64_t na = n ? n : 1, npa = nph ? nph : 1;
+  std::vector<double> x1(na), x2(na), x3(na), p1(na), p2(na), p3(na), E(na), w(na);
+  std::vector<uint32_t> cell(na), flag(na);
+  std::vector<uint64_t> id(na);
+  bool ok = read_array(file.id, "ptc_x1", x1.data(), n) &&
+            read_array(file.id, "ptc_x2", x2.data(), n) &&
+            read_array(file.id, "ptc_x3", x3.data(), n) &&
+            read_array(file.id, "ptc_p1", p1.data(), n) &&
+            read_array(file.id, "ptc_p2", p2.data(), n) &&
+            read_array(file.id, "ptc_p3", p3.data(), n) &&
+            read_array(file.id, "ptc_E", E.data(), n) &&
+            read_array(file.id, "ptc_weight", w.data(), n) &&
+            read_array(file.id, "ptc_cell", cell.data(), n) &&
+            read_array(file.id, "ptc_id", id.data(), n) &&
+            read_array(file.id, "ptc_flag", flag.data(), n);
+  if (!ok) return fail(APT_ERR_INVALID_ARG, "%s: particle arrays missing", filename);
+  apt_particles pv{cell.data(), x1.data(), x2.data(), x3.data(), p1.data(), p2.data(),
+                   p3.data(),   E.data(),  w.data(),  flag.data(), id.data()};
+  if ((st = apt_ptc_set(ctx, &pv, n)) != APT_OK)
+    return fail(st, "apt_ptc_set: %s", apt_last_error(ctx));
+  std::vector<double> hx1(npa), hx2(npa), hx3(npa), hp1(npa), hp2(npa), hp3(npa), hE(npa),
+      hw(npa), hpath(npa);
+  std::vector<uint32_t> hcell(npa), hflag(npa);
+  std::vector<uint64_t> hid(npa);
+  ok = read_array(file.id, "ph_x1", hx1.data(), nph) &&
+       read_array(file.id, "ph_x2", hx2.data(), nph) &&
+       read_array(file.id, "ph_x3", hx3.data(), nph) &&
+       read_array(file.id, "ph_p1", hp1.data(), nph) &&
+       read_array(file.id, "ph_p2", hp2.data(), nph) &&
+       read_array(file.id, "ph_p3", hp3.data(), nph) &&
+       read_array(file.id, "ph_E", hE.data(), nph) &&
+       read_array(file.id, "ph_weight", hw.data(), nph) &&
+       read_array(file.id, "ph_path_left", hpath.data(), nph) &&
+       read_array(file.id, "ph_cell", hcell.data(), nph) &&
+       read_array(file.id, "ph_id", hid.data(), nph) &&
+       read_array(file.id, "ph_flag", hflag.data(), nph);
+  if (!ok) return fail(APT_ERR_INVALID_ARG, "%s: photon arrays missing", filename);
+  apt_photons hv{hcell.data(), hx1.data(), hx2.data(), hx3.data(), hp1.data(), hp2.data(),
+                 hp3.data(),   hE.data(),  hw.data(),  hpath.data(), hflag.data(),
+                 hid.data()};
+  apt_info cur{};
+  if ((st = apt_get_info(ctx, &cur)) != APT_OK) return fail(st, "apt_get_info failed");
+  // a context without photon storage takes no photon set
+  if ((nph > 0 || cur.ph_number > 0) && (st = apt_ph_set(ctx, &hv, nph)) != APT_OK)
+    return fail(st, "apt_ph_set: %s", apt_last_error(ctx));
+  apt_info info{};
+  uint32_t s = 0;
+  double t = 0.0;
+  int on = 0;
+  ok = read_scalar<uint32_t>(file.id, "rt_epoch", &info.rt_epoch) &&
+       read_scalar<uint32_t>(file.id, "inject_epoch", &info.inject_epoch) &&
+       read_scalar<uint64_t>(file.id, "ptc_id_next", &info.id_next) &&
+       read_scalar<uint64_t>(file.id, "ph_id_next", &info.ph_id_next) &&
+       read_scalar<int>(file.id, "jexp", &info.H) &&
+       read_scalar<int>(file.id, "surface_exp", &info.Hs) &&
+       read_scalar<uint32_t>(file.id, "step", &s) && read_scalar<double>(file.id, "time", &t) &&
+       read_scalar<int>(file.id, "output_num", &on);
+  if (!ok) return fail(APT_ERR_INVALID_ARG, "%s: run scalars missing", filename);
+  // particle_base::clear_guard_cells after the restore (data_exporter.cpp:580)
+  if ((st = apt_clear_guard_cells(ctx)) != APT_OK)
+    return fail(st, "apt_clear_guard_cells: %s", apt_last_error(ctx));
+  if ((st = apt_set_counters(ctx, &info)) != APT_OK)
+    return fail(st, "apt_set_counters: %s", apt_last_error(ctx));
+  if (step) *step = s;
+  if (time) *time = t;
+  if (output_num) *output_num = on;
+  return APT_OK;
+}
+
+apt_status apt_io_write_grid(const apt_grid *grid, int coord, const char *dir,
+                             int downsample) {
+  if (!grid || !dir || downsample < 1)
+    return fail(APT_ERR_INVALID_ARG, "write_grid: bad arguments");
+  for (int a = 0; a < 3; a++)
+    if (grid->N[a] < 1 || grid->guard[a] < 0 || !(grid->size[a] > 0.0))
+      return fail(APT_ERR_INVALID_ARG, "write_grid: bad grid");
+  MeshInfo m;
+  m.dim = 1;
+  for (int a = 0; a < 3; a++)
+    if (grid->N[a] > 1) m.dim = a + 1;
+  int ext[3];
+  out_extent(grid, m.dim, downsample, ext);
+  const uint64_t n = (uint64_t)ext[0] * ext[1] * ext[2];
+  std::vector<float> x[3];
+  for (int a = 0; a < m.dim; a++) x[a].resize(n);
+  for (int k = 0; k < ext[2]; k++)
+    for (int j = 0; j < ext[1]; j++)
+      for (int i = 0; i < ext[0]; i++) {
+        const uint64_t o = i + (uint64_t)j * ext[0] + (uint64_t)k * ext[0] * ext[1];
+        const double p0 = pos(grid, m.dim, 0, i * downsample + grid->guard[0]);
+        if (m.dim == 1) {
+          x[0][o] = (float)p0;
+        } else if (m.dim == 2 && coord == APT_COORD_LOGSPH) {
+          // exp in FP64 (Scalar), the rest in FP32 (data_exporter.cpp:150-156)
+          const float r = (float)std::exp(p0);
+          const float th = (float)pos(grid, m.dim, 1, j * downsample + grid->guard[1]);
+          x[0][o] = r * std::sin(th);
+          x[1][o] = r * std::cos(th);
+        } else {
+          x[0][o] = (float)p0;
+          x[1][o] = (float)pos(grid, m.dim, 1, j * downsample + grid->guard[1]);
+          if (m.dim == 3)
+            x[2][o] = (float)pos(grid, m.dim, 2, k * downsample + grid->guard[2]);
+        }
+      }
+  const std::string path = std::string(dir) + "/grid.h5";
+  H5 file;
+  file.id = H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+  if (file.id < 0) return fail(APT_ERR_INVALID_ARG, "cannot create %s", path.c_str());
+  hsize_t hd[3];
+  const int rank = h5_dims(m.dim, ext, hd);
+  static const char *names[3] = {"x1", "x2", "x3"};
+  for (int a = 0; a < m.dim; a++)
+    if (!write_array<float>(file.id, names[a], rank, hd, x[a].data()))
+      return fail(APT_ERR_HIP, "writing %s", names[a]);
+  return APT_OK;
+}
+
+apt_status apt_io_write_fields(apt_ctx *ctx, const apt_grid *grid, int num_species,
+                               const char *dir, int output_num, int downsample) {
+  if (!ctx || !grid || !dir || downsample < 1 || output_num < 0 || output_num > 99999)
+    return fail(APT_ERR_INVALID_ARG, "write_fields: bad arguments");
+  MeshInfo m;
+  apt_status st = mesh_of(ctx, m);
+  if (st != APT_OK) return st;
+  int ext[3];
+  out_extent(grid, m.dim, downsample, ext);
+  const uint64_t n = (uint64_t)ext[0] * ext[1] * ext[2];
+  char fname[32];
+  snprintf(fname, sizeof(fname), "/fld.%05d.h5", output_num);
+  const std::string path = std::string(dir) + fname;
+  H5 file;
+  file.id = H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+  if (file.id < 0) return fail(APT_ERR_INVALID_ARG, "cannot create %s", path.c_str());
+  hsize_t hd[3];
+  const int rank = h5_dims(m.dim, ext, hd);
+  std::vector<double> buf(m.ncell);
+  std::vector<float> out(n);
+  struct Item {
+    const char *name;
+    int which, comp;
+  };
+  std::vector<Item> items = {{"E1", APT_FIELD_E, 0},   {"E2", APT_FIELD_E, 1},
+                             {"E3", APT_FIELD_E, 2},   {"B1", APT_FIELD_B, 0},
+                             {"B2", APT_FIELD_B, 1},   {"B3", APT_FIELD_B, 2},
+                             {"J1", APT_FIELD_J, 0},   {"J2", APT_FIELD_J, 1},
+                             {"J3", APT_FIELD_J, 2},   {"Rho_e", APT_FIELD_RHO, 0},
+                             {"Rho_p", APT_FIELD_RHO, 1}};
+  if (num_species > 2) items.push_back({"Rho_i", APT_FIELD_RHO, 2});
+  items.push_back({"divE", APT_FIELD_DIVE, 0});
+  items.push_back({"divB", APT_FIELD_DIVB, 0});
+  for (const Item &it : items) {
+    if ((st = apt_field_get(ctx, it.which, it.comp, buf.data(), m.ncell)) != APT_OK)
+      return fail(st, "apt_field_get %s: %s", it.name, apt_last_error(ctx));
+    if ((st = apt_io_downsample(buf.data(), m.dims, grid->guard, m.dim,
+                                stagger_of(it.which, it.comp), downsample, out.data())) !=
+        APT_OK)
+      return st;
+    if (!write_array<float>(file.id, it.name, rank, hd, out.data()))
+      return fail(APT_ERR_HIP, "writing %s", it.name);
+  }
+  return APT_OK;
+}
+
+apt_status apt_io_write_tracked(apt_ctx *ctx, const apt_grid *grid, int num_species,
+                                const char *dir, int output_num, uint64_t max_tracked,
+                                uint64_t max_tracked_ph) {
+  if (!ctx || !grid || !dir || output_num < 0 || output_num > 99999 || num_species < 1 ||
+      num_species > 3 || max_tracked < 2 || max_tracked_ph < 2)
+    return fail(APT_ERR_INVALID_ARG, "write_tracked: bad arguments");
+  MeshInfo m;
+  apt_status st = mesh_of(ctx, m);
+  if (st != APT_OK) return st;
+  apt_info info;
+  if ((st = apt_get_info(ctx, &info)) != APT_OK) return fail(st, "apt_get_info failed");
+  char fname[32];
+  snprintf(fname, sizeof(fname), "/ptc.%05d.h5", output_num);
+  const std::string path = std::string(dir) + fname;
+  H5 file;
+  file.id = H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+  if (file.id < 0) return fail(APT_ERR_INVALID_ARG, "cannot create %s", path.c_str());
+  // position of a particle: Quadmesh::pos(i, c_i, x_i) (quadmesh.h:143-149)
+  auto position = [&](int a, uint32_t cell, double x) {
+    const int c = a == 0 ? (int)(cell % m.dims[0])
+                         : (a == 1 ? (int)((cell / m.dims[0]) % m.dims[1])
+                                   : (int)(cell / ((uint64_t)m.dims[0] * m.dims[1])));
+    if (a < m.dim) return (float)(grid->lower[a] + m.delta[a] * (c - grid->guard[a] + x));
+    return (float)(x * m.delta[a]);
+  };
+  // the tracked entries in slot order, at most max_tracked - 1 of them
+  // (map_tracked_ptc, kernels.cu:71-84; particle_base_impl.cu:512-513)
+  auto write_group = [&](uint64_t n, const uint32_t *cell, const uint32_t *flag,
+                         const double *const *xs, const double *const *ps, const uint64_t *id,
+                         int sp_of_all, bool photons) -> bool {
+    const uint64_t cap = photons ? max_tracked_ph : max_tracked;
+    std::vector<uint64_t> sel;
+    for (uint64_t i = 0; i < n; i++)
+      if (cell[i] != 0xFFFFFFFFu && (flag[i] & 2u)) sel.push_back(i);
+    if (sel.size() >= cap) sel.resize(cap - 1);
+    const int groups = photons ? 1 : num_species;
+    for (int g = 0; g < groups; g++) {
+      const int sp = photons ? sp_of_all : g;
+      std::vector<uint64_t> oid;
+      std::vector<float> ox[3], op[3];
+      for (uint64_t i : sel) {
+        if (!photons && (int)(flag[i] >> 29) != sp) continue;
+        oid.push_back(id[i]);
+        for (int a = 0; a < 3; a++) {
+          ox[a].push_back(position(a, cell[i], xs[a][i]));
+          op[a].push_back((float)ps[a][i]);
+        }
+      }
+      // particle_type_name (enum_types.h:25-37) of the species index; the
+      // photon group takes index num_species (data_exporter.cpp:705-755)
+      static const char *tname[5] = {"electron", "positron", "ion", "photon", "unknown"};
+      const std::string pre = std::string(tname[sp < 4 ? sp : 4]) + "_";
+      static const char *xn[3] = {"x1", "x2", "x3"}, *pn[3] = {"p1", "p2", "p3"};
+      if (!write_vec(file.id, pre + "id", oid)) return false;
+      for (int a = 0; a < 3; a++)
+        if (!write_vec(file.id, pre + xn[a], ox[a])) return false;
+      for (int a = 0; a < 3; a++)
+        if (!write_vec(file.id, pre + pn[a], op[a])) return false;
+    }
+    return true;
+  };
+  {
+    const uint64_t n = info.number, na = n ? n : 1;
+    std::vector<uint32_t> cell(na), flag(na);
+    std::vector<double> x1(na), x2(na), x3(na), p1(na), p2(na), p3(na);
+    std::vector<uint64_t> id(na);
+    apt_particles pv{cell.data(), x1.data(), x2.data(), x3.data(), p1.data(), p2.data(),
+                     p3.data(),   nullptr,   nullptr,   flag.data(), id.data()};
+    uint64_t got = 0;
+    if (n > 0 && (st = apt_ptc_get(ctx, &pv, n, &got)) != APT_OK)
+      return fail(st, "apt_ptc_get: %s", apt_last_error(ctx));
+    const double *xs[3] = {x1.data(), x2.data(), x3.data()};
+    const double *ps[3] = {p1.data(), p2.data(), p3.data()};
+    if (!write_group(n, cell.data(), flag.data(), xs, ps, id.data(), 0, false))
+      return fail(APT_ERR_HIP, "writing tracked particles to %s", path.c_str());
+  }
+  {
+    const uint64_t n = info.ph_number, na = n ? n : 1;
+    std::vector<uint32_t> cell(na), flag(na);
+    std::vector<double> x1(na), x2(na), x3(na), p1(na), p2(na), p3(na), E(na), w(na),
+        path_left(na);
+    std::vector<uint64_t> id(na);
+    apt_photons hv{cell.data(), x1.data(), x2.data(), x3.data(), p1.data(), p2.data(),
+                   p3.data(),   E.data(),  w.data(),  path_left.data(), flag.data(),
+                   id.data()};
+    uint64_t got = 0;
+    if (n > 0 && (st = apt_ph_get(ctx, &hv, n, &got)) != APT_OK)
+      return fail(st, "apt_ph_get: %s", apt_last_error(ctx));
+    const double *xs[3] = {x1.data(), x2.data(), x3.data()};
+    const double *ps[3] = {p1.data(), p2.data(), p3.data()};
+    if (!write_group(n, cell.data(), flag.data(), xs, ps, id.data(), num_species, true))
+      return fail(APT_ERR_HIP, "writing tracked photons to %s", path.c_str());
+  }
+  return APT_OK;
+}
+
+apt_status apt_io_write_xmf(const char *dir, int dim, const int out_ext[3],
+                            const double *times, const int *output_nums, int n_outputs,
+                            int num_species) {
+  if (!dir || !out_ext || (n_outputs > 0 && !times) || dim < 1 || dim > 3)
+    return fail(APT_ERR_INVALID_ARG, "write_xmf: bad arguments");
+  for (int o = 0; o < n_outputs; o++)
+    if (output_nums && (output_nums[o] < 0 || output_nums[o] > 99999))
+      return fail(APT_ERR_INVALID_ARG, "write_xmf: output number out of range");
+  char dim_str[96];
+  if (dim == 3)
+    snprintf(dim_str, sizeof(dim_str), "%d %d %d", out_ext[2], out_ext[1], out_ext[0]);
+  else if (dim == 2)
+    snprintf(dim_str, sizeof(dim_str), "%d %d", out_ext[1], out_ext[0]);
+  else
+    snprintf(dim_str, sizeof(dim_str), "%d 1", out_ext[0]);
+  std::string s;
+  s += "<?xml version=\"1.0\" ?>\n<!DOCTYPE Xdmf SYSTEM \"Xdmf.dtd\" []>\n<Xdmf>\n";
+  s += "<Domain>\n<Grid Name=\"Aperture\" GridType=\"Collection\" "
+       "CollectionType=\"Temporal\" >\n";
+  std::vector<std::string> fields = {"E1", "E2", "E3", "B1", "B2", "B3", "J1",
+                                     "J2", "J3", "Rho_e", "Rho_p"};
+  if (num_species > 2) fields.push_back("Rho_i");
+  fields.push_back("divE");
+  fields.push_back("divB");
+  const char *topo = dim == 3 ? "3DSMesh" : "2DSMesh";
+  const char *geo = dim == 3 ? "X_Y_Z" : "X_Y";
+  char line[512];
+  for (int o = 0; o < n_outputs; o++) {
+    s += "<Grid Name=\"quadmesh\" Type=\"Uniform\">\n";
+    snprintf(line, sizeof(line), "  <Time Type=\"Single\" Value=\"%g\"/>\n", times[o]);
+    s += line;
+    snprintf(line, sizeof(line), "  <Topology Type=\"%s\" NumberOfElements=\"%s\"/>\n", topo,
+             dim_str);
+    s += line;
+    snprintf(line, sizeof(line), "  <Geometry GeometryType=\"%s\">\n", geo);
+    s += line;
+    for (int a = 0; a < (dim == 1 ? 1 : dim); a++) {
+      snprintf(line, sizeof(line),
+               "    <DataItem Dimensions=\"%s\" NumberType=\"Float\" Precision=\"4\" "
+               "Format=\"HDF\">\n      grid.h5:x%d\n    </DataItem>\n",
+               dim_str, a + 1);
+      s += line;
+    }
+    s += "  </Geometry>\n";
+    for (const std::string &f : fields) {
+      snprintf(line, sizeof(line),
+               "  <Attribute Name=\"%s\" Center=\"Node\" AttributeType=\"Scalar\">\n"
+               "    <DataItem Dimensions=\"%s\" NumberType=\"Float\" Precision=\"4\" "
+               "Format=\"HDF\">\n      fld.%05d.h5:%s\n    </DataItem>\n  </Attribute>\n",
+               f.c_str(), dim_str, output_nums ? output_nums[o] : o, f.c_str());
+      s += line;
+    }
+    s += "</Grid>\n";
+  }
+  s += "</Grid>\n</Domain>\n</Xdmf>\n";
+  const std::string path = std::string(dir) + "/data.xmf";
+  FILE *fp = fopen(path.c_str(), "w");
+  if (!fp) return fail(APT_ERR_INVALID_ARG, "cannot create %s", path.c_str());
+  const bool ok = fwrite(s.data(), 1, s.size(), fp) == s.size();
+  fclose(fp);
+  return ok ? APT_OK : fail(APT_ERR_INVALID_ARG, "writing %s", path.c_str());
+}
+
+}  // extern "C"

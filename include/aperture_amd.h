@@ -185,6 +185,11 @@ const char *apt_last_error(const apt_ctx *ctx);
 apt_status apt_get_jexp(const apt_ctx *ctx, int *H);
 apt_status apt_get_mesh(const apt_ctx *ctx, int dims[3], double delta[3]);
 apt_status apt_get_info(const apt_ctx *ctx, apt_info *out);
+/* Restores id_next, ph_id_next, rt_epoch, inject_epoch and the exponents H
+ * and Hs from `in` (the counter part of a snapshot; the reference stores
+ * its XORWOW states instead, data_exporter.cpp:463-468).  Call after the
+ * particles are set. */
+apt_status apt_set_counters(apt_ctx *ctx, const apt_info *in);
 /* Log-spherical metric coefficient `which` (0..12 in the order l1_e, l2_e,
  * l3_e, l1_b, l2_b, l3_b, A1_e, A2_e, A3_e, A1_b, A2_b, A3_b, dV;
  * Grid_LogSph members, grid_log_sph.h:45-49), dims0*dims1 doubles. */

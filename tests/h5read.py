@@ -1,0 +1,127 @@
+"""Minimal HDF5 reader over the HDF5 C library with ctypes (h5py is not
+installed): lists the datasets of a file and reads one into numpy with its
+stored type and shape.  Used by the output/snapshot tests to check what the
+I/O library wrote independently of the library itself."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HDF5_LIB = next((p for p in ("/opt/conda/lib/libhdf5.so", "/usr/lib/x86_64-linux-gnu/libhdf5.so")
+                 if os.path.exists(p)), None)
+
+_h5 = None
+hid_t = C.c_int64
+hsize_t = C.c_uint64
+
+# H5T_class_t values (H5Tpublic.h)
+H5T_INTEGER, H5T_FLOAT = 0, 1
+H5T_SGN_NONE = 0
+
+
+def lib():
+    global _h5
+    if _h5 is None:
+        if HDF5_LIB is None:
+            raise RuntimeError("libhdf5 not found")
+        h = C.CDLL(HDF5_LIB)
+        h.H5open()
+        h.H5Fopen.restype = hid_t
+        h.H5Fopen.argtypes = [C.c_char_p, C.c_uint, hid_t]
+        h.H5Fclose.argtypes = [hid_t]
+        h.H5Dopen2.restype = hid_t
+        h.H5Dopen2.argtypes = [hid_t, C.c_char_p, hid_t]
+        h.H5Dclose.argtypes = [hid_t]
+        h.H5Dget_space.restype = hid_t
+        h.H5Dget_space.argtypes = [hid_t]
+        h.H5Dget_type.restype = hid_t
+        h.H5Dget_type.argtypes = [hid_t]
+        h.H5Sget_simple_extent_ndims.argtypes = [hid_t]
+        h.H5Sget_simple_extent_dims.argtypes = [hid_t, C.POINTER(hsize_t), C.POINTER(hsize_t)]
+        h.H5Sclose.argtypes = [hid_t]
+        h.H5Tget_class.argtypes = [hid_t]
+        h.H5Tget_size.restype = C.c_size_t
+        h.H5Tget_size.argtypes = [hid_t]
+        h.H5Tget_sign.argtypes = [hid_t]
+        h.H5Tclose.argtypes = [hid_t]
+        h.H5Dread.argtypes = [hid_t, hid_t, hid_t, hid_t, hid_t, C.c_void_p]
+        h.H5Gget_info.argtypes = [hid_t, C.c_void_p]
+        h.H5Lget_name_by_idx.restype = C.c_ssize_t
+        h.H5Lget_name_by_idx.argtypes = [hid_t, C.c_char_p, C.c_int, C.c_int, hsize_t,
+                                         C.c_char_p, C.c_size_t, hid_t]
+        _h5 = h
+    return _h5
+
+
+def _native(name):
+    return hid_t.in_dll(lib(), name).value
+
+
+class _GInfo(C.Structure):
+    _fields_ = [("storage_type", C.c_int), ("nlinks", hsize_t), ("max_corder", C.c_int64),
+                ("mounted", C.c_bool)]
+
+
+class File:
+    def __init__(self, path):
+        self.h = lib()
+        self.id = self.h.H5Fopen(path.encode(), 0, 0)  # H5F_ACC_RDONLY, H5P_DEFAULT
+        if self.id < 0:
+            raise OSError(f"cannot open {path}")
+
+    def close(self):
+        if self.id >= 0:
+            self.h.H5Fclose(self.id)
+            self.id = -1
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def names(self):
+        info = _GInfo()
+        if self.h.H5Gget_info(self.id, C.byref(info)) < 0:
+            raise OSError("H5Gget_info failed")
+        out = []
+        buf = C.create_string_buffer(256)
+        for i in range(info.nlinks):
+            # H5_INDEX_NAME = 0, H5_ITER_INC = 0
+            self.h.H5Lget_name_by_idx(self.id, b".", 0, 0, i, buf, 256, 0)
+            out.append(buf.value.decode())
+        return out
+
+    def read(self, name):
+        h = self.h
+        ds = h.H5Dopen2(self.id, name.encode(), 0)
+        if ds < 0:
+            raise KeyError(name)
+        sp = h.H5Dget_space(ds)
+        nd = h.H5Sget_simple_extent_ndims(sp)
+        dims = (hsize_t * max(nd, 1))()
+        h.H5Sget_simple_extent_dims(sp, dims, None)
+        shape = tuple(dims[i] for i in range(nd))
+        ty = h.H5Dget_type(ds)
+        cls, size, sign = h.H5Tget_class(ty), h.H5Tget_size(ty), h.H5Tget_sign(ty)
+        h.H5Tclose(ty)
+        if cls == H5T_FLOAT:
+            dt, mem = {4: (np.float32, "H5T_NATIVE_FLOAT_g"),
+                       8: (np.float64, "H5T_NATIVE_DOUBLE_g")}[size]
+        elif cls == H5T_INTEGER:
+            key = (size, sign == H5T_SGN_NONE)
+            dt, mem = {(4, True): (np.uint32, "H5T_NATIVE_UINT32_g"),
+                       (4, False): (np.int32, "H5T_NATIVE_INT32_g"),
+                       (8, True): (np.uint64, "H5T_NATIVE_UINT64_g"),
+                       (8, False): (np.int64, "H5T_NATIVE_INT64_g")}[key]
+        else:
+            raise TypeError(f"{name}: type class {cls}")
+        out = np.empty(shape, dt)
+        if out.size:
+            if h.H5Dread(ds, _native(mem), 0, 0, 0, out.ctypes.data) < 0:
+                raise OSError(f"reading {name}")
+        h.H5Sclose(sp)
+        h.H5Dclose(ds)
+        return out

@@ -1,0 +1,200 @@
+"""Snapshot/restart and HDF5 output of a GPU context in the reference's
+data_exporter formats (src/utils/data_exporter.cpp:399-593 snapshots,
+:639-758 field and tracked-particle output, user_data_output.hpp:11-48).
+
+Bar: a run restarted from a snapshot continues bit-exactly like the run
+that wrote it (FP64 form, log-spherical pulsar loop with injection and
+pair creation, so the random-stream epochs and id counters are part of
+the restored state); the files hold the reference's dataset names, types
+and shapes, and their values are the context's own arrays (snapshot) or
+their downsampled node values (field output)."""
+import os
+
+import numpy as np
+import pytest
+
+import aperture3_amd as apt
+import h5read
+from aperture3_amd.exporter import IO_LIB_PATH
+from logsph_cases import LogSphCase
+
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skipif(h5read.HDF5_LIB is None or not os.path.exists(IO_LIB_PATH),
+                       reason="HDF5 library or libaperture_io.so not available"),
+]
+
+PTC_KEYS = ("cell", "x1", "x2", "x3", "p1", "p2", "p3", "E", "w", "flag", "id")
+PH_KEYS = ("cell", "x1", "x2", "x3", "p1", "p2", "p3", "E", "w", "path", "flag", "id")
+RT = dict(rt_model=1, gamma_thr=3.0, photon_path=0.05, r_cutoff=25.0, e_frac=0.2,
+          max_photon_number=40000)
+
+
+def pulsar_steps(env, d, c, steps, omega=0.1):
+    """The labs/pulsar/src/main.cpp:91-125 loop body without output."""
+    pu, fs = apt.ptc_updater_logsph(env), apt.field_solver_logsph(env)
+    rad = apt.radiative_transfer(env)
+    for step in steps:
+        pu.inject_ptc(d, 1, 0.01, 0.0, 0.0, 1.0, omega)
+        pu.update_particles(d, c.dt, step)
+        fs.update_fields(d, c.dt, step * c.dt)
+        fs.apply_boundary(d, omega, step * c.dt)
+        rad.emit_photons(d)
+        rad.produce_pairs(d)
+        if step % 3 == 0 and step != 0:
+            d.sort_particles()
+
+
+def state(d):
+    ptc, ph = d.get_particles(), d.get_photons()
+    fields = {(w, k): d.get_field(w, k) for w in (apt.FIELD_E, apt.FIELD_B, apt.FIELD_J)
+              for k in range(3)}
+    return ptc, ph, fields, d.info()
+
+
+def test_logsph_restart_bit_exact(tmp_path):
+    print("restart: log-spherical 48x48, FP64, injection + pair creation", flush=True)
+    c = LogSphCase(nptc=4000, N=(48, 48), cap=60000)
+    env1, d1 = c.gpu(precision=apt.APT_PREC_FP64, data_dir=str(tmp_path), **RT)
+    pulsar_steps(env1, d1, c, range(0, 5))
+    # the restore erases particles in guard cells (particle_base::
+    # clear_guard_cells, data_exporter.cpp:580); do the same here so both
+    # runs continue from one state
+    d1.clear_guard_cells()
+    ex1 = apt.data_exporter(env1)
+    snap = str(tmp_path / "snapshot.h5")
+    ex1.save_snapshot(snap, d1, 4, 4 * c.dt)
+    saved = d1.info()
+    assert saved["ph_number"] > 0 and saved["rt_epoch"] == 5 and saved["inject_epoch"] == 5
+    pulsar_steps(env1, d1, c, range(5, 10))
+    ref = state(d1)
+    d1.close()
+
+    env2, d2 = c.gpu(precision=apt.APT_PREC_FP64, data_dir=str(tmp_path), **RT)
+    ex2 = apt.data_exporter(env2)
+    step, time = ex2.load_snapshot(snap, d2)
+    assert step == 4 and time == 4 * c.dt and ex2.m_output_num == 0
+    assert d2.info() == saved
+    pulsar_steps(env2, d2, c, range(5, 10))
+    got = state(d2)
+    d2.close()
+    for k in PTC_KEYS:
+        assert np.array_equal(got[0][k], ref[0][k]), ("particle", k)
+    for k in PH_KEYS:
+        assert np.array_equal(got[1][k], ref[1][k]), ("photon", k)
+    for key in ref[2]:
+        assert np.array_equal(got[2][key], ref[2][key]), ("field", key)
+    assert got[3] == ref[3]
+
+
+def test_snapshot_datasets(tmp_path):
+    """Names, types, shapes and values of the snapshot datasets."""
+    c = LogSphCase(nptc=2000, N=(32, 24), cap=20000)
+    env, d = c.gpu(precision=apt.APT_PREC_FP64, data_dir=str(tmp_path), **RT)
+    pulsar_steps(env, d, c, range(0, 3))
+    ex = apt.data_exporter(env)
+    snap = str(tmp_path / "snap.h5")
+    ex.save_snapshot(snap, d, 2, 0.25)
+    ptc, ph, _, info = state(d)
+    with h5read.File(snap) as f:
+        names = set(f.names())
+        fields = {"Ex": (apt.FIELD_E, 0), "Ey": (apt.FIELD_E, 1), "Ez": (apt.FIELD_E, 2),
+                  "Bx": (apt.FIELD_B, 0), "By": (apt.FIELD_B, 1), "Bz": (apt.FIELD_B, 2),
+                  "E0x": (apt.FIELD_EBG, 0), "E0y": (apt.FIELD_EBG, 1),
+                  "E0z": (apt.FIELD_EBG, 2), "B0x": (apt.FIELD_BBG, 0),
+                  "B0y": (apt.FIELD_BBG, 1), "B0z": (apt.FIELD_BBG, 2)}
+        ptc_names = {"ptc_" + k for k in ("x1", "x2", "x3", "p1", "p2", "p3", "E", "weight",
+                                          "cell", "id", "flag")}
+        ph_names = {"ph_" + k for k in ("x1", "x2", "x3", "p1", "p2", "p3", "E", "weight",
+                                        "path_left", "cell", "id", "flag")}
+        scalars = {"ptc_num", "ph_num", "step", "time", "data_interval", "num_ranks",
+                   "output_num"}
+        ours = {"rng_seed", "rt_epoch", "inject_epoch", "ptc_id_next", "ph_id_next", "jexp",
+                "surface_exp"}
+        assert names == set(fields) | ptc_names | ph_names | scalars | ours
+        dims = env.dims
+        for name, (w, k) in fields.items():
+            a = f.read(name)
+            assert a.dtype == np.float64 and a.shape == (dims[1], dims[0]), name
+            assert np.array_equal(a.ravel(), d.get_field(w, k)), name
+        assert f.read("ptc_num").tolist() == [info["number"]]
+        assert f.read("ph_num").tolist() == [info["ph_number"]]
+        assert f.read("step")[()] == 2 and f.read("step").dtype == np.uint32
+        assert f.read("time")[()] == 0.25
+        assert f.read("num_ranks")[()] == 1 and f.read("output_num")[()] == 0
+        assert f.read("data_interval")[()] == c.data_interval
+        assert f.read("rt_epoch")[()] == 3 and f.read("jexp")[()] == info["H"]
+        ren = {"w": "weight"}
+        for k in PTC_KEYS:
+            a = f.read("ptc_" + ren.get(k, k))
+            assert a.dtype == ptc[k].dtype and np.array_equal(a, ptc[k]), k
+        for k in PH_KEYS:
+            a = f.read("ph_" + {"w": "weight", "path": "path_left"}.get(k, k))
+            assert a.dtype == ph[k].dtype and np.array_equal(a, ph[k]), ("photon", k)
+    d.close()
+
+
+def node_average(f, dims, guard, st, d):
+    """Node value of the reference's downsample (interpolate2d with output
+    stagger 0b111, interpolation.cuh:238-251) written with numpy slicing:
+    along an axis whose stagger bit is 0 the average of the cells -2 and -1
+    from the sampled node, else the node itself."""
+    a = f.reshape(dims[1], dims[0])
+    out_n = [(dims[i] - 2 * guard[i]) // d for i in range(2)]
+    xs = guard[0] + d * np.arange(out_n[0])
+    ys = guard[1] + d * np.arange(out_n[1])
+
+    def along(idx, bit):
+        return [(idx, 1.0)] if bit else [(idx - 2, 0.5), (idx - 1, 0.5)]
+    out = np.zeros((out_n[1], out_n[0]))
+    for yy, wy in along(ys, (st >> 1) & 1):
+        for xx, wx in along(xs, st & 1):
+            out += wy * wx * a[np.ix_(yy, xx)]
+    return out
+
+
+@pytest.mark.parametrize("ds", [1, 2])
+def test_field_and_tracked_output(tmp_path, ds):
+    c = LogSphCase(nptc=3000, N=(32, 24), cap=20000)
+    env, d = c.gpu(precision=apt.APT_PREC_FP64, data_dir=str(tmp_path), downsample=ds, **RT)
+    # mark every 7th particle tracked (ParticleFlag::tracked = bit 1)
+    ptc = d.get_particles()
+    ptc["flag"] = ptc["flag"] | np.where(np.arange(len(ptc["flag"])) % 7 == 0, 2, 0).astype(
+        np.uint32)
+    d.set_particles(ptc)
+    pulsar_steps(env, d, c, range(0, 3))
+    ex = apt.data_exporter(env)
+    ex.write_output(d, 3, 3 * c.dt)
+    ex.write_output(d, 6, 6 * c.dt)
+    files = set(os.listdir(tmp_path))
+    assert {"grid.h5", "fld.00000.h5", "fld.00001.h5", "ptc.00000.h5", "ptc.00001.h5",
+            "data.xmf"} <= files
+    dims, guard = env.dims, env.params().guard
+    with h5read.File(str(tmp_path / "fld.00001.h5")) as f:
+        assert set(f.names()) == {"E1", "E2", "E3", "B1", "B2", "B3", "J1", "J2", "J3",
+                                  "Rho_e", "Rho_p", "Rho_i", "divE", "divB"}
+        for name, w, k in (("E1", apt.FIELD_E, 0), ("E2", apt.FIELD_E, 1),
+                           ("B1", apt.FIELD_B, 0), ("B3", apt.FIELD_B, 2),
+                           ("J2", apt.FIELD_J, 1), ("Rho_p", apt.FIELD_RHO, 1)):
+            a = f.read(name)
+            st = apt.STAGGER.get(w, (0b111,) * 3)[k]
+            exp = node_average(d.get_field(w, k), dims, guard, st, ds)
+            assert a.dtype == np.float32 and a.shape == exp.shape, name
+            np.testing.assert_array_equal(a, exp.astype(np.float32), err_msg=name)
+    # tracked particles: global positions, grouped by species
+    ptc = d.get_particles()
+    live = (ptc["cell"] != apt.MAX_CELL) & ((ptc["flag"] & 2) != 0)
+    sp = ptc["flag"] >> 29
+    delta = env.delta
+    with h5read.File(str(tmp_path / "ptc.00001.h5")) as f:
+        for s, name in enumerate(("electron", "positron", "ion")):
+            sel = live & (sp == s)
+            assert np.array_equal(f.read(f"{name}_id"), ptc["id"][sel]), name
+            c1 = ptc["cell"][sel] % dims[0]
+            x1 = (env.params().lower[0] + delta[0] * (c1 - guard[0] + ptc["x1"][sel]))
+            np.testing.assert_array_equal(f.read(f"{name}_x1"), x1.astype(np.float32))
+            np.testing.assert_array_equal(f.read(f"{name}_p3"),
+                                          ptc["p3"][sel].astype(np.float32))
+        # photons sit under the type name of index num_species (3: photon)
+        assert len(f.read("photon_id")) == 0
+    d.close()
