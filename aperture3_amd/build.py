@@ -138,6 +138,12 @@ def build_io() -> str | None:
           f"-L{HERE}", "-laperture_amd", f"-L{libdir}", "-lhdf5",
           "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{libdir}"])
     os.replace(tmp, IO_LIB)
+    # lab-style driver on the C++ host mirror and data_exporter.hpp
+    drv = os.path.join(HERE, "pulsar_driver")
+    _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", f"-I{os.path.join(CSRC, 'host')}",
+          os.path.join(CSRC, "host", "pulsar_driver.cpp"), "-o", drv + ".tmp",
+          f"-L{HERE}", "-laperture_io", "-laperture_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(drv + ".tmp", drv)
     return IO_LIB
 
 

@@ -54,6 +54,13 @@ struct SimParams {
   Scalar photon_path = 1.0;
   Scalar E_secondary = 4.0;
   Scalar r_cutoff = 4.0;
+  // output and restart (sim_params.h:103-120)
+  std::string data_dir = "Data/";
+  std::string conf_file = "config.toml";
+  int downsample = 1;
+  int snapshot_interval = 1000;
+  bool is_restart = false;
+  std::string restart_file = "";
   // not in the reference: numeric form, mixed-form tile, pair-creation
   // model (APT_RT_DEFAULT: the default hooks; APT_RT_EXP_PATH: exponential
   // free path, photon energy e_frac gamma)

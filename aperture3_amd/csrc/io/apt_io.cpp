@@ -302,12 +302,16 @@ apt_status apt_io_save_snapshot(apt_ctx *ctx, const char *filename, uint32_t ste
   return APT_OK;
 }
 
-apt_status apt_io_load_snapshot(apt_ctx *ctx, const char *filename, uint32_t *step,
-                                double *time, int *output_num) {
-  if (!ctx || !filename) return fail(APT_ERR_INVALID_ARG, "load_snapshot: null argument");
+apt_status apt_io_load_snapshot(apt_ctx *ctx, const apt_grid *grid, const char *filename,
+                                uint32_t *step, double *time, int *output_num) {
+  if (!ctx || !grid || !filename)
+    return fail(APT_ERR_INVALID_ARG, "load_snapshot: null argument");
   MeshInfo m;
   apt_status st = mesh_of(ctx, m);
   if (st != APT_OK) return st;
+  for (int a = 0; a < 3; a++)
+    if (grid->N[a] + 2 * grid->guard[a] != m.dims[a])
+      return fail(APT_ERR_INVALID_ARG, "load_snapshot: grid does not match the context");
   H5 file;
   file.id = H5Fopen(filename, H5F_ACC_RDONLY, H5P_DEFAULT);
   if (file.id < 0) return fail(APT_ERR_INVALID_ARG, "cannot open %s", filename);
@@ -369,6 +373,19 @@ apt_status apt_io_load_snapshot(apt_ctx *ctx, const char *filename, uint32_t *st
        read_array(file.id, "ph_id", hid.data(), nph) &&
        read_array(file.id, "ph_flag", hflag.data(), nph);
   if (!ok) return fail(APT_ERR_INVALID_ARG, "%s: photon arrays missing", filename);
+  // photons in guard cells are erased like the particles
+  // (particle_base::clear_guard_cells, data_exporter.cpp:582-583)
+  for (uint64_t i = 0; i < nph; i++) {
+    const uint32_t c = hcell[i];
+    if (c == 0xFFFFFFFFu) continue;
+    const int64_t cc[3] = {c % m.dims[0], (c / m.dims[0]) % m.dims[1],
+                           c / ((uint64_t)m.dims[0] * m.dims[1])};
+    for (int a = 0; a < m.dim; a++)
+      if (cc[a] < grid->guard[a] || cc[a] >= m.dims[a] - grid->guard[a]) {
+        hcell[i] = 0xFFFFFFFFu;
+        break;
+      }
+  }
   apt_photons hv{hcell.data(), hx1.data(), hx2.data(), hx3.data(), hp1.data(), hp2.data(),
                  hp3.data(),   hE.data(),  hw.data(),  hpath.data(), hflag.data(),
                  hid.data()};
@@ -390,7 +407,7 @@ apt_status apt_io_load_snapshot(apt_ctx *ctx, const char *filename, uint32_t *st
        read_scalar<uint32_t>(file.id, "step", &s) && read_scalar<double>(file.id, "time", &t) &&
        read_scalar<int>(file.id, "output_num", &on);
   if (!ok) return fail(APT_ERR_INVALID_ARG, "%s: run scalars missing", filename);
-  // particle_base::clear_guard_cells after the restore (data_exporter.cpp:580)
+  // particle_base::clear_guard_cells after the restore (data_exporter.cpp:582)
   if ((st = apt_clear_guard_cells(ctx)) != APT_OK)
     return fail(st, "apt_clear_guard_cells: %s", apt_last_error(ctx));
   if ((st = apt_set_counters(ctx, &info)) != APT_OK)

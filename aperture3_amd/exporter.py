@@ -29,8 +29,8 @@ IO_SIGNATURES = {
     "apt_io_last_error": [],
     "apt_io_save_snapshot": [_P, C.c_char_p, C.c_uint32, C.c_double, C.c_int, C.c_int,
                              C.c_uint64],
-    "apt_io_load_snapshot": [_P, C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double),
-                             C.POINTER(C.c_int)],
+    "apt_io_load_snapshot": [_P, C.POINTER(AptGrid), C.c_char_p, C.POINTER(C.c_uint32),
+                             C.POINTER(C.c_double), C.POINTER(C.c_int)],
     "apt_io_write_grid": [C.POINTER(AptGrid), C.c_int, C.c_char_p, C.c_int],
     "apt_io_write_fields": [_P, C.POINTER(AptGrid), C.c_int, C.c_char_p, C.c_int, C.c_int],
     "apt_io_write_tracked": [_P, C.POINTER(AptGrid), C.c_int, C.c_char_p, C.c_int,
@@ -184,7 +184,8 @@ class data_exporter:
         time = C.c_double()
         num = C.c_int()
         _check(self.lib, self.lib.apt_io_load_snapshot(
-            data.ctx, filename.encode(), C.byref(step), C.byref(time), C.byref(num)),
+            data.ctx, C.byref(self._grid), filename.encode(), C.byref(step), C.byref(time),
+            C.byref(num)),
             "apt_io_load_snapshot")
         self.m_output_num = num.value
         self.prepare_xmf_restart(step.value, time.value)

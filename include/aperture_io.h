@@ -35,10 +35,11 @@ apt_status apt_io_save_snapshot(apt_ctx *ctx, const char *filename, uint32_t ste
                                 uint64_t rng_seed);
 
 /* data_exporter::load_snapshot (:490-593): restores fields, particles,
- * photons and the random-stream counters; returns step, time and
- * output_num.  The grid must match the file. */
-apt_status apt_io_load_snapshot(apt_ctx *ctx, const char *filename, uint32_t *step,
-                                double *time, int *output_num);
+ * photons and the random-stream counters, erases particles and photons in
+ * guard cells (:582-583); returns step, time and output_num.  `grid` is
+ * the context's grid and must match the file. */
+apt_status apt_io_load_snapshot(apt_ctx *ctx, const apt_grid *grid, const char *filename,
+                                uint32_t *step, double *time, int *output_num);
 
 /* data_exporter::write_grid (:120-216): grid.h5 with x1 (, x2, x3) at the
  * downsampled nodes, FP32; log-spherical grids give x1 = r sin(theta),

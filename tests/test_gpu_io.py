@@ -58,7 +58,7 @@ def test_logsph_restart_bit_exact(tmp_path):
     env1, d1 = c.gpu(precision=apt.APT_PREC_FP64, data_dir=str(tmp_path), **RT)
     pulsar_steps(env1, d1, c, range(0, 5))
     # the restore erases particles in guard cells (particle_base::
-    # clear_guard_cells, data_exporter.cpp:580); do the same here so both
+    # clear_guard_cells, data_exporter.cpp:582); do the same here so both
     # runs continue from one state
     d1.clear_guard_cells()
     ex1 = apt.data_exporter(env1)
@@ -198,3 +198,33 @@ def test_field_and_tracked_output(tmp_path, ds):
         # photons sit under the type name of index num_species (3: photon)
         assert len(f.read("photon_id")) == 0
     d.close()
+
+
+def test_cpp_pulsar_driver_restart(tmp_path):
+    """The C++ lab (pulsar_driver: the labs/pulsar/src/main.cpp loop over the
+    host classes and data_exporter.hpp) run to step 6, restarted from its
+    snapshot to step 12 in the same directory, against an uninterrupted run
+    to step 12: same final state, same data.xmf, same field and tracked
+    output files."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(apt.LIB_PATH), "pulsar_driver")
+    a, c = tmp_path / "a", tmp_path / "c"
+
+    def run(*args):
+        print("pulsar_driver", *args, flush=True)
+        r = subprocess.run([exe, *map(str, args)], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+    run(a, 6)
+    run(a, 12, a / "snapshot.h5")
+    run(c, 12)
+    assert open(a / "data.xmf").read() == open(c / "data.xmf").read()
+    for name in ("final.h5", "snapshot.h5", "fld.00002.h5", "fld.00003.h5", "ptc.00003.h5",
+                 "grid.h5"):
+        with h5read.File(str(a / name)) as fa, h5read.File(str(c / name)) as fc:
+            names = fa.names()
+            assert names == fc.names(), name
+            for n in names:
+                va, vc = fa.read(n), fc.read(n)
+                assert va.dtype == vc.dtype and np.array_equal(va, vc), (name, n)
+    with h5read.File(str(c / "final.h5")) as f:
+        assert f.read("ph_num")[0] > 0 and f.read("output_num")[()] == 4
