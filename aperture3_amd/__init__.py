@@ -23,6 +23,7 @@ APT_OK = 0
 APT_PREC_FP64 = 0
 APT_PREC_MIXED = 1
 FIELD_E, FIELD_B, FIELD_EBG, FIELD_BBG, FIELD_J, FIELD_RHO, FIELD_DIVE, FIELD_DIVB = range(8)
+FIELD_EDOTB, FIELD_PHOTON_PRODUCED, FIELD_PAIR_PRODUCED = 8, 9, 10
 KERNEL_PUSH_DEPOSIT, KERNEL_FIELD, KERNEL_SORT = range(3)
 MAX_CELL = 0xFFFFFFFF
 FLAG_IGNORE_CURRENT = 1 << 3

@@ -36,6 +36,8 @@ struct apt_ctx {
   // fields (double* in the FP64 form, float* in the mixed form)
   void *E[3] = {}, *B[3] = {}, *Ebg[3] = {}, *Bbg[3] = {}, *J[3] = {};
   void *Rho[4] = {}, *divE = nullptr, *divB = nullptr, *jtmp = nullptr;
+  // sim_data::EdotB, photon_produced, pair_produced (sim_data.h:57-59)
+  void *EdotB = nullptr, *ph_produced = nullptr, *pair_produced = nullptr;
   long long *Jacc[3][2] = {};
   long long *Racc[4][2] = {};
 
@@ -353,7 +355,8 @@ apt_status apt_create(const apt_grid *grid, const apt_params *params,
   // secondaries never index a missing array, whatever num_species says
   for (int s = 0; s < 4 && ok; s++) ok = alloc_field(&ctx->Rho[s]);
   ok = ok && alloc_field(&ctx->divE) && alloc_field(&ctx->divB) &&
-       alloc_field(&ctx->jtmp);
+       alloc_field(&ctx->jtmp) && alloc_field(&ctx->EdotB) &&
+       alloc_field(&ctx->ph_produced) && alloc_field(&ctx->pair_produced);
   int limbs = ctx->prec == APT_PREC_FP64 ? 2 : 1;
   for (int c = 0; c < 3 && ok; c++)
     for (int l = 0; l < limbs && ok; l++)
@@ -473,6 +476,8 @@ apt_status apt_destroy(apt_ctx *ctx) {
     (void)hipFree(ctx->Racc[s][1]);
   }
   (void)hipFree(ctx->divE); (void)hipFree(ctx->divB); (void)hipFree(ctx->jtmp);
+  (void)hipFree(ctx->EdotB); (void)hipFree(ctx->ph_produced);
+  (void)hipFree(ctx->pair_produced);
   (void)hipFree(ctx->d_counters);
   for (int k = 0; k < LS_NCOEF; k++) (void)hipFree(ctx->coef_mem[k]);
   for (int k = 0; k < 12; k++) (void)hipFree(ctx->ffe[k]);
@@ -574,6 +579,9 @@ static void **field_slot(apt_ctx *ctx, int which, int comp) {
       return comp < ctx->params.num_species ? &ctx->Rho[comp] : nullptr;
     case APT_FIELD_DIVE: return comp == 0 ? &ctx->divE : nullptr;
     case APT_FIELD_DIVB: return comp == 0 ? &ctx->divB : nullptr;
+    case APT_FIELD_EDOTB: return comp == 0 ? &ctx->EdotB : nullptr;
+    case APT_FIELD_PHOTON_PRODUCED: return comp == 0 ? &ctx->ph_produced : nullptr;
+    case APT_FIELD_PAIR_PRODUCED: return comp == 0 ? &ctx->pair_produced : nullptr;
   }
   return nullptr;
 }
@@ -1148,6 +1156,7 @@ static FieldSet<T> field_set(apt_ctx *ctx) {
   }
   f.divE = (T *)ctx->divE;
   f.divB = (T *)ctx->divB;
+  f.EdotB = (T *)ctx->EdotB;
   return f;
 }
 
@@ -1189,8 +1198,12 @@ static apt_status field_update_t(apt_ctx *ctx, double dt, uint32_t step) {
 template <typename T>
 static apt_status logsph_field_update_t(apt_ctx *ctx, double dt) {
   time_begin(ctx, APT_KERNEL_FIELD);
+  // compute_edotb's delta is the float 1.0f / data_interval
+  // (cuda/sim_data.cu:60)
+  const int di = ctx->params.data_interval > 0 ? ctx->params.data_interval : 1;
+  const double delta = (double)(1.0f / (float)di);
   HIPCHK(launch_logsph_field_update<T>(ctx->mesh, ctx->coef, ctx->lp,
-                                       field_set<T>(ctx), dt, ctx->stream));
+                                       field_set<T>(ctx), dt, delta, ctx->stream));
   time_end(ctx, APT_KERNEL_FIELD);
   return APT_OK;
 }
@@ -1360,12 +1373,12 @@ apt_status apt_emit_photons(apt_ctx *ctx, uint64_t *emitted) {
     HIPCHK(rt_emit<double>(ctx->mesh, ctx->rt, view_of(ctx->p64), ctx->number,
                            view_of(ctx->ph64), ctx->ph_number, seed,
                            ctx->rt_epoch, ctx->ph_id_next, ctx->rtw,
-                           ctx->d_counters, ctx->stream, &k));
+                           ctx->d_counters, ctx->stream, (double *)ctx->ph_produced, &k));
   else
     HIPCHK(rt_emit<float>(ctx->mesh, ctx->rt, view_of(ctx->p32), ctx->number,
                           view_of(ctx->ph32), ctx->ph_number, seed,
                           ctx->rt_epoch, ctx->ph_id_next, ctx->rtw,
-                          ctx->d_counters, ctx->stream, &k));
+                          ctx->d_counters, ctx->stream, (float *)ctx->ph_produced, &k));
   ctx->rt_epoch++;
   ctx->ph_number += k;
   ctx->ph_id_next += k;
@@ -1381,11 +1394,13 @@ apt_status apt_produce_pairs(apt_ctx *ctx, uint64_t *pairs) {
   if (ctx->prec == APT_PREC_FP64)
     HIPCHK(rt_pairs<double>(ctx->mesh, ctx->rt, view_of(ctx->p64), ctx->number,
                             view_of(ctx->ph64), ctx->ph_number, ctx->id_next,
-                            ctx->rtw, ctx->d_counters, ctx->stream, &k));
+                            ctx->rtw, ctx->d_counters, ctx->stream,
+                            (double *)ctx->pair_produced, &k));
   else
     HIPCHK(rt_pairs<float>(ctx->mesh, ctx->rt, view_of(ctx->p32), ctx->number,
                            view_of(ctx->ph32), ctx->ph_number, ctx->id_next,
-                           ctx->rtw, ctx->d_counters, ctx->stream, &k));
+                           ctx->rtw, ctx->d_counters, ctx->stream,
+                           (float *)ctx->pair_produced, &k));
   if (k > 0) {
     // secondaries carry weights from the photons (<= any parent weight),
     // so the fixed-point scale stays valid

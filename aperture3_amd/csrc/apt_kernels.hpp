@@ -117,6 +117,7 @@ struct FieldSet {
   const T *Ebg[3], *Bbg[3];
   const T *J[3];
   T *divE, *divB;
+  T *EdotB = nullptr;  // time-averaged E.B/|B| at cell centres (log-spherical)
 };
 
 template <typename T>
@@ -183,7 +184,7 @@ template <typename T>
 hipError_t launch_logsph_field_update(const Mesh &m, const LsCoef &cf,
                                       const LsParams &lp,
                                       const FieldSet<T> &f, double dt,
-                                      hipStream_t st);
+                                      double edotb_delta, hipStream_t st);
 template <typename T>
 hipError_t launch_logsph_field_boundary(const Mesh &m, const LsParams &lp,
                                         const FieldSet<T> &f, double omega,
@@ -273,12 +274,12 @@ hipError_t rt_emit(const Mesh &m, const RtParams &rt, const PtcView<R> &p,
                    uint64_t n, const PhoView<R> &ph, uint64_t ph_base,
                    uint64_t seed, uint32_t epoch, uint64_t id_base,
                    RtWork &w, unsigned long long *counters, hipStream_t st,
-                   uint64_t *emitted);
+                   R *produced, uint64_t *emitted);
 template <typename R>
 hipError_t rt_pairs(const Mesh &m, const RtParams &rt, const PtcView<R> &p,
                     uint64_t p_base, const PhoView<R> &ph, uint64_t nph,
                     uint64_t id_base, RtWork &w, unsigned long long *counters,
-                    hipStream_t st, uint64_t *made);
+                    hipStream_t st, R *produced, uint64_t *made);
 template <typename R>
 hipError_t rt_move_photons(const Mesh &m, int logsph, double r_cutoff,
                            const PhoView<R> &ph, uint64_t n, double dt,

@@ -68,7 +68,10 @@ void field_ref::initialize(
 }
 
 void field_ref::initialize() {
-  for (int c = 0; c < 3; c++)
+  // vector fields have three components, the scalar fields one
+  const bool scalar = m_which == APT_FIELD_DIVE || m_which == APT_FIELD_DIVB ||
+                      m_which >= APT_FIELD_EDOTB;
+  for (int c = 0; c < (scalar ? 1 : 3); c++)
     initialize(c, [](Scalar, Scalar, Scalar) { return 0.0; });
 }
 
@@ -97,7 +100,10 @@ sim_data::sim_data(sim_environment &e)
       Bbg(this, APT_FIELD_BBG),
       J(this, APT_FIELD_J),
       divE(this, APT_FIELD_DIVE),
-      divB(this, APT_FIELD_DIVB) {
+      divB(this, APT_FIELD_DIVB),
+      EdotB(this, APT_FIELD_EDOTB),
+      photon_produced(this, APT_FIELD_PHOTON_PRODUCED),
+      pair_produced(this, APT_FIELD_PAIR_PRODUCED) {
   const SimParams &p = env.params();
   apt_grid g{};
   apt_params q{};

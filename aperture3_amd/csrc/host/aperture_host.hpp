@@ -126,6 +126,8 @@ struct sim_data {
   sim_environment &env;
   apt_ctx *ctx = nullptr;
   field_ref E, B, Ebg, Bbg, J, divE, divB;
+  // sim_data.h:57-59; zero them after output with initialize()
+  field_ref EdotB, photon_produced, pair_produced;
 };
 
 class ptc_updater {

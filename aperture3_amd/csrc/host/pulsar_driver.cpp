@@ -84,7 +84,12 @@ int main(int argc, char **argv) {
       // time from the step number alone, so that a restarted run takes
       // the same times as an uninterrupted one
       const double time = step * dt;
-      if (step % p.data_interval == 0) exporter.write_output(data, step, time);
+      if (step % p.data_interval == 0) {
+        exporter.write_output(data, step, time);
+        data.photon_produced.initialize();
+        data.pair_produced.initialize();
+        data.EdotB.initialize();
+      }
       ptc_updater.inject_ptc(data, 1, 0.01, 0.0, 0.0, 1.0, p.omega);
       ptc_updater.update_particles(data, dt, step);
       field_solver.update_fields(data, dt, time);

@@ -500,6 +500,8 @@ apt_status apt_io_write_fields(apt_ctx *ctx, const apt_grid *grid, int num_speci
   if (num_species > 2) items.push_back({"Rho_i", APT_FIELD_RHO, 2});
   items.push_back({"divE", APT_FIELD_DIVE, 0});
   items.push_back({"divB", APT_FIELD_DIVB, 0});
+  items.push_back({"photon_produced", APT_FIELD_PHOTON_PRODUCED, 0});
+  items.push_back({"pair_produced", APT_FIELD_PAIR_PRODUCED, 0});
   for (const Item &it : items) {
     if ((st = apt_field_get(ctx, it.which, it.comp, buf.data(), m.ncell)) != APT_OK)
       return fail(st, "apt_field_get %s: %s", it.name, apt_last_error(ctx));
@@ -633,6 +635,8 @@ apt_status apt_io_write_xmf(const char *dir, int dim, const int out_ext[3],
   if (num_species > 2) fields.push_back("Rho_i");
   fields.push_back("divE");
   fields.push_back("divB");
+  fields.push_back("photon_produced");
+  fields.push_back("pair_produced");
   const char *topo = dim == 3 ? "3DSMesh" : "2DSMesh";
   const char *geo = dim == 3 ? "X_Y_Z" : "X_Y";
   char line[512];

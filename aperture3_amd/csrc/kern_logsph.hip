@@ -597,6 +597,35 @@ __global__ void k_logsph_divs(Mesh m, LsCoef cf, FieldSet<T> f, uint64_t nb) {
               den;
 }
 
+// compute_EdotB_2d (cuda/sim_data.cu:50-79): E and B interpolated to the
+// cell centre (stagger 0b000) with interpolate2d's neighbour offsets, then
+// EdotB += delta (E.B) / |B| with delta = 1.0f / data_interval
+template <typename T>
+__device__ inline T centre2d(const T *a, uint64_t o, int st, int d0) {
+  const int64_t im = (st & 1) ? -1 : 0, jm = (st & 2) ? -(int64_t)d0 : 0;
+  const T *p = a + o;
+  const T f1 = (T)0.5 * (p[0] + p[jm]);
+  const T f0 = (T)0.5 * (p[im] + p[im + jm]);
+  return (T)0.5 * (f1 + f0);
+}
+
+template <typename T>
+__global__ void k_logsph_edotb(Mesh m, FieldSet<T> f, T delta, uint64_t nb) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb) return;
+  int i, j;
+  bulk_ij<T>(m, t, i, j);
+  const int d0 = m.dims[0];
+  const uint64_t o = (uint64_t)i + (uint64_t)j * d0;
+  const T E1 = centre2d<T>(f.E[0], o, 0b110, d0);
+  const T E2 = centre2d<T>(f.E[1], o, 0b101, d0);
+  const T E3 = centre2d<T>(f.E[2], o, 0b011, d0);
+  const T B1 = centre2d<T>(f.B[0], o, 0b001, d0);
+  const T B2 = centre2d<T>(f.B[1], o, 0b010, d0);
+  const T B3 = centre2d<T>(f.B[2], o, 0b100, d0);
+  f.EdotB[o] += delta * (E1 * B1 + E2 * B2 + E3 * B3) / sqrt(B1 * B1 + B2 * B2 + B3 * B3);
+}
+
 template <typename T>
 __global__ void k_logsph_axis_dive(Mesh m, LsCoef cf, FieldSet<T> f) {
   int i = m.guard[0] + blockIdx.x * blockDim.x + threadIdx.x;
@@ -861,7 +890,7 @@ hipError_t launch_logsph_ptc_boundary(const Mesh &m, const LsParams &lp,
 template <typename T>
 hipError_t launch_logsph_field_update(const Mesh &m, const LsCoef &cf,
                                       const LsParams &lp, const FieldSet<T> &f,
-                                      double dt, hipStream_t st) {
+                                      double dt, double edotb_delta, hipStream_t st) {
   const uint64_t nb = (uint64_t)(m.dims[0] - 2 * m.guard[0]) *
                       (m.dims[1] - 2 * m.guard[1]);
   const int nr = m.dims[0] - 2 * m.guard[0];
@@ -871,6 +900,8 @@ hipError_t launch_logsph_field_update(const Mesh &m, const LsCoef &cf,
   k_logsph_axis_e<T><<<grid1(nr), 256, 0, st>>>(m, f, (T)dt);
   k_logsph_divs<T><<<grid1(nb), 256, 0, st>>>(m, cf, f, nb);
   k_logsph_axis_dive<T><<<grid1(nr), 256, 0, st>>>(m, cf, f);
+  if (f.EdotB)
+    k_logsph_edotb<T><<<grid1(nb), 256, 0, st>>>(m, f, (T)edotb_delta, nb);
   return hipGetLastError();
 }
 
@@ -920,7 +951,7 @@ hipError_t launch_logsph_inject(const Mesh &m, const LsParams &lp,
       hipStream_t);                                                            \
   template hipError_t launch_logsph_field_update<T>(                           \
       const Mesh &, const LsCoef &, const LsParams &, const FieldSet<T> &,     \
-      double, hipStream_t);                                                    \
+      double, double, hipStream_t);                                            \
   template hipError_t launch_logsph_field_boundary<T>(                         \
       const Mesh &, const LsParams &, const FieldSet<T> &, double,             \
       hipStream_t);                                                            \

@@ -63,10 +63,12 @@ __device__ inline R ptc_gamma(const PtcView<R> &p, uint64_t i) {
   return sqrt((R)1 + a * a + b * b + c * c);
 }
 
-// count_photon_produced: the emit decision (check_emit_photon)
+// count_photon_produced: the emit decision (check_emit_photon) and the
+// emitter's weight added to photon_produced of its cell
+// (radiative_transfer.cu:48-57)
 template <typename R>
 __global__ void k_emit_flag(Mesh m, RtParams rt, PtcView<R> p, uint64_t n,
-                            uint32_t *__restrict__ flags) {
+                            uint32_t *__restrict__ flags, R *__restrict__ produced) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t c = p.cell[i], f = 0;
@@ -78,6 +80,7 @@ __global__ void k_emit_flag(Mesh m, RtParams rt, PtcView<R> p, uint64_t n,
       f = 1;
   }
   flags[i] = f;
+  if (f && produced) atomicAdd(&produced[c], p.w[i]);
 }
 
 // produce_photons + emit_photon
@@ -141,10 +144,11 @@ __global__ void k_emit(Mesh m, RtParams rt, PtcView<R> p, PhoView<R> ph,
   ph.cell[slot] = c;
 }
 
-// count_pairs_produced: check_produce_pair
+// count_pairs_produced: check_produce_pair, and the photon's weight added
+// to pair_produced of its cell (radiative_transfer.cu:120-130)
 template <typename R>
 __global__ void k_pair_flag(Mesh m, PhoView<R> ph, uint64_t n,
-                            uint32_t *__restrict__ flags) {
+                            uint32_t *__restrict__ flags, R *__restrict__ produced) {
   uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   uint32_t c = ph.cell[t], f = 0;
@@ -154,6 +158,7 @@ __global__ void k_pair_flag(Mesh m, PhoView<R> ph, uint64_t n,
     if (m.in_bulk(c1, c2, c3) && ph.path[t] <= (R)0) f = 1;
   }
   flags[t] = f;
+  if (f && produced) atomicAdd(&produced[c], ph.w[t]);
 }
 
 // produce_pairs + produce_pair
@@ -394,12 +399,12 @@ hipError_t rt_emit(const Mesh &m, const RtParams &rt, const PtcView<R> &p,
                    uint64_t n, const PhoView<R> &ph, uint64_t ph_base,
                    uint64_t seed, uint32_t epoch, uint64_t id_base,
                    RtWork &w, unsigned long long *counters, hipStream_t st,
-                   uint64_t *emitted) {
+                   R *produced, uint64_t *emitted) {
   *emitted = 0;
   if (n == 0) return hipSuccess;
   hipError_t e = w.ensure(n);
   if (e != hipSuccess) return e;
-  k_emit_flag<R><<<rt_blocks(n), 256, 0, st>>>(m, rt, p, n, w.flags);
+  k_emit_flag<R><<<rt_blocks(n), 256, 0, st>>>(m, rt, p, n, w.flags, produced);
   uint64_t total = 0;
   if ((e = scan_events(w, n, st, &total)) != hipSuccess) return e;
   if (total == 0) return hipSuccess;
@@ -415,12 +420,12 @@ template <typename R>
 hipError_t rt_pairs(const Mesh &m, const RtParams &rt, const PtcView<R> &p,
                     uint64_t p_base, const PhoView<R> &ph, uint64_t nph,
                     uint64_t id_base, RtWork &w, unsigned long long *counters,
-                    hipStream_t st, uint64_t *made) {
+                    hipStream_t st, R *produced, uint64_t *made) {
   *made = 0;
   if (nph == 0) return hipSuccess;
   hipError_t e = w.ensure(nph);
   if (e != hipSuccess) return e;
-  k_pair_flag<R><<<rt_blocks(nph), 256, 0, st>>>(m, ph, nph, w.flags);
+  k_pair_flag<R><<<rt_blocks(nph), 256, 0, st>>>(m, ph, nph, w.flags, produced);
   uint64_t total = 0;
   if ((e = scan_events(w, nph, st, &total)) != hipSuccess) return e;
   if (total == 0) return hipSuccess;
@@ -450,13 +455,13 @@ hipError_t rt_move_photons(const Mesh &m, int logsph, double r_cutoff,
                                  const PtcView<R> &, uint64_t,                \
                                  const PhoView<R> &, uint64_t, uint64_t,      \
                                  uint32_t, uint64_t, RtWork &,                \
-                                 unsigned long long *, hipStream_t,           \
+                                 unsigned long long *, hipStream_t, R *,      \
                                  uint64_t *);                                 \
   template hipError_t rt_pairs<R>(const Mesh &, const RtParams &,             \
                                   const PtcView<R> &, uint64_t,               \
                                   const PhoView<R> &, uint64_t, uint64_t,     \
                                   RtWork &, unsigned long long *,             \
-                                  hipStream_t, uint64_t *);                   \
+                                  hipStream_t, R *, uint64_t *);              \
   template hipError_t rt_move_photons<R>(const Mesh &, int, double,           \
                                          const PhoView<R> &, uint64_t,        \
                                          double, hipStream_t);

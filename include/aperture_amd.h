@@ -65,7 +65,17 @@ enum {
   APT_FIELD_J = 4,
   APT_FIELD_RHO = 5, /* component = species */
   APT_FIELD_DIVE = 6,
-  APT_FIELD_DIVB = 7
+  APT_FIELD_DIVB = 7,
+  /* sim_data::EdotB (sim_data.h:57): E.B/|B| at cell centres, accumulated
+   * with weight 1/data_interval by every log-spherical field update
+   * (compute_edotb, cuda/sim_data.cu:235-265); zero it with apt_field_set
+   * after output, as the pulsar lab does (labs/pulsar/src/main.cpp:106-108) */
+  APT_FIELD_EDOTB = 8,
+  /* photon_produced / pair_produced (sim_data.h:58-59): summed weights of
+   * the particles that emitted and the photons that converted, per cell
+   * (radiative_transfer.cu:48-57, 120-130); zeroed the same way */
+  APT_FIELD_PHOTON_PRODUCED = 9,
+  APT_FIELD_PAIR_PRODUCED = 10
 };
 
 /* Timed kernels (apt_timing_get). */

@@ -48,10 +48,9 @@ apt_status apt_io_write_grid(const apt_grid *grid, int coord, const char *dir,
                              int downsample);
 
 /* write_field_output with the default user_write_field_output: fld.%05d.h5
- * holding E1..E3, B1..B3, J1..J3, Rho_e, Rho_p (, Rho_i), divE, divB, each
- * downsampled to FP32 by multi_array::downsample (cuda/core/
- * multi_array.cu:18-68, 296-326).  The reference's photon_produced /
- * pair_produced counters are not kept by this build and are not written. */
+ * holding E1..E3, B1..B3, J1..J3, Rho_e, Rho_p (, Rho_i), divE, divB,
+ * photon_produced, pair_produced, each downsampled to FP32 by
+ * multi_array::downsample (cuda/core/multi_array.cu:18-68, 296-326). */
 apt_status apt_io_write_fields(apt_ctx *ctx, const apt_grid *grid, int num_species,
                                const char *dir, int output_num, int downsample);
 

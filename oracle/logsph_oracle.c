@@ -520,6 +520,33 @@ void orc_logsph_field_update(const orc_mesh *m, const orc_logsph_coef *c,
   logsph_divs(m, c, f);
 }
 
+/* sim_data::compute_edotb, 2D form (cuda/sim_data.cu:50-79, called at the
+ * end of field_solver_logsph::update_fields, field_solver_logsph.cu:393):
+ * every field component interpolated to the cell centre (interpolate2d,
+ * interpolation.cuh:238-251, output stagger 0b000: along an axis where the
+ * component is staggered, the mean of the cell and the one before), then
+ * edotb += delta (E.B) / |B| over the bulk. */
+static double centre2d(const double *a, size_t o, int st, int d0) {
+  const long im = (st & 1) ? -1 : 0, jm = (st & 2) ? -(long)d0 : 0;
+  const double *q = a + o;
+  double f1 = 0.5 * (q[0] + q[jm]);
+  double f0 = 0.5 * (q[im] + q[im + jm]);
+  return 0.5 * (f1 + f0);
+}
+
+void orc_logsph_edotb(const orc_mesh *m, const orc_fields *f, double *edotb,
+                      double delta) {
+  const int d0 = m->dims[0];
+  for (int j = m->guard[1]; j < m->dims[1] - m->guard[1]; j++)
+    for (int i = m->guard[0]; i < m->dims[0] - m->guard[0]; i++) {
+      size_t o = (size_t)i + (size_t)j * d0;
+      double E1 = centre2d(f->E[0], o, 0x6, d0), E2 = centre2d(f->E[1], o, 0x5, d0);
+      double E3 = centre2d(f->E[2], o, 0x3, d0), B1 = centre2d(f->B[0], o, 0x1, d0);
+      double B2 = centre2d(f->B[1], o, 0x2, d0), B3 = centre2d(f->B[2], o, 0x4, d0);
+      edotb[o] += delta * (E1 * B1 + E2 * B2 + E3 * B3) / sqrt(B1 * B1 + B2 * B2 + B3 * B3);
+    }
+}
+
 /* field_solver_logsph::apply_boundary (field_solver_logsph.cu:403-428):
  * stellar_boundary (:252-275), outflow_boundary_sph (:322-345),
  * axis_boundary_lower/upper (:277-320), in that order. */

@@ -328,6 +328,7 @@ class LogSphOracle(OracleSim):
             self.C.a[i] = _d(self.coef[i])
         self.lib.orc_logsph_compute_coef(C.byref(self.mesh), C.byref(self.C))
         self.jtmp = np.zeros(self.ncell)
+        self.edotb = np.zeros(self.ncell)
         self.multi = 0
 
     def coefficient(self, name):
@@ -353,6 +354,14 @@ class LogSphOracle(OracleSim):
     def field_update(self, dt, step=0):
         self.lib.orc_logsph_field_update(C.byref(self.mesh), C.byref(self.C),
                                          C.byref(self.lp), C.byref(self.F), C.c_double(dt))
+        # compute_edotb with delta = 1.0f / data_interval (cuda/sim_data.cu:60)
+        di = self.data_interval if self.data_interval > 0 else 1
+        self.edotb_accumulate(float(np.float32(1.0) / np.float32(di)))
+
+    def edotb_accumulate(self, delta):
+        """compute_edotb alone: edotb += delta (E.B)/|B| at cell centres."""
+        self.lib.orc_logsph_edotb(C.byref(self.mesh), C.byref(self.F), _d(self.edotb),
+                                  C.c_double(delta))
 
     def field_boundary(self, omega):
         self.lib.orc_logsph_field_boundary(C.byref(self.mesh), C.byref(self.lp),
@@ -466,6 +475,8 @@ def _set_rt(self, model=0, gamma_thr=20.0, photon_path=1.0, E_secondary=4.0,
     self.ph = {k: np.zeros(max_photon, dt) for k, dt in self.PH_FIELDS}
     self.ph["cell"][:] = MAX_CELL
     self.ph_number = 0
+    self.photon_produced = np.zeros(self.ncell)
+    self.pair_produced = np.zeros(self.ncell)
 
 
 def _emit_photons(self):
@@ -474,7 +485,7 @@ def _emit_photons(self):
     k = self.lib.orc_emit_photons(C.byref(self.mesh), C.byref(self.rt), C.byref(P),
                                   C.byref(Ph), C.c_uint64(self.seed),
                                   C.c_uint32(self.rt_epoch), C.c_uint64(self.ph_id_next),
-                                  C.byref(ov))
+                                  C.byref(ov), _d(self.photon_produced))
     self.rt_epoch += 1
     self.ph_number = int(Ph.number)
     self.ph_id_next += int(k)
@@ -486,7 +497,8 @@ def _produce_pairs(self):
     ov = C.c_uint64(0)
     P, Ph = self._ptc_struct(), self._ph_struct()
     k = self.lib.orc_produce_pairs(C.byref(self.mesh), C.byref(self.rt), C.byref(P),
-                                   C.byref(Ph), C.c_uint64(self.id_next), C.byref(ov))
+                                   C.byref(Ph), C.c_uint64(self.id_next), C.byref(ov),
+                                   _d(self.pair_produced))
     self.number = int(P.number)
     self.id_next += 2 * int(k)
     self.overflow += int(ov.value)

@@ -144,6 +144,8 @@ void orc_logsph_field_update(const orc_mesh *m, const orc_logsph_coef *c,
                              double dt);
 void orc_logsph_field_boundary(const orc_mesh *m, const orc_logsph_params *lp,
                                orc_fields *f, double omega);
+void orc_logsph_edotb(const orc_mesh *m, const orc_fields *f, double *edotb,
+                      double delta);
 uint64_t orc_logsph_inject(const orc_mesh *m, const orc_logsph_params *lp,
                            orc_ptc *p, int inj_per_cell, double p1, double p2,
                            double p3, double w, double omega, double q_e,
@@ -188,10 +190,10 @@ typedef struct {
 uint64_t orc_emit_photons(const orc_mesh *m, const orc_rt_params *rt,
                           orc_ptc *p, orc_photons *ph, uint64_t seed,
                           uint32_t epoch, uint64_t id_base,
-                          uint64_t *overflow);
+                          uint64_t *overflow, double *produced);
 uint64_t orc_produce_pairs(const orc_mesh *m, const orc_rt_params *rt,
                            orc_ptc *p, orc_photons *ph, uint64_t id_base,
-                           uint64_t *overflow);
+                           uint64_t *overflow, double *produced);
 void orc_move_photons(const orc_mesh *m, int logsph, double r_cutoff,
                       orc_photons *ph, double dt);
 void orc_clear_photon_guard_cells(const orc_mesh *m, orc_photons *ph);

@@ -54,11 +54,13 @@ static inline void split_cell(const orc_mesh *m, uint32_t c, int *c1, int *c2,
 /* count_photon_produced + produce_photons (radiative_transfer.cu:28-96)
  * with check_emit_photon / emit_photon.  Returns the number of emitting
  * particles (= photon slots reserved); *overflow counts emitters dropped
- * for lack of photon capacity. */
+ * for lack of photon capacity.  `produced` (may be NULL) gains every
+ * emitter's weight at its cell, before any capacity or path test
+ * (:48-57). */
 uint64_t orc_emit_photons(const orc_mesh *m, const orc_rt_params *rt,
                           orc_ptc *p, orc_photons *ph, uint64_t seed,
                           uint32_t epoch, uint64_t id_base,
-                          uint64_t *overflow) {
+                          uint64_t *overflow, double *produced) {
   uint64_t base = ph->number, k = 0;
   for (uint64_t idx = 0; idx < p->number; idx++) {
     uint32_t c = p->cell[idx];
@@ -67,6 +69,7 @@ uint64_t orc_emit_photons(const orc_mesh *m, const orc_rt_params *rt,
     if ((flag >> 29) == 2) continue; /* ions do not radiate */
     double gamma = p->E[idx];
     if (!(gamma > rt->gamma_thr)) continue;
+    if (produced) produced[c] += p->w[idx];
     uint64_t slot = base + k;
     if (slot >= rt->max_photon) {
       if (overflow) (*overflow)++;
@@ -122,15 +125,18 @@ uint64_t orc_emit_photons(const orc_mesh *m, const orc_rt_params *rt,
 
 /* count_pairs_produced + produce_pairs (radiative_transfer.cu:99-177) with
  * check_produce_pair / produce_pair.  Returns the number of pairs made;
- * photons past the particle capacity are removed and counted. */
+ * photons past the particle capacity are removed and counted.  `produced`
+ * (may be NULL) gains every converting photon's weight at its cell
+ * (:120-130). */
 uint64_t orc_produce_pairs(const orc_mesh *m, const orc_rt_params *rt,
                            orc_ptc *p, orc_photons *ph, uint64_t id_base,
-                           uint64_t *overflow) {
+                           uint64_t *overflow, double *produced) {
   uint64_t base = p->number, k = 0;
   for (uint64_t t = 0; t < ph->number; t++) {
     uint32_t c = ph->cell[t];
     if (c == ORC_MAX_CELL || !is_in_bulk(m, c)) continue;
     if (!(ph->path[t] <= 0.0)) continue;
+    if (produced) produced[c] += ph->w[t];
     uint64_t off = base + 2 * k;
     if (off + 1 >= rt->max_ptc) {
       if (overflow) (*overflow)++;

@@ -172,12 +172,14 @@ def test_field_and_tracked_output(tmp_path, ds):
     dims, guard = env.dims, env.params().guard
     with h5read.File(str(tmp_path / "fld.00001.h5")) as f:
         assert set(f.names()) == {"E1", "E2", "E3", "B1", "B2", "B3", "J1", "J2", "J3",
-                                  "Rho_e", "Rho_p", "Rho_i", "divE", "divB"}
+                                  "Rho_e", "Rho_p", "Rho_i", "divE", "divB",
+                                  "photon_produced", "pair_produced"}
         for name, w, k in (("E1", apt.FIELD_E, 0), ("E2", apt.FIELD_E, 1),
                            ("B1", apt.FIELD_B, 0), ("B3", apt.FIELD_B, 2),
-                           ("J2", apt.FIELD_J, 1), ("Rho_p", apt.FIELD_RHO, 1)):
+                           ("J2", apt.FIELD_J, 1), ("Rho_p", apt.FIELD_RHO, 1),
+                           ("photon_produced", apt.FIELD_PHOTON_PRODUCED, 0)):
             a = f.read(name)
-            st = apt.STAGGER.get(w, (0b111,) * 3)[k]
+            st = apt.STAGGER.get(w, (0b000 if w >= apt.FIELD_EDOTB else 0b111,) * 3)[k]
             exp = node_average(d.get_field(w, k), dims, guard, st, ds)
             assert a.dtype == np.float32 and a.shape == exp.shape, name
             np.testing.assert_array_equal(a, exp.astype(np.float32), err_msg=name)
@@ -218,12 +220,18 @@ def test_cpp_pulsar_driver_restart(tmp_path):
     run(a, 12, a / "snapshot.h5")
     run(c, 12)
     assert open(a / "data.xmf").read() == open(c / "data.xmf").read()
+    # the per-cell event counters accumulate between outputs and are not
+    # part of a snapshot (nor in the reference's), so the restarted run's
+    # first output after the restart lacks the steps before it
+    skip = {"fld.00002.h5": {"photon_produced", "pair_produced"}}
     for name in ("final.h5", "snapshot.h5", "fld.00002.h5", "fld.00003.h5", "ptc.00003.h5",
                  "grid.h5"):
         with h5read.File(str(a / name)) as fa, h5read.File(str(c / name)) as fc:
             names = fa.names()
             assert names == fc.names(), name
             for n in names:
+                if n in skip.get(name, ()):
+                    continue
                 va, vc = fa.read(n), fc.read(n)
                 assert va.dtype == vc.dtype and np.array_equal(va, vc), (name, n)
     with h5read.File(str(c / "final.h5")) as f:

@@ -70,6 +70,9 @@ def test_fp64_logsph_step_bit_exact(variant):
         assert np.array_equal(d.get_field(apt.FIELD_RHO, s), o.rho[s]), ("rho", s)
     assert np.array_equal(d.get_field(apt.FIELD_DIVE, 0), o.divE)
     assert np.array_equal(d.get_field(apt.FIELD_DIVB, 0), o.divB)
+    # compute_edotb accumulated by every field update (sim_data.cu:235-265)
+    assert np.array_equal(d.get_field(apt.FIELD_EDOTB, 0), o.edotb)
+    assert np.any(o.edotb != 0)
     cnt = d.counters()
     assert cnt["nan_momenta"] == 0 and cnt["multi_cell_moves"] == o.multi == 0
     d.close()

@@ -66,6 +66,11 @@ def test_reference_count_kat_gpu():
         assert rad.produce_pairs(d) == 100
         d.sort_particles()
         assert d.num_particles() == 1200 and d.info()["ph_number"] == 0
+        # photon_produced / pair_produced: 100 unit weights in the one cell
+        cell = 10 + 10 * env.dims[0]
+        for which in (apt.FIELD_PHOTON_PRODUCED, apt.FIELD_PAIR_PRODUCED):
+            f = d.get_field(which, 0)
+            assert f[cell] == 100.0 and f.sum() == 100.0
         d.close()
 
 
@@ -99,6 +104,10 @@ def test_cartesian_pairs_bit_exact(model):
             d.sort_particles()
     assert o.rt_epoch == 6
     assert_state(o, d)
+    # per-cell event weights: unit weights, so the atomic sums are exact
+    assert np.array_equal(d.get_field(apt.FIELD_PHOTON_PRODUCED, 0), o.photon_produced)
+    assert np.array_equal(d.get_field(apt.FIELD_PAIR_PRODUCED, 0), o.pair_produced)
+    assert o.photon_produced.sum() > 0
     d.close()
 
 
@@ -126,6 +135,12 @@ def test_logsph_pairs_bit_exact():
             d.sort_particles()
     assert d.info()["ph_number"] > 0
     assert_state(o, d)
+    # event weights summed by atomics in any order, as in the reference:
+    # equal to the oracle's slot-order sums up to rounding
+    for which, ref in ((apt.FIELD_PHOTON_PRODUCED, o.photon_produced),
+                       (apt.FIELD_PAIR_PRODUCED, o.pair_produced)):
+        np.testing.assert_allclose(d.get_field(which, 0), ref, rtol=1e-13, atol=0)
+    assert np.array_equal(d.get_field(apt.FIELD_EDOTB, 0), o.edotb)
     d.close()
 
 

@@ -148,7 +148,7 @@ def test_xmf_layout(tmp_path):
     assert items == ["grid.h5:x1", "grid.h5:x2"]
     attrs = [a.get("Name") for a in g.findall("Attribute")]
     assert attrs == ["E1", "E2", "E3", "B1", "B2", "B3", "J1", "J2", "J3", "Rho_e", "Rho_p",
-                     "Rho_i", "divE", "divB"]
+                     "Rho_i", "divE", "divB", "photon_produced", "pair_produced"]
     assert g.findall("Attribute")[3].find("DataItem").text.strip() == "fld.00007.h5:B1"
 
 

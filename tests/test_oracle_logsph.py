@@ -274,3 +274,83 @@ def test_injection_layout_and_gate():
     new = P["cell"][k:]
     got_rows = set((new[new != oc.MAX_CELL] // o2.dims[0]).tolist())
     assert row not in got_rows and (row - 5) in got_rows
+
+
+def edotb_case(**kw):
+    o = make((16, 12), **kw)
+    for c in range(3):
+        o.f["E"][c][:] = 0.0
+        o.f["B"][c][:] = 0.0
+    return o
+
+
+def bulk_mask(o):
+    d, g = o.dims, o.mesh.guard
+    m = np.zeros((d[1], d[0]), bool)
+    m[g[1]:d[1] - g[1], g[0]:d[0] - g[0]] = True
+    return m.ravel()
+
+
+def test_edotb_uniform_fields_closed_form():
+    """Uniform E, B: every bulk cell gains delta (E.B)/|B|; guards stay 0;
+    E perpendicular to B gives 0."""
+    o = edotb_case()
+    E, B = (0.3, -1.25, 2.0), (1.5, 0.5, -2.0)
+    for c in range(3):
+        o.f["E"][c][:] = E[c]
+        o.f["B"][c][:] = B[c]
+    o.edotb_accumulate(0.125)
+    want = 0.125 * (E[0] * B[0] + E[1] * B[1] + E[2] * B[2]) / math.sqrt(
+        B[0] ** 2 + B[1] ** 2 + B[2] ** 2)
+    m = bulk_mask(o)
+    np.testing.assert_allclose(o.edotb[m], want, rtol=1e-15)
+    assert not np.any(o.edotb[~m])
+    o.edotb_accumulate(0.125)
+    np.testing.assert_allclose(o.edotb[m], 2 * want, rtol=1e-15)
+    o2 = edotb_case()
+    o2.f["E"][0][:] = 1.0
+    o2.f["B"][1][:] = 3.0
+    o2.edotb_accumulate(1.0)
+    assert not np.any(o2.edotb)
+
+
+def test_edotb_interpolation_offsets():
+    """A component staggered along an axis is averaged over the cell and
+    the one before it along that axis (interpolate2d with output stagger
+    0b000): E1 (stagger 0b110) = j ramp -> j - 1/2; E3 (0b011) = i + 10 j
+    -> i + 10 j - 5.5; B2 (0b010) is staggered in y only."""
+    o = edotb_case()
+    d = o.dims
+    J, I = np.meshgrid(np.arange(d[1]), np.arange(d[0]), indexing="ij")
+    o.f["E"][0][:] = J.ravel().astype(float)
+    o.f["B"][0][:] = 2.0
+    o.edotb_accumulate(1.0)
+    m = bulk_mask(o)
+    np.testing.assert_array_equal(o.edotb[m], (J.ravel() - 0.5)[m])
+    o = edotb_case()
+    o.f["E"][2][:] = (I + 10.0 * J).ravel()
+    o.f["B"][2][:] = -4.0
+    o.edotb_accumulate(1.0)
+    np.testing.assert_array_equal(o.edotb[m], -(I + 10.0 * J - 5.5).ravel()[m])
+    o = edotb_case()
+    o.f["E"][1][:] = 1.0
+    o.f["B"][1][:] = (3.0 + I + 10.0 * J).ravel()
+    o.edotb_accumulate(1.0)
+    np.testing.assert_array_equal(o.edotb[m], 1.0)
+
+
+def test_field_update_accumulates_edotb_with_float_delta():
+    """field_update adds compute_edotb with delta = 1.0f / data_interval
+    (a float: 1/3 rounds to 0.3333333432674408)."""
+    o = make((16, 12), data_interval=3)
+    dipole(o)
+    o.f["E"][2][:] = 0.01  # E.B = E3 B3 = 0 for the poloidal dipole
+    o.f["E"][0][:] = 0.02
+    ref = edotb_case()
+    o.field_update(0.0)  # dt = 0 keeps E and B as they are
+    for c in range(3):
+        ref.f["E"][c][:] = o.f["E"][c]
+        ref.f["B"][c][:] = o.f["B"][c]
+    ref.edotb_accumulate(0.3333333432674408)
+    np.testing.assert_array_equal(o.edotb, ref.edotb)
+    assert np.any(o.edotb != 0)

@@ -145,3 +145,32 @@ def test_photon_radial_logsph():
     assert r == pytest.approx(r0 + 20 * dt, rel=1e-12)
     assert c // o.dims[0] == 12 and o.ph["x2"][0] == pytest.approx(0.5, abs=1e-12)
     assert o.ph["p1"][0] == pytest.approx(5.0, rel=1e-12)
+
+
+@pytest.mark.parametrize("logsph", [False, True])
+def test_event_counters_kat(logsph):
+    """photon_produced / pair_produced (radiative_transfer.cu:48-57,
+    120-130) on the known answer: the 100 emitters and the 100 converting
+    photons all sit in one cell with unit weight."""
+    o = kat_state(logsph)
+    o.set_rt(model=0, gamma_thr=20.0, photon_path=-1.0, E_secondary=4.0, r_cutoff=4.0,
+             max_photon=1000)
+    cell = 10 + 10 * o.dims[0]
+    o.emit_photons()
+    assert o.photon_produced[cell] == 100.0 and o.photon_produced.sum() == 100.0
+    o.produce_pairs()
+    assert o.pair_produced[cell] == 100.0 and o.pair_produced.sum() == 100.0
+
+
+def test_event_counters_count_before_capacity():
+    """The reference counts every emitter in its counting pass, before the
+    photon slots are assigned: emitters dropped for capacity still count;
+    weights add as weights."""
+    o = kat_state()
+    P = o.particles()
+    P["w"] = np.where(np.arange(len(P["w"])) % 20 == 0, 0.25, 1.0)
+    o.set_particles(P)
+    o.set_rt(model=0, gamma_thr=20.0, photon_path=-1.0, r_cutoff=4.0, max_photon=30)
+    assert o.emit_photons() == 30
+    # emitters are every 10th particle; every 20th of all carries w = 0.25
+    assert o.photon_produced.sum() == 50 * 0.25 + 50 * 1.0
