@@ -24,6 +24,7 @@ APT_PREC_FP64 = 0
 APT_PREC_MIXED = 1
 FIELD_E, FIELD_B, FIELD_EBG, FIELD_BBG, FIELD_J, FIELD_RHO, FIELD_DIVE, FIELD_DIVB = range(8)
 FIELD_EDOTB, FIELD_PHOTON_PRODUCED, FIELD_PAIR_PRODUCED = 8, 9, 10
+FIELD_PTC_NUM, FIELD_GAMMA, FIELD_PHOTON_NUM = 11, 12, 13
 KERNEL_PUSH_DEPOSIT, KERNEL_FIELD, KERNEL_SORT = range(3)
 MAX_CELL = 0xFFFFFFFF
 FLAG_IGNORE_CURRENT = 1 << 3
@@ -154,6 +155,7 @@ SIGNATURES = {
     "apt_field_compute_divs": [_P],
     "apt_sort_particles": [_P],
     "apt_clear_guard_cells": [_P],
+    "apt_collect_diagnostics": [_P],
     "apt_energy_histogram": [_P, C.POINTER(C.c_uint32), C.c_int, C.c_double, C.c_int],
     "apt_tile_offsets": [_P, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint32),
                          C.POINTER(C.c_int)],
@@ -542,6 +544,19 @@ class sim_data:
         self._check(self.lib.apt_timing_get(self.ctx, kernel, C.byref(ms), C.byref(n)),
                     "apt_timing_get")
         return ms.value, n.value
+
+
+class AdditionalDiagnostics:
+    """AdditionalDiagnostics (cuda/core/additional_diagnostics.cu:63-100):
+    per-cell summed weight and mean gamma per species, summed photon
+    weight; read them with get_field(FIELD_PTC_NUM / FIELD_GAMMA, species)
+    and get_field(FIELD_PHOTON_NUM, 0)."""
+
+    def __init__(self, env: sim_environment):
+        self.env = env
+
+    def collect_diagnostics(self, data: sim_data):
+        data._check(data.lib.apt_collect_diagnostics(data.ctx), "apt_collect_diagnostics")
 
 
 class ptc_updater:

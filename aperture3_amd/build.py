@@ -34,6 +34,7 @@ HIP_SOURCES = [
     "kern_rt.hip",
     "kern_ffe.hip",
     "kern_batch.hip",
+    "kern_diag.hip",
     "apt_context.hip",
 ]
 HOST_SOURCES = ["host/aperture_host.cpp"]

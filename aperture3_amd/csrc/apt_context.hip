@@ -38,6 +38,8 @@ struct apt_ctx {
   void *Rho[4] = {}, *divE = nullptr, *divB = nullptr, *jtmp = nullptr;
   // sim_data::EdotB, photon_produced, pair_produced (sim_data.h:57-59)
   void *EdotB = nullptr, *ph_produced = nullptr, *pair_produced = nullptr;
+  // collect_diagnostics moments, allocated on the first collection
+  void *diag_num[4] = {}, *diag_gamma[4] = {}, *diag_ph = nullptr;
   long long *Jacc[3][2] = {};
   long long *Racc[4][2] = {};
 
@@ -477,7 +479,11 @@ apt_status apt_destroy(apt_ctx *ctx) {
   }
   (void)hipFree(ctx->divE); (void)hipFree(ctx->divB); (void)hipFree(ctx->jtmp);
   (void)hipFree(ctx->EdotB); (void)hipFree(ctx->ph_produced);
-  (void)hipFree(ctx->pair_produced);
+  (void)hipFree(ctx->pair_produced); (void)hipFree(ctx->diag_ph);
+  for (int s = 0; s < 4; s++) {
+    (void)hipFree(ctx->diag_num[s]);
+    (void)hipFree(ctx->diag_gamma[s]);
+  }
   (void)hipFree(ctx->d_counters);
   for (int k = 0; k < LS_NCOEF; k++) (void)hipFree(ctx->coef_mem[k]);
   for (int k = 0; k < 12; k++) (void)hipFree(ctx->ffe[k]);
@@ -582,6 +588,11 @@ static void **field_slot(apt_ctx *ctx, int which, int comp) {
     case APT_FIELD_EDOTB: return comp == 0 ? &ctx->EdotB : nullptr;
     case APT_FIELD_PHOTON_PRODUCED: return comp == 0 ? &ctx->ph_produced : nullptr;
     case APT_FIELD_PAIR_PRODUCED: return comp == 0 ? &ctx->pair_produced : nullptr;
+    case APT_FIELD_PTC_NUM:
+      return comp < ctx->params.num_species ? &ctx->diag_num[comp] : nullptr;
+    case APT_FIELD_GAMMA:
+      return comp < ctx->params.num_species ? &ctx->diag_gamma[comp] : nullptr;
+    case APT_FIELD_PHOTON_NUM: return comp == 0 ? &ctx->diag_ph : nullptr;
   }
   return nullptr;
 }
@@ -592,6 +603,8 @@ apt_status apt_field_set(apt_ctx *ctx, int which, int comp,
   void **slot = field_slot(ctx, which, comp);
   if (!slot || !src || count != ctx->ncell)
     return ctx->fail(APT_ERR_INVALID_ARG, "field_set: bad field/size");
+  if (!*slot)
+    return ctx->fail(APT_ERR_STATE, "field_set: moments not collected yet");
   HIPCHK(hipStreamSynchronize(ctx->stream));
   if (ctx->prec == APT_PREC_FP64) {
     HIPCHK(hipMemcpy(*slot, src, count * 8, hipMemcpyHostToDevice));
@@ -609,6 +622,8 @@ apt_status apt_field_get(apt_ctx *ctx, int which, int comp, double *dst,
   void **slot = field_slot(ctx, which, comp);
   if (!slot || !dst || count != ctx->ncell)
     return ctx->fail(APT_ERR_INVALID_ARG, "field_get: bad field/size");
+  if (!*slot)
+    return ctx->fail(APT_ERR_STATE, "field_get: moments not collected yet");
   HIPCHK(hipStreamSynchronize(ctx->stream));
   if (ctx->prec == APT_PREC_FP64) {
     HIPCHK(hipMemcpy(dst, *slot, count * 8, hipMemcpyDeviceToHost));
@@ -1600,6 +1615,34 @@ static apt_status sort_particles_only(apt_ctx *ctx) {
   }
   ctx->number = kept;
   return APT_OK;
+}
+
+template <typename R, typename P, typename H>
+static apt_status diagnostics_t(apt_ctx *ctx, const P &p, const H &ph) {
+  DiagArrays<R> d;
+  for (int s = 0; s < 4; s++) {
+    d.num[s] = (R *)ctx->diag_num[s];
+    d.gamma[s] = (R *)ctx->diag_gamma[s];
+  }
+  d.photon_num = (R *)ctx->diag_ph;
+  HIPCHK(launch_diagnostics<R>(view_of(p), ctx->number, view_of(ph),
+                               ctx->ph_cap ? ctx->ph_number : 0, d,
+                               ctx->params.num_species, ctx->ncell, ctx->stream));
+  return APT_OK;
+}
+
+apt_status apt_collect_diagnostics(apt_ctx *ctx) {
+  CHECK_CTX(ctx);
+  const size_t bytes = ctx->ncell * (ctx->prec == APT_PREC_FP64 ? 8 : 4);
+  if (!ctx->diag_ph) {
+    HIPCHK(hipMalloc(&ctx->diag_ph, bytes));
+    for (int s = 0; s < 4; s++) {
+      HIPCHK(hipMalloc(&ctx->diag_num[s], bytes));
+      HIPCHK(hipMalloc(&ctx->diag_gamma[s], bytes));
+    }
+  }
+  if (ctx->prec == APT_PREC_FP64) return diagnostics_t<double>(ctx, ctx->p64, ctx->ph64);
+  return diagnostics_t<float>(ctx, ctx->p32, ctx->ph32);
 }
 
 apt_status apt_clear_guard_cells(apt_ctx *ctx) {

@@ -310,4 +310,16 @@ hipError_t launch_energy_histogram_f64(const double *E, const uint32_t *flag,
                                        double E_max, int flag_bit,
                                        hipStream_t st);
 
+// per-cell moments of collect_diagnostics (kern_diag.hip): num[s], gamma[s]
+// for the four species slots and photon_num, ncell values each
+template <typename R>
+struct DiagArrays {
+  R *num[4], *gamma[4];
+  R *photon_num;
+};
+template <typename R>
+hipError_t launch_diagnostics(const PtcView<R> &p, uint64_t n, const PhoView<R> &ph,
+                              uint64_t nph, const DiagArrays<R> &d, int ns,
+                              uint64_t ncell, hipStream_t st);
+
 }  // namespace apt

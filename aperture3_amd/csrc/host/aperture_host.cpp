@@ -70,7 +70,8 @@ void field_ref::initialize(
 void field_ref::initialize() {
   // vector fields have three components, the scalar fields one
   const bool scalar = m_which == APT_FIELD_DIVE || m_which == APT_FIELD_DIVB ||
-                      m_which >= APT_FIELD_EDOTB;
+                      (m_which >= APT_FIELD_EDOTB && m_which <= APT_FIELD_PAIR_PRODUCED) ||
+                      m_which == APT_FIELD_PHOTON_NUM;
   for (int c = 0; c < (scalar ? 1 : 3); c++)
     initialize(c, [](Scalar, Scalar, Scalar) { return 0.0; });
 }
@@ -103,7 +104,10 @@ sim_data::sim_data(sim_environment &e)
       divB(this, APT_FIELD_DIVB),
       EdotB(this, APT_FIELD_EDOTB),
       photon_produced(this, APT_FIELD_PHOTON_PRODUCED),
-      pair_produced(this, APT_FIELD_PAIR_PRODUCED) {
+      pair_produced(this, APT_FIELD_PAIR_PRODUCED),
+      ptc_num(this, APT_FIELD_PTC_NUM),
+      gamma(this, APT_FIELD_GAMMA),
+      photon_num(this, APT_FIELD_PHOTON_NUM) {
   const SimParams &p = env.params();
   apt_grid g{};
   apt_params q{};
@@ -320,6 +324,10 @@ uint64_t batch_ensemble::pushes() {
   uint64_t t = 0;
   check(apt_batch_pushes(m_b, &t), "apt_batch_pushes");
   return t;
+}
+
+void AdditionalDiagnostics::collect_diagnostics(sim_data &data) {
+  check(data.ctx, apt_collect_diagnostics(data.ctx), "apt_collect_diagnostics");
 }
 
 }  // namespace Aperture

@@ -128,6 +128,19 @@ struct sim_data {
   field_ref E, B, Ebg, Bbg, J, divE, divB;
   // sim_data.h:57-59; zero them after output with initialize()
   field_ref EdotB, photon_produced, pair_produced;
+  // sim_data.h:45-46, 60 (component = species), filled by
+  // AdditionalDiagnostics::collect_diagnostics
+  field_ref ptc_num, gamma, photon_num;
+};
+
+// AdditionalDiagnostics (cuda/core/additional_diagnostics.h)
+class AdditionalDiagnostics {
+ public:
+  explicit AdditionalDiagnostics(sim_environment &env) : m_env(env) {}
+  void collect_diagnostics(sim_data &data);
+
+ private:
+  sim_environment &m_env;
 };
 
 class ptc_updater {

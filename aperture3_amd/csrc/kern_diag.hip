@@ -1,0 +1,87 @@
+// kern_diag.hip -- per-cell particle moments: the summed weight and the
+// weight-averaged Lorentz factor of every species, and the summed photon
+// weight (AdditionalDiagnostics::collect_diagnostics,
+// cuda/core/additional_diagnostics.cu:14-100).  HBM-bound scatter: each live
+// particle reads cell, flag, weight and gamma (or u) and adds two values to
+// its cell with device atomics, as the reference does.
+#include "apt_common.hpp"
+#include "apt_kernels.hpp"
+
+namespace apt {
+
+template <typename R>
+__device__ inline R diag_gamma(const PtcView<R> &p, uint64_t i) {
+  if (p.E) return p.E[i];
+  const R a = p.p1[i], b = p.p2[i], c = p.p3[i];
+  return sqrt((R)1 + a * a + b * b + c * c);
+}
+
+// collect_ptc_diagnostics (:14-41): num[s][c] += w, gam[s][c] += w gamma
+template <typename R>
+__global__ void k_diag_ptc(PtcView<R> p, uint64_t n, DiagArrays<R> d) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t c = p.cell[i];
+    if (c == MAX_CELL) continue;
+    const int s = (int)(p.flag[i] >> 29);
+    const R w = p.w[i];
+    atomicAdd(&d.num[s][c], w);
+    atomicAdd(&d.gamma[s][c], w * diag_gamma(p, i));
+  }
+}
+
+// collect_photon_diagnostics (:43-60): photon_num[c] += w
+template <typename R>
+__global__ void k_diag_photons(PhoView<R> ph, uint64_t n, R *__restrict__ num) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t c = ph.cell[i];
+    if (c == MAX_CELL) continue;
+    atomicAdd(&num[c], ph.w[i]);
+  }
+}
+
+// gamma[s].divideBy(ptc_num[s]) with Op_DivAssign (core/detail/op.hpp:42-49):
+// 0 where |num| < 1e-5, else gamma / num
+template <typename R>
+__global__ void k_diag_divide(DiagArrays<R> d, int ns, uint64_t ncell) {
+  const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  for (int s = 0; s < ns; s++) {
+    const R v = d.num[s][c];
+    d.gamma[s][c] = fabs(v) < (R)1.0e-5 ? (R)0 : d.gamma[s][c] / v;
+  }
+}
+
+template <typename R>
+hipError_t launch_diagnostics(const PtcView<R> &p, uint64_t n, const PhoView<R> &ph,
+                              uint64_t nph, const DiagArrays<R> &d, int ns,
+                              uint64_t ncell, hipStream_t st) {
+  for (int s = 0; s < 4; s++) {
+    hipError_t e = launch_fill<R>(d.num[s], (R)0, ncell, st);
+    if (e == hipSuccess) e = launch_fill<R>(d.gamma[s], (R)0, ncell, st);
+    if (e != hipSuccess) return e;
+  }
+  hipError_t e = launch_fill<R>(d.photon_num, (R)0, ncell, st);
+  if (e != hipSuccess) return e;
+  // grid-stride over the particles: a few waves per CU fill the chip
+  auto blocks = [](uint64_t m) {
+    uint64_t b = (m + 255) / 256;
+    return (unsigned)(b < 4096 ? (b ? b : 1) : 4096);
+  };
+  if (n > 0) k_diag_ptc<R><<<blocks(n), 256, 0, st>>>(p, n, d);
+  if (nph > 0 && ph.cell) k_diag_photons<R><<<blocks(nph), 256, 0, st>>>(ph, nph, d.photon_num);
+  k_diag_divide<R><<<(unsigned)((ncell + 255) / 256), 256, 0, st>>>(d, ns, ncell);
+  return hipGetLastError();
+}
+
+template hipError_t launch_diagnostics<double>(const PtcView<double> &, uint64_t,
+                                               const PhoView<double> &, uint64_t,
+                                               const DiagArrays<double> &, int, uint64_t,
+                                               hipStream_t);
+template hipError_t launch_diagnostics<float>(const PtcView<float> &, uint64_t,
+                                              const PhoView<float> &, uint64_t,
+                                              const DiagArrays<float> &, int, uint64_t,
+                                              hipStream_t);
+
+}  // namespace apt

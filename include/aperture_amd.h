@@ -75,7 +75,14 @@ enum {
    * the particles that emitted and the photons that converted, per cell
    * (radiative_transfer.cu:48-57, 120-130); zeroed the same way */
   APT_FIELD_PHOTON_PRODUCED = 9,
-  APT_FIELD_PAIR_PRODUCED = 10
+  APT_FIELD_PAIR_PRODUCED = 10,
+  /* the moments of apt_collect_diagnostics (sim_data.h:45-46, 60):
+   * summed weight per species (component = species), weight-averaged
+   * gamma per species, summed photon weight; readable after the first
+   * collection */
+  APT_FIELD_PTC_NUM = 11,
+  APT_FIELD_GAMMA = 12,
+  APT_FIELD_PHOTON_NUM = 13
 };
 
 /* Timed kernels (apt_timing_get). */
@@ -284,6 +291,14 @@ apt_status apt_field_compute_divs(apt_ctx *ctx);
 apt_status apt_sort_particles(apt_ctx *ctx);
 /* particle_base::clear_guard_cells (particle_base_impl.cu:489-493) */
 apt_status apt_clear_guard_cells(apt_ctx *ctx);
+/* AdditionalDiagnostics::collect_diagnostics
+ * (cuda/core/additional_diagnostics.cu:78-100): per cell and species the
+ * summed weight (APT_FIELD_PTC_NUM) and the weight-averaged gamma
+ * (APT_FIELD_GAMMA: 0 where the summed weight is below 1e-5, Op_DivAssign,
+ * core/detail/op.hpp:42-49), and the summed photon weight
+ * (APT_FIELD_PHOTON_NUM).  Indexed by the full cell in 3D (the reference
+ * indexes (c1, c2) only).  Sums are device atomics, as in the reference. */
+apt_status apt_collect_diagnostics(apt_ctx *ctx);
 /* Kernels::compute_energy_histogram (cuda/kernels.cu:36-69); hist has nbins
  * entries and is overwritten.  flag_bit < 0: every particle slot. */
 apt_status apt_energy_histogram(apt_ctx *ctx, uint32_t *hist, int nbins,

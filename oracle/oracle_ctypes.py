@@ -303,6 +303,21 @@ class OracleSim:
                                       C.c_int(nbins), C.c_double(E_max), C.c_int(flag_bit))
         return h
 
+    def collect_diagnostics(self, num_species=None):
+        """collect_diagnostics: (num[4], gamma[4], photon_num) per cell."""
+        ns = self.sp.num_species if num_species is None else num_species
+        num = [np.zeros(self.ncell) for _ in range(4)]
+        gam = [np.zeros(self.ncell) for _ in range(4)]
+        phn = np.zeros(self.ncell)
+        P = self._ptc_struct()
+        Ph = self._ph_struct() if self.ph_number else None
+        nump = (_PD * 4)(*[_d(a) for a in num])
+        gamp = (_PD * 4)(*[_d(a) for a in gam])
+        self.lib.orc_collect_diagnostics(C.byref(P), C.byref(Ph) if Ph is not None else None,
+                                         nump, gamp, _d(phn), C.c_int(ns),
+                                         C.c_uint64(self.ncell))
+        return num, gam, phn
+
 
 class LogSphOracle(OracleSim):
     """2D log-spherical (ln r, theta) state: the Cartesian state plus the 13

@@ -825,3 +825,29 @@ void orc_energy_histogram(const orc_ptc *p, uint32_t *hist, int nbins,
     hist[idx] += 1;
   }
 }
+
+/* collect_ptc_diagnostics + collect_photon_diagnostics + divideBy
+ * (cuda/core/additional_diagnostics.cu:14-100, Op_DivAssign in
+ * core/detail/op.hpp:42-49): per cell and species the summed weight and the
+ * summed w gamma, then gamma = 0 where |num| < 1e-5, else gamma / num; the
+ * summed photon weight.  Arrays are zeroed first; ph may be NULL. */
+void orc_collect_diagnostics(const orc_ptc *p, const orc_photons *ph, double **num,
+                             double **gamma, double *photon_num, int ns,
+                             uint64_t ncell) {
+  for (int s = 0; s < 4; s++)
+    for (uint64_t c = 0; c < ncell; c++) num[s][c] = gamma[s][c] = 0.0;
+  for (uint64_t c = 0; c < ncell; c++) photon_num[c] = 0.0;
+  for (uint64_t i = 0; i < p->number; i++) {
+    uint32_t c = p->cell[i];
+    if (c == ORC_MAX_CELL) continue;
+    int s = (int)(p->flag[i] >> 29);
+    num[s][c] += p->w[i];
+    gamma[s][c] += p->w[i] * p->E[i];
+  }
+  if (ph)
+    for (uint64_t i = 0; i < ph->number; i++)
+      if (ph->cell[i] != ORC_MAX_CELL) photon_num[ph->cell[i]] += ph->w[i];
+  for (int s = 0; s < ns; s++)
+    for (uint64_t c = 0; c < ncell; c++)
+      gamma[s][c] = fabs(num[s][c]) < 1.0e-5 ? 0.0 : gamma[s][c] / num[s][c];
+}

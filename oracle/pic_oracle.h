@@ -207,6 +207,9 @@ uint64_t orc_inject_left(const orc_mesh *m, orc_ptc *p, int n_pairs,
 /* --- diagnostics --- */
 void orc_energy_histogram(const orc_ptc *p, uint32_t *hist, int nbins,
                           double E_max, int flag_bit);
+void orc_collect_diagnostics(const orc_ptc *p, const orc_photons *ph, double **num,
+                             double **gamma, double *photon_num, int ns,
+                             uint64_t ncell);
 
 #ifdef __cplusplus
 }
