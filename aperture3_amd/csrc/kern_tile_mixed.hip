@@ -476,16 +476,227 @@ __global__ void __launch_bounds__(NT)
 // 3D tiled push + deposit: fields gathered through L1/L2, current in an
 // LDS int64 window.
 // ---------------------------------------------------------------------
+struct P3 {
+  uint32_t cell, flag;
+  float x1, x2, x3, u1, u2, u3, w;
+};
+
+// set on a particle's flag inside one launch only: pushed after the tile
+// (bit 27 is unused by the reference's ParticleFlag bits 0-9 and the
+// species bits 29-31)
+constexpr uint32_t FLAG_DEFER = 1u << 27;
+
+// Push and move one particle (Vay, ptc_updater.cu:21-80 and the 3D move of
+// :400-440), write it back, and leave the old/new positions and the cell
+// crossing for the deposit.  Returns the species.
+struct Move3 {
+  float x[3], n[3];
+  int dc[3];
+  float v3;
+};
+
+__device__ __forceinline__ void push_move_3d(const Mesh &m, const float *qm_tab, float dt,
+                                             const float idx[3], const PtcF32 &p,
+                                             uint32_t i, const P3 &r, int c1, int c2,
+                                             int c3, const float F[6], bool &multi,
+                                             Move3 &mv) {
+  const int s = ptc_species(r.flag);
+  float u1 = r.u1, u2 = r.u2, u3 = r.u3;
+  const float gamma = hw_sqrt(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
+  float ig;
+  if (!(r.flag & FLAG_IGNORE_EM)) {
+    const float qm = qm_tab[s];
+    ig = vay_push_f(u1, u2, u3, gamma, F[0] * qm, F[1] * qm, F[2] * qm, F[3] * qm,
+                    F[4] * qm, F[5] * qm);
+    p.u1[i] = u1;
+    p.u2[i] = u2;
+    p.u3[i] = u3;
+  } else {
+    ig = hw_rcp(gamma);
+  }
+  const float v[3] = {u1 * ig, u2 * ig, u3 * ig};
+  mv.x[0] = r.x1;
+  mv.x[1] = r.x2;
+  mv.x[2] = r.x3;
+  multi = false;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    float n = mv.x[d] + v[d] * idx[d];
+    int dc = (int)floorf(n);
+    n -= (float)dc;
+    if (n >= 1.0f) { n -= 1.0f; dc += 1; }
+    if (dc > 1 || dc < -1) {
+      multi = true;
+      dc = max(-1, min(1, dc));
+    }
+    mv.n[d] = n;
+    mv.dc[d] = dc;
+  }
+  p.cell[i] = (uint32_t)m.idx(m.wrap(0, c1 + mv.dc[0]), m.wrap(1, c2 + mv.dc[1]),
+                              m.wrap(2, c3 + mv.dc[2]));
+  p.x1[i] = mv.n[0];
+  p.x2[i] = mv.n[1];
+  p.x3[i] = mv.n[2];
+  mv.v3 = v[2];
+}
+
+// Esirkepov in separable form.  Along each axis the shapes live on the 3
+// positions t = 0..2 starting at b = (dc == -1 ? 0 : 1) (offsets b + t - 2);
+// the cumulative shape difference D(t) is non-zero only for t <= |dc|, the
+// transverse weights only for t < 2 + |dc|.  With
+//   c_yz(j,k) = (sz1 (2 sy1 + sy0) + sz0 (sy1 + 2 sy0)) / 6 (likewise c_xz,
+//   c_xy), J1(t,j,k) = -q w Dx(t) c_yz(j,k), J2 = -q w Dy(t) c_xz(i,k),
+// J3 = -q w Dz(t) c_xy(i,j) (ptc_updater.cu:445-487): the sums of the
+// cumulative loops.  The 54 candidate terms are split into a core (12
+// terms, every particle) and one group per crossed axis (21, 13 and 8
+// terms): a term outside its particle's support is an exact zero (the
+// shape at position 2 is 0 without a crossing and D(1) is set to 0), so a
+// group adds a few zeros instead of testing each term.
+//
+// add(comp, i, j, k, q) receives window offsets from (b - 2) per axis.
+template <class Add>
+__device__ __forceinline__ void deposit_3d(const Move3 &mv, float qs, Add add) {
+  float s0[3][3], s1[3][3], D1[3], D0[3], h[3][3], g[3][3];
+  int b[3], ac[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int dc = mv.dc[d];
+    b[d] = dc == -1 ? 0 : 1;
+    ac[d] = dc != 0;
+    const float x = mv.x[d], n = mv.n[d];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      const int q = b[d] + t;
+      s0[d][t] = q == 1 ? 1.0f - x : (q == 2 ? x : 0.0f);
+      s1[d][t] = (q == 1 + dc) ? 1.0f - n : ((q == 2 + dc) ? n : 0.0f);
+      h[d][t] = 2.0f * s1[d][t] + s0[d][t];
+      g[d][t] = s1[d][t] + 2.0f * s0[d][t];
+    }
+    D0[d] = qs * (s1[d][0] - s0[d][0]);
+    D1[d] = ac[d] ? qs * (s1[d][0] - s0[d][0] + (s1[d][1] - s0[d][1])) : 0.0f;
+  }
+  // transverse weights: cx(u,v) over (y,z) for J1, (x,z) for J2, (x,y) for J3
+  auto cyz = [&](int u, int v) { return s1[2][v] * h[1][u] + s0[2][v] * g[1][u]; };
+  auto cxz = [&](int u, int v) { return s1[2][v] * h[0][u] + s0[2][v] * g[0][u]; };
+  auto cxy = [&](int u, int v) { return s1[1][v] * h[0][u] + s0[1][v] * g[0][u]; };
+  // J1(t,u,v) at (t+1, u, v); J2(t,u,v) at (u, t+1, v); J3(t,u,v) at (u, v, t+1)
+  auto j1 = [&](int t, int u, int v) {
+    add(0, t + 1, u, v, fxi((t ? D1[0] : D0[0]) * cyz(u, v)));
+  };
+  auto j2 = [&](int t, int u, int v) {
+    add(1, u, t + 1, v, fxi((t ? D1[1] : D0[1]) * cxz(u, v)));
+  };
+  auto j3 = [&](int t, int u, int v) {
+    add(2, u, v, t + 1, fxi((t ? D1[2] : D0[2]) * cxy(u, v)));
+  };
+#pragma unroll
+  for (int u = 0; u < 2; u++)
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      j1(0, u, v);
+      j2(0, u, v);
+      j3(0, u, v);
+    }
+  if (ac[0]) {  // x crossed: J1 at t = 1, J2 and J3 at u = 2
+#pragma unroll
+    for (int u = 0; u < 3; u++)
+#pragma unroll
+      for (int v = 0; v < 3; v++) j1(1, u, v);
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+      for (int v = 0; v < 3; v++) {
+        j2(t, 2, v);
+        j3(t, 2, v);
+      }
+  }
+  if (ac[1]) {  // y crossed: J1 at u = 2, J2 at t = 1, J3 at v = 2 (u < 2)
+#pragma unroll
+    for (int v = 0; v < 3; v++) j1(0, 2, v);
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+#pragma unroll
+      for (int v = 0; v < 3; v++) j2(1, u, v);
+#pragma unroll
+      for (int t = 0; t < 2; t++) j3(t, u, 2);
+    }
+  }
+  if (ac[2]) {  // z crossed: J1 and J2 at v = 2 (u < 2), J3 at t = 1
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      j1(0, u, 2);
+      j2(0, u, 2);
+#pragma unroll
+      for (int v = 0; v < 2; v++) j3(1, u, v);
+    }
+  }
+}
+
+// The whole update of one particle through global memory: taps from the
+// field arrays, current by global int64 atomics.  For the particles whose
+// stencil left the tile's LDS window, queued during the tile and pushed
+// after it.
+__device__ __forceinline__ void push_global_3d(const Mesh &m, const float *qm_tab,
+                                            const float *q_tab, float dt,
+                                            const float idx[3], float scale,
+                                            const PtcF32 &p, const FieldsF32 &f,
+                                            const AccMixed &a, uint32_t i,
+                                            unsigned long long *counters) {
+  P3 r;
+  r.cell = p.cell[i];
+  r.flag = p.flag[i];
+  r.x1 = p.x1[i];
+  r.x2 = p.x2[i];
+  r.x3 = p.x3[i];
+  r.u1 = p.u1[i];
+  r.u2 = p.u2[i];
+  r.u3 = p.u3[i];
+  r.w = p.w[i];
+  int c1, c2, c3;
+  m.split(r.cell, c1, c2, c3);
+  const int64_t d0 = m.dims[0], d01 = (int64_t)m.dims[0] * m.dims[1];
+  const float F[6] = {
+      gather3g(f.E[0], d0, d01, c1, c2, c3, r.x1, r.x2, r.x3, ST_E1),
+      gather3g(f.E[1], d0, d01, c1, c2, c3, r.x1, r.x2, r.x3, ST_E2),
+      gather3g(f.E[2], d0, d01, c1, c2, c3, r.x1, r.x2, r.x3, ST_E3),
+      gather3g(f.B[0], d0, d01, c1, c2, c3, r.x1, r.x2, r.x3, ST_B1),
+      gather3g(f.B[1], d0, d01, c1, c2, c3, r.x1, r.x2, r.x3, ST_B2),
+      gather3g(f.B[2], d0, d01, c1, c2, c3, r.x1, r.x2, r.x3, ST_B3)};
+  bool multi;
+  Move3 mv;
+  push_move_3d(m, qm_tab, dt, idx, p, i, r, c1, c2, c3, F, multi, mv);
+  if (multi) atomicAdd(&counters[CTR_MULTICELL], 1ull);
+  if (r.flag & FLAG_IGNORE_CURRENT) return;
+  const int s = ptc_species(r.flag);
+  const float qs = q_tab[s] * r.w * scale * (-1.0f / 6.0f);
+  const int gx0 = c1 + (mv.dc[0] == -1 ? 0 : 1) - 2;
+  const int gy0 = c2 + (mv.dc[1] == -1 ? 0 : 1) - 2;
+  const int gz0 = c3 + (mv.dc[2] == -1 ? 0 : 1) - 2;
+  deposit_3d(mv, qs, [&](int comp, int di, int dj, int dk, long long q) {
+    glb_add(a.J[comp] + m.widx(gx0 + di, gy0 + dj, gz0 + dk), q);
+  });
+}
+
 template <int T, int H, int NT>
 __global__ void __launch_bounds__(NT)
     k_push_deposit_tile3d(Mesh m, Species sp, float dt, float scale,
                           PtcF32 p, FieldsF32 f, AccMixed a, TileInfo ti,
                           unsigned long long *__restrict__ counters) {
   constexpr int W = T + 2 * H, WN = W * W * W;
+  constexpr int QCAP = 512;
   __shared__ unsigned long long sJ[3][WN];
   __shared__ float s_qm[8];  // dt q / (2 m) per species
-  if (threadIdx.x < 8)
+  __shared__ float s_q[8];
+  __shared__ uint32_t s_queue[QCAP];
+  __shared__ uint32_t s_nq, s_ovf;
+  if (threadIdx.x < 8) {
     s_qm[threadIdx.x] = (float)(0.5 * sp.q[threadIdx.x] / sp.m[threadIdx.x]) * dt;
+    s_q[threadIdx.x] = (float)sp.q[threadIdx.x];
+  }
+  if (threadIdx.x == 0) {
+    s_nq = 0;
+    s_ovf = 0;
+  }
 
   const uint32_t t = xcd_tile(blockIdx.x, ti.ntiles);
   const int tx = (int)(t % (uint32_t)ti.nt[0]);
@@ -504,17 +715,12 @@ __global__ void __launch_bounds__(NT)
   __syncthreads();
 
   const uint32_t beg = ti.tile_off[t], end = ti.tile_off[t + 1];
-  const float idx0 = (float)m.inv_delta[0] * dt;
-  const float idx1 = (float)m.inv_delta[1] * dt;
-  const float idx2 = (float)m.inv_delta[2] * dt;
-  unsigned long long outside = 0, multi = 0;
+  const float idx[3] = {(float)m.inv_delta[0] * dt, (float)m.inv_delta[1] * dt,
+                        (float)m.inv_delta[2] * dt};
+  unsigned long long outside = 0, multi_n = 0;
   // Software pipeline over the tile's particles: the next particle's
   // arrays are requested before the current one's field taps are used.
   // Loads are straight-line (a lane past the end re-reads the last slot).
-  struct P3 {
-    uint32_t cell, flag;
-    float x1, x2, x3, u1, u2, u3, w;
-  };
   auto load = [&](uint32_t i, P3 &r) {
     const uint32_t j = i < end ? i : end - 1;
     r.cell = p.cell[j];
@@ -529,7 +735,7 @@ __global__ void __launch_bounds__(NT)
   };
   P3 cur, nxt;
   if (beg + threadIdx.x < end) load(beg + threadIdx.x, cur);
-  for (uint32_t idx = beg + threadIdx.x; idx < end; idx += NT) {
+  for (uint32_t i = beg + threadIdx.x; i < end; i += NT) {
     const uint32_t c = cur.cell;
     int c1, c2, c3;
     m.split(c == MAX_CELL ? (uint32_t)m.idx(g0, g1, g2) : c, c1, c2, c3);
@@ -539,134 +745,73 @@ __global__ void __launch_bounds__(NT)
       c2 = g1;
       c3 = g2;
     }
-    const uint32_t flag = cur.flag;
-    const int s = ptc_species(flag);
-    const float x1 = cur.x1, x2 = cur.x2, x3 = cur.x3;
-    float u1 = cur.u1, u2 = cur.u2, u3 = cur.u3;
-    const float w = cur.w;
     // field taps of the current particle (issued before the prefetch)
-    const float E1 = gather3g(f.E[0], d0, d01, c1, c2, c3, x1, x2, x3, ST_E1);
-    const float E2 = gather3g(f.E[1], d0, d01, c1, c2, c3, x1, x2, x3, ST_E2);
-    const float E3 = gather3g(f.E[2], d0, d01, c1, c2, c3, x1, x2, x3, ST_E3);
-    const float B1 = gather3g(f.B[0], d0, d01, c1, c2, c3, x1, x2, x3, ST_B1);
-    const float B2 = gather3g(f.B[1], d0, d01, c1, c2, c3, x1, x2, x3, ST_B2);
-    const float B3 = gather3g(f.B[2], d0, d01, c1, c2, c3, x1, x2, x3, ST_B3);
-    load(idx + NT, nxt);
-    if (c != MAX_CELL && !live) p.cell[idx] = MAX_CELL;
+    const float F[6] = {
+        gather3g(f.E[0], d0, d01, c1, c2, c3, cur.x1, cur.x2, cur.x3, ST_E1),
+        gather3g(f.E[1], d0, d01, c1, c2, c3, cur.x1, cur.x2, cur.x3, ST_E2),
+        gather3g(f.E[2], d0, d01, c1, c2, c3, cur.x1, cur.x2, cur.x3, ST_E3),
+        gather3g(f.B[0], d0, d01, c1, c2, c3, cur.x1, cur.x2, cur.x3, ST_B1),
+        gather3g(f.B[1], d0, d01, c1, c2, c3, cur.x1, cur.x2, cur.x3, ST_B2),
+        gather3g(f.B[2], d0, d01, c1, c2, c3, cur.x1, cur.x2, cur.x3, ST_B3)};
+    load(i + NT, nxt);
+    if (c != MAX_CELL && !live) p.cell[i] = MAX_CELL;
     if (live) {
-    int l1 = c1 - w0, l2 = c2 - w1, l3 = c3 - w2;
-    if (m.periodic[0]) { if (l1 < -H) l1 += N0; else if (l1 >= N0 - H) l1 -= N0; }
-    if (m.periodic[1]) { if (l2 < -H) l2 += N1; else if (l2 >= N1 - H) l2 -= N1; }
-    if (m.periodic[2]) { if (l3 < -H) l3 += N2; else if (l3 >= N2 - H) l3 -= N2; }
-    const bool in = (l1 >= 2) & (l1 <= W - 3) & (l2 >= 2) & (l2 <= W - 3) &
-                    (l3 >= 2) & (l3 <= W - 3);
-    outside += in ? 0 : 1;
-    const float gamma = hw_sqrt(1.0f + u1 * u1 + u2 * u2 + u3 * u3);
-    float ig;
-    if (!(flag & FLAG_IGNORE_EM)) {
-      const float qm = s_qm[s];
-      ig = vay_push_f(u1, u2, u3, gamma, E1 * qm, E2 * qm, E3 * qm, B1 * qm, B2 * qm,
-                      B3 * qm);
-      p.u1[idx] = u1;
-      p.u2[idx] = u2;
-      p.u3[idx] = u3;
-    } else {
-      ig = hw_rcp(gamma);
-    }
-    const float v1 = u1 * ig, v2 = u2 * ig, v3 = u3 * ig;
-    float n1 = x1 + v1 * idx0, n2 = x2 + v2 * idx1, n3 = x3 + v3 * idx2;
-    int dc1 = (int)floorf(n1), dc2 = (int)floorf(n2), dc3 = (int)floorf(n3);
-    n1 -= (float)dc1;
-    n2 -= (float)dc2;
-    n3 -= (float)dc3;
-    if (n1 >= 1.0f) { n1 -= 1.0f; dc1 += 1; }
-    if (n2 >= 1.0f) { n2 -= 1.0f; dc2 += 1; }
-    if (n3 >= 1.0f) { n3 -= 1.0f; dc3 += 1; }
-    if (dc1 > 1 || dc1 < -1 || dc2 > 1 || dc2 < -1 || dc3 > 1 || dc3 < -1) {
-      multi++;
-      dc1 = max(-1, min(1, dc1));
-      dc2 = max(-1, min(1, dc2));
-      dc3 = max(-1, min(1, dc3));
-    }
-    p.cell[idx] = (uint32_t)m.idx(m.wrap(0, c1 + dc1), m.wrap(1, c2 + dc2),
-                                  m.wrap(2, c3 + dc3));
-    p.x1[idx] = n1;
-    p.x2[idx] = n2;
-    p.x3[idx] = n3;
-    if (!(flag & FLAG_IGNORE_CURRENT)) {
-    // Esirkepov in separable form.  Along each axis the shapes live on the
-    // 3 positions t = 0..2 starting at b = (dc == -1 ? 0 : 1) (offsets
-    // b + t - 2); the cumulative shape difference D(t) is non-zero only for
-    // t <= |dc|, and the transverse weights only for t < 2 + |dc|.  With
-    //   c_yz(j,k) = (sz1 (2 sy1 + sy0) + sz0 (sy1 + 2 sy0)) / 6 (and
-    //   likewise c_xz, c_xy),
-    // J1(t,j,k) = -q w Dx(t) c_yz(j,k), J2 = -q w Dy(t) c_xz(i,k),
-    // J3 = -q w Dz(t) c_xy(i,j) (ptc_updater.cu:445-487): the same sums as
-    // the cumulative loops, with 4 + 4 + 4 adds for a particle that stays
-    // in its cell (up to 18 each when it crosses).
-    const int bx = dc1 == -1 ? 0 : 1, by = dc2 == -1 ? 0 : 1, bz = dc3 == -1 ? 0 : 1;
-    const int ax1 = dc1 != 0, ay1 = dc2 != 0, az1 = dc3 != 0;  // |dc|
-    float sx0[3], sy0[3], sz0[3], sx1[3], sy1[3], sz1[3];
-#pragma unroll
-    for (int t3 = 0; t3 < 3; t3++) {
-      const int qx = bx + t3, qy = by + t3, qz = bz + t3;
-      sx0[t3] = qx == 1 ? 1.0f - x1 : (qx == 2 ? x1 : 0.0f);
-      sy0[t3] = qy == 1 ? 1.0f - x2 : (qy == 2 ? x2 : 0.0f);
-      sz0[t3] = qz == 1 ? 1.0f - x3 : (qz == 2 ? x3 : 0.0f);
-      sx1[t3] = (qx == 1 + dc1) ? 1.0f - n1 : ((qx == 2 + dc1) ? n1 : 0.0f);
-      sy1[t3] = (qy == 1 + dc2) ? 1.0f - n2 : ((qy == 2 + dc2) ? n2 : 0.0f);
-      sz1[t3] = (qz == 1 + dc3) ? 1.0f - n3 : ((qz == 2 + dc3) ? n3 : 0.0f);
-    }
-    const float Dx[2] = {sx1[0] - sx0[0], sx1[0] - sx0[0] + (sx1[1] - sx0[1])};
-    const float Dy[2] = {sy1[0] - sy0[0], sy1[0] - sy0[0] + (sy1[1] - sy0[1])};
-    const float Dz[2] = {sz1[0] - sz0[0], sz1[0] - sz0[0] + (sz1[1] - sz0[1])};
-    float hx[3], gx[3], hy[3], gy[3];
-#pragma unroll
-    for (int t3 = 0; t3 < 3; t3++) {
-      hx[t3] = 2.0f * sx1[t3] + sx0[t3];
-      gx[t3] = sx1[t3] + 2.0f * sx0[t3];
-      hy[t3] = 2.0f * sy1[t3] + sy0[t3];
-      gy[t3] = sy1[t3] + 2.0f * sy0[t3];
-    }
-    // -q w scale / 6, pre-scaled so each add is one multiply and one cvt
-    const float qs = (float)sp.q[s] * w * scale * (-1.0f / 6.0f);
-    const int b0 = (l1 + bx - 2) + ((l2 + by - 2) + (l3 + bz - 2) * W) * W;
-    const int gx0 = c1 + bx - 2, gy0 = c2 + by - 2, gz0 = c3 + bz - 2;
-#pragma unroll
-    for (int u = 0; u < 3; u++) {
-#pragma unroll
-      for (int v = 0; v < 3; v++) {
-        // J1 at (t, j=u, k=v), J2 at (i=u, t, k=v), J3 at (i=u, j=v, t)
-        const float cyz = sz1[v] * hy[u] + sz0[v] * gy[u];
-        const float cxz = sz1[v] * hx[u] + sz0[v] * gx[u];
-        const float cxy = sy1[v] * hx[u] + sy0[v] * gx[u];
-#pragma unroll
-        for (int t = 0; t < 2; t++) {
-          const bool on1 = (t <= ax1) & (u < 2 + ay1) & (v < 2 + az1);
-          const bool on2 = (t <= ay1) & (u < 2 + ax1) & (v < 2 + az1);
-          const bool on3 = (t <= az1) & (u < 2 + ax1) & (v < 2 + ay1);
-          const long long q1 = fxi(qs * Dx[t] * cyz);
-          const long long q2 = fxi(qs * Dy[t] * cxz);
-          const long long q3 = fxi(qs * Dz[t] * cxy);
-          if (in) {
-            if (on1) lds_add(&sJ[0][b0 + t + (u + v * W) * W + 1], q1);
-            if (on2) lds_add(&sJ[1][b0 + u + (t + v * W) * W + W], q2);
-            if (on3) lds_add(&sJ[2][b0 + u + (v + t * W) * W + W * W], q3);
-          } else {
-            if (on1) glb_add(a.J[0] + m.widx(gx0 + t + 1, gy0 + u, gz0 + v), q1);
-            if (on2) glb_add(a.J[1] + m.widx(gx0 + u, gy0 + t + 1, gz0 + v), q2);
-            if (on3) glb_add(a.J[2] + m.widx(gx0 + u, gy0 + v, gz0 + t + 1), q3);
-          }
+      int l1 = c1 - w0, l2 = c2 - w1, l3 = c3 - w2;
+      if (m.periodic[0]) { if (l1 < -H) l1 += N0; else if (l1 >= N0 - H) l1 -= N0; }
+      if (m.periodic[1]) { if (l2 < -H) l2 += N1; else if (l2 >= N1 - H) l2 -= N1; }
+      if (m.periodic[2]) { if (l3 < -H) l3 += N2; else if (l3 >= N2 - H) l3 -= N2; }
+      const bool in = (l1 >= 2) & (l1 <= W - 3) & (l2 >= 2) & (l2 <= W - 3) &
+                      (l3 >= 2) & (l3 <= W - 3);
+      if (!in) {
+        // pushed after the tile through global memory: from the queue, or,
+        // once it is full, marked in the flag word and found by a sweep
+        const uint32_t q = atomicAdd(&s_nq, 1u);
+        if (q < QCAP) {
+          s_queue[q] = i;
+        } else {
+          p.flag[i] = cur.flag | FLAG_DEFER;
+          s_ovf = 1u;
+        }
+        outside++;
+      } else {
+        bool multi;
+        Move3 mv;
+        push_move_3d(m, s_qm, dt, idx, p, i, cur, c1, c2, c3, F, multi, mv);
+        multi_n += multi ? 1 : 0;
+        if (!(cur.flag & FLAG_IGNORE_CURRENT)) {
+          // -q w scale / 6, pre-scaled so each term is two multiplies and a cvt
+          const float qs = s_q[ptc_species(cur.flag)] * cur.w * scale * (-1.0f / 6.0f);
+          const int b0 = (l1 + (mv.dc[0] == -1 ? 0 : 1) - 2) +
+                         ((l2 + (mv.dc[1] == -1 ? 0 : 1) - 2) +
+                          (l3 + (mv.dc[2] == -1 ? 0 : 1) - 2) * W) * W;
+          deposit_3d(mv, qs, [&](int comp, int di, int dj, int dk, long long q) {
+            lds_add(&sJ[comp][b0 + di + (dj + dk * W) * W], q);
+          });
         }
       }
     }
-    }  // current
-    }  // live
     cur = nxt;
   }
-  if (multi) atomicAdd(&counters[CTR_MULTICELL], multi);
+  if (multi_n) atomicAdd(&counters[CTR_MULTICELL], multi_n);
   if (outside) atomicAdd(&counters[CTR_OUTSIDE], outside);
   __syncthreads();
+  // the queued particles, then (if the queue overflowed) every particle of
+  // the tile carrying the deferral mark
+  const uint32_t nq = min(s_nq, (uint32_t)QCAP);
+  const uint32_t nwork = nq + (s_ovf ? end - beg : 0u);
+  for (uint32_t k = threadIdx.x; k < nwork; k += NT) {
+    uint32_t i;
+    bool go = true;
+    if (k < nq) {
+      i = s_queue[k];
+    } else {
+      i = beg + (k - nq);
+      const uint32_t fl = p.flag[i];
+      go = (fl & FLAG_DEFER) != 0;
+      if (go) p.flag[i] = fl & ~FLAG_DEFER;
+    }
+    if (go) push_global_3d(m, s_qm, s_q, dt, idx, scale, p, f, a, i, counters);
+  }
   for (int i = threadIdx.x; i < WN; i += NT) {
     int l1 = i % W, l2 = (i / W) % W, l3 = i / (W * W);
     int64_t o = (int64_t)fold(w0 + l1, g0, N0) +
