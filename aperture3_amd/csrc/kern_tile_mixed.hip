@@ -35,6 +35,11 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t ntiles) {
   return (b & 7u) * per + (b >> 3);
 }
 
+// set on a particle's flag inside one launch only: pushed after the tile
+// (bit 27 is unused by the reference's ParticleFlag bits 0-9 and the
+// species bits 29-31)
+constexpr uint32_t FLAG_DEFER = 1u << 27;
+
 // index of a possibly out-of-range mesh coordinate folded into the bulk
 __device__ __forceinline__ int fold(int i, int g, int N) {
   int r = (i - g) % N;
@@ -319,7 +324,7 @@ __global__ void __launch_bounds__(NT)
   __shared__ unsigned long long sR[RHO ? 2 : 1][RHO ? WN : 1];
   __shared__ float s_qm[8], s_q[8];
   __shared__ uint32_t s_queue[QCAP];
-  __shared__ uint32_t s_nq;
+  __shared__ uint32_t s_nq, s_ovf;
 
   const uint32_t t = xcd_tile(blockIdx.x, ti.ntiles);
   const int tx = (int)(t % (uint32_t)ti.nt[0]);
@@ -333,7 +338,10 @@ __global__ void __launch_bounds__(NT)
     s_qm[threadIdx.x] = spf.qm[threadIdx.x];
     s_q[threadIdx.x] = spf.q[threadIdx.x];
   }
-  if (threadIdx.x == 0) s_nq = 0;
+  if (threadIdx.x == 0) {
+    s_nq = 0;
+    s_ovf = 0;
+  }
   for (int i = threadIdx.x; i < WN; i += NT) {
     int l1 = i % W1, l2 = i / W1;
     int64_t o = (int64_t)fold(w0 + l1, g0, N0) +
@@ -383,12 +391,15 @@ __global__ void __launch_bounds__(NT)
     else if (l2 >= N1 - H) l2 -= N1;
     const bool in = (l1 >= 2) & (l1 <= W1 - 3) & (l2 >= 2) & (l2 <= W2 - 3);
     if (!in) {
+      // pushed after the tile through global memory: from the queue, or,
+      // once it is full, marked in the flag word and found by a sweep
       uint32_t q = atomicAdd(&s_nq, 1u);
-      if (q < QCAP)
+      if (q < QCAP) {
         s_queue[q] = idx;
-      else
-        push_global_2d<RHO>(m, s_qm, s_q, dt, idx0, idx1, scale, p, f, a, idx,
-                            counters);
+      } else {
+        pt.flag[li] = r.flag | FLAG_DEFER;
+        s_ovf = 1u;
+      }
       outside++;
       continue;
     }
@@ -452,11 +463,24 @@ __global__ void __launch_bounds__(NT)
     }
   }
   __syncthreads();
-  // particles outside the window: global gathers and atomics
+  // particles outside the window: global gathers and atomics, the queued
+  // ones, then (if the queue overflowed) the ones carrying the mark
   const uint32_t nq = min(s_nq, (uint32_t)QCAP);
-  for (uint32_t k = threadIdx.x; k < nq; k += NT)
-    push_global_2d<RHO>(m, s_qm, s_q, dt, idx0, idx1, scale, p, f, a,
-                        s_queue[k], counters);
+  const uint32_t nwork = nq + (s_ovf ? cnt : 0u);
+  for (uint32_t k = threadIdx.x; k < nwork; k += NT) {
+    uint32_t i;
+    bool go = true;
+    if (k < nq) {
+      i = s_queue[k];
+    } else {
+      i = beg + (k - nq);
+      const uint32_t fl = p.flag[i];
+      go = (fl & FLAG_DEFER) != 0;
+      if (go) p.flag[i] = fl & ~FLAG_DEFER;
+    }
+    if (go)
+      push_global_2d<RHO>(m, s_qm, s_q, dt, idx0, idx1, scale, p, f, a, i, counters);
+  }
   if (outside) atomicAdd(&counters[CTR_OUTSIDE], outside);
   __syncthreads();
   for (int i = threadIdx.x; i < WN; i += NT) {
@@ -481,10 +505,7 @@ struct P3 {
   float x1, x2, x3, u1, u2, u3, w;
 };
 
-// set on a particle's flag inside one launch only: pushed after the tile
-// (bit 27 is unused by the reference's ParticleFlag bits 0-9 and the
-// species bits 29-31)
-constexpr uint32_t FLAG_DEFER = 1u << 27;
+
 
 // Push and move one particle (Vay, ptc_updater.cu:21-80 and the 3D move of
 // :400-440), write it back, and leave the old/new positions and the cell
