@@ -357,7 +357,9 @@ def measure_c4(apt, steps, warmup):
             "tile": list(tile_ext), "tuned": bool(tuned),
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                         "bytes_per_push": BYTES_PER_PUSH["mixed3d"]},
+                         "bytes_per_push": BYTES_PER_PUSH["mixed3d"],
+                         "traffic": profiled_traffic("push_deposit_c4_3d_mixed.md"),
+                         "traffic_source": "profiles/push_deposit_c4_3d_mixed.md"},
             "counters": counters}
 
 
