@@ -29,7 +29,7 @@ KERNEL_PUSH_DEPOSIT, KERNEL_FIELD, KERNEL_SORT = range(3)
 MAX_CELL = 0xFFFFFFFF
 FLAG_IGNORE_CURRENT = 1 << 3
 FLAG_IGNORE_EM = 1 << 4
-COORD_CARTESIAN, COORD_LOGSPH = 0, 1
+COORD_CARTESIAN, COORD_LOGSPH, COORD_1DGR = 0, 1, 2
 RT_DEFAULT, RT_EXP_PATH = 0, 1
 # the 13 Grid_LogSph coefficient arrays (grid_log_sph.h:45-49), ABI order
 COEF_NAMES = ("l1_e", "l2_e", "l3_e", "l1_b", "l2_b", "l3_b", "A1_e", "A2_e", "A3_e",
@@ -64,7 +64,8 @@ class AptParams(C.Structure):
                 ("track_percent", C.c_double), ("rt_model", C.c_int),
                 ("gamma_thr", C.c_double), ("photon_path", C.c_double),
                 ("E_secondary", C.c_double), ("r_cutoff", C.c_double),
-                ("e_frac", C.c_double), ("max_photon_number", C.c_uint64)]
+                ("e_frac", C.c_double), ("max_photon_number", C.c_uint64),
+                ("spin_a", C.c_double)]
 
 # SimParams names copied into AptParams one to one
 _PARAM_FIELDS = ("compactness", "omega", "gravity_on", "gravity", "rad_cooling_on",
@@ -156,6 +157,8 @@ SIGNATURES = {
     "apt_sort_particles": [_P],
     "apt_clear_guard_cells": [_P],
     "apt_collect_diagnostics": [_P],
+    "apt_1dgr_set_profile": [_P] + [C.POINTER(C.c_double)] * 7 + [C.c_uint64],
+    "apt_1dgr_initial_condition": [_P, C.c_int],
     "apt_energy_histogram": [_P, C.POINTER(C.c_uint32), C.c_int, C.c_double, C.c_int],
     "apt_tile_offsets": [_P, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint32),
                          C.POINTER(C.c_int)],
@@ -274,6 +277,8 @@ class SimParams:
         self.r_cutoff = 4.0
         self.e_frac = 0.2
         self.max_photon_number = 0
+        # 1D GR gap (sim_params.h:63): black hole spin
+        self.a = 0.0
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise AttributeError(f"unknown parameter {k}")
@@ -332,7 +337,9 @@ class sim_data:
         q.random_seed = p.random_seed
         q.tile_halo = p.tile_halo
         q.tile_threads = p.tile_threads
-        q.coord = COORD_LOGSPH if p.coord_system == "LogSpherical" else COORD_CARTESIAN
+        q.coord = {"LogSpherical": COORD_LOGSPH, "1DGR": COORD_1DGR}.get(p.coord_system,
+                                                                       COORD_CARTESIAN)
+        q.spin_a = p.a
         for k in _PARAM_FIELDS:
             v = getattr(p, k)
             setattr(q, k, int(v) if isinstance(v, bool) else v)
@@ -544,6 +551,59 @@ class sim_data:
         self._check(self.lib.apt_timing_get(self.ctx, kernel, C.byref(ms), C.byref(n)),
                     "apt_timing_get")
         return ms.value, n.value
+
+
+# the field-line profile of the 1D GR gap, as the datasets of coef.h5
+GR1D_PROFILE = ("r", "Bu1", "Bu3", "rho", "ju1", "th", "dpsidth")
+# Grid_1dGR coefficient arrays in apt_get_coef order (grid_1dgr.h:17-21)
+GR1D_COEF_NAMES = ("D1", "D2", "D3", "alpha", "theta", "K1", "K1_j", "j0", "agrr", "agrf",
+                   "rho0", "gamma_rr", "gamma_ff", "beta_phi", "B3B1")
+
+
+class Grid_1dGR:
+    """Grid_1dGR::compute_coef (grid_1dgr.cpp:21-175) for a 1D GR context:
+    the profile is a dict of the coef.h5 datasets (GR1D_PROFILE)."""
+
+    @staticmethod
+    def compute_coef(data: sim_data, profile: dict):
+        arrs = [np.ascontiguousarray(profile[k], np.float64) for k in GR1D_PROFILE]
+        n = len(arrs[0])
+        if any(len(a) != n for a in arrs):
+            raise ValueError("profile arrays differ in length")
+        data._check(data.lib.apt_1dgr_set_profile(data.ctx, *[_dptr(a) for a in arrs], n),
+                    "apt_1dgr_set_profile")
+
+    @staticmethod
+    def coefficient(data: sim_data, name: str):
+        k = GR1D_COEF_NAMES.index(name)
+        a = np.empty(data.env.dims[0], np.float64)
+        data._check(data.lib.apt_get_coef(data.ctx, k, _dptr(a), a.size), "apt_get_coef")
+        return a
+
+
+class ptc_updater_1dgr:
+    """ptc_updater_1dgr (algorithms/ptc_updater_1dgr.h): the particle and
+    photon pushers along the field line, with the K-weighted smoothing."""
+
+    def __init__(self, env: sim_environment):
+        self.env = env
+
+    def update_particles(self, data: sim_data, dt: float, step: int = 0):
+        data._check(data.lib.apt_ptc_update(data.ctx, dt, step), "apt_ptc_update")
+
+    def prepare_initial_condition(self, data: sim_data, multiplicity: int):
+        data._check(data.lib.apt_1dgr_initial_condition(data.ctx, multiplicity),
+                    "apt_1dgr_initial_condition")
+
+
+class field_solver_1dgr:
+    """field_solver_1dgr (algorithms/field_solver_1dgr.h): E1 += (j0 - J1) dt."""
+
+    def __init__(self, env: sim_environment):
+        self.env = env
+
+    def update_fields(self, data: sim_data, dt: float, time: float = 0.0):
+        data._check(data.lib.apt_field_update(data.ctx, dt, 0), "apt_field_update")
 
 
 class AdditionalDiagnostics:

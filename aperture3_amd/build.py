@@ -35,6 +35,7 @@ HIP_SOURCES = [
     "kern_ffe.hip",
     "kern_batch.hip",
     "kern_diag.hip",
+    "kern_1dgr.hip",
     "apt_context.hip",
 ]
 HOST_SOURCES = ["host/aperture_host.cpp"]

@@ -9,12 +9,14 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
 #include "../../include/aperture_amd.h"
 #include "apt_common.hpp"
 #include "apt_kernels.hpp"
+#include "apt_math.hpp"
 #include "apt_sort.hpp"
 #include "apt_tiles.hpp"
 
@@ -63,6 +65,12 @@ struct apt_ctx {
 
   // log-spherical path
   bool logsph = false;
+  // 1D GR gap (Grid_1dGR): coefficient arrays on the device and their host
+  // copy, and the reference's float parameters a, omega, B0, track_percent
+  bool gr1d = false;
+  double *gr_dev[GR_NCOEF] = {};
+  std::vector<double> gr_host[GR_NCOEF];
+  float gr_a = 0.0f, gr_omega = 0.0f, gr_B0 = 0.0f, gr_track = 0.0f;
   LsParams lp{};
   LsCoef coef{};
   double *coef_mem[LS_NCOEF] = {};
@@ -388,9 +396,33 @@ apt_status apt_create(const apt_grid *grid, const apt_params *params,
     return st;
   }
   ctx->logsph = params->coord == APT_COORD_LOGSPH;
-  if (params->coord != APT_COORD_CARTESIAN && !ctx->logsph) {
+  ctx->gr1d = params->coord == APT_COORD_1DGR;
+  if (params->coord != APT_COORD_CARTESIAN && !ctx->logsph && !ctx->gr1d) {
     apt_destroy(ctx);
     return APT_ERR_INVALID_ARG;
+  }
+  if (ctx->gr1d) {
+    // one field line, FP64 form; the stencils reach two cells past the
+    // particle's cell (ptc_updater_1dgr.cu:172-175, 261-262)
+    if (m.dim != 1 || m.guard[0] < 3 || ctx->prec != APT_PREC_FP64) {
+      apt_destroy(ctx);
+      return APT_ERR_INVALID_ARG;
+    }
+    ctx->gr_a = (float)params->spin_a;
+    ctx->gr_omega = (float)params->omega;
+    ctx->gr_B0 = (float)params->B0;
+    ctx->gr_track = (float)params->track_percent;
+    bool okc = true;
+    for (int k = 0; k < GR_NCOEF && okc; k++) {
+      ctx->gr_host[k].assign(m.dims[0], 0.0);
+      okc = hipMalloc(&ctx->gr_dev[k], (size_t)m.dims[0] * 8) == hipSuccess &&
+            hipMemcpy(ctx->gr_dev[k], ctx->gr_host[k].data(), (size_t)m.dims[0] * 8,
+                      hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (!okc) {
+      apt_destroy(ctx);
+      return APT_ERR_HIP;
+    }
   }
   if (ctx->logsph) {
     // 2D: the particle path and Yee solver (field_solver_logsph.cu:357-358);
@@ -467,6 +499,7 @@ apt_status apt_create(const apt_grid *grid, const apt_params *params,
 apt_status apt_destroy(apt_ctx *ctx) {
   CHECK_CTX(ctx);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (int k = 0; k < GR_NCOEF; k++) (void)hipFree(ctx->gr_dev[k]);
   for (int c = 0; c < 3; c++) {
     (void)hipFree(ctx->E[c]); (void)hipFree(ctx->B[c]); (void)hipFree(ctx->Ebg[c]);
     (void)hipFree(ctx->Bbg[c]); (void)hipFree(ctx->J[c]);
@@ -563,6 +596,12 @@ apt_status apt_set_counters(apt_ctx *ctx, const apt_info *in) {
 
 apt_status apt_get_coef(apt_ctx *ctx, int which, double *dst, uint64_t count) {
   CHECK_CTX(ctx);
+  if (ctx->gr1d) {
+    if (which < 0 || which >= GR_NCOEF || !dst || count != (uint64_t)ctx->mesh.dims[0])
+      return ctx->fail(APT_ERR_INVALID_ARG, "get_coef: bad coefficient/size");
+    std::copy(ctx->gr_host[which].begin(), ctx->gr_host[which].end(), dst);
+    return APT_OK;
+  }
   if (!ctx->logsph)
     return ctx->fail(APT_ERR_UNSUPPORTED, "get_coef: not a log-spherical context");
   const uint64_t n2 = (uint64_t)ctx->mesh.dims[0] * ctx->mesh.dims[1];
@@ -1074,8 +1113,213 @@ static apt_status logsph_ptc_update(apt_ctx *ctx, double dt, int rho) {
   return move_photons(ctx, dt);
 }
 
+// ---- 1D GR gap -------------------------------------------------------
+// Grid_1dGR::compute_coef (grid_1dgr.cpp:21-175) on the host: the profile
+// arrays are taken as float, as the reference reads coef.h5
+// (grid_1dgr.cpp:60-66); the build's own exp/sin/cos keep the result
+// identical to the CPU restatement.
+static double gr_sq(double x) { return x * x; }
+
+static bool gr1d_coef_host(const Mesh &m, const float *vr, const float *vB1,
+                           const float *vB3, const float *vrho, const float *vjr,
+                           const float *vth, const float *vdpsi, uint64_t nprof,
+                           double a, double omega, double B0, std::vector<double> *coef) {
+  const int n_size = m.dims[0];
+  for (int k = 0; k < GR_NCOEF; k++) coef[k].assign(n_size, 0.0);
+  auto r_of_xi = [&](double xi) {
+    const double rp = 1.0 + sqrt(1.0 - a * a), rm = 1.0 - sqrt(1.0 - a * a);
+    const double e = apt_exp(xi * (rp - rm));
+    return (rp - rm * e) / (1.0 - e);
+  };
+  uint64_t n_data = 0, n_mid = 0;
+  for (int n = 0; n < n_size; n++) {
+    const double xi = m.lower[0] + (double)(n - m.guard[0] + 1) * m.delta[0];
+    const double r = r_of_xi(xi);
+    while (n_data + 1 < nprof && vr[n_data + 1] < r) n_data += 1;
+    if (n_data + 1 >= nprof) return false;
+    const double x = (r - vr[n_data]) / (vr[n_data + 1] - vr[n_data]);
+    const double B1 = vB1[n_data] * (1.0 - x) + vB1[n_data + 1] * x;
+    const double B3 = vB3[n_data] * (1.0 - x) + vB3[n_data + 1] * x;
+    const double theta = vth[n_data] * (1.0 - x) + vth[n_data + 1] * x;
+    const double b31 = B3 / B1;
+    coef[GR_B3B1][n] = b31;
+    coef[GR_RHO0][n] = B0 * (vrho[n_data] * (1.0 - x) + vrho[n_data + 1] * x);
+    double st, ct;
+    apt_sincos(theta, st, ct);
+    double Sigma = r * r + a * a * gr_sq(ct);
+    double Delta = r * r - 2.0 * r + a * a;
+    double A = gr_sq(r * r + a * a) - Delta * a * a * gr_sq(st);
+    const double w = 2.0 * a * r / A;
+    double g11 = Sigma / Delta;
+    double g22 = Sigma;
+    double g33 = A * gr_sq(st) / Sigma;
+    double sqrt_gamma = sqrt(g11 * g22 * g33);
+    coef[GR_K1][n] = Delta * sqrt_gamma / (vdpsi[n_data] * (1.0 - x) + vdpsi[n_data + 1] * x);
+    coef[GR_D1][n] = (b31 * g33 * (omega - w)) * Delta;
+    coef[GR_D2][n] = (g11 + b31 * b31 * g33) * Delta * Delta;
+    coef[GR_D3][n] = g33 * gr_sq(omega - w);
+    coef[GR_ALPHA][n] = sqrt(Delta * Sigma / A);
+    coef[GR_GRR][n] = 1.0 / g11;
+    coef[GR_GFF][n] = 1.0 / g33;
+    coef[GR_BETA][n] = -2.0 * a * r / A;
+
+    const double xi_mid = m.lower[0] + ((double)(n - m.guard[0]) + 0.5) * m.delta[0];
+    const double r_mid = r_of_xi(xi_mid);
+    while (n_mid + 1 < nprof && vr[n_mid + 1] < r_mid) n_mid += 1;
+    if (n_mid + 1 >= nprof) return false;
+    // the mid-point quantities are interpolated with the node r, as in
+    // the reference (grid_1dgr.cpp:133)
+    const double y = (r - vr[n_mid]) / (vr[n_mid + 1] - vr[n_mid]);
+    coef[GR_J0][n] = B0 * (vjr[n_mid] * (1.0 - y) + vjr[n_mid + 1] * y) / Delta;
+    const double theta_mid = vth[n_mid] * (1.0 - y) + vth[n_mid + 1] * y;
+    apt_sincos(theta_mid, st, ct);
+    Sigma = r_mid * r_mid + a * a * gr_sq(ct);
+    Delta = r_mid * r_mid - 2.0 * r_mid + a * a;
+    A = gr_sq(r_mid * r_mid + a * a) - Delta * a * a * gr_sq(st);
+    g11 = Sigma / Delta;
+    g22 = Sigma;
+    g33 = A * gr_sq(st) / Sigma;
+    sqrt_gamma = sqrt(g11 * g22 * g33);
+    coef[GR_K1J][n] = Delta * sqrt_gamma / (vdpsi[n_mid] * (1.0 - y) + vdpsi[n_mid + 1] * y);
+    coef[GR_AGRR][n] = Delta * Delta * sqrt(Delta * Sigma / A) * g11;
+  }
+  return true;
+}
+
+apt_status apt_1dgr_set_profile(apt_ctx *ctx, const double *r, const double *B1,
+                                const double *B3, const double *rho, const double *jr,
+                                const double *theta, const double *dpsidth, uint64_t n) {
+  CHECK_CTX(ctx);
+  if (!ctx->gr1d) return ctx->fail(APT_ERR_UNSUPPORTED, "1dgr_set_profile: not a 1D GR context");
+  if (!r || !B1 || !B3 || !rho || !jr || !theta || !dpsidth || n < 2)
+    return ctx->fail(APT_ERR_INVALID_ARG, "1dgr_set_profile: missing arrays");
+  const double *src[7] = {r, B1, B3, rho, jr, theta, dpsidth};
+  std::vector<float> f[7];
+  for (int k = 0; k < 7; k++) {
+    f[k].resize(n);
+    for (uint64_t i = 0; i < n; i++) f[k][i] = (float)src[k][i];
+  }
+  std::vector<double> c[GR_NCOEF];
+  if (!gr1d_coef_host(ctx->mesh, f[0].data(), f[1].data(), f[2].data(), f[3].data(),
+                      f[4].data(), f[5].data(), f[6].data(), n, ctx->gr_a, ctx->gr_omega,
+                      ctx->gr_B0, c))
+    return ctx->fail(APT_ERR_INVALID_ARG,
+                     "1dgr_set_profile: the grid reaches past the profile's last r");
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < GR_NCOEF; k++) {
+    ctx->gr_host[k] = c[k];
+    HIPCHK(hipMemcpy(ctx->gr_dev[k], c[k].data(), c[k].size() * 8, hipMemcpyHostToDevice));
+  }
+  return APT_OK;
+}
+
+apt_status apt_1dgr_initial_condition(apt_ctx *ctx, int multiplicity) {
+  CHECK_CTX(ctx);
+  if (!ctx->gr1d)
+    return ctx->fail(APT_ERR_UNSUPPORTED, "1dgr_initial_condition: not a 1D GR context");
+  const Mesh &m = ctx->mesh;
+  if (multiplicity < 1)
+    return ctx->fail(APT_ERR_INVALID_ARG, "1dgr_initial_condition: multiplicity < 1");
+  const uint64_t number = (uint64_t)(m.dims[0] - 2 * m.guard[0]) * 2 * (uint64_t)multiplicity;
+  if (number > ctx->cap)
+    return ctx->fail(APT_ERR_CAPACITY, "1dgr_initial_condition: %llu > capacity %llu",
+                     (unsigned long long)number, (unsigned long long)ctx->cap);
+  // prepare_initial_condition (ptc_updater_1dgr.cu:16-72): slot
+  // cell * 2 mult + 2 n (+1), so the first guard * 2 mult slots stay empty
+  // and the last guard cells' pairs fall past number, as in the reference
+  std::vector<uint32_t> cell(number, MAX_CELL), flag(number, 0);
+  std::vector<double> x1(number, 0.0), zero(number, 0.0), E(number, 0.0), w(number, 0.0);
+  std::vector<uint64_t> id(number);
+  const double B0 = ctx->gr_B0, track = ctx->gr_track;
+  const std::vector<double> *c = ctx->gr_host;
+  for (int ci = m.guard[0]; ci < m.dims[0] - m.guard[0]; ci++)
+    for (int n = 0; n < multiplicity; n++) {
+      const uint64_t idx = (uint64_t)ci * (uint64_t)multiplicity * 2 + (uint64_t)n * 2;
+      if (idx + 1 >= number) continue;
+      const double D1 = c[GR_D1][ci], D2 = c[GR_D2][ci], D3 = c[GR_D3][ci];
+      const double alpha = c[GR_ALPHA][ci], K1 = c[GR_K1][ci], rho0 = c[GR_RHO0][ci];
+      const double u0 = sqrt(D2 / (D2 * (alpha * alpha - D3) + D1 * D1));
+      const double rk = rho0 * K1;
+      double u[2];
+      uniform2(ctx->params.random_seed, idx, 0, 8, u[0], u[1]);
+      for (int s = 0; s < 2; s++) {
+        const uint64_t k = idx + (uint64_t)s;
+        x1[k] = 1.0;
+        E[k] = u0;
+        cell[k] = (uint32_t)ci;
+        w[k] = s == 0 ? 1.0 * B0 / multiplicity + fabs(fmin(rk, 0.0)) / multiplicity
+                      : 1.0 * B0 / multiplicity + fmax(rk, 0.0) / multiplicity;
+        flag[k] = ((uint32_t)s << 29) | (u[s] < track ? FLAG_TRACKED : 0u);
+      }
+    }
+  for (uint64_t k = 0; k < number; k++) id[k] = k;
+  apt_particles v{cell.data(), x1.data(), zero.data(), zero.data(), zero.data(),
+                  zero.data(),  zero.data(), E.data(), w.data(), flag.data(), id.data()};
+  return apt_ptc_set(ctx, &v, number);
+}
+
+static Gr1dCoef gr1d_coef(const apt_ctx *ctx) {
+  Gr1dCoef cf;
+  for (int k = 0; k < GR_NCOEF; k++) cf.a[k] = ctx->gr_dev[k];
+  return cf;
+}
+
+// ptc_updater_1dgr::update_particles (ptc_updater_1dgr.cu:418-489): the
+// filtered E in E[2] (E[1] as the filter's scratch), the push with the
+// current and charge deposit, the K-weighted smoothing of J1 and Rho, then
+// the photons
+static apt_status gr1d_ptc_update(apt_ctx *ctx, double dt, uint32_t step) {
+  const Mesh &m = ctx->mesh;
+  const Gr1dCoef cf = gr1d_coef(ctx);
+  const uint64_t nc = ctx->ncell;
+  const int di = ctx->params.data_interval;
+  if (ctx->number > 0) {
+    double *E0 = (double *)ctx->E[0], *E1 = (double *)ctx->E[1], *E2 = (double *)ctx->E[2];
+    HIPCHK(hipMemcpyAsync(E2, E0, nc * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    for (int pass = 0; pass < ctx->params.current_smoothing; pass++) {
+      HIPCHK(launch_1dgr_filter(m, E2, E1, cf.a[GR_K1J], ctx->stream));
+      HIPCHK(hipMemcpyAsync(E2, E1, nc * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    AccF64 acc;
+    for (int c = 0; c < 3; c++) {
+      acc.J[c][0] = ctx->Jacc[c][0];
+      acc.J[c][1] = ctx->Jacc[c][1];
+    }
+    for (int s = 0; s < 4; s++) {
+      acc.R[s][0] = ctx->Racc[s][0];
+      acc.R[s][1] = ctx->Racc[s][1];
+    }
+    time_begin(ctx, APT_KERNEL_PUSH_DEPOSIT);
+    HIPCHK(launch_1dgr_push(m, cf, ctx->sp, ctx->p64, E2, acc, dt, step, di, ctx->gr_a,
+                            ctx->gr_omega, ctx->H, ctx->number, ctx->stream));
+    time_end(ctx, APT_KERNEL_PUSH_DEPOSIT);
+    for (int c = 0; c < 3; c++)
+      HIPCHK(launch_finalize_f64(ctx->Jacc[c][0], ctx->Jacc[c][1], (double *)ctx->J[c], nc,
+                                 ctx->H, ctx->stream));
+    for (int s = 0; s < 4; s++)
+      HIPCHK(launch_finalize_f64(ctx->Racc[s][0], ctx->Racc[s][1], (double *)ctx->Rho[s], nc,
+                                 ctx->H, ctx->stream));
+    double *tmp = (double *)ctx->jtmp;
+    for (int pass = 0; pass < ctx->params.current_smoothing; pass++) {
+      HIPCHK(launch_1dgr_filter(m, (double *)ctx->J[0], tmp, cf.a[GR_K1J], ctx->stream));
+      HIPCHK(hipMemcpyAsync(ctx->J[0], tmp, nc * 8, hipMemcpyDeviceToDevice, ctx->stream));
+      for (int s = 0; s < ctx->params.num_species; s++) {
+        HIPCHK(launch_1dgr_filter(m, (double *)ctx->Rho[s], tmp, cf.a[GR_K1], ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->Rho[s], tmp, nc * 8, hipMemcpyDeviceToDevice, ctx->stream));
+      }
+    }
+  }
+  if (ctx->ph_number > 0)
+    HIPCHK(launch_1dgr_photons(m, cf, ctx->ph64, dt, step, di, ctx->ph_number, ctx->stream));
+  return APT_OK;
+}
+
 apt_status apt_ptc_update(apt_ctx *ctx, double dt, uint32_t step) {
   CHECK_CTX(ctx);
+  if (ctx->gr1d) {
+    if (!(dt > 0.0)) return ctx->fail(APT_ERR_INVALID_ARG, "ptc_update: dt must be > 0");
+    return gr1d_ptc_update(ctx, dt, step);
+  }
   const int di = ctx->params.data_interval > 0 ? ctx->params.data_interval : 1;
   const int rho = ((step + 1) % (uint32_t)di) == 0;
   const uint64_t nc = ctx->ncell;
@@ -1225,6 +1469,13 @@ static apt_status logsph_field_update_t(apt_ctx *ctx, double dt) {
 
 apt_status apt_field_update(apt_ctx *ctx, double dt, uint32_t step) {
   CHECK_CTX(ctx);
+  if (ctx->gr1d) {  // field_solver_1dgr::update_fields (field_solver_1dgr.cu:46-58)
+    time_begin(ctx, APT_KERNEL_FIELD);
+    HIPCHK(launch_1dgr_update_e(ctx->mesh, gr1d_coef(ctx), (double *)ctx->E[0],
+                                (const double *)ctx->J[0], dt, ctx->stream));
+    time_end(ctx, APT_KERNEL_FIELD);
+    return APT_OK;
+  }
   if (ctx->logsph && ctx->mesh.dim != 2)
     return ctx->fail(APT_ERR_UNSUPPORTED, "field_update: use apt_ffe_update in 3D");
   if (ctx->logsph)

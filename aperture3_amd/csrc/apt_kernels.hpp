@@ -310,6 +310,27 @@ hipError_t launch_energy_histogram_f64(const double *E, const uint32_t *flag,
                                        double E_max, int flag_bit,
                                        hipStream_t st);
 
+// ---- 1D GR gap (kern_1dgr.hip) --------------------------------------
+// Grid_1dGR::mesh_ptrs (grid_1dgr.h:17-21), one dims[0] array each
+enum {
+  GR_D1 = 0, GR_D2, GR_D3, GR_ALPHA, GR_THETA, GR_K1, GR_K1J, GR_J0, GR_AGRR,
+  GR_AGRF, GR_RHO0, GR_GRR, GR_GFF, GR_BETA, GR_B3B1, GR_NCOEF
+};
+struct Gr1dCoef {
+  const double *a[GR_NCOEF];
+};
+hipError_t launch_1dgr_push(const Mesh &m, const Gr1dCoef &cf, const Species &sp,
+                            const PtcF64 &p, const double *e3, const AccF64 &acc,
+                            double dt, uint32_t step, int data_interval, double a,
+                            double omega, int H, uint64_t n, hipStream_t st);
+hipError_t launch_1dgr_photons(const Mesh &m, const Gr1dCoef &cf, const PhoF64 &ph,
+                               double dt, uint32_t step, int data_interval, uint64_t n,
+                               hipStream_t st);
+hipError_t launch_1dgr_filter(const Mesh &m, const double *j, double *tmp, const double *k1,
+                              hipStream_t st);
+hipError_t launch_1dgr_update_e(const Mesh &m, const Gr1dCoef &cf, double *e1,
+                                const double *j1, double dt, hipStream_t st);
+
 // per-cell moments of collect_diagnostics (kern_diag.hip): num[s], gamma[s]
 // for the four species slots and photon_num, ncell values each
 template <typename R>

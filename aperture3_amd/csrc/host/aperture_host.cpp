@@ -133,6 +133,8 @@ sim_data::sim_data(sim_environment &e)
     q.coord = APT_COORD_LOGSPH;
   else if (p.coord_system == "Cartesian")
     q.coord = APT_COORD_CARTESIAN;
+  else if (p.coord_system == "1DGR")
+    q.coord = APT_COORD_1DGR;
   else
     throw std::runtime_error("coord_system: " + p.coord_system +
                              " is not built (Cartesian, LogSpherical)");
@@ -154,6 +156,7 @@ sim_data::sim_data(sim_environment &e)
   q.r_cutoff = p.r_cutoff;
   q.e_frac = p.e_frac;
   q.max_photon_number = p.max_photon_number;
+  q.spin_a = p.a;
   check(nullptr, apt_create(&g, &q, &ctx), "apt_create");
 }
 

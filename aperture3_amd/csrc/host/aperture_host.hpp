@@ -36,7 +36,7 @@ struct SimParams {
   Scalar lower[3] = {0.0, 0.0, 0.0};
   Scalar size[3] = {1.0, 1.0, 1.0};
   // log-spherical path (sim_params.h:31-64, coord_system :101)
-  std::string coord_system = "Cartesian";  // or "LogSpherical"
+  std::string coord_system = "Cartesian";  // or "LogSpherical", "1DGR"
   Scalar compactness = 0.5;
   Scalar omega = 0.2;
   bool gravity_on = false;
@@ -54,6 +54,8 @@ struct SimParams {
   Scalar photon_path = 1.0;
   Scalar E_secondary = 4.0;
   Scalar r_cutoff = 4.0;
+  // 1D GR gap: black hole spin (sim_params.h:63)
+  Scalar a = 0.0;
   // output and restart (sim_params.h:103-120)
   std::string data_dir = "Data/";
   std::string conf_file = "config.toml";

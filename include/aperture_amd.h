@@ -46,7 +46,9 @@ enum {
 /* Coordinate systems (SimParams::coord_system, sim_params.h:101). */
 enum {
   APT_COORD_CARTESIAN = 0,
-  APT_COORD_LOGSPH = 1 /* 2D (ln r, theta), Grid_LogSph (grid_log_sph.h) */
+  APT_COORD_LOGSPH = 1, /* 2D (ln r, theta), Grid_LogSph (grid_log_sph.h) */
+  APT_COORD_1DGR = 2    /* 1D along a Kerr field line, Grid_1dGR (grid_1dgr.h),
+                           FP64 form */
 };
 
 /* Pair-creation models (radiative_transfer hooks). */
@@ -142,6 +144,9 @@ typedef struct {
   double r_cutoff;          /* r_cutoff (:79) */
   double e_frac;            /* APT_RT_EXP_PATH: photon energy / gamma */
   uint64_t max_photon_number; /* max_photon_number (:23) */
+  /* ---- 1D GR gap (sim_params.h:58-63) ---- */
+  double spin_a;            /* a, the black hole spin; taken as float like the
+                               reference's omega, B0 and track_percent */
 } apt_params;
 
 /* Host view of particle_data (src/data/particle_data.h:50-61) in FP64.
@@ -291,6 +296,22 @@ apt_status apt_field_compute_divs(apt_ctx *ctx);
 apt_status apt_sort_particles(apt_ctx *ctx);
 /* particle_base::clear_guard_cells (particle_base_impl.cu:489-493) */
 apt_status apt_clear_guard_cells(apt_ctx *ctx);
+
+/* ---- 1D GR gap (APT_COORD_1DGR) -------------------------------------- */
+/* Grid_1dGR::compute_coef (grid_1dgr.cpp:21-175) from the field-line
+ * profile the reference reads from coef.h5 (datasets r, Bu1, Bu3, rho,
+ * ju1, th, dpsidth; taken as float): the 15 arrays D1, D2, D3, alpha,
+ * theta, K1, K1_j, j0, agrr, agrf, rho0, gamma_rr, gamma_ff, beta_phi,
+ * B3B1 (read back with apt_get_coef, which = 0..14, dims[0] each). */
+apt_status apt_1dgr_set_profile(apt_ctx *ctx, const double *r, const double *B1,
+                                const double *B3, const double *rho, const double *jr,
+                                const double *theta, const double *dpsidth, uint64_t n);
+/* ptc_updater_1dgr::prepare_initial_condition (ptc_updater_1dgr.cu:16-72,
+ * 491-503): `multiplicity` pairs per bulk cell at rest, weights from B0 and
+ * rho0 K1, tracked with probability track_percent.  apt_ptc_update and
+ * apt_field_update on a 1D GR context run ptc_updater_1dgr and
+ * field_solver_1dgr. */
+apt_status apt_1dgr_initial_condition(apt_ctx *ctx, int multiplicity);
 /* AdditionalDiagnostics::collect_diagnostics
  * (cuda/core/additional_diagnostics.cu:78-100): per cell and species the
  * summed weight (APT_FIELD_PTC_NUM) and the weight-averaged gamma

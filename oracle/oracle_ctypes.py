@@ -592,3 +592,90 @@ def sample_mj(theta, beta, seed, pid):
     u = (C.c_double * 3)()
     ok = lib.orc_sample_mj(theta, beta, seed, pid, u)
     return ok, (u[0], u[1], u[2])
+
+
+_PF = C.POINTER(C.c_float)
+GR1D_PROFILE = ("r", "Bu1", "Bu3", "rho", "ju1", "th", "dpsidth")
+
+
+class GR1dOracle(OracleSim):
+    """The 1D GR gap (gr1d_oracle.c): Grid_1dGR coefficients from a
+    field-line profile, prepare_initial_condition, the particle and photon
+    pushers of ptc_updater_1dgr with the K-weighted smoothing, and
+    field_solver_1dgr."""
+
+    def __init__(self, N, guard, lower, size, profile, a, omega, B0, track=0.0,
+                 capacity=0, jexp=22, data_interval=100, smoothing=0, q_e=1.0,
+                 seed=43210, num_species=2):
+        super().__init__((N,), (guard,), size=(size, 1.0, 1.0), lower=(lower, 0.0, 0.0),
+                         periodic=(False, False, False), q_e=q_e, num_species=num_species,
+                         capacity=capacity, jexp=jexp, data_interval=data_interval)
+        lib = self.lib
+        lib.orc_gr1d_coef.restype = C.c_int
+        lib.orc_gr1d_initial.restype = C.c_uint64
+        self.a_f, self.omega_f, self.B0_f = np.float32(a), np.float32(omega), np.float32(B0)
+        self.track_f = np.float32(track)
+        self.seed = seed
+        self.smoothing = smoothing
+        self.gcoef = [np.zeros(self.dims[0]) for _ in range(15)]
+        self._cp = (_PD * 15)(*[_d(c) for c in self.gcoef])
+        arrs = [np.ascontiguousarray(profile[k], np.float32) for k in GR1D_PROFILE]
+        rc = lib.orc_gr1d_coef(C.byref(self.mesh), *[x.ctypes.data_as(_PF) for x in arrs],
+                               C.c_uint64(len(arrs[0])), C.c_float(self.a_f),
+                               C.c_float(self.omega_f), C.c_float(self.B0_f), self._cp)
+        if rc != 0:
+            raise ValueError("grid reaches past the profile")
+        self.tmp = np.zeros(self.ncell)
+
+    def initial(self, multiplicity):
+        P = self._ptc_struct()
+        self.number = int(self.lib.orc_gr1d_initial(
+            C.byref(self.mesh), self._cp, C.byref(P), C.c_int(multiplicity),
+            C.c_float(self.B0_f), C.c_float(self.track_f), C.c_uint64(self.seed),
+            C.c_uint64(self.capacity)))
+        return self.number
+
+    def filter(self, j, k1):
+        self.lib.orc_gr1d_filter(C.byref(self.mesh), _d(j), _d(self.tmp), _d(k1))
+        j[:] = self.tmp
+
+    def ptc_update(self, dt, step, deposit_rho=None):
+        H = self.sp.jexp
+        if self.number > 0:
+            E = self.f["E"]
+            E[2][:] = E[0]
+            for _ in range(self.smoothing):
+                self.lib.orc_gr1d_filter(C.byref(self.mesh), _d(E[2]), _d(E[1]),
+                                         _d(self.gcoef[6]))
+                E[2][:] = E[1]
+            P = self._ptc_struct()
+            self.lib.orc_gr1d_push(C.byref(self.mesh), self._cp, C.byref(P),
+                                   C.byref(self.sp), _d(E[2]), C.byref(self.F),
+                                   C.c_double(dt), C.c_uint32(step),
+                                   C.c_int(self.data_interval), C.c_float(self.a_f),
+                                   C.c_float(self.omega_f))
+            for c in range(3):
+                hi, lo = self.jacc[c]
+                self.f["J"][c][:] = (np.ldexp(hi.astype(np.float64), -H)
+                                     + np.ldexp(lo.astype(np.float64), -(H + 40)))
+                hi[:] = 0
+                lo[:] = 0
+            for s in range(4):
+                hi, lo = self.racc[s]
+                self.rho[s][:] = (np.ldexp(hi.astype(np.float64), -H)
+                                  + np.ldexp(lo.astype(np.float64), -(H + 40)))
+                hi[:] = 0
+                lo[:] = 0
+            for _ in range(self.smoothing):
+                self.filter(self.f["J"][0], self.gcoef[6])
+                for s in range(self.sp.num_species):
+                    self.filter(self.rho[s], self.gcoef[5])
+        if self.ph_number:
+            Ph = self._ph_struct()
+            self.lib.orc_gr1d_photons(C.byref(self.mesh), self._cp, C.byref(Ph),
+                                      C.c_double(dt), C.c_uint32(step),
+                                      C.c_int(self.data_interval))
+
+    def field_update(self, dt, step=0):
+        self.lib.orc_gr1d_field_update(C.byref(self.mesh), self._cp, _d(self.f["E"][0]),
+                                       _d(self.f["J"][0]), C.c_double(dt))

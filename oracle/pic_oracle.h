@@ -204,6 +204,24 @@ uint64_t orc_inject_left(const orc_mesh *m, orc_ptc *p, int n_pairs,
                          uint32_t epoch, uint64_t id_base, uint64_t cap,
                          uint64_t *overflow);
 
+/* --- 1D GR gap (gr1d_oracle.c) --- */
+/* coefficient arrays, Grid_1dGR::mesh_ptrs order (grid_1dgr.h:17-21) */
+#define ORC_GR_NCOEF 15
+int orc_gr1d_coef(const orc_mesh *m, const float *vr, const float *vB1, const float *vB3,
+                  const float *vrho, const float *vjr, const float *vth,
+                  const float *vdpsi, uint64_t nprof, float a_f, float omega_f,
+                  float B0_f, double *const *coef);
+uint64_t orc_gr1d_initial(const orc_mesh *m, double *const *coef, orc_ptc *p, int mult,
+                          float B0_f, float track_f, uint64_t seed, uint64_t cap);
+void orc_gr1d_push(const orc_mesh *m, double *const *coef, orc_ptc *p,
+                   const orc_species *sp, const double *e3, orc_fields *f, double dt,
+                   uint32_t step, int data_interval, float a_f, float omega_f);
+void orc_gr1d_photons(const orc_mesh *m, double *const *coef, orc_photons *ph, double dt,
+                      uint32_t step, int data_interval);
+void orc_gr1d_filter(const orc_mesh *m, const double *j, double *tmp, const double *k1);
+void orc_gr1d_field_update(const orc_mesh *m, double *const *coef, double *e1,
+                           const double *j1, double dt);
+
 /* --- diagnostics --- */
 void orc_energy_histogram(const orc_ptc *p, uint32_t *hist, int nbins,
                           double E_max, int flag_bit);
