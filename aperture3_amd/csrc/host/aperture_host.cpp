@@ -333,4 +333,27 @@ void AdditionalDiagnostics::collect_diagnostics(sim_data &data) {
   check(data.ctx, apt_collect_diagnostics(data.ctx), "apt_collect_diagnostics");
 }
 
+void compute_coef_1dgr(sim_data &data, const Gr1dProfile &g) {
+  const size_t n = g.r.size();
+  for (const std::vector<Scalar> *v : {&g.Bu1, &g.Bu3, &g.rho, &g.ju1, &g.th, &g.dpsidth})
+    if (v->size() != n) throw std::runtime_error("compute_coef_1dgr: profile sizes differ");
+  check(data.ctx,
+        apt_1dgr_set_profile(data.ctx, g.r.data(), g.Bu1.data(), g.Bu3.data(), g.rho.data(),
+                             g.ju1.data(), g.th.data(), g.dpsidth.data(), n),
+        "apt_1dgr_set_profile");
+}
+
+void ptc_updater_1dgr::update_particles(sim_data &data, double dt, uint32_t step) {
+  check(data.ctx, apt_ptc_update(data.ctx, dt, step), "apt_ptc_update");
+}
+
+void ptc_updater_1dgr::prepare_initial_condition(sim_data &data, int multiplicity) {
+  check(data.ctx, apt_1dgr_initial_condition(data.ctx, multiplicity),
+        "apt_1dgr_initial_condition");
+}
+
+void field_solver_1dgr::update_fields(sim_data &data, double dt, double) {
+  check(data.ctx, apt_field_update(data.ctx, dt, 0), "apt_field_update");
+}
+
 }  // namespace Aperture

@@ -135,6 +135,34 @@ struct sim_data {
   field_ref ptc_num, gamma, photon_num;
 };
 
+// Grid_1dGR::compute_coef (grid_1dgr.cpp:21-175) on a 1D GR context from
+// the field-line profile of coef.h5 (r, Bu1, Bu3, rho, ju1, th, dpsidth)
+struct Gr1dProfile {
+  std::vector<Scalar> r, Bu1, Bu3, rho, ju1, th, dpsidth;
+};
+void compute_coef_1dgr(sim_data &data, const Gr1dProfile &profile);
+
+// ptc_updater_1dgr (algorithms/ptc_updater_1dgr.h)
+class ptc_updater_1dgr {
+ public:
+  explicit ptc_updater_1dgr(sim_environment &env) : m_env(env) {}
+  void update_particles(sim_data &data, double dt, uint32_t step = 0);
+  void prepare_initial_condition(sim_data &data, int multiplicity);
+
+ private:
+  sim_environment &m_env;
+};
+
+// field_solver_1dgr (algorithms/field_solver_1dgr.h)
+class field_solver_1dgr {
+ public:
+  explicit field_solver_1dgr(sim_environment &env) : m_env(env) {}
+  void update_fields(sim_data &data, double dt, double time = 0.0);
+
+ private:
+  sim_environment &m_env;
+};
+
 // AdditionalDiagnostics (cuda/core/additional_diagnostics.h)
 class AdditionalDiagnostics {
  public:

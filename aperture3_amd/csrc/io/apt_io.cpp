@@ -224,6 +224,23 @@ apt_status apt_io_downsample(const double *f, const int dims[3], const int guard
   return APT_OK;
 }
 
+apt_status apt_io_read_dataset(const char *filename, const char *name, double *out,
+                               uint64_t cap, uint64_t *n) {
+  if (!filename || !name || !n) return fail(APT_ERR_INVALID_ARG, "read_dataset: null argument");
+  H5 file;
+  file.id = H5Fopen(filename, H5F_ACC_RDONLY, H5P_DEFAULT);
+  if (file.id < 0) return fail(APT_ERR_INVALID_ARG, "cannot open %s", filename);
+  const uint64_t count = dataset_size(file.id, name);
+  if (count == (uint64_t)-1) return fail(APT_ERR_INVALID_ARG, "%s: no dataset %s", filename, name);
+  *n = count;
+  if (!out) return APT_OK;  // size query
+  if (count > cap) return fail(APT_ERR_CAPACITY, "%s: %s has %llu values > %llu", filename, name,
+                               (unsigned long long)count, (unsigned long long)cap);
+  if (!read_array<double>(file.id, name, out, count))
+    return fail(APT_ERR_INVALID_ARG, "%s: reading %s", filename, name);
+  return APT_OK;
+}
+
 apt_status apt_io_save_snapshot(apt_ctx *ctx, const char *filename, uint32_t step,
                                 double time, int data_interval, int output_num,
                                 uint64_t rng_seed) {

@@ -37,6 +37,8 @@ IO_SIGNATURES = {
                              C.c_uint64, C.c_uint64],
     "apt_io_write_xmf": [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
                          C.POINTER(C.c_int), C.c_int, C.c_int],
+    "apt_io_read_dataset": [C.c_char_p, C.c_char_p, C.POINTER(C.c_double), C.c_uint64,
+                            C.POINTER(C.c_uint64)],
     "apt_io_downsample": [C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int),
                           C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)],
 }
@@ -85,6 +87,27 @@ def downsample(f, dims, guard, dim, stagger, d):
                                       stagger, d, out.ctypes.data_as(C.POINTER(C.c_float))),
            "apt_io_downsample")
     return out
+
+
+def read_dataset(filename: str, name: str):
+    """A whole HDF5 dataset as FP64 (H5File::read_vector)."""
+    lib = load_io_library()
+    n = C.c_uint64()
+    _check(lib, lib.apt_io_read_dataset(filename.encode(), name.encode(), None, 0,
+                                        C.byref(n)), "apt_io_read_dataset")
+    out = np.empty(n.value, np.float64)
+    _check(lib, lib.apt_io_read_dataset(filename.encode(), name.encode(),
+                                        out.ctypes.data_as(C.POINTER(C.c_double)), out.size,
+                                        C.byref(n)), "apt_io_read_dataset")
+    return out
+
+
+def read_coef_h5(filename: str):
+    """The field-line profile of the 1D GR gap from coef.h5 (the datasets
+    Grid_1dGR::compute_coef reads, grid_1dgr.cpp:60-66), for
+    Grid_1dGR.compute_coef."""
+    from . import GR1D_PROFILE
+    return {k: read_dataset(filename, k) for k in GR1D_PROFILE}
 
 
 class data_exporter:

@@ -75,6 +75,12 @@ apt_status apt_io_write_xmf(const char *dir, int dim, const int out_ext[3],
                             const double *times, const int *output_nums, int n_outputs,
                             int num_species);
 
+/* H5File::read_vector (hdf_wrapper_impl.hpp): a whole dataset as doubles,
+ * e.g. the field-line profile of coef.h5 for apt_1dgr_set_profile
+ * (grid_1dgr.cpp:60-66).  With out == NULL only *n is set. */
+apt_status apt_io_read_dataset(const char *filename, const char *name, double *out,
+                               uint64_t cap, uint64_t *n);
+
 /* multi_array::downsample on a host array: sample every d-th node from
  * `guard` and interpolate from stagger `st` to the node stagger 0b111 with
  * the reference's neighbour offsets.  out holds N/d values per axis. */

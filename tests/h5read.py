@@ -1,7 +1,8 @@
-"""Minimal HDF5 reader over the HDF5 C library with ctypes (h5py is not
-installed): lists the datasets of a file and reads one into numpy with its
-stored type and shape.  Used by the output/snapshot tests to check what the
-I/O library wrote independently of the library itself."""
+"""Minimal HDF5 reader (and 1D writer) over the HDF5 C library with ctypes
+(h5py is not installed): lists the datasets of a file and reads one into
+numpy with its stored type and shape.  Used by the output/snapshot tests to
+check what the I/O library wrote independently of the library itself, and
+to write input files for its readers."""
 from __future__ import annotations
 
 import ctypes as C
@@ -125,3 +126,29 @@ class File:
         h.H5Sclose(sp)
         h.H5Dclose(ds)
         return out
+
+
+def write(path, arrays):
+    """Writes 1D float32/float64 arrays as datasets of a new file (for
+    tests of the I/O library's readers)."""
+    h = lib()
+    h.H5Fcreate.restype = hid_t
+    h.H5Fcreate.argtypes = [C.c_char_p, C.c_uint, hid_t, hid_t]
+    h.H5Screate_simple.restype = hid_t
+    h.H5Screate_simple.argtypes = [C.c_int, C.POINTER(hsize_t), C.POINTER(hsize_t)]
+    h.H5Dcreate2.restype = hid_t
+    h.H5Dcreate2.argtypes = [hid_t, C.c_char_p, hid_t, hid_t, hid_t, hid_t, hid_t]
+    h.H5Dwrite.argtypes = [hid_t, hid_t, hid_t, hid_t, hid_t, C.c_void_p]
+    f = h.H5Fcreate(path.encode(), 2, 0, 0)  # H5F_ACC_TRUNC
+    if f < 0:
+        raise OSError(path)
+    for name, a in arrays.items():
+        a = np.ascontiguousarray(a)
+        ty = _native("H5T_NATIVE_DOUBLE_g" if a.dtype == np.float64 else "H5T_NATIVE_FLOAT_g")
+        dims = (hsize_t * 1)(a.size)
+        sp = h.H5Screate_simple(1, dims, None)
+        ds = h.H5Dcreate2(f, name.encode(), ty, sp, 0, 0, 0)
+        h.H5Dwrite(ds, ty, 0, 0, 0, a.ctypes.data)
+        h.H5Dclose(ds)
+        h.H5Sclose(sp)
+    h.H5Fclose(f)

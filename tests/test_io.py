@@ -26,7 +26,7 @@ pytestmark = pytest.mark.skipif(h5read.HDF5_LIB is None or not os.path.exists(IO
 def test_io_library_exports_header():
     text = open(os.path.join(ROOT, "include", "aperture_io.h")).read()
     names = set(re.findall(r"\b(apt_io_\w+)\s*\(", text))
-    assert len(names) == 8
+    assert len(names) == 9
     lib = load_io_library()
     for n in names:
         assert hasattr(lib, n), n
@@ -172,3 +172,21 @@ def test_xmf_restart_keeps_outputs_to_restart_time(tmp_path):
     ex2.prepare_xmf_restart(10, 1.0)
     assert ex2._times == [0.0, 0.5, 1.0] and ex2._nums == [0, 1, 2]
     assert len(xmf_grids(tmp_path / "data.xmf")) == 3
+
+
+def test_read_dataset_and_coef_profile(tmp_path):
+    """apt_io_read_dataset reads what the HDF5 C library wrote (any stored
+    float type, converted to FP64); read_coef_h5 returns the seven profile
+    datasets of coef.h5."""
+    from aperture3_amd.exporter import read_coef_h5, read_dataset
+    path = str(tmp_path / "coef.h5")
+    rng = np.random.default_rng(1)
+    data = {k: rng.random(17) for k in apt.GR1D_PROFILE}
+    data["th"] = data["th"].astype(np.float32)
+    h5read.write(path, data)
+    np.testing.assert_array_equal(read_dataset(path, "r"), data["r"])
+    prof = read_coef_h5(path)
+    for k in apt.GR1D_PROFILE:
+        np.testing.assert_array_equal(prof[k], data[k].astype(np.float64))
+    with pytest.raises(apt.AptError):
+        read_dataset(path, "missing")
