@@ -1168,7 +1168,7 @@ static bool gr1d_coef_host(const Mesh &m, const float *vr, const float *vB1,
     while (n_mid + 1 < nprof && vr[n_mid + 1] < r_mid) n_mid += 1;
     if (n_mid + 1 >= nprof) return false;
     // the mid-point quantities are interpolated with the node r, as in
-    // the reference (grid_1dgr.cpp:133)
+    // the reference (grid_1dgr.cpp:146)
     const double y = (r - vr[n_mid]) / (vr[n_mid + 1] - vr[n_mid]);
     coef[GR_J0][n] = B0 * (vjr[n_mid] * (1.0 - y) + vjr[n_mid + 1] * y) / Delta;
     const double theta_mid = vth[n_mid] * (1.0 - y) + vth[n_mid + 1] * y;
@@ -1264,7 +1264,7 @@ static Gr1dCoef gr1d_coef(const apt_ctx *ctx) {
   return cf;
 }
 
-// ptc_updater_1dgr::update_particles (ptc_updater_1dgr.cu:418-489): the
+// ptc_updater_1dgr::update_particles (ptc_updater_1dgr.cu:443-501): the
 // filtered E in E[2] (E[1] as the filter's scratch), the push with the
 // current and charge deposit, the K-weighted smoothing of J1 and Rho, then
 // the photons

@@ -1,7 +1,7 @@
 // kern_1dgr.hip -- the 1D general-relativistic gap along one Kerr field
 // line (the labs/1dgr problem): the particle and photon pushers of
 // ptc_updater_1dgr (src/cuda/algorithms/ptc_updater_1dgr.cu:132-376), its
-// K-weighted current filter (:378-398) and the E update of
+// K-weighted current filter (:408-427) and the E update of
 // field_solver_1dgr (src/cuda/algorithms/field_solver_1dgr.cu:14-38).
 //
 // FP64, one thread per particle (the 1D problem has at most a few million
@@ -33,7 +33,7 @@ __device__ inline bool bulk1(const Mesh &m, uint32_t c) {
   return (int)c >= m.guard[0] && (int)c < m.dims[0] - m.guard[0];
 }
 
-// update_ptc_1dgr (ptc_updater_1dgr.cu:132-277)
+// update_ptc_1dgr (ptc_updater_1dgr.cu:136-299)
 __global__ void k_1dgr_push(Mesh m, Gr1dCoef cf, Species sp, PtcF64 p,
                             const double *__restrict__ e3, AccF64 acc, double dt,
                             uint32_t step, int data_interval, double a, double omega,
@@ -133,7 +133,7 @@ __global__ void k_1dgr_push(Mesh m, Gr1dCoef cf, Species sp, PtcF64 p,
   }
 }
 
-// update_photon_1dgr (ptc_updater_1dgr.cu:279-376), with the reference's
+// update_photon_1dgr (ptc_updater_1dgr.cu:301-406), with the reference's
 // fixed a = 0.99
 __global__ void k_1dgr_photons(Mesh m, Gr1dCoef cf, PhoF64 ph, double dt, uint32_t step,
                                int data_interval, uint64_t n) {
@@ -195,7 +195,7 @@ __global__ void k_1dgr_photons(Mesh m, Gr1dCoef cf, PhoF64 ph, double dt, uint32
   ph.x1[idx] = new_x1;
 }
 
-// filter_current1d (ptc_updater_1dgr.cu:378-398); the first and last array
+// filter_current1d (ptc_updater_1dgr.cu:408-427); the first and last array
 // cells are copied (the reference reads one past each end there)
 __global__ void k_1dgr_filter(Mesh m, const double *__restrict__ j, double *__restrict__ tmp,
                               const double *__restrict__ k1) {

@@ -95,7 +95,7 @@ int orc_gr1d_coef(const orc_mesh *m, const float *vr, const float *vB1, const fl
     while (n_mid + 1 < nprof && vr[n_mid + 1] < r_mid) n_mid += 1;
     if (n_mid + 1 >= nprof) return -1;
     /* the reference interpolates the mid-point quantities with the node
-     * r, not r_mid (grid_1dgr.cpp:133) */
+     * r, not r_mid (grid_1dgr.cpp:146) */
     double y = (r - vr[n_mid]) / (vr[n_mid + 1] - vr[n_mid]);
     coef[GR_J0][n] = B0 * (vjr[n_mid] * (1.0 - y) + vjr[n_mid + 1] * y) / Delta;
     double theta_mid = vth[n_mid] * (1.0 - y) + vth[n_mid + 1] * y;
@@ -165,7 +165,7 @@ static double lerp_c(const double *v, int c, double x1, double nx1) {
   return v[c] * x1 + v[c - 1] * nx1;
 }
 
-/* update_ptc_1dgr (ptc_updater_1dgr.cu:132-277): Er from the filtered
+/* update_ptc_1dgr (ptc_updater_1dgr.cu:136-299): Er from the filtered
  * E3, the GR momentum update of p1 along the field line, the move, and the
  * current J1 / charge Rho[sp] deposit divided by K1_j / K1.  p2 keeps
  * u_0 at infinity for tracked particles on output steps.  e3 is the
@@ -273,7 +273,7 @@ void orc_gr1d_push(const orc_mesh *m, double *const *coef, orc_ptc *p,
   }
 }
 
-/* update_photon_1dgr (ptc_updater_1dgr.cu:279-376): the photon's p1 along
+/* update_photon_1dgr (ptc_updater_1dgr.cu:301-406): the photon's p1 along
  * the field line with the reference's fixed a = 0.99; p2 keeps u_0 at
  * infinity for tracked photons on output steps. */
 void orc_gr1d_photons(const orc_mesh *m, double *const *coef, orc_photons *ph, double dt,
@@ -334,7 +334,7 @@ void orc_gr1d_photons(const orc_mesh *m, double *const *coef, orc_photons *ph, d
   }
 }
 
-/* filter_current1d (ptc_updater_1dgr.cu:378-398): a K-weighted [1 2 1]/4
+/* filter_current1d (ptc_updater_1dgr.cu:408-427): a K-weighted [1 2 1]/4
  * pass; at the bulk edges (single rank: both are boundaries) the neighbour
  * outside is replaced by the cell itself.  The reference also filters the
  * first and last array cells, reading one past each end; here those two

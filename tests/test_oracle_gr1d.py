@@ -1,5 +1,5 @@
 """Pins the oracle's 1D GR gap (oracle/gr1d_oracle.c: Grid_1dGR::compute_coef,
-grid_1dgr.cpp:21-175; ptc_updater_1dgr.cu:16-398; field_solver_1dgr.cu:14-38)
+grid_1dgr.cpp:21-175; ptc_updater_1dgr.cu:16-427; field_solver_1dgr.cu:14-38)
 with what the mathematics fixes:
 
   * r(xi) inverts xi = ln((r - r+)/(r - r-)) / (r+ - r-);
