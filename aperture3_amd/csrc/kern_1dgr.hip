@@ -1,6 +1,6 @@
 // kern_1dgr.hip -- the 1D general-relativistic gap along one Kerr field
 // line (the labs/1dgr problem): the particle and photon pushers of
-// ptc_updater_1dgr (src/cuda/algorithms/ptc_updater_1dgr.cu:132-376), its
+// ptc_updater_1dgr (src/cuda/algorithms/ptc_updater_1dgr.cu:136-406), its
 // K-weighted current filter (:408-427) and the E update of
 // field_solver_1dgr (src/cuda/algorithms/field_solver_1dgr.cu:14-38).
 //
