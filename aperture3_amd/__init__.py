@@ -120,6 +120,10 @@ class AptBatchParams(C.Structure):
                 ("e_frac", C.c_double), ("r_cutoff", C.c_double)]
 
 
+class AptICSpectrum(C.Structure):
+    _fields_ = [("kind", C.c_int), ("p", C.c_double * 5)]
+
+
 # every entry point declared in include/aperture_amd.h: name -> argtypes
 _P = C.c_void_p
 SIGNATURES = {
@@ -195,6 +199,18 @@ SIGNATURES = {
     "apt_batch_set_field": [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
                             C.c_uint64],
     "apt_batch_sample": [C.c_uint64, C.c_int, C.POINTER(AptBatchInst)],
+    "apt_ic_create": [C.c_int, C.c_int, C.POINTER(_P)],
+    "apt_ic_destroy": [_P],
+    "apt_ic_last_error": [_P],
+    "apt_ic_init": [_P, C.POINTER(AptICSpectrum), C.c_double, C.c_double, C.c_double],
+    "apt_ic_get_table": [_P, C.c_int, C.POINTER(C.c_double), C.c_uint64],
+    "apt_ic_gen_photon_e": [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint64,
+                            C.c_uint64, C.c_uint32],
+    "apt_ic_gen_photon_e_dev": [_P, _P, _P, C.c_uint64, C.c_uint64, C.c_uint32, _P],
+    "apt_ic_generate_photon_energies": [_P, _P, _P, C.c_uint64, C.c_uint64, C.c_uint32, _P],
+    "apt_ic_generate_random_gamma": [_P, _P, C.c_uint64, C.c_uint64, C.c_uint32, _P],
+    "apt_ic_rates": [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                     C.c_uint64],
 }
 
 _lib = None
@@ -279,6 +295,10 @@ class SimParams:
         self.max_photon_number = 0
         # 1D GR gap (sim_params.h:63): black hole spin
         self.a = 0.0
+        # inverse-Compton tables (sim_params.h:72, 87-88)
+        self.e_min = float(np.float32(1.0e-3))
+        self.n_gamma = 600
+        self.n_ep = 600
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise AttributeError(f"unknown parameter {k}")
@@ -719,6 +739,203 @@ class ffe_solver_logsph:
 
     def check_eGTb(self, data: sim_data):
         data._check(data.lib.apt_ffe_check_egtb(data.ctx), "apt_ffe_check_egtb")
+
+
+# ---------------------------------------------------------------------------
+# soft-photon spectra (src/radiation/spectra.h:10-95) as (kind, parameters)
+SPEC_POWER_LAW_HARD, SPEC_POWER_LAW_SOFT, SPEC_BROKEN_POWER_LAW = 0, 1, 2
+SPEC_BLACK_BODY, SPEC_MONO_ENERGETIC = 3, 4
+IC_GAMMAS, IC_EP, IC_LOG_EP, IC_RATE, IC_GG_RATE, IC_DNDE, IC_DNDE_THOMSON = range(7)
+
+
+class Spectra:
+    """Spectra:: (spectra.h): each carries its kind, parameters and the
+    emin()/emax() the reference gives it."""
+
+    class _Spec:
+        kind = -1
+
+        def __init__(self, *p):
+            self.p = tuple(float(v) for v in p)
+
+        def struct(self):
+            s = AptICSpectrum()
+            s.kind = self.kind
+            for i, v in enumerate(self.p):
+                s.p[i] = v
+            return s
+
+    class power_law_hard(_Spec):
+        kind = SPEC_POWER_LAW_HARD  # (delta, emin, emax), spectra.h:10-26
+
+        def emin(self):
+            return self.p[1]
+
+        def emax(self):
+            return self.p[2]
+
+    class power_law_soft(_Spec):
+        kind = SPEC_POWER_LAW_SOFT  # (alpha, emin, emax), spectra.h:28-44
+
+        def emin(self):
+            return self.p[1]
+
+        def emax(self):
+            return self.p[2]
+
+    class broken_power_law(_Spec):
+        kind = SPEC_BROKEN_POWER_LAW  # (alpha, delta, epeak, emin, emax), :46-68
+
+        def emin(self):
+            return self.p[3]
+
+        def emax(self):
+            return self.p[4]
+
+    class black_body(_Spec):
+        kind = SPEC_BLACK_BODY  # (kT), spectra.h:70-81
+
+        def emin(self):
+            return 1e-10 * self.p[0]
+
+        def emax(self):
+            return 1e3 * self.p[0]
+
+    class mono_energetic(_Spec):
+        kind = SPEC_MONO_ENERGETIC  # (e0, de), spectra.h:83-95
+
+        def emin(self):
+            return self.p[0] * 1e-2
+
+        def emax(self):
+            return self.p[0] * 1e2
+
+
+class inverse_compton:
+    """inverse_compton (src/cuda/radiation/rt_ic.h:13-52): the tables live on
+    the GPU (built by apt_ic_init), the accessors copy them to numpy."""
+
+    MIN_GAMMA = float(np.float32(1.01))
+    MAX_GAMMA = 1.0e15
+
+    def __init__(self, params: SimParams):
+        self.lib = load_library()
+        self.n_gamma, self.n_ep = int(params.n_gamma), int(params.n_ep)
+        h = C.c_void_p()
+        st = self.lib.apt_ic_create(self.n_gamma, self.n_ep, C.byref(h))
+        if st != 0:
+            raise AptError(f"apt_ic_create({self.n_gamma}, {self.n_ep}): status {st}")
+        self.h = h
+        g = self._table(IC_GAMMAS)
+        self.m_dg = (np.log(self.MAX_GAMMA) - np.log(self.MIN_GAMMA)) / (self.n_gamma - 1)
+        self._gammas = g
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.lib.apt_ic_destroy(h)
+            self.h = None
+
+    def _check(self, st, what):
+        if st != 0:
+            raise AptError(f"{what}: status {st}: {self.lib.apt_ic_last_error(self.h).decode()}")
+
+    def init(self, n_e, emin: float, emax: float, n0: float = 1.0):
+        spec = n_e.struct()
+        self._check(self.lib.apt_ic_init(self.h, C.byref(spec), emin, emax, n0), "apt_ic_init")
+
+    def _table(self, which):
+        n = self.n_gamma if which in (IC_GAMMAS, IC_RATE, IC_GG_RATE) else self.n_ep
+        if which in (IC_DNDE, IC_DNDE_THOMSON):
+            n = self.n_gamma * self.n_ep
+        out = np.empty(n)
+        self._check(self.lib.apt_ic_get_table(self.h, which, _dptr(out), n), "apt_ic_get_table")
+        if which in (IC_DNDE, IC_DNDE_THOMSON):
+            out = out.reshape(self.n_gamma, self.n_ep)
+        return out
+
+    def ic_rate(self):
+        return self._table(IC_RATE)
+
+    def gg_rate(self):
+        return self._table(IC_GG_RATE)
+
+    def gammas(self):
+        return self._table(IC_GAMMAS)
+
+    def ep(self):
+        return self._table(IC_EP)
+
+    def log_ep(self):
+        return self._table(IC_LOG_EP)
+
+    def dNde(self):
+        """m_dNde, indexed [gamma, e1] (the reference's multi_array is
+        (e1, gamma) with e1 fastest)."""
+        return self._table(IC_DNDE)
+
+    def dNde_thomson(self):
+        return self._table(IC_DNDE_THOMSON)
+
+    def gen_photon_e(self, gamma, seed: int = 0, epoch: int = 0):
+        """Kernels::gen_photon_e (rt_ic.cu:127-153) for each Lorentz factor,
+        the uniform draw of index k being Philox (seed, k, epoch)."""
+        g = np.ascontiguousarray(gamma, dtype=np.float64).ravel()
+        out = np.empty_like(g)
+        self._check(self.lib.apt_ic_gen_photon_e(self.h, _dptr(g), _dptr(out), g.size, seed,
+                                                 epoch), "apt_ic_gen_photon_e")
+        return out
+
+    @staticmethod
+    def _device_f64(*ts):
+        import torch
+        n = ts[0].numel()
+        for t in ts:
+            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() or \
+                    t.numel() != n:
+                raise AptError("contiguous float64 device tensors of one length expected")
+        return torch.cuda.current_stream().cuda_stream
+
+    def generate_photon_energies(self, e_ph, gammas, seed: int = 0, epoch: int = 0):
+        """inverse_compton::generate_photon_energies (rt_ic.cu:515-522) on
+        device tensors, ordered on torch's current stream: e_ph[k] is a
+        scattered photon's energy when the first uniform of Philox (seed, k,
+        epoch) is below find_ic_rate(gammas[k]), else 0."""
+        st = self._device_f64(e_ph, gammas)
+        self._check(self.lib.apt_ic_generate_photon_energies(
+            self.h, e_ph.data_ptr(), gammas.data_ptr(), gammas.numel(), seed, epoch, st),
+            "apt_ic_generate_photon_energies")
+
+    def gen_photon_e_device(self, e_ph, gammas, seed: int = 0, epoch: int = 0):
+        """gen_photon_e on device tensors (every sample scatters)."""
+        st = self._device_f64(e_ph, gammas)
+        self._check(self.lib.apt_ic_gen_photon_e_dev(self.h, gammas.data_ptr(),
+                                                     e_ph.data_ptr(), gammas.numel(), seed,
+                                                     epoch, st), "apt_ic_gen_photon_e_dev")
+
+    def generate_random_gamma(self, gammas, seed: int = 0, epoch: int = 0):
+        """inverse_compton::generate_random_gamma (rt_ic.cu:524-529): fills a
+        device tensor with Lorentz factors log-uniform in [1.01, 1e10]."""
+        st = self._device_f64(gammas)
+        self._check(self.lib.apt_ic_generate_random_gamma(self.h, gammas.data_ptr(),
+                                                          gammas.numel(), seed, epoch, st),
+                    "apt_ic_generate_random_gamma")
+
+    def rates(self, gamma):
+        """(find_ic_rate, find_gg_rate) at each Lorentz factor (rt_ic.cu:156-171)."""
+        g = np.ascontiguousarray(gamma, dtype=np.float64).ravel()
+        ic, gg = np.empty_like(g), np.empty_like(g)
+        self._check(self.lib.apt_ic_rates(self.h, _dptr(g), _dptr(ic), _dptr(gg), g.size),
+                    "apt_ic_rates")
+        return ic, gg
+
+    def find_n_gamma(self, gamma: float) -> int:
+        """inverse_compton::find_n_gamma (rt_ic.cu:430-435)."""
+        if gamma < self.MIN_GAMMA:
+            return 0
+        if gamma > self.MAX_GAMMA:
+            return self.n_gamma - 1
+        return int((np.log(gamma) - np.log(self.MIN_GAMMA)) / self.m_dg)
 
 
 # ---------------------------------------------------------------------------

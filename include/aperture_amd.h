@@ -432,6 +432,65 @@ apt_status apt_batch_set_field(apt_batch *b, int inst, int which, int comp,
  * theta = U[0.01, 0.3], gamma_thr = U[1e2, 1e4] */
 apt_status apt_batch_sample(uint64_t seed, int n, apt_batch_inst *out);
 
+/* ---- inverse-Compton tables (rt_ic.cu:22-547) ------------------------ */
+/* The reference's inverse_compton: the IC scattering rate, the
+ * gamma-gamma pair-creation rate and the lab-frame spectra of scattered
+ * photons (as cumulative distributions over e1 = E_ph / gamma, linear and
+ * logarithmic grids) on a grid of n_gamma Lorentz factors in [1.01, 1e15],
+ * for a soft-photon spectrum of src/radiation/spectra.h.  Independent of a
+ * PIC context. */
+typedef struct apt_ic apt_ic;
+enum {
+  APT_SPEC_POWER_LAW_HARD = 0,   /* p = delta, emin, emax */
+  APT_SPEC_POWER_LAW_SOFT = 1,   /* p = alpha, emin, emax */
+  APT_SPEC_BROKEN_POWER_LAW = 2, /* p = alpha, delta, epeak, emin, emax */
+  APT_SPEC_BLACK_BODY = 3,       /* p = kT */
+  APT_SPEC_MONO_ENERGETIC = 4    /* p = e0, de */
+};
+typedef struct {
+  int kind;
+  double p[5];
+} apt_ic_spectrum;
+enum {
+  APT_IC_GAMMAS = 0,       /* n_gamma */
+  APT_IC_EP = 1,           /* n_ep, e1 = i / (n_ep - 1) */
+  APT_IC_LOG_EP = 2,       /* n_ep, 1e-10 .. 1 logarithmic */
+  APT_IC_RATE = 3,         /* n_gamma: ic_rate */
+  APT_IC_GG_RATE = 4,      /* n_gamma: gg_rate */
+  APT_IC_DNDE = 5,         /* n_gamma x n_ep (e1 fastest): CDF on EP */
+  APT_IC_DNDE_THOMSON = 6  /* n_gamma x n_ep: CDF on LOG_EP */
+};
+/* inverse_compton(params) (rt_ic.cu:227-282): params.n_gamma, n_ep grids */
+apt_status apt_ic_create(int n_gamma, int n_ep, apt_ic **out);
+apt_status apt_ic_destroy(apt_ic *h);
+const char *apt_ic_last_error(const apt_ic *h);
+/* inverse_compton::init(n_e, emin, emax, n0) (rt_ic.cu:279-421), on the GPU */
+apt_status apt_ic_init(apt_ic *h, const apt_ic_spectrum *spec, double emin, double emax,
+                       double n0);
+apt_status apt_ic_get_table(apt_ic *h, int which, double *dst, uint64_t count);
+/* Kernels::gen_photon_e (rt_ic.cu:127-153) for n Lorentz factors, with the
+ * uniform draw from the Philox stream (seed, index, epoch); host arrays */
+apt_status apt_ic_gen_photon_e(apt_ic *h, const double *gamma, double *eph, uint64_t n,
+                               uint64_t seed, uint32_t epoch);
+/* the same on device arrays, ordered on `stream` (a hipStream_t; NULL: the
+ * handle's own stream) */
+apt_status apt_ic_gen_photon_e_dev(apt_ic *h, const double *gamma, double *eph, uint64_t n,
+                                   uint64_t seed, uint32_t epoch, void *stream);
+/* inverse_compton::generate_photon_energies (rt_ic.cu:515-522): device
+ * arrays; sample k scatters when the first uniform of its Philox block
+ * (seed, k, epoch) is below find_ic_rate(gammas[k]) and then takes
+ * gen_photon_e with the second; e_ph[k] = 0 otherwise */
+apt_status apt_ic_generate_photon_energies(apt_ic *h, double *e_ph, const double *gammas,
+                                           uint64_t n, uint64_t seed, uint32_t epoch,
+                                           void *stream);
+/* inverse_compton::generate_random_gamma (rt_ic.cu:524-529): device array
+ * of n Lorentz factors log-uniform in [1.01, 1e10] */
+apt_status apt_ic_generate_random_gamma(apt_ic *h, double *gammas, uint64_t n, uint64_t seed,
+                                        uint32_t epoch, void *stream);
+/* Kernels::find_ic_rate / find_gg_rate (rt_ic.cu:156-171); host arrays */
+apt_status apt_ic_rates(apt_ic *h, const double *gamma, double *ic_rate, double *gg_rate,
+                        uint64_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -679,3 +679,66 @@ class GR1dOracle(OracleSim):
     def field_update(self, dt, step=0):
         self.lib.orc_gr1d_field_update(C.byref(self.mesh), self._cp, _d(self.f["E"][0]),
                                        _d(self.f["J"][0]), C.c_double(dt))
+
+
+class ICOracle:
+    """The inverse-Compton tables and sampling of ic_oracle.c (the
+    reference's inverse_compton, rt_ic.cu:22-547) for one spectrum."""
+
+    def __init__(self, n_gamma, n_ep):
+        self.lib = lib = load()
+        lib.orc_ic_spectrum.restype = C.c_double
+        lib.orc_ic_spectrum.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_double]
+        for name in ("orc_sigma_ic", "orc_sigma_gg"):
+            getattr(lib, name).restype = C.c_double
+            getattr(lib, name).argtypes = [C.c_double]
+        self.ng, self.nep = n_gamma, n_ep
+        self.gammas, self.ep, self.log_ep = np.empty(n_gamma), np.empty(n_ep), np.empty(n_ep)
+        dg, dep, dlep = C.c_double(), C.c_double(), C.c_double()
+        lib.orc_ic_grids(C.c_int(n_gamma), C.c_int(n_ep), _d(self.gammas), _d(self.ep),
+                         _d(self.log_ep), C.byref(dg), C.byref(dep), C.byref(dlep))
+        self.dg, self.dep, self.dlep = dg.value, dep.value, dlep.value
+
+    def init(self, kind, p, emin, emax, n0=1.0):
+        self.ic_rate, self.gg_rate = np.empty(self.ng), np.empty(self.ng)
+        self.dnde = np.empty((self.ng, self.nep))
+        self.dnde_th = np.empty((self.ng, self.nep))
+        pp = (C.c_double * 5)(*(list(p) + [0.0] * (5 - len(p))))
+        self.lib.orc_ic_init(C.c_int(kind), pp, C.c_double(emin), C.c_double(emax),
+                             C.c_double(n0), C.c_int(self.ng), C.c_int(self.nep),
+                             _d(self.gammas), _d(self.ep), _d(self.log_ep), _d(self.ic_rate),
+                             _d(self.gg_rate), _d(self.dnde), _d(self.dnde_th))
+
+    def spectrum(self, kind, p, e):
+        pp = (C.c_double * 5)(*(list(p) + [0.0] * (5 - len(p))))
+        return self.lib.orc_ic_spectrum(kind, pp, e)
+
+    def sigma_ic(self, x):
+        return self.lib.orc_sigma_ic(x)
+
+    def sigma_gg(self, b):
+        return self.lib.orc_sigma_gg(b)
+
+    def gen_photon_e(self, gamma, seed=0, epoch=0, accept=False):
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        out = np.empty_like(g)
+        self.lib.orc_ic_gen_photon_e(C.c_int(self.ng), C.c_int(self.nep), C.c_double(self.dg),
+                                     C.c_double(self.dep), C.c_double(self.dlep),
+                                     _d(self.gammas), _d(self.ic_rate), _d(self.dnde),
+                                     _d(self.dnde_th), _d(g), _d(out), C.c_uint64(g.size),
+                                     C.c_uint64(seed), C.c_uint32(epoch), C.c_int(int(accept)))
+        return out
+
+    def random_gamma(self, n, seed=0, epoch=0):
+        out = np.empty(n)
+        self.lib.orc_ic_random_gamma(_d(out), C.c_uint64(n), C.c_uint64(seed),
+                                     C.c_uint32(epoch))
+        return out
+
+    def rates(self, gamma):
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        ic, gg = np.empty_like(g), np.empty_like(g)
+        self.lib.orc_ic_rates(C.c_int(self.ng), C.c_double(self.dg), _d(self.gammas),
+                              _d(self.ic_rate), _d(self.gg_rate), _d(g), _d(ic), _d(gg),
+                              C.c_uint64(g.size))
+        return ic, gg

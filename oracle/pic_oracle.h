@@ -222,6 +222,24 @@ void orc_gr1d_filter(const orc_mesh *m, const double *j, double *tmp, const doub
 void orc_gr1d_field_update(const orc_mesh *m, double *const *coef, double *e1,
                            const double *j1, double dt);
 
+/* --- inverse-Compton tables (ic_oracle.c) --- */
+double orc_ic_spectrum(int kind, const double *p, double e);
+double orc_sigma_ic(double x);
+double orc_sigma_gg(double b);
+void orc_ic_grids(int ng, int nep, double *gammas, double *ep, double *log_ep, double *dg,
+                  double *dep, double *dlep);
+void orc_ic_init(int kind, const double *p, double emin, double emax, double n0, int ng,
+                 int nep, const double *gammas, const double *ep, const double *log_ep,
+                 double *ic_rate, double *gg_rate, double *dnde, double *dnde_th);
+void orc_ic_gen_photon_e(int ng, int nep, double dg, double dep, double dlep,
+                         const double *gammas, const double *ic_rate, const double *dnde,
+                         const double *dnde_th, const double *gamma, double *eph, uint64_t n,
+                         uint64_t seed, uint32_t epoch, int accept);
+void orc_ic_random_gamma(double *gammas, uint64_t n, uint64_t seed, uint32_t epoch);
+void orc_ic_rates(int ng, double dg, const double *gammas, const double *ic_rate,
+                  const double *gg_rate, const double *gamma, double *ic, double *gg,
+                  uint64_t n);
+
 /* --- diagnostics --- */
 void orc_energy_histogram(const orc_ptc *p, uint32_t *hist, int nbins,
                           double E_max, int flag_bit);

@@ -1,0 +1,113 @@
+"""GPU parity of the inverse-Compton tables and sampling (apt_ic_*, the
+reference's inverse_compton, src/cuda/radiation/rt_ic.cu:22-547) against
+the oracle: FP64 bit-exact in the grids, the IC and gamma-gamma rates, both
+cumulative spectra, the sampled photon energies (with and without the
+accept step), the interpolated rates and the random Lorentz factors; the
+reference's own test case (tests/test_ic.cu:24-69: broken power law, gamma
+= 10) gives energies in [0, gamma - 1.01]."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import aperture3_amd as apt
+import oracle_ctypes as oc
+from ic_cases import SPECTRA, THOMSON
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(name, ng=40, nep=40, n0=None):
+    kind, p, emin, emax, n0_ = SPECTRA[name]
+    n0 = n0_ if n0 is None else n0
+    ic = apt.inverse_compton(apt.SimParams(n_gamma=ng, n_ep=nep))
+    spec = [apt.Spectra.power_law_hard, apt.Spectra.power_law_soft,
+            apt.Spectra.broken_power_law, apt.Spectra.black_body,
+            apt.Spectra.mono_energetic][kind](*p)
+    ic.init(spec, emin, emax, n0)
+    o = oc.ICOracle(ng, nep)
+    o.init(kind, p, emin, emax, n0)
+    return ic, o
+
+
+def same(a, b, what):
+    assert a.shape == b.shape, what
+    assert np.array_equal(a, b, equal_nan=True), (what, np.nanmax(np.abs(a - b)))
+
+
+@pytest.mark.parametrize("name", sorted(SPECTRA))
+def test_tables_bit_exact(name):
+    print(f"ic tables: {name}", flush=True)
+    ic, o = pair(name)
+    same(ic.gammas(), o.gammas, "gammas")
+    same(ic.ep(), o.ep, "ep")
+    same(ic.log_ep(), o.log_ep, "log_ep")
+    same(ic.ic_rate(), o.ic_rate, "ic_rate")
+    same(ic.gg_rate(), o.gg_rate, "gg_rate")
+    same(ic.dNde(), o.dnde, "dNde")
+    same(ic.dNde_thomson(), o.dnde_th, "dNde_thomson")
+    assert np.isfinite(o.ic_rate).all() and (o.ic_rate > 0).any()
+
+
+def test_ragged_grid_and_sampling_bit_exact():
+    print("ic: 23 x 37 grid, samples and rates", flush=True)
+    ic, o = pair("soft_pl", ng=23, nep=37)
+    same(ic.dNde(), o.dnde, "dNde")
+    same(ic.dNde_thomson(), o.dnde_th, "dNde_thomson")
+    rng = np.random.default_rng(4)
+    g = np.concatenate([np.exp(rng.uniform(np.log(0.5), np.log(1e16), 5000)),
+                        ic.gammas(), [1.01, 2.0, 1e15, 2e15, 1.0]])
+    same(ic.gen_photon_e(g, seed=77, epoch=5), o.gen_photon_e(g, seed=77, epoch=5), "eph")
+    a, b = ic.rates(g), o.rates(g)
+    same(a[0], b[0], "ic rate")
+    same(a[1], b[1], "gg rate")
+
+
+def test_device_entry_points_bit_exact():
+    """generate_photon_energies (accept step), gen_photon_e and
+    generate_random_gamma on device tensors, n0 chosen so that about half
+    of the samples scatter."""
+    print("ic: device tensors", flush=True)
+    ic, o = pair("soft_pl", n0=0.5 / THOMSON)
+    n = 30001
+    gam = torch.empty(n, dtype=torch.float64, device="cuda")
+    ic.generate_random_gamma(gam, seed=9, epoch=1)
+    gh = gam.cpu().numpy()
+    same(gh, o.random_gamma(n, seed=9, epoch=1), "random gamma")
+    eph = torch.empty_like(gam)
+    ic.generate_photon_energies(eph, gam, seed=9, epoch=2)
+    expect = o.gen_photon_e(gh, seed=9, epoch=2, accept=True)
+    same(eph.cpu().numpy(), expect, "generate_photon_energies")
+    assert 0.05 < (expect > 0).mean() < 0.95
+    ic.gen_photon_e_device(eph, gam, seed=9, epoch=3)
+    same(eph.cpu().numpy(), o.gen_photon_e(gh, seed=9, epoch=3), "gen_photon_e device")
+
+
+def test_reference_case_gamma_10():
+    """tests/test_ic.cu:24-69 (which prints the energies): 100 electrons at
+    gamma = 10 scattering the broken power law; the reference tables are
+    600 x 600 (sim_params.h:87-88)."""
+    print("ic: reference case 600 x 600", flush=True)
+    kind, p, emin, emax, n0 = SPECTRA["test_ic_cu"]
+    ic = apt.inverse_compton(apt.SimParams())
+    assert (ic.n_gamma, ic.n_ep) == (600, 600)
+    ic.init(apt.Spectra.broken_power_law(*p), emin, emax, n0)
+    e = ic.gen_photon_e(np.full(100, 10.0), seed=1234)
+    assert np.all(np.isfinite(e)) and np.all(e >= 0) and np.all(e <= 10.0 - 1.01)
+    cdf = ic.dNde()
+    rows = ~np.isnan(cdf).any(axis=1)
+    assert np.all(cdf[rows, -1] == 1.0)
+
+
+def test_errors():
+    lib = apt.load_library()
+    h = C.c_void_p()
+    assert lib.apt_ic_create(2, 40, C.byref(h)) == 1
+    ic = apt.inverse_compton(apt.SimParams(n_gamma=8, n_ep=8))
+    with pytest.raises(apt.AptError, match="apt_ic_init has not run"):
+        ic.ic_rate()
+    with pytest.raises(apt.AptError, match="apt_ic_init has not run"):
+        ic.gen_photon_e(np.ones(4))
+    with pytest.raises(apt.AptError, match="bad spectrum"):
+        ic.init(apt.Spectra.mono_energetic(1e-5, 1e-6), 1e-3, 1e-7)
