@@ -209,6 +209,15 @@ SIGNATURES = {
     "apt_ic_gen_photon_e_dev": [_P, _P, _P, C.c_uint64, C.c_uint64, C.c_uint32, _P],
     "apt_ic_generate_photon_energies": [_P, _P, _P, C.c_uint64, C.c_uint64, C.c_uint32, _P],
     "apt_ic_generate_random_gamma": [_P, _P, C.c_uint64, C.c_uint64, C.c_uint32, _P],
+    "apt_tpp_create": [C.c_int, C.POINTER(_P)],
+    "apt_tpp_destroy": [_P],
+    "apt_tpp_last_error": [_P],
+    "apt_tpp_init": [_P, C.POINTER(AptICSpectrum), C.c_double, C.c_double, C.c_double],
+    "apt_tpp_get_table": [_P, C.c_int, C.POINTER(C.c_double), C.c_uint64],
+    "apt_tpp_gen_ep": [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint64,
+                       C.c_uint64, C.c_uint32],
+    "apt_tpp_rates": [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                      C.c_uint64],
     "apt_ic_rates": [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                      C.c_uint64],
 }
@@ -936,6 +945,74 @@ class inverse_compton:
         if gamma > self.MAX_GAMMA:
             return self.n_gamma - 1
         return int((np.log(gamma) - np.log(self.MIN_GAMMA)) / self.m_dg)
+
+
+TPP_GAMMAS, TPP_RATE, TPP_EM, TPP_DNDE = range(4)
+
+
+class triplet_pairs:
+    """triplet_pairs (src/cuda/radiation/rt_tpp.h:10-33): triplet rate, mean
+    pair energy and pair-energy CDF on the inverse-Compton gamma grid, built
+    on the GPU by apt_tpp_init."""
+
+    def __init__(self, params: SimParams):
+        self.lib = load_library()
+        self.n_gamma = int(params.n_gamma)
+        h = C.c_void_p()
+        st = self.lib.apt_tpp_create(self.n_gamma, C.byref(h))
+        if st != 0:
+            raise AptError(f"apt_tpp_create({self.n_gamma}): status {st}")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.lib.apt_tpp_destroy(h)
+            self.h = None
+
+    def _check(self, st, what):
+        if st != 0:
+            raise AptError(f"{what}: status {st}: {self.lib.apt_tpp_last_error(self.h).decode()}")
+
+    def init(self, n_e, emin: float, emax: float, n0: float = 1.0):
+        spec = n_e.struct()
+        self._check(self.lib.apt_tpp_init(self.h, C.byref(spec), emin, emax, n0), "apt_tpp_init")
+
+    def _table(self, which):
+        n = self.n_gamma * (self.n_gamma if which == TPP_DNDE else 1)
+        out = np.empty(n)
+        self._check(self.lib.apt_tpp_get_table(self.h, which, _dptr(out), n),
+                    "apt_tpp_get_table")
+        return out.reshape(self.n_gamma, self.n_gamma) if which == TPP_DNDE else out
+
+    def gammas(self):
+        return self._table(TPP_GAMMAS)
+
+    def rate(self):
+        return self._table(TPP_RATE)
+
+    def Em(self):
+        return self._table(TPP_EM)
+
+    def dNde(self):
+        """m_dNde indexed [gamma, pair energy]."""
+        return self._table(TPP_DNDE)
+
+    def gen_tpp_Ep(self, gamma, seed: int = 0, epoch: int = 0):
+        """Kernels::gen_tpp_Ep (rt_tpp.cu:132-143) per Lorentz factor."""
+        g = np.ascontiguousarray(gamma, dtype=np.float64).ravel()
+        out = np.empty_like(g)
+        self._check(self.lib.apt_tpp_gen_ep(self.h, _dptr(g), _dptr(out), g.size, seed, epoch),
+                    "apt_tpp_gen_ep")
+        return out
+
+    def rates(self, gamma):
+        """(find_tpp_rate, find_tpp_Em) per Lorentz factor (rt_tpp.cu:114-130)."""
+        g = np.ascontiguousarray(gamma, dtype=np.float64).ravel()
+        r, m = np.empty_like(g), np.empty_like(g)
+        self._check(self.lib.apt_tpp_rates(self.h, _dptr(g), _dptr(r), _dptr(m), g.size),
+                    "apt_tpp_rates")
+        return r, m
 
 
 # ---------------------------------------------------------------------------

@@ -1,7 +1,8 @@
 // kern_ic.hip -- inverse-Compton scattering tables and photon-energy
 // sampling (the reference's inverse_compton class,
 // src/cuda/radiation/rt_ic.cu:22-547, with the soft-photon spectra of
-// src/radiation/spectra.h).
+// src/radiation/spectra.h), and the triplet pair-production tables of
+// triplet_pairs (src/cuda/radiation/rt_tpp.cu:21-294) on the same grids.
 //
 // The reference builds its tables with host loops (rt_ic.cu:292-421, the
 // device kernel is commented out); here every table entry is one GPU thread
@@ -263,6 +264,141 @@ __global__ void k_ic_rates(Tables t, const double *gamma, double *ic, double *gg
   gg[k] = g < 2.0 ? 0.0 : ic_find_rate(t, t.gg_rate, g, gn);
 }
 
+// ---- triplet pair production (rt_tpp.cu:21-294) ----------------------
+
+// total_f (rt_tpp.cu:21-43): the triplet cross section over alpha_f r_e^2
+__host__ __device__ inline double tpp_total_f(double s) {
+  if (s < 4.0) return 0.0;
+  if (s < 4.6) {
+    const double s4 = s - 4.0;
+    return 0.001 * s4 * s4 *
+           (5.6 + 20.4 * s4 - 10.9 * s4 * s4 - 3.6 * s4 * s4 * s4 + 7.4 * s4 * s4 * s4 * s4);
+  }
+  if (s < 6.0) return 0.582814 - 0.29842 * s + 0.04354 * s * s - 0.0012977 * s * s * s;
+  if (s < 14.0)
+    return (3.1247 - 1.3394 * s + 0.14612 * s * s) / (1 + 0.4648 * s + 0.016683 * s * s);
+  const double l2s = apt_log(2.0 * s);
+  return l2s * 28.0 / 9.0 - 218.0 / 27.0 +
+         (-(4.0 / 3.0) * l2s * l2s * l2s + 3.863 * l2s * l2s - 11.0 * l2s + 27.9) / s;
+}
+
+// approx_Em (rt_tpp.cu:45-60): mean pair energy fraction
+__host__ __device__ inline double tpp_approx_em(double s) {
+  if (s <= 4.01) return 0.0;
+  if (s <= 100.0)
+    return apt_exp((apt_log(1.337) + apt_log(5.2495 / 1.337) * apt_log(s / 4.01) /
+                                         apt_log(100.0 / 4.01))) /
+           s;
+  const double l2s = apt_log(2.0 * s);
+  return 0.187 * l2s * l2s / s;
+}
+
+// E_plus_min / E_plus_max (rt_tpp.cu:62-74); NaN below s = 4
+__host__ __device__ inline double tpp_e_plus_min(double gamma, double s) {
+  return (gamma * (s - 1.0f) - sqrt((gamma * gamma - 1.0f) * s * (s - 4.0f))) / (1.0f + 2.0f * s);
+}
+__host__ __device__ inline double tpp_e_plus_max(double gamma, double s) {
+  return (gamma * (s - 1.0f) + sqrt((gamma * gamma - 1.0f) * s * (s - 4.0f))) / (1.0f + 2.0f * s);
+}
+
+// rate and mean pair energy per gamma (rt_tpp.cu:200-229)
+__global__ void k_tpp_rate(Spectrum ne, Grid g, const double *gammas, int ng, double *rate,
+                           double *em) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= ng) return;
+  const double gamma = gammas[n];
+  double result = 0.0, emean = 0.0;
+  for (int i_mu = 0; i_mu < IC_N_MU; i_mu++) {
+    const double mu = i_mu * g.dmu - 1.0;
+    for (int i_e = 0; i_e < IC_N_E; i_e++) {
+      const double e = apt_exp(apt_log(g.emin) + i_e * g.de);
+      const double b = ic_beta(gamma);
+      const double s = gamma * e * (1.0 - b * mu);
+      result += 0.5f * ne(e) * tpp_total_f(s) * (1.0 - b * mu) * e;
+      emean += 0.5f * ne(e) * tpp_total_f(s) * tpp_approx_em(s) * (1.0 - b * mu) * e;
+    }
+  }
+  rate[n] = result * g.dmu * g.de * g.n0 * RE_SQUARE / 137.0;
+  em[n] = result > 0.0 ? emean / result : 0.0;
+}
+
+// pair-energy spectrum at (gamma n, E_p = gammas[i]) before the cumulative
+// sum (rt_tpp.cu:233-262)
+__global__ void k_tpp_dnde(Spectrum ne, Grid g, const double *gammas, int ng, const double *rate,
+                           const double *em, double *table) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ng * ng) return;
+  const int n = t / ng, i = t % ng;
+  const double gamma = gammas[n];
+  const double e_p = gammas[i];
+  double *out = table + (size_t)n * ng + i;
+  if (e_p > gamma) {
+    *out = 0.0;
+    return;
+  }
+  double result = 0.0;
+  for (int i_e = 0; i_e < IC_N_E; i_e++) {
+    const double e = apt_exp(apt_log(g.emin) + i_e * g.de);
+    const double nv = ne(e);
+    const double s = gamma * e;
+    const double emin = tpp_e_plus_min(gamma, s);
+    const double emax = tpp_e_plus_max(gamma, s);
+    if (e_p < emin || e_p > emax) continue;
+    const double norm = 4.0 * (ic_pow(emax, 0.25) - ic_pow(emin, 0.25));
+    result += nv * rate[n] * em[n] * ic_pow(e_p, -1.75) * e / norm;
+  }
+  *out = result * g.de * g.dmu;
+}
+
+struct TppTables {
+  const double *gammas, *rate, *em, *dnde;
+  int ng;
+  double dg;
+};
+
+__device__ int tpp_find_n_gamma(const TppTables &t, double gamma) {
+  if (gamma < IC_MIN_GAMMA) return 0;
+  if (gamma > IC_MAX_GAMMA) return t.ng - 1;
+  const int gn = (int)((apt_log(gamma) - apt_log(IC_MIN_GAMMA)) / t.dg);
+  return gn < 0 ? 0 : (gn > t.ng - 1 ? t.ng - 1 : gn);
+}
+
+// gen_tpp_Ep (rt_tpp.cu:132-143) with u the first uniform of the Philox
+// block (seed, k, epoch, 11)
+__global__ void k_tpp_sample(TppTables t, const double *gamma, double *ep, uint64_t n,
+                             uint64_t seed, uint32_t epoch) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double u0, u1;
+  uniform2(seed, k, epoch, 11, u0, u1);
+  const float u = (float)u0;
+  const double g = gamma[k];
+  const int gn = tpp_find_n_gamma(t, g);
+  double l = 0.0, h = 0.0;
+  const int b = ic_search(u, t.dnde + (size_t)gn * t.ng, t.ng, l, h);
+  const double bb = (l == h ? b : (u - l) / (h - l) + b);
+  const double result = apt_exp(apt_log(IC_MIN_GAMMA) + t.dg * bb);
+  ep[k] = fmin(result, g - 1.01);
+}
+
+// find_tpp_rate / find_tpp_Em (rt_tpp.cu:114-130)
+__global__ void k_tpp_rates(TppTables t, const double *gamma, double *rate, double *em,
+                            uint64_t n) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double g = gamma[k];
+  const int gn = tpp_find_n_gamma(t, g);
+  const int gu = gn + 1 < t.ng ? gn + 1 : gn;
+  if (g > IC_MAX_GAMMA) {
+    rate[k] = t.rate[gn];
+    em[k] = t.em[gn];
+  } else {
+    const double x = (apt_log(g) - apt_log(t.gammas[gn])) / t.dg;
+    rate[k] = t.rate[gn] * (1.0f - x) + t.rate[gu] * x;
+    em[k] = t.em[gn] * (1.0f - x) + t.em[gu] * x;
+  }
+}
+
 }  // namespace
 }  // namespace apt
 
@@ -275,6 +411,20 @@ struct apt_ic {
   std::vector<double> gammas, ep, log_ep;
   double *d_gammas = nullptr, *d_ep = nullptr, *d_log_ep = nullptr;
   double *d_ic = nullptr, *d_gg = nullptr, *d_dnde = nullptr, *d_dnde_th = nullptr;
+  bool ready = false;
+  std::string err;
+  apt_status fail(apt_status st, const char *msg) {
+    err = msg;
+    return st;
+  }
+};
+
+struct apt_tpp {
+  hipStream_t stream = nullptr;
+  int ng = 0;
+  double dg = 0;
+  std::vector<double> gammas;
+  double *d_gammas = nullptr, *d_rate = nullptr, *d_em = nullptr, *d_dnde = nullptr;
   bool ready = false;
   std::string err;
   apt_status fail(apt_status st, const char *msg) {
@@ -499,6 +649,136 @@ apt_status apt_ic_rates(apt_ic *h, const double *gamma, double *ic_rate, double 
   if (e == hipSuccess) e = hipMemcpy(gg_rate, d + 2 * n, n * 8, hipMemcpyDeviceToHost);
   (void)hipFree(d);
   return e == hipSuccess ? APT_OK : h->fail(APT_ERR_HIP, hipGetErrorString(e));
+}
+
+// triplet_pairs::triplet_pairs (rt_tpp.cu:147-184): the gamma grid, the
+// same as the inverse-Compton one
+apt_status apt_tpp_create(int n_gamma, apt_tpp **out) {
+  if (!out || n_gamma < 3 || n_gamma > 20000) return APT_ERR_INVALID_ARG;
+  apt_tpp *h = new apt_tpp();
+  h->ng = n_gamma;
+  h->dg = (apt_log(IC_MAX_GAMMA) - apt_log(IC_MIN_GAMMA)) / ((double)n_gamma - 1.0);
+  h->gammas.resize(n_gamma);
+  for (int n = 0; n < n_gamma; n++)
+    h->gammas[n] = apt_exp(apt_log(IC_MIN_GAMMA) + h->dg * (double)n);
+  hipError_t e = hipStreamCreate(&h->stream);
+  if (e == hipSuccess) e = hipMalloc(&h->d_gammas, n_gamma * 8);
+  if (e == hipSuccess) e = hipMalloc(&h->d_rate, n_gamma * 8);
+  if (e == hipSuccess) e = hipMalloc(&h->d_em, n_gamma * 8);
+  if (e == hipSuccess) e = hipMalloc(&h->d_dnde, (size_t)n_gamma * n_gamma * 8);
+  if (e == hipSuccess)
+    e = hipMemcpy(h->d_gammas, h->gammas.data(), n_gamma * 8, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    apt_tpp_destroy(h);
+    return APT_ERR_HIP;
+  }
+  *out = h;
+  return APT_OK;
+}
+
+apt_status apt_tpp_destroy(apt_tpp *h) {
+  if (!h) return APT_ERR_INVALID_ARG;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (double *p : {h->d_gammas, h->d_rate, h->d_em, h->d_dnde}) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return APT_OK;
+}
+
+const char *apt_tpp_last_error(const apt_tpp *h) { return h ? h->err.c_str() : "null handle"; }
+
+// triplet_pairs::init (rt_tpp.cu:188-275)
+apt_status apt_tpp_init(apt_tpp *h, const apt_ic_spectrum *spec, double emin, double emax,
+                        double n0) {
+  if (!h) return APT_ERR_INVALID_ARG;
+  if (!spec || spec->kind < 0 || spec->kind > APT_SPEC_MONO_ENERGETIC || !(emin > 0.0) ||
+      !(emax > emin))
+    return h->fail(APT_ERR_INVALID_ARG, "tpp_init: bad spectrum or energy range");
+  Spectrum ne;
+  ne.kind = spec->kind;
+  for (int k = 0; k < 5; k++) ne.p[k] = spec->p[k];
+  Grid g;
+  g.emin = emin;
+  g.n0 = n0;
+  g.dmu = 2.0 / (IC_N_MU - 1.0);
+  g.de = (apt_log(emax) - apt_log(emin)) / (IC_N_E - 1.0);
+  const int ng = h->ng;
+  k_tpp_rate<<<(ng + 63) / 64, 64, 0, h->stream>>>(ne, g, h->d_gammas, ng, h->d_rate, h->d_em);
+  const int cells = ng * ng;
+  k_tpp_dnde<<<(cells + 255) / 256, 256, 0, h->stream>>>(ne, g, h->d_gammas, ng, h->d_rate,
+                                                         h->d_em, h->d_dnde);
+  k_ic_cdf<<<(ng + 63) / 64, 64, 0, h->stream>>>(h->d_dnde, ng, ng);
+  ICCHK(hipGetLastError());
+  ICCHK(hipStreamSynchronize(h->stream));
+  h->ready = true;
+  return APT_OK;
+}
+
+apt_status apt_tpp_get_table(apt_tpp *h, int which, double *dst, uint64_t count) {
+  if (!h) return APT_ERR_INVALID_ARG;
+  const uint64_t ng = h->ng;
+  const double *src = nullptr;
+  uint64_t n = 0;
+  switch (which) {
+    case APT_TPP_GAMMAS: src = h->d_gammas; n = ng; break;
+    case APT_TPP_RATE: src = h->d_rate; n = ng; break;
+    case APT_TPP_EM: src = h->d_em; n = ng; break;
+    case APT_TPP_DNDE: src = h->d_dnde; n = ng * ng; break;
+    default: return h->fail(APT_ERR_INVALID_ARG, "tpp_get_table: unknown table");
+  }
+  if (!dst || count != n) return h->fail(APT_ERR_INVALID_ARG, "tpp_get_table: bad size");
+  if (which != APT_TPP_GAMMAS && !h->ready)
+    return h->fail(APT_ERR_STATE, "tpp_get_table: apt_tpp_init has not run");
+  ICCHK(hipStreamSynchronize(h->stream));
+  ICCHK(hipMemcpy(dst, src, n * 8, hipMemcpyDeviceToHost));
+  return APT_OK;
+}
+
+static TppTables tpp_tables_of(const apt_tpp *h) {
+  TppTables t;
+  t.gammas = h->d_gammas;
+  t.rate = h->d_rate;
+  t.em = h->d_em;
+  t.dnde = h->d_dnde;
+  t.ng = h->ng;
+  t.dg = h->dg;
+  return t;
+}
+
+// host arrays in and out: gen_tpp_Ep and (find_tpp_rate, find_tpp_Em)
+static apt_status tpp_run(apt_tpp *h, const double *gamma, double *o1, double *o2, uint64_t n,
+                          uint64_t seed, uint32_t epoch, bool sample) {
+  if (!h) return APT_ERR_INVALID_ARG;
+  if (!h->ready) return h->fail(APT_ERR_STATE, "tpp: apt_tpp_init has not run");
+  if (n == 0) return APT_OK;
+  if (!gamma || !o1 || (!sample && !o2)) return h->fail(APT_ERR_INVALID_ARG, "tpp: null arrays");
+  if (n > IC_MAX_SAMPLES) return h->fail(APT_ERR_INVALID_ARG, "tpp: too many samples");
+  double *d = nullptr;
+  hipError_t e = hipMalloc(&d, n * 8 * 3);
+  if (e == hipSuccess) e = hipMemcpy(d, gamma, n * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (sample)
+      k_tpp_sample<<<blocks, 256, 0, h->stream>>>(tpp_tables_of(h), d, d + n, n, seed, epoch);
+    else
+      k_tpp_rates<<<blocks, 256, 0, h->stream>>>(tpp_tables_of(h), d, d + n, d + 2 * n, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(o1, d + n, n * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && !sample) e = hipMemcpy(o2, d + 2 * n, n * 8, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return e == hipSuccess ? APT_OK : h->fail(APT_ERR_HIP, hipGetErrorString(e));
+}
+
+apt_status apt_tpp_gen_ep(apt_tpp *h, const double *gamma, double *e_p, uint64_t n,
+                          uint64_t seed, uint32_t epoch) {
+  return tpp_run(h, gamma, e_p, nullptr, n, seed, epoch, true);
+}
+
+apt_status apt_tpp_rates(apt_tpp *h, const double *gamma, double *rate, double *em,
+                         uint64_t n) {
+  return tpp_run(h, gamma, rate, em, n, 0, 0, false);
 }
 
 }  // extern "C"

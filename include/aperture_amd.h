@@ -491,6 +491,36 @@ apt_status apt_ic_generate_random_gamma(apt_ic *h, double *gammas, uint64_t n, u
 apt_status apt_ic_rates(apt_ic *h, const double *gamma, double *ic_rate, double *gg_rate,
                         uint64_t n);
 
+/* ---- triplet pair-production tables (rt_tpp.cu:21-294) --------------- */
+/* The reference's triplet_pairs (a file its build does not compile): the
+ * triplet rate and mean pair energy per Lorentz factor on the
+ * inverse-Compton gamma grid, and the cumulative pair-energy spectrum
+ * (n_gamma x n_gamma, pair energy fastest).  Rows where some soft photon
+ * has s = gamma e < 4 are NaN, as in the reference (E_plus_min takes the
+ * square root of a negative number there). */
+typedef struct apt_tpp apt_tpp;
+enum {
+  APT_TPP_GAMMAS = 0, /* n_gamma */
+  APT_TPP_RATE = 1,   /* n_gamma */
+  APT_TPP_EM = 2,     /* n_gamma: mean pair energy fraction */
+  APT_TPP_DNDE = 3    /* n_gamma x n_gamma: CDF over pair energy = gammas[i] */
+};
+/* triplet_pairs(params) (rt_tpp.cu:147-184) */
+apt_status apt_tpp_create(int n_gamma, apt_tpp **out);
+apt_status apt_tpp_destroy(apt_tpp *h);
+const char *apt_tpp_last_error(const apt_tpp *h);
+/* triplet_pairs::init(n_e, emin, emax, n0) (rt_tpp.cu:188-275), on the GPU */
+apt_status apt_tpp_init(apt_tpp *h, const apt_ic_spectrum *spec, double emin, double emax,
+                        double n0);
+apt_status apt_tpp_get_table(apt_tpp *h, int which, double *dst, uint64_t count);
+/* Kernels::gen_tpp_Ep (rt_tpp.cu:132-143) for n Lorentz factors, the draw
+ * of index k from the Philox stream (seed, k, epoch); host arrays */
+apt_status apt_tpp_gen_ep(apt_tpp *h, const double *gamma, double *e_p, uint64_t n,
+                          uint64_t seed, uint32_t epoch);
+/* Kernels::find_tpp_rate / find_tpp_Em (rt_tpp.cu:114-130); host arrays */
+apt_status apt_tpp_rates(apt_tpp *h, const double *gamma, double *rate, double *em,
+                         uint64_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,8 +1,9 @@
 /*
  * ic_oracle.c -- CPU restatement of the reference's inverse-Compton tables
  * and sampling (src/cuda/radiation/rt_ic.cu:22-547; soft-photon spectra of
- * src/radiation/spectra.h:10-95).  TEST INFRASTRUCTURE ONLY: the checker
- * for the HIP path, never linked into the product.
+ * src/radiation/spectra.h:10-95) and of its triplet pair-production tables
+ * (src/cuda/radiation/rt_tpp.cu:21-294).  TEST INFRASTRUCTURE ONLY: the
+ * checker for the HIP path, never linked into the product.
  *
  * The tables are the reference's host loops (rt_ic.cu:279-421) with this
  * build's exp/log (orc_math.c) and pow(x, y) = exp(y ln x), so the GPU
@@ -241,5 +242,118 @@ void orc_ic_rates(int ng, double dg, const double *gammas, const double *ic_rate
     int gn = find_n_gamma(g, ng, dg);
     ic[k] = find_rate(ng, dg, gammas, ic_rate, g, gn);
     gg[k] = g < 2.0 ? 0.0 : find_rate(ng, dg, gammas, gg_rate, g, gn);
+  }
+}
+
+/* ---- triplet pair production (rt_tpp.cu:21-294) ---------------------- */
+
+/* total_f (rt_tpp.cu:21-43) */
+double orc_tpp_total_f(double s) {
+  if (s < 4.0) return 0.0;
+  if (s < 4.6) {
+    double s4 = s - 4.0;
+    return 0.001 * s4 * s4 *
+           (5.6 + 20.4 * s4 - 10.9 * s4 * s4 - 3.6 * s4 * s4 * s4 + 7.4 * s4 * s4 * s4 * s4);
+  }
+  if (s < 6.0) return 0.582814 - 0.29842 * s + 0.04354 * s * s - 0.0012977 * s * s * s;
+  if (s < 14.0)
+    return (3.1247 - 1.3394 * s + 0.14612 * s * s) / (1 + 0.4648 * s + 0.016683 * s * s);
+  double l2s = orc_log(2.0 * s);
+  return l2s * 28.0 / 9.0 - 218.0 / 27.0 +
+         (-(4.0 / 3.0) * l2s * l2s * l2s + 3.863 * l2s * l2s - 11.0 * l2s + 27.9) / s;
+}
+
+/* approx_Em (rt_tpp.cu:45-60) */
+double orc_tpp_approx_em(double s) {
+  if (s <= 4.01) return 0.0;
+  if (s <= 100.0)
+    return orc_exp((orc_log(1.337) + orc_log(5.2495 / 1.337) * orc_log(s / 4.01) /
+                                         orc_log(100.0 / 4.01))) /
+           s;
+  double l2s = orc_log(2.0 * s);
+  return 0.187 * l2s * l2s / s;
+}
+
+/* E_plus_min / E_plus_max (rt_tpp.cu:62-74): NaN below s = 4 */
+void orc_tpp_e_plus(double gamma, double s, double *emin, double *emax) {
+  double r = sqrt((gamma * gamma - 1.0f) * s * (s - 4.0f));
+  *emin = (gamma * (s - 1.0f) - r) / (1.0f + 2.0f * s);
+  *emax = (gamma * (s - 1.0f) + r) / (1.0f + 2.0f * s);
+}
+
+/* triplet_pairs::init (rt_tpp.cu:188-275): rate and Em per gamma, then the
+ * n_gamma x n_gamma pair-energy CDF; the gamma grid is orc_ic_grids' */
+void orc_tpp_init(int kind, const double *p, double emin, double emax, double n0, int ng,
+                  const double *gammas, double *rate, double *em, double *dnde) {
+  const double dmu = 2.0 / (IC_N_MU - 1.0);
+  const double de = (orc_log(emax) - orc_log(emin)) / (IC_N_E - 1.0);
+  for (int n = 0; n < ng; n++) {
+    double gamma = gammas[n], result = 0.0, emean = 0.0;
+    for (int i_mu = 0; i_mu < IC_N_MU; i_mu++) {
+      double mu = i_mu * dmu - 1.0;
+      for (int i_e = 0; i_e < IC_N_E; i_e++) {
+        double e = orc_exp(orc_log(emin) + i_e * de);
+        double b = ic_beta(gamma);
+        double s = gamma * e * (1.0 - b * mu);
+        double nv = orc_ic_spectrum(kind, p, e);
+        result += 0.5f * nv * orc_tpp_total_f(s) * (1.0 - b * mu) * e;
+        emean += 0.5f * nv * orc_tpp_total_f(s) * orc_tpp_approx_em(s) * (1.0 - b * mu) * e;
+      }
+    }
+    rate[n] = result * dmu * de * n0 * IC_RE_SQUARE / 137.0;
+    em[n] = result > 0.0 ? emean / result : 0.0;
+  }
+  for (int n = 0; n < ng; n++) {
+    double gamma = gammas[n];
+    double *row = dnde + (size_t)n * ng;
+    for (int i = 0; i < ng; i++) {
+      double e_p = gammas[i];
+      if (e_p > gamma) {
+        row[i] = 0.0;
+        continue;
+      }
+      double result = 0.0;
+      for (int i_e = 0; i_e < IC_N_E; i_e++) {
+        double e = orc_exp(orc_log(emin) + i_e * de);
+        double nv = orc_ic_spectrum(kind, p, e);
+        double s = gamma * e, lo, hi;
+        orc_tpp_e_plus(gamma, s, &lo, &hi);
+        if (e_p < lo || e_p > hi) continue;
+        double norm = 4.0 * (ic_pow(hi, 0.25) - ic_pow(lo, 0.25));
+        result += nv * rate[n] * em[n] * ic_pow(e_p, -1.75) * e / norm;
+      }
+      row[i] = result * de * dmu;
+    }
+    for (int i = 1; i < ng; i++) row[i] += row[i - 1];
+    double last = row[ng - 1];
+    for (int i = 0; i < ng; i++) row[i] /= last;
+  }
+}
+
+/* gen_tpp_Ep (rt_tpp.cu:132-143), u the first uniform of the Philox block
+ * (seed, k, epoch, purpose 11) */
+void orc_tpp_gen_ep(int ng, double dg, const double *dnde, const double *gamma, double *ep,
+                    uint64_t n, uint64_t seed, uint32_t epoch) {
+  for (uint64_t k = 0; k < n; k++) {
+    double uu[2], l = 0.0, h = 0.0;
+    orc_uniform2(seed, k, epoch, 11, uu);
+    float u = (float)uu[0];
+    double g = gamma[k];
+    int gn = find_n_gamma(g, ng, dg);
+    int b = ic_search(u, dnde + (size_t)gn * ng, ng, &l, &h);
+    double bb = (l == h ? b : (u - l) / (h - l) + b);
+    double result = orc_exp(orc_log(IC_MIN_GAMMA) + dg * bb);
+    ep[k] = fmin(result, g - 1.01);
+  }
+}
+
+/* find_tpp_rate / find_tpp_Em (rt_tpp.cu:114-130) */
+void orc_tpp_rates(int ng, double dg, const double *gammas, const double *rate,
+                   const double *em, const double *gamma, double *r, double *m, uint64_t n) {
+  for (uint64_t k = 0; k < n; k++) {
+    double g = gamma[k];
+    int gn = find_n_gamma(g, ng, dg);
+    r[k] = find_rate(ng, dg, gammas, rate, g, gn);
+    m[k] = find_rate(ng, dg, gammas, em, g, gn);
   }
 }

@@ -742,3 +742,52 @@ class ICOracle:
                               _d(self.ic_rate), _d(self.gg_rate), _d(g), _d(ic), _d(gg),
                               C.c_uint64(g.size))
         return ic, gg
+
+
+class TPPOracle:
+    """The triplet pair-production tables and sampling of ic_oracle.c (the
+    reference's triplet_pairs, rt_tpp.cu:21-294)."""
+
+    def __init__(self, n_gamma):
+        self.lib = lib = load()
+        for name in ("orc_tpp_total_f", "orc_tpp_approx_em"):
+            getattr(lib, name).restype = C.c_double
+            getattr(lib, name).argtypes = [C.c_double]
+        self.ng = n_gamma
+        ic = ICOracle(n_gamma, 3)
+        self.gammas, self.dg = ic.gammas, ic.dg
+
+    def init(self, kind, p, emin, emax, n0=1.0):
+        self.rate, self.em = np.empty(self.ng), np.empty(self.ng)
+        self.dnde = np.empty((self.ng, self.ng))
+        pp = (C.c_double * 5)(*(list(p) + [0.0] * (5 - len(p))))
+        self.lib.orc_tpp_init(C.c_int(kind), pp, C.c_double(emin), C.c_double(emax),
+                              C.c_double(n0), C.c_int(self.ng), _d(self.gammas), _d(self.rate),
+                              _d(self.em), _d(self.dnde))
+
+    def total_f(self, s):
+        return self.lib.orc_tpp_total_f(s)
+
+    def approx_em(self, s):
+        return self.lib.orc_tpp_approx_em(s)
+
+    def e_plus(self, gamma, s):
+        lo, hi = C.c_double(), C.c_double()
+        self.lib.orc_tpp_e_plus(C.c_double(gamma), C.c_double(s), C.byref(lo), C.byref(hi))
+        return lo.value, hi.value
+
+    def gen_ep(self, gamma, seed=0, epoch=0):
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        out = np.empty_like(g)
+        self.lib.orc_tpp_gen_ep(C.c_int(self.ng), C.c_double(self.dg), _d(self.dnde), _d(g),
+                                _d(out), C.c_uint64(g.size), C.c_uint64(seed),
+                                C.c_uint32(epoch))
+        return out
+
+    def rates(self, gamma):
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        r, m = np.empty_like(g), np.empty_like(g)
+        self.lib.orc_tpp_rates(C.c_int(self.ng), C.c_double(self.dg), _d(self.gammas),
+                               _d(self.rate), _d(self.em), _d(g), _d(r), _d(m),
+                               C.c_uint64(g.size))
+        return r, m

@@ -236,6 +236,15 @@ void orc_ic_gen_photon_e(int ng, int nep, double dg, double dep, double dlep,
                          const double *dnde_th, const double *gamma, double *eph, uint64_t n,
                          uint64_t seed, uint32_t epoch, int accept);
 void orc_ic_random_gamma(double *gammas, uint64_t n, uint64_t seed, uint32_t epoch);
+double orc_tpp_total_f(double s);
+double orc_tpp_approx_em(double s);
+void orc_tpp_e_plus(double gamma, double s, double *emin, double *emax);
+void orc_tpp_init(int kind, const double *p, double emin, double emax, double n0, int ng,
+                  const double *gammas, double *rate, double *em, double *dnde);
+void orc_tpp_gen_ep(int ng, double dg, const double *dnde, const double *gamma, double *ep,
+                    uint64_t n, uint64_t seed, uint32_t epoch);
+void orc_tpp_rates(int ng, double dg, const double *gammas, const double *rate,
+                   const double *em, const double *gamma, double *r, double *m, uint64_t n);
 void orc_ic_rates(int ng, double dg, const double *gammas, const double *ic_rate,
                   const double *gg_rate, const double *gamma, double *ic, double *gg,
                   uint64_t n);

@@ -4,7 +4,10 @@ the oracle: FP64 bit-exact in the grids, the IC and gamma-gamma rates, both
 cumulative spectra, the sampled photon energies (with and without the
 accept step), the interpolated rates and the random Lorentz factors; the
 reference's own test case (tests/test_ic.cu:24-69: broken power law, gamma
-= 10) gives energies in [0, gamma - 1.01]."""
+= 10) gives energies in [0, gamma - 1.01].  The triplet tables
+(apt_tpp_*, rt_tpp.cu:21-294) are bit-exact in the rate, the mean pair
+energy, the pair-energy CDF, the sampled pair energies and the
+interpolated rates."""
 import ctypes as C
 
 import numpy as np
@@ -111,3 +114,42 @@ def test_errors():
         ic.gen_photon_e(np.ones(4))
     with pytest.raises(apt.AptError, match="bad spectrum"):
         ic.init(apt.Spectra.mono_energetic(1e-5, 1e-6), 1e-3, 1e-7)
+
+
+def tpp_pair(name, ng=40):
+    kind, p, emin, emax, n0 = SPECTRA[name]
+    tp = apt.triplet_pairs(apt.SimParams(n_gamma=ng))
+    spec = [apt.Spectra.power_law_hard, apt.Spectra.power_law_soft,
+            apt.Spectra.broken_power_law, apt.Spectra.black_body,
+            apt.Spectra.mono_energetic][kind](*p)
+    tp.init(spec, emin, emax, n0)
+    o = oc.TPPOracle(ng)
+    o.init(kind, p, emin, emax, n0)
+    return tp, o
+
+
+@pytest.mark.parametrize("name", sorted(SPECTRA))
+def test_triplet_tables_bit_exact(name):
+    """rt_tpp.cu:188-275 (NaN rows where a soft photon has s < 4 included)."""
+    print(f"tpp tables: {name}", flush=True)
+    tp, o = tpp_pair(name)
+    same(tp.gammas(), o.gammas, "gammas")
+    same(tp.rate(), o.rate, "rate")
+    same(tp.Em(), o.em, "Em")
+    same(tp.dNde(), o.dnde, "dNde")
+
+
+def test_triplet_sampling_and_rates_bit_exact():
+    print("tpp: 29-point grid, samples and rates", flush=True)
+    tp, o = tpp_pair("mono", ng=29)
+    same(tp.dNde(), o.dnde, "dNde")
+    rng = np.random.default_rng(8)
+    g = np.concatenate([np.exp(rng.uniform(np.log(0.5), np.log(1e16), 5000)),
+                        tp.gammas(), [1.01, 1e15, 2e15]])
+    same(tp.gen_tpp_Ep(g, seed=21, epoch=6), o.gen_ep(g, seed=21, epoch=6), "E_p")
+    a, b = tp.rates(g), o.rates(g)
+    same(a[0], b[0], "rate")
+    same(a[1], b[1], "Em")
+    tp2 = apt.triplet_pairs(apt.SimParams(n_gamma=8))
+    with pytest.raises(apt.AptError, match="apt_tpp_init has not run"):
+        tp2.rate()
