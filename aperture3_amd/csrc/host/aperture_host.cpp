@@ -356,4 +356,78 @@ void field_solver_1dgr::update_fields(sim_data &data, double dt, double) {
   check(data.ctx, apt_field_update(data.ctx, dt, 0), "apt_field_update");
 }
 
+// ---- inverse_compton / triplet_pairs -----------------------------------
+
+inverse_compton::inverse_compton(const SimParams &params)
+    : m_ng(params.n_gamma), m_nep(params.n_ep) {
+  check(apt_ic_create(m_ng, m_nep, &m_h), "apt_ic_create");
+}
+
+inverse_compton::~inverse_compton() {
+  if (m_h) apt_ic_destroy(m_h);
+}
+
+void inverse_compton::check(apt_status st, const char *what) {
+  if (st == APT_OK) return;
+  std::string msg = std::string(what) + " failed (status " + std::to_string(st) + ")";
+  if (m_h) msg += ": " + std::string(apt_ic_last_error(m_h));
+  throw std::runtime_error(msg);
+}
+
+std::vector<Scalar> inverse_compton::table(int which, size_t n) {
+  std::vector<Scalar> out(n);
+  check(apt_ic_get_table(m_h, which, out.data(), n), "apt_ic_get_table");
+  return out;
+}
+
+std::vector<Scalar> inverse_compton::gen_photon_e(const std::vector<Scalar> &gamma,
+                                                  uint64_t seed, uint32_t epoch) {
+  std::vector<Scalar> out(gamma.size());
+  check(apt_ic_gen_photon_e(m_h, gamma.data(), out.data(), gamma.size(), seed, epoch),
+        "apt_ic_gen_photon_e");
+  return out;
+}
+
+void inverse_compton::generate_photon_energies(Scalar *e_ph_dev, const Scalar *gammas_dev,
+                                               size_t n, uint64_t seed, uint32_t epoch,
+                                               void *stream) {
+  check(apt_ic_generate_photon_energies(m_h, e_ph_dev, gammas_dev, n, seed, epoch, stream),
+        "apt_ic_generate_photon_energies");
+}
+
+void inverse_compton::generate_random_gamma(Scalar *gammas_dev, size_t n, uint64_t seed,
+                                            uint32_t epoch, void *stream) {
+  check(apt_ic_generate_random_gamma(m_h, gammas_dev, n, seed, epoch, stream),
+        "apt_ic_generate_random_gamma");
+}
+
+triplet_pairs::triplet_pairs(const SimParams &params) : m_ng(params.n_gamma) {
+  check(apt_tpp_create(m_ng, &m_h), "apt_tpp_create");
+}
+
+triplet_pairs::~triplet_pairs() {
+  if (m_h) apt_tpp_destroy(m_h);
+}
+
+void triplet_pairs::check(apt_status st, const char *what) {
+  if (st == APT_OK) return;
+  std::string msg = std::string(what) + " failed (status " + std::to_string(st) + ")";
+  if (m_h) msg += ": " + std::string(apt_tpp_last_error(m_h));
+  throw std::runtime_error(msg);
+}
+
+std::vector<Scalar> triplet_pairs::table(int which, size_t n) {
+  std::vector<Scalar> out(n);
+  check(apt_tpp_get_table(m_h, which, out.data(), n), "apt_tpp_get_table");
+  return out;
+}
+
+std::vector<Scalar> triplet_pairs::gen_tpp_Ep(const std::vector<Scalar> &gamma, uint64_t seed,
+                                              uint32_t epoch) {
+  std::vector<Scalar> out(gamma.size());
+  check(apt_tpp_gen_ep(m_h, gamma.data(), out.data(), gamma.size(), seed, epoch),
+        "apt_tpp_gen_ep");
+  return out;
+}
+
 }  // namespace Aperture

@@ -8,6 +8,7 @@
 #pragma once
 #include <cstdint>
 #include <functional>
+#include <initializer_list>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -56,6 +57,10 @@ struct SimParams {
   Scalar r_cutoff = 4.0;
   // 1D GR gap: black hole spin (sim_params.h:63)
   Scalar a = 0.0;
+  // inverse-Compton / triplet tables (sim_params.h:72, 87-88)
+  float e_min = 1.0e-3f;
+  int n_gamma = 600;
+  int n_ep = 600;
   // output and restart (sim_params.h:103-120)
   std::string data_dir = "Data/";
   std::string conf_file = "config.toml";
@@ -250,6 +255,116 @@ class ffe_solver_logsph {
 
  private:
   sim_environment &m_env;
+};
+
+// Soft-photon spectra (radiation/spectra.h:10-95) as ABI descriptors
+namespace Spectra {
+inline apt_ic_spectrum make(int kind, std::initializer_list<double> p) {
+  apt_ic_spectrum s{kind, {0, 0, 0, 0, 0}};
+  int i = 0;
+  for (double v : p) s.p[i++] = v;
+  return s;
+}
+struct power_law_hard {
+  apt_ic_spectrum s;
+  power_law_hard(Scalar delta, Scalar emin, Scalar emax)
+      : s(make(APT_SPEC_POWER_LAW_HARD, {delta, emin, emax})) {}
+  Scalar emin() const { return s.p[1]; }
+  Scalar emax() const { return s.p[2]; }
+};
+struct power_law_soft {
+  apt_ic_spectrum s;
+  power_law_soft(Scalar alpha, Scalar emin, Scalar emax)
+      : s(make(APT_SPEC_POWER_LAW_SOFT, {alpha, emin, emax})) {}
+  Scalar emin() const { return s.p[1]; }
+  Scalar emax() const { return s.p[2]; }
+};
+struct broken_power_law {
+  apt_ic_spectrum s;
+  broken_power_law(Scalar alpha, Scalar delta, Scalar ep, Scalar emin, Scalar emax)
+      : s(make(APT_SPEC_BROKEN_POWER_LAW, {alpha, delta, ep, emin, emax})) {}
+  Scalar emin() const { return s.p[3]; }
+  Scalar emax() const { return s.p[4]; }
+};
+struct black_body {
+  apt_ic_spectrum s;
+  explicit black_body(Scalar kT) : s(make(APT_SPEC_BLACK_BODY, {kT})) {}
+  Scalar emin() const { return 1e-10 * s.p[0]; }
+  Scalar emax() const { return 1e3 * s.p[0]; }
+};
+struct mono_energetic {
+  apt_ic_spectrum s;
+  mono_energetic(Scalar e0, Scalar de) : s(make(APT_SPEC_MONO_ENERGETIC, {e0, de})) {}
+  Scalar emin() const { return s.p[0] * 1e-2; }
+  Scalar emax() const { return s.p[0] * 1e2; }
+};
+}  // namespace Spectra
+
+// inverse_compton (cuda/radiation/rt_ic.h:13-52): the tables live on the
+// GPU; the accessors copy them to host vectors.  The random draws come from
+// the counter-based stream (seed, index, epoch) instead of curand states.
+class inverse_compton {
+ public:
+  explicit inverse_compton(const SimParams &params);
+  ~inverse_compton();
+  inverse_compton(const inverse_compton &) = delete;
+  inverse_compton &operator=(const inverse_compton &) = delete;
+
+  template <typename F>
+  void init(const F &n_e, Scalar emin, Scalar emax, double n0 = 1.0) {
+    check(apt_ic_init(m_h, &n_e.s, emin, emax, n0), "apt_ic_init");
+  }
+
+  std::vector<Scalar> ic_rate() { return table(APT_IC_RATE, m_ng); }
+  std::vector<Scalar> gg_rate() { return table(APT_IC_GG_RATE, m_ng); }
+  std::vector<Scalar> gammas() { return table(APT_IC_GAMMAS, m_ng); }
+  std::vector<Scalar> ep() { return table(APT_IC_EP, m_nep); }
+  std::vector<Scalar> log_ep() { return table(APT_IC_LOG_EP, m_nep); }
+  // n_gamma rows of n_ep (e1 fastest), as the reference's (e1, gamma) array
+  std::vector<Scalar> dNde() { return table(APT_IC_DNDE, (size_t)m_ng * m_nep); }
+  std::vector<Scalar> dNde_thomson() {
+    return table(APT_IC_DNDE_THOMSON, (size_t)m_ng * m_nep);
+  }
+  // gen_photon_e for each Lorentz factor (host vectors)
+  std::vector<Scalar> gen_photon_e(const std::vector<Scalar> &gamma, uint64_t seed,
+                                   uint32_t epoch = 0);
+  // generate_photon_energies / generate_random_gamma on device arrays
+  void generate_photon_energies(Scalar *e_ph_dev, const Scalar *gammas_dev, size_t n,
+                                uint64_t seed, uint32_t epoch = 0, void *stream = nullptr);
+  void generate_random_gamma(Scalar *gammas_dev, size_t n, uint64_t seed,
+                             uint32_t epoch = 0, void *stream = nullptr);
+  apt_ic *handle() { return m_h; }
+
+ private:
+  std::vector<Scalar> table(int which, size_t n);
+  void check(apt_status st, const char *what);
+  apt_ic *m_h = nullptr;
+  int m_ng, m_nep;
+};
+
+// triplet_pairs (cuda/radiation/rt_tpp.h:10-33)
+class triplet_pairs {
+ public:
+  explicit triplet_pairs(const SimParams &params);
+  ~triplet_pairs();
+  triplet_pairs(const triplet_pairs &) = delete;
+  triplet_pairs &operator=(const triplet_pairs &) = delete;
+
+  template <typename F>
+  void init(const F &n_e, Scalar emin, Scalar emax, double n0 = 1.0) {
+    check(apt_tpp_init(m_h, &n_e.s, emin, emax, n0), "apt_tpp_init");
+  }
+  std::vector<Scalar> rate() { return table(APT_TPP_RATE, m_ng); }
+  std::vector<Scalar> Em() { return table(APT_TPP_EM, m_ng); }
+  std::vector<Scalar> dNde() { return table(APT_TPP_DNDE, (size_t)m_ng * m_ng); }
+  std::vector<Scalar> gen_tpp_Ep(const std::vector<Scalar> &gamma, uint64_t seed,
+                                 uint32_t epoch = 0);
+
+ private:
+  std::vector<Scalar> table(int which, size_t n);
+  void check(apt_status st, const char *what);
+  apt_tpp *m_h = nullptr;
+  int m_ng;
 };
 
 // The batched ensemble of 1D open gap boxes (no reference class: the
