@@ -444,7 +444,8 @@ extern "C" {
 // inverse_compton::inverse_compton (rt_ic.cu:227-282): the gamma, e1 and
 // log e1 grids
 apt_status apt_ic_create(int n_gamma, int n_ep, apt_ic **out) {
-  if (!out || n_gamma < 3 || n_ep < 3 || n_gamma > 100000 || n_ep > 100000)
+  // each cumulative table is n_gamma x n_ep doubles, one thread per entry
+  if (!out || n_gamma < 3 || n_ep < 3 || (uint64_t)n_gamma * (uint64_t)n_ep > (1ull << 27))
     return APT_ERR_INVALID_ARG;
   apt_ic *h = new apt_ic();
   h->ng = n_gamma;
@@ -654,7 +655,8 @@ apt_status apt_ic_rates(apt_ic *h, const double *gamma, double *ic_rate, double 
 // triplet_pairs::triplet_pairs (rt_tpp.cu:147-184): the gamma grid, the
 // same as the inverse-Compton one
 apt_status apt_tpp_create(int n_gamma, apt_tpp **out) {
-  if (!out || n_gamma < 3 || n_gamma > 20000) return APT_ERR_INVALID_ARG;
+  // the pair-energy table is n_gamma x n_gamma doubles, one thread per entry
+  if (!out || n_gamma < 3 || n_gamma > 11585) return APT_ERR_INVALID_ARG;
   apt_tpp *h = new apt_tpp();
   h->ng = n_gamma;
   h->dg = (apt_log(IC_MAX_GAMMA) - apt_log(IC_MIN_GAMMA)) / ((double)n_gamma - 1.0);

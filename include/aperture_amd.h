@@ -460,7 +460,8 @@ enum {
   APT_IC_DNDE = 5,         /* n_gamma x n_ep (e1 fastest): CDF on EP */
   APT_IC_DNDE_THOMSON = 6  /* n_gamma x n_ep: CDF on LOG_EP */
 };
-/* inverse_compton(params) (rt_ic.cu:227-282): params.n_gamma, n_ep grids */
+/* inverse_compton(params) (rt_ic.cu:227-282): params.n_gamma, n_ep grids
+ * (each at least 3, n_gamma * n_ep at most 2^27) */
 apt_status apt_ic_create(int n_gamma, int n_ep, apt_ic **out);
 apt_status apt_ic_destroy(apt_ic *h);
 const char *apt_ic_last_error(const apt_ic *h);
@@ -505,7 +506,7 @@ enum {
   APT_TPP_EM = 2,     /* n_gamma: mean pair energy fraction */
   APT_TPP_DNDE = 3    /* n_gamma x n_gamma: CDF over pair energy = gammas[i] */
 };
-/* triplet_pairs(params) (rt_tpp.cu:147-184) */
+/* triplet_pairs(params) (rt_tpp.cu:147-184); 3 <= n_gamma <= 11585 */
 apt_status apt_tpp_create(int n_gamma, apt_tpp **out);
 apt_status apt_tpp_destroy(apt_tpp *h);
 const char *apt_tpp_last_error(const apt_tpp *h);
