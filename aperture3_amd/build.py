@@ -107,6 +107,11 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
           os.path.join(CSRC, "host", "pic_step_driver.cpp"), "-o", drv + ".tmp",
           f"-L{HERE}", "-laperture_amd", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN"])
     os.replace(drv + ".tmp", drv)
+    ic = os.path.join(HERE, "ic_driver")
+    _run([hipcc, "-O2", "-std=c++17", f"-I{INCLUDE}", f"-I{CSRC}",
+          os.path.join(CSRC, "host", "ic_driver.cpp"), "-o", ic + ".tmp",
+          f"-L{HERE}", "-laperture_amd", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN"])
+    os.replace(ic + ".tmp", ic)
     build_io()
     return LIB
 

@@ -153,3 +153,24 @@ def test_triplet_sampling_and_rates_bit_exact():
     tp2 = apt.triplet_pairs(apt.SimParams(n_gamma=8))
     with pytest.raises(apt.AptError, match="apt_tpp_init has not run"):
         tp2.rate()
+
+
+def test_cpp_ic_driver():
+    """The C++ host mirror (ic_driver: tests/test_ic.cu's scenario through
+    inverse_compton / triplet_pairs / Spectra) prints the same photon
+    energies and triplet rates as the oracle, bit for bit."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(apt.LIB_PATH), "ic_driver")
+    print("ic_driver 40 40 1234", flush=True)
+    out = subprocess.run([exe, "40", "40", "1234"], capture_output=True, text=True,
+                         timeout=120, check=True).stdout.split()
+    vals = np.array([float(v) for v in out])
+    assert vals.size == 140
+    kind, p, emin, emax, n0 = SPECTRA["test_ic_cu"]
+    o = oc.ICOracle(40, 40)
+    o.init(kind, p, emin, emax, n0)
+    same(vals[:100], o.gen_photon_e(np.full(100, 10.0), seed=1234), "ic_driver energies")
+    t = oc.TPPOracle(40)
+    t.init(kind, p, emin, emax, n0)
+    same(vals[100:], t.rate, "ic_driver triplet rates")
