@@ -124,6 +124,12 @@ class AptICSpectrum(C.Structure):
     _fields_ = [("kind", C.c_int), ("p", C.c_double * 5)]
 
 
+class AptICModel(C.Structure):
+    _fields_ = [("kind", C.c_int), ("alpha", C.c_double), ("e_s", C.c_double),
+                ("e_min", C.c_double), ("photon_path", C.c_double), ("ic_rate", C.c_double),
+                ("E_cutoff", C.c_double), ("E_ph", C.c_double)]
+
+
 # every entry point declared in include/aperture_amd.h: name -> argtypes
 _P = C.c_void_p
 SIGNATURES = {
@@ -218,6 +224,8 @@ SIGNATURES = {
                        C.c_uint64, C.c_uint32],
     "apt_tpp_rates": [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                       C.c_uint64],
+    "apt_ic_model_emit": [C.POINTER(AptICModel), _P, _P, _P, _P, _P, C.c_uint64, C.c_uint64,
+                          C.c_uint32, _P],
     "apt_ic_rates": [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                      C.c_uint64],
 }
@@ -304,7 +312,12 @@ class SimParams:
         self.max_photon_number = 0
         # 1D GR gap (sim_params.h:63): black hole spin
         self.a = 0.0
-        # inverse-Compton tables (sim_params.h:72, 87-88)
+        # inverse-Compton tables and emission models (sim_params.h:66-74, 87-88)
+        self.E_cutoff = float(np.float32(10.0))
+        self.E_ph = float(np.float32(4.0))
+        self.spectral_alpha = float(np.float32(2.0))
+        self.e_s = float(np.float32(0.2))
+        self.ic_path = float(np.float32(1.0))
         self.e_min = float(np.float32(1.0e-3))
         self.n_gamma = 600
         self.n_ep = 600
@@ -945,6 +958,57 @@ class inverse_compton:
         if gamma > self.MAX_GAMMA:
             return self.n_gamma - 1
         return int((np.log(gamma) - np.log(self.MIN_GAMMA)) / self.m_dg)
+
+
+ICM_DUMMY, ICM_POWER_LAW = 0, 1
+
+
+class _ICModel:
+    """Shared driver of the analytic emission models: for each particle
+    (gamma, p) on the GPU, emit_photon, then draw_photon_energy and
+    draw_photon_freepath."""
+
+    kind = -1
+
+    def __init__(self, params: SimParams):
+        self.lib = load_library()
+        self.m = AptICModel(self.kind, params.spectral_alpha, params.e_s, params.e_min,
+                            params.photon_path,
+                            float(params.delta_t) / float(np.float32(params.ic_path)),
+                            params.E_cutoff, params.E_ph)
+
+    def emit(self, gamma, p, seed: int = 0, epoch: int = 0):
+        """Device float64 tensors of one length in; (emit uint8, eph, path)
+        device tensors out, on torch's current stream."""
+        import torch
+        n = gamma.numel()
+        for t in (gamma, p):
+            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() or \
+                    t.numel() != n:
+                raise AptError("contiguous float64 device tensors of one length expected")
+        e = torch.empty(n, dtype=torch.uint8, device=gamma.device)
+        eph = torch.empty_like(gamma)
+        path = torch.empty_like(gamma)
+        st = self.lib.apt_ic_model_emit(C.byref(self.m), gamma.data_ptr(), p.data_ptr(),
+                                        e.data_ptr(), eph.data_ptr(), path.data_ptr(), n, seed,
+                                        epoch, torch.cuda.current_stream().cuda_stream)
+        if st != 0:
+            raise AptError(f"apt_ic_model_emit: status {st}")
+        return e, eph, path
+
+
+class InverseComptonPL1D(_ICModel):
+    """InverseComptonPL1D (cuda/radiation/inverse_compton_power_law.h:10-40):
+    power-law soft photons of slope spectral_alpha above e_min, with the
+    two regimes split at e_s."""
+    kind = ICM_POWER_LAW
+
+
+class InverseComptonDummy(_ICModel):
+    """InverseComptonDummy (cuda/radiation/inverse_compton_dummy.h:10-28):
+    particles above E_cutoff emit photons of energy E_ph and free path
+    photon_path."""
+    kind = ICM_DUMMY
 
 
 TPP_GAMMAS, TPP_RATE, TPP_EM, TPP_DNDE = range(4)

@@ -57,7 +57,12 @@ struct SimParams {
   Scalar r_cutoff = 4.0;
   // 1D GR gap: black hole spin (sim_params.h:63)
   Scalar a = 0.0;
-  // inverse-Compton / triplet tables (sim_params.h:72, 87-88)
+  // inverse-Compton tables and emission models (sim_params.h:66-74, 87-88)
+  float E_cutoff = 10.0f;
+  float E_ph = 4.0f;
+  float spectral_alpha = 2.0f;
+  float e_s = 0.2f;
+  float ic_path = 1.0f;
   float e_min = 1.0e-3f;
   int n_gamma = 600;
   int n_ep = 600;
@@ -341,6 +346,31 @@ class inverse_compton {
   apt_ic *m_h = nullptr;
   int m_ng, m_nep;
 };
+
+// InverseComptonPL1D / InverseComptonDummy (cuda/radiation/
+// inverse_compton_power_law.h:10-40, inverse_compton_dummy.h:10-28) applied to
+// n particles at once: emit_photon, draw_photon_energy, draw_photon_freepath
+// (device arrays; the draws come from the counter-based stream).
+template <int KIND>
+class InverseComptonModel {
+ public:
+  explicit InverseComptonModel(const SimParams &params)
+      : m{KIND, params.spectral_alpha, params.e_s, params.e_min, params.photon_path,
+          params.delta_t / params.ic_path, params.E_cutoff, params.E_ph} {}
+  void emit(const Scalar *gamma_dev, const Scalar *p_dev, uint8_t *emit_dev, Scalar *eph_dev,
+            Scalar *path_dev, size_t n, uint64_t seed, uint32_t epoch = 0,
+            void *stream = nullptr) const {
+    apt_status st = apt_ic_model_emit(&m, gamma_dev, p_dev, emit_dev, eph_dev, path_dev, n,
+                                      seed, epoch, stream);
+    if (st != APT_OK)
+      throw std::runtime_error("apt_ic_model_emit failed (status " + std::to_string(st) + ")");
+  }
+
+ private:
+  apt_ic_model m;
+};
+using InverseComptonPL1D = InverseComptonModel<APT_ICM_POWER_LAW>;
+using InverseComptonDummy = InverseComptonModel<APT_ICM_DUMMY>;
 
 // triplet_pairs (cuda/radiation/rt_tpp.h:10-33)
 class triplet_pairs {

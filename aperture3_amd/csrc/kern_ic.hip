@@ -399,6 +399,123 @@ __global__ void k_tpp_rates(TppTables t, const double *gamma, double *rate, doub
   }
 }
 
+// ---- analytic emission models (inverse_compton_power_law.cuh,
+// inverse_compton_dummy.cuh) ---------------------------------------------
+
+struct IcModel {
+  int kind;
+  double alpha, es, emin, mfp, icrate, cutoff, eph;
+};
+
+// std::log of a float in the reference: the build's log, rounded to float
+__host__ __device__ inline float logf_(float x) { return (float)apt_log((double)x); }
+
+// InverseComptonPL1D::compute_A1 / compute_A2 (inverse_compton_power_law.cuh:11-26)
+__host__ __device__ inline double pl_A1(const IcModel &m, double er) {
+  return 1.0f / (er * (0.5f + 1.0f / m.alpha -
+                       (1.0f / (m.alpha * (m.alpha + 1.0f))) * ic_pow(er / m.es, m.alpha)));
+}
+__host__ __device__ inline double pl_A2(const IcModel &m, double er, double et) {
+  return 1.0f / (et * (et * 0.5f / er + apt_log(er / et) + 1.0f / (1.0f + m.alpha)));
+}
+
+// f_inv1 / f_inv2 (:28-63): inverse CDFs of the rest-frame photon energy
+__host__ __device__ inline double pl_f_inv1(const IcModel &m, float u, double gamma) {
+  const double er = 2.0f * gamma * m.emin;
+  const double A1 = pl_A1(m, er);
+  if (u < A1 * er * 0.5f) return sqrt(2.0f * u * er / A1);
+  if (u < 1.0f - A1 * er * ic_pow(m.es / er, -m.alpha) / (1.0f + m.alpha))
+    return er * ic_pow(m.alpha * (1.0f / m.alpha + 0.5f - u / (A1 * er)), -1.0f / m.alpha);
+  return er * ic_pow((1.0f - u) * (1.0f + m.alpha) / (A1 * m.es), -1.0f / (m.alpha + 1.0f));
+}
+__host__ __device__ inline double pl_f_inv2(const IcModel &m, float u, double gamma) {
+  const double er = 2.0f * gamma * m.emin;
+  const double et = er / (2.0f * er + 1.0f);
+  const double A2 = pl_A2(m, er, et);
+  if (u < A2 * et * et * 0.5f / er) return sqrt(2.0f * u * er / A2);
+  if (u < 1.0f - A2 * et / (1.0f + m.alpha)) return et * apt_exp(u / (A2 * et) - et * 0.5f / er);
+  return er * ic_pow((1.0f - u) * (1.0f + m.alpha) / (A2 * et), -1.0f / (m.alpha + 1.0f));
+}
+
+// draw_photon_ep (:79-121): the original energy given the rest-frame one
+__host__ __device__ inline double pl_draw_ep(const IcModel &m, double e1p, double gamma,
+                                             float u) {
+  const double gemin2 = 2.0f * gamma * m.emin;
+  const double al = m.alpha;
+  if (e1p < 0.5f && e1p / (1.0f - 2.0f * e1p) <= gemin2) {
+    const double e_lim = e1p / (1.0f - 2.0f * e1p);
+    const double a1 = (gemin2 * gemin2 * (al + 2.0f)) / (gamma * (e_lim * e_lim - e1p * e1p));
+    return sqrt(u * (al + 2.0f) * gemin2 * gemin2 / (a1 * gamma) + e1p * e1p);
+  }
+  if (e1p > gemin2) {
+    double a2 = (al * (al + 2.0f) * 0.5f / gamma) * ic_pow(e1p / gemin2, al);
+    if (e1p < 0.5f) a2 /= (1.0f - ic_pow(1.0f - 2.0f * e1p, al));
+    return gemin2 * ic_pow(ic_pow(gemin2 / e1p, al) - u * al * (al + 2.0f) / (2.0f * gamma * a2),
+                           -1.0f / al);
+  }
+  double G = 0.0f;
+  if (e1p < 0.5f) G = ic_pow((1.0f - 2.0f * e1p) * gemin2 / e1p, al);
+  const double U_0 = (gemin2 * gemin2 - e1p * e1p) * gamma / (gemin2 * gemin2 * (al + 2.0f));
+  const double a3 = 1.0f / (U_0 + (1.0f - G) * 2.0f * gamma / (al * (al + 2.0f)));
+  if (u < U_0 * a3) return sqrt(u * (al + 2.0f) * gemin2 * gemin2 / (a3 * gamma) + e1p * e1p);
+  return gemin2 *
+         ic_pow(1.0f - (u - a3 * U_0) * al * (al + 2.0) / (2.0f * a3 * gamma), -1.0f / al);
+}
+
+// emit_photon, draw_photon_energy, draw_photon_freepath (:149-182) for
+// one particle; u[0..3] are the four draws in the reference's call order
+__host__ __device__ inline void ic_model_one(const IcModel &m, double gamma, double p,
+                                             const float u[4], uint8_t &emit, double &eph,
+                                             double &path) {
+  emit = 0;
+  eph = 0.0;
+  path = 0.0;
+  if (m.kind == APT_ICM_DUMMY) {  // inverse_compton_dummy.cuh:19-35
+    if (gamma > m.cutoff) {
+      emit = 1;
+      eph = m.eph;
+      path = m.mfp;
+    }
+    return;
+  }
+  const float e_p = (float)(gamma * m.emin);
+  const float rate = (float)(e_p < 0.1f ? m.icrate
+                                        : m.icrate * logf_(2.0f * e_p + 0.90517f) / e_p);
+  if (!(u[0] < rate)) return;
+  emit = 1;
+  const double e1p = gamma < m.es * 0.5f / m.emin ? pl_f_inv1(m, u[1], gamma)
+                                                   : pl_f_inv2(m, u[1], gamma);
+  const double ep = pl_draw_ep(m, e1p, gamma, u[2]);
+  const double u1p = 1.0f - 1.0f / e1p + 1.0f / ep;
+  const double sg = (double)((0.0 < p) - (p < 0.0));
+  eph = sg * (gamma + fabs(p) * (-u1p)) * e1p;
+  double r;
+  if (eph * m.emin < 2.0f)
+    r = ic_pow(fabs(eph) * m.emin / 2.0f, m.alpha);
+  else
+    r = 2.0f / (fabs(eph) * m.emin);
+  path = -m.mfp * logf_(u[3]) / r;
+}
+
+// one thread per particle: the draws are the four uniforms of the Philox
+// blocks (seed, k, epoch, 12) and (seed, k, epoch, 13)
+__global__ void k_ic_model(IcModel m, const double *gamma, const double *p, uint8_t *emit,
+                           double *eph, double *path, uint64_t n, uint64_t seed,
+                           uint32_t epoch) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double a0, a1, b0, b1;
+  uniform2(seed, k, epoch, 12, a0, a1);
+  uniform2(seed, k, epoch, 13, b0, b1);
+  const float u[4] = {(float)a0, (float)a1, (float)b0, (float)b1};
+  uint8_t e;
+  double E, L;
+  ic_model_one(m, gamma[k], p[k], u, e, E, L);
+  emit[k] = e;
+  eph[k] = E;
+  path[k] = L;
+}
+
 }  // namespace
 }  // namespace apt
 
@@ -781,6 +898,32 @@ apt_status apt_tpp_gen_ep(apt_tpp *h, const double *gamma, double *e_p, uint64_t
 apt_status apt_tpp_rates(apt_tpp *h, const double *gamma, double *rate, double *em,
                          uint64_t n) {
   return tpp_run(h, gamma, rate, em, n, 0, 0, false);
+}
+
+// InverseComptonPL1D / InverseComptonDummy (inverse_compton_power_law.cuh,
+// inverse_compton_dummy.cuh) over n particles, device arrays
+apt_status apt_ic_model_emit(const apt_ic_model *model, const double *gamma, const double *p,
+                             uint8_t *emit, double *eph, double *path, uint64_t n,
+                             uint64_t seed, uint32_t epoch, void *stream) {
+  if (!model || (model->kind != APT_ICM_DUMMY && model->kind != APT_ICM_POWER_LAW))
+    return APT_ERR_INVALID_ARG;
+  if (n == 0) return APT_OK;
+  if (!gamma || !p || !emit || !eph || !path || n > IC_MAX_SAMPLES) return APT_ERR_INVALID_ARG;
+  if (model->kind == APT_ICM_POWER_LAW &&
+      !(model->alpha > 0.0 && model->e_s > 0.0 && model->e_min > 0.0))
+    return APT_ERR_INVALID_ARG;
+  IcModel m;
+  m.kind = model->kind;
+  m.alpha = model->alpha;
+  m.es = model->e_s;
+  m.emin = model->e_min;
+  m.mfp = model->photon_path;
+  m.icrate = model->ic_rate;
+  m.cutoff = model->E_cutoff;
+  m.eph = model->E_ph;
+  k_ic_model<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+      m, gamma, p, emit, eph, path, n, seed, epoch);
+  return hipGetLastError() == hipSuccess ? APT_OK : APT_ERR_HIP;
 }
 
 }  // extern "C"

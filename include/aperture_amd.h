@@ -522,6 +522,26 @@ apt_status apt_tpp_gen_ep(apt_tpp *h, const double *gamma, double *e_p, uint64_t
 apt_status apt_tpp_rates(apt_tpp *h, const double *gamma, double *rate, double *em,
                          uint64_t n);
 
+/* ---- analytic emission models (inverse_compton_*.{h,cuh}) ------------ */
+/* InverseComptonPL1D (inverse_compton_power_law.cuh:11-190) and
+ * InverseComptonDummy (inverse_compton_dummy.cuh:7-40): for each particle
+ * (gamma, p), emit_photon, then draw_photon_energy and
+ * draw_photon_freepath (emit = 0 leaves eph = path = 0).  The four draws
+ * of particle k are the uniforms of the Philox blocks (seed, k, epoch)
+ * instead of a curand state.  Fields as SimParams: alpha =
+ * spectral_alpha, e_s, e_min, photon_path, ic_rate = delta_t / ic_path,
+ * E_cutoff, E_ph (the dummy's). */
+enum { APT_ICM_DUMMY = 0, APT_ICM_POWER_LAW = 1 };
+typedef struct {
+  int kind;
+  double alpha, e_s, e_min, photon_path, ic_rate, E_cutoff, E_ph;
+} apt_ic_model;
+/* device arrays, ordered on `stream` (a hipStream_t; NULL: the default
+ * stream) */
+apt_status apt_ic_model_emit(const apt_ic_model *model, const double *gamma, const double *p,
+                             uint8_t *emit, double *eph, double *path, uint64_t n,
+                             uint64_t seed, uint32_t epoch, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

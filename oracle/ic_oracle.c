@@ -357,3 +357,101 @@ void orc_tpp_rates(int ng, double dg, const double *gammas, const double *rate,
     m[k] = find_rate(ng, dg, gammas, em, g, gn);
   }
 }
+
+/* ---- analytic emission models (inverse_compton_power_law.cuh,
+ * inverse_compton_dummy.cuh) ------------------------------------------- */
+
+/* std::log of a float in the reference: this build's log, rounded to float */
+static float logf_(float x) { return (float)orc_log((double)x); }
+
+/* compute_A1 / compute_A2 (inverse_compton_power_law.cuh:11-26) */
+static double pl_A1(const orc_ic_model *m, double er) {
+  return 1.0f / (er * (0.5f + 1.0f / m->alpha -
+                       (1.0f / (m->alpha * (m->alpha + 1.0f))) * ic_pow(er / m->es, m->alpha)));
+}
+static double pl_A2(const orc_ic_model *m, double er, double et) {
+  return 1.0f / (et * (et * 0.5f / er + orc_log(er / et) + 1.0f / (1.0f + m->alpha)));
+}
+
+/* f_inv1 / f_inv2 (:28-63) */
+double orc_pl_f_inv1(const orc_ic_model *m, float u, double gamma) {
+  double er = 2.0f * gamma * m->emin;
+  double A1 = pl_A1(m, er);
+  if (u < A1 * er * 0.5f) return sqrt(2.0f * u * er / A1);
+  if (u < 1.0f - A1 * er * ic_pow(m->es / er, -m->alpha) / (1.0f + m->alpha))
+    return er * ic_pow(m->alpha * (1.0f / m->alpha + 0.5f - u / (A1 * er)), -1.0f / m->alpha);
+  return er * ic_pow((1.0f - u) * (1.0f + m->alpha) / (A1 * m->es), -1.0f / (m->alpha + 1.0f));
+}
+double orc_pl_f_inv2(const orc_ic_model *m, float u, double gamma) {
+  double er = 2.0f * gamma * m->emin;
+  double et = er / (2.0f * er + 1.0f);
+  double A2 = pl_A2(m, er, et);
+  if (u < A2 * et * et * 0.5f / er) return sqrt(2.0f * u * er / A2);
+  if (u < 1.0f - A2 * et / (1.0f + m->alpha)) return et * orc_exp(u / (A2 * et) - et * 0.5f / er);
+  return er * ic_pow((1.0f - u) * (1.0f + m->alpha) / (A2 * et), -1.0f / (m->alpha + 1.0f));
+}
+
+/* draw_photon_ep (:79-121) */
+double orc_pl_draw_ep(const orc_ic_model *m, double e1p, double gamma, float u) {
+  double gemin2 = 2.0f * gamma * m->emin;
+  double al = m->alpha;
+  if (e1p < 0.5f && e1p / (1.0f - 2.0f * e1p) <= gemin2) {
+    double e_lim = e1p / (1.0f - 2.0f * e1p);
+    double a1 = (gemin2 * gemin2 * (al + 2.0f)) / (gamma * (e_lim * e_lim - e1p * e1p));
+    return sqrt(u * (al + 2.0f) * gemin2 * gemin2 / (a1 * gamma) + e1p * e1p);
+  }
+  if (e1p > gemin2) {
+    double a2 = (al * (al + 2.0f) * 0.5f / gamma) * ic_pow(e1p / gemin2, al);
+    if (e1p < 0.5f) a2 /= (1.0f - ic_pow(1.0f - 2.0f * e1p, al));
+    return gemin2 * ic_pow(ic_pow(gemin2 / e1p, al) - u * al * (al + 2.0f) / (2.0f * gamma * a2),
+                           -1.0f / al);
+  }
+  double G = 0.0f;
+  if (e1p < 0.5f) G = ic_pow((1.0f - 2.0f * e1p) * gemin2 / e1p, al);
+  double U_0 = (gemin2 * gemin2 - e1p * e1p) * gamma / (gemin2 * gemin2 * (al + 2.0f));
+  double a3 = 1.0f / (U_0 + (1.0f - G) * 2.0f * gamma / (al * (al + 2.0f)));
+  if (u < U_0 * a3) return sqrt(u * (al + 2.0f) * gemin2 * gemin2 / (a3 * gamma) + e1p * e1p);
+  return gemin2 * ic_pow(1.0f - (u - a3 * U_0) * al * (al + 2.0) / (2.0f * a3 * gamma), -1.0f / al);
+}
+
+/* emit_photon, draw_photon_energy, draw_photon_freepath (:149-182) and the
+ * dummy (inverse_compton_dummy.cuh:19-35) over n particles; the draws of
+ * particle k are the uniforms of Philox (seed, k, epoch, 12) and (..., 13) */
+void orc_ic_model_emit(const orc_ic_model *m, const double *gamma, const double *p,
+                       uint8_t *emit, double *eph, double *path, uint64_t n, uint64_t seed,
+                       uint32_t epoch) {
+  for (uint64_t k = 0; k < n; k++) {
+    double a[2], b[2];
+    orc_uniform2(seed, k, epoch, 12, a);
+    orc_uniform2(seed, k, epoch, 13, b);
+    float u[4] = {(float)a[0], (float)a[1], (float)b[0], (float)b[1]};
+    double g = gamma[k];
+    emit[k] = 0;
+    eph[k] = 0.0;
+    path[k] = 0.0;
+    if (m->kind == 0) {
+      if (g > m->cutoff) {
+        emit[k] = 1;
+        eph[k] = m->eph;
+        path[k] = m->mfp;
+      }
+      continue;
+    }
+    float e_p = (float)(g * m->emin);
+    float rate = (float)(e_p < 0.1f ? m->icrate : m->icrate * logf_(2.0f * e_p + 0.90517f) / e_p);
+    if (!(u[0] < rate)) continue;
+    emit[k] = 1;
+    double e1p = g < m->es * 0.5f / m->emin ? orc_pl_f_inv1(m, u[1], g) : orc_pl_f_inv2(m, u[1], g);
+    double ep = orc_pl_draw_ep(m, e1p, g, u[2]);
+    double u1p = 1.0f - 1.0f / e1p + 1.0f / ep;
+    double sg = (double)((0.0 < p[k]) - (p[k] < 0.0));
+    double E = sg * (g + fabs(p[k]) * (-u1p)) * e1p;
+    double r;
+    if (E * m->emin < 2.0f)
+      r = ic_pow(fabs(E) * m->emin / 2.0f, m->alpha);
+    else
+      r = 2.0f / (fabs(E) * m->emin);
+    eph[k] = E;
+    path[k] = -m->mfp * logf_(u[3]) / r;
+  }
+}

@@ -791,3 +791,43 @@ class TPPOracle:
                                _d(self.rate), _d(self.em), _d(g), _d(r), _d(m),
                                C.c_uint64(g.size))
         return r, m
+
+
+class OrcICModel(C.Structure):
+    _fields_ = [("kind", C.c_int), ("alpha", C.c_double), ("es", C.c_double),
+                ("emin", C.c_double), ("mfp", C.c_double), ("icrate", C.c_double),
+                ("cutoff", C.c_double), ("eph", C.c_double)]
+
+
+class ICModelOracle:
+    """InverseComptonPL1D / InverseComptonDummy of ic_oracle.c
+    (inverse_compton_power_law.cuh, inverse_compton_dummy.cuh)."""
+
+    def __init__(self, kind, alpha=2.0, e_s=0.2, e_min=1e-3, photon_path=1.0, ic_rate=0.01,
+                 E_cutoff=10.0, E_ph=4.0):
+        self.lib = lib = load()
+        for name in ("orc_pl_f_inv1", "orc_pl_f_inv2"):
+            getattr(lib, name).restype = C.c_double
+            getattr(lib, name).argtypes = [C.POINTER(OrcICModel), C.c_float, C.c_double]
+        lib.orc_pl_draw_ep.restype = C.c_double
+        lib.orc_pl_draw_ep.argtypes = [C.POINTER(OrcICModel), C.c_double, C.c_double, C.c_float]
+        self.m = OrcICModel(kind, alpha, e_s, e_min, photon_path, ic_rate, E_cutoff, E_ph)
+
+    def f_inv1(self, u, gamma):
+        return self.lib.orc_pl_f_inv1(C.byref(self.m), u, gamma)
+
+    def f_inv2(self, u, gamma):
+        return self.lib.orc_pl_f_inv2(C.byref(self.m), u, gamma)
+
+    def draw_ep(self, e1p, gamma, u):
+        return self.lib.orc_pl_draw_ep(C.byref(self.m), e1p, gamma, u)
+
+    def emit(self, gamma, p, seed=0, epoch=0):
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        pp = np.ascontiguousarray(p, dtype=np.float64)
+        e = np.empty(g.size, np.uint8)
+        eph, path = np.empty_like(g), np.empty_like(g)
+        self.lib.orc_ic_model_emit(C.byref(self.m), _d(g), _d(pp),
+                                   e.ctypes.data_as(C.POINTER(C.c_uint8)), _d(eph), _d(path),
+                                   C.c_uint64(g.size), C.c_uint64(seed), C.c_uint32(epoch))
+        return e, eph, path

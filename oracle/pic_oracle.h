@@ -236,6 +236,17 @@ void orc_ic_gen_photon_e(int ng, int nep, double dg, double dep, double dlep,
                          const double *dnde_th, const double *gamma, double *eph, uint64_t n,
                          uint64_t seed, uint32_t epoch, int accept);
 void orc_ic_random_gamma(double *gammas, uint64_t n, uint64_t seed, uint32_t epoch);
+/* analytic emission models: kind 0 dummy (cutoff, eph, mfp), 1 power law */
+typedef struct {
+  int kind;
+  double alpha, es, emin, mfp, icrate, cutoff, eph;
+} orc_ic_model;
+double orc_pl_f_inv1(const orc_ic_model *m, float u, double gamma);
+double orc_pl_f_inv2(const orc_ic_model *m, float u, double gamma);
+double orc_pl_draw_ep(const orc_ic_model *m, double e1p, double gamma, float u);
+void orc_ic_model_emit(const orc_ic_model *m, const double *gamma, const double *p,
+                       uint8_t *emit, double *eph, double *path, uint64_t n, uint64_t seed,
+                       uint32_t epoch);
 double orc_tpp_total_f(double s);
 double orc_tpp_approx_em(double s);
 void orc_tpp_e_plus(double gamma, double s, double *emin, double *emax);

@@ -174,3 +174,30 @@ def test_cpp_ic_driver():
     t = oc.TPPOracle(40)
     t.init(kind, p, emin, emax, n0)
     same(vals[100:], t.rate, "ic_driver triplet rates")
+
+
+@pytest.mark.parametrize("kind", ["power_law", "dummy"])
+def test_emission_models_bit_exact(kind):
+    """InverseComptonPL1D / InverseComptonDummy (apt_ic_model_emit) against
+    the oracle: emission flags, signed photon energies and free paths bit
+    for bit over Lorentz factors spanning both f_inv regimes, both signs
+    of p, two epochs."""
+    print(f"ic model: {kind}", flush=True)
+    p = apt.SimParams(delta_t=0.3)
+    m = (apt.InverseComptonPL1D if kind == "power_law" else apt.InverseComptonDummy)(p)
+    o = oc.ICModelOracle(0 if kind == "dummy" else 1, alpha=p.spectral_alpha, e_s=p.e_s,
+                         e_min=p.e_min, photon_path=p.photon_path,
+                         ic_rate=0.3 / p.ic_path, E_cutoff=p.E_cutoff, E_ph=p.E_ph)
+    rng = np.random.default_rng(12)
+    n = 50000
+    pp = np.exp(rng.uniform(np.log(0.01), np.log(1e5), n)) * rng.choice([-1.0, 1.0], n)
+    g = np.sqrt(1 + pp * pp)
+    gd = torch.tensor(g, device="cuda")
+    pd = torch.tensor(pp, device="cuda")
+    for epoch in (0, 7):
+        e, eph, path = m.emit(gd, pd, seed=5, epoch=epoch)
+        oe, oeph, opath = o.emit(g, pp, seed=5, epoch=epoch)
+        same(e.cpu().numpy(), oe, "emit")
+        same(eph.cpu().numpy(), oeph, "eph")
+        same(path.cpu().numpy(), opath, "path")
+        assert 0 < oe.sum() < n
