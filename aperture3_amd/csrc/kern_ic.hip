@@ -731,6 +731,8 @@ apt_status apt_ic_gen_photon_e(apt_ic *h, const double *gamma, double *eph, uint
   if (!h) return APT_ERR_INVALID_ARG;
   if (n == 0) return APT_OK;
   if (!gamma || !eph) return h->fail(APT_ERR_INVALID_ARG, "ic_gen_photon_e: null arrays");
+  if (!h->ready) return h->fail(APT_ERR_STATE, "ic sampling: apt_ic_init has not run");
+  if (n > IC_MAX_SAMPLES) return h->fail(APT_ERR_INVALID_ARG, "ic sampling: too many samples");
   double *dg = nullptr, *de = nullptr;
   hipError_t e = hipMalloc(&dg, n * 8);
   if (e == hipSuccess) e = hipMalloc(&de, n * 8);
