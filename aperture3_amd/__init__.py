@@ -835,7 +835,9 @@ class Spectra:
 
 class inverse_compton:
     """inverse_compton (src/cuda/radiation/rt_ic.h:13-52): the tables live on
-    the GPU (built by apt_ic_init), the accessors copy them to numpy."""
+    the GPU (built by apt_ic_init), the accessors copy them to numpy.  The
+    device-tensor methods take torch tensors: import torch before the
+    library is loaded, so that the process holds one HIP runtime."""
 
     MIN_GAMMA = float(np.float32(1.01))
     MAX_GAMMA = 1.0e15

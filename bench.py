@@ -214,6 +214,92 @@ def measure_c1(apt, steps):
             "field_cell_updates_per_s": n * steps / el}
 
 
+def measure_ic(apt):
+    """Inverse-Compton and triplet tables at the reference's default grid
+    (n_gamma = n_ep = 600) for the broken power law of tests/test_ic.cu,
+    then gen_photon_e and the accept/reject generate_photon_energies over
+    2^26 device-resident Lorentz factors (16 B of HBM per sample: gamma in,
+    energy out; the table rows stay in L2)."""
+    import torch
+    p = apt.SimParams(e_min=float(__import__("numpy").float32(1e-5)))
+    ne = apt.Spectra.broken_power_law(1.25, 1.1, p.e_min, 1e-10, 0.1)
+    ic = apt.inverse_compton(p)
+    t0 = time.perf_counter()
+    ic.init(ne, ne.emin(), ne.emax(), 1.5e24 / 0.01)
+    init_ic = time.perf_counter() - t0
+    tp = apt.triplet_pairs(p)
+    t0 = time.perf_counter()
+    tp.init(ne, ne.emin(), ne.emax(), 1.5e24 / 0.01)
+    init_tpp = time.perf_counter() - t0
+    n = 1 << 26
+    g = torch.empty(n, dtype=torch.float64, device="cuda")
+    ic.generate_random_gamma(g, seed=1)
+    e = torch.empty_like(g)
+    res = {}
+    for name, fn in (("gen_photon_e", ic.gen_photon_e_device),
+                     ("generate_photon_energies", ic.generate_photon_energies)):
+        for k in range(2):
+            fn(e, g, seed=2, epoch=k)
+        torch.cuda.synchronize()
+        reps = 10
+        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s0.record()
+        for k in range(reps):
+            fn(e, g, seed=2, epoch=k)
+        s1.record()
+        torch.cuda.synchronize()
+        ms = s0.elapsed_time(s1) / reps
+        gbs = n * 16 / (ms * 1e-3) / 1e9
+        res[name] = {"samples": n, "ms": ms, "samples_per_s": n / (ms * 1e-3),
+                     "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                                  "bytes_per_sample": 16}}
+    del ic, tp
+    return {"workload": "IC + triplet tables 600 x 600, broken power law (1.25, 1.1, "
+                        "1e-5, 1e-10, 0.1), n0 = 1.5e26; 2^26 samples on the GPU",
+            "ic_init_s": init_ic, "tpp_init_s": init_tpp, **res}
+
+
+def measure_gr1d(apt, steps):
+    """The 1D GR gap along a Kerr field line (a = 0.99): 4096 cells, the
+    initial condition at multiplicity 100 (819,200 slots), FP64, with a
+    smooth synthetic coef.h5 profile (r in [1.15, 12])."""
+    import numpy as np
+    n = 4096
+    mult = 100
+    r = np.linspace(1.15, 12.0, 400)
+    prof = {"r": r, "Bu1": 1.0 / r ** 3 + 0.01, "Bu3": 0.3 / r ** 2, "rho": 0.2 * np.sin(r),
+            "ju1": 0.05 * np.cos(r), "th": 0.3 + 0.02 * r, "dpsidth": 1.0 + 0.1 * r}
+    p = apt.SimParams(N=[n, 1, 1], guard=[3, 0, 0], lower=[-1.2, 0.0, 0.0],
+                      size=[1.0, 1.0, 1.0], coord_system="1DGR", precision=apt.APT_PREC_FP64,
+                      max_ptc_number=n * 2 * mult + 4096, max_photon_number=1024,
+                      track_percent=0.05, current_smoothing=1, data_interval=10 ** 9,
+                      a=0.99, omega=0.2, B0=10.0)
+    env = apt.sim_environment(p)
+    data = apt.sim_data(env)
+    try:
+        apt.Grid_1dGR.compute_coef(data, prof)
+        pu, fs = apt.ptc_updater_1dgr(env), apt.field_solver_1dgr(env)
+        pu.prepare_initial_condition(data, mult)
+        npart = data.num_particles()
+        dt = 0.5 / n
+        for step in range(5):
+            pu.update_particles(data, dt, step)
+            fs.update_fields(data, dt)
+        data.sync()
+        t0 = time.perf_counter()
+        for step in range(5, 5 + steps):
+            pu.update_particles(data, dt, step)
+            fs.update_fields(data, dt)
+        data.sync()
+        el = time.perf_counter() - t0
+    finally:
+        data.close()
+    return {"workload": "1D GR gap, Kerr a=0.99, 4096 cells, multiplicity 100, FP64, "
+                        "current smoothing 1", "steps": steps, "particles": npart,
+            "pushes_per_s": npart * steps / el, "ms_per_step": 1e3 * el / steps}
+
+
 def measure_c3(apt, steps):
     """The 2D log-spherical magnetosphere in the mixed form: 1024 x 1024
     cells over r in [1, 30], theta in [0, pi]; dipole mu = 1e4 as B and
@@ -416,8 +502,11 @@ def main():
     ap.add_argument("--c1-steps", type=int, default=1000)
     ap.add_argument("--c3-steps", type=int, default=2000)
     ap.add_argument("--c4-steps", type=int, default=100)
+    ap.add_argument("--gr1d-steps", type=int, default=500)
+    ap.add_argument("--no-next", action="store_true",
+                    help="skip the IC-table and 1D GR measurements")
     ap.add_argument("--only", default="",
-                    help="comma list of c1,c3,c4,c5: run only those (for profiling)")
+                    help="comma list of c1,c3,c4,c5,ic,gr1d: run only those (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (read 1 GiB + write 1 GiB), "
                          "the reference point for the FETCH_SIZE/WRITE_SIZE counters")
@@ -436,6 +525,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU: the PIC step is "
                           "not sharded (see the sharding design); use --gpus 1"}))
         return 2
+    if (not args.no_next and not args.only) or "ic" in args.only.split(","):
+        # the device-tensor IC measurement uses torch: its HIP runtime must be
+        # the one loaded first in the process
+        import torch  # noqa: F401
     import aperture3_amd as apt
     apt.load_library()
     if args.c5_only:
@@ -452,6 +545,10 @@ def main():
                 out["c4_3d_mixed"] = measure_c4(apt, args.c4_steps, 2)
             elif name == "c5":
                 out["batched_ensemble"] = measure_c5(apt, args.c5_steps, args.warmup)
+            elif name == "ic":
+                out["ic_tables"] = measure_ic(apt)
+            elif name == "gr1d":
+                out["gr1d"] = measure_gr1d(apt, args.gr1d_steps)
         print(json.dumps(out))
         return 0
 
@@ -529,6 +626,12 @@ def main():
         progress("batched ensemble, 256 boxes")
         line["batched_ensemble"] = measure_c5(apt, args.c5_steps, args.warmup)
         progress(f"ensemble: {line['batched_ensemble']['pushes_per_s']:.3e} pushes/s")
+    if not args.no_next:
+        progress("IC / triplet tables and sampling")
+        nxt = {"ic_tables": measure_ic(apt)}
+        progress("1D GR gap")
+        nxt["gr1d"] = measure_gr1d(apt, args.gr1d_steps)
+        line["next_rows"] = nxt
     if not args.no_cpu:
         progress("CPU baseline (FP64 oracle)")
         line["cpu_baseline"] = cpu_baseline()
