@@ -192,6 +192,13 @@ __global__ void __launch_bounds__(NT)
     __syncthreads();
     const uint64_t i = b0 + threadIdx.x;
     uint32_t c = i < end ? src.cell[i] : MAX_CELL;
+    // the particle's other arrays are requested now, so they arrive while
+    // the round's ranks are computed
+    const uint64_t j = i < end ? i : beg;
+    const float x1 = src.x1[j], x2 = src.x2[j], x3 = src.x3[j];
+    const float u1 = src.u1[j], u2 = src.u2[j], u3 = src.u3[j];
+    const float pw = src.w[j];
+    const uint32_t fl = src.flag[j];
     uint32_t k = i < end ? dir_code<DIM>(m, ti, t, c) : K_NONE;
     if (k == K_FAR) k = K_NONE;  // only reached when the caller checked far == 0
     uint32_t rank = 0;
@@ -218,14 +225,14 @@ __global__ void __launch_bounds__(NT)
     if (k != K_NONE) {
       uint64_t o = (uint64_t)s_base[k] + s_wcnt[wave][k] + rank;
       dst.cell[o] = c;
-      dst.x1[o] = src.x1[i];
-      dst.x2[o] = src.x2[i];
-      dst.x3[o] = src.x3[i];
-      dst.u1[o] = src.u1[i];
-      dst.u2[o] = src.u2[i];
-      dst.u3[o] = src.u3[i];
-      dst.w[o] = src.w[i];
-      dst.flag[o] = src.flag[i];
+      dst.x1[o] = x1;
+      dst.x2[o] = x2;
+      dst.x3[o] = x3;
+      dst.u1[o] = u1;
+      dst.u2[o] = u2;
+      dst.u3[o] = u3;
+      dst.w[o] = pw;
+      dst.flag[o] = fl;
     }
     __syncthreads();
   }
