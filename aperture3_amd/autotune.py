@@ -25,8 +25,10 @@ DEFAULT_DB = os.path.join(HERE, "tuning.json")
 # kern_tile_mixed.hip (TILE2D_SHAPES and TILE3D_SHAPES)
 CANDIDATES = {
     2: [((16, 16, 0), 6, 256), ((16, 16, 0), 6, 512), ((16, 16, 0), 5, 256),
-        ((16, 16, 0), 4, 256), ((32, 16, 0), 6, 512), ((32, 32, 0), 6, 1024),
-        ((8, 8, 0), 4, 256)],
+        ((16, 16, 0), 4, 256), ((32, 16, 0), 6, 512), ((16, 16, 0), 7, 512),
+        ((16, 16, 0), 8, 512), ((16, 16, 0), 9, 512), ((16, 16, 0), 10, 512),
+        ((16, 8, 0), 8, 512), ((32, 16, 0), 7, 512), ((32, 16, 0), 8, 512),
+        ((32, 32, 0), 6, 1024), ((8, 8, 0), 4, 256)],
     3: [((8, 8, 8), 3, 256), ((8, 8, 8), 4, 256), ((4, 4, 4), 3, 256),
         ((8, 8, 8), 4, 512), ((8, 8, 8), 4, 1024), ((8, 8, 8), 5, 512),
         ((8, 8, 8), 5, 1024)],

@@ -861,6 +861,13 @@ __global__ void __launch_bounds__(NT)
   X(16, 16, 5, 256)      \
   X(16, 16, 4, 256)      \
   X(32, 16, 6, 512)      \
+  X(16, 16, 7, 512)      \
+  X(16, 16, 8, 512)      \
+  X(16, 16, 9, 512)      \
+  X(16, 16, 10, 512)     \
+  X(16, 8, 8, 512)       \
+  X(32, 16, 7, 512)      \
+  X(32, 16, 8, 512)      \
   X(32, 32, 6, 1024)     \
   X(8, 8, 4, 256)
 

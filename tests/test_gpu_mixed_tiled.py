@@ -78,6 +78,7 @@ def test_tiled_loader_draws_the_fp64_particle_set():
 
 @pytest.mark.parametrize("N,ppc,tile,halo,threads", [
     ((64, 64), 4, (16, 16), 0, 0), ((32, 32), 3, (8, 8), 0, 0),
+    ((64, 64), 4, (16, 16), 8, 512), ((64, 48), 4, (16, 8), 8, 512),
     ((16, 16, 16), 2, (8, 8, 8), 0, 0), ((24, 24, 24), 2, (8, 8, 8), 4, 512),
     ((24, 24, 24), 2, (8, 8, 8), 5, 1024)])
 def test_tiled_step_vs_oracle(N, ppc, tile, halo, threads):
