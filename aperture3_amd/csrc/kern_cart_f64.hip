@@ -6,12 +6,106 @@
 // reference's operation order and this file is built with -ffp-contract=off,
 // so every particle's arithmetic is bit-identical to the CPU restatement.
 // The current goes through two-limb int64 fixed point (order independent)
-// with device-scope atomics on the wrapped (periodic) index.
+// with device-scope atomics on the wrapped (periodic) index; in 1D the lanes
+// of a wave that add to the same element are summed first (one atomic pair
+// per element: 4.4x faster at 200 particles per cell), in 2D and 3D each
+// lane adds its own terms (the wave sums cost more than the atomics there).
 #include "apt_common.hpp"
 #include "apt_f64.hpp"
 #include "apt_kernels.hpp"
 
 namespace apt {
+
+// Adds v (two-limb fixed point at 2^-H) at `o` of hi/lo for every lane with
+// `on`, first summing the lanes of the wave that target the same element
+// (same `key`): one atomic pair per distinct element instead of one per
+// lane.  Cell-sorted particles put most lanes of a wave in one cell, where
+// per-lane atomics on a handful of addresses serialise.  The sums are of
+// integers, so the result is the same in any order.  Every lane of the wave
+// must call this together.
+__device__ __forceinline__ void wave_acc_add(long long *hi, long long *lo, uint64_t o,
+                                             uint64_t key, double v, int H, bool on) {
+  long long h = 0, l = 0;
+  if (on) fx_split(v, H, h, l);
+  const bool act = on && (h != 0 || l != 0);
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(act);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const unsigned long long k = __shfl((unsigned long long)key, leader);
+    const bool mine = act && key == k;
+    const unsigned long long mm = __ballot(mine);
+    long long sh = mine ? h : 0, sl = mine ? l : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      sh += __shfl_xor(sh, off);
+      sl += __shfl_xor(sl, off);
+    }
+    if (lane == leader) {
+      if (sh != 0)
+        __hip_atomic_fetch_add((unsigned long long *)(hi + o), (unsigned long long)sh,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (sl != 0)
+        __hip_atomic_fetch_add((unsigned long long *)(lo + o), (unsigned long long)sl,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    todo &= ~mm;
+  }
+}
+
+// Push (ptc_push_cart_1d, ptc_updater.cu:21-76) and move (the first part
+// of deposit_current_cart_1d, :220-245) of one live particle: stores
+// momenta, gamma, cell and offsets, and returns what the deposit needs.
+__device__ __forceinline__ void push_move_1d_f64(const Mesh &m, const Species &sp, double dt,
+                                                 const PtcF64 &p, const FieldsF64 &f,
+                                                 unsigned long long *counters, uint64_t idx,
+                                                 uint32_t c, int &s, bool &dep, double &weight,
+                                                 double &old_x1, double &new_x1, int &dc1,
+                                                 double &v2, double &v3) {
+  uint32_t flag = p.flag[idx];
+  s = ptc_species(flag);
+  double x1 = p.x1[idx], x2 = p.x2[idx], x3 = p.x3[idx];
+  double p1 = p.p1[idx], p2 = p.p2[idx], p3 = p.p3[idx];
+  double gamma = p.E[idx];
+  double q_over_m = dt * 0.5 * sp.q[s] / sp.m[s];
+  if (p1 != p1 || p2 != p2 || p3 != p3)
+    atomicAdd(&counters[CTR_NAN], 1ull);
+  gamma = sqrt(1.0 + p1 * p1 + p2 * p2 + p3 * p3);
+  if (!(flag & FLAG_IGNORE_EM)) {
+    double E1 = gather1(f.E[0], x1, (int)c, ST_E1) * q_over_m;
+    double E2 = gather1(f.E[1], x1, (int)c, ST_E2) * q_over_m;
+    double E3 = gather1(f.E[2], x1, (int)c, ST_E3) * q_over_m;
+    double B1 = gather1(f.B[0], x1, (int)c, ST_B1) * q_over_m;
+    double B2 = gather1(f.B[1], x1, (int)c, ST_B2) * q_over_m;
+    double B3 = gather1(f.B[2], x1, (int)c, ST_B3) * q_over_m;
+    vay_push(p1, p2, p3, gamma, E1, E2, E3, B1, B2, B3);
+    p.p1[idx] = p1;
+    p.p2[idx] = p2;
+    p.p3[idx] = p3;
+    p.E[idx] = gamma;
+  } else {
+    // ignore_EM: stored momenta and gamma are left as they were
+    p1 = p.p1[idx];
+    p2 = p.p2[idx];
+    p3 = p.p3[idx];
+    gamma = p.E[idx];
+  }
+  double v1 = p1 / gamma;
+  v2 = p2 / gamma;
+  v3 = p3 / gamma;
+  double w = p.w[idx];
+  old_x1 = x1;
+  new_x1 = old_x1 + (v1 * dt) / m.delta[0];
+  dc1 = (int)floor(new_x1);
+  if (dc1 > 1 || dc1 < -1) atomicAdd(&counters[CTR_MULTICELL], 1ull);
+  new_x1 -= (double)dc1;
+  p.cell[idx] = (uint32_t)m.wrap(0, (int)c + dc1);
+  p.x1[idx] = new_x1;
+  p.x2[idx] = x2 + v2 * dt;
+  p.x3[idx] = x3 + v3 * dt;
+  weight = sp.q[s] * w;
+  dep = !(flag & FLAG_IGNORE_CURRENT);
+}
 
 template <int DIM>
 __global__ void __launch_bounds__(256)
@@ -20,13 +114,54 @@ __global__ void __launch_bounds__(256)
                        unsigned long long *__restrict__ counters,
                        uint64_t n) {
   uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  uint32_t c = p.cell[idx];
-  if (c == MAX_CELL) return;
-  int c1, c2, c3;
-  m.split(c, c1, c2, c3);
-  if (!m.in_bulk(c1, c2, c3)) {
-    p.cell[idx] = MAX_CELL;
+  // the 1D deposit goes through wave_acc_add, which every lane of the wave
+  // enters: lanes without a particle take part with nothing to add
+  bool live = idx < n;
+  uint32_t c = live ? p.cell[idx] : MAX_CELL;
+  live = live && c != MAX_CELL;
+  int c1 = 0, c2 = 0, c3 = 0;
+  if (live) {
+    m.split(c, c1, c2, c3);
+    if (!m.in_bulk(c1, c2, c3)) {
+      p.cell[idx] = MAX_CELL;
+      live = false;
+    }
+  }
+  if (DIM != 1 && !live) return;
+  if (DIM == 1) {
+    bool dep = false;
+    int s = 0, dc1 = 0;
+    double weight = 0.0, old_x1 = 0.0, new_x1 = 0.0, v2 = 0.0, v3 = 0.0;
+    if (live)
+      push_move_1d_f64(m, sp, dt, p, f, counters, idx, c, s, dep, weight, old_x1, new_x1, dc1,
+                       v2, v3);
+    {
+      // deposit_current_cart_1d (ptc_updater.cu:220-293): i over i0..i1
+      const int i0 = (dc1 == -1 ? -2 : -1), i1 = (dc1 == 1 ? 1 : 0);
+      double djx = 0.0;
+      for (int i = -2; i <= 1; i++) {
+        const bool in = dep && i >= i0 && i <= i1;
+        double vj1 = 0.0, vj2 = 0.0, vj3 = 0.0, vr = 0.0;
+        uint64_t o1 = 0, o = 0;
+        if (in) {
+          double sx0 = cic(-old_x1 + i + 1);
+          double sx1 = cic(-new_x1 + (i + 1 - dc1));
+          djx += sx1 - sx0;
+          o1 = m.widx(i + (int)c + 1, 0, 0);
+          o = m.widx(i + (int)c, 0, 0);
+          vj1 = -weight * djx;
+          double val1 = 0.5 * (sx0 + sx1);
+          vj2 = weight * v2 * val1;
+          vj3 = weight * v3 * val1;
+          vr = weight * sx1;
+        }
+        wave_acc_add(a.J[0][0], a.J[0][1], o1, o1, vj1, H, in);
+        wave_acc_add(a.J[1][0], a.J[1][1], o, o, vj2, H, in);
+        wave_acc_add(a.J[2][0], a.J[2][1], o, o, vj3, H, in);
+        if (rho)
+          wave_acc_add(a.R[s][0], a.R[s][1], o, o | ((uint64_t)s << 56), vr, H, in);
+      }
+    }
     return;
   }
   const int d0 = m.dims[0], d1 = m.dims[1];
@@ -41,14 +176,7 @@ __global__ void __launch_bounds__(256)
   gamma = sqrt(1.0 + p1 * p1 + p2 * p2 + p3 * p3);
   if (!(flag & FLAG_IGNORE_EM)) {
     double E1, E2, E3, B1, B2, B3;
-    if (DIM == 1) {
-      E1 = gather1(f.E[0], x1, (int)c, ST_E1) * q_over_m;
-      E2 = gather1(f.E[1], x1, (int)c, ST_E2) * q_over_m;
-      E3 = gather1(f.E[2], x1, (int)c, ST_E3) * q_over_m;
-      B1 = gather1(f.B[0], x1, (int)c, ST_B1) * q_over_m;
-      B2 = gather1(f.B[1], x1, (int)c, ST_B2) * q_over_m;
-      B3 = gather1(f.B[2], x1, (int)c, ST_B3) * q_over_m;
-    } else if (DIM == 2) {
+    if (DIM == 2) {
       E1 = gather2(f.E[0], d0, x1, x2, c1, c2, ST_E1) * q_over_m;
       E2 = gather2(f.E[1], d0, x1, x2, c1, c2, ST_E2) * q_over_m;
       E3 = gather2(f.E[2], d0, x1, x2, c1, c2, ST_E3) * q_over_m;
@@ -81,33 +209,7 @@ __global__ void __launch_bounds__(256)
   double w = p.w[idx];
   const double old_x1 = x1, old_x2 = x2, old_x3 = x3;
 
-  if (DIM == 1) {
-    double new_x1 = old_x1 + (v1 * dt) / m.delta[0];
-    int dc1 = (int)floor(new_x1);
-    if (dc1 > 1 || dc1 < -1) atomicAdd(&counters[CTR_MULTICELL], 1ull);
-    new_x1 -= (double)dc1;
-    p.cell[idx] = (uint32_t)m.wrap(0, (int)c + dc1);
-    p.x1[idx] = new_x1;
-    p.x2[idx] = old_x2 + v2 * dt;
-    p.x3[idx] = old_x3 + v3 * dt;
-    if (flag & FLAG_IGNORE_CURRENT) return;
-    double weight = sp.q[s] * w;
-    int i0 = (dc1 == -1 ? -2 : -1);
-    int i1 = (dc1 == 1 ? 1 : 0);
-    double djx = 0.0;
-    for (int i = i0; i <= i1; i++) {
-      double sx0 = cic(-old_x1 + i + 1);
-      double sx1 = cic(-new_x1 + (i + 1 - dc1));
-      djx += sx1 - sx0;
-      uint64_t o1 = m.widx(i + (int)c + 1, 0, 0);
-      uint64_t o = m.widx(i + (int)c, 0, 0);
-      acc_add(a.J[0][0], a.J[0][1], o1, -weight * djx, H);
-      double val1 = 0.5 * (sx0 + sx1);
-      acc_add(a.J[1][0], a.J[1][1], o, weight * v2 * val1, H);
-      acc_add(a.J[2][0], a.J[2][1], o, weight * v3 * val1, H);
-      if (rho) acc_add(a.R[s][0], a.R[s][1], o, weight * sx1, H);
-    }
-  } else if (DIM == 2) {
+  if (DIM == 2) {
     double new_x1 = old_x1 + (v1 * dt) * m.inv_delta[0];
     double new_x2 = old_x2 + (v2 * dt) * m.inv_delta[1];
     int dc1 = (int)floor(new_x1);
