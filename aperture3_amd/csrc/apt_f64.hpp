@@ -120,4 +120,41 @@ __device__ inline void acc_add(long long *__restrict__ hi,
                            __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Adds v (two-limb fixed point at 2^-H) at `o` of hi/lo for every lane with
+// `on`, first summing the lanes of the wave that target the same element
+// (same `key`): one atomic pair per distinct element instead of one per
+// lane.  Cell-sorted particles put most lanes of a wave in one cell, where
+// per-lane atomics on a handful of addresses serialise.  The sums are of
+// integers, so the result is the same in any order.  Every lane of the wave
+// must call this together.
+__device__ __forceinline__ void wave_acc_add(long long *hi, long long *lo, uint64_t o,
+                                             uint64_t key, double v, int H, bool on) {
+  long long h = 0, l = 0;
+  if (on) fx_split(v, H, h, l);
+  const bool act = on && (h != 0 || l != 0);
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(act);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const unsigned long long k = __shfl((unsigned long long)key, leader);
+    const bool mine = act && key == k;
+    const unsigned long long mm = __ballot(mine);
+    long long sh = mine ? h : 0, sl = mine ? l : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      sh += __shfl_xor(sh, off);
+      sl += __shfl_xor(sl, off);
+    }
+    if (lane == leader) {
+      if (sh != 0)
+        __hip_atomic_fetch_add((unsigned long long *)(hi + o), (unsigned long long)sh,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (sl != 0)
+        __hip_atomic_fetch_add((unsigned long long *)(lo + o), (unsigned long long)sl,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    todo &= ~mm;
+  }
+}
+
 }  // namespace apt

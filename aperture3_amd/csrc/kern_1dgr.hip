@@ -33,19 +33,16 @@ __device__ inline bool bulk1(const Mesh &m, uint32_t c) {
   return (int)c >= m.guard[0] && (int)c < m.dims[0] - m.guard[0];
 }
 
-// update_ptc_1dgr (ptc_updater_1dgr.cu:136-299)
-__global__ void k_1dgr_push(Mesh m, Gr1dCoef cf, Species sp, PtcF64 p,
-                            const double *__restrict__ e3, AccF64 acc, double dt,
-                            uint32_t step, int data_interval, double a, double omega,
-                            int H, uint64_t n) {
-  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  const uint32_t c32 = p.cell[idx];
-  if (c32 == MAX_CELL) return;
-  if (!bulk1(m, c32)) {
-    p.cell[idx] = MAX_CELL;
-    return;
-  }
+// push and move of one live particle (the body of update_ptc_1dgr before
+// its deposit); returns what the deposit needs
+__device__ __forceinline__ void gr1d_push_one(const Mesh &m, const Gr1dCoef &cf,
+                                              const Species &sp, const PtcF64 &p,
+                                              const double *__restrict__ e3, double dt,
+                                              uint32_t step, int data_interval, double a,
+                                              double omega, uint64_t idx, uint32_t c32,
+                                              bool &dep, int &c_out, int &dc_out, int &s_out,
+                                              double &x1_out, double &nx1_out,
+                                              double &weight_out) {
   const int c = (int)c32;
   const double rp = 1.0 + sqrt(1.0 - a * a);
   const double rm = 1.0 - sqrt(1.0 - a * a);
@@ -117,21 +114,58 @@ __global__ void k_1dgr_push(Mesh m, Gr1dCoef cf, Species sp, PtcF64 p,
   u0 = sqrt((D2 + p1 * p1) / (D2 * (alpha * alpha - D3) + D1 * D1));
   p.E[idx] = u0;
 
-  if (flag & FLAG_IGNORE_CURRENT) return;
-  const double weight = -sp.q[s] * p.w[idx];
+  c_out = c;
+  dc_out = dc1;
+  s_out = s;
+  x1_out = x1;
+  nx1_out = new_x1;
+  dep = !(flag & FLAG_IGNORE_CURRENT);
+  if (dep) weight_out = -sp.q[s] * p.w[idx];
+}
+
+// update_ptc_1dgr (ptc_updater_1dgr.cu:136-299)
+__global__ void k_1dgr_push(Mesh m, Gr1dCoef cf, Species sp, PtcF64 p,
+                            const double *__restrict__ e3, AccF64 acc, double dt,
+                            uint32_t step, int data_interval, double a, double omega,
+                            int H, uint64_t n) {
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // every lane of the wave reaches the deposit (wave_acc_add); lanes without
+  // a live particle take part with nothing to add
+  bool live = idx < n;
+  const uint32_t c32 = live ? p.cell[idx] : MAX_CELL;
+  live = live && c32 != MAX_CELL;
+  if (live && !bulk1(m, c32)) {
+    p.cell[idx] = MAX_CELL;
+    live = false;
+  }
+  bool dep = false;
+  int c = 0, dc1 = 0, s = 0;
+  double x1 = 0.0, new_x1 = 0.0, weight = 0.0;
+  if (live) gr1d_push_one(m, cf, sp, p, e3, dt, step, data_interval, a, omega, idx, c32, dep,
+                          c, dc1, s, x1, new_x1, weight);
+  const double *K1j = cf.a[GR_K1J], *K1 = cf.a[GR_K1];
   const int i_0 = dc1 == -1 ? -2 : -1;
   const int i_1 = dc1 == 1 ? 1 : 0;
-  const double *K1j = cf.a[GR_K1J], *K1 = cf.a[GR_K1];
   double djx = 0.0;
-  for (int i = i_0; i <= i_1; i++) {
-    const double sx0 = cic(-x1 + i + 1);
-    const double sx1 = cic(-new_x1 + (i + 1 - dc1));
-    djx += sx1 - sx0;
-    acc_add(acc.J[0][0], acc.J[0][1], (uint64_t)(i + c + 1),
-            weight * djx / K1j[i + c + 1] * m.delta[0] / dt, H);
-    acc_add(acc.R[s][0], acc.R[s][1], (uint64_t)(i + c), -weight * sx1 / K1[i + c], H);
+  for (int i = -2; i <= 1; i++) {
+    const bool in = dep && i >= i_0 && i <= i_1;
+    double vj = 0.0, vr = 0.0;
+    uint64_t o1 = 0, o = 0;
+    if (in) {
+      const double sx0 = cic(-x1 + i + 1);
+      const double sx1 = cic(-new_x1 + (i + 1 - dc1));
+      djx += sx1 - sx0;
+      o1 = (uint64_t)(i + c + 1);
+      o = (uint64_t)(i + c);
+      vj = weight * djx / K1j[i + c + 1] * m.delta[0] / dt;
+      vr = -weight * sx1 / K1[i + c];
+    }
+    wave_acc_add(acc.J[0][0], acc.J[0][1], o1, o1, vj, H, in);
+    wave_acc_add(acc.R[s][0], acc.R[s][1], o, o | ((uint64_t)s << 56), vr, H, in);
   }
 }
+
+
 
 // update_photon_1dgr (ptc_updater_1dgr.cu:301-406), with the reference's
 // fixed a = 0.99
